@@ -1,0 +1,144 @@
+// common.h -- shared definitions of the gfx950 kernels and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/hrnn_dev.h"
+
+namespace hrnn {
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define HIPC(expr)                                                                        \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw ::hrnn::Error{HRNN_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_) + \
+                                    " (" + __FILE__ + ":" + std::to_string(__LINE__) + ")"}; \
+    } while (0)
+
+#define REQUIRE(cond, code, text)                                                        \
+    do {                                                                                  \
+        if (!(cond)) throw ::hrnn::Error{(code), std::string(text)};                      \
+    } while (0)
+
+// Growable device buffer owned by a handle.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    void reserve(size_t b) {
+        if (b <= bytes) return;
+        if (p) HIPC(hipFree(p));
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, b ? b : 1);
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error{HRNN_E_OOM, "hipMalloc(" + std::to_string(b) + " B): " + hipGetErrorString(e)};
+        }
+        bytes = b;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+inline unsigned ceil_div(uint64_t a, uint64_t b) { return (unsigned)((a + b - 1) / b); }
+
+constexpr uint32_t kInvalidId = 0xFFFFFFFFu;
+
+// Squared L2 with the reference's summation order (space_l2.h:25-54): 16
+// partial sums, element e into partial e % 16 with a fused multiply-add, then
+// the partials added left to right.  Computed by a group of 16 consecutive
+// lanes: lane g (0..15) of the group owns partial g.  Every lane of the group
+// returns the full sum.  d must be a multiple of 16.
+__device__ __forceinline__ float l2_ref16_group(const float *__restrict__ a,
+                                                const float *__restrict__ b, uint32_t d,
+                                                int g) {
+    float p = 0.f;
+    for (uint32_t e = (uint32_t)g; e < d; e += 16) {
+        float t = a[e] - b[e];
+        p = __builtin_fmaf(t, t, p);
+    }
+    // left-to-right sum of the 16 partials of this group
+    int base = (int)(threadIdx.x & 63u) & ~15;
+    float r = __shfl(p, base + 0, 64);
+#pragma unroll
+    for (int l = 1; l < 16; l++) r += __shfl(p, base + l, 64);
+    return r;
+}
+
+// Sequential-order squared L2 (space_l2.h:6-20 with contraction), used when
+// d is not a multiple of 16.
+__device__ __forceinline__ float l2_seq(const float *__restrict__ a, const float *__restrict__ b,
+                                        uint32_t d) {
+    float r = 0.f;
+    for (uint32_t e = 0; e < d; e++) {
+        float t = a[e] - b[e];
+        r = __builtin_fmaf(t, t, r);
+    }
+    return r;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    uint32_t lane = threadIdx.x & 63u;
+    return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+// (dist, id) ordering used everywhere a ranked list is built: ascending
+// distance, ties to the smaller id.
+__device__ __forceinline__ bool pair_less(float da, uint32_t ia, float db, uint32_t ib) {
+    return da < db || (da == db && ia < ib);
+}
+
+// Order-preserving float -> u32 map for non-negative and negative floats.
+__device__ __forceinline__ uint32_t float_key(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_float(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    return __uint_as_float(u);
+}
+
+// In-LDS bitonic sort of n (power of two) keys, ascending, by the whole block.
+template <class T>
+__device__ void bitonic_sort_lds(T *keys, uint32_t n) {
+    for (uint32_t size = 2; size <= n; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            __syncthreads();
+            for (uint32_t t = threadIdx.x; t < (n >> 1); t += blockDim.x) {
+                uint32_t lo = 2 * t - (t & (stride - 1));
+                uint32_t hi = lo + stride;
+                bool up = ((lo & size) == 0);
+                T a = keys[lo], b = keys[hi];
+                if ((a > b) == up) {
+                    keys[lo] = b;
+                    keys[hi] = a;
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+inline uint32_t next_pow2(uint32_t v) {
+    uint32_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+}  // namespace hrnn

@@ -1,0 +1,82 @@
+// handle.h -- the per-index device state behind an hrnn_dev_t.
+#pragma once
+#include "common.h"
+
+struct hrnn_dev {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int n_cu = 0;
+    size_t lds_per_block = 0;
+
+    // base vectors (row-major n x d): FP32, BF16 and FP16 mirrors, FP32 norms
+    uint64_t n = 0;
+    uint64_t n_cap = 0;
+    uint32_t d = 0;
+    hrnn::DevBuf x, xb, xh, norms;
+    int small_int = 0;   // every value an integer in [-256, 256]
+
+    // ranked KNN graph (SoA)
+    uint32_t K = 0;
+    hrnn::DevBuf knn_ids, knn_d, knn_cnt, knn_new;
+
+    // reverse CSR
+    bool csr_valid = false;
+    uint32_t rank_bits = 0;
+    int wide = 0;
+    uint64_t n_ent = 0;
+    uint64_t max_deg = 0;   // longest reverse list
+    hrnn::DevBuf csr_off, csr_ent;
+
+    // RkNN search private state: per-workgroup visited bitmaps / u16 counts
+    // (all-zero between launches) and candidate buffers
+    hrnn::DevBuf search_mark, search_cand;
+
+    // scratch shared by the build kernels
+    hrnn::DevBuf s0, s1, s2, s3, s4, s5, s6, s7, s8;
+    // small device counters, by byte offset: 0 base-int flag, 32 long-list
+    // count, 128 CSR total, 192 longest list, 256 search queue, 512 NNDescent stats
+    hrnn::DevBuf counters;
+
+    ~hrnn_dev();
+};
+
+namespace hrnn {
+
+template <class F>
+int guarded(hrnn_dev_t h, F &&f) {
+    try {
+        if (h) HIPC(hipSetDevice(h->device));
+        f();
+        return HRNN_OK;
+    } catch (Error &e) {
+        if (h) h->err = e.msg;
+        return e.code;
+    } catch (std::exception &e) {
+        if (h) h->err = e.what();
+        return HRNN_E_ARG;
+    }
+}
+
+// Scoped host <-> device staging helpers (synchronous on the handle stream).
+void h2d(hrnn_dev_t h, void *dst, const void *src, size_t bytes);
+void d2h(hrnn_dev_t h, void *dst, const void *src, size_t bytes);
+
+// Launchers defined in the kernel files.
+void launch_base_mirrors(hrnn_dev_t h, uint64_t row0, uint64_t rows);
+void csr_build(hrnn_dev_t h);
+uint64_t max_list_len(hrnn_dev_t h);
+void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint32_t *proxies,
+                          uint32_t m, const uint32_t *k_per_q, uint32_t k, uint32_t K_prime,
+                          float threshold, uint32_t *res_cnt, uint32_t *res_ids,
+                          uint32_t cap_per_q);
+void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t bucket, hrnn_nnd_iter_stats *st);
+void bf_topk_base(hrnn_dev_t h, const float *q_dev, uint32_t nq, bool self_base, uint32_t kk,
+                  int prec, uint32_t *ids_dev, float *d_dev);
+void bf_rknn_count_fill(hrnn_dev_t h, const float *q_dev, uint32_t nq, const float *kth_dev,
+                        int prec, uint32_t *cnt_dev, uint32_t *ids_dev, float *d_dev,
+                        uint32_t cap_per_q);
+void l2_pairs(hrnn_dev_t h, const float *a_dev, const uint32_t *ia, const uint32_t *ib,
+              uint64_t npairs, float *out);
+
+}  // namespace hrnn

@@ -1,0 +1,487 @@
+// nndescent.hip -- (a) NNDescent refinement of the ranked KNN graph.
+//
+// Reference: HRNN::refineKNNG (include/hnswlib/hrnn.h:1725-1862) with
+// try_update_knng_nnd (hrnn.h:1660-1722).  Per iteration:
+//   UPDATE (hrnn.h:1765-1810)  k_nnd_sample: one wave per node walks its list
+//       in rank order; the first S new entries become nn_new, the first S old
+//       ones nn_old; every new flag is cleared; entry (i -> nbr) is pushed into
+//       nbr's reverse sample (new or old, capped at R = S) when
+//       dist(i, nbr) > nbr's K-th distance (FLT_MAX unless nbr's list is full).
+//   MERGE (hrnn.h:1813-1819)   the join reads forward + reverse samples
+//       directly (old side cut at 2R).
+//   JOIN (hrnn.h:1823-1853)    k_nnd_join: one workgroup per node stages the
+//       <= 4S sample vectors in LDS (BF16, 64-dim chunks) and forms their Gram
+//       matrix on MFMA (v_mfma_f32_16x16x32_bf16): rows = new samples, columns
+//       = new (upper triangle) and old samples.  d2 = |a|^2 + |b|^2 - 2 a.b.
+//       A pair (a, b), a != b, is proposed to a's list when d2 < worst(a) and
+//       to b's list when d2 < worst(b), worst taken at the iteration start.
+//       Proposals go to per-node buckets of capacity P (atomic slot).
+//   k_nnd_merge: per node, sort the bucket by (d2, id), drop duplicates and
+//       ids already listed, and merge with the sorted list (existing entries
+//       first on equal distance, as the reference's strict-< insertion sort
+//       leaves them); the first K survive, inserted entries flagged new.
+// The reference applies each proposal at once under a per-node lock; the
+// final list of an iteration is the same top-K of (list U proposals) up to
+// ties, so the contract with the reference is recall parity.
+//
+// Distances: for integer data in [-256, 256] at d <= 128 (all 0..255 configs)
+// the BF16 Gram form is exact FP32 arithmetic, identical to the reference's
+// direct-difference L2Sqr.  Otherwise each proposal that passes the BF16
+// filter is re-computed in the reference's FP32 order before it is bucketed.
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <vector>
+
+#include "handle.h"
+
+namespace hrnn {
+
+typedef __bf16 bf16x8_n __attribute__((ext_vector_type(8)));
+typedef float f32x4_n __attribute__((ext_vector_type(4)));
+
+enum { ST_DISTS = 0, ST_PROPOSALS = 1, ST_DROPPED = 2, ST_UPDATES = 3, ST_N = 4 };
+
+// worst(u) and per-iteration counter reset
+__global__ void k_nnd_prepare(const float *__restrict__ kd, const uint32_t *__restrict__ cnt,
+                              uint64_t n, uint32_t K, float *__restrict__ worst,
+                              uint32_t *__restrict__ rn_cnt, uint32_t *__restrict__ ro_cnt,
+                              uint32_t *__restrict__ prop_cnt, unsigned long long *__restrict__ stats) {
+    uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u == 0) {
+        for (int s = 0; s < ST_N; s++) stats[s] = 0;
+    }
+    if (u >= n) return;
+    uint32_t c = cnt[u];
+    worst[u] = (c >= K) ? kd[u * K + (K - 1)] : FLT_MAX;
+    rn_cnt[u] = 0;
+    ro_cnt[u] = 0;
+    prop_cnt[u] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_nnd_sample(
+    const uint32_t *__restrict__ ids, const float *__restrict__ kd, uint8_t *__restrict__ isnew,
+    const uint32_t *__restrict__ cnt, uint64_t n, uint32_t K, uint32_t S, uint32_t R,
+    const float *__restrict__ worst, uint32_t *__restrict__ fn, uint32_t *__restrict__ fn_cnt,
+    uint32_t *__restrict__ fo, uint32_t *__restrict__ fo_cnt, uint32_t *__restrict__ rn,
+    uint32_t *__restrict__ rn_cnt, uint32_t *__restrict__ ro, uint32_t *__restrict__ ro_cnt) {
+    uint64_t i = (uint64_t)blockIdx.x * 4 + threadIdx.x / 64;
+    if (i >= n) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt = lanemask_lt();
+    uint32_t c = cnt[i];
+    uint32_t new_before = 0, old_before = 0;
+    for (uint32_t base = 0; base < c; base += 64) {
+        uint32_t j = base + lane;
+        bool valid = j < c;
+        uint32_t nbr = 0;
+        float dij = 0.f;
+        bool f = false;
+        if (valid) {
+            nbr = ids[i * K + j];
+            dij = kd[i * K + j];
+            f = isnew[i * K + j] != 0;
+        }
+        bool isn = valid && f, iso = valid && !f;
+        uint64_t mn = __ballot(isn), mo = __ballot(iso);
+        uint32_t rnk_n = new_before + __popcll(mn & lt);
+        uint32_t rnk_o = old_before + __popcll(mo & lt);
+        if (isn) {
+            if (rnk_n < S) fn[i * S + rnk_n] = nbr;
+            isnew[i * K + j] = 0;
+        }
+        if (iso && rnk_o < S) fo[i * S + rnk_o] = nbr;
+        if (valid && dij > worst[nbr]) {
+            if (isn) {
+                uint32_t s = atomicAdd(&rn_cnt[nbr], 1u);
+                if (s < R) rn[(uint64_t)nbr * R + s] = (uint32_t)i;
+            } else {
+                uint32_t s = atomicAdd(&ro_cnt[nbr], 1u);
+                if (s < R) ro[(uint64_t)nbr * R + s] = (uint32_t)i;
+            }
+        }
+        new_before += __popcll(mn);
+        old_before += __popcll(mo);
+    }
+    if (lane == 0) {
+        fn_cnt[i] = new_before < S ? new_before : S;
+        fo_cnt[i] = old_before < S ? old_before : S;
+    }
+}
+
+constexpr int kJMax = 256;      // sample members per node (4S, S <= 64)
+constexpr int kJDC = 64;        // dims per LDS chunk
+constexpr int kJDP = kJDC + 8;  // padded row stride
+constexpr int kJMaxB = 32;      // Gram blocks per wave
+
+__device__ float l2_ref16_lane(const float *__restrict__ a, const float *__restrict__ b, uint32_t d) {
+    if (d & 15u) return l2_seq(a, b, d);
+    float p[16];
+#pragma unroll
+    for (int l = 0; l < 16; l++) p[l] = 0.f;
+    for (uint32_t e = 0; e < d; e += 16) {
+#pragma unroll
+        for (int l = 0; l < 16; l++) {
+            float t = a[e + l] - b[e + l];
+            p[l] = __builtin_fmaf(t, t, p[l]);
+        }
+    }
+    float r = p[0];
+#pragma unroll
+    for (int l = 1; l < 16; l++) r += p[l];
+    return r;
+}
+
+__device__ __forceinline__ void nnd_propose(uint32_t target, uint32_t cand, float dv,
+                                            uint32_t *__restrict__ prop_cnt,
+                                            uint32_t *__restrict__ prop_id, float *__restrict__ prop_d,
+                                            uint32_t P, uint32_t &dropped) {
+    uint32_t s = atomicAdd(&prop_cnt[target], 1u);
+    if (s < P) {
+        prop_id[(uint64_t)target * P + s] = cand;
+        prop_d[(uint64_t)target * P + s] = dv;
+    } else {
+        dropped++;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_nnd_join(
+    const __hip_bfloat16 *__restrict__ xb, const float *__restrict__ xn, const float *__restrict__ x32,
+    uint32_t d, uint64_t n, uint32_t S, uint32_t R, const uint32_t *__restrict__ fn,
+    const uint32_t *__restrict__ fn_cnt, const uint32_t *__restrict__ fo,
+    const uint32_t *__restrict__ fo_cnt, const uint32_t *__restrict__ rn,
+    const uint32_t *__restrict__ rn_cnt, const uint32_t *__restrict__ ro,
+    const uint32_t *__restrict__ ro_cnt, const float *__restrict__ worst,
+    uint32_t *__restrict__ prop_cnt, uint32_t *__restrict__ prop_id, float *__restrict__ prop_d,
+    uint32_t P, int exact, unsigned long long *__restrict__ stats) {
+    __shared__ uint32_t sid[kJMax];
+    __shared__ float sthr[kJMax], snorm[kJMax];
+    __shared__ __attribute__((aligned(16))) __hip_bfloat16 vec[kJMax * kJDP];
+    __shared__ uint8_t blk_r[kJMaxB * 4], blk_c[kJMaxB * 4];
+    __shared__ uint32_t s_nn, s_T, s_nb;
+
+    const uint64_t node = blockIdx.x;
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x / 64;
+    if (threadIdx.x == 0) {
+        uint32_t a = fn_cnt[node], b = min(rn_cnt[node], R);
+        uint32_t c = fo_cnt[node], e = min(ro_cnt[node], R);
+        uint32_t nn = a + b;
+        uint32_t no = c + e;
+        if (no > 2 * R) no = 2 * R;
+        s_nn = nn;
+        s_T = nn + no;
+    }
+    __syncthreads();
+    const uint32_t nn = s_nn, T = s_T;
+    if (nn == 0) return;   // no new sample: no pair to evaluate
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
+        uint32_t a = fn_cnt[node], b = min(rn_cnt[node], R), c = fo_cnt[node];
+        uint32_t id;
+        if (t < a) id = fn[node * S + t];
+        else if (t < nn) id = rn[node * R + (t - a)];
+        else if (t < nn + c) id = fo[node * S + (t - nn)];
+        else id = ro[node * R + (t - nn - c)];
+        sid[t] = id;
+        sthr[t] = worst[id];
+        snorm[t] = xn[id];
+    }
+    const uint32_t RB = (nn + 15) / 16, CB = (T + 15) / 16;
+    if (threadIdx.x == 0) {
+        uint32_t nb = 0;
+        for (uint32_t rb = 0; rb < RB; rb++)
+            for (uint32_t cb = rb; cb < CB; cb++) {
+                blk_r[nb] = (uint8_t)rb;
+                blk_c[nb] = (uint8_t)cb;
+                nb++;
+            }
+        s_nb = nb;
+    }
+    __syncthreads();
+    const uint32_t NB = s_nb;
+    const uint32_t Tpad = CB * 16;
+
+    f32x4_n acc[kJMaxB];
+#pragma unroll
+    for (int j = 0; j < kJMaxB; j++) acc[j] = f32x4_n{0.f, 0.f, 0.f, 0.f};
+
+    for (uint32_t ch = 0; ch < d; ch += kJDC) {
+        const uint32_t dims = (d - ch) < (uint32_t)kJDC ? (d - ch) : (uint32_t)kJDC;   // 32 or 64
+        __syncthreads();
+        const uint32_t parts = dims / 8;
+        for (uint32_t u = threadIdx.x; u < Tpad * parts; u += blockDim.x) {
+            uint32_t t = u / parts, part = u % parts;
+            bf16x8_n v;
+            if (t < T) {
+                v = *reinterpret_cast<const bf16x8_n *>(xb + (uint64_t)sid[t] * d + ch + part * 8);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; e++) v[e] = 0;
+            }
+            *reinterpret_cast<bf16x8_n *>(&vec[t * kJDP + part * 8]) = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kJMaxB; j++) {
+            uint32_t b = w + 4 * j;
+            if (b < NB) {
+                uint32_t rb = blk_r[b], cb = blk_c[b];
+                for (uint32_t kl = 0; kl < dims / 32; kl++) {
+                    bf16x8_n av = *reinterpret_cast<const bf16x8_n *>(
+                        &vec[(rb * 16 + (lane & 15u)) * kJDP + kl * 32 + 8 * (lane >> 4)]);
+                    bf16x8_n bv = *reinterpret_cast<const bf16x8_n *>(
+                        &vec[(cb * 16 + (lane & 15u)) * kJDP + kl * 32 + 8 * (lane >> 4)]);
+                    acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[j], 0, 0, 0);
+                }
+            }
+        }
+    }
+
+    uint32_t ndist = 0, nprop = 0, dropped = 0;
+#pragma unroll
+    for (int j = 0; j < kJMaxB; j++) {
+        uint32_t b = w + 4 * j;
+        if (b < NB) {
+            uint32_t rb = blk_r[b], cb = blk_c[b];
+            uint32_t c = cb * 16 + (lane & 15u);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                uint32_t r = rb * 16 + 4 * (lane >> 4) + i;
+                if (r < nn && c > r && c < T) {
+                    uint32_t a = sid[r], bb = sid[c];
+                    if (a != bb) {
+                        ndist++;
+                        float dv = snorm[r] + snorm[c] - 2.f * acc[j][i];
+                        dv = dv > 0.f ? dv : 0.f;
+                        bool pa = dv < sthr[r], pb = dv < sthr[c];
+                        if ((pa || pb) && !exact) {
+                            dv = l2_ref16_lane(x32 + (uint64_t)a * d, x32 + (uint64_t)bb * d, d);
+                            pa = dv < sthr[r];
+                            pb = dv < sthr[c];
+                        }
+                        if (pa) { nprop++; nnd_propose(a, bb, dv, prop_cnt, prop_id, prop_d, P, dropped); }
+                        if (pb) { nprop++; nnd_propose(bb, a, dv, prop_cnt, prop_id, prop_d, P, dropped); }
+                    }
+                }
+            }
+        }
+    }
+    // wave-level reduction of the counters
+    for (int off = 32; off > 0; off >>= 1) {
+        ndist += __shfl_down(ndist, off, 64);
+        nprop += __shfl_down(nprop, off, 64);
+        dropped += __shfl_down(dropped, off, 64);
+    }
+    if (lane == 0) {
+        if (ndist) atomicAdd(&stats[ST_DISTS], (unsigned long long)ndist);
+        if (nprop) atomicAdd(&stats[ST_PROPOSALS], (unsigned long long)nprop);
+        if (dropped) atomicAdd(&stats[ST_DROPPED], (unsigned long long)dropped);
+    }
+}
+
+constexpr uint32_t kMergeMaxP = 2048;
+constexpr uint32_t kMergeMaxK = 1024;
+constexpr uint32_t kHashSlots = 2048;
+
+__global__ __launch_bounds__(256) void k_nnd_merge(
+    uint32_t *__restrict__ ids, float *__restrict__ kd, uint8_t *__restrict__ isnew,
+    uint32_t *__restrict__ cnt, uint32_t K, const uint32_t *__restrict__ prop_cnt,
+    const uint32_t *__restrict__ prop_id, const float *__restrict__ prop_d, uint32_t P,
+    unsigned long long *__restrict__ stats) {
+    __shared__ unsigned long long keys[kMergeMaxP];
+    __shared__ uint32_t scan[kMergeMaxP + 1];
+    __shared__ uint32_t lid[kMergeMaxK];
+    __shared__ float ld[kMergeMaxK];
+    __shared__ uint32_t hash[kHashSlots];
+    __shared__ uint32_t wsum[4];
+
+    const uint64_t u = blockIdx.x;
+    uint32_t np = prop_cnt[u];
+    if (np == 0) return;
+    if (np > P) np = P;
+    const uint32_t c = cnt[u];
+    for (uint32_t t = threadIdx.x; t < kHashSlots; t += blockDim.x) hash[t] = kInvalidId;
+    for (uint32_t t = threadIdx.x; t < c; t += blockDim.x) {
+        lid[t] = ids[u * K + t];
+        ld[t] = kd[u * K + t];
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < c; t += blockDim.x) {
+        uint32_t v = lid[t];
+        uint32_t h = (v * 2654435761u) & (kHashSlots - 1);
+        for (;;) {
+            uint32_t old = atomicCAS(&hash[h], kInvalidId, v);
+            if (old == kInvalidId || old == v) break;
+            h = (h + 1) & (kHashSlots - 1);
+        }
+    }
+    const float worst_full = (c >= K) ? ld[K - 1] : FLT_MAX;
+    uint32_t PP = 1;
+    while (PP < np) PP <<= 1;
+    for (uint32_t t = threadIdx.x; t < PP; t += blockDim.x) {
+        unsigned long long key = ~0ull;
+        if (t < np) {
+            float dv = prop_d[u * P + t];
+            uint32_t id = prop_id[u * P + t];
+            if (dv < worst_full) key = ((unsigned long long)float_key(dv) << 32) | id;
+        }
+        keys[t] = key;
+    }
+    __syncthreads();
+    bitonic_sort_lds(keys, PP);
+    // validity: not a repeat of the previous key, not already in the list
+    for (uint32_t t = threadIdx.x; t < PP; t += blockDim.x) {
+        unsigned long long key = keys[t];
+        uint32_t ok = 0;
+        if (key != ~0ull && (t == 0 || keys[t - 1] != key)) {
+            uint32_t v = (uint32_t)key;
+            uint32_t h = (v * 2654435761u) & (kHashSlots - 1);
+            ok = 1;
+            for (;;) {
+                uint32_t e = hash[h];
+                if (e == kInvalidId) break;
+                if (e == v) { ok = 0; break; }
+                h = (h + 1) & (kHashSlots - 1);
+            }
+        }
+        scan[t] = ok;
+    }
+    __syncthreads();
+    // exclusive scan of scan[0..PP) (serial per thread chunk + wave sums)
+    {
+        const uint32_t per = (PP + blockDim.x - 1) / blockDim.x;
+        uint32_t beg = threadIdx.x * per, end = min(beg + per, PP);
+        uint32_t local = 0;
+        for (uint32_t t = beg; t < end; t++) local += scan[t];
+        uint32_t lane = threadIdx.x & 63u, w = threadIdx.x / 64;
+        uint32_t incl = local;
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t y = __shfl_up(incl, off, 64);
+            if (lane >= (uint32_t)off) incl += y;
+        }
+        if (lane == 63) wsum[w] = incl;
+        __syncthreads();
+        uint32_t woff = 0;
+        for (uint32_t k = 0; k < w; k++) woff += wsum[k];
+        uint32_t run = woff + incl - local;
+        __syncthreads();
+        for (uint32_t t = beg; t < end; t++) {
+            uint32_t v = scan[t];
+            scan[t] = run;
+            run += v;
+        }
+        if (threadIdx.x == blockDim.x - 1) scan[PP] = run;
+    }
+    __syncthreads();
+    const uint32_t nvalid = scan[PP];
+    if (nvalid == 0) return;   // uniform
+    // proposals: pos = #valid before + #existing with dist <= d
+    uint32_t upd = 0;
+    for (uint32_t t = threadIdx.x; t < PP; t += blockDim.x) {
+        unsigned long long key = keys[t];
+        if (key == ~0ull) continue;
+        bool ok = (scan[t + 1] - scan[t]) != 0;
+        if (!ok) continue;
+        float dv = key_float((uint32_t)(key >> 32));
+        uint32_t lo = 0, hi = c;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (ld[mid] <= dv) lo = mid + 1; else hi = mid;
+        }
+        uint32_t pos = scan[t] + lo;
+        if (pos < K) {
+            ids[u * K + pos] = (uint32_t)key;
+            kd[u * K + pos] = dv;
+            isnew[u * K + pos] = 1;
+            upd++;
+        }
+    }
+    // existing entries: pos = e + #valid proposals with dist < ld[e]
+    for (uint32_t e = threadIdx.x; e < c; e += blockDim.x) {
+        uint32_t kf = float_key(ld[e]);
+        uint32_t lo = 0, hi = PP;
+        while (lo < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            if ((uint32_t)(keys[mid] >> 32) < kf) lo = mid + 1; else hi = mid;
+        }
+        uint32_t pos = e + scan[lo];
+        if (pos < K) {
+            ids[u * K + pos] = lid[e];
+            kd[u * K + pos] = ld[e];
+            isnew[u * K + pos] = 0;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) upd += __shfl_down(upd, off, 64);
+    if ((threadIdx.x & 63u) == 0 && upd) atomicAdd(&stats[ST_UPDATES], (unsigned long long)upd);
+    if (threadIdx.x == 0) cnt[u] = min(c + nvalid, K);
+}
+
+void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t bucket, hrnn_nnd_iter_stats *st) {
+    const uint64_t n = h->n;
+    const uint32_t K = h->K;
+    REQUIRE(S >= 1 && S <= 64, HRNN_E_ARG, "nndescent: sample_size must be in [1, 64]");
+    REQUIRE(K <= kMergeMaxK, HRNN_E_ARG, "nndescent: K_knng must be <= 1024");
+    REQUIRE(h->d % 32 == 0, HRNN_E_ARG, "nndescent: d must be a multiple of 32");
+    uint32_t P = bucket;
+    if (P == 0) {
+        P = 1;
+        while (P < 4 * K) P <<= 1;
+        P = std::min<uint32_t>(std::max<uint32_t>(P, 256), kMergeMaxP);
+    }
+    REQUIRE(P <= kMergeMaxP, HRNN_E_ARG, "nndescent: bucket must be <= 2048");
+    const uint32_t R = (uint32_t)S;
+    const int exact = (h->small_int && h->d <= 128) ? 1 : 0;
+
+    // scratch layout (u32 words unless noted)
+    DevBuf &w = h->s0;    // worst f32[n]
+    DevBuf &fwd = h->s1;  // fn[n*S], fo[n*S]
+    DevBuf &rev = h->s2;  // rn[n*R], ro[n*R]
+    DevBuf &cnts = h->s4; // fn_cnt, fo_cnt, rn_cnt, ro_cnt, prop_cnt [n each]
+    DevBuf &prop = h->s3; // prop_id[n*P], prop_d[n*P]
+    w.reserve(n * 4 + 64);
+    fwd.reserve(n * S * 8 + 64);
+    rev.reserve(n * R * 8 + 64);
+    cnts.reserve(n * 20 + 64);
+    prop.reserve(n * (uint64_t)P * 8 + 64);
+    h->csr_valid = false;   // the graph changes: any CSR built before is stale
+    float *worst = w.as<float>();
+    uint32_t *fn = fwd.as<uint32_t>(), *fo = fn + n * S;
+    uint32_t *rn = rev.as<uint32_t>(), *ro = rn + n * R;
+    uint32_t *fn_cnt = cnts.as<uint32_t>(), *fo_cnt = fn_cnt + n, *rn_cnt = fo_cnt + n,
+             *ro_cnt = rn_cnt + n, *prop_cnt = ro_cnt + n;
+    uint32_t *prop_id = prop.as<uint32_t>();
+    float *prop_d = reinterpret_cast<float *>(prop_id + n * (uint64_t)P);
+    unsigned long long *stats = reinterpret_cast<unsigned long long *>(h->counters.as<uint64_t>() + 64);
+
+    uint32_t *ids = h->knn_ids.as<uint32_t>();
+    float *kd = h->knn_d.as<float>();
+    uint32_t *cnt = h->knn_cnt.as<uint32_t>();
+    uint8_t *isnew = h->knn_new.as<uint8_t>();
+
+    for (int it = 0; it < iters; it++) {
+        auto t0 = std::chrono::steady_clock::now();
+        hipLaunchKernelGGL(k_nnd_prepare, dim3(ceil_div(n, 256)), dim3(256), 0, h->stream, kd, cnt, n,
+                           K, worst, rn_cnt, ro_cnt, prop_cnt, stats);
+        hipLaunchKernelGGL(k_nnd_sample, dim3(ceil_div(n, 4)), dim3(256), 0, h->stream, ids, kd, isnew,
+                           cnt, n, K, (uint32_t)S, R, worst, fn, fn_cnt, fo, fo_cnt, rn, rn_cnt, ro,
+                           ro_cnt);
+        hipLaunchKernelGGL(k_nnd_join, dim3((unsigned)n), dim3(256), 0, h->stream,
+                           h->xb.as<__hip_bfloat16>(), h->norms.as<float>(), h->x.as<float>(), h->d,
+                           n, (uint32_t)S, R, fn, fn_cnt, fo, fo_cnt, rn, rn_cnt, ro, ro_cnt, worst,
+                           prop_cnt, prop_id, prop_d, P, exact, stats);
+        hipLaunchKernelGGL(k_nnd_merge, dim3((unsigned)n), dim3(256), 0, h->stream, ids, kd, isnew,
+                           cnt, K, prop_cnt, prop_id, prop_d, P, stats);
+        HIPC(hipGetLastError());
+        unsigned long long sv[ST_N];
+        d2h(h, sv, stats, sizeof(sv));
+        auto t1 = std::chrono::steady_clock::now();
+        if (st) {
+            st[it].dists = sv[ST_DISTS];
+            st[it].proposals = sv[ST_PROPOSALS];
+            st[it].dropped = sv[ST_DROPPED];
+            st[it].updates = sv[ST_UPDATES];
+            st[it].ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        }
+    }
+}
+
+}  // namespace hrnn

@@ -1,0 +1,309 @@
+"""ctypes binding of libhrnn_hip.so (the C-ABI declared in include/hrnn_dev.h).
+
+This is the Python host mirror of the device layer: every method calls one
+C-ABI entry point; nothing here computes results on the CPU.  The library must
+have been built (``python -m hrnn_amd.build``); importing this module without
+it raises at first use, and a missing GPU raises HrnnError from ``Device()``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libhrnn_hip.so")
+
+PREC_F32, PREC_BF16, PREC_F16 = 0, 1, 2
+_PREC = {"f32": PREC_F32, "fp32": PREC_F32, "bf16": PREC_BF16, "f16": PREC_F16, "fp16": PREC_F16}
+
+STATUS = {0: "OK", -1: "E_ARG", -2: "E_HIP", -3: "E_STATE", -4: "E_OOM", -5: "E_CAP",
+          -6: "E_RANGE"}
+
+# Every symbol include/hrnn_dev.h declares (checked by the tests).
+EXPORTS = [
+    "hrnn_dev_device_count", "hrnn_dev_open", "hrnn_dev_close", "hrnn_dev_last_error",
+    "hrnn_dev_sync", "hrnn_dev_stream", "hrnn_dev_set_base", "hrnn_dev_set_base_dptr",
+    "hrnn_dev_append_base", "hrnn_dev_base_is_small_int", "hrnn_dev_base_ptr",
+    "hrnn_dev_set_knng", "hrnn_dev_get_knng", "hrnn_dev_knng_ptrs", "hrnn_dev_nndescent",
+    "hrnn_dev_build_rknn_csr", "hrnn_dev_get_rknn_csr", "hrnn_dev_set_rknn_csr",
+    "hrnn_dev_rknn_search", "hrnn_dev_rknn_search_dptr", "hrnn_dev_bf_kth", "hrnn_dev_bf_knn",
+    "hrnn_dev_bf_rknn", "hrnn_dev_l2sqr_pairs", "hrnn_synth_mixture",
+    "hrnn_synth_halfnormal_scaled", "hrnn_synth_sphere_mixture",
+]
+
+
+class HrnnError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class NndIterStats(C.Structure):
+    _fields_ = [("dists", C.c_uint64), ("updates", C.c_uint64), ("proposals", C.c_uint64),
+                ("dropped", C.c_uint64), ("ms", C.c_double)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libhrnn_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(
+            f"{LIB_PATH} is missing: build it with `python -m hrnn_amd.build` "
+            "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp, u32, u64, i, f = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_float
+    P = C.POINTER
+    sig = {
+        "hrnn_dev_device_count": [P(i)],
+        "hrnn_dev_open": [i, P(vp)],
+        "hrnn_dev_close": [vp],
+        "hrnn_dev_sync": [vp],
+        "hrnn_dev_set_base": [vp, vp, u64, u32],
+        "hrnn_dev_set_base_dptr": [vp, vp, u64, u32],
+        "hrnn_dev_append_base": [vp, vp, u64],
+        "hrnn_dev_base_is_small_int": [vp, P(i)],
+        "hrnn_dev_set_knng": [vp, u32, vp, vp, vp],
+        "hrnn_dev_get_knng": [vp, vp, vp, vp],
+        "hrnn_dev_knng_ptrs": [vp, vp, vp, vp],
+        "hrnn_dev_nndescent": [vp, i, i, u32, vp],
+        "hrnn_dev_build_rknn_csr": [vp, P(u32), P(i), P(u64)],
+        "hrnn_dev_get_rknn_csr": [vp, vp, vp],
+        "hrnn_dev_set_rknn_csr": [vp, u32, i, u64, vp, vp, u64],
+        "hrnn_dev_rknn_search": [vp, vp, u32, vp, u32, vp, u32, u32, f, vp, vp, u64, P(u64)],
+        "hrnn_dev_rknn_search_dptr": [vp, vp, u32, vp, u32, vp, u32, u32, f, vp, vp, u32],
+        "hrnn_dev_bf_kth": [vp, u32, i, vp, vp],
+        "hrnn_dev_bf_knn": [vp, vp, u32, u32, i, vp, vp],
+        "hrnn_dev_bf_rknn": [vp, vp, u32, vp, i, vp, vp, vp, u64, P(u64)],
+        "hrnn_dev_l2sqr_pairs": [vp, vp, u64, vp, vp, u64, vp],
+        "hrnn_synth_mixture": [u64, u64, u32, u32, f, f, f, i, C.c_double, u64, vp, vp],
+        "hrnn_synth_halfnormal_scaled": [u64, u64, u64, u32, vp],
+        "hrnn_synth_sphere_mixture": [u64, u64, u64, u32, u32, f, vp, vp],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    L.hrnn_dev_last_error.argtypes = [vp]
+    L.hrnn_dev_last_error.restype = C.c_char_p
+    L.hrnn_dev_stream.argtypes = [vp]
+    L.hrnn_dev_stream.restype = vp
+    L.hrnn_dev_base_ptr.argtypes = [vp]
+    L.hrnn_dev_base_ptr.restype = vp
+    _lib = L
+    return L
+
+
+def _p(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def device_count() -> int:
+    c = C.c_int(0)
+    rc = lib().hrnn_dev_device_count(C.byref(c))
+    return c.value if rc == 0 else 0
+
+
+# ---- synthetic data (host, deterministic) ---------------------------------
+
+def synth_mixture(seed: int, n: int, d: int, n_centres: int, sigma: float, lo=0.0, hi=255.0,
+                  round_int=True, zipf_s=0.0, centre_seed: int = 0, return_centres=False):
+    out = np.empty((n, d), np.float32)
+    cen = np.empty(n, np.uint32) if return_centres else None
+    rc = lib().hrnn_synth_mixture(seed, n, d, n_centres, sigma, lo, hi, int(round_int),
+                                  float(zipf_s), centre_seed, _p(out), _p(cen))
+    if rc:
+        raise HrnnError(rc, "synth_mixture")
+    return (out, cen) if return_centres else out
+
+
+def synth_halfnormal_scaled(seed: int, scale_seed: int, n: int, d: int):
+    out = np.empty((n, d), np.float32)
+    rc = lib().hrnn_synth_halfnormal_scaled(seed, scale_seed, n, d, _p(out))
+    if rc:
+        raise HrnnError(rc, "synth_halfnormal_scaled")
+    return out
+
+
+def synth_sphere_mixture(seed: int, centre_seed: int, n: int, d: int, n_centres: int,
+                         kappa: float, return_centres=False):
+    out = np.empty((n, d), np.float32)
+    cen = np.empty(n, np.uint32) if return_centres else None
+    rc = lib().hrnn_synth_sphere_mixture(seed, centre_seed, n, d, n_centres, kappa, _p(out),
+                                         _p(cen))
+    if rc:
+        raise HrnnError(rc, "synth_sphere_mixture")
+    return (out, cen) if return_centres else out
+
+
+@dataclass
+class CsrInfo:
+    rank_bits: int
+    wide: int
+    n_entries: int
+
+
+class Device:
+    """One device-resident HRNN index state (an hrnn_dev_t handle)."""
+
+    def __init__(self, device: int = 0):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.hrnn_dev_open(device, C.byref(h))
+        if rc:
+            raise HrnnError(rc, f"hrnn_dev_open({device}) failed (no usable GPU?)")
+        self.h = h
+        self.n = 0
+        self.d = 0
+        self.K = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.hrnn_dev_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc: int):
+        if rc != 0:
+            raise HrnnError(rc, self.L.hrnn_dev_last_error(self.h).decode())
+
+    # -- base / graph ------------------------------------------------------
+    def set_base(self, x: np.ndarray):
+        x = np.ascontiguousarray(x, np.float32)
+        self._ck(self.L.hrnn_dev_set_base(self.h, _p(x), x.shape[0], x.shape[1]))
+        self.n, self.d = x.shape
+
+    def set_base_dptr(self, ptr: int, n: int, d: int):
+        self._ck(self.L.hrnn_dev_set_base_dptr(self.h, C.c_void_p(ptr), n, d))
+        self.n, self.d = n, d
+
+    def append_base(self, x: np.ndarray):
+        x = np.ascontiguousarray(x, np.float32)
+        self._ck(self.L.hrnn_dev_append_base(self.h, _p(x), x.shape[0]))
+        self.n += x.shape[0]
+
+    def base_is_small_int(self) -> bool:
+        v = C.c_int(0)
+        self._ck(self.L.hrnn_dev_base_is_small_int(self.h, C.byref(v)))
+        return bool(v.value)
+
+    def set_knng(self, ids, dists, cnt):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        dists = np.ascontiguousarray(dists, np.float32)
+        cnt = np.ascontiguousarray(cnt, np.uint16)
+        K = ids.shape[1] if ids.ndim == 2 else ids.size // self.n
+        self._ck(self.L.hrnn_dev_set_knng(self.h, K, _p(ids), _p(dists), _p(cnt)))
+        self.K = K
+
+    def get_knng(self):
+        ids = np.empty((self.n, self.K), np.uint32)
+        dists = np.empty((self.n, self.K), np.float32)
+        cnt = np.empty(self.n, np.uint16)
+        self._ck(self.L.hrnn_dev_get_knng(self.h, _p(ids), _p(dists), _p(cnt)))
+        return ids, dists, cnt
+
+    def nndescent(self, iters: int, sample: int, bucket: int = 0):
+        st = (NndIterStats * max(iters, 1))()
+        self._ck(self.L.hrnn_dev_nndescent(self.h, iters, sample, bucket, st))
+        return [dict(dists=s.dists, updates=s.updates, proposals=s.proposals,
+                     dropped=s.dropped, ms=s.ms) for s in st[:iters]]
+
+    # -- reverse CSR -----------------------------------------------------------
+    def build_rknn_csr(self) -> CsrInfo:
+        rb, wide, ne = C.c_uint32(0), C.c_int(0), C.c_uint64(0)
+        self._ck(self.L.hrnn_dev_build_rknn_csr(self.h, C.byref(rb), C.byref(wide), C.byref(ne)))
+        self.csr = CsrInfo(rb.value, wide.value, ne.value)
+        return self.csr
+
+    def get_rknn_csr(self):
+        off = np.empty(self.n + 1, np.uint64)
+        ent = np.empty(self.csr.n_entries, np.uint64 if self.csr.wide else np.uint32)
+        self._ck(self.L.hrnn_dev_get_rknn_csr(self.h, _p(off), _p(ent)))
+        return off, ent
+
+    def set_rknn_csr(self, rank_bits, wide, offsets, entries):
+        offsets = np.ascontiguousarray(offsets, np.uint64)
+        entries = np.ascontiguousarray(entries, np.uint64 if wide else np.uint32)
+        self._ck(self.L.hrnn_dev_set_rknn_csr(self.h, rank_bits, int(wide), self.n, _p(offsets),
+                                              _p(entries), entries.size))
+        self.csr = CsrInfo(rank_bits, int(wide), entries.size)
+
+    # -- search ----------------------------------------------------------------
+    def rknn_search(self, q, proxies, k, K_prime, threshold=1.0, k_per_q=None):
+        q = np.ascontiguousarray(q, np.float32)
+        nq = q.shape[0]
+        proxies = np.ascontiguousarray(proxies, np.uint32).reshape(nq, -1)
+        m = proxies.shape[1]
+        kp = None if k_per_q is None else np.ascontiguousarray(k_per_q, np.uint32)
+        off = np.zeros(nq + 1, np.uint64)
+        need = C.c_uint64(0)
+        cap = max(1024, nq * 64)
+        while True:
+            ids = np.empty(cap, np.uint32)
+            rc = self.L.hrnn_dev_rknn_search(self.h, _p(q), nq, _p(proxies), m, _p(kp), k,
+                                             K_prime, threshold, _p(off), _p(ids), cap,
+                                             C.byref(need))
+            if rc == -5 and need.value > cap:
+                cap = int(need.value)
+                continue
+            self._ck(rc)
+            return off, ids[: int(off[-1])].copy()
+
+    # -- brute force -----------------------------------------------------------
+    def bf_kth(self, k: int, prec="bf16", with_ids=False):
+        kk = min(k + 1, self.n)
+        kth = np.empty(self.n, np.float32)
+        ids = np.empty((self.n, kk), np.uint32) if with_ids else None
+        self._ck(self.L.hrnn_dev_bf_kth(self.h, k, _PREC.get(prec, prec), _p(kth), _p(ids)))
+        return (kth, ids) if with_ids else kth
+
+    def bf_knn(self, q, k: int, prec="bf16"):
+        q = np.ascontiguousarray(q, np.float32)
+        ids = np.empty((q.shape[0], k), np.uint32)
+        d2 = np.empty((q.shape[0], k), np.float32)
+        self._ck(self.L.hrnn_dev_bf_knn(self.h, _p(q), q.shape[0], k, _PREC.get(prec, prec),
+                                        _p(ids), _p(d2)))
+        return ids, d2
+
+    def bf_rknn(self, q, kth, prec="bf16"):
+        q = np.ascontiguousarray(q, np.float32)
+        kth = np.ascontiguousarray(kth, np.float32)
+        nq = q.shape[0]
+        off = np.zeros(nq + 1, np.uint64)
+        need = C.c_uint64(0)
+        cap = max(1024, nq * 64)
+        while True:
+            ids = np.empty(cap, np.uint32)
+            d2 = np.empty(cap, np.float32)
+            rc = self.L.hrnn_dev_bf_rknn(self.h, _p(q), nq, _p(kth), _PREC.get(prec, prec),
+                                         _p(off), _p(ids), _p(d2), cap, C.byref(need))
+            if rc == -5 and need.value > cap:
+                cap = int(need.value)
+                continue
+            self._ck(rc)
+            n = int(off[-1])
+            return off, ids[:n].copy(), d2[:n].copy()
+
+    def l2sqr_pairs(self, a, ia, ib):
+        a = np.ascontiguousarray(a, np.float32)
+        ia = np.ascontiguousarray(ia, np.uint32)
+        ib = np.ascontiguousarray(ib, np.uint32)
+        out = np.empty(ia.size, np.float32)
+        self._ck(self.L.hrnn_dev_l2sqr_pairs(self.h, _p(a), a.shape[0], _p(ia), _p(ib), ia.size,
+                                             _p(out)))
+        return out
+
+    def sync(self):
+        self._ck(self.L.hrnn_dev_sync(self.h))

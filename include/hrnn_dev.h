@@ -1,0 +1,186 @@
+/*
+ * hrnn_dev.h -- C-ABI of libhrnn_hip.so, the MI355X (gfx950) device layer
+ * beneath hnswlib::HRNN<float>.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (HRNN_OK == 0, negative on error) and never throws.  The text of
+ * the last error of a handle is available from hrnn_dev_last_error().
+ *
+ * A handle owns one HIP stream and all device memory of one index on one GPU:
+ * base vectors (FP32 plus a BF16 mirror and FP32 row norms), the ranked KNN
+ * graph (SoA: ids u32[n*K], dists f32[n*K], counts u32[n], new-flags u8[n*K])
+ * and the reverse-KNN CSR.  Host buffers passed in are caller-owned and are
+ * only read or written during the call.  A handle is not re-entrant: calls on
+ * one handle must be serialised by the caller.
+ *
+ * Entry points whose name ends in _dptr take device pointers (memory already
+ * resident in HBM) instead of host pointers.
+ *
+ * Reference interfaces replaced (file:line in the reference repository):
+ *   hrnn_dev_nndescent        HRNN::refineKNNG              include/hnswlib/hrnn.h:1725
+ *   hrnn_dev_build_rknn_csr   HRNN::buildRKNNG / _fill      include/hnswlib/hrnn.h:1869, :1912
+ *   hrnn_dev_rknn_search      HRNN::searchRknn_impl phases B+C include/hnswlib/hrnn.h:2021-2069
+ *   hrnn_dev_bf_kth           compute_kth_nn_dists          scripts/gen_rknn_gt.py:42-79
+ *   hrnn_dev_bf_knn           BruteforceSearch::searchKnn   include/hnswlib/bruteforce.h:106-135
+ *   hrnn_dev_bf_rknn          compute_rknn                  scripts/gen_rknn_gt.py:82-151
+ *   hrnn_dev_l2sqr_pairs      L2Sqr / L2SqrSIMD16ExtAVX512  include/hnswlib/space_l2.h:6-54
+ */
+#ifndef HRNN_DEV_H
+#define HRNN_DEV_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hrnn_dev *hrnn_dev_t;
+
+enum hrnn_status {
+    HRNN_OK = 0,
+    HRNN_E_ARG = -1,      /* invalid argument (null pointer, size out of range) */
+    HRNN_E_HIP = -2,      /* a HIP runtime call failed */
+    HRNN_E_STATE = -3,    /* call out of order (e.g. search before the CSR exists) */
+    HRNN_E_OOM = -4,      /* device allocation failed */
+    HRNN_E_CAP = -5,      /* an output capacity was too small; *needed is set */
+    HRNN_E_RANGE = -6     /* packing overflow (node_bits + rank_bits > 64) */
+};
+
+enum hrnn_precision {
+    HRNN_PREC_F32 = 0,    /* FP32 direct difference (reference summation order) */
+    HRNN_PREC_BF16 = 1,   /* BF16 inputs on MFMA, FP32 accumulate, FP32 re-rank */
+    HRNN_PREC_F16 = 2     /* FP16 inputs on MFMA, FP32 accumulate, FP32 re-rank */
+};
+
+/* ---- handle ---------------------------------------------------------- */
+int hrnn_dev_device_count(int *count);
+int hrnn_dev_open(int device, hrnn_dev_t *out);
+int hrnn_dev_close(hrnn_dev_t h);
+const char *hrnn_dev_last_error(hrnn_dev_t h);
+/* Wait for all work queued on the handle's stream. */
+int hrnn_dev_sync(hrnn_dev_t h);
+/* The handle's HIP stream (a hipStream_t), for callers that time or order work. */
+void *hrnn_dev_stream(hrnn_dev_t h);
+
+/* ---- base vectors ------------------------------------------------------ */
+/* Copy n x d row-major FP32 vectors into HBM; builds the BF16/FP16 mirrors and
+ * FP32 norms.  Replaces any previous base and invalidates the KNN graph and CSR. */
+int hrnn_dev_set_base(hrnn_dev_t h, const float *x, uint64_t n, uint32_t d);
+int hrnn_dev_set_base_dptr(hrnn_dev_t h, const float *x_dev, uint64_t n, uint32_t d);
+/* Append rows [n_old, n_old + n_add) (dynamic insert); keeps the KNN graph. */
+int hrnn_dev_append_base(hrnn_dev_t h, const float *x, uint64_t n_add);
+/* 1 if every base value is an integer in [-256, 256] (BF16/FP16-exact data). */
+int hrnn_dev_base_is_small_int(hrnn_dev_t h, int *is_int);
+/* Device pointer of the FP32 base (n*d floats). */
+const float *hrnn_dev_base_ptr(hrnn_dev_t h);
+
+/* ---- ranked KNN graph --------------------------------------------------- */
+/* Host layout (SoA): ids[n*K], dists[n*K] (squared L2), cnt[n] (u16, <= K).
+ * Entries j >= cnt[i] are ignored.  All entries start with the "new" flag set,
+ * as at the start of refineKNNG (hrnn.h:1744-1748). */
+int hrnn_dev_set_knng(hrnn_dev_t h, uint32_t K, const uint32_t *ids,
+                      const float *dists, const uint16_t *cnt);
+int hrnn_dev_get_knng(hrnn_dev_t h, uint32_t *ids, float *dists, uint16_t *cnt);
+/* Device pointers of the resident graph (ids n*K u32, dists n*K f32, cnt n u32). */
+int hrnn_dev_knng_ptrs(hrnn_dev_t h, uint32_t **ids, float **dists, uint32_t **cnt);
+
+/* ---- (a) NNDescent refinement ------------------------------------------ */
+typedef struct hrnn_nnd_iter_stats {
+    uint64_t dists;          /* distance evaluations in the local join */
+    uint64_t updates;        /* entries inserted into K-lists (both directions) */
+    uint64_t proposals;      /* proposals that passed the threshold filter */
+    uint64_t dropped;        /* proposals lost to a full proposal bucket */
+    double ms;               /* wall time of the iteration */
+} hrnn_nnd_iter_stats;
+
+/* num_iters rounds of UPDATE (sample S new / S old, reverse samples capped at
+ * R = S) + JOIN (new x new, new x old) + MERGE on the resident graph.
+ * bucket = proposal capacity per node and iteration (0 = default).  stats may
+ * be NULL or point to num_iters records. */
+int hrnn_dev_nndescent(hrnn_dev_t h, int num_iters, int sample_size, uint32_t bucket,
+                       hrnn_nnd_iter_stats *stats);
+
+/* ---- (b) reverse-KNN CSR ----------------------------------------------- */
+/* Transpose the resident KNN graph.  rank_bits = ceil(log2 K), node_bits =
+ * ceil(log2 n); entries are u32 if node_bits + rank_bits <= 32, else u64
+ * (*wide = 1); each entry is (src << rank_bits) | rank, and each list is in
+ * (rank, src) order.  Offsets are u64[n+1].  The CSR stays resident. */
+int hrnn_dev_build_rknn_csr(hrnn_dev_t h, uint32_t *rank_bits, int *wide,
+                            uint64_t *n_entries);
+/* Download offsets (n+1 u64) and entries (n_entries x 4 or 8 bytes). */
+int hrnn_dev_get_rknn_csr(hrnn_dev_t h, uint64_t *offsets, void *entries);
+/* Upload a CSR built elsewhere (e.g. loaded from an index file). */
+int hrnn_dev_set_rknn_csr(hrnn_dev_t h, uint32_t rank_bits, int wide, uint64_t n,
+                          const uint64_t *offsets, const void *entries, uint64_t n_entries);
+
+/* ---- (c) RkNN search, phases B and C ----------------------------------- */
+/* For each query q (nq rows of d floats) and its m proxies (internal ids,
+ * proxies[q*m .. q*m+m)): collect the proxies' reverse-list prefixes with
+ * rank < K_prime, keep candidates whose proxy count >= threshold, and accept
+ * candidate c if L2Sqr(q, c) <= dist(c, k-1) (FLT_MAX when c has < k
+ * neighbours).  k_per_q may be NULL (then every query uses k).  Accepted ids
+ * are returned per query in ascending id order: res_off[nq+1] (u64) and
+ * res_ids[res_off[nq]].  If cap < total, returns HRNN_E_CAP and sets *needed.
+ * proxy ids equal to 0xFFFFFFFF are skipped (a query with fewer proxies). */
+int hrnn_dev_rknn_search(hrnn_dev_t h, const float *q, uint32_t nq,
+                         const uint32_t *proxies, uint32_t m,
+                         const uint32_t *k_per_q, uint32_t k, uint32_t K_prime,
+                         float threshold, uint64_t *res_off, uint32_t *res_ids,
+                         uint64_t cap, uint64_t *needed);
+/* Resident form: q_dev, proxies_dev, k_per_q_dev in HBM; per-query results
+ * written to res_cnt_dev[nq] (u32) and res_ids_dev[q * cap_per_q ...].  A query
+ * whose result count exceeds cap_per_q keeps the first cap_per_q ids and its
+ * count is the true count.  Runs as one persistent launch that pulls queries
+ * from a device work queue.  Asynchronous on the handle's stream. */
+int hrnn_dev_rknn_search_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq,
+                              const uint32_t *proxies_dev, uint32_t m,
+                              const uint32_t *k_per_q_dev, uint32_t k, uint32_t K_prime,
+                              float threshold, uint32_t *res_cnt_dev,
+                              uint32_t *res_ids_dev, uint32_t cap_per_q);
+
+/* ---- (d) exact brute force --------------------------------------------- */
+/* kth[i] = (k+1)-th smallest squared distance from base row i to all base rows
+ * (self included), i.e. the k-th neighbour distance r_k(i).  prec selects the
+ * candidate pass; the final value is the FP32 direct-difference distance after
+ * re-ranking the top 2(k+1) candidates by (dist, id).  ids (may be NULL) gets
+ * the (k+1) nearest ids per row in (dist, id) order, self included. */
+int hrnn_dev_bf_kth(hrnn_dev_t h, uint32_t k, int prec, float *kth, uint32_t *ids);
+/* Exact kNN of nq host queries against the base: ids/d2 [nq*k] in (dist, id)
+ * order (ties to the smaller id). */
+int hrnn_dev_bf_knn(hrnn_dev_t h, const float *q, uint32_t nq, uint32_t k, int prec,
+                    uint32_t *ids, float *d2);
+/* RkNN groundtruth: for each query, every base row x with
+ * d2(q, x) <= kth[x] (FP32 direct difference), sorted by (d2, id). */
+int hrnn_dev_bf_rknn(hrnn_dev_t h, const float *q, uint32_t nq, const float *kth,
+                     int prec, uint64_t *res_off, uint32_t *res_ids, float *res_d2,
+                     uint64_t cap, uint64_t *needed);
+
+/* ---- distances ---------------------------------------------------------- */
+/* out[i] = L2Sqr(a[ia[i]], base[ib[i]]) in FP32 with the reference's 16-lane
+ * partial-sum order (d % 16 == 0) or sequential order otherwise. */
+int hrnn_dev_l2sqr_pairs(hrnn_dev_t h, const float *a, uint64_t na, const uint32_t *ia,
+                         const uint32_t *ib, uint64_t npairs, float *out);
+
+/* ---- synthetic data (host) ---------------------------------------------- */
+/* Gaussian mixture: n_centres centres uniform in [lo, hi]^d; each row picks a
+ * centre (uniform, or Zipf(zipf_s) when zipf_s > 0), adds sigma * N(0,1) per
+ * coordinate, rounds to an integer if round_int, and clips to [lo, hi].
+ * Counter-based (Philox4x32-10 + Box-Muller): the same seed gives the same
+ * bytes on any host and thread count.  centre_of (may be NULL) gets the centre
+ * index of each row. */
+int hrnn_synth_mixture(uint64_t seed, uint64_t n, uint32_t d, uint32_t n_centres,
+                       float sigma, float lo, float hi, int round_int, double zipf_s,
+                       uint64_t centre_seed, float *out, uint32_t *centre_of);
+/* |N(0,1)| * s_j with s_j ~ U(0.05, 1) per dimension (scale seeded by scale_seed). */
+int hrnn_synth_halfnormal_scaled(uint64_t seed, uint64_t scale_seed, uint64_t n,
+                                 uint32_t d, float *out);
+/* L2-normalised vMF-like mixture: x = normalise(mu_c + z / sqrt(kappa)). */
+int hrnn_synth_sphere_mixture(uint64_t seed, uint64_t centre_seed, uint64_t n, uint32_t d,
+                              uint32_t n_centres, float kappa, float *out,
+                              uint32_t *centre_of);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HRNN_DEV_H */

@@ -1,0 +1,305 @@
+"""GPU parity: each HIP kernel through the C-ABI against the CPU oracle.
+
+Bars: bit-exact for ids, CSR entries and integer-data distances; FP32
+distances in the reference's summation order bit-exact; NNDescent (a
+randomised, race-ordered algorithm in the reference too) within 0.5 recall
+points of the oracle's single-threaded restatement.
+"""
+import numpy as np
+import pytest
+
+import hrnn_amd
+import oracle
+from conftest import mixture
+
+pytestmark = pytest.mark.gpu
+
+
+def _canon(off, ent, rb):
+    out = ent.copy()
+    mask = (1 << rb) - 1
+    for dst in range(len(off) - 1):
+        seg = ent[off[dst]:off[dst + 1]].astype(np.int64)
+        keys = np.lexsort((seg >> rb, seg & mask))
+        out[off[dst]:off[dst + 1]] = seg[keys].astype(ent.dtype)
+    return out
+
+
+# ---- distances -------------------------------------------------------------------
+
+@pytest.mark.parametrize("d", [96, 128, 960, 40])
+def test_l2_pairs_reference_order(d):
+    rng = np.random.default_rng(d)
+    dev = hrnn_amd.Device(0)
+    x = rng.standard_normal((300, d)).astype(np.float32)
+    a = rng.standard_normal((50, d)).astype(np.float32) * 2
+    dev.set_base(x)
+    ia = rng.integers(0, 50, 2000).astype(np.uint32)
+    ib = rng.integers(0, 300, 2000).astype(np.uint32)
+    got = dev.l2sqr_pairs(a, ia, ib)
+    W = 16 if d % 16 == 0 else 1
+    exp = np.array([oracle.l2sqr(a[i], x[j], W=W) for i, j in zip(ia, ib)], np.float32)
+    assert np.array_equal(got, exp)
+
+
+# ---- reverse CSR -------------------------------------------------------------------
+
+def _graph(n, K, seed, full=False):
+    rng = np.random.default_rng(seed)
+    cnt = np.full(n, K, np.uint16) if full else rng.integers(0, K + 1, n).astype(np.uint16)
+    ids = np.argsort(rng.random((n, n)) if n <= 3000 else rng.random((n, K * 2)), axis=1)[:, :K]
+    return ids.astype(np.uint32), cnt
+
+
+def _csr_check(dev, x, ids, cnt, K):
+    dev.set_base(x)
+    dev.set_knng(ids, np.zeros(ids.shape, np.float32), cnt)
+    info = dev.build_rknn_csr()
+    off, ent = dev.get_rknn_csr()
+    eoff, eent, erb, ewide = oracle.csr(ids, cnt, K)
+    assert (info.rank_bits, info.wide) == (erb, ewide)
+    assert np.array_equal(off, eoff)
+    assert np.array_equal(ent, eent)
+    return off, ent
+
+
+def test_csr_bit_exact_random():
+    dev = hrnn_amd.Device(0)
+    x = np.zeros((2500, 16), np.float32)
+    ids, cnt = _graph(2500, 100, 1)
+    _csr_check(dev, x, ids, cnt, 100)
+
+
+def test_csr_bit_exact_reference_graph(small_data, small_ref):
+    x, _ = small_data
+    r, _ = small_ref
+    ids, dists, cnt = r.get_knng()
+    dev = hrnn_amd.Device(0)
+    off, ent = _csr_check(dev, x, ids, cnt, r.K)
+    roff, rent, rb, _ = r.get_csr()
+    assert np.array_equal(ent, _canon(roff, rent, rb))
+
+
+def test_csr_long_lists_and_empty():
+    """Hub lists longer than the LDS sort capacity, and nodes with no in-edges."""
+    n, K = 40000, 8
+    rng = np.random.default_rng(9)
+    ids = rng.integers(5, n, (n, K)).astype(np.uint32)
+    ids[:, 0] = 0                       # node 0: in-degree n at rank 0
+    ids[: n // 2, 1] = 1                # node 1: n/2 at rank 1
+    ids[:, 2] = np.where(np.arange(n) % 2 == 0, 2, ids[:, 2])
+    for i in range(n):                  # keep each row duplicate-free
+        row = ids[i]
+        _, first = np.unique(row, return_index=True)
+        dup = np.setdiff1d(np.arange(K), first)
+        row[dup] = (row[dup] + 7 + i) % (n - 5) + 5
+    cnt = np.full(n, K, np.uint16)
+    cnt[::3] = 3
+    dev = hrnn_amd.Device(0)
+    _csr_check(dev, np.zeros((n, 16), np.float32), ids, cnt, K)
+
+
+def test_csr_k500_shape():
+    """K = 500 (rank_bits 9) on a graph with ties of in-degree across ranks."""
+    n, K = 3000, 500
+    ids, cnt = _graph(n, K, 4, full=True)
+    dev = hrnn_amd.Device(0)
+    _csr_check(dev, np.zeros((n, 16), np.float32), ids, cnt, K)
+
+
+def test_csr_deterministic_across_runs():
+    dev = hrnn_amd.Device(0)
+    x = np.zeros((2000, 16), np.float32)
+    ids, cnt = _graph(2000, 64, 2)
+    dev.set_base(x)
+    dev.set_knng(ids, np.zeros(ids.shape, np.float32), cnt)
+    dev.build_rknn_csr()
+    a = dev.get_rknn_csr()
+    dev.build_rknn_csr()
+    b = dev.get_rknn_csr()
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+# ---- search phases B + C ------------------------------------------------------------
+
+@pytest.mark.parametrize("m,Kp,k,thr", [(1, 48, 10, 1.0), (5, 20, 10, 1.0), (10, 48, 5, 1.0),
+                                         (10, 48, 10, 2.0), (20, 48, 48, 1.0), (3, 1, 1, 1.0)])
+def test_search_vs_oracle(small_data, small_ref, m, Kp, k, thr):
+    x, q = small_data
+    r, _ = small_ref
+    ids, dists, cnt = r.get_knng()
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    dev.set_knng(ids, dists, cnt)
+    dev.build_rknn_csr()
+    off, ent = dev.get_rknn_csr()
+    prox = r.proxies(q, m, 64)
+    goff, gids = dev.rknn_search(q, prox, k, Kp, thr)
+    eoff, eids = oracle.search_bc(off, ent, dev.csr.wide, dev.csr.rank_bits, dists, cnt, r.K, x,
+                                  q, prox, k, Kp, thr)
+    assert np.array_equal(goff, eoff)
+    assert np.array_equal(gids, eids)
+    if thr == 1.0 and Kp == r.K:
+        # end to end against the reference's own searchRknn
+        roff, rlab = r.search(q, m, thr, k, Kp, 64)
+        assert np.array_equal(gids.astype(np.uint64), rlab)
+
+
+def test_search_float_data_and_k_per_query(float_data):
+    """Float vectors: accept/reject uses the reference's FP32 order, so results are identical."""
+    x, q = float_data
+    n, K = x.shape[0], 24
+    ids, d2 = oracle.knn(x, x, K + 1)
+    ids, d2 = ids[:, 1:].copy(), d2[:, 1:].copy()       # drop self
+    cnt = np.full(n, K, np.uint16)
+    cnt[::7] = 5                                         # some short lists
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    dev.set_knng(ids, d2, cnt)
+    dev.build_rknn_csr()
+    off, ent = dev.get_rknn_csr()
+    rng = np.random.default_rng(3)
+    prox = rng.integers(0, n, (q.shape[0], 12)).astype(np.uint32)
+    prox[0, 5:] = 0xFFFFFFFF                             # a query with fewer proxies
+    kq = rng.choice([1, 5, 10], q.shape[0]).astype(np.uint32)
+    goff, gids = dev.rknn_search(q, prox, 0, 16, 1.0, k_per_q=kq)
+    eoff, eids = oracle.search_bc(off, ent, 0, dev.csr.rank_bits, d2, cnt, K, x, q, prox, 0, 16,
+                                  1.0, k_per_q=kq)
+    assert np.array_equal(goff, eoff) and np.array_equal(gids, eids)
+
+
+def test_search_rejects_bad_k(small_data, small_ref):
+    x, q = small_data
+    r, _ = small_ref
+    ids, dists, cnt = r.get_knng()
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    dev.set_knng(ids, dists, cnt)
+    with pytest.raises(hrnn_amd.HrnnError):
+        dev.rknn_search(q, np.zeros((q.shape[0], 2), np.uint32), 10, 10)   # no CSR yet
+    dev.build_rknn_csr()
+    with pytest.raises(hrnn_amd.HrnnError):
+        dev.rknn_search(q, np.zeros((q.shape[0], 2), np.uint32), r.K + 1, 10)
+
+
+# ---- brute force ---------------------------------------------------------------------
+
+@pytest.mark.parametrize("prec", ["bf16", "f16", "f32"])
+def test_bf_kth_and_ids_integer_data(small_data, prec):
+    x, _ = small_data
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    assert dev.base_is_small_int()
+    k = 10
+    kth, ids = dev.bf_kth(k, prec, with_ids=True)
+    eids, ed = oracle.knn(x, x, k + 1)
+    assert np.array_equal(ids, eids)
+    assert np.array_equal(kth, ed[:, k])
+
+
+@pytest.mark.parametrize("prec", ["bf16", "f32"])
+def test_bf_knn_queries(small_data, prec):
+    x, q = small_data
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    ids, d2 = dev.bf_knn(q, 17, prec)
+    eids, ed = oracle.knn(x, q, 17)
+    assert np.array_equal(ids, eids) and np.array_equal(d2, ed)
+
+
+def test_bf_kth_float_data_f32(float_data):
+    x, _ = float_data
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    kth, ids = dev.bf_kth(10, "f32", with_ids=True)
+    eids, ed = oracle.knn(x, x, 11)
+    assert np.array_equal(ids, eids)
+    assert np.array_equal(kth, ed[:, 10])
+
+
+def test_bf_kth_float_data_bf16_rerank(float_data):
+    """BF16 candidates + FP32 re-rank: distances are exact FP32 values of the chosen ids."""
+    x, _ = float_data
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    kth, ids = dev.bf_kth(10, "bf16", with_ids=True)
+    eids, ed = oracle.knn(x, x, 11)
+    rel = np.abs(kth - ed[:, 10]) / np.maximum(ed[:, 10], 1e-12)
+    assert np.median(rel) == 0.0
+    assert float((ids == eids).mean()) > 0.97
+
+
+@pytest.mark.parametrize("prec", ["bf16", "f32"])
+def test_bf_rknn_gt(small_data, prec):
+    x, q = small_data
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    kth = oracle.kth(x, 10)
+    off, ids, d2 = dev.bf_rknn(q, kth, prec)
+    eoff, eids, ed = oracle.rknn_gt(x, q, kth)
+    assert np.array_equal(off, eoff)
+    assert np.array_equal(ids, eids) and np.array_equal(d2, ed)
+
+
+def test_bf_rknn_float_data(float_data):
+    x, q = float_data
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    kth = oracle.kth(x, 10)
+    off, ids, d2 = dev.bf_rknn(q, kth, "f32")
+    eoff, eids, ed = oracle.rknn_gt(x, q, kth)
+    assert np.array_equal(off, eoff) and np.array_equal(ids, eids)
+
+
+# ---- NNDescent --------------------------------------------------------------------------
+
+def _knn_recall(ids, cnt, K, exact, kk):
+    return oracle.knng_recall(ids, cnt, K, exact, kk)
+
+
+def test_nndescent_recall_parity(small_data, small_ref):
+    """GPU refineKNNG vs the oracle's single-threaded restatement from the same HNSW seeds."""
+    x, _ = small_data
+    r, seed = small_ref
+    sid, sd, sc = seed
+    K, iters, S = r.K, 3, 12
+    oid, od, oc, _, _ = oracle.nndescent(x, sid, sd, sc, K, iters, S)
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    dev.set_knng(sid, sd, sc)
+    st = dev.nndescent(iters, S)
+    gid, gd, gc = dev.get_knng()
+    exact, _ = oracle.knn(x, x, K + 1)
+    exact = exact[:, 1:].copy()
+    for kk in (10, K):
+        ro = _knn_recall(oid, oc, K, exact, kk)
+        rg = _knn_recall(gid, gc, K, exact, kk)
+        assert abs(rg - ro) <= 0.005, (kk, rg, ro)
+    # every list is sorted, duplicate-free, self-free, with exact distances
+    for i in range(0, x.shape[0], 17):
+        c = int(gc[i])
+        assert np.all(np.diff(gd[i, :c]) >= 0)
+        assert len(set(gid[i, :c].tolist())) == c and i not in gid[i, :c]
+        dd = ((x[gid[i, :c]].astype(np.int64) - x[i].astype(np.int64)) ** 2).sum(1)
+        assert np.array_equal(dd.astype(np.float32), gd[i, :c])
+    assert st[0]["dists"] > 0 and st[0]["updates"] > 0
+
+
+def test_nndescent_non_full_seed():
+    """Lists that start below K (the K=500 > ef_construction=400 regime) fill to K."""
+    x = mixture(1200, d=64, seed=31)
+    r = oracle.RefIndex(1200, 64, M=8, efc=24, K=40)
+    r.add_points(x, nthreads=1)
+    sid, sd, sc = r.get_knng()
+    oid, od, oc, _, _ = oracle.nndescent(x, sid, sd, sc, 40, 3, 8)
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    dev.set_knng(sid, sd, sc)
+    dev.nndescent(3, 8)
+    gid, gd, gc = dev.get_knng()
+    exact, _ = oracle.knn(x, x, 41)
+    exact = exact[:, 1:].copy()
+    ro = _knn_recall(oid, oc, 40, exact, 40)
+    rg = _knn_recall(gid, gc, 40, exact, 40)
+    assert abs(rg - ro) <= 0.005, (rg, ro)
+    assert int(gc.min()) == int(oc.min())
