@@ -93,7 +93,7 @@ def build_drivers() -> list[str]:
         return []
     os.makedirs(BIN, exist_ok=True)
     outs = []
-    for name in ("hrnn_build", "hrnn_search", "hrnn_dynamic_test"):
+    for name in ("hrnn_build", "hrnn_search", "hrnn_dynamic_test", "hrnn_gt"):
         src = os.path.join(ddir, name + ".cpp")
         if not os.path.exists(src):
             continue

@@ -75,6 +75,7 @@ def _lib(variant: str | None = None) -> C.CDLL:
     lib.ref_insert_dynamic.argtypes = [vp, vp, u64, vp, u32, i, i]
     lib.ref_get_level0.argtypes = [vp, vp, vp, u32]
     lib.ref_levels.argtypes = [vp, vp, C.POINTER(i), C.POINTER(u32)]
+    lib.ref_get_labels.argtypes = [vp, vp]
     _libs[path] = lib
     return lib
 
@@ -206,6 +207,11 @@ class RefIndex:
         labels = np.ascontiguousarray(labels, np.uint64)
         self._check(self.lib.ref_insert_dynamic(self.h, _p(x), x.shape[0], _p(labels), m_update,
                                                 int(bidir), nthreads))
+
+    def labels(self):
+        out = np.empty(self.count, np.uint64)
+        self.lib.ref_get_labels(self.h, _p(out))
+        return out
 
     def level0(self, maxM0):
         n = self.count

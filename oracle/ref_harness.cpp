@@ -286,6 +286,13 @@ int ref_get_level0(void *h, uint32_t *links, uint32_t *cnt, uint32_t maxM0) {
     return 0;
 }
 
+int ref_get_labels(void *h, uint64_t *labels) {
+    auto *r = static_cast<RefIndex *>(h);
+    for (size_t i = 0; i < r->idx->cur_element_count; i++)
+        labels[i] = r->idx->getExternalLabel((hnswlib::tableint)i);
+    return 0;
+}
+
 int ref_levels(void *h, int *levels, int *maxlevel, uint32_t *entry) {
     auto *r = static_cast<RefIndex *>(h);
     auto &ix = *r->idx;

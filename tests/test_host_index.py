@@ -1,0 +1,140 @@
+"""Host HRNN<float> (include/hnswlib/hrnn.h) on the CPU: HNSW build quality,
+index file compatibility with the reference (hrnn.h:861-1047) and error paths."""
+import os
+
+import numpy as np
+import pytest
+
+import hrnn_amd
+import oracle
+from conftest import mixture
+from hrnn_amd.index import Index
+
+needs_ref = pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built")
+
+
+@pytest.fixture(scope="module")
+def data():
+    return mixture(2500, seed=51, centre_seed=1051), mixture(40, seed=52, centre_seed=1051)
+
+
+@needs_ref
+def test_hnsw_seed_matches_reference_single_thread(data):
+    """addPoint KNN seeding (hrnn.h:1479-1501): same counts, same ids up to equal-distance order."""
+    x, q = data
+    ix = Index(x.shape[0], 128, 16, 100, 100, 40)
+    ix.add_points(x, nthreads=1)
+    r = oracle.RefIndex(x.shape[0], 128, M=16, efc=100, K=40)
+    r.add_points(x, nthreads=1)
+    ids, d, cnt = ix.get_knng()
+    rid, rd, rc = r.get_knng()
+    assert np.array_equal(cnt, rc)
+    m = np.arange(40) < cnt[:, None]
+    assert np.array_equal(np.where(m, d, 0), np.where(m, rd, 0))   # same distance profile
+    assert float((np.where(m, ids, 0) == np.where(m, rid, 0)).mean()) > 0.98
+    exact, _ = oracle.knn(x, q, 10)
+    po, pr = ix.proxies(q, 10, 64), r.proxies(q, 10, 64)
+    rec = lambda p: np.mean([len(set(p[i]) & set(exact[i])) / 10 for i in range(len(q))])
+    assert abs(rec(po) - rec(pr)) <= 0.02
+
+
+def test_multithreaded_build_labels(data):
+    x, _ = data
+    ix = Index(x.shape[0], 128, 16, 64, 100, 24)
+    ix.add_points(x, nthreads=8)
+    lab = ix.labels()
+    assert sorted(lab.tolist()) == list(range(x.shape[0]))
+    ids, d, cnt = ix.get_knng()
+    exact, _ = oracle.knn(x, x, 25)
+    # map internal ids to labels before comparing with label-indexed truth
+    inv = np.empty_like(lab)
+    inv[lab] = np.arange(len(lab))
+    lab_ids = lab[ids]
+    hits = [len(set(lab_ids[inv[i], :cnt[inv[i]]].tolist()) & set(exact[i, 1:].tolist()))
+            for i in range(0, x.shape[0], 5)]
+    assert np.mean(hits) / 24 > 0.3   # seeds see only earlier-inserted points
+
+
+@needs_ref
+def test_reference_file_loads_and_round_trips_byte_identical(tmp_path, data):
+    x, _ = data
+    r = oracle.RefIndex(x.shape[0], 128, M=12, efc=64, K=32)
+    r.add_points(x, nthreads=1)
+    r.refine(2, 8, nthreads=1)
+    r.build_rknng()
+    p_ref = str(tmp_path / "ref.idx")
+    r.save(p_ref)
+    ix = Index.load(p_ref, 128)
+    ids, d, cnt = ix.get_knng()
+    rid, rd, rc = r.get_knng()
+    m = np.arange(32) < cnt[:, None]
+    assert np.array_equal(cnt, rc)
+    assert np.array_equal(np.where(m, ids, 0), np.where(m, rid, 0))
+    off, ent, rb, wide = ix.get_rknng()
+    roff, rent, rrb, rwide = r.get_csr()
+    assert np.array_equal(off, roff) and np.array_equal(ent, rent) and rb == rrb
+    p_ours = str(tmp_path / "ours.idx")
+    ix.save(p_ours)
+    assert open(p_ref, "rb").read() == open(p_ours, "rb").read()
+
+
+@needs_ref
+def test_our_file_loads_in_reference(tmp_path, data):
+    x, q = data
+    ix = Index(x.shape[0], 128, 16, 64, 100, 24)
+    ix.add_points(x, nthreads=1)
+    p = str(tmp_path / "ours.idx")
+    ix.save(p)
+    r = oracle.RefIndex.load(p, 128)
+    r.K = 24
+    assert r.count == x.shape[0]
+    rid, rd, rc = r.get_knng()
+    ids, d, cnt = ix.get_knng()
+    m = np.arange(24) < cnt[:, None]
+    assert np.array_equal(rc, cnt) and np.array_equal(np.where(m, rid, 0), np.where(m, ids, 0))
+    links, levels = ix.level0()
+    rlinks, rcnt = r.level0(32)
+    assert np.array_equal(rcnt, links[:, 0])
+    assert np.array_equal(np.where(rlinks == 0xFFFFFFFF, 0, rlinks),
+                          np.where(links[:, 1:] == 0xFFFFFFFF, 0, links[:, 1:]))
+    # the reference searches our graph and finds the same proxies (up to distance ties)
+    rp, op = r.proxies(q, 5, 32), ix.proxies(q, 5, 32)
+    same = np.mean([len(set(rp[i]) & set(op[i])) / 5 for i in range(len(q))])
+    assert same >= 0.97
+
+
+def test_loader_rejects_bad_files(tmp_path, data):
+    x, _ = data
+    ix = Index(600, 128, 8, 32, 100, 16)
+    ix.add_points(x[:600], nthreads=1)
+    p = str(tmp_path / "a.idx")
+    ix.save(p)
+    blob = open(p, "rb").read()
+    bad = str(tmp_path / "bad.idx")
+    open(bad, "wb").write(b"XXXX" + blob[4:])
+    with pytest.raises(hrnn_amd.HrnnError, match="bad magic"):
+        Index.load(bad, 128)
+    open(bad, "wb").write(blob[: len(blob) // 2])
+    with pytest.raises(hrnn_amd.HrnnError, match="truncated"):
+        Index.load(bad, 128)
+    with pytest.raises(hrnn_amd.HrnnError, match="layout"):
+        Index.load(p, 64)   # wrong dimension
+    with pytest.raises(hrnn_amd.HrnnError):
+        Index.load(str(tmp_path / "missing.idx"), 128)
+
+
+def test_capacity_and_preconditions(data):
+    x, _ = data
+    ix = Index(100, 128, 8, 32, 100, 16)
+    ix.add_points(x[:100])
+    with pytest.raises(hrnn_amd.HrnnError, match="exceeds"):
+        ix.add_points(x[100:101])
+    with pytest.raises(hrnn_amd.HrnnError, match="KNNG not built"):
+        ix.compact_for_serving(8)
+    with pytest.raises(hrnn_amd.HrnnError, match="buildRKNNG_mutable"):
+        ix.insert_dynamic(x[:1], np.array([5000], np.uint64))
+    with pytest.raises(hrnn_amd.HrnnError, match="RKNNG not built"):
+        ix.search(x[:2], 5, 1.0, 5, 16, 32)
+    with pytest.raises(hrnn_amd.HrnnError, match="already present"):
+        ix2 = Index(10, 128, 8, 32, 100, 0)
+        ix2.add_points(x[:2], labels=np.array([7, 7], np.uint64))
