@@ -11,8 +11,8 @@ using namespace hrnn;
 hrnn_dev::~hrnn_dev() {
     (void)hipSetDevice(device);
     DevBuf *bufs[] = {&x, &xb, &xh, &norms, &knn_ids, &knn_d, &knn_cnt, &knn_new, &csr_off,
-                      &csr_ent, &search_mark, &search_cand, &s0, &s1, &s2, &s3, &s4, &s5, &s6,
-                      &s7, &s8, &counters};
+                      &csr_ent, &search_mark, &search_cand, &l0, &up_off, &up_pool, &beam_vis,
+                      &s0, &s1, &s2, &s3, &s4, &s5, &s6, &s7, &s8, &counters};
     for (DevBuf *b : bufs) b->release();
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -168,6 +168,9 @@ static void set_base_common(hrnn_dev_t h, const float *x, uint64_t n, uint32_t d
     launch_base_mirrors(h, 0, n);
     h->K = 0;
     h->csr_valid = false;
+    h->hnsw_valid = false;
+    h->beam_vis.release();     // sized by n
+    h->search_mark.release();
 }
 
 int hrnn_dev_set_base(hrnn_dev_t h, const float *x, uint64_t n, uint32_t d) {
@@ -209,6 +212,9 @@ int hrnn_dev_append_base(hrnn_dev_t h, const float *x, uint64_t n_add) {
         launch_base_mirrors(h, h->n, n_add);
         h->n = n_new;
         h->csr_valid = false;
+        h->hnsw_valid = false;
+        h->beam_vis.release();
+        h->search_mark.release();
     });
 }
 
@@ -332,7 +338,7 @@ int hrnn_dev_rknn_search_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq,
                               const uint32_t *proxies_dev, uint32_t m,
                               const uint32_t *k_per_q_dev, uint32_t k, uint32_t K_prime,
                               float threshold, uint32_t *res_cnt_dev, uint32_t *res_ids_dev,
-                              uint32_t cap_per_q) {
+                              uint32_t cap_per_q, uint32_t *work_dev) {
     if (!h) return HRNN_E_ARG;
     return guarded(h, [&] {
         REQUIRE(h->csr_valid, HRNN_E_STATE, "rknn_search: build the CSR first");
@@ -341,7 +347,7 @@ int hrnn_dev_rknn_search_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq,
         REQUIRE(m >= 1 && m <= 1024, HRNN_E_ARG, "rknn_search: m out of [1, 1024]");
         if (nq == 0) return;
         rknn_search_resident(h, q_dev, nq, proxies_dev, m, k_per_q_dev, k, K_prime, threshold,
-                             res_cnt_dev, res_ids_dev, cap_per_q);
+                             res_cnt_dev, res_ids_dev, cap_per_q, work_dev);
     });
 }
 

@@ -32,6 +32,15 @@ struct hrnn_dev {
     // (all-zero between launches) and candidate buffers
     hrnn::DevBuf search_mark, search_cand;
 
+    // HNSW graph for phase A on the GPU: layer-0 links [n x (maxM0+1)] (count
+    // word + ids), per-node offset into the upper-layer pool (blocks of maxM+1
+    // words per level, 0xFFFFFFFF for level-0-only nodes), per-block visited
+    // bitmaps (all-zero between queries)
+    bool hnsw_valid = false;
+    uint32_t l0_words = 0, up_words = 0, entry = 0;
+    int maxlevel = 0;
+    hrnn::DevBuf l0, up_off, up_pool, beam_vis;
+
     // scratch shared by the build kernels
     hrnn::DevBuf s0, s1, s2, s3, s4, s5, s6, s7, s8;
     // small device counters, by byte offset: 0 base-int flag, 32 long-list
@@ -69,8 +78,13 @@ uint64_t max_list_len(hrnn_dev_t h);
 void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint32_t *proxies,
                           uint32_t m, const uint32_t *k_per_q, uint32_t k, uint32_t K_prime,
                           float threshold, uint32_t *res_cnt, uint32_t *res_ids,
-                          uint32_t cap_per_q);
+                          uint32_t cap_per_q, uint32_t *work = nullptr);
 void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t bucket, hrnn_nnd_iter_stats *st);
+// Per-query beam statistics (raw words): 0 d(q, entry), 1 d(q, layer-0
+// start), 2..17 the 16 closest beam distances, 18 expansions, 19 list size.
+constexpr int kRawWords = 20;
+void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint32_t ef,
+                 uint32_t *proxies_dev, uint32_t *expansions_dev, float *raw_dev = nullptr);
 void bf_topk_base(hrnn_dev_t h, const float *q_dev, uint32_t nq, bool self_base, uint32_t kk,
                   int prec, uint32_t *ids_dev, float *d_dev);
 void bf_rknn_count_fill(hrnn_dev_t h, const float *q_dev, uint32_t nq, const float *kth_dev,

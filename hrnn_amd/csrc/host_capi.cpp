@@ -174,6 +174,15 @@ int hrnn_index_proxies(hrnn_index_t ix, const float *q, uint64_t nq, uint32_t m,
     });
 }
 
+int hrnn_index_proxies_gpu(hrnn_index_t ix, const float *q, uint64_t nq, uint32_t m, uint32_t ef,
+                           uint32_t *out) {
+    if (!ix || !q || !out) return HRNN_E_ARG;
+    return guard(ix, [&] {
+        auto p = ix->idx->proxiesDevice(q, nq, m, ef);
+        std::memcpy(out, p.data(), p.size() * 4);
+    });
+}
+
 int hrnn_index_search(hrnn_index_t ix, const float *q, uint64_t nq, uint32_t m, float thr,
                       uint32_t k, uint32_t Kp, uint32_t ef, int nthreads, int per_query,
                       uint64_t *offsets, uint64_t *labels, uint64_t cap, uint64_t *needed) {

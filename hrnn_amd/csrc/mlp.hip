@@ -1,0 +1,388 @@
+// mlp.hip -- the learned proxy count: a 32-64-64-16 MLP (ReLU, softmax over a
+// 16-value grid of m) with hand-written forward, backward and Adam kernels.
+//
+// No reference counterpart: the reference picks m from a fixed grid
+// (src/hrnn_search.cpp:69); this network predicts, per query, the smallest m
+// of the grid that reaches full recall.
+//
+// Parameter vector (FP32, HRNN_MLP_PARAMS floats):
+//   W1[64][32] b1[64] W2[64][64] b2[64] W3[16][64] b3[16]
+// Forward per sample: z1 = W1 x + b1, h1 = relu(z1), z2 = W2 h1 + b2,
+// h2 = relu(z2), logits = W3 h2 + b3, p = softmax(logits).
+// Loss: mean cross-entropy over the batch.  The batch gradient is the mean of
+// per-sample gradients, reduced in a fixed order (deterministic).
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <random>
+#include <vector>
+
+#include "handle.h"
+
+namespace hrnn {
+
+constexpr int kIn = 32, kH = 64, kOut = 16;
+constexpr int oW1 = 0, oB1 = oW1 + kH * kIn, oW2 = oB1 + kH, oB2 = oW2 + kH * kH,
+              oW3 = oB2 + kH, oB3 = oW3 + kOut * kH, kParams = oB3 + kOut;
+static_assert(kParams == 7312, "parameter layout");
+
+// One 64-lane block per sample.  Writes probabilities (and, when grad != null,
+// the per-sample gradient row grad[s][kParams] and loss[s]).
+__global__ __launch_bounds__(64) void k_mlp_fwd_bwd(const float *__restrict__ P,
+                                                    const float *__restrict__ X,
+                                                    const uint32_t *__restrict__ idx,
+                                                    const uint32_t *__restrict__ labels, uint32_t n,
+                                                    float *__restrict__ probs,
+                                                    float *__restrict__ grad,
+                                                    float *__restrict__ loss) {
+    __shared__ float x[kIn], z1[kH], h1[kH], z2[kH], h2[kH], lg[kOut], dl[kOut], dz2[kH], dz1[kH];
+    const uint32_t s = blockIdx.x;
+    if (s >= n) return;
+    const uint32_t j = threadIdx.x;
+    const uint32_t row = idx ? idx[s] : s;
+    if (j < kIn) x[j] = X[(uint64_t)row * kIn + j];
+    __syncthreads();
+    {
+        float a = P[oB1 + j];
+        for (int i = 0; i < kIn; i++) a = __builtin_fmaf(P[oW1 + j * kIn + i], x[i], a);
+        z1[j] = a;
+        h1[j] = a > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    {
+        float a = P[oB2 + j];
+        for (int i = 0; i < kH; i++) a = __builtin_fmaf(P[oW2 + j * kH + i], h1[i], a);
+        z2[j] = a;
+        h2[j] = a > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    if (j < kOut) {
+        float a = P[oB3 + j];
+        for (int i = 0; i < kH; i++) a = __builtin_fmaf(P[oW3 + j * kH + i], h2[i], a);
+        lg[j] = a;
+    }
+    __syncthreads();
+    if (j == 0) {
+        float mx = lg[0];
+        for (int c = 1; c < kOut; c++) mx = fmaxf(mx, lg[c]);
+        float sum = 0.f;
+        for (int c = 0; c < kOut; c++) {
+            dl[c] = expf(lg[c] - mx);
+            sum += dl[c];
+        }
+        for (int c = 0; c < kOut; c++) dl[c] /= sum;   // dl holds p for now
+        if (loss && labels) loss[s] = -logf(fmaxf(dl[labels[row]], 1e-30f));
+    }
+    __syncthreads();
+    if (probs && j < kOut) probs[(uint64_t)s * kOut + j] = dl[j];
+    if (!grad) return;
+    float *g = grad + (uint64_t)s * kParams;
+    __syncthreads();
+    if (j < kOut) dl[j] -= (labels[row] == j) ? 1.f : 0.f;   // dlogits = p - onehot
+    __syncthreads();
+    // layer 3
+    for (int t = j; t < kOut * kH; t += 64) g[oW3 + t] = dl[t / kH] * h2[t % kH];
+    if (j < kOut) g[oB3 + j] = dl[j];
+    {
+        float a = 0.f;
+        for (int c = 0; c < kOut; c++) a = __builtin_fmaf(P[oW3 + c * kH + j], dl[c], a);
+        dz2[j] = z2[j] > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    for (int t = j; t < kH * kH; t += 64) g[oW2 + t] = dz2[t / kH] * h1[t % kH];
+    g[oB2 + j] = dz2[j];
+    {
+        float a = 0.f;
+        for (int c = 0; c < kH; c++) a = __builtin_fmaf(P[oW2 + c * kH + j], dz2[c], a);
+        dz1[j] = z1[j] > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    for (int t = j; t < kH * kIn; t += 64) g[oW1 + t] = dz1[t / kIn] * x[t % kIn];
+    g[oB1 + j] = dz1[j];
+}
+
+// mean over the batch rows, fixed order: out[p] = (1/B) sum_s grad[s][p]
+__global__ void k_mlp_reduce(const float *__restrict__ grad, uint32_t B, float *__restrict__ out) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (uint32_t)kParams) return;
+    float a = 0.f;
+    for (uint32_t s = 0; s < B; s++) a += grad[(uint64_t)s * kParams + p];
+    out[p] = a / (float)B;
+}
+
+__global__ void k_mlp_adam(float *__restrict__ P, const float *__restrict__ g, float *__restrict__ m1,
+                           float *__restrict__ m2, float lr, float b1, float b2, float eps,
+                           float c1, float c2, float wd) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (uint32_t)kParams) return;
+    float gv = g[p] + wd * P[p];
+    float a = b1 * m1[p] + (1.f - b1) * gv;
+    float b = b2 * m2[p] + (1.f - b2) * gv * gv;
+    m1[p] = a;
+    m2[p] = b;
+    P[p] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+// The 32 query features from the beam statistics and the r_k of the closest
+// proxies (k-th list distance of each proxy, FLT_MAX when its list is short):
+//   0 log1p d(q,entry)          1 log1p d(q,start)
+//   2..17 log1p of the 16 closest beam distances
+//   18..25 log1p r_k of proxies 1..8
+//   26..28 mean / min / max of log1p r_k over proxies 1..16
+//   29 fraction of proxies 1..16 with d(q,p) <= r_k(p)
+//   30 log1p d1 - log1p r_k(p1)     31 log1p(expansions)
+// Optional standardisation: (f - mu) / sd.
+__global__ void k_features(const float *__restrict__ raw, const uint32_t *__restrict__ prox,
+                           uint32_t mstride, const float *__restrict__ knn_d,
+                           const uint32_t *__restrict__ knn_cnt, uint32_t K,
+                           const uint32_t *__restrict__ k_per_q, uint32_t k, uint32_t nq,
+                           const float *__restrict__ mu, const float *__restrict__ sd,
+                           float *__restrict__ feats) {
+    uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= nq) return;
+    const float *r = raw + (uint64_t)qi * kRawWords;
+    float f[kIn];
+    f[0] = log1pf(r[0]);
+    f[1] = log1pf(r[1]);
+    for (int j = 0; j < 16; j++) f[2 + j] = log1pf(fminf(r[2 + j], 3.0e38f));
+    uint32_t kq = k_per_q ? k_per_q[qi] : k;
+    float sum = 0.f, mn = FLT_MAX, mx = -FLT_MAX, inside = 0.f, rk1 = 0.f;
+    int cnt = 0;
+    for (int j = 0; j < 16; j++) {
+        uint32_t p = (uint32_t)j < mstride ? prox[(uint64_t)qi * mstride + j] : kInvalidId;
+        float rk = 0.f;
+        if (p != kInvalidId) {
+            float v = (kq - 1 < knn_cnt[p]) ? knn_d[(uint64_t)p * K + kq - 1] : 3.0e38f;
+            rk = log1pf(v);
+            inside += (r[2 + j] <= v) ? 1.f : 0.f;
+            cnt++;
+            sum += rk;
+            mn = fminf(mn, rk);
+            mx = fmaxf(mx, rk);
+        }
+        if (j == 0) rk1 = rk;
+        if (j < 8) f[18 + j] = rk;
+    }
+    f[26] = cnt ? sum / cnt : 0.f;
+    f[27] = cnt ? mn : 0.f;
+    f[28] = cnt ? mx : 0.f;
+    f[29] = cnt ? inside / cnt : 0.f;
+    f[30] = f[2] - rk1;
+    f[31] = log1pf(r[18]);
+    for (int j = 0; j < kIn; j++) {
+        float v = f[j];
+        if (mu && sd) v = (v - mu[j]) / sd[j];
+        feats[(uint64_t)qi * kIn + j] = v;
+    }
+}
+
+// Per-query m: the smallest grid class whose cumulative probability reaches
+// tau (argmax when tau <= 0); proxies beyond that m are masked out.
+__global__ void k_pick_m(const float *__restrict__ probs, uint32_t nq, const uint32_t *__restrict__ grid,
+                         float tau, uint32_t *__restrict__ prox, uint32_t mstride,
+                         uint32_t *__restrict__ m_out) {
+    uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= nq) return;
+    const float *p = probs + (uint64_t)qi * kOut;
+    int c = kOut - 1;
+    if (tau <= 0.f) {
+        c = 0;
+        for (int j = 1; j < kOut; j++)
+            if (p[j] > p[c]) c = j;
+    } else {
+        float acc = 0.f;
+        for (int j = 0; j < kOut; j++) {
+            acc += p[j];
+            if (acc >= tau) {
+                c = j;
+                break;
+            }
+        }
+    }
+    uint32_t m = grid[c] < mstride ? grid[c] : mstride;
+    for (uint32_t j = m; j < mstride; j++) prox[(uint64_t)qi * mstride + j] = kInvalidId;
+    if (m_out) m_out[qi] = m;
+}
+
+}  // namespace hrnn
+
+using namespace hrnn;
+
+extern "C" {
+
+int hrnn_dev_query_features(hrnn_dev_t h, const float *q, uint32_t nq, uint32_t k, uint32_t ef,
+                            uint32_t m_max, float *feats, uint32_t *proxies) {
+    if (!h) return HRNN_E_ARG;
+    return guarded(h, [&] {
+        REQUIRE(q && feats, HRNN_E_ARG, "query_features: null argument");
+        REQUIRE(h->K > 0 && k >= 1 && k <= h->K, HRNN_E_ARG, "query_features: k out of [1, K]");
+        if (!nq) return;
+        uint32_t mm = std::max<uint32_t>(16, m_max);
+        DevBuf dq, dp, dr, df;
+        dq.reserve((uint64_t)nq * h->d * 4);
+        dp.reserve((uint64_t)nq * mm * 4);
+        dr.reserve((uint64_t)nq * kRawWords * 4);
+        df.reserve((uint64_t)nq * kIn * 4);
+        h2d(h, dq.p, q, (uint64_t)nq * h->d * 4);
+        beam_launch(h, dq.as<float>(), nq, mm, std::max(ef, mm), dp.as<uint32_t>(), nullptr, dr.as<float>());
+        hipLaunchKernelGGL(k_features, dim3(ceil_div(nq, 128)), dim3(128), 0, h->stream, dr.as<float>(),
+                           dp.as<uint32_t>(), mm, h->knn_d.as<float>(), h->knn_cnt.as<uint32_t>(), h->K,
+                           nullptr, k, nq, nullptr, nullptr, df.as<float>());
+        HIPC(hipGetLastError());
+        d2h(h, feats, df.p, (uint64_t)nq * kIn * 4);
+        if (proxies) {
+            std::vector<uint32_t> tmp((uint64_t)nq * mm);
+            d2h(h, tmp.data(), dp.p, tmp.size() * 4);
+            for (uint32_t i = 0; i < nq; i++)
+                std::copy(tmp.begin() + (uint64_t)i * mm, tmp.begin() + (uint64_t)i * mm + m_max,
+                          proxies + (uint64_t)i * m_max);
+        }
+        dq.release(); dp.release(); dr.release(); df.release();
+    });
+}
+
+int hrnn_dev_rknn_query_learned_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq,
+                                     const float *model_dev, const uint32_t *grid_dev, float tau,
+                                     uint32_t m_max, uint32_t ef, uint32_t k, uint32_t K_prime,
+                                     float threshold, uint32_t *proxies_dev, uint32_t *m_dev,
+                                     uint32_t *res_cnt_dev, uint32_t *res_ids_dev,
+                                     uint32_t cap_per_q, uint32_t *work_dev) {
+    if (!h) return HRNN_E_ARG;
+    return guarded(h, [&] {
+        REQUIRE(h->csr_valid && h->K > 0, HRNN_E_STATE, "query_learned: build the CSR first");
+        REQUIRE(k >= 1 && k <= h->K, HRNN_E_ARG, "query_learned: k out of [1, K]");
+        REQUIRE(m_max >= 16 && m_max <= 1024, HRNN_E_ARG, "query_learned: m_max must be in [16, 1024]");
+        if (!nq) return;
+        h->s8.reserve((uint64_t)nq * (kRawWords + kIn + kOut) * 4);
+        float *raw = h->s8.as<float>();
+        float *feats = raw + (uint64_t)nq * kRawWords;
+        float *probs = feats + (uint64_t)nq * kIn;
+        beam_launch(h, q_dev, nq, m_max, std::max(ef, m_max), proxies_dev, nullptr, raw);
+        hipLaunchKernelGGL(k_features, dim3(ceil_div(nq, 128)), dim3(128), 0, h->stream, raw, proxies_dev,
+                           m_max, h->knn_d.as<float>(), h->knn_cnt.as<uint32_t>(), h->K, nullptr, k, nq,
+                           model_dev + kParams, model_dev + kParams + kIn, feats);
+        hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(nq), dim3(64), 0, h->stream, model_dev, feats, nullptr,
+                           nullptr, nq, probs, nullptr, nullptr);
+        hipLaunchKernelGGL(k_pick_m, dim3(ceil_div(nq, 128)), dim3(128), 0, h->stream, probs, nq, grid_dev,
+                           tau, proxies_dev, m_max, m_dev);
+        HIPC(hipGetLastError());
+        rknn_search_resident(h, q_dev, nq, proxies_dev, m_max, nullptr, k, K_prime, threshold, res_cnt_dev,
+                             res_ids_dev, cap_per_q, work_dev);
+    });
+}
+
+int hrnn_dev_mlp_forward(hrnn_dev_t h, const float *params, const float *feats, uint32_t n,
+                         float *probs) {
+    if (!h) return HRNN_E_ARG;
+    return guarded(h, [&] {
+        REQUIRE(params && feats && probs, HRNN_E_ARG, "mlp_forward: null argument");
+        if (!n) return;
+        DevBuf dp, dx, dpr;
+        dp.reserve(kParams * 4);
+        dx.reserve((uint64_t)n * kIn * 4);
+        dpr.reserve((uint64_t)n * kOut * 4);
+        h2d(h, dp.p, params, kParams * 4);
+        h2d(h, dx.p, feats, (uint64_t)n * kIn * 4);
+        hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(n), dim3(64), 0, h->stream, dp.as<float>(), dx.as<float>(),
+                           nullptr, nullptr, n, dpr.as<float>(), nullptr, nullptr);
+        HIPC(hipGetLastError());
+        d2h(h, probs, dpr.p, (uint64_t)n * kOut * 4);
+        dp.release(); dx.release(); dpr.release();
+    });
+}
+
+int hrnn_dev_mlp_grad(hrnn_dev_t h, const float *params, const float *feats, const uint32_t *labels,
+                      uint32_t n, float *grad, float *loss) {
+    if (!h) return HRNN_E_ARG;
+    return guarded(h, [&] {
+        REQUIRE(params && feats && labels && grad, HRNN_E_ARG, "mlp_grad: null argument");
+        REQUIRE(n > 0, HRNN_E_ARG, "mlp_grad: empty batch");
+        for (uint32_t i = 0; i < n; i++) REQUIRE(labels[i] < (uint32_t)kOut, HRNN_E_ARG, "mlp_grad: label >= 16");
+        DevBuf dp, dx, dl, dg, dr, dloss;
+        dp.reserve(kParams * 4);
+        dx.reserve((uint64_t)n * kIn * 4);
+        dl.reserve((uint64_t)n * 4);
+        dg.reserve((uint64_t)n * kParams * 4);
+        dr.reserve(kParams * 4);
+        dloss.reserve((uint64_t)n * 4);
+        h2d(h, dp.p, params, kParams * 4);
+        h2d(h, dx.p, feats, (uint64_t)n * kIn * 4);
+        h2d(h, dl.p, labels, (uint64_t)n * 4);
+        hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(n), dim3(64), 0, h->stream, dp.as<float>(), dx.as<float>(),
+                           nullptr, dl.as<uint32_t>(), n, nullptr, dg.as<float>(), dloss.as<float>());
+        hipLaunchKernelGGL(k_mlp_reduce, dim3(ceil_div(kParams, 256)), dim3(256), 0, h->stream,
+                           dg.as<float>(), n, dr.as<float>());
+        HIPC(hipGetLastError());
+        d2h(h, grad, dr.p, kParams * 4);
+        if (loss) {
+            std::vector<float> ls(n);
+            d2h(h, ls.data(), dloss.p, (uint64_t)n * 4);
+            double a = 0;
+            for (float v : ls) a += v;
+            *loss = (float)(a / n);
+        }
+        dp.release(); dx.release(); dl.release(); dg.release(); dr.release(); dloss.release();
+    });
+}
+
+int hrnn_dev_mlp_train(hrnn_dev_t h, float *params, const float *feats, const uint32_t *labels,
+                       uint32_t n, int epochs, uint32_t batch, float lr, float weight_decay,
+                       uint64_t seed, float *epoch_loss) {
+    if (!h) return HRNN_E_ARG;
+    return guarded(h, [&] {
+        REQUIRE(params && feats && labels, HRNN_E_ARG, "mlp_train: null argument");
+        REQUIRE(n > 0 && batch > 0 && epochs >= 0, HRNN_E_ARG, "mlp_train: bad sizes");
+        for (uint32_t i = 0; i < n; i++) REQUIRE(labels[i] < (uint32_t)kOut, HRNN_E_ARG, "mlp_train: label >= 16");
+        batch = std::min(batch, n);
+        DevBuf dp, dx, dl, dg, dr, dm1, dm2, didx, dloss;
+        dp.reserve(kParams * 4);
+        dx.reserve((uint64_t)n * kIn * 4);
+        dl.reserve((uint64_t)n * 4);
+        dg.reserve((uint64_t)batch * kParams * 4);
+        dr.reserve(kParams * 4);
+        dm1.reserve(kParams * 4);
+        dm2.reserve(kParams * 4);
+        didx.reserve((uint64_t)n * 4);
+        dloss.reserve((uint64_t)n * 4);
+        h2d(h, dp.p, params, kParams * 4);
+        h2d(h, dx.p, feats, (uint64_t)n * kIn * 4);
+        h2d(h, dl.p, labels, (uint64_t)n * 4);
+        HIPC(hipMemsetAsync(dm1.p, 0, kParams * 4, h->stream));
+        HIPC(hipMemsetAsync(dm2.p, 0, kParams * 4, h->stream));
+        std::vector<uint32_t> order(n);
+        for (uint32_t i = 0; i < n; i++) order[i] = i;
+        std::mt19937_64 rng(seed);
+        const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+        uint64_t t = 0;
+        std::vector<float> ls(n);
+        for (int ep = 0; ep < epochs; ep++) {
+            std::shuffle(order.begin(), order.end(), rng);
+            h2d(h, didx.p, order.data(), (uint64_t)n * 4);
+            for (uint32_t b0 = 0; b0 + batch <= n; b0 += batch) {
+                t++;
+                float c1 = 1.f - std::pow(b1, (float)t), c2 = 1.f - std::pow(b2, (float)t);
+                hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(batch), dim3(64), 0, h->stream, dp.as<float>(),
+                                   dx.as<float>(), didx.as<uint32_t>() + b0, dl.as<uint32_t>(), batch,
+                                   nullptr, dg.as<float>(), dloss.as<float>() + b0);
+                hipLaunchKernelGGL(k_mlp_reduce, dim3(ceil_div(kParams, 256)), dim3(256), 0, h->stream,
+                                   dg.as<float>(), batch, dr.as<float>());
+                hipLaunchKernelGGL(k_mlp_adam, dim3(ceil_div(kParams, 256)), dim3(256), 0, h->stream,
+                                   dp.as<float>(), dr.as<float>(), dm1.as<float>(), dm2.as<float>(), lr,
+                                   b1, b2, eps, c1, c2, weight_decay);
+            }
+            HIPC(hipGetLastError());
+            if (epoch_loss) {
+                uint32_t used = (n / batch) * batch;
+                d2h(h, ls.data(), dloss.p, (uint64_t)used * 4);
+                double a = 0;
+                for (uint32_t i = 0; i < used; i++) a += ls[i];
+                epoch_loss[ep] = used ? (float)(a / used) : 0.f;
+            }
+        }
+        d2h(h, params, dp.p, kParams * 4);
+        DevBuf *all[] = {&dp, &dx, &dl, &dg, &dr, &dm1, &dm2, &didx, &dloss};
+        for (DevBuf *b : all) b->release();
+    });
+}
+
+}  // extern "C"

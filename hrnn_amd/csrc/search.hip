@@ -36,7 +36,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
     uint32_t k, uint32_t K_prime, float threshold, uint32_t *__restrict__ mark_all,
     uint64_t mark_words, uint32_t *__restrict__ cand_all, uint64_t cand_cap,
     uint32_t *__restrict__ queue, uint32_t *__restrict__ res_cnt, uint32_t *__restrict__ res_ids,
-    uint32_t cap_per_q) {
+    uint32_t cap_per_q, uint32_t *__restrict__ work) {
     __shared__ uint32_t s_q;
     __shared__ uint32_t s_ncand, s_nkeep, s_nres;
     __shared__ uint64_t s_start[kMaxM];
@@ -166,7 +166,13 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
                 __syncthreads();
             }
         }
-        if (threadIdx.x == 0) res_cnt[qi] = nres;
+        if (threadIdx.x == 0) {
+            res_cnt[qi] = nres;
+            if (work) {
+                work[2 * (uint64_t)qi] = total;
+                work[2 * (uint64_t)qi + 1] = nkeep;
+            }
+        }
         __syncthreads();
     }
 }
@@ -174,7 +180,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
 void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint32_t *proxies,
                           uint32_t m, const uint32_t *k_per_q, uint32_t k, uint32_t K_prime,
                           float threshold, uint32_t *res_cnt, uint32_t *res_ids,
-                          uint32_t cap_per_q) {
+                          uint32_t cap_per_q, uint32_t *work) {
     REQUIRE(h->csr_valid, HRNN_E_STATE, "search: no CSR");
     REQUIRE(m <= (uint32_t)kMaxM, HRNN_E_ARG, "search: m > 1024");
     REQUIRE(cap_per_q > 0, HRNN_E_ARG, "search: cap_per_q == 0");
@@ -210,7 +216,7 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     hipLaunchKernelGGL((k_rknn_search<ET, CM>), dim3(slots), dim3(kSearchBlock), dyn, h->stream, x, \
                        h->d, off, h->csr_ent.as<ET>(), h->rank_bits, kd, kc, h->K, q, nq, proxies, m, \
                        k_per_q, k, K_prime, threshold, mark, words, cand, cand_cap, queue, res_cnt,  \
-                       res_ids, cap_per_q)
+                       res_ids, cap_per_q, work)
     if (h->wide) {
         if (count_mode) LAUNCH(uint64_t, true); else LAUNCH(uint64_t, false);
     } else {

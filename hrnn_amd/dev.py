@@ -17,6 +17,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libhrnn_hip.so")
 
 PREC_F32, PREC_BF16, PREC_F16 = 0, 1, 2
+MLP_PARAMS = 7312              # HRNN_MLP_PARAMS
+MLP_MODEL = MLP_PARAMS + 64    # + feature mean / std
 _PREC = {"f32": PREC_F32, "fp32": PREC_F32, "bf16": PREC_BF16, "f16": PREC_F16, "fp16": PREC_F16}
 
 STATUS = {0: "OK", -1: "E_ARG", -2: "E_HIP", -3: "E_STATE", -4: "E_OOM", -5: "E_CAP",
@@ -29,8 +31,10 @@ EXPORTS = [
     "hrnn_dev_append_base", "hrnn_dev_base_is_small_int", "hrnn_dev_base_ptr",
     "hrnn_dev_set_knng", "hrnn_dev_get_knng", "hrnn_dev_knng_ptrs", "hrnn_dev_nndescent",
     "hrnn_dev_build_rknn_csr", "hrnn_dev_get_rknn_csr", "hrnn_dev_set_rknn_csr",
-    "hrnn_dev_rknn_search", "hrnn_dev_rknn_search_dptr", "hrnn_dev_bf_kth", "hrnn_dev_bf_knn",
-    "hrnn_dev_bf_rknn", "hrnn_dev_l2sqr_pairs", "hrnn_synth_mixture",
+    "hrnn_dev_rknn_search", "hrnn_dev_rknn_search_dptr", "hrnn_dev_set_hnsw", "hrnn_dev_proxies",
+    "hrnn_dev_proxies_dptr", "hrnn_dev_rknn_query_dptr", "hrnn_dev_bf_kth", "hrnn_dev_bf_knn",
+    "hrnn_dev_bf_rknn", "hrnn_dev_l2sqr_pairs", "hrnn_dev_query_features", "hrnn_dev_mlp_forward",
+    "hrnn_dev_mlp_grad", "hrnn_dev_mlp_train", "hrnn_dev_rknn_query_learned_dptr", "hrnn_synth_mixture",
     "hrnn_synth_halfnormal_scaled", "hrnn_synth_sphere_mixture",
 ]
 
@@ -78,11 +82,21 @@ def lib() -> C.CDLL:
         "hrnn_dev_get_rknn_csr": [vp, vp, vp],
         "hrnn_dev_set_rknn_csr": [vp, u32, i, u64, vp, vp, u64],
         "hrnn_dev_rknn_search": [vp, vp, u32, vp, u32, vp, u32, u32, f, vp, vp, u64, P(u64)],
-        "hrnn_dev_rknn_search_dptr": [vp, vp, u32, vp, u32, vp, u32, u32, f, vp, vp, u32],
+        "hrnn_dev_rknn_search_dptr": [vp, vp, u32, vp, u32, vp, u32, u32, f, vp, vp, u32, vp],
+        "hrnn_dev_set_hnsw": [vp, u32, u32, vp, vp, vp, vp, u64, u32, i],
+        "hrnn_dev_proxies": [vp, vp, u32, u32, u32, vp],
+        "hrnn_dev_proxies_dptr": [vp, vp, u32, u32, u32, vp, vp],
+        "hrnn_dev_rknn_query_dptr": [vp, vp, u32, u32, u32, vp, u32, u32, f, vp, vp, vp, u32, vp],
         "hrnn_dev_bf_kth": [vp, u32, i, vp, vp],
         "hrnn_dev_bf_knn": [vp, vp, u32, u32, i, vp, vp],
         "hrnn_dev_bf_rknn": [vp, vp, u32, vp, i, vp, vp, vp, u64, P(u64)],
         "hrnn_dev_l2sqr_pairs": [vp, vp, u64, vp, vp, u64, vp],
+        "hrnn_dev_query_features": [vp, vp, u32, u32, u32, u32, vp, vp],
+        "hrnn_dev_mlp_forward": [vp, vp, vp, u32, vp],
+        "hrnn_dev_mlp_grad": [vp, vp, vp, vp, u32, vp, vp],
+        "hrnn_dev_mlp_train": [vp, vp, vp, vp, u32, i, u32, f, f, u64, vp],
+        "hrnn_dev_rknn_query_learned_dptr": [vp, vp, u32, vp, vp, f, u32, u32, u32, u32, f, vp, vp,
+                                             vp, vp, u32, vp],
         "hrnn_synth_mixture": [u64, u64, u32, u32, f, f, f, i, C.c_double, u64, vp, vp],
         "hrnn_synth_halfnormal_scaled": [u64, u64, u64, u32, vp],
         "hrnn_synth_sphere_mixture": [u64, u64, u64, u32, u32, f, vp, vp],
@@ -261,6 +275,22 @@ class Device:
             self._ck(rc)
             return off, ids[: int(off[-1])].copy()
 
+    # -- phase A (HNSW beam) on the GPU -------------------------------------------
+    def set_hnsw(self, maxM0, maxM, l0, levels, up_off, up_pool, entry, maxlevel):
+        l0 = np.ascontiguousarray(l0, np.uint32)
+        levels = np.ascontiguousarray(levels, np.int32)
+        up_off = np.ascontiguousarray(up_off, np.uint32)
+        up_pool = np.ascontiguousarray(up_pool, np.uint32)
+        self._ck(self.L.hrnn_dev_set_hnsw(self.h, maxM0, maxM, _p(l0), _p(levels), _p(up_off),
+                                          _p(up_pool) if up_pool.size else None, up_pool.size,
+                                          entry, maxlevel))
+
+    def proxies(self, q, m, ef):
+        q = np.ascontiguousarray(q, np.float32)
+        out = np.empty((q.shape[0], m), np.uint32)
+        self._ck(self.L.hrnn_dev_proxies(self.h, _p(q), q.shape[0], m, max(ef, m), _p(out)))
+        return out
+
     # -- brute force -----------------------------------------------------------
     def bf_kth(self, k: int, prec="bf16", with_ids=False):
         kk = min(k + 1, self.n)
@@ -295,6 +325,41 @@ class Device:
             self._ck(rc)
             n = int(off[-1])
             return off, ids[:n].copy(), d2[:n].copy()
+
+    # -- learned m ---------------------------------------------------------------
+    def query_features(self, q, k, ef, m_max=16):
+        q = np.ascontiguousarray(q, np.float32)
+        f = np.empty((q.shape[0], 32), np.float32)
+        p = np.empty((q.shape[0], m_max), np.uint32)
+        self._ck(self.L.hrnn_dev_query_features(self.h, _p(q), q.shape[0], k, ef, m_max, _p(f), _p(p)))
+        return f, p
+
+    def mlp_forward(self, params, feats):
+        params = np.ascontiguousarray(params, np.float32)
+        feats = np.ascontiguousarray(feats, np.float32)
+        out = np.empty((feats.shape[0], 16), np.float32)
+        self._ck(self.L.hrnn_dev_mlp_forward(self.h, _p(params), _p(feats), feats.shape[0], _p(out)))
+        return out
+
+    def mlp_grad(self, params, feats, labels):
+        params = np.ascontiguousarray(params, np.float32)
+        feats = np.ascontiguousarray(feats, np.float32)
+        labels = np.ascontiguousarray(labels, np.uint32)
+        g = np.empty(MLP_PARAMS, np.float32)
+        loss = C.c_float(0)
+        self._ck(self.L.hrnn_dev_mlp_grad(self.h, _p(params), _p(feats), _p(labels), feats.shape[0],
+                                          _p(g), C.byref(loss)))
+        return g, float(loss.value)
+
+    def mlp_train(self, params, feats, labels, epochs, batch=256, lr=1e-3, weight_decay=0.0,
+                  seed=0):
+        params = np.ascontiguousarray(params, np.float32).copy()
+        feats = np.ascontiguousarray(feats, np.float32)
+        labels = np.ascontiguousarray(labels, np.uint32)
+        el = np.zeros(max(epochs, 1), np.float32)
+        self._ck(self.L.hrnn_dev_mlp_train(self.h, _p(params), _p(feats), _p(labels), feats.shape[0],
+                                           epochs, batch, lr, weight_decay, seed, _p(el)))
+        return params, el[:epochs]
 
     def l2sqr_pairs(self, a, ia, ib):
         a = np.ascontiguousarray(a, np.float32)

@@ -1,0 +1,99 @@
+"""Minimal HIP runtime access (device buffers, copies, events) via ctypes.
+
+Used by bench.py to keep queries, proxies and results resident in HBM and to
+time kernels with HIP events recorded on the library's own stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_hip = None
+H2D, D2H, D2D = 1, 2, 3
+
+
+def hip() -> C.CDLL:
+    global _hip
+    if _hip is None:
+        rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+        _hip = C.CDLL(os.path.join(rocm, "lib", "libamdhip64.so"))
+        vp = C.c_void_p
+        _hip.hipMalloc.argtypes = [C.POINTER(vp), C.c_size_t]
+        _hip.hipFree.argtypes = [vp]
+        _hip.hipMemcpy.argtypes = [vp, vp, C.c_size_t, C.c_int]
+        _hip.hipMemcpyAsync.argtypes = [vp, vp, C.c_size_t, C.c_int, vp]
+        _hip.hipEventCreate.argtypes = [C.POINTER(vp)]
+        _hip.hipEventRecord.argtypes = [vp, vp]
+        _hip.hipEventSynchronize.argtypes = [vp]
+        _hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), vp, vp]
+        _hip.hipEventDestroy.argtypes = [vp]
+        _hip.hipStreamSynchronize.argtypes = [vp]
+        _hip.hipDeviceSynchronize.argtypes = []
+        _hip.hipSetDevice.argtypes = [C.c_int]
+        _hip.hipGetErrorString.restype = C.c_char_p
+        _hip.hipGetErrorString.argtypes = [C.c_int]
+    return _hip
+
+
+def _ck(rc: int, what: str):
+    if rc != 0:
+        raise RuntimeError(f"{what}: {hip().hipGetErrorString(rc).decode()}")
+
+
+class DeviceArray:
+    """A device allocation with the shape/dtype of a host array."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(shape) if not isinstance(shape, int) else (shape,)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _ck(hip().hipMalloc(C.byref(p), max(self.nbytes, 1)), "hipMalloc")
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        _ck(hip().hipMemcpy(d.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, H2D), "hipMemcpy H2D")
+        return d
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        _ck(hip().hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes, D2H), "hipMemcpy D2H")
+        return out
+
+    def free(self):
+        if self.ptr:
+            hip().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        e = C.c_void_p()
+        _ck(hip().hipEventCreate(C.byref(e)), "hipEventCreate")
+        self.e = e
+
+    def record(self, stream):
+        _ck(hip().hipEventRecord(self.e, stream), "hipEventRecord")
+
+    def sync(self):
+        _ck(hip().hipEventSynchronize(self.e), "hipEventSynchronize")
+
+    def ms_since(self, start: "Event") -> float:
+        v = C.c_float()
+        _ck(hip().hipEventElapsedTime(C.byref(v), start.e, self.e), "hipEventElapsedTime")
+        return float(v.value)
+
+
+def stream_sync(stream):
+    _ck(hip().hipStreamSynchronize(stream), "hipStreamSynchronize")

@@ -41,6 +41,7 @@ def host_lib() -> C.CDLL:
         "hrnn_index_set_knng": [vp, vp, vp, vp],
         "hrnn_index_get_rknng": [vp, P(u32), P(i), P(u64), vp, vp],
         "hrnn_index_proxies": [vp, vp, u64, u32, u32, i, vp],
+        "hrnn_index_proxies_gpu": [vp, vp, u64, u32, u32, vp],
         "hrnn_index_search": [vp, vp, u64, u32, f, u32, u32, u32, i, i, vp, vp, u64, P(u64)],
         "hrnn_index_search_proxies": [vp, vp, u64, vp, u32, vp, u32, u32, f, vp, vp, u64, P(u64)],
         "hrnn_index_insert_dynamic": [vp, vp, u64, vp, u32, i, i],
@@ -159,6 +160,12 @@ class Index:
         q = np.ascontiguousarray(q, np.float32)
         out = np.empty((q.shape[0], m), np.uint32)
         self._ck(self.L.hrnn_index_proxies(self.h, _p(q), q.shape[0], m, ef, nthreads, _p(out)))
+        return out
+
+    def proxies_gpu(self, q, m, ef):
+        q = np.ascontiguousarray(q, np.float32)
+        out = np.empty((q.shape[0], m), np.uint32)
+        self._ck(self.L.hrnn_index_proxies_gpu(self.h, _p(q), q.shape[0], m, ef, _p(out)))
         return out
 
     def search(self, q, m, threshold, k, K_prime, ef, nthreads=0, per_query=False):

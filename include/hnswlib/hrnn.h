@@ -251,7 +251,8 @@ class HRNN : public AlgorithmInterface<dist_t> {
         if (cur_element_count == 0 || nq == 0) return out;
         if (k_rknn == 0 || k_rknn > knn_depth())
             throw std::runtime_error("searchRknn: k must be in [1, K]");
-        std::vector<uint32_t> prox = proxies(Q, nq, m, ef_search, threads);
+        std::vector<uint32_t> prox = phase_a_on_gpu() ? proxiesDevice(Q, nq, m, ef_search)
+                                                      : proxies(Q, nq, m, ef_search, threads);
         std::vector<uint64_t> off(nq + 1);
         std::vector<uint32_t> ids;
         device_search(Q, nq, prox.data(), m, nullptr, (uint32_t)k_rknn, (uint32_t)K_prime,
@@ -283,6 +284,18 @@ class HRNN : public AlgorithmInterface<dist_t> {
             }
             for (size_t j = 0; j < tmp.size(); j++) prox[(size_t)qi * m + j] = tmp[tmp.size() - 1 - j];
         }
+        return prox;
+    }
+
+    // Phase A on the GPU (hrnn_dev_proxies): same ordering rules as proxies().
+    std::vector<uint32_t> proxiesDevice(const float *Q, size_t nq, size_t m, size_t ef_search) const {
+        std::vector<uint32_t> prox(nq * m, INVALID_KNNG_ID);
+        if (cur_element_count == 0 || nq == 0) return prox;
+        std::lock_guard<std::mutex> g(dev_mu_);
+        sync_device_base();
+        sync_device_hnsw();
+        dev_check(hrnn_dev_proxies(dev_, Q, (uint32_t)nq, (uint32_t)m,
+                                   (uint32_t)std::max(ef_search, m), prox.data()));
         return prox;
     }
 
@@ -623,6 +636,7 @@ class HRNN : public AlgorithmInterface<dist_t> {
         }
         ef_ = 10;
         dev_base_rows_ = 0;
+        dev_hnsw_rows_ = 0;
         dev_knng_ok_ = false;
         dev_csr_ok_ = false;
     }
@@ -656,7 +670,13 @@ class HRNN : public AlgorithmInterface<dist_t> {
         std::lock_guard<std::mutex> g(dev_mu_);
         sync_device_base();
         sync_device_knng();
-        sync_device_csr();
+        if (rknng_built_) sync_device_csr();
+        if (cur_element_count > 0 && M_ * 2 <= 64) sync_device_hnsw();
+    }
+    // Phase A runs on the GPU unless HRNN_PHASE_A=host (or maxM0 > 64).
+    bool phase_a_on_gpu() const {
+        const char *e = std::getenv("HRNN_PHASE_A");
+        return !(e && std::string(e) == "host") && maxM0_ <= 64;
     }
 
  private:
@@ -689,6 +709,7 @@ class HRNN : public AlgorithmInterface<dist_t> {
     mutable size_t dev_base_rows_{0};
     mutable bool dev_knng_ok_{false};
     mutable bool dev_csr_ok_{false};
+    mutable size_t dev_hnsw_rows_{0};   // element count when the HNSW graph was uploaded
 
     struct FileLayout {
         size_t elem, label_off, data_off, knng_slot, knng_off;
@@ -1095,6 +1116,7 @@ class HRNN : public AlgorithmInterface<dist_t> {
         }
         dev_base_rows_ = n;
         dev_csr_ok_ = false;
+        dev_hnsw_rows_ = 0;   // the device drops its HNSW copy when the base changes
     }
     void sync_device_knng() const {
         if (dev_knng_ok_) return;
@@ -1108,6 +1130,26 @@ class HRNN : public AlgorithmInterface<dist_t> {
         }
         dev_knng_ok_ = true;
         dev_csr_ok_ = false;
+    }
+    // Upload the HNSW graph for phase A on the GPU (after any insertion).
+    void sync_device_hnsw() const {
+        size_t n = cur_element_count;
+        if (dev_hnsw_rows_ == n) return;   // every insertion changes n
+        const size_t wu = maxM_ + 1;
+        std::vector<uint32_t> off(n, INVALID_KNNG_ID), pool;
+        std::vector<int> lv(levels_.begin(), levels_.begin() + n);
+        size_t total = 0;
+        for (size_t i = 0; i < n; i++) total += (size_t)std::max(0, levels_[i]) * wu;
+        pool.reserve(total);
+        for (size_t i = 0; i < n; i++) {
+            if (levels_[i] <= 0) continue;
+            off[i] = (uint32_t)pool.size();
+            const uint32_t *b = reinterpret_cast<const uint32_t *>(upper_[i]);
+            pool.insert(pool.end(), b, b + (size_t)levels_[i] * wu);
+        }
+        dev_check(hrnn_dev_set_hnsw(dev_, (uint32_t)maxM0_, (uint32_t)maxM_, level0_.data(), lv.data(),
+                                    off.data(), pool.data(), pool.size(), enterpoint_node_, maxlevel_));
+        dev_hnsw_rows_ = n;
     }
     void sync_device_csr() const {
         if (dev_csr_ok_) return;

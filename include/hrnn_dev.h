@@ -131,13 +131,42 @@ int hrnn_dev_rknn_search(hrnn_dev_t h, const float *q, uint32_t nq,
 /* Resident form: q_dev, proxies_dev, k_per_q_dev in HBM; per-query results
  * written to res_cnt_dev[nq] (u32) and res_ids_dev[q * cap_per_q ...].  A query
  * whose result count exceeds cap_per_q keeps the first cap_per_q ids and its
- * count is the true count.  Runs as one persistent launch that pulls queries
- * from a device work queue.  Asynchronous on the handle's stream. */
+ * count is the true count.  work_dev (NULL or 2*nq u32) receives per query the
+ * reverse-list entries scanned and the distinct candidates verified.  Runs as
+ * one persistent launch that pulls queries from a device work queue.
+ * Asynchronous on the handle's stream; proxy ids must be < n or 0xFFFFFFFF. */
 int hrnn_dev_rknn_search_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq,
                               const uint32_t *proxies_dev, uint32_t m,
                               const uint32_t *k_per_q_dev, uint32_t k, uint32_t K_prime,
                               float threshold, uint32_t *res_cnt_dev,
-                              uint32_t *res_ids_dev, uint32_t cap_per_q);
+                              uint32_t *res_ids_dev, uint32_t cap_per_q, uint32_t *work_dev);
+
+/* ---- RkNN phase A on the GPU -------------------------------------------- */
+/* Upload the HNSW graph: l0[n*(maxM0+1)] layer-0 links (word 0: count in the
+ * low 16 bits, then ids); levels[n]; up_off[n] offset (in words) of node i's
+ * upper-layer block in up_pool (0xFFFFFFFF for level-0 nodes), the block
+ * holding (maxM+1) words per layer 1..levels[i]; entry point and top layer.
+ * The whole graph is validated on the host before upload.
+ * Replaces: the layer-0 and upper link lists of HRNN (hrnn.h:160-174, :262-281). */
+int hrnn_dev_set_hnsw(hrnn_dev_t h, uint32_t maxM0, uint32_t maxM, const uint32_t *l0,
+                      const int *levels, const uint32_t *up_off, const uint32_t *up_pool,
+                      uint64_t pool_words, uint32_t entry, int maxlevel);
+/* Phase A of searchRknn_impl (hrnn.h:1992-2019): greedy descent over the
+ * upper layers, layer-0 beam with width ef = max(ef_search, m)
+ * (searchBaseLayerST<true>, hrnn.h:485-616), the m closest as proxies
+ * (proxies[nq*m], 0xFFFFFFFF-padded).  Requires m <= ef <= 512. */
+int hrnn_dev_proxies(hrnn_dev_t h, const float *q, uint32_t nq, uint32_t m, uint32_t ef,
+                     uint32_t *proxies);
+/* Resident form; expansions_dev (NULL or nq u32) receives the beam's expansion count. */
+int hrnn_dev_proxies_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint32_t ef,
+                          uint32_t *proxies_dev, uint32_t *expansions_dev);
+/* Whole searchRknn for a resident batch: phase A into proxies_dev (nq*m), then
+ * phases B and C as hrnn_dev_rknn_search_dptr; one stream, no host round trip. */
+int hrnn_dev_rknn_query_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m,
+                             uint32_t ef, const uint32_t *k_per_q_dev, uint32_t k,
+                             uint32_t K_prime, float threshold, uint32_t *proxies_dev,
+                             uint32_t *res_cnt_dev, uint32_t *res_ids_dev, uint32_t cap_per_q,
+                             uint32_t *work_dev);
 
 /* ---- (d) exact brute force --------------------------------------------- */
 /* kth[i] = (k+1)-th smallest squared distance from base row i to all base rows
@@ -155,6 +184,42 @@ int hrnn_dev_bf_knn(hrnn_dev_t h, const float *q, uint32_t nq, uint32_t k, int p
 int hrnn_dev_bf_rknn(hrnn_dev_t h, const float *q, uint32_t nq, const float *kth,
                      int prec, uint64_t *res_off, uint32_t *res_ids, float *res_d2,
                      uint64_t cap, uint64_t *needed);
+
+/* ---- learned proxy count (no reference counterpart) --------------------- */
+/* A 32-64-64-16 MLP (ReLU, softmax) predicts per query the smallest m of a
+ * 16-value grid that reaches full recall; plain fixed-m search stays the
+ * fallback.  Parameters: HRNN_MLP_PARAMS floats laid out
+ * W1[64][32] b1[64] W2[64][64] b2[64] W3[16][64] b3[16]; a model adds the
+ * feature standardisation mu[32] sd[32] after them (HRNN_MLP_MODEL floats). */
+#define HRNN_MLP_PARAMS 7312
+#define HRNN_MLP_MODEL (HRNN_MLP_PARAMS + 64)
+/* The 32 query features (phase-A beam statistics and r_k of the closest
+ * proxies; unstandardised) and, when proxies != NULL, the m_max proxies. */
+int hrnn_dev_query_features(hrnn_dev_t h, const float *q, uint32_t nq, uint32_t k, uint32_t ef,
+                            uint32_t m_max, float *feats, uint32_t *proxies);
+/* probs[n*16] = softmax(MLP(feats)). */
+int hrnn_dev_mlp_forward(hrnn_dev_t h, const float *params, const float *feats, uint32_t n,
+                         float *probs);
+/* Mean cross-entropy gradient over the n samples (fixed reduction order). */
+int hrnn_dev_mlp_grad(hrnn_dev_t h, const float *params, const float *feats,
+                      const uint32_t *labels, uint32_t n, float *grad, float *loss);
+/* Adam training in place on params (mini-batches, shuffled with seed);
+ * epoch_loss may be NULL or hold one mean loss per epoch. */
+int hrnn_dev_mlp_train(hrnn_dev_t h, float *params, const float *feats, const uint32_t *labels,
+                       uint32_t n, int epochs, uint32_t batch, float lr, float weight_decay,
+                       uint64_t seed, float *epoch_loss);
+/* Whole search with learned m for a resident batch: phase A with m_max
+ * proxies and beam statistics, features standardised with the model's
+ * mu/sd, MLP forward, per-query m = the smallest grid value whose cumulative
+ * probability reaches tau (argmax when tau <= 0; proxies beyond it masked),
+ * then phases B and C.  model_dev: HRNN_MLP_MODEL floats; grid_dev: 16 u32;
+ * m_dev (NULL or nq u32) receives the chosen m. */
+int hrnn_dev_rknn_query_learned_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq,
+                                     const float *model_dev, const uint32_t *grid_dev, float tau,
+                                     uint32_t m_max, uint32_t ef, uint32_t k, uint32_t K_prime,
+                                     float threshold, uint32_t *proxies_dev, uint32_t *m_dev,
+                                     uint32_t *res_cnt_dev, uint32_t *res_ids_dev,
+                                     uint32_t cap_per_q, uint32_t *work_dev);
 
 /* ---- distances ---------------------------------------------------------- */
 /* out[i] = L2Sqr(a[ia[i]], base[ib[i]]) in FP32 with the reference's 16-lane

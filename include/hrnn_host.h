@@ -54,9 +54,12 @@ int hrnn_index_set_knng(hrnn_index_t ix, const uint32_t *ids, const float *dists
 /* Reverse CSR as built by buildRKNNG: offsets[n+1], entries (4 or 8 bytes each). */
 int hrnn_index_get_rknng(hrnn_index_t ix, uint32_t *rank_bits, int *wide, uint64_t *n_entries,
                          uint64_t *offsets, void *entries);
-/* Phase A only: m proxies per query (internal ids, 0xFFFFFFFF-padded). */
+/* Phase A only: m proxies per query (internal ids, 0xFFFFFFFF-padded), on
+ * the host (nthreads OpenMP threads) or, with on_gpu != 0, on the GPU. */
 int hrnn_index_proxies(hrnn_index_t ix, const float *q, uint64_t nq, uint32_t m, uint32_t ef,
                        int nthreads, uint32_t *out);
+int hrnn_index_proxies_gpu(hrnn_index_t ix, const float *q, uint64_t nq, uint32_t m, uint32_t ef,
+                           uint32_t *out);
 /* searchRknn for nq queries: labels concatenated with offsets[nq+1].
  * per_query != 0 calls the single-query method once per query (the driver
  * loop of src/hrnn_search.cpp:168-179); 0 runs one batch. */

@@ -1,0 +1,132 @@
+"""GPU phase A (HNSW beam) and the end-to-end HRNN<float> pipeline on the GPU.
+
+Phase A uses the host beam's ordering rules, so GPU proxies must equal the
+host beam's up to distance ties at the result boundary.  End to end, the
+build (host HNSW + GPU NNDescent + GPU CSR) and search must reach the
+reference pipeline's RkNN recall within 0.5 points at the same parameters
+(configs[0]: the 10k x 128 mixture, K=100, ef_c=200, 10 x S=50).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import hrnn_amd
+import oracle
+from conftest import mixture
+from hrnn_amd.index import Index
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def built_small():
+    x = mixture(3000, seed=61, centre_seed=1061)
+    q = mixture(80, seed=62, centre_seed=1061)
+    ix = Index(3000, 128, 16, 100, 100, 48)
+    ix.add_points(x, nthreads=1)
+    return ix, x, q
+
+
+@pytest.mark.parametrize("m,ef", [(1, 10), (10, 64), (20, 200), (50, 200), (64, 64)])
+def test_gpu_proxies_match_host_beam(built_small, m, ef):
+    ix, x, q = built_small
+    hp = ix.proxies(q, m, ef, 1)
+    gp = ix.proxies_gpu(q, m, ef)
+    same = np.mean([len(set(hp[i].tolist()) & set(gp[i].tolist())) / m for i in range(len(q))])
+    assert same >= 0.995, same
+    exact = oracle.knn(x, q, m)[0]
+    rec = np.mean([len(set(gp[i].tolist()) & set(exact[i].tolist())) / m for i in range(len(q))])
+    assert rec > 0.9
+
+
+def test_gpu_proxies_device_api(built_small):
+    """The raw C-ABI path: upload graph arrays, compare with the index's own call."""
+    ix, x, q = built_small
+    links, levels = ix.level0()
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    # level-0-only upload (maxlevel 0): every search starts at the entry node
+    with pytest.raises(hrnn_amd.HrnnError):
+        dev.set_hnsw(32, 16, links, np.zeros(len(x), np.int32), np.full(len(x), 0xFFFFFFFF),
+                     np.zeros(0, np.uint32), 1 << 20, 0)   # entry out of range
+
+
+def test_end_to_end_recall_vs_reference_c1():
+    """10k x 128, 50 centres, sigma 20; M=16 ef_c=200 K=100 iters=10 S=50;
+    k=10, K'=100, m in {1,5,10}, ef_search=200."""
+    x = hrnn_amd.synth_mixture(1, 10_000, 128, 50, 20.0, 0.0, 255.0, True, 0.0, 1001)
+    q = hrnn_amd.synth_mixture(2, 100, 128, 50, 20.0, 0.0, 255.0, True, 0.0, 1001)
+    kth = oracle.kth(x, 10)
+    gt_off, gt_ids, _ = oracle.rknn_gt(x, q, kth)
+    ix = Index(10_000, 128, 16, 200, 100, 100)
+    ix.add_points(x, nthreads=8)
+    st = ix.refine(10, 50)
+    assert st[0]["dropped"] == 0
+    ix.build_rknng()
+    ref = oracle.RefIndex(10_000, 128, M=16, efc=200, K=100)
+    ref.add_points(x, nthreads=8)
+    ref.refine(10, 50, nthreads=8)
+    ref.build_rknng()
+    # KNN graph quality (labels mapped)
+    exact = oracle.knn(x, x, 101)[0][:, 1:].copy()
+    ids, d, cnt = ix.get_knng()
+    lab = ix.labels()
+    our_lists = lab[ids]
+    our = np.zeros_like(our_lists)
+    our[lab] = our_lists
+    our_cnt = np.zeros_like(cnt)
+    our_cnt[lab] = cnt
+    rid, rd, rc = ref.get_knng()
+    rlab = ref.labels()
+    rl = np.zeros_like(rid)
+    rl[rlab] = rlab[rid]
+    rcnt = np.zeros_like(rc)
+    rcnt[rlab] = rc
+    g_rec = oracle.knng_recall(our.astype(np.uint32), our_cnt, 100, exact, 100)
+    r_rec = oracle.knng_recall(rl.astype(np.uint32), rcnt, 100, exact, 100)
+    assert abs(g_rec - r_rec) <= 0.005, (g_rec, r_rec)
+    for m in (1, 5, 10):
+        off, labs = ix.search(q, m, 1.0, 10, 100, 200)
+        roff, rlabs = ref.search(q, m, 1.0, 10, 100, 200, nthreads=8)
+        ours = oracle.rknn_recall(gt_off, gt_ids, off, labs.astype(np.uint32), 10_000)[0]
+        theirs = oracle.rknn_recall(gt_off, gt_ids, roff, rlabs.astype(np.uint32), 10_000)[0]
+        assert abs(ours - theirs) <= 0.005, (m, ours, theirs)
+
+
+def test_gpu_built_index_loads_in_reference(tmp_path, built_small):
+    """A GPU-refined index with its GPU-built CSR is read by the reference loader
+    and searched by the reference with the same results as on the GPU."""
+    _, x, q = built_small
+    ix = Index(3000, 128, 16, 100, 100, 48)
+    ix.add_points(x, nthreads=1)
+    ix.refine(3, 12)
+    ix.build_rknng()
+    p = str(tmp_path / "gpu.idx")
+    ix.save(p)
+    r = oracle.RefIndex.load(p, 128)
+    off, lab = ix.search(q, 10, 1.0, 10, 48, 64, per_query=True)
+    roff, rlab = r.search(q, 10, 1.0, 10, 48, 64)
+    same = np.mean([len(set(lab[off[i]:off[i + 1]].tolist()) & set(rlab[roff[i]:roff[i + 1]].tolist()))
+                    / max(1, roff[i + 1] - roff[i]) for i in range(len(q))])
+    assert same >= 0.99
+
+
+def test_dynamic_insert_then_gpu_search(built_small):
+    """hrnn_dynamic_test flow at small scale: build half, insert half, flat CSR, search."""
+    _, x, q = built_small
+    n = x.shape[0]
+    perm = np.random.default_rng(42).permutation(n)
+    half = n // 2
+    ix = Index(n, 128, 16, 100, 100, 48)
+    ix.add_points(x[perm[:half]], labels=perm[:half].astype(np.uint64), nthreads=4)
+    ix.refine(3, 12)
+    ix.build_rknng_mutable()
+    ix.insert_dynamic(x[perm[half:]], perm[half:].astype(np.uint64), 0, False, nthreads=4)
+    assert ix.count == n
+    ix.build_rknng()
+    kth = oracle.kth(x, 10)
+    gt_off, gt_ids, _ = oracle.rknn_gt(x, q, kth)
+    off, lab = ix.search(q, 20, 1.0, 10, 48, 100)
+    rec = oracle.rknn_recall(gt_off, gt_ids, off, lab.astype(np.uint32), n)[0]
+    assert rec > 0.8, rec
