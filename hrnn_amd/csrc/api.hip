@@ -226,6 +226,12 @@ int hrnn_dev_base_is_small_int(hrnn_dev_t h, int *is_int) {
 
 const float *hrnn_dev_base_ptr(hrnn_dev_t h) { return h ? h->x.as<float>() : nullptr; }
 
+int hrnn_dev_set_vector_precision(hrnn_dev_t h, int prec) {
+    if (!h || (prec != HRNN_PREC_F32 && prec != HRNN_PREC_BF16)) return HRNN_E_ARG;
+    h->vec_bf16 = (prec == HRNN_PREC_BF16);
+    return HRNN_OK;
+}
+
 int hrnn_dev_set_knng(hrnn_dev_t h, uint32_t K, const uint32_t *ids, const float *dists,
                       const uint16_t *cnt) {
     if (!h) return HRNN_E_ARG;

@@ -25,13 +25,9 @@ namespace hrnn {
 constexpr int kBeamMaxEf = 512;
 constexpr int kBeamMaxDeg = 64;
 
-__device__ __forceinline__ float beam_dist(const float *__restrict__ qv, const float *__restrict__ xv,
-                                           uint32_t d, uint32_t g) {
-    return ((d & 15u) == 0) ? l2_ref16_group(qv, xv, d, (int)g) : l2_seq(qv, xv, d);
-}
-
+template <class VT>
 __global__ __launch_bounds__(64) void k_beam(
-    const float *__restrict__ x, uint32_t d, uint64_t n, const uint32_t *__restrict__ l0,
+    const VT *__restrict__ x, uint32_t d, uint64_t n, const uint32_t *__restrict__ l0,
     uint32_t l0_words, const uint32_t *__restrict__ up_off, const uint32_t *__restrict__ up_pool,
     uint32_t up_words, uint32_t entry, int maxlevel, const float *__restrict__ q, uint32_t nq,
     uint32_t ef, uint32_t m, uint32_t *__restrict__ visited_all, uint64_t vis_words,
@@ -61,7 +57,7 @@ __global__ __launch_bounds__(64) void k_beam(
         uint32_t cur = entry;
         float cd;
         {
-            float t = beam_dist(s_qv, x + (uint64_t)cur * d, d, g);
+            float t = dist_row<VT>(s_qv, x, cur, d, g);
             cd = __shfl(t, 0, 64);
         }
         const float d_entry = cd;
@@ -77,7 +73,7 @@ __global__ __launch_bounds__(64) void k_beam(
                     uint32_t id = 0;
                     if (t < cnt) {   // uniform per 16-lane group
                         id = ll[1 + t];
-                        dv = beam_dist(s_qv, x + (uint64_t)id * d, d, g);
+                        dv = dist_row<VT>(s_qv, x, id, d, g);
                     }
                     // sequential scan order of the host: first strictly smaller wins
                     for (uint32_t s = 0; s < 4; s++) {
@@ -106,7 +102,7 @@ __global__ __launch_bounds__(64) void k_beam(
         }
         if (lane == 0) atomicOr(&vis[cur >> 5], 1u << (cur & 31u));
         __syncthreads();
-        uint32_t n_exp = 0;
+        uint32_t n_exp = 0, n_eval = 1;
         for (;;) {
             // closest unexpanded entry of the sorted list
             uint32_t pick = 0xFFFFFFFFu;
@@ -135,6 +131,7 @@ __global__ __launch_bounds__(64) void k_beam(
             }
             uint64_t fm = __ballot(fresh);
             uint32_t nf = (uint32_t)__popcll(fm);
+            n_eval += nf;
             uint32_t slot = (uint32_t)__popcll(fm & lanemask_lt());
             if (fresh) s_ni[slot] = nb;
             __syncthreads();
@@ -142,7 +139,7 @@ __global__ __launch_bounds__(64) void k_beam(
             for (uint32_t b = 0; b < nf; b += 4) {
                 uint32_t t = b + grp;
                 if (t < nf) {
-                    float dv = beam_dist(s_qv, x + (uint64_t)s_ni[t] * d, d, g);
+                    float dv = dist_row<VT>(s_qv, x, s_ni[t], d, g);
                     if (g == 0) s_nd[t] = dv;
                 }
             }
@@ -181,7 +178,10 @@ __global__ __launch_bounds__(64) void k_beam(
         // ---- output: the m closest ----
         for (uint32_t t = lane; t < m; t += 64)
             proxies[(uint64_t)qi * m + t] = (t < L) ? s_i[buf][t] : kInvalidId;
-        if (expansions && lane == 0) expansions[qi] = n_exp;
+        if (expansions && lane == 0) {
+            expansions[2 * (uint64_t)qi] = n_exp;
+            expansions[2 * (uint64_t)qi + 1] = n_eval;
+        }
         if (raw) {   // per-query statistics for the learned m: see kRawWords
             float *r = raw + (uint64_t)qi * kRawWords;
             if (lane < 16) r[2 + lane] = (lane < L) ? s_d[buf][lane] : s_d[buf][L - 1];
@@ -215,11 +215,17 @@ void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint
     }
     uint32_t *queue = h->counters.as<uint32_t>() + 80;
     HIPC(hipMemsetAsync(queue, 0, 4, h->stream));
-    hipLaunchKernelGGL(k_beam, dim3(blocks), dim3(64), h->d * sizeof(float), h->stream,
-                       h->x.as<float>(), h->d, h->n, h->l0.as<uint32_t>(), h->l0_words,
-                       h->up_off.as<uint32_t>(), h->up_pool.as<uint32_t>(), h->up_words, h->entry,
-                       h->maxlevel, q_dev, nq, ef, m, h->beam_vis.as<uint32_t>(), vis_words, queue,
-                       proxies_dev, expansions_dev, raw_dev);
+#define LAUNCH(VT, XP)                                                                          \
+    hipLaunchKernelGGL(k_beam<VT>, dim3(blocks), dim3(64), h->d * sizeof(float), h->stream, XP, h->d, \
+                       h->n, h->l0.as<uint32_t>(), h->l0_words, h->up_off.as<uint32_t>(),            \
+                       h->up_pool.as<uint32_t>(), h->up_words, h->entry, h->maxlevel, q_dev, nq, ef, m, \
+                       h->beam_vis.as<uint32_t>(), vis_words, queue, proxies_dev, expansions_dev, raw_dev)
+    // BF16 mirror where it is exact (same distances, half the bytes)
+    if (h->small_int && h->vec_bf16)
+        LAUNCH(__hip_bfloat16, h->xb.as<__hip_bfloat16>());
+    else
+        LAUNCH(float, h->x.as<float>());
+#undef LAUNCH
     HIPC(hipGetLastError());
 }
 

@@ -81,6 +81,24 @@ __device__ __forceinline__ float l2_ref16_group(const float *__restrict__ a,
     return r;
 }
 
+// Same order with the second operand stored as BF16 (exactly representable
+// data only: integers in [-256, 256], where the result equals the FP32 one
+// bit for bit).
+__device__ __forceinline__ float l2_ref16_group_bf16(const float *__restrict__ a,
+                                                     const __hip_bfloat16 *__restrict__ b,
+                                                     uint32_t d, int g) {
+    float p = 0.f;
+    for (uint32_t e = (uint32_t)g; e < d; e += 16) {
+        float t = a[e] - __bfloat162float(b[e]);
+        p = __builtin_fmaf(t, t, p);
+    }
+    int base = (int)(threadIdx.x & 63u) & ~15;
+    float r = __shfl(p, base + 0, 64);
+#pragma unroll
+    for (int l = 1; l < 16; l++) r += __shfl(p, base + l, 64);
+    return r;
+}
+
 // Sequential-order squared L2 (space_l2.h:6-20 with contraction), used when
 // d is not a multiple of 16.
 __device__ __forceinline__ float l2_seq(const float *__restrict__ a, const float *__restrict__ b,
@@ -88,6 +106,33 @@ __device__ __forceinline__ float l2_seq(const float *__restrict__ a, const float
     float r = 0.f;
     for (uint32_t e = 0; e < d; e++) {
         float t = a[e] - b[e];
+        r = __builtin_fmaf(t, t, r);
+    }
+    return r;
+}
+
+// Distance from an FP32 vector (LDS) to row `id` of the base stored as VT
+// (float, or __hip_bfloat16 for exactly representable data), computed by a
+// 16-lane group in the reference's order (sequential when d % 16 != 0).
+template <class VT>
+__device__ __forceinline__ float dist_row(const float *__restrict__ qv, const VT *__restrict__ base,
+                                          uint64_t id, uint32_t d, uint32_t g);
+template <>
+__device__ __forceinline__ float dist_row<float>(const float *__restrict__ qv,
+                                                 const float *__restrict__ base, uint64_t id,
+                                                 uint32_t d, uint32_t g) {
+    const float *xv = base + id * d;
+    return ((d & 15u) == 0) ? l2_ref16_group(qv, xv, d, (int)g) : l2_seq(qv, xv, d);
+}
+template <>
+__device__ __forceinline__ float dist_row<__hip_bfloat16>(const float *__restrict__ qv,
+                                                          const __hip_bfloat16 *__restrict__ base,
+                                                          uint64_t id, uint32_t d, uint32_t g) {
+    const __hip_bfloat16 *xv = base + id * d;
+    if ((d & 15u) == 0) return l2_ref16_group_bf16(qv, xv, d, (int)g);
+    float r = 0.f;
+    for (uint32_t e = 0; e < d; e++) {
+        float t = qv[e] - __bfloat162float(xv[e]);
         r = __builtin_fmaf(t, t, r);
     }
     return r;

@@ -15,6 +15,9 @@ struct hrnn_dev {
     uint32_t d = 0;
     hrnn::DevBuf x, xb, xh, norms;
     int small_int = 0;   // every value an integer in [-256, 256]
+    // read vectors from the BF16 mirror where that is exact (small_int);
+    // HRNN_VEC_FP32=1 forces the FP32 copy (used to compare the two paths)
+    bool vec_bf16 = true;
 
     // ranked KNN graph (SoA)
     uint32_t K = 0;

@@ -27,9 +27,9 @@ constexpr int kSearchBlock = 256;
 constexpr int kMaxM = 1024;
 constexpr uint32_t kResLds = 4096;   // results sorted in LDS up to this count
 
-template <class EntryT, bool kCount>
+template <class EntryT, bool kCount, class VT>
 __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
-    const float *__restrict__ x, uint32_t d, const uint64_t *__restrict__ off,
+    const VT *__restrict__ x, uint32_t d, const uint64_t *__restrict__ off,
     const EntryT *__restrict__ ent, uint32_t rank_bits, const float *__restrict__ knn_d,
     const uint32_t *__restrict__ knn_cnt, uint32_t K, const float *__restrict__ q, uint32_t nq,
     const uint32_t *__restrict__ proxies, uint32_t m, const uint32_t *__restrict__ k_per_q,
@@ -134,8 +134,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
         uint32_t grp = threadIdx.x / 16, g = threadIdx.x & 15u;
         for (uint32_t i = grp; i < nkeep; i += blockDim.x / 16) {
             uint32_t c = keep[i];
-            const float *xc = x + (uint64_t)c * d;
-            float dist = ((d & 15u) == 0) ? l2_ref16_group(s_qv, xc, d, (int)g) : l2_seq(s_qv, xc, d);
+            float dist = dist_row<VT>(s_qv, x, c, d, g);
             if (g == 0) {
                 float kd = (kidx < knn_cnt[c]) ? knn_d[(uint64_t)c * K + kidx] : FLT_MAX;
                 if (dist <= kd) {
@@ -206,22 +205,28 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     uint32_t *queue = h->counters.as<uint32_t>() + 64;
     HIPC(hipMemsetAsync(queue, 0, 4, h->stream));
     size_t dyn = (size_t)h->d * sizeof(float);
-    const float *x = h->x.as<float>();
     const uint64_t *off = h->csr_off.as<uint64_t>();
     const float *kd = h->knn_d.as<float>();
     const uint32_t *kc = h->knn_cnt.as<uint32_t>();
     uint32_t *mark = h->search_mark.as<uint32_t>();
     uint32_t *cand = h->search_cand.as<uint32_t>();
-#define LAUNCH(ET, CM)                                                                           \
-    hipLaunchKernelGGL((k_rknn_search<ET, CM>), dim3(slots), dim3(kSearchBlock), dyn, h->stream, x, \
-                       h->d, off, h->csr_ent.as<ET>(), h->rank_bits, kd, kc, h->K, q, nq, proxies, m, \
-                       k_per_q, k, K_prime, threshold, mark, words, cand, cand_cap, queue, res_cnt,  \
+    // Candidate vectors are read from the BF16 mirror when every base value is
+    // exactly representable in it (same distances, half the bytes).
+    const bool bf16 = h->small_int && h->vec_bf16;
+#define LAUNCH(ET, CM, VT, XP)                                                                      \
+    hipLaunchKernelGGL((k_rknn_search<ET, CM, VT>), dim3(slots), dim3(kSearchBlock), dyn, h->stream, \
+                       XP, h->d, off, h->csr_ent.as<ET>(), h->rank_bits, kd, kc, h->K, q, nq, proxies, \
+                       m, k_per_q, k, K_prime, threshold, mark, words, cand, cand_cap, queue, res_cnt, \
                        res_ids, cap_per_q, work)
+#define LAUNCH_VT(ET, CM)                                                  \
+    if (bf16) LAUNCH(ET, CM, __hip_bfloat16, h->xb.as<__hip_bfloat16>());  \
+    else LAUNCH(ET, CM, float, h->x.as<float>())
     if (h->wide) {
-        if (count_mode) LAUNCH(uint64_t, true); else LAUNCH(uint64_t, false);
+        if (count_mode) { LAUNCH_VT(uint64_t, true); } else { LAUNCH_VT(uint64_t, false); }
     } else {
-        if (count_mode) LAUNCH(uint32_t, true); else LAUNCH(uint32_t, false);
+        if (count_mode) { LAUNCH_VT(uint32_t, true); } else { LAUNCH_VT(uint32_t, false); }
     }
+#undef LAUNCH_VT
 #undef LAUNCH
     HIPC(hipGetLastError());
 }

@@ -29,6 +29,7 @@ EXPORTS = [
     "hrnn_dev_device_count", "hrnn_dev_open", "hrnn_dev_close", "hrnn_dev_last_error",
     "hrnn_dev_sync", "hrnn_dev_stream", "hrnn_dev_set_base", "hrnn_dev_set_base_dptr",
     "hrnn_dev_append_base", "hrnn_dev_base_is_small_int", "hrnn_dev_base_ptr",
+    "hrnn_dev_set_vector_precision",
     "hrnn_dev_set_knng", "hrnn_dev_get_knng", "hrnn_dev_knng_ptrs", "hrnn_dev_nndescent",
     "hrnn_dev_build_rknn_csr", "hrnn_dev_get_rknn_csr", "hrnn_dev_set_rknn_csr",
     "hrnn_dev_rknn_search", "hrnn_dev_rknn_search_dptr", "hrnn_dev_set_hnsw", "hrnn_dev_proxies",
@@ -74,6 +75,7 @@ def lib() -> C.CDLL:
         "hrnn_dev_set_base_dptr": [vp, vp, u64, u32],
         "hrnn_dev_append_base": [vp, vp, u64],
         "hrnn_dev_base_is_small_int": [vp, P(i)],
+        "hrnn_dev_set_vector_precision": [vp, i],
         "hrnn_dev_set_knng": [vp, u32, vp, vp, vp],
         "hrnn_dev_get_knng": [vp, vp, vp, vp],
         "hrnn_dev_knng_ptrs": [vp, vp, vp, vp],
@@ -212,6 +214,9 @@ class Device:
         v = C.c_int(0)
         self._ck(self.L.hrnn_dev_base_is_small_int(self.h, C.byref(v)))
         return bool(v.value)
+
+    def set_vector_precision(self, prec: str):
+        self._ck(self.L.hrnn_dev_set_vector_precision(self.h, _PREC[prec]))
 
     def set_knng(self, ids, dists, cnt):
         ids = np.ascontiguousarray(ids, np.uint32)

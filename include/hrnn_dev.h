@@ -74,6 +74,11 @@ int hrnn_dev_append_base(hrnn_dev_t h, const float *x, uint64_t n_add);
 int hrnn_dev_base_is_small_int(hrnn_dev_t h, int *is_int);
 /* Device pointer of the FP32 base (n*d floats). */
 const float *hrnn_dev_base_ptr(hrnn_dev_t h);
+/* Storage the search and phase-A kernels read vectors from: HRNN_PREC_BF16
+ * (default) uses the BF16 mirror when every base value is exactly
+ * representable in it (then distances are bit-identical to FP32 and the
+ * vector bytes halve) and FP32 otherwise; HRNN_PREC_F32 always reads FP32. */
+int hrnn_dev_set_vector_precision(hrnn_dev_t h, int prec);
 
 /* ---- ranked KNN graph --------------------------------------------------- */
 /* Host layout (SoA): ids[n*K], dists[n*K] (squared L2), cnt[n] (u16, <= K).
@@ -157,7 +162,8 @@ int hrnn_dev_set_hnsw(hrnn_dev_t h, uint32_t maxM0, uint32_t maxM, const uint32_
  * (proxies[nq*m], 0xFFFFFFFF-padded).  Requires m <= ef <= 512. */
 int hrnn_dev_proxies(hrnn_dev_t h, const float *q, uint32_t nq, uint32_t m, uint32_t ef,
                      uint32_t *proxies);
-/* Resident form; expansions_dev (NULL or nq u32) receives the beam's expansion count. */
+/* Resident form; expansions_dev (NULL or 2*nq u32) receives per query the
+ * layer-0 nodes expanded and the distances evaluated at layer 0. */
 int hrnn_dev_proxies_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint32_t ef,
                           uint32_t *proxies_dev, uint32_t *expansions_dev);
 /* Whole searchRknn for a resident batch: phase A into proxies_dev (nq*m), then
