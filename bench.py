@@ -206,6 +206,13 @@ def main() -> int:
         m_star = cfg["ms"][-1]
     rec_star = next(s["recall"] for s in sweep if s["m"] == m_star)
 
+    # ---- launch shapes of the persistent kernels from the autotuner ------------------------
+    from hrnn_amd import autotune
+    tuned = autotune.tune(h, qd, nq, m_star, ef, k, Kp,
+                          dict(n=n, d=d, K=K, m=m_star, ef=ef, nq=nq), pd, cd, idd, cap,
+                          use_cache=False)
+    log(f"autotune: beam_wg_per_cu={tuned['beam_wg_per_cu']} search_wg_per_cu={tuned['search_wg_per_cu']}")
+
     paths = {}
     ck(L.hrnn_dev_set_vector_precision(h, 0))          # FP32 vectors
     run_fixed(m_star)
@@ -354,6 +361,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "paths": paths,
         "learned_m": learned_info,
+        "autotune": {kk: v for kk, v in tuned.items() if kk != "timings_ms"},
         "nndescent_build": {"seconds": round(nnd_s, 3), "iters": cfg["iters"], "sample": cfg["S"],
                             "knn_recall_at10_sample": round(knn_rec10, 4),
                             "per_iter": [{k2: (round(v, 2) if isinstance(v, float) else v)

@@ -1,0 +1,108 @@
+"""Autotuner for the launch shapes of the persistent search kernels.
+
+For a resident index and query batch it times each candidate launch shape of
+the phase-A beam kernel ("beam_wg_per_cu") and of the phase-B/C search
+kernel ("search_wg_per_cu") with HIP events on the library's stream, keeps
+the fastest, and caches the choice as JSON keyed by the GPU and the
+workload shape (n, d, K, m, ef, nq).  Results are identical for every
+shape; only the time changes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+
+import numpy as np
+
+from . import hiprt
+from .dev import lib
+
+CANDIDATES = {
+    "beam_wg_per_cu": [4, 8, 16, 24, 32],
+    "search_wg_per_cu": [2, 4, 8, 16],
+}
+CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "cache.json")
+
+
+def _gpu_name() -> str:
+    try:
+        out = os.popen("rocm-smi --showproductname 2>/dev/null").read()
+        for line in out.splitlines():
+            if "Card Series" in line or "Card series" in line:
+                return line.split(":")[-1].strip()
+    except Exception:
+        pass
+    return "gpu"
+
+
+def _key(shape: dict) -> str:
+    return _gpu_name() + "|" + ",".join(f"{k}={shape[k]}" for k in sorted(shape))
+
+
+def load_cache() -> dict:
+    try:
+        with open(CACHE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def save_cache(c: dict) -> None:
+    os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+    tmp = CACHE + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(c, f, indent=1, sort_keys=True)
+    os.replace(tmp, CACHE)
+
+
+def tune(h, qd, nq: int, m: int, ef: int, k: int, K_prime: int, shape: dict, pd, cd, idd, cap,
+         reps: int = 5, use_cache: bool = True) -> dict:
+    """Tune on the device handle h with resident queries qd and scratch pd/cd/idd.
+    Returns {"beam_wg_per_cu": v, "search_wg_per_cu": v, "timings": {...}}."""
+    L = lib()
+    key = _key(shape)
+    cache = load_cache() if use_cache else {}
+    if use_cache and key in cache:
+        for k_, v in cache[key].items():
+            if k_ in CANDIDATES:
+                L.hrnn_dev_set_tuning(h, k_.encode(), int(v))
+        return dict(cache[key], cached=True)
+    stream = L.hrnn_dev_stream(h)
+    e0, e1 = hiprt.Event(), hiprt.Event()
+
+    def ck(rc):
+        if rc:
+            raise RuntimeError(L.hrnn_dev_last_error(h).decode())
+
+    def time_it(fn):
+        fn()
+        hiprt.stream_sync(stream)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.sync()
+        return e1.ms_since(e0) / reps
+
+    beam = lambda: ck(L.hrnn_dev_proxies_dptr(h, qd.ptr, nq, m, max(ef, m), pd.ptr, None))
+    search = lambda: ck(L.hrnn_dev_rknn_search_dptr(h, qd.ptr, nq, pd.ptr, m, None, k, K_prime, 1.0,
+                                                    cd.ptr, idd.ptr, cap, None))
+    out, timings = {}, {}
+    for name, fn in (("beam_wg_per_cu", beam), ("search_wg_per_cu", search)):
+        best, best_ms = None, float("inf")
+        for v in CANDIDATES[name]:
+            ck(L.hrnn_dev_set_tuning(h, name.encode(), v))
+            if name == "search_wg_per_cu":
+                beam()          # proxies for the search
+            ms = time_it(fn)
+            timings[f"{name}={v}"] = round(ms, 4)
+            if ms < best_ms:
+                best, best_ms = v, ms
+        ck(L.hrnn_dev_set_tuning(h, name.encode(), best))
+        out[name] = best
+    out["timings_ms"] = timings
+    if use_cache:
+        cache[key] = out
+        save_cache(cache)
+    return out

@@ -226,6 +226,35 @@ int hrnn_dev_base_is_small_int(hrnn_dev_t h, int *is_int) {
 
 const float *hrnn_dev_base_ptr(hrnn_dev_t h) { return h ? h->x.as<float>() : nullptr; }
 
+static int *tuning_slot(hrnn_dev_t h, const char *key) {
+    std::string k = key ? key : "";
+    if (k == "search_wg_per_cu") return &h->tune_search_wg_per_cu;
+    if (k == "beam_wg_per_cu") return &h->tune_beam_wg_per_cu;
+    return nullptr;
+}
+
+int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value) {
+    if (!h) return HRNN_E_ARG;
+    int *slot = tuning_slot(h, key);
+    if (!slot || value < 1 || value > 64) {
+        h->err = std::string("set_tuning: unknown key or value out of [1, 64]: ") + (key ? key : "");
+        return HRNN_E_ARG;
+    }
+    *slot = value;
+    // the per-workgroup private state is sized by the workgroup count
+    h->search_mark.release();
+    h->beam_vis.release();
+    return HRNN_OK;
+}
+
+int hrnn_dev_get_tuning(hrnn_dev_t h, const char *key, int *value) {
+    if (!h || !value) return HRNN_E_ARG;
+    int *slot = tuning_slot(h, key);
+    if (!slot) return HRNN_E_ARG;
+    *value = *slot;
+    return HRNN_OK;
+}
+
 int hrnn_dev_set_vector_precision(hrnn_dev_t h, int prec) {
     if (!h || (prec != HRNN_PREC_F32 && prec != HRNN_PREC_BF16)) return HRNN_E_ARG;
     h->vec_bf16 = (prec == HRNN_PREC_BF16);

@@ -207,7 +207,7 @@ void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint
     REQUIRE(h->l0_words - 1 <= (uint32_t)kBeamMaxDeg, HRNN_E_ARG, "proxies: layer-0 degree > 64");
     if (nq == 0) return;
     uint64_t vis_words = (h->n + 31) / 32;
-    uint32_t blocks = (uint32_t)std::min<uint64_t>(nq, (uint64_t)h->n_cu * 16);
+    uint32_t blocks = (uint32_t)std::min<uint64_t>(nq, (uint64_t)h->n_cu * h->tune_beam_wg_per_cu);
     uint64_t need = (uint64_t)blocks * vis_words * 4;
     if (h->beam_vis.bytes < need) {
         h->beam_vis.reserve(need);

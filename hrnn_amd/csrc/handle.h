@@ -15,9 +15,12 @@ struct hrnn_dev {
     uint32_t d = 0;
     hrnn::DevBuf x, xb, xh, norms;
     int small_int = 0;   // every value an integer in [-256, 256]
-    // read vectors from the BF16 mirror where that is exact (small_int);
-    // HRNN_VEC_FP32=1 forces the FP32 copy (used to compare the two paths)
+    // read vectors from the BF16 mirror where that is exact (small_int)
     bool vec_bf16 = true;
+
+    // launch shapes chosen by the autotuner (hrnn_dev_set_tuning)
+    int tune_search_wg_per_cu = 8;   // persistent RkNN search workgroups per CU
+    int tune_beam_wg_per_cu = 16;    // persistent beam workgroups (one wave each) per CU
 
     // ranked KNN graph (SoA)
     uint32_t K = 0;

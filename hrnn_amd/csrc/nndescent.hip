@@ -450,15 +450,17 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t bucket, hrnn_nnd_ite
              *ro_cnt = rn_cnt + n, *prop_cnt = ro_cnt + n;
     uint32_t *prop_id = prop.as<uint32_t>();
     float *prop_d = reinterpret_cast<float *>(prop_id + n * (uint64_t)P);
-    unsigned long long *stats = reinterpret_cast<unsigned long long *>(h->counters.as<uint64_t>() + 64);
 
     uint32_t *ids = h->knn_ids.as<uint32_t>();
     float *kd = h->knn_d.as<float>();
     uint32_t *cnt = h->knn_cnt.as<uint32_t>();
     uint8_t *isnew = h->knn_new.as<uint8_t>();
+    // per-iteration counters: ST_N u64 per iteration (zeroed by k_nnd_prepare)
+    h->s5.reserve((uint64_t)std::max(iters, 1) * ST_N * 8 + 64);
+    unsigned long long *all_stats = h->s5.as<unsigned long long>();
 
-    for (int it = 0; it < iters; it++) {
-        auto t0 = std::chrono::steady_clock::now();
+    auto enqueue_iteration = [&](int it) {
+        unsigned long long *stats = all_stats + (uint64_t)it * ST_N;
         hipLaunchKernelGGL(k_nnd_prepare, dim3(ceil_div(n, 256)), dim3(256), 0, h->stream, kd, cnt, n,
                            K, worst, rn_cnt, ro_cnt, prop_cnt, stats);
         hipLaunchKernelGGL(k_nnd_sample, dim3(ceil_div(n, 4)), dim3(256), 0, h->stream, ids, kd, isnew,
@@ -470,18 +472,50 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t bucket, hrnn_nnd_ite
                            prop_cnt, prop_id, prop_d, P, exact, stats);
         hipLaunchKernelGGL(k_nnd_merge, dim3((unsigned)n), dim3(256), 0, h->stream, ids, kd, isnew,
                            cnt, K, prop_cnt, prop_id, prop_d, P, stats);
-        HIPC(hipGetLastError());
-        unsigned long long sv[ST_N];
-        d2h(h, sv, stats, sizeof(sv));
-        auto t1 = std::chrono::steady_clock::now();
-        if (st) {
-            st[it].dists = sv[ST_DISTS];
-            st[it].proposals = sv[ST_PROPOSALS];
-            st[it].dropped = sv[ST_DROPPED];
-            st[it].updates = sv[ST_UPDATES];
-            st[it].ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    };
+
+    const char *env = std::getenv("HRNN_NND_GRAPH");
+    const bool use_graph = !(env && env[0] == '0') && iters > 0;
+    std::vector<unsigned long long> sv((uint64_t)std::max(iters, 1) * ST_N, 0);
+    if (use_graph) {
+        // All iterations captured as one HIP graph: 4 kernels per iteration, no
+        // host synchronisation in between; counters are reset by kernels only.
+        auto t0 = std::chrono::steady_clock::now();
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        HIPC(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        for (int it = 0; it < iters; it++) enqueue_iteration(it);
+        hipError_t ce = hipStreamEndCapture(h->stream, &graph);
+        if (ce != hipSuccess) throw Error{HRNN_E_HIP, std::string("nndescent capture: ") + hipGetErrorString(ce)};
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            throw Error{HRNN_E_HIP, std::string("nndescent graph instantiate: ") + hipGetErrorString(ie)};
+        }
+        HIPC(hipGraphLaunch(exec, h->stream));
+        d2h(h, sv.data(), all_stats, sv.size() * 8);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        double total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (st)
+            for (int it = 0; it < iters; it++) st[it].ms = total / iters;   // per-iteration share
+    } else {
+        for (int it = 0; it < iters; it++) {
+            auto t0 = std::chrono::steady_clock::now();
+            enqueue_iteration(it);
+            HIPC(hipGetLastError());
+            d2h(h, sv.data() + (uint64_t)it * ST_N, all_stats + (uint64_t)it * ST_N, ST_N * 8);
+            if (st) st[it].ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
     }
+    if (st)
+        for (int it = 0; it < iters; it++) {
+            const unsigned long long *s = sv.data() + (uint64_t)it * ST_N;
+            st[it].dists = s[ST_DISTS];
+            st[it].proposals = s[ST_PROPOSALS];
+            st[it].dropped = s[ST_DROPPED];
+            st[it].updates = s[ST_UPDATES];
+        }
 }
 
 }  // namespace hrnn

@@ -189,7 +189,7 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     // distinct candidates <= sum of the m prefix lengths <= m * longest list
     uint64_t cand_cap = std::max<uint64_t>(1, (uint64_t)m * std::max<uint64_t>(h->max_deg, 1));
     cand_cap = std::min<uint64_t>(cand_cap, n);
-    uint32_t slots = (uint32_t)std::min<uint64_t>(nq, (uint64_t)h->n_cu * 8);
+    uint32_t slots = (uint32_t)std::min<uint64_t>(nq, (uint64_t)h->n_cu * h->tune_search_wg_per_cu);
     // bound the private state to ~4 GB
     uint64_t per_slot = words * 4 + cand_cap * 8;
     uint64_t max_slots = std::max<uint64_t>(1, (4ull << 30) / per_slot);

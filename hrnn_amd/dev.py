@@ -29,7 +29,7 @@ EXPORTS = [
     "hrnn_dev_device_count", "hrnn_dev_open", "hrnn_dev_close", "hrnn_dev_last_error",
     "hrnn_dev_sync", "hrnn_dev_stream", "hrnn_dev_set_base", "hrnn_dev_set_base_dptr",
     "hrnn_dev_append_base", "hrnn_dev_base_is_small_int", "hrnn_dev_base_ptr",
-    "hrnn_dev_set_vector_precision",
+    "hrnn_dev_set_vector_precision", "hrnn_dev_set_tuning", "hrnn_dev_get_tuning",
     "hrnn_dev_set_knng", "hrnn_dev_get_knng", "hrnn_dev_knng_ptrs", "hrnn_dev_nndescent",
     "hrnn_dev_build_rknn_csr", "hrnn_dev_get_rknn_csr", "hrnn_dev_set_rknn_csr",
     "hrnn_dev_rknn_search", "hrnn_dev_rknn_search_dptr", "hrnn_dev_set_hnsw", "hrnn_dev_proxies",
@@ -76,6 +76,8 @@ def lib() -> C.CDLL:
         "hrnn_dev_append_base": [vp, vp, u64],
         "hrnn_dev_base_is_small_int": [vp, P(i)],
         "hrnn_dev_set_vector_precision": [vp, i],
+        "hrnn_dev_set_tuning": [vp, C.c_char_p, i],
+        "hrnn_dev_get_tuning": [vp, C.c_char_p, P(i)],
         "hrnn_dev_set_knng": [vp, u32, vp, vp, vp],
         "hrnn_dev_get_knng": [vp, vp, vp, vp],
         "hrnn_dev_knng_ptrs": [vp, vp, vp, vp],
@@ -217,6 +219,14 @@ class Device:
 
     def set_vector_precision(self, prec: str):
         self._ck(self.L.hrnn_dev_set_vector_precision(self.h, _PREC[prec]))
+
+    def set_tuning(self, key: str, value: int):
+        self._ck(self.L.hrnn_dev_set_tuning(self.h, key.encode(), int(value)))
+
+    def get_tuning(self, key: str) -> int:
+        v = C.c_int(0)
+        self._ck(self.L.hrnn_dev_get_tuning(self.h, key.encode(), C.byref(v)))
+        return v.value
 
     def set_knng(self, ids, dists, cnt):
         ids = np.ascontiguousarray(ids, np.uint32)

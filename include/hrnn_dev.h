@@ -79,6 +79,11 @@ const float *hrnn_dev_base_ptr(hrnn_dev_t h);
  * representable in it (then distances are bit-identical to FP32 and the
  * vector bytes halve) and FP32 otherwise; HRNN_PREC_F32 always reads FP32. */
 int hrnn_dev_set_vector_precision(hrnn_dev_t h, int prec);
+/* Launch shapes set by the autotuner (hrnn_amd/autotune.py): "search_wg_per_cu"
+ * (persistent RkNN search workgroups per CU, default 8) and "beam_wg_per_cu"
+ * (persistent phase-A waves per CU, default 16); values in [1, 64]. */
+int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value);
+int hrnn_dev_get_tuning(hrnn_dev_t h, const char *key, int *value);
 
 /* ---- ranked KNN graph --------------------------------------------------- */
 /* Host layout (SoA): ids[n*K], dists[n*K] (squared L2), cnt[n] (u16, <= K).

@@ -285,6 +285,31 @@ def test_nndescent_recall_parity(small_data, small_ref):
     assert st[0]["dists"] > 0 and st[0]["updates"] > 0
 
 
+def test_nndescent_graph_capture_matches_eager(small_data, small_ref, monkeypatch):
+    """The iterations captured as one HIP graph (default) and launched eagerly
+    (HRNN_NND_GRAPH=0) reach the same KNN recall; the graph is launched twice
+    in a row with host work in between."""
+    x, _ = small_data
+    r, seed = small_ref
+    sid, sd, sc = seed
+    exact, _ = oracle.knn(x, x, r.K + 1)
+    exact = exact[:, 1:].copy()
+    rec = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("HRNN_NND_GRAPH", mode)
+        dev = hrnn_amd.Device(0)
+        dev.set_base(x)
+        dev.set_knng(sid, sd, sc)
+        st1 = dev.nndescent(2, 12)
+        mid = dev.get_knng()
+        st2 = dev.nndescent(2, 12)
+        gid, gd, gc = dev.get_knng()
+        rec[mode] = _knn_recall(gid, gc, r.K, exact, r.K)
+        assert st1[0]["dists"] > 0 and st2[0]["dists"] > 0
+        assert _knn_recall(mid[0], mid[2], r.K, exact, r.K) <= rec[mode] + 1e-9
+    assert abs(rec["1"] - rec["0"]) <= 0.005, rec
+
+
 def test_nndescent_non_full_seed():
     """Lists that start below K (the K=500 > ef_construction=400 regime) fill to K."""
     x = mixture(1200, d=64, seed=31)
