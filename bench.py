@@ -11,15 +11,17 @@ Before timing: host HNSW build with KNN seeding; GPU refineKNNG (timed: the
 NNDescent build metric); GPU reverse CSR; exact groundtruth from the GPU
 brute force; an m sweep picking the cheapest m with macro recall >= 0.95; a
 learned-m model trained on separate queries.  A timed step is one whole
-search of all queries with queries resident in HBM and results left in HBM:
-phase A (HNSW descent + layer-0 beam) -> phases B and C, all on the GPU.
-Three paths are timed: FP32 vectors with fixed m, BF16 vectors (exact for
-these integer data) with fixed m, and BF16 with the learned per-query m.
+search of all queries, in batches of 1000: phase A (HNSW descent + layer-0
+beam) on the host cores, the proxies copied to HBM, phases B and C on the GPU
+with queries resident in HBM and results left there; the host computes the
+next batch's phase A while the GPU searches the previous one.  Three paths are
+timed: FP32 vectors with fixed m, BF16 vectors (exact for these integer data)
+with fixed m, and BF16 with the learned per-query m (host features, GPU MLP).
 value = the highest QPS among the paths whose macro recall is >= 0.95.
 
-roofline: the dominant kernel of that path (HIP events on the library's
-stream over resident launches, algorithmic bytes from the kernels' own work
-counters).  cpu_baseline: the reference's own searchRknn (compiled from its
+roofline: the GPU kernel of the path, k_rknn_search, timed with HIP events on
+the library's stream over launches on resident proxies (algorithmic bytes
+from the kernel's own work counters).  cpu_baseline: the reference's own searchRknn (compiled from its
 sources, oracle/_ref), single-threaded as its hrnn_search driver runs it, on a
 bounded query sample of this run's index (loaded from its saved file).
 """
@@ -128,8 +130,7 @@ def main() -> int:
 
     # ---- groundtruth with the GPU brute force; KNN-graph quality sample -----------------
     h = ix.device_handle()
-    dev = Device.__new__(Device)
-    dev.L, dev.h, dev.n, dev.d, dev.K = L, h, n, d, K
+    dev = Device.borrow(h, n, d, K)
     t = time.time()
     kth = dev.bf_kth(k, "bf16")
     kth_s = time.time() - t
@@ -146,30 +147,53 @@ def main() -> int:
 
     # ---- resident buffers and the timed path ---------------------------------------------
     stream = L.hrnn_dev_stream(h)
-    m_max = max(max(cfg["ms"]), int(GRID.max()))
+    m_max = int(max(max(cfg["ms"]), int(GRID.max())))
     cap = 4096
+    B = min(nq, 1000)                      # queries per pipelined batch
     qd = hiprt.DeviceArray.from_host(q)
     pd = hiprt.DeviceArray((nq * m_max,), np.uint32)
+    fd = hiprt.DeviceArray((nq * 32,), np.float32)
     cd = hiprt.DeviceArray((nq,), np.uint32)
     idd = hiprt.DeviceArray((nq * cap,), np.uint32)
     wd = hiprt.DeviceArray((2 * nq,), np.uint32)
-    ed = hiprt.DeviceArray((2 * nq,), np.uint32)
     md = hiprt.DeviceArray((nq,), np.uint32)
 
     def ck(rc):
         if rc:
             raise RuntimeError(L.hrnn_dev_last_error(h).decode())
 
+    def search_batch(b0, nb, m):
+        ck(L.hrnn_dev_rknn_search_dptr(h, qd.at(b0 * d), nb, pd.at(b0 * m), m, None, k, Kp, 1.0,
+                                       cd.at(b0), idd.at(b0 * cap), cap, wd.at(2 * b0)))
+
     def run_fixed(m):
-        ck(L.hrnn_dev_rknn_query_dptr(h, qd.ptr, nq, m, max(ef, m), None, k, Kp, 1.0, pd.ptr,
-                                      cd.ptr, idd.ptr, cap, wd.ptr))
+        """One step: host phase A per batch (OpenMP) -> async H2D of the proxies ->
+        GPU phases B+C; the host works on batch i+1 while the GPU runs batch i."""
+        keep = []
+        for b0 in range(0, nq, B):
+            nb = min(B, nq - b0)
+            prox = ix.proxies(q[b0:b0 + nb], m, ef, threads)
+            pd.copy_from_host_async(prox, stream, b0 * m)
+            keep.append(prox)
+            search_batch(b0, nb, m)
+        return keep
 
     model_d = grid_d = None
 
     def run_learned(tau):
-        ck(L.hrnn_dev_rknn_query_learned_dptr(h, qd.ptr, nq, model_d.ptr, grid_d.ptr, tau, m_max,
-                                              ef, k, Kp, 1.0, pd.ptr, md.ptr, cd.ptr, idd.ptr, cap,
-                                              wd.ptr))
+        """Host phase A with beam statistics + the 32 features -> async H2D ->
+        GPU MLP forward, per-query m, masked proxies, phases B+C."""
+        keep = []
+        for b0 in range(0, nq, B):
+            nb = min(B, nq - b0)
+            f_b, p_b = ix.query_features(q[b0:b0 + nb], k, ef, m_max, threads)
+            fd.copy_from_host_async(f_b, stream, b0 * 32)
+            pd.copy_from_host_async(p_b, stream, b0 * m_max)
+            keep.append((f_b, p_b))
+            ck(L.hrnn_dev_rknn_search_learned_dptr(h, qd.at(b0 * d), nb, fd.at(b0 * 32), model_d.ptr,
+                                                   grid_d.ptr, tau, pd.at(b0 * m_max), m_max, md.at(b0), k,
+                                                   Kp, 1.0, cd.at(b0), idd.at(b0 * cap), cap, wd.at(2 * b0)))
+        return keep
 
     def results():
         hiprt.stream_sync(stream)
@@ -192,10 +216,15 @@ def main() -> int:
         hiprt.stream_sync(stream)
         return (time.perf_counter() - t0) / args.steps
 
-    # ---- m sweep (BF16 vectors; identical results to FP32 on these data) -----------------
+    # ---- m sweep: the first m of m_max beam proxies (same beam, ef >= m) ------------------
+    t = time.time()
+    prox_all = ix.proxies(q, m_max, ef, threads)
+    phase_a_s = time.time() - t
+    log(f"host phase A for {nq} queries (m={m_max}, ef={ef}, {threads} threads): {phase_a_s:.2f}s")
     sweep, m_star = [], None
     for m in cfg["ms"]:
-        run_fixed(m)
+        pd.copy_from_host_async(np.ascontiguousarray(prox_all[:, :m]), stream, 0)
+        search_batch(0, nq, m)
         r_off, r_ids = results()
         rec = macro_recall(gt_off, gt_ids, r_off, r_ids)
         sweep.append({"m": m, "recall": round(rec, 5), "avg_results": round(len(r_ids) / nq, 2)})
@@ -206,12 +235,12 @@ def main() -> int:
         m_star = cfg["ms"][-1]
     rec_star = next(s["recall"] for s in sweep if s["m"] == m_star)
 
-    # ---- launch shapes of the persistent kernels from the autotuner ------------------------
+    # ---- launch shape of the persistent search kernel from the autotuner ------------------
     from hrnn_amd import autotune
-    tuned = autotune.tune(h, qd, nq, m_star, ef, k, Kp,
-                          dict(n=n, d=d, K=K, m=m_star, ef=ef, nq=nq), pd, cd, idd, cap,
+    pd.copy_from_host_async(np.ascontiguousarray(prox_all[:, :m_star]), stream, 0)
+    tuned = autotune.tune(h, qd, nq, m_star, k, Kp, dict(n=n, d=d, K=K, m=m_star, nq=nq), pd, cd, idd, cap,
                           use_cache=False)
-    log(f"autotune: beam_wg_per_cu={tuned['beam_wg_per_cu']} search_wg_per_cu={tuned['search_wg_per_cu']}")
+    log(f"autotune: search_wg_per_cu={tuned['search_wg_per_cu']}  {tuned['timings_ms']}")
 
     paths = {}
     ck(L.hrnn_dev_set_vector_precision(h, 0))          # FP32 vectors
@@ -229,7 +258,7 @@ def main() -> int:
     if not args.no_learned:
         t = time.time()
         qt = gen(cfg["train_seed"], cfg["ntrain"])
-        ft, pt = dev.query_features(qt, k, ef, int(GRID.max()))
+        ft, pt = ix.query_features(qt, k, ef, m_max, threads)
         gt_t_off, gt_t_ids, _ = dev.bf_rknn(qt, kth, "bf16")
         lab, rec_grid = label_queries(dev, qt, pt, gt_t_off, gt_t_ids, k, Kp)
         model, losses = LearnedM.train(dev, ft, lab, epochs=60, batch=256, lr=2e-3)
@@ -265,44 +294,38 @@ def main() -> int:
     best_name = max(ok or paths, key=lambda kk: (ok or paths)[kk]["qps"])
     best = paths[best_name]
 
-    # ---- roofline of the dominant kernel of the best path (HIP events) ---------------------
-    m_b = best.get("m", m_max)
+    # ---- roofline of the GPU kernel of the best path (HIP events, resident inputs) ----------
+    # Phase A runs on the host; the GPU kernel of every path is k_rknn_search.
     reps = 20
     e0, e1 = hiprt.Event(), hiprt.Event()
-    # phase A alone
-    prox_m = m_max if best_name == "bf16_learned_m" else m_b
-    e0.record(stream)
-    for _ in range(reps):
-        ck(L.hrnn_dev_proxies_dptr(h, qd.ptr, nq, prox_m, max(ef, prox_m), pd.ptr, ed.ptr))
-    e1.record(stream)
-    e1.sync()
-    beam_ms = e1.ms_since(e0) / reps
-    ex = ed.to_host().reshape(nq, 2).astype(np.int64)
-    s_v = 2 if best_name != "fp32_fixed_m" else 4
-    words0 = 2 * cfg["M"] + 1
-    beam_bytes = float((4 * d + 4 * prox_m + ex[:, 0] * 4 * words0 + ex[:, 1] * s_v * d).sum())
-    # phases B + C alone on the same proxies (learned path: masked proxies)
     if best_name == "bf16_learned_m":
-        run_learned(best["tau"])
+        run_learned(best["tau"])            # leaves the masked proxies in pd (stride m_max)
         hiprt.stream_sync(stream)
+        prox_m = m_max
+    else:
+        prox_m = best["m"]
+        pd.copy_from_host_async(np.ascontiguousarray(prox_all[:, :prox_m]), stream, 0)
+    ck(L.hrnn_dev_set_vector_precision(h, 0 if best_name == "fp32_fixed_m" else 1))
+    search_batch(0, nq, prox_m)
+    hiprt.stream_sync(stream)
     e0.record(stream)
     for _ in range(reps):
-        ck(L.hrnn_dev_rknn_search_dptr(h, qd.ptr, nq, pd.ptr, prox_m, None, k, Kp, 1.0, cd.ptr, idd.ptr,
-                                       cap, wd.ptr))
+        search_batch(0, nq, prox_m)
     e1.record(stream)
     e1.sync()
     search_ms = e1.ms_since(e0) / reps
     wk = wd.to_host().reshape(nq, 2).astype(np.int64)
     rc_ = cd.to_host().astype(np.int64)
-    search_bytes = float((4 * d + 16 * prox_m + 4 * wk[:, 0] + wk[:, 1] * (s_v * d + 4 + 4) + 8 * rc_).sum())
-    kern = {
-        "k_beam": {"ms": beam_ms, "bytes": beam_bytes},
-        "k_rknn_search": {"ms": search_ms, "bytes": search_bytes},
-    }
-    dom = max(kern, key=lambda kk: kern[kk]["ms"])
-    achieved = kern[dom]["bytes"] / (kern[dom]["ms"] * 1e-3) / 1e9
-    for kk, v in kern.items():
-        log(f"  {kk}: {v['ms']:.3f} ms/launch, {v['bytes'] / (v['ms'] * 1e-3) / 1e9:.1f} GB/s algorithmic")
+    s_v = 4 if best_name == "fp32_fixed_m" else 2
+    # per query: the query vector, 16 B per proxy (CSR offsets), 4 B per scanned
+    # reverse-list entry, and per distinct candidate its vector, r_k and mark word,
+    # plus 4 B per result id
+    search_bytes = float((4 * d + 16 * prox_m + 4 * wk[:, 0] + wk[:, 1] * (s_v * d + 4 + 4) + 4 * rc_).sum())
+    achieved = search_bytes / (search_ms * 1e-3) / 1e9
+    log(f"  k_rknn_search: {search_ms:.3f} ms/launch over {nq} queries, "
+        f"{achieved:.1f} GB/s algorithmic; scanned/query {wk[:, 0].mean():.0f}, "
+        f"distinct candidates/query {wk[:, 1].mean():.0f}")
+    gpu_only_qps = nq / (search_ms * 1e-3)
 
     # ---- cpu baseline: the reference's searchRknn, one thread, same index --------------
     cpu = None
@@ -352,12 +375,11 @@ def main() -> int:
                    "nn_sample": cfg["S"], "k": k, "K_prime": Kp, "ef_search": ef, "path": best_name,
                    "recall": best["recall"], "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
-                     "kernel_ms": round(kern[dom]["ms"], 4),
-                     "alg_bytes_per_launch": int(kern[dom]["bytes"]),
-                     "other_kernels": {kk: {"ms": round(v["ms"], 4),
-                                            "gbs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 2)}
-                                       for kk, v in kern.items() if kk != dom}},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_rknn_search",
+                     "kernel_ms": round(search_ms, 4), "alg_bytes_per_launch": int(search_bytes),
+                     "queries_per_launch": nq, "gpu_phases_bc_qps": round(gpu_only_qps, 1)},
+        "phase_a_host": {"threads": threads, "sec_all_queries_m_max": round(phase_a_s, 3),
+                         "qps": round(nq / phase_a_s, 1)},
         "cpu_baseline": cpu,
         "paths": paths,
         "learned_m": learned_info,

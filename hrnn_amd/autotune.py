@@ -1,11 +1,11 @@
 """Autotuner for the launch shapes of the persistent search kernels.
 
-For a resident index and query batch it times each candidate launch shape of
-the phase-A beam kernel ("beam_wg_per_cu") and of the phase-B/C search
-kernel ("search_wg_per_cu") with HIP events on the library's stream, keeps
-the fastest, and caches the choice as JSON keyed by the GPU and the
-workload shape (n, d, K, m, ef, nq).  Results are identical for every
-shape; only the time changes.
+For a resident index, query batch and proxy set it times each candidate
+launch shape of the persistent phase-B/C search kernel ("search_wg_per_cu":
+workgroups per CU) with HIP events on the library's stream, keeps the
+fastest, and caches the choice as JSON keyed by the GPU and the workload
+shape (n, d, K, m, nq).  Results are identical for every shape; only the
+time changes.
 """
 from __future__ import annotations
 
@@ -19,8 +19,7 @@ from . import hiprt
 from .dev import lib
 
 CANDIDATES = {
-    "beam_wg_per_cu": [4, 8, 16, 24, 32],
-    "search_wg_per_cu": [2, 4, 8, 16],
+    "search_wg_per_cu": [1, 2, 4, 8, 16, 32],
 }
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "cache.json")
 
@@ -56,10 +55,11 @@ def save_cache(c: dict) -> None:
     os.replace(tmp, CACHE)
 
 
-def tune(h, qd, nq: int, m: int, ef: int, k: int, K_prime: int, shape: dict, pd, cd, idd, cap,
+def tune(h, qd, nq: int, m: int, k: int, K_prime: int, shape: dict, pd, cd, idd, cap,
          reps: int = 5, use_cache: bool = True) -> dict:
-    """Tune on the device handle h with resident queries qd and scratch pd/cd/idd.
-    Returns {"beam_wg_per_cu": v, "search_wg_per_cu": v, "timings": {...}}."""
+    """Tune on the device handle h with resident queries qd, resident proxies
+    pd (nq x m) and result buffers cd/idd.
+    Returns {"search_wg_per_cu": v, "timings_ms": {...}}."""
     L = lib()
     key = _key(shape)
     cache = load_cache() if use_cache else {}
@@ -85,16 +85,13 @@ def tune(h, qd, nq: int, m: int, ef: int, k: int, K_prime: int, shape: dict, pd,
         e1.sync()
         return e1.ms_since(e0) / reps
 
-    beam = lambda: ck(L.hrnn_dev_proxies_dptr(h, qd.ptr, nq, m, max(ef, m), pd.ptr, None))
     search = lambda: ck(L.hrnn_dev_rknn_search_dptr(h, qd.ptr, nq, pd.ptr, m, None, k, K_prime, 1.0,
                                                     cd.ptr, idd.ptr, cap, None))
     out, timings = {}, {}
-    for name, fn in (("beam_wg_per_cu", beam), ("search_wg_per_cu", search)):
+    for name, fn in (("search_wg_per_cu", search),):
         best, best_ms = None, float("inf")
         for v in CANDIDATES[name]:
             ck(L.hrnn_dev_set_tuning(h, name.encode(), v))
-            if name == "search_wg_per_cu":
-                beam()          # proxies for the search
             ms = time_it(fn)
             timings[f"{name}={v}"] = round(ms, 4)
             if ms < best_ms:

@@ -174,6 +174,22 @@ int hrnn_index_proxies(hrnn_index_t ix, const float *q, uint64_t nq, uint32_t m,
     });
 }
 
+int hrnn_index_query_features(hrnn_index_t ix, const float *q, uint64_t nq, uint32_t k, uint32_t ef,
+                              uint32_t m_max, int nthreads, float *feats, uint32_t *proxies, float *raw) {
+    if (!ix || !q || !feats || !proxies) return HRNN_E_ARG;
+    return guard(ix, [&] {
+        if (m_max < 16) throw std::runtime_error("query_features: m_max must be >= 16");
+        std::vector<float> r;
+        float *rp = raw;
+        if (!rp) {
+            r.resize(nq * 20);
+            rp = r.data();
+        }
+        ix->idx->proxiesWithStats(q, nq, m_max, ef, nthreads, proxies, rp);
+        ix->idx->queryFeatures(rp, proxies, nq, m_max, k, feats);
+    });
+}
+
 int hrnn_index_proxies_gpu(hrnn_index_t ix, const float *q, uint64_t nq, uint32_t m, uint32_t ef,
                            uint32_t *out) {
     if (!ix || !q || !out) return HRNN_E_ARG;
@@ -273,6 +289,27 @@ int hrnn_index_get_labels(hrnn_index_t ix, uint64_t *labels) {
     return guard(ix, [&] {
         size_t n = ix->idx->getCurrentElementCount();
         for (size_t i = 0; i < n; i++) labels[i] = ix->idx->getExternalLabel((hnswlib::tableint)i);
+    });
+}
+
+int hrnn_index_export_hnsw(hrnn_index_t ix, uint32_t *l0, int *levels, uint32_t *up_off,
+                           uint32_t *up_pool, uint64_t *pool_words, uint32_t *entry, int *maxlevel) {
+    if (!ix || !pool_words) return HRNN_E_ARG;
+    return guard(ix, [&] {
+        std::vector<uint32_t> off, pool;
+        std::vector<int> lv;
+        ix->idx->exportUpper(off, pool, lv);
+        *pool_words = pool.size();
+        if (entry) *entry = ix->idx->entryPoint();
+        if (maxlevel) *maxlevel = ix->idx->maxLevel();
+        if (!up_pool) return;
+        size_t n = ix->idx->getCurrentElementCount();
+        const size_t w0 = ix->idx->maxM0_ + 1;
+        if (l0)
+            for (size_t i = 0; i < n; i++) std::memcpy(l0 + i * w0, ix->idx->level0Links((hnswlib::tableint)i), w0 * 4);
+        if (levels) std::copy(lv.begin(), lv.end(), levels);
+        if (up_off) std::copy(off.begin(), off.end(), up_off);
+        std::copy(pool.begin(), pool.end(), up_pool);
     });
 }
 

@@ -176,6 +176,15 @@ __global__ void k_features(const float *__restrict__ raw, const uint32_t *__rest
     }
 }
 
+// In-place feature standardisation f = (f - mu) / sd over n rows of kIn.
+__global__ void k_standardize(float *__restrict__ f, uint32_t n, const float *__restrict__ mu,
+                              const float *__restrict__ sd) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)n * kIn) return;
+    uint32_t j = (uint32_t)(i % kIn);
+    f[i] = (f[i] - mu[j]) / sd[j];
+}
+
 // Per-query m: the smallest grid class whose cumulative probability reaches
 // tau (argmax when tau <= 0); proxies beyond that m are masked out.
 __global__ void k_pick_m(const float *__restrict__ probs, uint32_t nq, const uint32_t *__restrict__ grid,
@@ -262,6 +271,33 @@ int hrnn_dev_rknn_query_learned_dptr(hrnn_dev_t h, const float *q_dev, uint32_t 
                            m_max, h->knn_d.as<float>(), h->knn_cnt.as<uint32_t>(), h->K, nullptr, k, nq,
                            model_dev + kParams, model_dev + kParams + kIn, feats);
         hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(nq), dim3(64), 0, h->stream, model_dev, feats, nullptr,
+                           nullptr, nq, probs, nullptr, nullptr);
+        hipLaunchKernelGGL(k_pick_m, dim3(ceil_div(nq, 128)), dim3(128), 0, h->stream, probs, nq, grid_dev,
+                           tau, proxies_dev, m_max, m_dev);
+        HIPC(hipGetLastError());
+        rknn_search_resident(h, q_dev, nq, proxies_dev, m_max, nullptr, k, K_prime, threshold, res_cnt_dev,
+                             res_ids_dev, cap_per_q, work_dev);
+    });
+}
+
+int hrnn_dev_rknn_search_learned_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, float *feats_dev,
+                                      const float *model_dev, const uint32_t *grid_dev, float tau,
+                                      uint32_t *proxies_dev, uint32_t m_max, uint32_t *m_dev, uint32_t k,
+                                      uint32_t K_prime, float threshold, uint32_t *res_cnt_dev,
+                                      uint32_t *res_ids_dev, uint32_t cap_per_q, uint32_t *work_dev) {
+    if (!h) return HRNN_E_ARG;
+    return guarded(h, [&] {
+        REQUIRE(h->csr_valid && h->K > 0, HRNN_E_STATE, "search_learned: build the CSR first");
+        REQUIRE(k >= 1 && k <= h->K, HRNN_E_ARG, "search_learned: k out of [1, K]");
+        REQUIRE(m_max >= 1 && m_max <= 1024, HRNN_E_ARG, "search_learned: m_max must be in [1, 1024]");
+        REQUIRE(q_dev && feats_dev && model_dev && grid_dev && proxies_dev && res_cnt_dev && res_ids_dev,
+                HRNN_E_ARG, "search_learned: null argument");
+        if (!nq) return;
+        h->s8.reserve((uint64_t)nq * kOut * 4);
+        float *probs = h->s8.as<float>();
+        hipLaunchKernelGGL(k_standardize, dim3(ceil_div((uint64_t)nq * kIn, 256)), dim3(256), 0, h->stream,
+                           feats_dev, nq, model_dev + kParams, model_dev + kParams + kIn);
+        hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(nq), dim3(64), 0, h->stream, model_dev, feats_dev, nullptr,
                            nullptr, nq, probs, nullptr, nullptr);
         hipLaunchKernelGGL(k_pick_m, dim3(ceil_div(nq, 128)), dim3(128), 0, h->stream, probs, nq, grid_dev,
                            tau, proxies_dev, m_max, m_dev);

@@ -35,7 +35,8 @@ EXPORTS = [
     "hrnn_dev_rknn_search", "hrnn_dev_rknn_search_dptr", "hrnn_dev_set_hnsw", "hrnn_dev_proxies",
     "hrnn_dev_proxies_dptr", "hrnn_dev_rknn_query_dptr", "hrnn_dev_bf_kth", "hrnn_dev_bf_knn",
     "hrnn_dev_bf_rknn", "hrnn_dev_l2sqr_pairs", "hrnn_dev_query_features", "hrnn_dev_mlp_forward",
-    "hrnn_dev_mlp_grad", "hrnn_dev_mlp_train", "hrnn_dev_rknn_query_learned_dptr", "hrnn_synth_mixture",
+    "hrnn_dev_mlp_grad", "hrnn_dev_mlp_train", "hrnn_dev_rknn_query_learned_dptr", "hrnn_dev_rknn_search_learned_dptr",
+    "hrnn_synth_mixture",
     "hrnn_synth_halfnormal_scaled", "hrnn_synth_sphere_mixture",
 ]
 
@@ -101,6 +102,8 @@ def lib() -> C.CDLL:
         "hrnn_dev_mlp_train": [vp, vp, vp, vp, u32, i, u32, f, f, u64, vp],
         "hrnn_dev_rknn_query_learned_dptr": [vp, vp, u32, vp, vp, f, u32, u32, u32, u32, f, vp, vp,
                                              vp, vp, u32, vp],
+        "hrnn_dev_rknn_search_learned_dptr": [vp, vp, u32, vp, vp, vp, f, vp, u32, vp, u32, u32, f, vp,
+                                              vp, u32, vp],
         "hrnn_synth_mixture": [u64, u64, u32, u32, f, f, f, i, C.c_double, u64, vp, vp],
         "hrnn_synth_halfnormal_scaled": [u64, u64, u64, u32, vp],
         "hrnn_synth_sphere_mixture": [u64, u64, u64, u32, u32, f, vp, vp],
@@ -182,7 +185,19 @@ class Device:
         self.d = 0
         self.K = 0
 
+    @classmethod
+    def borrow(cls, h, n: int, d: int, K: int = 0) -> "Device":
+        """A view of a handle owned elsewhere (e.g. Index.device_handle()):
+        never closes it; the owner must outlive the view."""
+        self = cls.__new__(cls)
+        self.L, self.h, self.n, self.d, self.K = lib(), h, n, d, K
+        self._borrowed = True
+        return self
+
     def close(self):
+        if getattr(self, "_borrowed", False):
+            self.h = None
+            return
         if getattr(self, "h", None):
             self.L.hrnn_dev_close(self.h)
             self.h = None

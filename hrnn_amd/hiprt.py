@@ -60,6 +60,18 @@ class DeviceArray:
         _ck(hip().hipMemcpy(d.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, H2D), "hipMemcpy H2D")
         return d
 
+    def at(self, elem: int) -> C.c_void_p:
+        """Device address of element `elem` (flat index)."""
+        return C.c_void_p(self.ptr.value + int(elem) * self.dtype.itemsize)
+
+    def copy_from_host_async(self, a: np.ndarray, stream, elem: int = 0) -> None:
+        """Queue a copy of host array a to element `elem` on `stream`."""
+        a = np.ascontiguousarray(a, self.dtype)
+        if (int(elem) + a.size) * self.dtype.itemsize > self.nbytes:
+            raise ValueError("copy_from_host_async: out of range")
+        _ck(hip().hipMemcpyAsync(self.at(elem), a.ctypes.data_as(C.c_void_p), a.nbytes, H2D, stream),
+            "hipMemcpyAsync H2D")
+
     def to_host(self) -> np.ndarray:
         out = np.empty(self.shape, self.dtype)
         _ck(hip().hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes, D2H), "hipMemcpy D2H")

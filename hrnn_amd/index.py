@@ -50,6 +50,8 @@ def host_lib() -> C.CDLL:
         "hrnn_index_device": [vp, P(vp)],
         "hrnn_index_get_level0": [vp, vp, u32, vp],
         "hrnn_index_get_labels": [vp, vp],
+        "hrnn_index_export_hnsw": [vp, vp, vp, vp, vp, P(u64), P(u32), P(i)],
+        "hrnn_index_query_features": [vp, vp, u64, u32, u32, u32, i, vp, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -162,6 +164,18 @@ class Index:
         self._ck(self.L.hrnn_index_proxies(self.h, _p(q), q.shape[0], m, ef, nthreads, _p(out)))
         return out
 
+    def query_features(self, q, k, ef, m_max=16, nthreads=0, with_raw=False):
+        """Host phase A with beam statistics and the 32 learned-m features:
+        returns (feats[nq, 32], proxies[nq, m_max]) (+ raw[nq, 20])."""
+        q = np.ascontiguousarray(q, np.float32)
+        nq = q.shape[0]
+        f = np.empty((nq, 32), np.float32)
+        p = np.empty((nq, m_max), np.uint32)
+        raw = np.empty((nq, 20), np.float32)
+        self._ck(self.L.hrnn_index_query_features(self.h, _p(q), nq, k, ef, m_max, nthreads, _p(f), _p(p),
+                                                  _p(raw)))
+        return (f, p, raw) if with_raw else (f, p)
+
     def proxies_gpu(self, q, m, ef):
         q = np.ascontiguousarray(q, np.float32)
         out = np.empty((q.shape[0], m), np.uint32)
@@ -233,3 +247,19 @@ class Index:
         levels = np.empty(n, np.int32)
         self._ck(self.L.hrnn_index_get_level0(self.h, _p(links), words, _p(levels)))
         return links, levels
+
+    def export_hnsw(self):
+        """The HNSW graph in the layout of Device.set_hnsw: a dict with maxM0,
+        maxM, l0, levels, up_off, up_pool, entry, maxlevel."""
+        pw, ent, ml = C.c_uint64(0), C.c_uint32(0), C.c_int(0)
+        self._ck(self.L.hrnn_index_export_hnsw(self.h, None, None, None, None, C.byref(pw), C.byref(ent),
+                                               C.byref(ml)))
+        n = self.count
+        l0 = np.empty((n, 2 * self.M + 1), np.uint32)
+        levels = np.empty(n, np.int32)
+        up_off = np.empty(n, np.uint32)
+        up_pool = np.empty(max(int(pw.value), 1), np.uint32)
+        self._ck(self.L.hrnn_index_export_hnsw(self.h, _p(l0), _p(levels), _p(up_off), _p(up_pool),
+                                               C.byref(pw), C.byref(ent), C.byref(ml)))
+        return dict(maxM0=2 * self.M, maxM=self.M, l0=l0, levels=levels, up_off=up_off,
+                    up_pool=up_pool[: int(pw.value)], entry=int(ent.value), maxlevel=int(ml.value))

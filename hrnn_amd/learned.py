@@ -6,14 +6,15 @@ predicts per query the smallest m of a 16-value grid that reaches full
 recall; plain fixed-m search stays the fallback.
 
 Pipeline:
-  1. features: hrnn_dev_query_features (phase-A beam statistics + r_k of the
-     closest proxies), 32 floats per query;
-  2. labels: for every grid m, phases B+C with the first m proxies, per-query
-     recall against the exact groundtruth (GPU brute force); label = the
-     smallest grid index whose recall reaches `full` (1.0 by default), else 15;
+  1. features: hrnn_index_query_features (host phase-A beam statistics + r_k
+     of the closest proxies), 32 floats per query;
+  2. labels: for every grid m, phases B+C on the GPU with the first m
+     proxies, per-query recall against the exact groundtruth (GPU brute
+     force); label = the smallest grid index whose recall reaches `full`
+     (1.0 by default), else 15;
   3. training: Adam on mean cross-entropy (hrnn_dev_mlp_train);
-  4. inference: hrnn_dev_rknn_query_learned_dptr, or per-query m from
-     `choose_m` and masked proxies.
+  4. inference: MLP forward on the GPU (hrnn_dev_mlp_forward), per-query m
+     from `choose_m`, proxies beyond m masked, phases B+C on the GPU.
 """
 from __future__ import annotations
 

@@ -287,6 +287,91 @@ class HRNN : public AlgorithmInterface<dist_t> {
         return prox;
     }
 
+    // Phase A with the per-query beam statistics the learned m uses
+    // (raw[nq x 20], same layout as the device library's beam statistics):
+    //   0 d(q, entry)   1 d(q, layer-0 start)   2..17 the 16 closest beam
+    //   distances (the last one repeated when the beam holds fewer)
+    //   18 expansions   19 beam size
+    void proxiesWithStats(const float *Q, size_t nq, size_t m, size_t ef_search, int threads,
+                          uint32_t *prox, float *raw) const {
+        std::fill(prox, prox + nq * m, INVALID_KNNG_ID);
+        if (cur_element_count == 0) {
+            std::fill(raw, raw + nq * 20, 0.f);
+            return;
+        }
+        size_t ef = std::max(ef_search, m);
+#pragma omp parallel for schedule(dynamic, 8) num_threads(threads > 0 ? threads : omp_threads_default()) if (nq > 1)
+        for (long long qi = 0; qi < (long long)nq; qi++) {
+            const float *q = Q + (size_t)qi * dim_;
+            float *r = raw + (size_t)qi * 20;
+            tableint ep = descend(q);
+            unsigned expanded = 0;
+            auto W = beam_query(ep, q, ef, &expanded);
+            const size_t beam = W.size();
+            while (W.size() > m) W.pop();       // the m proxies, as proxies() keeps them
+            std::vector<std::pair<dist_t, tableint>> v;
+            v.reserve(W.size());
+            while (!W.empty()) {
+                v.push_back(W.top());
+                W.pop();
+            }
+            std::reverse(v.begin(), v.end());   // closest first
+            for (size_t j = 0; j < v.size(); j++) prox[(size_t)qi * m + j] = v[j].second;
+            r[0] = dist_to(q, enterpoint_node_);
+            r[1] = dist_to(q, ep);
+            for (size_t j = 0; j < 16; j++) r[2 + j] = v[std::min(j, v.size() - 1)].first;
+            r[18] = (float)expanded;
+            r[19] = (float)beam;
+        }
+    }
+
+    // The 32 learned-m features of each query from its beam statistics (raw,
+    // as proxiesWithStats) and the ranked KNN lists of its 16 closest proxies
+    // (prox rows of mstride >= 16 ids, closest first):
+    //   0 log1p d(q,entry)   1 log1p d(q,start)   2..17 log1p beam distances
+    //   18..25 log1p r_k of proxies 1..8 (0 for a missing proxy)
+    //   26..28 mean / min / max of log1p r_k over proxies 1..16
+    //   29 fraction of proxies 1..16 with d(q,p) <= r_k(p)
+    //   30 log1p d1 - log1p r_k(p1)   31 log1p(expansions)
+    // r_k(p) is the k-th list distance, 3e38 when p's list is shorter.
+    void queryFeatures(const float *raw, const uint32_t *prox, size_t nq, size_t mstride, size_t k,
+                       float *feats) const {
+        if (mstride < 16) throw std::runtime_error("queryFeatures: need at least 16 proxies per query");
+        if (k == 0 || k > K_knng_) throw std::runtime_error("queryFeatures: k must be in [1, K]");
+        const size_t n = cur_element_count;
+        for (size_t qi = 0; qi < nq; qi++) {
+            const float *r = raw + qi * 20;
+            float *f = feats + qi * 32;
+            f[0] = std::log1p(r[0]);
+            f[1] = std::log1p(r[1]);
+            for (int j = 0; j < 16; j++) f[2 + j] = std::log1p(std::min(r[2 + j], 3.0e38f));
+            float sum = 0.f, mn = FLT_MAX, mx = -FLT_MAX, inside = 0.f, rk1 = 0.f;
+            int cnt = 0;
+            for (int j = 0; j < 16; j++) {
+                uint32_t p = prox[qi * mstride + j];
+                float rk = 0.f;
+                if (p != INVALID_KNNG_ID) {
+                    if (p >= n) throw std::runtime_error("queryFeatures: proxy id >= n");
+                    float v = (k - 1 < knn_cnt_[p]) ? knn_d_[(size_t)p * K_knng_ + k - 1] : 3.0e38f;
+                    rk = std::log1p(v);
+                    inside += (r[2 + j] <= v) ? 1.f : 0.f;
+                    cnt++;
+                    sum += rk;
+                    mn = std::min(mn, rk);
+                    mx = std::max(mx, rk);
+                }
+                if (j == 0) rk1 = rk;
+                if (j < 8) f[18 + j] = rk;
+            }
+            f[26] = cnt ? sum / cnt : 0.f;
+            f[27] = cnt ? mn : 0.f;
+            f[28] = cnt ? mx : 0.f;
+            f[29] = cnt ? inside / cnt : 0.f;
+            f[30] = f[2] - rk1;
+            f[31] = std::log1p(r[18]);
+        }
+    }
+
     // Phase A on the GPU (hrnn_dev_proxies): same ordering rules as proxies().
     std::vector<uint32_t> proxiesDevice(const float *Q, size_t nq, size_t m, size_t ef_search) const {
         std::vector<uint32_t> prox(nq * m, INVALID_KNNG_ID);
@@ -673,11 +758,9 @@ class HRNN : public AlgorithmInterface<dist_t> {
         if (rknng_built_) sync_device_csr();
         if (cur_element_count > 0 && M_ * 2 <= 64) sync_device_hnsw();
     }
-    // Phase A runs on the GPU unless HRNN_PHASE_A=host (or maxM0 > 64).
-    bool phase_a_on_gpu() const {
-        const char *e = std::getenv("HRNN_PHASE_A");
-        return !(e && std::string(e) == "host") && maxM0_ <= 64;
-    }
+    // Phase A runs on the host: the GPU beam kernel is switched off in the
+    // device library (its entry points return HRNN_E_STATE).
+    bool phase_a_on_gpu() const { return false; }
 
  private:
     // ---- space ------------------------------------------------------------------
@@ -803,6 +886,15 @@ class HRNN : public AlgorithmInterface<dist_t> {
     typedef std::pair<dist_t, tableint> DP;
     typedef std::priority_queue<DP> MaxHeap;
     typedef std::priority_queue<DP, std::vector<DP>, std::greater<DP>> MinHeap;
+    // Heap ordered by the distance alone: equal distances leave in the order
+    // the standard library's heap operations give them.  The query beam uses
+    // it for both of its queues, with the same sequence of pushes and pops as
+    // the reference's searchBaseLayerST (hrnn.h:485-616, CompareByFirst at
+    // hrnn.h:218-223), so distance ties resolve the same way.
+    struct ByDist {
+        bool operator()(const DP &a, const DP &b) const noexcept { return a.first < b.first; }
+    };
+    typedef std::priority_queue<DP, std::vector<DP>, ByDist> DistHeap;
 
     int random_level() {
         std::lock_guard<std::mutex> g(rng_mu_);
@@ -878,20 +970,22 @@ class HRNN : public AlgorithmInterface<dist_t> {
     // Query beam at layer 0 with the bare stop rule of the reference's
     // searchBaseLayerST<true> (hrnn.h:485-616): stop when the closest open
     // candidate is farther than the current worst result.
-    MaxHeap beam_query(tableint ep, const float *q, size_t ef) const {
+    // W holds the results (largest distance on top); C the open candidates
+    // keyed by negated distance (closest on top).
+    DistHeap beam_query(tableint ep, const float *q, size_t ef, unsigned *expanded = nullptr) const {
         thread_local Visited vis;
         uint32_t tag = next_epoch(vis);
-        MaxHeap W;
-        MinHeap C;
+        DistHeap W, C;
         dist_t d0 = dist_to(q, ep);
         W.emplace(d0, ep);
-        C.emplace(d0, ep);
+        C.emplace(-d0, ep);
         vis.tag[ep] = tag;
         dist_t lb = d0;
         while (!C.empty()) {
             DP c = C.top();
-            if (c.first > lb) break;
+            if (-c.first > lb) break;
             C.pop();
+            if (expanded) ++*expanded;
             const uint32_t *ll = links0(c.second);   // layer 0 is immutable at query time
             unsigned cnt = count_of(ll);
             for (unsigned t = 1; t <= cnt; t++) {
@@ -899,8 +993,8 @@ class HRNN : public AlgorithmInterface<dist_t> {
                 if (vis.tag[e] == tag) continue;
                 vis.tag[e] = tag;
                 dist_t de = dist_to(q, e);
-                if (W.size() < ef || de < lb) {
-                    C.emplace(de, e);
+                if (W.size() < ef || lb > de) {
+                    C.emplace(-de, e);
                     W.emplace(de, e);
                     if (W.size() > ef) W.pop();
                     lb = W.top().first;
@@ -1135,9 +1229,23 @@ class HRNN : public AlgorithmInterface<dist_t> {
     void sync_device_hnsw() const {
         size_t n = cur_element_count;
         if (dev_hnsw_rows_ == n) return;   // every insertion changes n
+        std::vector<uint32_t> off, pool;
+        std::vector<int> lv;
+        exportUpper(off, pool, lv);
+        dev_check(hrnn_dev_set_hnsw(dev_, (uint32_t)maxM0_, (uint32_t)maxM_, level0_.data(), lv.data(),
+                                    off.data(), pool.data(), pool.size(), enterpoint_node_, maxlevel_));
+        dev_hnsw_rows_ = n;
+    }
+
+ public:
+    // Upper layers packed for the device: node i of level L >= 1 owns L
+    // blocks of maxM + 1 words at pool[off[i]] (off = 0xFFFFFFFF at level 0).
+    void exportUpper(std::vector<uint32_t> &off, std::vector<uint32_t> &pool, std::vector<int> &lv) const {
+        size_t n = cur_element_count;
         const size_t wu = maxM_ + 1;
-        std::vector<uint32_t> off(n, INVALID_KNNG_ID), pool;
-        std::vector<int> lv(levels_.begin(), levels_.begin() + n);
+        off.assign(n, INVALID_KNNG_ID);
+        pool.clear();
+        lv.assign(levels_.begin(), levels_.begin() + n);
         size_t total = 0;
         for (size_t i = 0; i < n; i++) total += (size_t)std::max(0, levels_[i]) * wu;
         pool.reserve(total);
@@ -1147,10 +1255,11 @@ class HRNN : public AlgorithmInterface<dist_t> {
             const uint32_t *b = reinterpret_cast<const uint32_t *>(upper_[i]);
             pool.insert(pool.end(), b, b + (size_t)levels_[i] * wu);
         }
-        dev_check(hrnn_dev_set_hnsw(dev_, (uint32_t)maxM0_, (uint32_t)maxM_, level0_.data(), lv.data(),
-                                    off.data(), pool.data(), pool.size(), enterpoint_node_, maxlevel_));
-        dev_hnsw_rows_ = n;
     }
+    tableint entryPoint() const { return enterpoint_node_; }
+    int maxLevel() const { return maxlevel_; }
+
+ private:
     void sync_device_csr() const {
         if (dev_csr_ok_) return;
         size_t n = cur_element_count;

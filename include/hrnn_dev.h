@@ -152,6 +152,13 @@ int hrnn_dev_rknn_search_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq,
                               uint32_t *res_ids_dev, uint32_t cap_per_q, uint32_t *work_dev);
 
 /* ---- RkNN phase A on the GPU -------------------------------------------- */
+/* SWITCHED OFF: the phase-A beam kernel is disabled until an unresolved GPU
+ * memory fault seen on its first run is explained.  hrnn_dev_proxies,
+ * hrnn_dev_proxies_dptr, hrnn_dev_rknn_query_dptr, hrnn_dev_query_features and
+ * hrnn_dev_rknn_query_learned_dptr return HRNN_E_STATE; phase A runs on the
+ * host (hrnn_index_proxies / hrnn_index_query_features in hrnn_host.h) and
+ * its proxies go to hrnn_dev_rknn_search[_dptr].  hrnn_dev_set_hnsw still
+ * validates and uploads the graph. */
 /* Upload the HNSW graph: l0[n*(maxM0+1)] layer-0 links (word 0: count in the
  * low 16 bits, then ids); levels[n]; up_off[n] offset (in words) of node i's
  * upper-layer block in up_pool (0xFFFFFFFF for level-0 nodes), the block
@@ -231,6 +238,18 @@ int hrnn_dev_rknn_query_learned_dptr(hrnn_dev_t h, const float *q_dev, uint32_t 
                                      float threshold, uint32_t *proxies_dev, uint32_t *m_dev,
                                      uint32_t *res_cnt_dev, uint32_t *res_ids_dev,
                                      uint32_t cap_per_q, uint32_t *work_dev);
+
+/* Phases B and C with learned m for a resident batch whose phase A ran on the
+ * host: feats_dev (nq*32 unstandardised features, standardised in place with
+ * the model's mu/sd), MLP forward, per-query m as in
+ * hrnn_dev_rknn_query_learned_dptr (proxies_dev rows of m_max masked beyond
+ * m; m_dev may be NULL), then phases B and C.  All work is queued on the
+ * library stream; nothing is copied to the host. */
+int hrnn_dev_rknn_search_learned_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, float *feats_dev,
+                                      const float *model_dev, const uint32_t *grid_dev, float tau,
+                                      uint32_t *proxies_dev, uint32_t m_max, uint32_t *m_dev, uint32_t k,
+                                      uint32_t K_prime, float threshold, uint32_t *res_cnt_dev,
+                                      uint32_t *res_ids_dev, uint32_t cap_per_q, uint32_t *work_dev);
 
 /* ---- distances ---------------------------------------------------------- */
 /* out[i] = L2Sqr(a[ia[i]], base[ib[i]]) in FP32 with the reference's 16-lane

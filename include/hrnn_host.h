@@ -60,6 +60,11 @@ int hrnn_index_proxies(hrnn_index_t ix, const float *q, uint64_t nq, uint32_t m,
                        int nthreads, uint32_t *out);
 int hrnn_index_proxies_gpu(hrnn_index_t ix, const float *q, uint64_t nq, uint32_t m, uint32_t ef,
                            uint32_t *out);
+/* Host phase A with beam statistics, then the 32 learned-m features per query
+ * (feats[nq*32]); proxies[nq*m_max] (m_max >= 16) receives the phase-A
+ * proxies, closest first; raw[nq*20] (may be NULL) the beam statistics. */
+int hrnn_index_query_features(hrnn_index_t ix, const float *q, uint64_t nq, uint32_t k, uint32_t ef,
+                              uint32_t m_max, int nthreads, float *feats, uint32_t *proxies, float *raw);
 /* searchRknn for nq queries: labels concatenated with offsets[nq+1].
  * per_query != 0 calls the single-query method once per query (the driver
  * loop of src/hrnn_search.cpp:168-179); 0 runs one batch. */
@@ -77,6 +82,11 @@ int hrnn_index_compact_for_serving(hrnn_index_t ix, uint32_t max_k_serve);
 int hrnn_index_save(hrnn_index_t ix, const char *path);
 /* The index's device handle (opened on first use) with base, graph and CSR resident. */
 int hrnn_index_device(hrnn_index_t ix, hrnn_dev_t *out);
+/* The HNSW arrays in the layout hrnn_dev_set_hnsw uploads: l0[n*(maxM0+1)],
+ * levels[n], up_off[n], up_pool[*pool_words]; with up_pool == NULL only
+ * *pool_words is set.  entry / maxlevel may be NULL. */
+int hrnn_index_export_hnsw(hrnn_index_t ix, uint32_t *l0, int *levels, uint32_t *up_off,
+                           uint32_t *up_pool, uint64_t *pool_words, uint32_t *entry, int *maxlevel);
 /* External label of every internal id (n entries).  Internal ids follow
  * insertion order, so they differ from labels after a multi-threaded build. */
 int hrnn_index_get_labels(hrnn_index_t ix, uint64_t *labels);

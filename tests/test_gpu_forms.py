@@ -1,0 +1,109 @@
+"""The precision, batched and learned forms at reduced sizes (same generators
+as the full configurations), checked against the oracle.
+
+* precision form: |N(0,1)| * U(0.05,1) data at d = 960; BF16 and FP16 MFMA
+  candidate passes with FP32 re-rank against the FP64 judge;
+* batched form: Zipf(1.1) query-to-cluster skew, k in {1, 10, 50} mixed per
+  query, one persistent launch, then streaming insertion of half the points;
+* learned form: L2-normalised von-Mises-Fisher-like mixture at d = 96.
+"""
+import numpy as np
+import pytest
+
+import hrnn_amd
+import oracle
+from hrnn_amd.index import Index
+
+pytestmark = pytest.mark.gpu
+
+
+def test_precision_form_bf16_fp16_vs_fp64():
+    x = hrnn_amd.synth_halfnormal_scaled(5, 55, 6000, 960)
+    q = hrnn_amd.synth_halfnormal_scaled(6, 55, 50, 960)
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    assert not dev.base_is_small_int()
+    k = 10
+    k64 = oracle.kth_f64(x, k)
+    e32 = oracle.kth(x, k)
+    report = {}
+    for prec in ("f32", "bf16", "f16"):
+        kth, ids = dev.bf_kth(k, prec, with_ids=True)
+        rel = np.abs(kth.astype(np.float64) - k64) / k64
+        exact_rows = float((kth == e32).mean())
+        report[prec] = {"max_rel": float(rel.max()), "p99_rel": float(np.quantile(rel, 0.99)),
+                        "rows_equal_fp32": exact_rows}
+        if prec == "f32":
+            assert np.array_equal(kth, e32)
+            assert rel.max() < 1e-5
+        else:
+            # FP32 re-rank of the 2(k+1) MFMA candidates: most rows are exact
+            assert exact_rows > 0.95, (prec, report[prec])
+    # RkNN against the FP64 judge: FP32 decisions differ only inside FP32 rounding of r_k
+    off, ids, d2 = dev.bf_rknn(q, e32, "f32")
+    eoff, eids, ed2 = oracle.rknn_gt(x, q, e32)
+    assert np.array_equal(off, eoff) and np.array_equal(ids, eids)
+    print("max relative r_k error vs FP64:", report)
+
+
+def test_batched_form_mixed_k_zipf_and_insert():
+    n = 6000
+    x, cen = hrnn_amd.synth_mixture(3, n, 128, 200, 25.0, 0.0, 255.0, True, 0.0, 1003,
+                                    return_centres=True)
+    q = hrnn_amd.synth_mixture(7, 800, 128, 200, 25.0, 0.0, 255.0, True, 1.1, 1003)
+    kq = np.random.default_rng(7).choice([1, 10, 50], q.shape[0]).astype(np.uint32)
+    K = 64
+    perm = np.random.default_rng(42).permutation(n)
+    half = n // 2
+    ix = Index(n, 128, 16, 100, 100, K)
+    ix.add_points(x[perm[:half]], labels=perm[:half].astype(np.uint64), nthreads=4)
+    ix.refine(4, 16)
+    ix.build_rknng_mutable()
+    ix.insert_dynamic(x[perm[half:]], perm[half:].astype(np.uint64), 0, False, nthreads=4)
+    ix.build_rknng()
+    prox = ix.proxies(q, 20, 100)
+    off, ids = ix.search_proxies(q, prox, 0, K, 1.0, k_per_q=kq)
+    # exact against the oracle on the same index state
+    gids, gd, gc = ix.get_knng()
+    coff, cent, rb, wide = ix.get_rknng()
+    lab = ix.labels()
+    xi = x[lab]                           # vectors in internal-id order
+    eoff, eids = oracle.search_bc(coff, cent, wide, rb, gd, gc, K, xi, q, prox, 0, K, 1.0,
+                                  k_per_q=kq)
+    assert np.array_equal(off, eoff) and np.array_equal(ids, eids)
+    # per-k recall against exact RkNN
+    for kv in (1, 10, 50):
+        sel = np.nonzero(kq == kv)[0]
+        kth = oracle.kth(xi, kv)
+        goff, gid, _ = oracle.rknn_gt(xi, q[sel], kth)
+        soff = np.zeros(len(sel) + 1, np.uint64)
+        sids = []
+        for j, i in enumerate(sel):
+            r = ids[off[i]:off[i + 1]]
+            sids.append(r)
+            soff[j + 1] = soff[j] + len(r)
+        rec = oracle.rknn_recall(goff, gid, soff, np.concatenate(sids).astype(np.uint32), n)[0]
+        assert rec > 0.8, (kv, rec)
+
+
+def test_learned_form_sphere_mixture_search_parity():
+    x = hrnn_amd.synth_sphere_mixture(8, 88, 8000, 96, 200, 50.0)
+    q = hrnn_amd.synth_sphere_mixture(10, 88, 100, 96, 200, 50.0)
+    ix = Index(8000, 96, 16, 100, 100, 48)
+    ix.add_points(x, labels=np.arange(8000, dtype=np.uint64), nthreads=1)
+    st = ix.refine(4, 16)
+    assert all(s["dists"] > 0 for s in st)
+    ix.build_rknng()
+    lab = ix.labels()
+    assert np.array_equal(lab, np.arange(8000))
+    # float data: proposals are re-computed in FP32; every stored distance is exact FP32
+    ids, d, cnt = ix.get_knng()
+    for i in range(0, 8000, 97):
+        c = int(cnt[i])
+        exp = np.array([oracle.l2sqr(x[i], x[j]) for j in ids[i, :c]], np.float32)
+        assert np.array_equal(exp, d[i, :c])
+    prox = ix.proxies(q, 10, 100)
+    off, res = ix.search_proxies(q, prox, 10, 48)
+    coff, cent, rb, wide = ix.get_rknng()
+    eoff, eres = oracle.search_bc(coff, cent, wide, rb, d, cnt, 48, x, q, prox, 10, 48)
+    assert np.array_equal(off, eoff) and np.array_equal(res, eres)

@@ -138,3 +138,47 @@ def test_capacity_and_preconditions(data):
     with pytest.raises(hrnn_amd.HrnnError, match="already present"):
         ix2 = Index(10, 128, 8, 32, 100, 0)
         ix2.add_points(x[:2], labels=np.array([7, 7], np.uint64))
+
+
+def test_host_query_features_consistent(data):
+    """Host phase A with beam statistics (the learned-m inputs): the beam's
+    top-16 distances are the exact distances of the first 16 proxies, the
+    proxies are the phase-A proxies of proxies(), and every feature follows
+    from them and the KNN lists."""
+    x, q = data
+    ix = Index(x.shape[0], 128, 16, 100, 100, 40)
+    ix.add_points(x, nthreads=1)
+    f, p, raw = ix.query_features(q, 10, 64, 20, nthreads=1, with_raw=True)
+    assert f.shape == (len(q), 32) and np.isfinite(f).all()
+    assert np.array_equal(p, ix.proxies(q, 20, 64, 1))
+    lab = ix.labels()
+    xi = x[lab]
+    ids, kd, cnt = ix.get_knng()
+    for i in range(len(q)):
+        dd = np.array([oracle.l2sqr(q[i], xi[j]) for j in p[i, :16]], np.float32)
+        assert np.array_equal(raw[i, 2:18], dd)                   # closest first
+        assert raw[i, 19] == 64 and raw[i, 18] >= 1               # beam size, expansions
+        assert raw[i, 1] <= raw[i, 0]                             # descent never moves away
+        rk = np.array([kd[j, 9] if cnt[j] >= 10 else 3e38 for j in p[i, :16]], np.float32)
+        assert f[i, 29] == pytest.approx(np.mean(dd <= rk))
+        assert f[i, 18] == pytest.approx(np.log1p(rk[0]))
+        assert f[i, 31] == pytest.approx(np.log1p(raw[i, 18]))
+    with pytest.raises(hrnn_amd.HrnnError):
+        ix.query_features(q, 10, 64, 8)                           # needs >= 16 proxies
+
+
+@needs_ref
+def test_phase_a_proxies_equal_reference_on_same_file(tmp_path):
+    """Phase A (HNSW descent + searchBaseLayerST<true> beam, hrnn.h:1992-2019)
+    on one index file: our proxies and the reference's are the same sets,
+    distance ties included (integer data make ties frequent)."""
+    x = mixture(3000, seed=61, centre_seed=1061)
+    q = mixture(80, seed=62, centre_seed=1061)
+    ix = Index(3000, 128, 16, 100, 100, 48)
+    ix.add_points(x, nthreads=1)
+    p = str(tmp_path / "a.idx")
+    ix.save(p)
+    r = oracle.RefIndex.load(p, 128)
+    for m, ef in [(1, 10), (10, 64), (20, 200), (50, 200), (64, 64)]:
+        a, b = ix.proxies(q, m, ef, 1), r.proxies(q, m, ef)
+        assert np.array_equal(np.sort(a, 1), np.sort(b, 1)), (m, ef)
