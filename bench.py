@@ -261,14 +261,16 @@ def main() -> int:
         ft, pt = ix.query_features(qt, k, ef, m_max, threads)
         gt_t_off, gt_t_ids, _ = dev.bf_rknn(qt, kth, "bf16")
         lab, rec_grid = label_queries(dev, qt, pt, gt_t_off, gt_t_ids, k, Kp)
-        model, losses = LearnedM.train(dev, ft, lab, epochs=60, batch=256, lr=2e-3)
-        # tau: the smallest one whose training-set recall reaches the target
-        probs = model.probs(dev, ft)
+        n_fit = int(0.8 * len(qt))                   # fit on 80%, choose tau on the rest
+        model, losses = LearnedM.train(dev, ft[:n_fit], lab[:n_fit], epochs=60, batch=256, lr=2e-3)
+        # tau: the smallest one whose validation recall reaches the target
+        probs = model.probs(dev, ft[n_fit:])
+        rv = rec_grid[:, n_fit:]
         tau_star = TAUS[-1]
         for tau in TAUS:
             mq = model.choose_m(probs, tau)
             gi = np.searchsorted(GRID, mq)
-            r = rec_grid[gi, np.arange(len(mq))]
+            r = rv[gi, np.arange(len(mq))]
             if np.mean(r) >= args.recall_target + 0.005:
                 tau_star = tau
                 break
@@ -333,12 +335,14 @@ def main() -> int:
         try:
             import oracle
             if oracle.ref_available():
-                tmpdir = tempfile.mkdtemp(prefix="hrnn_bench_")
+                shm = "/dev/shm" if os.path.isdir("/dev/shm") else None
+                tmpdir = tempfile.mkdtemp(prefix="hrnn_bench_", dir=shm)
                 path = os.path.join(tmpdir, "c.idx")
                 t = time.time()
                 ix.save(path)
                 r = oracle.RefIndex.load(path, d)
                 os.remove(path)
+                os.rmdir(tmpdir)
                 log(f"reference loaded this run's index in {time.time() - t:.1f}s")
                 budget_s, done, spent, bs = 20.0, 0, 0.0, 25
                 while spent < budget_s and done < nq:

@@ -12,7 +12,7 @@ hrnn_dev::~hrnn_dev() {
     (void)hipSetDevice(device);
     DevBuf *bufs[] = {&x, &xb, &xh, &norms, &knn_ids, &knn_d, &knn_cnt, &knn_new, &csr_off,
                       &csr_ent, &search_mark, &search_cand, &l0, &up_off, &up_pool, &beam_vis,
-                      &s0, &s1, &s2, &s3, &s4, &s5, &s6, &s7, &s8, &counters};
+                      &s0, &s1, &s2, &s3, &s4, &s5, &s6, &s7, &s8, &nnd_aux, &counters};
     for (DevBuf *b : bufs) b->release();
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -320,8 +320,8 @@ int hrnn_dev_nndescent(hrnn_dev_t h, int num_iters, int sample_size, uint32_t bu
     if (!h) return HRNN_E_ARG;
     return guarded(h, [&] {
         REQUIRE(h->K > 0, HRNN_E_STATE, "nndescent: set_knng first");
-        REQUIRE(num_iters >= 0 && sample_size > 0 && sample_size <= 128, HRNN_E_ARG,
-                "nndescent: sample_size must be in [1, 128]");
+        REQUIRE(num_iters >= 0 && sample_size > 0 && sample_size <= 64, HRNN_E_ARG,
+                "nndescent: sample_size must be in [1, 64]");
         nndescent_run(h, num_iters, sample_size, bucket, stats);
         h->csr_valid = false;
     });

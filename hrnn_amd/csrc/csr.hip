@@ -106,6 +106,16 @@ __global__ void k_scan_add(uint64_t *__restrict__ out, uint64_t n,
     if (i == 0) out[n] = *total;
 }
 
+uint32_t scan_block_count(uint64_t n) { return ceil_div(n, kScanBlock * kScanItems); }
+
+void scan_u32(hrnn_dev_t h, const uint32_t *cnt, uint64_t n, uint64_t *off, uint64_t *bsum,
+              uint64_t *total) {
+    uint32_t nblk = scan_block_count(n);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nblk), dim3(kScanBlock), 0, h->stream, cnt, n, off, bsum);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(64), 0, h->stream, bsum, nblk, total);
+    hipLaunchKernelGGL(k_scan_add, dim3(ceil_div(n, 256)), dim3(256), 0, h->stream, off, n, bsum, total);
+}
+
 template <class KeyT>
 __global__ void k_scatter(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ cnt,
                           uint64_t n, uint32_t K, const uint64_t *__restrict__ off,

@@ -49,6 +49,7 @@ struct hrnn_dev {
 
     // scratch shared by the build kernels
     hrnn::DevBuf s0, s1, s2, s3, s4, s5, s6, s7, s8;
+    hrnn::DevBuf nnd_aux;   // NNDescent proposal offsets, scan partials, total
     // small device counters, by byte offset: 0 base-int flag, 32 long-list
     // count, 128 CSR total, 192 longest list, 256 search queue, 512 NNDescent stats
     hrnn::DevBuf counters;
@@ -80,6 +81,11 @@ void d2h(hrnn_dev_t h, void *dst, const void *src, size_t bytes);
 // Launchers defined in the kernel files.
 void launch_base_mirrors(hrnn_dev_t h, uint64_t row0, uint64_t rows);
 void csr_build(hrnn_dev_t h);
+// Exclusive scan of n u32 counts into off[n + 1] (u64; off[n] = *total = the
+// sum); bsum holds scan_block_count(n) u64.  Kernels only (capturable).
+uint32_t scan_block_count(uint64_t n);
+void scan_u32(hrnn_dev_t h, const uint32_t *cnt, uint64_t n, uint64_t *off, uint64_t *bsum,
+              uint64_t *total);
 uint64_t max_list_len(hrnn_dev_t h);
 void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint32_t *proxies,
                           uint32_t m, const uint32_t *k_per_q, uint32_t k, uint32_t K_prime,

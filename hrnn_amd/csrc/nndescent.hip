@@ -4,9 +4,12 @@
 // try_update_knng_nnd (hrnn.h:1660-1722).  Per iteration:
 //   UPDATE (hrnn.h:1765-1810)  k_nnd_sample: one wave per node walks its list
 //       in rank order; the first S new entries become nn_new, the first S old
-//       ones nn_old; every new flag is cleared; entry (i -> nbr) is pushed into
-//       nbr's reverse sample (new or old, capped at R = S) when
-//       dist(i, nbr) > nbr's K-th distance (FLT_MAX unless nbr's list is full).
+//       ones nn_old; every new flag is cleared; entry (i -> nbr) is offered to
+//       nbr's reverse sample (new or old) when dist(i, nbr) > nbr's K-th
+//       distance (FLT_MAX unless nbr's list is full).  The reference keeps the
+//       first R = S offers, which a single thread makes in increasing i; here
+//       up to 256 offers are collected per node and the join keeps the R
+//       smallest source ids, the same set whenever a node gets <= 256 offers.
 //   MERGE (hrnn.h:1813-1819)   the join reads forward + reverse samples
 //       directly (old side cut at 2R).
 //   JOIN (hrnn.h:1823-1853)    k_nnd_join: one workgroup per node stages the
@@ -15,19 +18,24 @@
 //       = new (upper triangle) and old samples.  d2 = |a|^2 + |b|^2 - 2 a.b.
 //       A pair (a, b), a != b, is proposed to a's list when d2 < worst(a) and
 //       to b's list when d2 < worst(b), worst taken at the iteration start.
-//       Proposals go to per-node buckets of capacity P (atomic slot).
-//   k_nnd_merge: per node, sort the bucket by (d2, id), drop duplicates and
-//       ids already listed, and merge with the sorted list (existing entries
-//       first on equal distance, as the reference's strict-< insertion sort
-//       leaves them); the first K survive, inserted entries flagged new.
-// The reference applies each proposal at once under a per-node lock; the
-// final list of an iteration is the same top-K of (list U proposals) up to
-// ties, so the contract with the reference is recall parity.
+//       The join runs twice: a counting pass (proposals per target), an
+//       exclusive scan into a proposal CSR, then a filling pass that writes
+//       every proposal at its target's offset.  Nothing is dropped unless the
+//       whole iteration exceeds the proposal buffer (reported as dropped).
+//   k_nnd_merge: per node, proposals in chunks of 2048: sort by (d2, id), drop
+//       duplicates and ids already listed, merge with the sorted list
+//       (existing entries first on equal distance, as the reference's strict-<
+//       insertion leaves them); the first K survive, inserted entries flagged
+//       new.  Chunk after chunk this is the top-K of (list U all proposals).
+// The reference applies each proposal at once under a per-node lock: an
+// offer rejected at some point would be rejected later too (the K-th
+// distance only falls), so its final list is the same top-K of (list U
+// proposals), up to the order of equal distances.
 //
 // Distances: for integer data in [-256, 256] at d <= 128 (all 0..255 configs)
 // the BF16 Gram form is exact FP32 arithmetic, identical to the reference's
 // direct-difference L2Sqr.  Otherwise each proposal that passes the BF16
-// filter is re-computed in the reference's FP32 order before it is bucketed.
+// filter is re-computed in the reference's FP32 order before it is counted.
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
@@ -46,7 +54,8 @@ enum { ST_DISTS = 0, ST_PROPOSALS = 1, ST_DROPPED = 2, ST_UPDATES = 3, ST_N = 4 
 __global__ void k_nnd_prepare(const float *__restrict__ kd, const uint32_t *__restrict__ cnt,
                               uint64_t n, uint32_t K, float *__restrict__ worst,
                               uint32_t *__restrict__ rn_cnt, uint32_t *__restrict__ ro_cnt,
-                              uint32_t *__restrict__ prop_cnt, unsigned long long *__restrict__ stats) {
+                              uint32_t *__restrict__ prop_cnt, uint32_t *__restrict__ prop_fill,
+                              unsigned long long *__restrict__ stats) {
     uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u == 0) {
         for (int s = 0; s < ST_N; s++) stats[s] = 0;
@@ -57,11 +66,12 @@ __global__ void k_nnd_prepare(const float *__restrict__ kd, const uint32_t *__re
     rn_cnt[u] = 0;
     ro_cnt[u] = 0;
     prop_cnt[u] = 0;
+    prop_fill[u] = 0;
 }
 
 __global__ __launch_bounds__(256) void k_nnd_sample(
     const uint32_t *__restrict__ ids, const float *__restrict__ kd, uint8_t *__restrict__ isnew,
-    const uint32_t *__restrict__ cnt, uint64_t n, uint32_t K, uint32_t S, uint32_t R,
+    const uint32_t *__restrict__ cnt, uint64_t n, uint32_t K, uint32_t S, uint32_t RB,
     const float *__restrict__ worst, uint32_t *__restrict__ fn, uint32_t *__restrict__ fn_cnt,
     uint32_t *__restrict__ fo, uint32_t *__restrict__ fo_cnt, uint32_t *__restrict__ rn,
     uint32_t *__restrict__ rn_cnt, uint32_t *__restrict__ ro, uint32_t *__restrict__ ro_cnt) {
@@ -94,10 +104,10 @@ __global__ __launch_bounds__(256) void k_nnd_sample(
         if (valid && dij > worst[nbr]) {
             if (isn) {
                 uint32_t s = atomicAdd(&rn_cnt[nbr], 1u);
-                if (s < R) rn[(uint64_t)nbr * R + s] = (uint32_t)i;
+                if (s < RB) rn[(uint64_t)nbr * RB + s] = (uint32_t)i;
             } else {
                 uint32_t s = atomicAdd(&ro_cnt[nbr], 1u);
-                if (s < R) ro[(uint64_t)nbr * R + s] = (uint32_t)i;
+                if (s < RB) ro[(uint64_t)nbr * RB + s] = (uint32_t)i;
             }
         }
         new_before += __popcll(mn);
@@ -113,6 +123,7 @@ constexpr int kJMax = 256;      // sample members per node (4S, S <= 64)
 constexpr int kJDC = 64;        // dims per LDS chunk
 constexpr int kJDP = kJDC + 8;  // padded row stride
 constexpr int kJMaxB = 32;      // Gram blocks per wave
+constexpr int kJRev = 256;      // reverse offers kept per node and kind
 
 __device__ float l2_ref16_lane(const float *__restrict__ a, const float *__restrict__ b, uint32_t d) {
     if (d & 15u) return l2_seq(a, b, d);
@@ -132,33 +143,43 @@ __device__ float l2_ref16_lane(const float *__restrict__ a, const float *__restr
     return r;
 }
 
+// Counting pass: one more proposal for `target`.  Filling pass: the proposal
+// goes to the next free slot of target's range of the proposal CSR.
+template <bool kFill>
 __device__ __forceinline__ void nnd_propose(uint32_t target, uint32_t cand, float dv,
                                             uint32_t *__restrict__ prop_cnt,
+                                            const uint64_t *__restrict__ prop_off,
                                             uint32_t *__restrict__ prop_id, float *__restrict__ prop_d,
-                                            uint32_t P, uint32_t &dropped) {
-    uint32_t s = atomicAdd(&prop_cnt[target], 1u);
-    if (s < P) {
-        prop_id[(uint64_t)target * P + s] = cand;
-        prop_d[(uint64_t)target * P + s] = dv;
+                                            uint64_t cap, uint32_t &dropped) {
+    if (!kFill) {
+        atomicAdd(&prop_cnt[target], 1u);
+        return;
+    }
+    uint64_t pos = prop_off[target] + atomicAdd(&prop_cnt[target], 1u);   // prop_cnt = fill cursor here
+    if (pos < cap) {
+        prop_id[pos] = cand;
+        prop_d[pos] = dv;
     } else {
         dropped++;
     }
 }
 
+template <bool kFill>
 __global__ __launch_bounds__(256) void k_nnd_join(
     const __hip_bfloat16 *__restrict__ xb, const float *__restrict__ xn, const float *__restrict__ x32,
-    uint32_t d, uint64_t n, uint32_t S, uint32_t R, const uint32_t *__restrict__ fn,
+    uint32_t d, uint64_t n, uint32_t S, uint32_t R, uint32_t RB, const uint32_t *__restrict__ fn,
     const uint32_t *__restrict__ fn_cnt, const uint32_t *__restrict__ fo,
     const uint32_t *__restrict__ fo_cnt, const uint32_t *__restrict__ rn,
     const uint32_t *__restrict__ rn_cnt, const uint32_t *__restrict__ ro,
     const uint32_t *__restrict__ ro_cnt, const float *__restrict__ worst,
-    uint32_t *__restrict__ prop_cnt, uint32_t *__restrict__ prop_id, float *__restrict__ prop_d,
-    uint32_t P, int exact, unsigned long long *__restrict__ stats) {
+    uint32_t *__restrict__ prop_cnt, const uint64_t *__restrict__ prop_off, uint32_t *__restrict__ prop_id,
+    float *__restrict__ prop_d, uint64_t cap, int exact, unsigned long long *__restrict__ stats) {
     __shared__ uint32_t sid[kJMax];
     __shared__ float sthr[kJMax], snorm[kJMax];
     __shared__ __attribute__((aligned(16))) __hip_bfloat16 vec[kJMax * kJDP];
+    __shared__ uint32_t rsel[2][kJRev];   // reverse offers (new, old), sorted by source id
     __shared__ uint8_t blk_r[kJMaxB * 4], blk_c[kJMaxB * 4];
-    __shared__ uint32_t s_nn, s_T, s_nb;
+    __shared__ uint32_t s_a, s_b, s_c, s_nn, s_T, s_nb;
 
     const uint64_t node = blockIdx.x;
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x / 64;
@@ -168,27 +189,54 @@ __global__ __launch_bounds__(256) void k_nnd_join(
         uint32_t nn = a + b;
         uint32_t no = c + e;
         if (no > 2 * R) no = 2 * R;
+        s_a = a;
+        s_b = b;
+        s_c = c;
         s_nn = nn;
         s_T = nn + no;
     }
     __syncthreads();
     const uint32_t nn = s_nn, T = s_T;
     if (nn == 0) return;   // no new sample: no pair to evaluate
+    // The R smallest source ids among the (<= RB) reverse offers of each kind.
+    {
+        const uint32_t cn = min(rn_cnt[node], RB), co = min(ro_cnt[node], RB);
+        for (uint32_t t = threadIdx.x; t < kJRev; t += blockDim.x) {
+            rsel[0][t] = t < cn ? rn[node * RB + t] : kInvalidId;
+            rsel[1][t] = t < co ? ro[node * RB + t] : kInvalidId;
+        }
+        for (uint32_t size = 2; size <= (uint32_t)kJRev; size <<= 1) {
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                __syncthreads();
+                for (uint32_t t = threadIdx.x; t < (uint32_t)kJRev; t += blockDim.x) {   // 2 x kJRev/2 pairs
+                    const uint32_t arr = t / (kJRev / 2), i = t % (kJRev / 2);
+                    const uint32_t lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
+                    const bool up = (lo & size) == 0;
+                    uint32_t x0 = rsel[arr][lo], x1 = rsel[arr][hi];
+                    if ((x0 > x1) == up) {
+                        rsel[arr][lo] = x1;
+                        rsel[arr][hi] = x0;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const uint32_t a = s_a, nnew = s_nn, c = s_c;
     for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
-        uint32_t a = fn_cnt[node], b = min(rn_cnt[node], R), c = fo_cnt[node];
         uint32_t id;
         if (t < a) id = fn[node * S + t];
-        else if (t < nn) id = rn[node * R + (t - a)];
-        else if (t < nn + c) id = fo[node * S + (t - nn)];
-        else id = ro[node * R + (t - nn - c)];
+        else if (t < nnew) id = rsel[0][t - a];
+        else if (t < nnew + c) id = fo[node * S + (t - nnew)];
+        else id = rsel[1][t - nnew - c];
         sid[t] = id;
         sthr[t] = worst[id];
         snorm[t] = xn[id];
     }
-    const uint32_t RB = (nn + 15) / 16, CB = (T + 15) / 16;
+    const uint32_t RBk = (nn + 15) / 16, CB = (T + 15) / 16;
     if (threadIdx.x == 0) {
         uint32_t nb = 0;
-        for (uint32_t rb = 0; rb < RB; rb++)
+        for (uint32_t rb = 0; rb < RBk; rb++)
             for (uint32_t cb = rb; cb < CB; cb++) {
                 blk_r[nb] = (uint8_t)rb;
                 blk_c[nb] = (uint8_t)cb;
@@ -242,214 +290,256 @@ __global__ __launch_bounds__(256) void k_nnd_join(
         uint32_t b = w + 4 * j;
         if (b < NB) {
             uint32_t rb = blk_r[b], cb = blk_c[b];
-            uint32_t c = cb * 16 + (lane & 15u);
+            uint32_t col = cb * 16 + (lane & 15u);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 uint32_t r = rb * 16 + 4 * (lane >> 4) + i;
-                if (r < nn && c > r && c < T) {
-                    uint32_t a = sid[r], bb = sid[c];
-                    if (a != bb) {
+                if (r < nn && col > r && col < T) {
+                    uint32_t ia = sid[r], ib = sid[col];
+                    if (ia != ib) {
                         ndist++;
-                        float dv = snorm[r] + snorm[c] - 2.f * acc[j][i];
+                        float dv = snorm[r] + snorm[col] - 2.f * acc[j][i];
                         dv = dv > 0.f ? dv : 0.f;
-                        bool pa = dv < sthr[r], pb = dv < sthr[c];
+                        bool pa = dv < sthr[r], pb = dv < sthr[col];
                         if ((pa || pb) && !exact) {
-                            dv = l2_ref16_lane(x32 + (uint64_t)a * d, x32 + (uint64_t)bb * d, d);
+                            dv = l2_ref16_lane(x32 + (uint64_t)ia * d, x32 + (uint64_t)ib * d, d);
                             pa = dv < sthr[r];
-                            pb = dv < sthr[c];
+                            pb = dv < sthr[col];
                         }
-                        if (pa) { nprop++; nnd_propose(a, bb, dv, prop_cnt, prop_id, prop_d, P, dropped); }
-                        if (pb) { nprop++; nnd_propose(bb, a, dv, prop_cnt, prop_id, prop_d, P, dropped); }
+                        if (pa) {
+                            nprop++;
+                            nnd_propose<kFill>(ia, ib, dv, prop_cnt, prop_off, prop_id, prop_d, cap, dropped);
+                        }
+                        if (pb) {
+                            nprop++;
+                            nnd_propose<kFill>(ib, ia, dv, prop_cnt, prop_off, prop_id, prop_d, cap, dropped);
+                        }
                     }
                 }
             }
         }
     }
-    // wave-level reduction of the counters
+    // wave-level reduction of the counters (distances and proposals from the
+    // counting pass, drops from the filling pass)
     for (int off = 32; off > 0; off >>= 1) {
         ndist += __shfl_down(ndist, off, 64);
         nprop += __shfl_down(nprop, off, 64);
         dropped += __shfl_down(dropped, off, 64);
     }
     if (lane == 0) {
-        if (ndist) atomicAdd(&stats[ST_DISTS], (unsigned long long)ndist);
-        if (nprop) atomicAdd(&stats[ST_PROPOSALS], (unsigned long long)nprop);
-        if (dropped) atomicAdd(&stats[ST_DROPPED], (unsigned long long)dropped);
+        if (!kFill && ndist) atomicAdd(&stats[ST_DISTS], (unsigned long long)ndist);
+        if (!kFill && nprop) atomicAdd(&stats[ST_PROPOSALS], (unsigned long long)nprop);
+        if (kFill && dropped) atomicAdd(&stats[ST_DROPPED], (unsigned long long)dropped);
     }
 }
 
-constexpr uint32_t kMergeMaxP = 2048;
+constexpr uint32_t kMergeChunk = 2048;   // proposals per merge round (LDS sort)
 constexpr uint32_t kMergeMaxK = 1024;
 constexpr uint32_t kHashSlots = 2048;
 
+// Exclusive scan of flags[0..P) in place (P <= kMergeChunk, 256 threads);
+// flags[P] receives the total.
+__device__ void block_excl_scan(uint32_t *flags, uint32_t P, uint32_t *wsum) {
+    const uint32_t per = (P + blockDim.x - 1) / blockDim.x;
+    const uint32_t beg = threadIdx.x * per, end = min(beg + per, P);
+    uint32_t local = 0;
+    for (uint32_t t = beg; t < end; t++) local += flags[t];
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x / 64;
+    uint32_t incl = local;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(incl, off, 64);
+        if (lane >= (uint32_t)off) incl += y;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t woff = 0;
+    for (uint32_t k = 0; k < w; k++) woff += wsum[k];
+    uint32_t run = woff + incl - local;
+    __syncthreads();
+    for (uint32_t t = beg; t < end; t++) {
+        uint32_t v = flags[t];
+        flags[t] = run;
+        run += v;
+    }
+    if (threadIdx.x == blockDim.x - 1) flags[P] = run;
+    __syncthreads();
+}
+
+// One workgroup (256 lanes) per node with proposals: the list lives in LDS
+// while the node's proposals are merged into it 2048 at a time.
 __global__ __launch_bounds__(256) void k_nnd_merge(
     uint32_t *__restrict__ ids, float *__restrict__ kd, uint8_t *__restrict__ isnew,
     uint32_t *__restrict__ cnt, uint32_t K, const uint32_t *__restrict__ prop_cnt,
-    const uint32_t *__restrict__ prop_id, const float *__restrict__ prop_d, uint32_t P,
-    unsigned long long *__restrict__ stats) {
-    __shared__ unsigned long long keys[kMergeMaxP];
-    __shared__ uint32_t scan[kMergeMaxP + 1];
-    __shared__ uint32_t lid[kMergeMaxK];
-    __shared__ float ld[kMergeMaxK];
+    const uint64_t *__restrict__ prop_off, const uint32_t *__restrict__ prop_id,
+    const float *__restrict__ prop_d, uint64_t cap, unsigned long long *__restrict__ stats) {
+    __shared__ unsigned long long keys[kMergeChunk];
+    __shared__ uint32_t scan[kMergeChunk + 1];
+    __shared__ uint32_t lid[kMergeMaxK], oid[kMergeMaxK];
+    __shared__ float ld[kMergeMaxK], od[kMergeMaxK];
+    __shared__ uint8_t lnw[kMergeMaxK], onw[kMergeMaxK];
     __shared__ uint32_t hash[kHashSlots];
     __shared__ uint32_t wsum[4];
 
     const uint64_t u = blockIdx.x;
-    uint32_t np = prop_cnt[u];
+    const uint64_t beg = prop_off[u];
+    uint64_t np64 = prop_cnt[u];
+    if (beg >= cap) np64 = 0;
+    else if (beg + np64 > cap) np64 = cap - beg;   // proposals past the buffer were dropped
+    const uint32_t np = (uint32_t)np64;
     if (np == 0) return;
-    if (np > P) np = P;
-    const uint32_t c = cnt[u];
-    for (uint32_t t = threadIdx.x; t < kHashSlots; t += blockDim.x) hash[t] = kInvalidId;
+    uint32_t c = cnt[u];
     for (uint32_t t = threadIdx.x; t < c; t += blockDim.x) {
         lid[t] = ids[u * K + t];
         ld[t] = kd[u * K + t];
+        lnw[t] = isnew[u * K + t];
     }
-    __syncthreads();
-    for (uint32_t t = threadIdx.x; t < c; t += blockDim.x) {
-        uint32_t v = lid[t];
-        uint32_t h = (v * 2654435761u) & (kHashSlots - 1);
-        for (;;) {
-            uint32_t old = atomicCAS(&hash[h], kInvalidId, v);
-            if (old == kInvalidId || old == v) break;
-            h = (h + 1) & (kHashSlots - 1);
-        }
-    }
-    const float worst_full = (c >= K) ? ld[K - 1] : FLT_MAX;
-    uint32_t PP = 1;
-    while (PP < np) PP <<= 1;
-    for (uint32_t t = threadIdx.x; t < PP; t += blockDim.x) {
-        unsigned long long key = ~0ull;
-        if (t < np) {
-            float dv = prop_d[u * P + t];
-            uint32_t id = prop_id[u * P + t];
-            if (dv < worst_full) key = ((unsigned long long)float_key(dv) << 32) | id;
-        }
-        keys[t] = key;
-    }
-    __syncthreads();
-    bitonic_sort_lds(keys, PP);
-    // validity: not a repeat of the previous key, not already in the list
-    for (uint32_t t = threadIdx.x; t < PP; t += blockDim.x) {
-        unsigned long long key = keys[t];
-        uint32_t ok = 0;
-        if (key != ~0ull && (t == 0 || keys[t - 1] != key)) {
-            uint32_t v = (uint32_t)key;
+    uint32_t upd = 0;
+    for (uint32_t base = 0; base < np; base += kMergeChunk) {
+        const uint32_t nb = min(kMergeChunk, np - base);
+        __syncthreads();
+        // ids currently listed
+        for (uint32_t t = threadIdx.x; t < kHashSlots; t += blockDim.x) hash[t] = kInvalidId;
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < c; t += blockDim.x) {
+            uint32_t v = lid[t];
             uint32_t h = (v * 2654435761u) & (kHashSlots - 1);
-            ok = 1;
             for (;;) {
-                uint32_t e = hash[h];
-                if (e == kInvalidId) break;
-                if (e == v) { ok = 0; break; }
+                uint32_t old = atomicCAS(&hash[h], kInvalidId, v);
+                if (old == kInvalidId || old == v) break;
                 h = (h + 1) & (kHashSlots - 1);
             }
         }
-        scan[t] = ok;
+        const float worst_full = (c >= K) ? ld[K - 1] : FLT_MAX;
+        uint32_t PP = 1;
+        while (PP < nb) PP <<= 1;
+        for (uint32_t t = threadIdx.x; t < PP; t += blockDim.x) {
+            unsigned long long key = ~0ull;
+            if (t < nb) {
+                float dv = prop_d[beg + base + t];
+                uint32_t id = prop_id[beg + base + t];
+                if (dv < worst_full) key = ((unsigned long long)float_key(dv) << 32) | id;
+            }
+            keys[t] = key;
+        }
+        __syncthreads();
+        bitonic_sort_lds(keys, PP);
+        __syncthreads();
+        // a proposal counts once: not a repeat of the previous key (the same
+        // pair always has the same distance) and not already listed
+        for (uint32_t t = threadIdx.x; t < PP; t += blockDim.x) {
+            unsigned long long key = keys[t];
+            uint32_t ok = 0;
+            if (key != ~0ull && (t == 0 || keys[t - 1] != key)) {
+                uint32_t v = (uint32_t)key;
+                uint32_t h = (v * 2654435761u) & (kHashSlots - 1);
+                ok = 1;
+                for (;;) {
+                    uint32_t e = hash[h];
+                    if (e == kInvalidId) break;
+                    if (e == v) { ok = 0; break; }
+                    h = (h + 1) & (kHashSlots - 1);
+                }
+            }
+            scan[t] = ok;
+        }
+        __syncthreads();
+        block_excl_scan(scan, PP, wsum);
+        const uint32_t nvalid = scan[PP];
+        if (nvalid == 0) continue;   // uniform
+        // proposals: pos = #valid before + #listed with dist <= d
+        for (uint32_t t = threadIdx.x; t < PP; t += blockDim.x) {
+            unsigned long long key = keys[t];
+            if (key == ~0ull || scan[t + 1] == scan[t]) continue;
+            float dv = key_float((uint32_t)(key >> 32));
+            uint32_t lo = 0, hi = c;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (ld[mid] <= dv) lo = mid + 1; else hi = mid;
+            }
+            uint32_t pos = scan[t] + lo;
+            if (pos < K) {
+                oid[pos] = (uint32_t)key;
+                od[pos] = dv;
+                onw[pos] = 1;
+                upd++;
+            }
+        }
+        // listed entries: pos = e + #valid proposals with dist < ld[e]
+        for (uint32_t e = threadIdx.x; e < c; e += blockDim.x) {
+            uint32_t kf = float_key(ld[e]);
+            uint32_t lo = 0, hi = PP;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi) >> 1;
+                if ((uint32_t)(keys[mid] >> 32) < kf) lo = mid + 1; else hi = mid;
+            }
+            uint32_t pos = e + scan[lo];
+            if (pos < K) {
+                oid[pos] = lid[e];
+                od[pos] = ld[e];
+                onw[pos] = lnw[e];
+            }
+        }
+        __syncthreads();
+        c = min(c + nvalid, K);
+        for (uint32_t t = threadIdx.x; t < c; t += blockDim.x) {
+            lid[t] = oid[t];
+            ld[t] = od[t];
+            lnw[t] = onw[t];
+        }
     }
     __syncthreads();
-    // exclusive scan of scan[0..PP) (serial per thread chunk + wave sums)
-    {
-        const uint32_t per = (PP + blockDim.x - 1) / blockDim.x;
-        uint32_t beg = threadIdx.x * per, end = min(beg + per, PP);
-        uint32_t local = 0;
-        for (uint32_t t = beg; t < end; t++) local += scan[t];
-        uint32_t lane = threadIdx.x & 63u, w = threadIdx.x / 64;
-        uint32_t incl = local;
-        for (int off = 1; off < 64; off <<= 1) {
-            uint32_t y = __shfl_up(incl, off, 64);
-            if (lane >= (uint32_t)off) incl += y;
-        }
-        if (lane == 63) wsum[w] = incl;
-        __syncthreads();
-        uint32_t woff = 0;
-        for (uint32_t k = 0; k < w; k++) woff += wsum[k];
-        uint32_t run = woff + incl - local;
-        __syncthreads();
-        for (uint32_t t = beg; t < end; t++) {
-            uint32_t v = scan[t];
-            scan[t] = run;
-            run += v;
-        }
-        if (threadIdx.x == blockDim.x - 1) scan[PP] = run;
-    }
-    __syncthreads();
-    const uint32_t nvalid = scan[PP];
-    if (nvalid == 0) return;   // uniform
-    // proposals: pos = #valid before + #existing with dist <= d
-    uint32_t upd = 0;
-    for (uint32_t t = threadIdx.x; t < PP; t += blockDim.x) {
-        unsigned long long key = keys[t];
-        if (key == ~0ull) continue;
-        bool ok = (scan[t + 1] - scan[t]) != 0;
-        if (!ok) continue;
-        float dv = key_float((uint32_t)(key >> 32));
-        uint32_t lo = 0, hi = c;
-        while (lo < hi) {
-            uint32_t mid = (lo + hi) >> 1;
-            if (ld[mid] <= dv) lo = mid + 1; else hi = mid;
-        }
-        uint32_t pos = scan[t] + lo;
-        if (pos < K) {
-            ids[u * K + pos] = (uint32_t)key;
-            kd[u * K + pos] = dv;
-            isnew[u * K + pos] = 1;
-            upd++;
-        }
-    }
-    // existing entries: pos = e + #valid proposals with dist < ld[e]
-    for (uint32_t e = threadIdx.x; e < c; e += blockDim.x) {
-        uint32_t kf = float_key(ld[e]);
-        uint32_t lo = 0, hi = PP;
-        while (lo < hi) {
-            uint32_t mid = (lo + hi) >> 1;
-            if ((uint32_t)(keys[mid] >> 32) < kf) lo = mid + 1; else hi = mid;
-        }
-        uint32_t pos = e + scan[lo];
-        if (pos < K) {
-            ids[u * K + pos] = lid[e];
-            kd[u * K + pos] = ld[e];
-            isnew[u * K + pos] = 0;
-        }
+    for (uint32_t t = threadIdx.x; t < c; t += blockDim.x) {
+        ids[u * K + t] = lid[t];
+        kd[u * K + t] = ld[t];
+        isnew[u * K + t] = lnw[t];
     }
     for (int off = 32; off > 0; off >>= 1) upd += __shfl_down(upd, off, 64);
     if ((threadIdx.x & 63u) == 0 && upd) atomicAdd(&stats[ST_UPDATES], (unsigned long long)upd);
-    if (threadIdx.x == 0) cnt[u] = min(c + nvalid, K);
+    if (threadIdx.x == 0) cnt[u] = c;
 }
 
-void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t bucket, hrnn_nnd_iter_stats *st) {
+void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_nnd_iter_stats *st) {
     const uint64_t n = h->n;
     const uint32_t K = h->K;
     REQUIRE(S >= 1 && S <= 64, HRNN_E_ARG, "nndescent: sample_size must be in [1, 64]");
     REQUIRE(K <= kMergeMaxK, HRNN_E_ARG, "nndescent: K_knng must be <= 1024");
     REQUIRE(h->d % 32 == 0, HRNN_E_ARG, "nndescent: d must be a multiple of 32");
-    uint32_t P = bucket;
-    if (P == 0) {
-        P = 1;
-        while (P < 4 * K) P <<= 1;
-        P = std::min<uint32_t>(std::max<uint32_t>(P, 256), kMergeMaxP);
-    }
-    REQUIRE(P <= kMergeMaxP, HRNN_E_ARG, "nndescent: bucket must be <= 2048");
     const uint32_t R = (uint32_t)S;
+    const uint32_t RB = kJRev;   // reverse offers kept per node and kind
     const int exact = (h->small_int && h->d <= 128) ? 1 : 0;
+
+    // Proposal buffer: per_node_cap entries per node (default max(8192, 16K)),
+    // at most half of the free device memory.
+    uint64_t want = n * (uint64_t)(per_node_cap ? per_node_cap : std::max<uint32_t>(8192, 16 * K));
+    size_t free_b = 0, total_b = 0;
+    HIPC(hipMemGetInfo(&free_b, &total_b));
+    uint64_t budget = std::max<uint64_t>(h->s3.bytes, free_b / 2) / 8;
+    const uint64_t cap = std::max<uint64_t>(1, std::min(want, budget));
 
     // scratch layout (u32 words unless noted)
     DevBuf &w = h->s0;    // worst f32[n]
     DevBuf &fwd = h->s1;  // fn[n*S], fo[n*S]
-    DevBuf &rev = h->s2;  // rn[n*R], ro[n*R]
-    DevBuf &cnts = h->s4; // fn_cnt, fo_cnt, rn_cnt, ro_cnt, prop_cnt [n each]
-    DevBuf &prop = h->s3; // prop_id[n*P], prop_d[n*P]
+    DevBuf &rev = h->s2;  // rn[n*RB], ro[n*RB]
+    DevBuf &cnts = h->s4; // fn_cnt, fo_cnt, rn_cnt, ro_cnt, prop_cnt, prop_fill [n each]
+    DevBuf &prop = h->s3; // prop_id[cap], prop_d[cap]
+    const uint32_t nblk = scan_block_count(n);
     w.reserve(n * 4 + 64);
     fwd.reserve(n * S * 8 + 64);
-    rev.reserve(n * R * 8 + 64);
-    cnts.reserve(n * 20 + 64);
-    prop.reserve(n * (uint64_t)P * 8 + 64);
+    rev.reserve(n * RB * 8 + 64);
+    cnts.reserve(n * 24 + 64);
+    prop.reserve(cap * 8 + 64);
+    h->nnd_aux.reserve((n + 1 + nblk + 1) * 8 + 64);
     h->csr_valid = false;   // the graph changes: any CSR built before is stale
     float *worst = w.as<float>();
     uint32_t *fn = fwd.as<uint32_t>(), *fo = fn + n * S;
-    uint32_t *rn = rev.as<uint32_t>(), *ro = rn + n * R;
+    uint32_t *rn = rev.as<uint32_t>(), *ro = rn + n * RB;
     uint32_t *fn_cnt = cnts.as<uint32_t>(), *fo_cnt = fn_cnt + n, *rn_cnt = fo_cnt + n,
-             *ro_cnt = rn_cnt + n, *prop_cnt = ro_cnt + n;
+             *ro_cnt = rn_cnt + n, *prop_cnt = ro_cnt + n, *prop_fill = prop_cnt + n;
     uint32_t *prop_id = prop.as<uint32_t>();
-    float *prop_d = reinterpret_cast<float *>(prop_id + n * (uint64_t)P);
+    float *prop_d = reinterpret_cast<float *>(prop_id + cap);
+    uint64_t *prop_off = h->nnd_aux.as<uint64_t>();
+    uint64_t *bsum = prop_off + n + 1;
+    uint64_t *ptotal = bsum + nblk;
 
     uint32_t *ids = h->knn_ids.as<uint32_t>();
     float *kd = h->knn_d.as<float>();
@@ -462,23 +552,28 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t bucket, hrnn_nnd_ite
     auto enqueue_iteration = [&](int it) {
         unsigned long long *stats = all_stats + (uint64_t)it * ST_N;
         hipLaunchKernelGGL(k_nnd_prepare, dim3(ceil_div(n, 256)), dim3(256), 0, h->stream, kd, cnt, n,
-                           K, worst, rn_cnt, ro_cnt, prop_cnt, stats);
+                           K, worst, rn_cnt, ro_cnt, prop_cnt, prop_fill, stats);
         hipLaunchKernelGGL(k_nnd_sample, dim3(ceil_div(n, 4)), dim3(256), 0, h->stream, ids, kd, isnew,
-                           cnt, n, K, (uint32_t)S, R, worst, fn, fn_cnt, fo, fo_cnt, rn, rn_cnt, ro,
+                           cnt, n, K, (uint32_t)S, RB, worst, fn, fn_cnt, fo, fo_cnt, rn, rn_cnt, ro,
                            ro_cnt);
-        hipLaunchKernelGGL(k_nnd_join, dim3((unsigned)n), dim3(256), 0, h->stream,
+        hipLaunchKernelGGL(k_nnd_join<false>, dim3((unsigned)n), dim3(256), 0, h->stream,
                            h->xb.as<__hip_bfloat16>(), h->norms.as<float>(), h->x.as<float>(), h->d,
-                           n, (uint32_t)S, R, fn, fn_cnt, fo, fo_cnt, rn, rn_cnt, ro, ro_cnt, worst,
-                           prop_cnt, prop_id, prop_d, P, exact, stats);
+                           n, (uint32_t)S, R, RB, fn, fn_cnt, fo, fo_cnt, rn, rn_cnt, ro, ro_cnt, worst,
+                           prop_cnt, prop_off, prop_id, prop_d, cap, exact, stats);
+        scan_u32(h, prop_cnt, n, prop_off, bsum, ptotal);
+        hipLaunchKernelGGL(k_nnd_join<true>, dim3((unsigned)n), dim3(256), 0, h->stream,
+                           h->xb.as<__hip_bfloat16>(), h->norms.as<float>(), h->x.as<float>(), h->d,
+                           n, (uint32_t)S, R, RB, fn, fn_cnt, fo, fo_cnt, rn, rn_cnt, ro, ro_cnt, worst,
+                           prop_fill, prop_off, prop_id, prop_d, cap, exact, stats);
         hipLaunchKernelGGL(k_nnd_merge, dim3((unsigned)n), dim3(256), 0, h->stream, ids, kd, isnew,
-                           cnt, K, prop_cnt, prop_id, prop_d, P, stats);
+                           cnt, K, prop_cnt, prop_off, prop_id, prop_d, cap, stats);
     };
 
     const char *env = std::getenv("HRNN_NND_GRAPH");
     const bool use_graph = !(env && env[0] == '0') && iters > 0;
     std::vector<unsigned long long> sv((uint64_t)std::max(iters, 1) * ST_N, 0);
     if (use_graph) {
-        // All iterations captured as one HIP graph: 4 kernels per iteration, no
+        // All iterations captured as one HIP graph: 8 kernels per iteration, no
         // host synchronisation in between; counters are reset by kernels only.
         auto t0 = std::chrono::steady_clock::now();
         hipGraph_t graph = nullptr;

@@ -100,14 +100,16 @@ typedef struct hrnn_nnd_iter_stats {
     uint64_t dists;          /* distance evaluations in the local join */
     uint64_t updates;        /* entries inserted into K-lists (both directions) */
     uint64_t proposals;      /* proposals that passed the threshold filter */
-    uint64_t dropped;        /* proposals lost to a full proposal bucket */
+    uint64_t dropped;        /* proposals beyond the proposal buffer (0 unless it overflows) */
     double ms;               /* wall time of the iteration */
 } hrnn_nnd_iter_stats;
 
 /* num_iters rounds of UPDATE (sample S new / S old, reverse samples capped at
- * R = S) + JOIN (new x new, new x old) + MERGE on the resident graph.
- * bucket = proposal capacity per node and iteration (0 = default).  stats may
- * be NULL or point to num_iters records. */
+ * R = S, the smallest source ids of up to 256 offers kept) + JOIN (new x new, new x old) + MERGE
+ * on the resident graph.  Every proposal of an iteration is kept in a proposal
+ * CSR of bucket * n entries (bucket = 0: max(8192, 16 K)), capped at half of
+ * the free device memory; proposals beyond it are counted as dropped.  stats
+ * may be NULL or point to num_iters records. */
 int hrnn_dev_nndescent(hrnn_dev_t h, int num_iters, int sample_size, uint32_t bucket,
                        hrnn_nnd_iter_stats *stats);
 

@@ -58,6 +58,8 @@ def _lib(variant: str | None = None) -> C.CDLL:
     lib.ref_set_raw.argtypes = [vp, vp, u64]
     lib.ref_count.restype = u64
     lib.ref_count.argtypes = [vp]
+    lib.ref_knn_depth.restype = u32
+    lib.ref_knn_depth.argtypes = [vp]
     lib.ref_set_knng.argtypes = [vp, vp, vp, vp]
     lib.ref_get_knng.argtypes = [vp, vp, vp, vp]
     lib.ref_refine.argtypes = [vp, i, i, i]
@@ -196,7 +198,7 @@ class RefIndex:
             raise RuntimeError(lib.ref_last_error().decode())
         obj = cls.__new__(cls)
         obj.lib, obj.d, obj.h = lib, d, h
-        obj.K = 0
+        obj.K = int(lib.ref_knn_depth(h))
         return obj
 
     def build_rknng_mutable(self):

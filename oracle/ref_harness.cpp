@@ -100,6 +100,8 @@ int ref_set_raw(void *h, const float *x, uint64_t n) {
 
 uint64_t ref_count(void *h) { return static_cast<RefIndex *>(h)->idx->cur_element_count; }
 
+uint32_t ref_knn_depth(void *h) { return (uint32_t)static_cast<RefIndex *>(h)->idx->K_knng_; }
+
 int ref_set_knng(void *h, const uint32_t *ids, const float *dists, const uint16_t *cnt) {
     auto *r = static_cast<RefIndex *>(h);
     return guarded([&] {

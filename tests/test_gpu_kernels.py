@@ -268,6 +268,7 @@ def test_nndescent_recall_parity(small_data, small_ref):
     dev.set_base(x)
     dev.set_knng(sid, sd, sc)
     st = dev.nndescent(iters, S)
+    assert all(t["dropped"] == 0 for t in st)
     gid, gd, gc = dev.get_knng()
     exact, _ = oracle.knn(x, x, K + 1)
     exact = exact[:, 1:].copy()
@@ -283,6 +284,40 @@ def test_nndescent_recall_parity(small_data, small_ref):
         dd = ((x[gid[i, :c]].astype(np.int64) - x[i].astype(np.int64)) ** 2).sum(1)
         assert np.array_equal(dd.astype(np.float32), gd[i, :c])
     assert st[0]["dists"] > 0 and st[0]["updates"] > 0
+
+
+def test_nndescent_first_iteration_exact(small_data, small_ref):
+    """One iteration from the same seeds: the GPU evaluates exactly the pairs
+    of the oracle's single-threaded join (same forward and reverse samples)
+    and every node ends with exactly the oracle's sorted distance list (the
+    top-K of list U proposals is unique as distances; ids may differ only
+    among equal distances).  Nothing is dropped."""
+    x, _ = small_data
+    r, seed = small_ref
+    sid, sd, sc = seed
+    K, S = r.K, 12
+    n = x.shape[0]
+    # reverse offers per node in this iteration (all seed entries are new)
+    worst = np.where(sc >= K, sd[np.arange(n), np.maximum(sc.astype(np.int64) - 1, 0)], np.inf)
+    offers = np.zeros(n, np.int64)
+    for i in range(n):
+        c = int(sc[i])
+        nb = sid[i, :c]
+        np.add.at(offers, nb[sd[i, :c] > worst[nb]], 1)
+    assert offers.max() <= 256        # every offer is kept: the reverse samples match exactly
+    oid, od, oc, ou, odist = oracle.nndescent(x, sid, sd, sc, K, 1, S)
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    dev.set_knng(sid, sd, sc)
+    st = dev.nndescent(1, S)
+    gid, gd, gc = dev.get_knng()
+    assert st[0]["dropped"] == 0
+    assert st[0]["dists"] == int(odist[0])
+    assert np.array_equal(gc, oc)
+    m = np.arange(K) < gc[:, None]
+    assert np.array_equal(np.where(m, gd, 0), np.where(m, od, 0))
+    same_ids = float((np.where(m, gid, 0) == np.where(m, oid, 0)).all(1).mean())
+    print("nodes with identical id lists:", same_ids)
 
 
 def test_nndescent_graph_capture_matches_eager(small_data, small_ref, monkeypatch):

@@ -104,15 +104,17 @@ def test_learned_m_gain_host_features(dev):
     assert np.isfinite(ft).all()
     gt_off, gt_ids, _ = d.bf_rknn(qt, kth, "bf16")
     labels, rec_grid = label_queries(d, qt, pt, gt_off, gt_ids, k, Kp)
-    model, losses = LearnedM.train(d, ft, labels, epochs=60, batch=128, lr=2e-3)
+    tr, va = slice(0, 1500), slice(1500, 2000)      # fit / choose tau on separate queries
+    model, losses = LearnedM.train(d, ft[tr], labels[tr], epochs=60, batch=128, lr=2e-3)
     assert losses[-1] < losses[0]
-    # tau from the training queries: the smallest one that keeps fixed m = 50's recall
-    pt_probs = model.probs(d, ft)
-    rec50_t = rec_grid[list(GRID).index(50)].mean()
+    # tau: the smallest one that keeps fixed m = 50's recall on the validation queries
+    pv = model.probs(d, ft[va])
+    rv = rec_grid[:, va]
+    rec50_v = rv[list(GRID).index(50)].mean()
     tau_star = 0.995
     for tau in (0.5, 0.7, 0.8, 0.9, 0.95, 0.98, 0.995):
-        gi = np.searchsorted(GRID, model.choose_m(pt_probs, tau))
-        if rec_grid[gi, np.arange(len(gi))].mean() >= rec50_t - 0.005:
+        gi = np.searchsorted(GRID, model.choose_m(pv, tau))
+        if rv[gi, np.arange(len(gi))].mean() >= rec50_v - 0.002:
             tau_star = tau
             break
     fe, pe = ix.query_features(qe, k, ef, 64)

@@ -182,3 +182,33 @@ def test_phase_a_proxies_equal_reference_on_same_file(tmp_path):
     for m, ef in [(1, 10), (10, 64), (20, 200), (50, 200), (64, 64)]:
         a, b = ix.proxies(q, m, ef, 1), r.proxies(q, m, ef)
         assert np.array_equal(np.sort(a, 1), np.sort(b, 1)), (m, ef)
+
+
+@needs_ref
+def test_saved_index_search_matches_reference(tmp_path):
+    """An index built here (host HNSW, KNN lists refined by the oracle's
+    NNDescent and set through set_knng) is saved, loaded by the reference,
+    and searched by it: its answers equal phases B+C of the oracle on our
+    host phase-A proxies for every query (empty answers included)."""
+    x = mixture(3000, seed=61, centre_seed=1061)
+    q = mixture(80, seed=62, centre_seed=1061)
+    ix = Index(3000, 128, 16, 100, 100, 48)
+    ix.add_points(x, nthreads=1)
+    sid, sd, sc = ix.get_knng()
+    oid, od, oc, _, _ = oracle.nndescent(x, sid, sd, sc, 48, 3, 12)
+    ix.set_knng(oid, od, oc)
+    ix.build_rknng_mutable()
+    p = str(tmp_path / "a.idx")
+    ix.save(p)
+    r = oracle.RefIndex.load(p, 128)
+    rid, rd, rc = r.get_knng()
+    assert np.array_equal(rc, oc)
+    m = np.arange(48) < oc[:, None]
+    assert np.array_equal(np.where(m, rid, 0), np.where(m, oid, 0))
+    r.build_rknng()
+    roff, rlab = r.search(q, 10, 1.0, 10, 48, 64)
+    prox = ix.proxies(q, 10, 64, 1)
+    coff, cent, rb, wide = oracle.csr(oid, oc, 48)
+    eoff, eids = oracle.search_bc(coff, cent, wide, rb, od, oc, 48, x, q, prox, 10, 48)
+    for i in range(len(q)):
+        assert set(eids[eoff[i]:eoff[i + 1]].tolist()) == set(rlab[roff[i]:roff[i + 1]].tolist()), i
