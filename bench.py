@@ -329,8 +329,11 @@ def main() -> int:
         f"distinct candidates/query {wk[:, 1].mean():.0f}")
     gpu_only_qps = nq / (search_ms * 1e-3)
 
-    # ---- cpu baseline: the reference's searchRknn, one thread, same index --------------
-    cpu = None
+    # ---- cpu baseline: the reference's searchRknn on this run's index ----------------------
+    # All host cores (OpenMP over queries, the reference's const re-entrant
+    # searchRknn) on every query, plus one thread (the hrnn_search driver loop)
+    # on a bounded sample.
+    cpu, cpu1 = None, None
     if not args.no_cpu_baseline:
         try:
             import oracle
@@ -344,7 +347,10 @@ def main() -> int:
                 os.remove(path)
                 os.rmdir(tmpdir)
                 log(f"reference loaded this run's index in {time.time() - t:.1f}s")
-                budget_s, done, spent, bs = 20.0, 0, 0.0, 25
+                t1 = time.perf_counter()
+                r.search(q, m_star, 1.0, k, Kp, ef, nthreads=threads)
+                all_s = time.perf_counter() - t1
+                budget_s, done, spent, bs = 10.0, 0, 0.0, 25
                 while spent < budget_s and done < nq:
                     qq = q[done:done + bs]
                     t1 = time.perf_counter()
@@ -352,12 +358,15 @@ def main() -> int:
                     spent += time.perf_counter() - t1
                     done += qq.shape[0]
                 r.close()
-                cpu = {"value": round(done / spent, 2), "unit": "queries/s", "cores": 1,
+                cpu = {"value": round(nq / all_s, 2), "unit": "queries/s", "cores": threads,
                        "kind": "reference",
-                       "sample": f"first {done} of the {nq} queries, reference searchRknn at m={m_star}, "
-                                 f"K'={Kp}, ef={ef}, one thread (the hrnn_search convention), on this "
+                       "sample": f"all {nq} queries, reference searchRknn at m={m_star}, K'={Kp}, "
+                                 f"ef={ef}, OpenMP over queries on {threads} host threads, on this "
                                  f"run's index loaded from its saved file"}
-                log(f"cpu baseline: {cpu['value']} QPS on {done} queries")
+                cpu1 = {"value": round(done / spent, 2), "unit": "queries/s", "cores": 1,
+                        "kind": "reference",
+                        "sample": f"first {done} queries, one thread (the hrnn_search driver loop)"}
+                log(f"cpu baseline: {cpu['value']} QPS on {threads} threads; {cpu1['value']} QPS on one")
         except Exception as e:  # reported when available, never required
             log(f"cpu baseline skipped: {e}")
 
@@ -385,6 +394,7 @@ def main() -> int:
         "phase_a_host": {"threads": threads, "sec_all_queries_m_max": round(phase_a_s, 3),
                          "qps": round(nq / phase_a_s, 1)},
         "cpu_baseline": cpu,
+        "cpu_baseline_one_thread": cpu1,
         "paths": paths,
         "learned_m": learned_info,
         "autotune": {kk: v for kk, v in tuned.items() if kk != "timings_ms"},

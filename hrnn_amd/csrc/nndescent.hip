@@ -508,12 +508,12 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
     const uint32_t RB = kJRev;   // reverse offers kept per node and kind
     const int exact = (h->small_int && h->d <= 128) ? 1 : 0;
 
-    // Proposal buffer: per_node_cap entries per node (default max(8192, 16K)),
-    // at most half of the free device memory.
-    uint64_t want = n * (uint64_t)(per_node_cap ? per_node_cap : std::max<uint32_t>(8192, 16 * K));
+    // Proposal buffer: per_node_cap entries per node (default max(16384, 32K)),
+    // at most 60% of the free device memory (8 B per proposal).
+    uint64_t want = n * (uint64_t)(per_node_cap ? per_node_cap : std::max<uint32_t>(16384, 32 * K));
     size_t free_b = 0, total_b = 0;
     HIPC(hipMemGetInfo(&free_b, &total_b));
-    uint64_t budget = std::max<uint64_t>(h->s3.bytes, free_b / 2) / 8;
+    uint64_t budget = std::max<uint64_t>(h->s3.bytes, free_b / 10 * 6) / 8;
     const uint64_t cap = std::max<uint64_t>(1, std::min(want, budget));
 
     // scratch layout (u32 words unless noted)

@@ -1037,7 +1037,7 @@ class HRNN : public AlgorithmInterface<dist_t> {
         std::vector<DP> sel = select_diverse(std::move(cand), M_);
         const size_t cap = level ? maxM_ : maxM0_;
         {
-            std::lock_guard<std::mutex> lk(node_locks_[cur]);
+            // cur's own lock is held by insert_internal for the whole insertion
             uint32_t *ll = links_at(cur, level);
             set_count(ll, (unsigned)sel.size());
             for (size_t t = 0; t < sel.size(); t++) ll[1 + t] = sel[t].second;
@@ -1075,6 +1075,11 @@ class HRNN : public AlgorithmInterface<dist_t> {
             label_map_[label] = cur;
             cur_element_count++;
         }
+        // Held until cur is linked at every level (as the reference's lock_el,
+        // hrnn.h:1411): a concurrent insertion that reaches cur through an
+        // upper layer waits here instead of starting its layer-0 beam from a
+        // node whose layer-0 list is still empty.
+        std::unique_lock<std::mutex> lock_el(node_locks_[cur]);
         int lvl = random_level();
         std::memcpy(vecs_.data() + (size_t)cur * dim_, x, data_size_);
         labels_[cur] = label;

@@ -107,7 +107,7 @@ typedef struct hrnn_nnd_iter_stats {
 /* num_iters rounds of UPDATE (sample S new / S old, reverse samples capped at
  * R = S, the smallest source ids of up to 256 offers kept) + JOIN (new x new, new x old) + MERGE
  * on the resident graph.  Every proposal of an iteration is kept in a proposal
- * CSR of bucket * n entries (bucket = 0: max(8192, 16 K)), capped at half of
+ * CSR of bucket * n entries (bucket = 0: max(16384, 32 K)), capped at 60% of
  * the free device memory; proposals beyond it are counted as dropped.  stats
  * may be NULL or point to num_iters records. */
 int hrnn_dev_nndescent(hrnn_dev_t h, int num_iters, int sample_size, uint32_t bucket,

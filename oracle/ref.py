@@ -168,7 +168,7 @@ class RefIndex:
         q = np.ascontiguousarray(q, np.float32)
         nq = q.shape[0]
         off = np.zeros(nq + 1, np.uint64)
-        cap = 1 << 16
+        cap = max(1 << 16, 64 * nq)
         while True:
             lab = np.empty(cap, np.uint64)
             need = C.c_uint64(0)

@@ -212,3 +212,18 @@ def test_saved_index_search_matches_reference(tmp_path):
     eoff, eids = oracle.search_bc(coff, cent, wide, rb, od, oc, 48, x, q, prox, 10, 48)
     for i in range(len(q)):
         assert set(eids[eoff[i]:eoff[i + 1]].tolist()) == set(rlab[roff[i]:roff[i + 1]].tolist()), i
+
+
+def test_parallel_build_seeds_every_node():
+    """Concurrent addPoint (8 threads): a node reached through an upper layer
+    while it is still being inserted is waited for (the reference holds the
+    node's lock for its whole insertion), so no insertion starts its layer-0
+    beam from an unlinked node.  Seed lists are then full for all but the
+    first few nodes, as in a single-threaded build."""
+    x = hrnn_amd.synth_mixture(1, 10_000, 128, 50, 20.0, 0.0, 255.0, True, 0.0, 1001)
+    for nt in (1, 8):
+        ix = Index(10_000, 128, 16, 200, 100, 100)
+        ix.add_points(x, nthreads=nt)
+        _, _, sc = ix.get_knng()
+        late = np.arange(10_000) >= 200          # past the first few hundred insertions
+        assert int((sc[late] < 50).sum()) <= 2, (nt, np.bincount(np.minimum(sc, 100))[:10])
