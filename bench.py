@@ -11,17 +11,19 @@ Before timing: host HNSW build with KNN seeding; GPU refineKNNG (timed: the
 NNDescent build metric); GPU reverse CSR; exact groundtruth from the GPU
 brute force; an m sweep picking the cheapest m with macro recall >= 0.95; a
 learned-m model trained on separate queries.  A timed step is one whole
-search of all queries, in batches of 1000: phase A (HNSW descent + layer-0
-beam) on the host cores, the proxies copied to HBM, phases B and C on the GPU
-with queries resident in HBM and results left there; the host computes the
-next batch's phase A while the GPU searches the previous one.  Three paths are
-timed: FP32 vectors with fixed m, BF16 vectors (exact for these integer data)
-with fixed m, and BF16 with the learned per-query m (host features, GPU MLP).
+search of all queries.  GPU paths: queries resident in HBM, phase A (HNSW
+descent + layer-0 beam, k_phase_a) then phases B and C (k_rknn_search) in one
+call, results left in HBM; FP32 vectors (fp32_fixed_m) or the BF16 mirror,
+exact for these integer data (bf16_fixed_m).  Host-phase-A paths (the
+reference's structure): phase A on the host cores in batches of 1000, proxies
+copied to HBM, phases B and C on the GPU, the host working on batch i+1 while
+the GPU searches batch i; with fixed m (host_a_bf16_fixed_m) or the learned
+per-query m (bf16_learned_m_host_a: host features, GPU MLP, GPU B and C).
 value = the highest QPS among the paths whose macro recall is >= 0.95.
 
-roofline: the GPU kernel of the path, k_rknn_search, timed with HIP events on
-the library's stream over launches on resident proxies (algorithmic bytes
-from the kernel's own work counters).  cpu_baseline: the reference's own searchRknn (compiled from its
+roofline: the dominant GPU kernel of that path (k_phase_a or k_rknn_search),
+timed with HIP events on the library's stream over resident launches
+(algorithmic bytes from the kernels' own work counters).  cpu_baseline: the reference's own searchRknn (compiled from its
 sources, oracle/_ref), single-threaded as its hrnn_search driver runs it, on a
 bounded query sample of this run's index (loaded from its saved file).
 """
@@ -216,14 +218,31 @@ def main() -> int:
         hiprt.stream_sync(stream)
         return (time.perf_counter() - t0) / args.steps
 
-    # ---- m sweep: the first m of m_max beam proxies (same beam, ef >= m) ------------------
+    ed = hiprt.DeviceArray((2 * nq,), np.uint32)
+
+    def run_gpu(m):
+        """One step, all on the GPU with queries resident: phase A (k_phase_a)
+        into pd, then phases B and C (k_rknn_search); results stay in HBM."""
+        ck(L.hrnn_dev_rknn_query_dptr(h, qd.ptr, nq, m, max(ef, m), None, k, Kp, 1.0, pd.ptr, cd.ptr,
+                                      idd.ptr, cap, wd.ptr))
+
+    # ---- phase A on the GPU and on the host; agreement of the proxy sets ---------------------
+    ck(L.hrnn_dev_set_vector_precision(h, 1))
+    ck(L.hrnn_dev_proxies_dptr(h, qd.ptr, nq, m_max, max(ef, m_max), pd.ptr, ed.ptr))
+    ck(L.hrnn_dev_sync(h))
+    gprox = pd.to_host()[: nq * m_max].reshape(nq, m_max).copy()
     t = time.time()
     prox_all = ix.proxies(q, m_max, ef, threads)
     phase_a_s = time.time() - t
-    log(f"host phase A for {nq} queries (m={m_max}, ef={ef}, {threads} threads): {phase_a_s:.2f}s")
+    agree = float(np.mean([len(set(gprox[i].tolist()) & set(prox_all[i].tolist())) / m_max
+                           for i in range(nq)]))
+    log(f"host phase A for {nq} queries (m={m_max}, ef={ef}, {threads} threads): {phase_a_s:.2f}s; "
+        f"GPU/host proxy agreement {agree:.4f}")
+
+    # ---- m sweep on the GPU proxies: the first m of m_max (same beam, ef >= m) -------------
     sweep, m_star = [], None
     for m in cfg["ms"]:
-        pd.copy_from_host_async(np.ascontiguousarray(prox_all[:, :m]), stream, 0)
+        pd.copy_from_host_async(np.ascontiguousarray(gprox[:, :m]), stream, 0)
         search_batch(0, nq, m)
         r_off, r_ids = results()
         rec = macro_recall(gt_off, gt_ids, r_off, r_ids)
@@ -233,25 +252,30 @@ def main() -> int:
             m_star = m
     if m_star is None:
         m_star = cfg["ms"][-1]
-    rec_star = next(s["recall"] for s in sweep if s["m"] == m_star)
 
     # ---- launch shape of the persistent search kernel from the autotuner ------------------
     from hrnn_amd import autotune
-    pd.copy_from_host_async(np.ascontiguousarray(prox_all[:, :m_star]), stream, 0)
+    pd.copy_from_host_async(np.ascontiguousarray(gprox[:, :m_star]), stream, 0)
     tuned = autotune.tune(h, qd, nq, m_star, k, Kp, dict(n=n, d=d, K=K, m=m_star, nq=nq), pd, cd, idd, cap,
                           use_cache=False)
     log(f"autotune: search_wg_per_cu={tuned['search_wg_per_cu']}  {tuned['timings_ms']}")
 
     paths = {}
-    ck(L.hrnn_dev_set_vector_precision(h, 0))          # FP32 vectors
-    run_fixed(m_star)
+    ck(L.hrnn_dev_set_vector_precision(h, 0))          # FP32 vectors in phases A, B, C
+    run_gpu(m_star)
     fp32_off, fp32_ids = results()
-    paths["fp32_fixed_m"] = {"m": m_star, "recall": rec_star, "sec": timed(lambda: run_fixed(m_star))}
-    ck(L.hrnn_dev_set_vector_precision(h, 1))          # BF16 mirror (exact here)
-    run_fixed(m_star)
+    rec = round(macro_recall(gt_off, gt_ids, fp32_off, fp32_ids), 5)
+    paths["fp32_fixed_m"] = {"m": m_star, "recall": rec, "phase_a": "gpu", "sec": timed(lambda: run_gpu(m_star))}
+    ck(L.hrnn_dev_set_vector_precision(h, 1))          # BF16 mirror (exact for these data)
+    run_gpu(m_star)
     bf_off, bf_ids = results()
     assert np.array_equal(fp32_off, bf_off) and np.array_equal(fp32_ids, bf_ids), "BF16 path differs"
-    paths["bf16_fixed_m"] = {"m": m_star, "recall": rec_star, "sec": timed(lambda: run_fixed(m_star))}
+    paths["bf16_fixed_m"] = {"m": m_star, "recall": rec, "phase_a": "gpu", "sec": timed(lambda: run_gpu(m_star))}
+    ck(L.hrnn_dev_sync(h))
+    run_fixed(m_star)                                   # host phase A, GPU phases B and C
+    ha_off, ha_ids = results()
+    paths["host_a_bf16_fixed_m"] = {"m": m_star, "recall": round(macro_recall(gt_off, gt_ids, ha_off, ha_ids), 5),
+                                    "phase_a": "host", "sec": timed(lambda: run_fixed(m_star))}
 
     # ---- learned m: train on separate queries, pick tau on them, evaluate here -------------
     learned_info = None
@@ -280,8 +304,8 @@ def main() -> int:
         l_off, l_ids = results()
         rec_l = macro_recall(gt_off, gt_ids, l_off, l_ids)
         mean_m = float(md.to_host().mean())
-        paths["bf16_learned_m"] = {"tau": tau_star, "recall": round(rec_l, 5), "mean_m": round(mean_m, 2),
-                                   "sec": timed(lambda: run_learned(tau_star))}
+        paths["bf16_learned_m_host_a"] = {"tau": tau_star, "recall": round(rec_l, 5), "mean_m": round(mean_m, 2),
+                                          "phase_a": "host", "sec": timed(lambda: run_learned(tau_star))}
         learned_info = {"train_queries": cfg["ntrain"], "epochs": 60, "loss_first": round(float(losses[0]), 4),
                         "loss_last": round(float(losses[-1]), 4), "tau": tau_star, "mean_m": round(mean_m, 2),
                         "label_hist": np.bincount(lab, minlength=16).tolist(),
@@ -296,38 +320,56 @@ def main() -> int:
     best_name = max(ok or paths, key=lambda kk: (ok or paths)[kk]["qps"])
     best = paths[best_name]
 
-    # ---- roofline of the GPU kernel of the best path (HIP events, resident inputs) ----------
-    # Phase A runs on the host; the GPU kernel of every path is k_rknn_search.
+    # ---- roofline: the GPU kernels of the best path, HIP events on the library stream -------
     reps = 20
     e0, e1 = hiprt.Event(), hiprt.Event()
-    if best_name == "bf16_learned_m":
+    s_v = 4 if best_name == "fp32_fixed_m" else 2
+    ck(L.hrnn_dev_set_vector_precision(h, 0 if best_name == "fp32_fixed_m" else 1))
+    m_b = best.get("m", m_max)
+    kern = {}
+    if best.get("phase_a") == "gpu":
+        ck(L.hrnn_dev_proxies_dptr(h, qd.ptr, nq, m_b, max(ef, m_b), pd.ptr, ed.ptr))
+        hiprt.stream_sync(stream)
+        e0.record(stream)
+        for _ in range(reps):
+            ck(L.hrnn_dev_proxies_dptr(h, qd.ptr, nq, m_b, max(ef, m_b), pd.ptr, ed.ptr))
+        e1.record(stream)
+        e1.sync()
+        beam_ms = e1.ms_since(e0) / reps
+        ex = ed.to_host().reshape(nq, 2).astype(np.int64)
+        # per query: the query, each expanded node's layer-0 row (count + 2M ids),
+        # each evaluated vector, the m proxy ids written
+        beam_bytes = float((4 * d + ex[:, 0] * 4 * (2 * cfg["M"] + 1) + ex[:, 1] * s_v * d + 4 * m_b).sum())
+        kern["k_phase_a"] = {"ms": beam_ms, "bytes": beam_bytes,
+                             "expansions_per_query": round(float(ex[:, 0].mean()), 1),
+                             "evaluations_per_query": round(float(ex[:, 1].mean()), 1)}
+    else:
+        pd.copy_from_host_async(np.ascontiguousarray(prox_all[:, :m_b]), stream, 0)
+    if best_name.startswith("bf16_learned_m"):
         run_learned(best["tau"])            # leaves the masked proxies in pd (stride m_max)
         hiprt.stream_sync(stream)
-        prox_m = m_max
-    else:
-        prox_m = best["m"]
-        pd.copy_from_host_async(np.ascontiguousarray(prox_all[:, :prox_m]), stream, 0)
-    ck(L.hrnn_dev_set_vector_precision(h, 0 if best_name == "fp32_fixed_m" else 1))
-    search_batch(0, nq, prox_m)
+    search_batch(0, nq, m_b)
     hiprt.stream_sync(stream)
     e0.record(stream)
     for _ in range(reps):
-        search_batch(0, nq, prox_m)
+        search_batch(0, nq, m_b)
     e1.record(stream)
     e1.sync()
     search_ms = e1.ms_since(e0) / reps
     wk = wd.to_host().reshape(nq, 2).astype(np.int64)
     rc_ = cd.to_host().astype(np.int64)
-    s_v = 4 if best_name == "fp32_fixed_m" else 2
     # per query: the query vector, 16 B per proxy (CSR offsets), 4 B per scanned
     # reverse-list entry, and per distinct candidate its vector, r_k and mark word,
     # plus 4 B per result id
-    search_bytes = float((4 * d + 16 * prox_m + 4 * wk[:, 0] + wk[:, 1] * (s_v * d + 4 + 4) + 4 * rc_).sum())
-    achieved = search_bytes / (search_ms * 1e-3) / 1e9
-    log(f"  k_rknn_search: {search_ms:.3f} ms/launch over {nq} queries, "
-        f"{achieved:.1f} GB/s algorithmic; scanned/query {wk[:, 0].mean():.0f}, "
-        f"distinct candidates/query {wk[:, 1].mean():.0f}")
-    gpu_only_qps = nq / (search_ms * 1e-3)
+    search_bytes = float((4 * d + 16 * m_b + 4 * wk[:, 0] + wk[:, 1] * (s_v * d + 4 + 4) + 4 * rc_).sum())
+    kern["k_rknn_search"] = {"ms": search_ms, "bytes": search_bytes,
+                             "scanned_per_query": round(float(wk[:, 0].mean()), 1),
+                             "candidates_per_query": round(float(wk[:, 1].mean()), 1)}
+    for kk, v in kern.items():
+        v["gbs"] = v["bytes"] / (v["ms"] * 1e-3) / 1e9
+        log(f"  {kk}: {v['ms']:.3f} ms/launch over {nq} queries, {v['gbs']:.1f} GB/s algorithmic")
+    dom = max(kern, key=lambda kk: kern[kk]["ms"])
+    achieved = kern[dom]["gbs"]
 
     # ---- cpu baseline: the reference's searchRknn on this run's index ----------------------
     # All host cores (OpenMP over queries, the reference's const re-entrant
@@ -381,18 +423,20 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16" if best_name.startswith("bf16") else "f32",
+        "dtype": "f32" if best_name == "fp32_fixed_m" else "bf16",
         "data": "synthetic",
         "config": {"workload": f"hrnn-{args.config}-{n // 1000}k-x{d}", "n": n, "nq": nq, "d": d,
                    "K_knng": K, "M": cfg["M"], "ef_construction": cfg["efc"], "nn_iters": cfg["iters"],
                    "nn_sample": cfg["S"], "k": k, "K_prime": Kp, "ef_search": ef, "path": best_name,
                    "recall": best["recall"], "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_rknn_search",
-                     "kernel_ms": round(search_ms, 4), "alg_bytes_per_launch": int(search_bytes),
-                     "queries_per_launch": nq, "gpu_phases_bc_qps": round(gpu_only_qps, 1)},
-        "phase_a_host": {"threads": threads, "sec_all_queries_m_max": round(phase_a_s, 3),
-                         "qps": round(nq / phase_a_s, 1)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                     "kernel_ms": round(kern[dom]["ms"], 4), "alg_bytes_per_launch": int(kern[dom]["bytes"]),
+                     "queries_per_launch": nq,
+                     "kernels": {kk: {k2: (round(v2, 4) if isinstance(v2, float) else v2) for k2, v2 in v.items()}
+                                 for kk, v in kern.items()}},
+        "phase_a": {"gpu_host_proxy_agreement": round(agree, 5), "host_threads": threads,
+                    "host_sec_all_queries_m_max": round(phase_a_s, 3), "host_qps": round(nq / phase_a_s, 1)},
         "cpu_baseline": cpu,
         "cpu_baseline_one_thread": cpu1,
         "paths": paths,

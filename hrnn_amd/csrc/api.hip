@@ -31,6 +31,18 @@ void d2h(hrnn_dev_t h, void *dst, const void *src, size_t bytes) {
     HIPC(hipStreamSynchronize(h->stream));
 }
 
+void check_device_error(hrnn_dev_t h) {
+    uint32_t *w = h->counters.as<uint32_t>() + kErrWord;
+    uint32_t e = 0;
+    d2h(h, &e, w, 4);
+    if (e) {
+        HIPC(hipMemsetAsync(w, 0, 4, h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
+        throw Error{HRNN_E_STATE, "a kernel met invalid graph data (ids or counts out of range); "
+                                  "results of the last launches are not valid"};
+    }
+}
+
 // BF16/FP16 mirrors, FP32 row norms (sum of squares, sequential FMA order) and
 // the small-integer flag for rows [row0, row0 + rows).
 __global__ void k_base_mirrors(const float *__restrict__ x, __hip_bfloat16 *__restrict__ xb,
@@ -127,6 +139,8 @@ int hrnn_dev_open(int device, hrnn_dev_t *out) {
         h->n_cu = prop.multiProcessorCount;
         h->lds_per_block = prop.sharedMemPerBlock;
         h->counters.reserve(4096);
+        HIPC(hipMemsetAsync(h->counters.p, 0, 4096, h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
     });
     if (rc != HRNN_OK) {
         delete h;
@@ -146,7 +160,10 @@ const char *hrnn_dev_last_error(hrnn_dev_t h) { return h ? h->err.c_str() : "nul
 
 int hrnn_dev_sync(hrnn_dev_t h) {
     if (!h) return HRNN_E_ARG;
-    return guarded(h, [&] { HIPC(hipStreamSynchronize(h->stream)); });
+    return guarded(h, [&] {
+        HIPC(hipStreamSynchronize(h->stream));
+        check_device_error(h);
+    });
 }
 
 void *hrnn_dev_stream(hrnn_dev_t h) { return h ? (void *)h->stream : nullptr; }

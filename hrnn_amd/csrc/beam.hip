@@ -3,18 +3,22 @@
 // closest nodes as proxies.
 //
 // Reference: searchRknn_impl phase A (include/hnswlib/hrnn.h:1992-2019) with
-// searchBaseLayerST<true> (hrnn.h:485-616).  Ordering rules are those of the
-// host beam of this build (include/hnswlib/hrnn.h, beam_query): candidates
-// and results ordered by (dist, id); a node enters the result list when the
-// list holds fewer than ef nodes or its distance is strictly below the worst;
-// the closest unexpanded result is expanded next; the search ends when every
-// result is expanded (= the closest open candidate is farther than the worst
-// result).  Distances use the reference's 16-partial FMA order, so GPU and
-// host beams visit the same nodes.
+// searchBaseLayerST<true> (hrnn.h:485-616).  The result list is kept sorted
+// by (dist, id); a node enters it when the list holds fewer than ef nodes or
+// its distance is strictly below the worst; the closest unexpanded entry is
+// expanded next; the search ends when every entry is expanded (= the closest
+// open candidate is farther than the worst result).  Distances use the
+// reference's 16-partial FMA order, so GPU and host beams agree up to the
+// order of equal distances.
 //
-// One wave per query (persistent blocks of 64 lanes pull queries from a
-// device queue); the sorted result list lives in LDS; visited nodes are
-// marked in a per-block bitmap over the n base ids (cleared after each query).
+// k_phase_a: one query per workgroup of one wave (64 lanes), no state kept
+// between workgroups.  The sorted list lives in LDS; visited nodes go into an
+// LDS hash set of 8192 ids (linear probing).  When a probe run is exhausted
+// the node is treated as unseen and admission checks the list for it, so the
+// result does not depend on the table size.  Every id or count read from the
+// graph is range-checked; a bad one is skipped and raises the handle's device
+// error word (reported by hrnn_dev_sync).  The descent and the beam run
+// bounded loops.
 #include <algorithm>
 #include <cfloat>
 
@@ -24,212 +28,210 @@ namespace hrnn {
 
 constexpr int kBeamMaxEf = 512;
 constexpr int kBeamMaxDeg = 64;
+constexpr uint32_t kVisSlots = 8192;
+constexpr uint32_t kVisProbes = 32;
+
+// Insert id into the LDS set; true when the id was not there (or could not be
+// recorded: *lost is set then).
+__device__ __forceinline__ bool vis_insert(uint32_t *tab, uint32_t id, uint32_t *lost) {
+    uint32_t h = (id * 2654435761u) >> 19;   // 13 bits -> [0, 8192)
+    for (uint32_t p = 0; p < kVisProbes; p++) {
+        uint32_t old = atomicCAS(&tab[h], kInvalidId, id);
+        if (old == kInvalidId) return true;
+        if (old == id) return false;
+        h = (h + 1) & (kVisSlots - 1);
+    }
+    *lost = 1u;
+    return true;
+}
 
 template <class VT>
-__global__ __launch_bounds__(64) void k_beam(
-    const VT *__restrict__ x, uint32_t d, uint64_t n, const uint32_t *__restrict__ l0,
+__global__ __launch_bounds__(64) void k_phase_a(
+    const VT *__restrict__ x, uint32_t d, uint32_t n, const uint32_t *__restrict__ l0,
     uint32_t l0_words, const uint32_t *__restrict__ up_off, const uint32_t *__restrict__ up_pool,
-    uint32_t up_words, uint32_t entry, int maxlevel, const float *__restrict__ q, uint32_t nq,
-    uint32_t ef, uint32_t m, uint32_t *__restrict__ visited_all, uint64_t vis_words,
-    uint32_t *__restrict__ queue, uint32_t *__restrict__ proxies, uint32_t *__restrict__ expansions,
-    float *__restrict__ raw) {
-    __shared__ float s_d[2][kBeamMaxEf];
-    __shared__ uint32_t s_i[2][kBeamMaxEf];
-    __shared__ uint8_t s_x[2][kBeamMaxEf];   // expanded flag
+    uint64_t pool_words, uint32_t up_words, uint32_t entry, int maxlevel, const float *__restrict__ q,
+    uint32_t ef, uint32_t m, uint32_t *__restrict__ proxies, uint32_t *__restrict__ expansions,
+    float *__restrict__ raw, uint32_t *__restrict__ err) {
+    __shared__ float s_d[kBeamMaxEf];
+    __shared__ uint32_t s_i[kBeamMaxEf];
+    __shared__ uint8_t s_x[kBeamMaxEf];   // expanded flag
     __shared__ float s_nd[kBeamMaxDeg];
     __shared__ uint32_t s_ni[kBeamMaxDeg];
-    __shared__ uint32_t s_q;
+    __shared__ uint32_t s_vis[kVisSlots];
+    __shared__ uint32_t s_L, s_lost, s_bad;
     extern __shared__ float s_qv[];
 
     const uint32_t lane = threadIdx.x;
     const uint32_t g = lane & 15u, grp = lane >> 4;
-    uint32_t *vis = visited_all + (uint64_t)blockIdx.x * vis_words;
-
-    for (;;) {
-        if (lane == 0) s_q = atomicAdd(queue, 1u);
-        __syncthreads();
-        const uint32_t qi = s_q;
-        if (qi >= nq) break;
-        for (uint32_t e = lane; e < d; e += 64) s_qv[e] = q[(uint64_t)qi * d + e];
-        __syncthreads();
-
-        // ---- greedy descent over the upper layers ----
-        uint32_t cur = entry;
-        float cd;
-        {
-            float t = dist_row<VT>(s_qv, x, cur, d, g);
-            cd = __shfl(t, 0, 64);
-        }
-        const float d_entry = cd;
-        for (int level = maxlevel; level > 0; level--) {
-            for (;;) {
-                const uint32_t *ll = up_pool + (uint64_t)up_off[cur] + (uint64_t)(level - 1) * up_words;
-                uint32_t cnt = ll[0] & 0xFFFFu;
-                float best = cd;
-                uint32_t best_id = cur;
-                for (uint32_t b = 0; b < cnt; b += 4) {
-                    uint32_t t = b + grp;
-                    float dv = FLT_MAX;
-                    uint32_t id = 0;
-                    if (t < cnt) {   // uniform per 16-lane group
-                        id = ll[1 + t];
-                        dv = dist_row<VT>(s_qv, x, id, d, g);
-                    }
-                    // sequential scan order of the host: first strictly smaller wins
-                    for (uint32_t s = 0; s < 4; s++) {
-                        float ds = __shfl(dv, (int)(s * 16), 64);
-                        uint32_t is = __shfl(id, (int)(s * 16), 64);
-                        if (b + s < cnt && ds < best) {
-                            best = ds;
-                            best_id = is;
-                        }
-                    }
-                }
-                if (best_id == cur) break;
-                cur = best_id;
-                cd = best;
-            }
-        }
-
-        // ---- layer-0 beam ----
-        const float d_start = cd;
-        int buf = 0;
-        uint32_t L = 1;
-        if (lane == 0) {
-            s_d[0][0] = cd;
-            s_i[0][0] = cur;
-            s_x[0][0] = 0;
-        }
-        if (lane == 0) atomicOr(&vis[cur >> 5], 1u << (cur & 31u));
-        __syncthreads();
-        uint32_t n_exp = 0, n_eval = 1;
-        for (;;) {
-            // closest unexpanded entry of the sorted list
-            uint32_t pick = 0xFFFFFFFFu;
-            for (uint32_t base = 0; base < L; base += 64) {
-                uint32_t t = base + lane;
-                bool open = t < L && s_x[buf][t] == 0;
-                uint64_t mask = __ballot(open);
-                if (mask) {
-                    pick = base + (uint32_t)__ffsll((long long)mask) - 1;
-                    break;
-                }
-            }
-            if (pick == 0xFFFFFFFFu) break;
-            const uint32_t c = s_i[buf][pick];
-            __syncthreads();
-            if (lane == 0) s_x[buf][pick] = 1;
-            n_exp++;
-            const uint32_t *ll = l0 + (uint64_t)c * l0_words;
-            const uint32_t cnt = ll[0] & 0xFFFFu;
-            // neighbours not visited yet
-            uint32_t nb = (lane < cnt) ? ll[1 + lane] : 0u;
-            bool fresh = false;
-            if (lane < cnt) {
-                uint32_t bit = 1u << (nb & 31u);
-                fresh = (atomicOr(&vis[nb >> 5], bit) & bit) == 0;
-            }
-            uint64_t fm = __ballot(fresh);
-            uint32_t nf = (uint32_t)__popcll(fm);
-            n_eval += nf;
-            uint32_t slot = (uint32_t)__popcll(fm & lanemask_lt());
-            if (fresh) s_ni[slot] = nb;
-            __syncthreads();
-            // distances of the fresh neighbours, 4 at a time (one per 16-lane group)
-            for (uint32_t b = 0; b < nf; b += 4) {
-                uint32_t t = b + grp;
-                if (t < nf) {
-                    float dv = dist_row<VT>(s_qv, x, s_ni[t], d, g);
-                    if (g == 0) s_nd[t] = dv;
-                }
-            }
-            __syncthreads();
-            // admission: list not full, or strictly closer than the worst entry
-            // (candidates are admitted in the host's scan order, so the worst
-            // entry moves as earlier candidates are inserted; emulate with a
-            // sequential pass by lane 0 over at most 64 fresh neighbours)
-            if (lane == 0) {
-                uint32_t LL = L;
-                float *dd = s_d[buf];
-                uint32_t *ii = s_i[buf];
-                uint8_t *xx = s_x[buf];
-                for (uint32_t t = 0; t < nf; t++) {
-                    float dv = s_nd[t];
-                    uint32_t id = s_ni[t];
-                    if (LL >= ef && !(dv < dd[LL - 1])) continue;
-                    uint32_t p = (LL < ef) ? LL : LL - 1;
-                    if (LL < ef) LL++;
-                    while (p > 0 && pair_less(dv, id, dd[p - 1], ii[p - 1])) {
-                        dd[p] = dd[p - 1];
-                        ii[p] = ii[p - 1];
-                        xx[p] = xx[p - 1];
-                        p--;
-                    }
-                    dd[p] = dv;
-                    ii[p] = id;
-                    xx[p] = 0;
-                }
-                s_q = LL;   // reuse as broadcast slot
-            }
-            __syncthreads();
-            L = s_q;
-            __syncthreads();
-        }
-        // ---- output: the m closest ----
-        for (uint32_t t = lane; t < m; t += 64)
-            proxies[(uint64_t)qi * m + t] = (t < L) ? s_i[buf][t] : kInvalidId;
-        if (expansions && lane == 0) {
-            expansions[2 * (uint64_t)qi] = n_exp;
-            expansions[2 * (uint64_t)qi + 1] = n_eval;
-        }
-        if (raw) {   // per-query statistics for the learned m: see kRawWords
-            float *r = raw + (uint64_t)qi * kRawWords;
-            if (lane < 16) r[2 + lane] = (lane < L) ? s_d[buf][lane] : s_d[buf][L - 1];
-            if (lane == 0) {
-                r[0] = d_entry;
-                r[1] = d_start;
-                r[18] = (float)n_exp;
-                r[19] = (float)L;
-            }
-        }
-        // clear the visited bitmap for the next query
-        __syncthreads();
-        for (uint64_t w = lane; w < vis_words; w += 64) vis[w] = 0;
-        __syncthreads();
+    const uint64_t qi = blockIdx.x;
+    for (uint32_t e = lane; e < d; e += 64) s_qv[e] = q[qi * d + e];
+    for (uint32_t t = lane; t < kVisSlots; t += 64) s_vis[t] = kInvalidId;
+    if (lane == 0) {
+        s_lost = 0;
+        s_bad = 0;
     }
-}
+    __syncthreads();
 
-// The GPU beam is switched off: its first GPU run (2000 queries, ef 200, m 64,
-// with the learned-m feature kernel after it) ended in a memory fault whose
-// cause has not been found; a CPU emulation of this kernel on the same index
-// keeps every access in bounds.  Phase A runs on the host until the cause is
-// shown; every entry point that would launch k_beam returns HRNN_E_STATE.
-constexpr bool kBeamEnabled = false;
+    // ---- greedy descent over the upper layers ----
+    uint32_t cur = entry < n ? entry : 0;
+    float cd = __shfl(dist_row<VT>(s_qv, x, cur, d, g), 0, 64);
+    const float d_entry = cd;
+    for (int level = maxlevel; level > 0; level--) {
+        for (uint32_t step = 0; step < n; step++) {   // each move strictly lowers cd
+            const uint32_t uo = up_off[cur];
+            const uint64_t base = (uint64_t)uo + (uint64_t)(level - 1) * up_words;
+            if (uo == kInvalidId || base + up_words > pool_words) {
+                if (lane == 0) s_bad = 1u;
+                break;
+            }
+            const uint32_t *ll = up_pool + base;
+            const uint32_t cnt = min(ll[0] & 0xFFFFu, up_words - 1);
+            float best = cd;
+            uint32_t best_id = cur;
+            for (uint32_t b = 0; b < cnt; b += 4) {
+                const uint32_t t = b + grp;
+                float dv = FLT_MAX;
+                uint32_t id = cur;
+                if (t < cnt) {   // uniform per 16-lane group
+                    id = ll[1 + t];
+                    if (id < n) {
+                        dv = dist_row<VT>(s_qv, x, id, d, g);
+                    } else {
+                        if (g == 0) s_bad = 1u;
+                        id = cur;
+                    }
+                }
+                // the host's sequential scan: the first strictly smaller wins
+                for (uint32_t s = 0; s < 4; s++) {
+                    float ds = __shfl(dv, (int)(s * 16), 64);
+                    uint32_t is = __shfl(id, (int)(s * 16), 64);
+                    if (b + s < cnt && ds < best) {
+                        best = ds;
+                        best_id = is;
+                    }
+                }
+            }
+            if (best_id == cur) break;
+            cur = best_id;
+            cd = best;
+        }
+    }
+
+    // ---- layer-0 beam ----
+    const float d_start = cd;
+    if (lane == 0) {
+        s_d[0] = cd;
+        s_i[0] = cur;
+        s_x[0] = 0;
+        s_L = 1;
+        vis_insert(s_vis, cur, &s_lost);
+    }
+    __syncthreads();
+    uint32_t L = 1, n_exp = 0, n_eval = 1;
+    for (uint32_t iter = 0; iter <= n; iter++) {   // each pass expands one list entry
+        uint32_t pick = kInvalidId;
+        for (uint32_t b0 = 0; b0 < L; b0 += 64) {
+            const uint32_t t = b0 + lane;
+            const uint64_t mask = __ballot(t < L && s_x[t] == 0);
+            if (mask) {
+                pick = b0 + (uint32_t)__ffsll((long long)mask) - 1;
+                break;
+            }
+        }
+        if (pick == kInvalidId) break;
+        const uint32_t c = s_i[pick];
+        __syncthreads();
+        if (lane == 0) s_x[pick] = 1;
+        n_exp++;
+        const uint32_t *ll = l0 + (uint64_t)c * l0_words;
+        const uint32_t cnt = min(ll[0] & 0xFFFFu, l0_words - 1);
+        uint32_t nb = 0;
+        bool fresh = false;
+        if (lane < cnt) {
+            nb = ll[1 + lane];
+            if (nb < n) fresh = vis_insert(s_vis, nb, &s_lost);
+            else s_bad = 1u;
+        }
+        const uint64_t fm = __ballot(fresh);
+        const uint32_t nf = (uint32_t)__popcll(fm);
+        n_eval += nf;
+        if (fresh) s_ni[__popcll(fm & lanemask_lt())] = nb;
+        __syncthreads();
+        // distances of the fresh neighbours, 4 at a time (one per 16-lane group)
+        for (uint32_t b = 0; b < nf; b += 4) {
+            const uint32_t t = b + grp;
+            if (t < nf) {
+                const float dv = dist_row<VT>(s_qv, x, s_ni[t], d, g);
+                if (g == 0) s_nd[t] = dv;
+            }
+        }
+        __syncthreads();
+        // admission in the host's scan order: list not full, or strictly
+        // closer than the worst entry
+        if (lane == 0) {
+            uint32_t LL = s_L;
+            const bool check_dup = s_lost != 0;
+            for (uint32_t t = 0; t < nf; t++) {
+                const float dv = s_nd[t];
+                const uint32_t id = s_ni[t];
+                if (LL >= ef && !(dv < s_d[LL - 1])) continue;
+                if (check_dup) {
+                    bool dup = false;
+                    for (uint32_t j = 0; j < LL && !dup; j++) dup = (s_i[j] == id);
+                    if (dup) continue;
+                }
+                uint32_t p = (LL < ef) ? LL : LL - 1;
+                if (LL < ef) LL++;
+                while (p > 0 && pair_less(dv, id, s_d[p - 1], s_i[p - 1])) {
+                    s_d[p] = s_d[p - 1];
+                    s_i[p] = s_i[p - 1];
+                    s_x[p] = s_x[p - 1];
+                    p--;
+                }
+                s_d[p] = dv;
+                s_i[p] = id;
+                s_x[p] = 0;
+            }
+            s_L = LL;
+        }
+        __syncthreads();
+        L = s_L;
+    }
+    // ---- output: the m closest ----
+    for (uint32_t t = lane; t < m; t += 64) proxies[qi * m + t] = (t < L) ? s_i[t] : kInvalidId;
+    if (expansions && lane == 0) {
+        expansions[2 * qi] = n_exp;
+        expansions[2 * qi + 1] = n_eval;
+    }
+    if (raw) {   // per-query statistics for the learned m (kRawWords floats)
+        float *r = raw + qi * kRawWords;
+        if (lane < 16) r[2 + lane] = (lane < L) ? s_d[lane] : s_d[L - 1];
+        if (lane == 0) {
+            r[0] = d_entry;
+            r[1] = d_start;
+            r[18] = (float)n_exp;
+            r[19] = (float)L;
+        }
+    }
+    if (lane == 0 && s_bad) atomicOr(err, 1u);
+}
 
 void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint32_t ef,
                  uint32_t *proxies_dev, uint32_t *expansions_dev, float *raw_dev) {
-    REQUIRE(kBeamEnabled, HRNN_E_STATE,
-            "GPU phase A (HNSW beam) is switched off pending an unresolved memory fault; "
-            "run phase A on the host (hrnn_index_proxies) and pass the proxies to rknn_search");
     REQUIRE(h->hnsw_valid, HRNN_E_STATE, "proxies: upload the HNSW graph first");
     REQUIRE(ef >= 1 && ef <= (uint32_t)kBeamMaxEf, HRNN_E_ARG, "proxies: ef must be in [1, 512]");
     REQUIRE(m >= 1 && m <= ef, HRNN_E_ARG, "proxies: m must be in [1, ef]");
-    REQUIRE(h->l0_words - 1 <= (uint32_t)kBeamMaxDeg, HRNN_E_ARG, "proxies: layer-0 degree > 64");
+    REQUIRE(h->l0_words >= 1 && h->l0_words - 1 <= (uint32_t)kBeamMaxDeg, HRNN_E_ARG,
+            "proxies: layer-0 degree must be in [0, 64]");
+    REQUIRE(h->d >= 1 && h->d <= 4096, HRNN_E_ARG, "proxies: d must be in [1, 4096]");
+    REQUIRE(q_dev && proxies_dev, HRNN_E_ARG, "proxies: null argument");
     if (nq == 0) return;
-    uint64_t vis_words = (h->n + 31) / 32;
-    uint32_t blocks = (uint32_t)std::min<uint64_t>(nq, (uint64_t)h->n_cu * h->tune_beam_wg_per_cu);
-    uint64_t need = (uint64_t)blocks * vis_words * 4;
-    if (h->beam_vis.bytes < need) {
-        h->beam_vis.reserve(need);
-        HIPC(hipMemsetAsync(h->beam_vis.p, 0, h->beam_vis.bytes, h->stream));
-    }
-    uint32_t *queue = h->counters.as<uint32_t>() + 80;
-    HIPC(hipMemsetAsync(queue, 0, 4, h->stream));
-#define LAUNCH(VT, XP)                                                                          \
-    hipLaunchKernelGGL(k_beam<VT>, dim3(blocks), dim3(64), h->d * sizeof(float), h->stream, XP, h->d, \
-                       h->n, h->l0.as<uint32_t>(), h->l0_words, h->up_off.as<uint32_t>(),            \
-                       h->up_pool.as<uint32_t>(), h->up_words, h->entry, h->maxlevel, q_dev, nq, ef, m, \
-                       h->beam_vis.as<uint32_t>(), vis_words, queue, proxies_dev, expansions_dev, raw_dev)
+    uint32_t *err = h->counters.as<uint32_t>() + kErrWord;
+#define LAUNCH(VT, XP)                                                                            \
+    hipLaunchKernelGGL(k_phase_a<VT>, dim3(nq), dim3(64), h->d * sizeof(float), h->stream, XP, h->d, \
+                       (uint32_t)h->n, h->l0.as<uint32_t>(), h->l0_words, h->up_off.as<uint32_t>(),   \
+                       h->up_pool.as<uint32_t>(), h->up_pool_words, h->up_words, h->entry,            \
+                       h->maxlevel, q_dev, ef, m, proxies_dev, expansions_dev, raw_dev, err)
     // BF16 mirror where it is exact (same distances, half the bytes)
     if (h->small_int && h->vec_bf16)
         LAUNCH(__hip_bfloat16, h->xb.as<__hip_bfloat16>());
@@ -287,6 +289,7 @@ int hrnn_dev_set_hnsw(hrnn_dev_t h, uint32_t maxM0, uint32_t maxM, const uint32_
         if (pool_words) h2d(h, h->up_pool.p, up_pool, pool_words * 4);
         h->l0_words = w0;
         h->up_words = wu;
+        h->up_pool_words = pool_words;
         h->entry = entry;
         h->maxlevel = maxlevel;
         h->hnsw_valid = true;
@@ -307,6 +310,7 @@ int hrnn_dev_proxies(hrnn_dev_t h, const float *q, uint32_t nq, uint32_t m, uint
         d2h(h, proxies, dp.p, (uint64_t)nq * m * 4);
         dq.release();
         dp.release();
+        check_device_error(h);
     });
 }
 

@@ -40,10 +40,10 @@ struct hrnn_dev {
 
     // HNSW graph for phase A on the GPU: layer-0 links [n x (maxM0+1)] (count
     // word + ids), per-node offset into the upper-layer pool (blocks of maxM+1
-    // words per level, 0xFFFFFFFF for level-0-only nodes), per-block visited
-    // bitmaps (all-zero between queries)
+    // words per level, 0xFFFFFFFF for level-0-only nodes)
     bool hnsw_valid = false;
     uint32_t l0_words = 0, up_words = 0, entry = 0;
+    uint64_t up_pool_words = 0;
     int maxlevel = 0;
     hrnn::DevBuf l0, up_off, up_pool, beam_vis;
 
@@ -51,7 +51,9 @@ struct hrnn_dev {
     hrnn::DevBuf s0, s1, s2, s3, s4, s5, s6, s7, s8;
     hrnn::DevBuf nnd_aux;   // NNDescent proposal offsets, scan partials, total
     // small device counters, by byte offset: 0 base-int flag, 32 long-list
-    // count, 128 CSR total, 192 longest list, 256 search queue, 512 NNDescent stats
+    // count, 128 CSR total, 192 longest list, 256 search queue, 384 device
+    // error word (set by a kernel that met invalid graph data; read and
+    // cleared by hrnn_dev_sync)
     hrnn::DevBuf counters;
 
     ~hrnn_dev();
@@ -77,6 +79,9 @@ int guarded(hrnn_dev_t h, F &&f) {
 // Scoped host <-> device staging helpers (synchronous on the handle stream).
 void h2d(hrnn_dev_t h, void *dst, const void *src, size_t bytes);
 void d2h(hrnn_dev_t h, void *dst, const void *src, size_t bytes);
+// Read the device error word (synchronous); throws HRNN_E_STATE and clears
+// it when a kernel reported invalid data.
+void check_device_error(hrnn_dev_t h);
 
 // Launchers defined in the kernel files.
 void launch_base_mirrors(hrnn_dev_t h, uint64_t row0, uint64_t rows);
@@ -95,6 +100,7 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t bucket, hrnn_nnd_ite
 // Per-query beam statistics (raw words): 0 d(q, entry), 1 d(q, layer-0
 // start), 2..17 the 16 closest beam distances, 18 expansions, 19 list size.
 constexpr int kRawWords = 20;
+constexpr uint32_t kErrWord = 96;   // u32 index of the device error word in counters
 void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint32_t ef,
                  uint32_t *proxies_dev, uint32_t *expansions_dev, float *raw_dev = nullptr);
 void bf_topk_base(hrnn_dev_t h, const float *q_dev, uint32_t nq, bool self_base, uint32_t kk,

@@ -219,10 +219,20 @@ using namespace hrnn;
 
 extern "C" {
 
+// The device feature path (phase-A beam with statistics followed by
+// k_features) is switched off: its first GPU run ended in a memory fault whose
+// cause is not found.  Features come from the host phase A
+// (hrnn_index_query_features); the MLP kernels and
+// hrnn_dev_rknn_search_learned_dptr take them from there.
+static const char *kFeaturesOff =
+    "device feature path switched off (unexplained fault on its first run); use "
+    "hrnn_index_query_features and hrnn_dev_rknn_search_learned_dptr";
+
 int hrnn_dev_query_features(hrnn_dev_t h, const float *q, uint32_t nq, uint32_t k, uint32_t ef,
                             uint32_t m_max, float *feats, uint32_t *proxies) {
     if (!h) return HRNN_E_ARG;
     return guarded(h, [&] {
+        REQUIRE(false, HRNN_E_STATE, kFeaturesOff);
         REQUIRE(q && feats, HRNN_E_ARG, "query_features: null argument");
         REQUIRE(h->K > 0 && k >= 1 && k <= h->K, HRNN_E_ARG, "query_features: k out of [1, K]");
         if (!nq) return;
@@ -258,6 +268,7 @@ int hrnn_dev_rknn_query_learned_dptr(hrnn_dev_t h, const float *q_dev, uint32_t 
                                      uint32_t cap_per_q, uint32_t *work_dev) {
     if (!h) return HRNN_E_ARG;
     return guarded(h, [&] {
+        REQUIRE(false, HRNN_E_STATE, kFeaturesOff);
         REQUIRE(h->csr_valid && h->K > 0, HRNN_E_STATE, "query_learned: build the CSR first");
         REQUIRE(k >= 1 && k <= h->K, HRNN_E_ARG, "query_learned: k out of [1, K]");
         REQUIRE(m_max >= 16 && m_max <= 1024, HRNN_E_ARG, "query_learned: m_max must be in [16, 1024]");
