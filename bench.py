@@ -84,6 +84,12 @@ def main() -> int:
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
         return 2
+    # The library's progress lines (refineKNNG / buildRKNNG, as the reference
+    # prints them) go to stdout at the C level; send fd 1 to stderr for the run
+    # and keep the real stdout for the one JSON line.
+    sys.stdout.flush()
+    json_out = os.fdopen(os.dup(1), "w")
+    os.dup2(2, 1)
 
     import hrnn_amd
     from hrnn_amd import hiprt
@@ -451,7 +457,7 @@ def main() -> int:
                      "groundtruth_gpu": round(gt_s, 3)},
         "sweep": sweep,
     }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(out), file=json_out, flush=True)
     return 0
 
 

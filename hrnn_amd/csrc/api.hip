@@ -44,28 +44,31 @@ void check_device_error(hrnn_dev_t h) {
 }
 
 // BF16/FP16 mirrors, FP32 row norms (sum of squares, sequential FMA order) and
-// the small-integer flag for rows [row0, row0 + rows).
+// the value-range flags for rows [row0, row0 + rows): flags bit 0 = some value
+// is not an integer in [-256, 256], bit 1 = some value is not one in [0, 255].
 __global__ void k_base_mirrors(const float *__restrict__ x, __hip_bfloat16 *__restrict__ xb,
                                __half *__restrict__ xh, float *__restrict__ norms,
                                uint64_t row0, uint64_t rows, uint32_t d,
-                               unsigned int *__restrict__ not_int) {
+                               unsigned int *__restrict__ flags) {
     uint64_t r = row0 + blockIdx.x * (uint64_t)(blockDim.x / 64) + (threadIdx.x / 64);
     if (r >= row0 + rows) return;
     uint32_t lane = threadIdx.x & 63u;
     const float *xr = x + r * d;
     float acc = 0.f;
-    bool bad = false;
+    bool bad = false, not_u8 = false;
     for (uint32_t e = lane; e < d; e += 64) {
         float v = xr[e];
         xb[r * d + e] = __float2bfloat16(v);
         xh[r * d + e] = __float2half(v);
         acc = __builtin_fmaf(v, v, acc);
-        bad |= !(v == rintf(v) && fabsf(v) <= 256.f);
+        const bool integral = v == rintf(v);
+        bad |= !(integral && fabsf(v) <= 256.f);
+        not_u8 |= !(integral && v >= 0.f && v <= 255.f);
     }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
     if (lane == 0) norms[r] = acc;
-    uint64_t anybad = __ballot(bad);
-    if (lane == 0 && anybad) atomicOr(not_int, 1u);
+    const unsigned f = (__ballot(bad) ? 1u : 0u) | (__ballot(not_u8) ? 2u : 0u);
+    if (lane == 0 && f) atomicOr(flags, f);
 }
 
 void launch_base_mirrors(hrnn_dev_t h, uint64_t row0, uint64_t rows) {
@@ -78,9 +81,10 @@ void launch_base_mirrors(hrnn_dev_t h, uint64_t row0, uint64_t rows) {
                        h->xb.as<__hip_bfloat16>(), h->xh.as<__half>(), h->norms.as<float>(),
                        row0, rows, h->d, flag);
     HIPC(hipGetLastError());
-    unsigned int bad = 0;
-    d2h(h, &bad, flag, sizeof(bad));
-    h->small_int = (row0 == 0) ? (bad == 0) : (h->small_int && bad == 0);
+    unsigned int f = 0;
+    d2h(h, &f, flag, sizeof(f));
+    h->small_int = (row0 == 0) ? ((f & 1u) == 0) : (h->small_int && (f & 1u) == 0);
+    h->base_u8 = (row0 == 0) ? ((f & 2u) == 0) : (h->base_u8 && (f & 2u) == 0);
 }
 
 __global__ void k_l2_pairs(const float *__restrict__ a, const float *__restrict__ x,

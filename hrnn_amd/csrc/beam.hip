@@ -12,13 +12,20 @@
 // order of equal distances.
 //
 // k_phase_a: one query per workgroup of one wave (64 lanes), no state kept
-// between workgroups.  The sorted list lives in LDS; visited nodes go into an
-// LDS hash set of 8192 ids (linear probing).  When a probe run is exhausted
-// the node is treated as unseen and admission checks the list for it, so the
-// result does not depend on the table size.  Every id or count read from the
-// graph is range-checked; a bad one is skipped and raises the handle's device
-// error word (reported by hrnn_dev_sync).  The descent and the beam run
-// bounded loops.
+// between workgroups.  The sorted list lives in LDS (double buffered); visited
+// nodes go into an LDS hash set of 4096 ids (linear probing).  When a probe
+// run is exhausted the node is treated as unseen and admission checks the
+// list for it, so the result does not depend on the table size.  Per
+// expansion the fresh neighbours' distances are computed by 16-lane groups,
+// the admissible ones sorted across the wave and merged into the list in
+// parallel (the first ef of the merged (dist, id) order: the same set a
+// sequential insertion keeps).  When every base and query value is an integer
+// in [0, 255] and d <= 258, every distance sum is an exact integer, so rows
+// are read with 16-byte loads and summed in any order; otherwise the
+// reference's 16-partial order is used.  Every id or count read from the graph
+// is range-checked; a bad one is skipped and raises the handle's device error
+// word (reported by hrnn_dev_sync).  The descent and the beam run bounded
+// loops.
 #include <algorithm>
 #include <cfloat>
 
@@ -28,13 +35,16 @@ namespace hrnn {
 
 constexpr int kBeamMaxEf = 512;
 constexpr int kBeamMaxDeg = 64;
-constexpr uint32_t kVisSlots = 8192;
+constexpr uint32_t kVisSlots = 4096;
 constexpr uint32_t kVisProbes = 32;
+
+typedef __bf16 beam_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float beam_f32x4 __attribute__((ext_vector_type(4)));
 
 // Insert id into the LDS set; true when the id was not there (or could not be
 // recorded: *lost is set then).
 __device__ __forceinline__ bool vis_insert(uint32_t *tab, uint32_t id, uint32_t *lost) {
-    uint32_t h = (id * 2654435761u) >> 19;   // 13 bits -> [0, 8192)
+    uint32_t h = (id * 2654435761u) >> 20;   // 12 bits -> [0, 4096)
     for (uint32_t p = 0; p < kVisProbes; p++) {
         uint32_t old = atomicCAS(&tab[h], kInvalidId, id);
         if (old == kInvalidId) return true;
@@ -45,26 +55,109 @@ __device__ __forceinline__ bool vis_insert(uint32_t *tab, uint32_t id, uint32_t 
     return true;
 }
 
+// Squared L2 from the LDS query to base row id by a 16-lane group, for data
+// whose every sum is an exact integer (values in [0, 255], d <= 258): the
+// order of the additions does not matter, so each lane reads 16 contiguous
+// bytes per step and the group reduces with a butterfly.  Every lane of the
+// group returns the sum.  d must be a multiple of 8 (BF16) or 4 (FP32).
+template <class VT>
+__device__ __forceinline__ float dist_row_exact(const float *__restrict__ qv, const VT *__restrict__ base,
+                                                uint64_t id, uint32_t d, uint32_t g);
+template <>
+__device__ __forceinline__ float dist_row_exact<__hip_bfloat16>(const float *__restrict__ qv,
+                                                                const __hip_bfloat16 *__restrict__ base,
+                                                                uint64_t id, uint32_t d, uint32_t g) {
+    const __hip_bfloat16 *xv = base + id * d;
+    float acc = 0.f;
+    for (uint32_t e = 8 * g; e < d; e += 128) {
+        const beam_bf16x8 v = *reinterpret_cast<const beam_bf16x8 *>(xv + e);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const float t = qv[e + j] - (float)v[j];
+            acc = __builtin_fmaf(t, t, acc);
+        }
+    }
+    acc += __shfl_xor(acc, 8, 16);
+    acc += __shfl_xor(acc, 4, 16);
+    acc += __shfl_xor(acc, 2, 16);
+    acc += __shfl_xor(acc, 1, 16);
+    return acc;
+}
+template <>
+__device__ __forceinline__ float dist_row_exact<float>(const float *__restrict__ qv,
+                                                       const float *__restrict__ base, uint64_t id,
+                                                       uint32_t d, uint32_t g) {
+    const float *xv = base + id * d;
+    float acc = 0.f;
+    for (uint32_t e = 4 * g; e < d; e += 64) {
+        const beam_f32x4 v = *reinterpret_cast<const beam_f32x4 *>(xv + e);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float t = qv[e + j] - v[j];
+            acc = __builtin_fmaf(t, t, acc);
+        }
+    }
+    acc += __shfl_xor(acc, 8, 16);
+    acc += __shfl_xor(acc, 4, 16);
+    acc += __shfl_xor(acc, 2, 16);
+    acc += __shfl_xor(acc, 1, 16);
+    return acc;
+}
+
+template <class VT>
+__device__ __forceinline__ float beam_dist(bool exact, const float *qv, const VT *x, uint64_t id, uint32_t d,
+                                           uint32_t g) {
+    return exact ? dist_row_exact<VT>(qv, x, id, d, g) : dist_row<VT>(qv, x, id, d, g);
+}
+
+__device__ __forceinline__ unsigned long long beam_key(float d, uint32_t id) {
+    return ((unsigned long long)float_key(d) << 32) | id;
+}
+
+// Ascending bitonic sort of one 64-bit key per lane across the wave.
+__device__ __forceinline__ unsigned long long wave_sort64(unsigned long long key, uint32_t lane) {
+    for (uint32_t k = 2; k <= 64; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t lo = (uint32_t)key, hi = (uint32_t)(key >> 32);
+            const unsigned long long other =
+                ((unsigned long long)(uint32_t)__shfl_xor((int)hi, (int)j, 64) << 32) |
+                (uint32_t)__shfl_xor((int)lo, (int)j, 64);
+            const bool up = (lane & k) == 0;
+            const bool keep_min = ((lane & j) == 0) == up;
+            key = keep_min ? (key < other ? key : other) : (key > other ? key : other);
+        }
+    }
+    return key;
+}
+
 template <class VT>
 __global__ __launch_bounds__(64) void k_phase_a(
     const VT *__restrict__ x, uint32_t d, uint32_t n, const uint32_t *__restrict__ l0,
     uint32_t l0_words, const uint32_t *__restrict__ up_off, const uint32_t *__restrict__ up_pool,
     uint64_t pool_words, uint32_t up_words, uint32_t entry, int maxlevel, const float *__restrict__ q,
-    uint32_t ef, uint32_t m, uint32_t *__restrict__ proxies, uint32_t *__restrict__ expansions,
-    float *__restrict__ raw, uint32_t *__restrict__ err) {
-    __shared__ float s_d[kBeamMaxEf];
-    __shared__ uint32_t s_i[kBeamMaxEf];
-    __shared__ uint8_t s_x[kBeamMaxEf];   // expanded flag
+    uint32_t ef, uint32_t m, int base_exact, uint32_t *__restrict__ proxies,
+    uint32_t *__restrict__ expansions, float *__restrict__ raw, uint32_t *__restrict__ err) {
+    __shared__ float s_d[2][kBeamMaxEf];
+    __shared__ uint32_t s_i[2][kBeamMaxEf];
+    __shared__ uint8_t s_x[2][kBeamMaxEf];   // expanded flag
     __shared__ float s_nd[kBeamMaxDeg];
     __shared__ uint32_t s_ni[kBeamMaxDeg];
+    __shared__ unsigned long long s_ck[kBeamMaxDeg];   // sorted candidate keys of one expansion
     __shared__ uint32_t s_vis[kVisSlots];
-    __shared__ uint32_t s_L, s_lost, s_bad;
+    __shared__ uint32_t s_lost, s_bad;
     extern __shared__ float s_qv[];
 
     const uint32_t lane = threadIdx.x;
     const uint32_t g = lane & 15u, grp = lane >> 4;
     const uint64_t qi = blockIdx.x;
-    for (uint32_t e = lane; e < d; e += 64) s_qv[e] = q[qi * d + e];
+    bool q_u8 = true;
+    for (uint32_t e = lane; e < d; e += 64) {
+        const float v = q[qi * d + e];
+        s_qv[e] = v;
+        q_u8 &= (v == rintf(v)) && v >= 0.f && v <= 255.f;
+    }
+    // exact sums for this query: every base and query value an integer in [0, 255]
+    const bool exact = base_exact && __ballot(!q_u8) == 0;
     for (uint32_t t = lane; t < kVisSlots; t += 64) s_vis[t] = kInvalidId;
     if (lane == 0) {
         s_lost = 0;
@@ -74,7 +167,7 @@ __global__ __launch_bounds__(64) void k_phase_a(
 
     // ---- greedy descent over the upper layers ----
     uint32_t cur = entry < n ? entry : 0;
-    float cd = __shfl(dist_row<VT>(s_qv, x, cur, d, g), 0, 64);
+    float cd = __shfl(beam_dist<VT>(exact, s_qv, x, cur, d, g), 0, 64);
     const float d_entry = cd;
     for (int level = maxlevel; level > 0; level--) {
         for (uint32_t step = 0; step < n; step++) {   // each move strictly lowers cd
@@ -95,7 +188,7 @@ __global__ __launch_bounds__(64) void k_phase_a(
                 if (t < cnt) {   // uniform per 16-lane group
                     id = ll[1 + t];
                     if (id < n) {
-                        dv = dist_row<VT>(s_qv, x, id, d, g);
+                        dv = beam_dist<VT>(exact, s_qv, x, id, d, g);
                     } else {
                         if (g == 0) s_bad = 1u;
                         id = cur;
@@ -120,28 +213,27 @@ __global__ __launch_bounds__(64) void k_phase_a(
     // ---- layer-0 beam ----
     const float d_start = cd;
     if (lane == 0) {
-        s_d[0] = cd;
-        s_i[0] = cur;
-        s_x[0] = 0;
-        s_L = 1;
+        s_d[0][0] = cd;
+        s_i[0][0] = cur;
+        s_x[0][0] = 0;
         vis_insert(s_vis, cur, &s_lost);
     }
     __syncthreads();
-    uint32_t L = 1, n_exp = 0, n_eval = 1;
+    uint32_t L = 1, buf = 0, n_exp = 0, n_eval = 1;
     for (uint32_t iter = 0; iter <= n; iter++) {   // each pass expands one list entry
         uint32_t pick = kInvalidId;
         for (uint32_t b0 = 0; b0 < L; b0 += 64) {
             const uint32_t t = b0 + lane;
-            const uint64_t mask = __ballot(t < L && s_x[t] == 0);
+            const uint64_t mask = __ballot(t < L && s_x[buf][t] == 0);
             if (mask) {
                 pick = b0 + (uint32_t)__ffsll((long long)mask) - 1;
                 break;
             }
         }
         if (pick == kInvalidId) break;
-        const uint32_t c = s_i[pick];
+        const uint32_t c = s_i[buf][pick];
         __syncthreads();
-        if (lane == 0) s_x[pick] = 1;
+        if (lane == 0) s_x[buf][pick] = 1;
         n_exp++;
         const uint32_t *ll = l0 + (uint64_t)c * l0_words;
         const uint32_t cnt = min(ll[0] & 0xFFFFu, l0_words - 1);
@@ -161,51 +253,73 @@ __global__ __launch_bounds__(64) void k_phase_a(
         for (uint32_t b = 0; b < nf; b += 4) {
             const uint32_t t = b + grp;
             if (t < nf) {
-                const float dv = dist_row<VT>(s_qv, x, s_ni[t], d, g);
+                const float dv = beam_dist<VT>(exact, s_qv, x, s_ni[t], d, g);
                 if (g == 0) s_nd[t] = dv;
             }
         }
         __syncthreads();
-        // admission in the host's scan order: list not full, or strictly
-        // closer than the worst entry
-        if (lane == 0) {
-            uint32_t LL = s_L;
-            const bool check_dup = s_lost != 0;
-            for (uint32_t t = 0; t < nf; t++) {
-                const float dv = s_nd[t];
-                const uint32_t id = s_ni[t];
-                if (LL >= ef && !(dv < s_d[LL - 1])) continue;
-                if (check_dup) {
-                    bool dup = false;
-                    for (uint32_t j = 0; j < LL && !dup; j++) dup = (s_i[j] == id);
-                    if (dup) continue;
-                }
-                uint32_t p = (LL < ef) ? LL : LL - 1;
-                if (LL < ef) LL++;
-                while (p > 0 && pair_less(dv, id, s_d[p - 1], s_i[p - 1])) {
-                    s_d[p] = s_d[p - 1];
-                    s_i[p] = s_i[p - 1];
-                    s_x[p] = s_x[p - 1];
-                    p--;
-                }
-                s_d[p] = dv;
-                s_i[p] = id;
-                s_x[p] = 0;
+        if (nf == 0) continue;
+        // candidates that can enter: the list is not full, or (dist, id) below
+        // its worst entry; not already listed (checked only when the visited
+        // set lost an insertion)
+        unsigned long long key = ~0ull;
+        if (lane < nf) {
+            const float dv = s_nd[lane];
+            const uint32_t id = s_ni[lane];
+            bool ok = L < ef || pair_less(dv, id, s_d[buf][L - 1], s_i[buf][L - 1]);
+            if (ok && s_lost) {
+                for (uint32_t j = 0; j < L && ok; j++) ok = s_i[buf][j] != id;
             }
-            s_L = LL;
+            if (ok) key = beam_key(dv, id);
+        }
+        const uint32_t nv = (uint32_t)__popcll(__ballot(key != ~0ull));
+        if (nv == 0) continue;   // uniform
+        key = wave_sort64(key, lane);
+        s_ck[lane] = key;
+        __syncthreads();
+        // merge the sorted candidates into the sorted list, keeping the first ef
+        const uint32_t nbuf = buf ^ 1u;
+        for (uint32_t e = lane; e < L; e += 64) {
+            const unsigned long long ke = beam_key(s_d[buf][e], s_i[buf][e]);
+            uint32_t lo = 0, hi = nv;   // candidates below entry e
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_ck[mid] < ke) lo = mid + 1; else hi = mid;
+            }
+            const uint32_t pos = e + lo;
+            if (pos < ef) {
+                s_d[nbuf][pos] = s_d[buf][e];
+                s_i[nbuf][pos] = s_i[buf][e];
+                s_x[nbuf][pos] = s_x[buf][e];
+            }
+        }
+        if (lane < nv) {
+            const unsigned long long kc = s_ck[lane];
+            uint32_t lo = 0, hi = L;    // list entries below candidate lane
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (beam_key(s_d[buf][mid], s_i[buf][mid]) < kc) lo = mid + 1; else hi = mid;
+            }
+            const uint32_t pos = lane + lo;
+            if (pos < ef) {
+                s_d[nbuf][pos] = key_float((uint32_t)(kc >> 32));
+                s_i[nbuf][pos] = (uint32_t)kc;
+                s_x[nbuf][pos] = 0;
+            }
         }
         __syncthreads();
-        L = s_L;
+        L = min(L + nv, ef);
+        buf = nbuf;
     }
     // ---- output: the m closest ----
-    for (uint32_t t = lane; t < m; t += 64) proxies[qi * m + t] = (t < L) ? s_i[t] : kInvalidId;
+    for (uint32_t t = lane; t < m; t += 64) proxies[qi * m + t] = (t < L) ? s_i[buf][t] : kInvalidId;
     if (expansions && lane == 0) {
         expansions[2 * qi] = n_exp;
         expansions[2 * qi + 1] = n_eval;
     }
     if (raw) {   // per-query statistics for the learned m (kRawWords floats)
         float *r = raw + qi * kRawWords;
-        if (lane < 16) r[2 + lane] = (lane < L) ? s_d[lane] : s_d[L - 1];
+        if (lane < 16) r[2 + lane] = (lane < L) ? s_d[buf][lane] : s_d[buf][L - 1];
         if (lane == 0) {
             r[0] = d_entry;
             r[1] = d_start;
@@ -227,11 +341,14 @@ void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint
     REQUIRE(q_dev && proxies_dev, HRNN_E_ARG, "proxies: null argument");
     if (nq == 0) return;
     uint32_t *err = h->counters.as<uint32_t>() + kErrWord;
+    // every sum exact in any order: values in [0, 255] and d <= 258, with the
+    // row width the wide loads need (8 BF16 or 4 FP32 elements)
+    const int base_exact = (h->base_u8 && h->d <= 258 && h->d % 8 == 0) ? 1 : 0;
 #define LAUNCH(VT, XP)                                                                            \
     hipLaunchKernelGGL(k_phase_a<VT>, dim3(nq), dim3(64), h->d * sizeof(float), h->stream, XP, h->d, \
                        (uint32_t)h->n, h->l0.as<uint32_t>(), h->l0_words, h->up_off.as<uint32_t>(),   \
                        h->up_pool.as<uint32_t>(), h->up_pool_words, h->up_words, h->entry,            \
-                       h->maxlevel, q_dev, ef, m, proxies_dev, expansions_dev, raw_dev, err)
+                       h->maxlevel, q_dev, ef, m, base_exact, proxies_dev, expansions_dev, raw_dev, err)
     // BF16 mirror where it is exact (same distances, half the bytes)
     if (h->small_int && h->vec_bf16)
         LAUNCH(__hip_bfloat16, h->xb.as<__hip_bfloat16>());

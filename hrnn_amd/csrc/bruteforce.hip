@@ -16,9 +16,10 @@
 // staged in LDS and merged into per-row sorted (dist, id) lists of length C
 // (C >= 2 * kk).  A re-rank pass then recomputes the C candidates with the
 // reference's FP32 direct-difference order and sorts by (dist, id).
-// For integer data in [-256, 256] (all configs with 0..255 values) BF16/FP16
-// inputs and FP32 accumulation are exact, so candidate distances equal the
-// reference's bit for bit.
+// For integer data in [0, 255] at d <= 128 (the integer configs) BF16/FP16
+// inputs with FP32 accumulation are exact (every norm, product and partial sum
+// an integer below 2^24), so candidate distances equal the reference's bit for
+// bit and the RkNN pass (mode 1) can decide on them directly.
 #include <algorithm>
 #include <cfloat>
 #include <vector>
@@ -488,7 +489,9 @@ void bf_rknn_count_fill(hrnn_dev_t h, const float *q_dev, uint32_t nq, const flo
                         uint32_t cap_per_q) {
     HIPC(hipMemsetAsync(cnt_dev, 0, (uint64_t)nq * 4, h->stream));
     unsigned grid = ceil_div(nq, kRB);
-    bool exact16 = h->small_int && mfma_ok(h) && prec != HRNN_PREC_F32;
+    // the Gram form decides exactly when every value is an integer in [0, 255]
+    // and d <= 128 (all norms and products below 2^24)
+    bool exact16 = h->base_u8 && h->d <= 128 && mfma_ok(h) && prec != HRNN_PREC_F32;
     if (exact16) {
         h->s7.reserve((uint64_t)nq * h->d * 4 + (uint64_t)nq * 4 + 256);
         __hip_bfloat16 *qb = h->s7.as<__hip_bfloat16>();
@@ -563,11 +566,11 @@ int hrnn_dev_bf_rknn(hrnn_dev_t h, const float *q, uint32_t nq, const float *kth
         res_off[0] = 0;
         if (!nq) { if (needed) *needed = 0; return; }
         uint64_t n = h->n;
-        // BF16 path only when the queries are small integers as well
+        // Gram-form path only when the queries are integers in [0, 255] as well
         bool q_int = true;
         for (uint64_t i = 0; i < (uint64_t)nq * h->d && q_int; i++) {
             float v = q[i];
-            q_int = (v == std::nearbyint(v)) && std::fabs(v) <= 256.f;
+            q_int = (v == std::nearbyint(v)) && v >= 0.f && v <= 255.f;
         }
         int p = q_int ? prec : HRNN_PREC_F32;
         const uint32_t chunk = 8192;

@@ -14,7 +14,11 @@ struct hrnn_dev {
     uint64_t n_cap = 0;
     uint32_t d = 0;
     hrnn::DevBuf x, xb, xh, norms;
-    int small_int = 0;   // every value an integer in [-256, 256]
+    int small_int = 0;   // every value an integer in [-256, 256] (exact in BF16/FP16)
+    // every value an integer in [0, 255]: with d <= 128 every sum formed by the
+    // distance kernels (direct difference or Gram form, any order) is an
+    // integer below 2^24, so all of them are exact
+    int base_u8 = 0;
     // read vectors from the BF16 mirror where that is exact (small_int)
     bool vec_bf16 = true;
 

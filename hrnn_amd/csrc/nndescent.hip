@@ -32,9 +32,9 @@
 // distance only falls), so its final list is the same top-K of (list U
 // proposals), up to the order of equal distances.
 //
-// Distances: for integer data in [-256, 256] at d <= 128 (all 0..255 configs)
-// the BF16 Gram form is exact FP32 arithmetic, identical to the reference's
-// direct-difference L2Sqr.  Otherwise each proposal that passes the BF16
+// Distances: for integer data in [0, 255] at d <= 128 (the integer configs)
+// the BF16 Gram form is exact FP32 arithmetic (every norm, product and sum an
+// integer below 2^24), identical to the reference's direct-difference L2Sqr.  Otherwise each proposal that passes the BF16
 // filter is re-computed in the reference's FP32 order before it is counted.
 #include <algorithm>
 #include <cfloat>
@@ -506,7 +506,7 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
     REQUIRE(h->d % 32 == 0, HRNN_E_ARG, "nndescent: d must be a multiple of 32");
     const uint32_t R = (uint32_t)S;
     const uint32_t RB = kJRev;   // reverse offers kept per node and kind
-    const int exact = (h->small_int && h->d <= 128) ? 1 : 0;
+    const int exact = (h->base_u8 && h->d <= 128) ? 1 : 0;
 
     // Proposal buffer: per_node_cap entries per node (default max(16384, 32K)),
     // at most 60% of the free device memory (8 B per proposal).
