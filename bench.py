@@ -377,6 +377,23 @@ def main() -> int:
     dom = max(kern, key=lambda kk: kern[kk]["ms"])
     achieved = kern[dom]["gbs"]
 
+    # ---- traffic calibration: one plain copy of known size -----------------------------------
+    # Under `rocprofv3 --pmc FETCH_SIZE` (or WRITE_SIZE) this copy's counter
+    # reading gives the KiB the counter reports per byte actually moved; the
+    # kernels' readings are scaled by it.
+    cal_bytes = 256 << 20
+    cal_src = hiprt.DeviceArray((cal_bytes // 4,), np.uint32)
+    cal_dst = hiprt.DeviceArray((cal_bytes // 4,), np.uint32)
+    cal_dst.copy_from_device(cal_src, cal_bytes)
+    del cal_src, cal_dst
+    traffic = None
+    pmc_file = os.path.join(ROOT, "profiles", f"traffic_{dom}_hrnn-{args.config}.json")
+    if os.path.exists(pmc_file):
+        with open(pmc_file) as f:
+            tr = json.load(f)
+        if tr.get("nq") == nq and tr.get("m") == m_b:
+            traffic = tr.get("hbm_bytes_per_launch")
+
     # ---- cpu baseline: the reference's searchRknn on this run's index ----------------------
     # All host cores (OpenMP over queries, the reference's const re-entrant
     # searchRknn) on every query, plus one thread (the hrnn_search driver loop)
@@ -436,7 +453,7 @@ def main() -> int:
                    "nn_sample": cfg["S"], "k": k, "K_prime": Kp, "ef_search": ef, "path": best_name,
                    "recall": best["recall"], "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom,
                      "kernel_ms": round(kern[dom]["ms"], 4), "alg_bytes_per_launch": int(kern[dom]["bytes"]),
                      "queries_per_launch": nq,
                      "kernels": {kk: {k2: (round(v2, 4) if isinstance(v2, float) else v2) for k2, v2 in v.items()}

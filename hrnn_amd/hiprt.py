@@ -72,6 +72,12 @@ class DeviceArray:
         _ck(hip().hipMemcpyAsync(self.at(elem), a.ctypes.data_as(C.c_void_p), a.nbytes, H2D, stream),
             "hipMemcpyAsync H2D")
 
+    def copy_from_device(self, src: "DeviceArray", nbytes: int) -> None:
+        """Synchronous device-to-device copy of nbytes from src."""
+        if nbytes > min(self.nbytes, src.nbytes):
+            raise ValueError("copy_from_device: out of range")
+        _ck(hip().hipMemcpy(self.ptr, src.ptr, nbytes, D2D), "hipMemcpy D2D")
+
     def to_host(self) -> np.ndarray:
         out = np.empty(self.shape, self.dtype)
         _ck(hip().hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes, D2H), "hipMemcpy D2H")

@@ -251,8 +251,8 @@ class HRNN : public AlgorithmInterface<dist_t> {
         if (cur_element_count == 0 || nq == 0) return out;
         if (k_rknn == 0 || k_rknn > knn_depth())
             throw std::runtime_error("searchRknn: k must be in [1, K]");
-        std::vector<uint32_t> prox = phase_a_on_gpu() ? proxiesDevice(Q, nq, m, ef_search)
-                                                      : proxies(Q, nq, m, ef_search, threads);
+        std::vector<uint32_t> prox = phase_a_on_gpu(nq) ? proxiesDevice(Q, nq, m, ef_search)
+                                                        : proxies(Q, nq, m, ef_search, threads);
         std::vector<uint64_t> off(nq + 1);
         std::vector<uint32_t> ids;
         device_search(Q, nq, prox.data(), m, nullptr, (uint32_t)k_rknn, (uint32_t)K_prime,
@@ -758,9 +758,15 @@ class HRNN : public AlgorithmInterface<dist_t> {
         if (rknng_built_) sync_device_csr();
         if (cur_element_count > 0 && M_ * 2 <= 64) sync_device_hnsw();
     }
-    // Phase A runs on the host: the GPU beam kernel is switched off in the
-    // device library (its entry points return HRNN_E_STATE).
-    bool phase_a_on_gpu() const { return false; }
+    // Phase A of a batch runs on the GPU (k_phase_a) from 32 queries up,
+    // unless HRNN_PHASE_A=host; single queries and small batches stay on the
+    // host, where a launch would cost more than the beam.
+    bool phase_a_on_gpu(size_t nq) const {
+        const char *e = std::getenv("HRNN_PHASE_A");
+        if (e && std::string(e) == "host") return false;
+        if (e && std::string(e) == "gpu") return maxM0_ <= 64;
+        return nq >= 32 && maxM0_ <= 64;
+    }
 
  private:
     // ---- space ------------------------------------------------------------------

@@ -80,8 +80,9 @@ const float *hrnn_dev_base_ptr(hrnn_dev_t h);
  * vector bytes halve) and FP32 otherwise; HRNN_PREC_F32 always reads FP32. */
 int hrnn_dev_set_vector_precision(hrnn_dev_t h, int prec);
 /* Launch shapes set by the autotuner (hrnn_amd/autotune.py): "search_wg_per_cu"
- * (persistent RkNN search workgroups per CU, default 8) and "beam_wg_per_cu"
- * (persistent phase-A waves per CU, default 16); values in [1, 64]. */
+ * (persistent RkNN search workgroups per CU, default 8); "beam_wg_per_cu" is
+ * accepted and unused (phase A launches one workgroup per query); values in
+ * [1, 64]. */
 int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value);
 int hrnn_dev_get_tuning(hrnn_dev_t h, const char *key, int *value);
 
@@ -154,13 +155,9 @@ int hrnn_dev_rknn_search_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq,
                               uint32_t *res_ids_dev, uint32_t cap_per_q, uint32_t *work_dev);
 
 /* ---- RkNN phase A on the GPU -------------------------------------------- */
-/* SWITCHED OFF: the phase-A beam kernel is disabled until an unresolved GPU
- * memory fault seen on its first run is explained.  hrnn_dev_proxies,
- * hrnn_dev_proxies_dptr, hrnn_dev_rknn_query_dptr, hrnn_dev_query_features and
- * hrnn_dev_rknn_query_learned_dptr return HRNN_E_STATE; phase A runs on the
- * host (hrnn_index_proxies / hrnn_index_query_features in hrnn_host.h) and
- * its proxies go to hrnn_dev_rknn_search[_dptr].  hrnn_dev_set_hnsw still
- * validates and uploads the graph. */
+/* One query per workgroup (k_phase_a).  Graph ids and counts read by the
+ * kernel are range-checked; a bad one sets the handle's device error word,
+ * which hrnn_dev_sync (and hrnn_dev_proxies) report as HRNN_E_STATE. */
 /* Upload the HNSW graph: l0[n*(maxM0+1)] layer-0 links (word 0: count in the
  * low 16 bits, then ids); levels[n]; up_off[n] offset (in words) of node i's
  * upper-layer block in up_pool (0xFFFFFFFF for level-0 nodes), the block
@@ -213,8 +210,9 @@ int hrnn_dev_bf_rknn(hrnn_dev_t h, const float *q, uint32_t nq, const float *kth
  * feature standardisation mu[32] sd[32] after them (HRNN_MLP_MODEL floats). */
 #define HRNN_MLP_PARAMS 7312
 #define HRNN_MLP_MODEL (HRNN_MLP_PARAMS + 64)
-/* The 32 query features (phase-A beam statistics and r_k of the closest
- * proxies; unstandardised) and, when proxies != NULL, the m_max proxies. */
+/* SWITCHED OFF (returns HRNN_E_STATE): the device feature path ended in an
+ * unexplained memory fault on its first run.  Use hrnn_index_query_features
+ * (hrnn_host.h).  Was: the 32 query features and the m_max proxies. */
 int hrnn_dev_query_features(hrnn_dev_t h, const float *q, uint32_t nq, uint32_t k, uint32_t ef,
                             uint32_t m_max, float *feats, uint32_t *proxies);
 /* probs[n*16] = softmax(MLP(feats)). */
@@ -228,7 +226,9 @@ int hrnn_dev_mlp_grad(hrnn_dev_t h, const float *params, const float *feats,
 int hrnn_dev_mlp_train(hrnn_dev_t h, float *params, const float *feats, const uint32_t *labels,
                        uint32_t n, int epochs, uint32_t batch, float lr, float weight_decay,
                        uint64_t seed, float *epoch_loss);
-/* Whole search with learned m for a resident batch: phase A with m_max
+/* SWITCHED OFF (returns HRNN_E_STATE), with hrnn_dev_query_features; use
+ * hrnn_dev_rknn_search_learned_dptr.  Was: whole search with learned m for a
+ * resident batch: phase A with m_max
  * proxies and beam statistics, features standardised with the model's
  * mu/sd, MLP forward, per-query m = the smallest grid value whose cumulative
  * probability reaches tau (argmax when tau <= 0; proxies beyond it masked),
