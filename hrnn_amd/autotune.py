@@ -2,7 +2,9 @@
 
 For a resident index, query batch and proxy set it times each candidate
 launch shape of the persistent phase-B/C search kernel ("search_wg_per_cu":
-workgroups per CU) with HIP events on the library's stream, keeps the
+workgroups per CU; "search_lds_kib": KiB of LDS for the per-query
+deduplication table, which trades occupancy against how many queries fall
+back to the HBM visited bitmap) with HIP events on the library's stream, keeps the
 fastest, and caches the choice as JSON keyed by the GPU and the workload
 shape (n, d, K, m, nq).  Results are identical for every shape; only the
 time changes.
@@ -19,6 +21,7 @@ from . import hiprt
 from .dev import lib
 
 CANDIDATES = {
+    "search_lds_kib": [8, 16, 32, 48],
     "search_wg_per_cu": [1, 2, 4, 8, 16, 32],
 }
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "cache.json")
@@ -59,7 +62,7 @@ def tune(h, qd, nq: int, m: int, k: int, K_prime: int, shape: dict, pd, cd, idd,
          reps: int = 5, use_cache: bool = True) -> dict:
     """Tune on the device handle h with resident queries qd, resident proxies
     pd (nq x m) and result buffers cd/idd.
-    Returns {"search_wg_per_cu": v, "timings_ms": {...}}."""
+    Returns {"search_lds_kib": v, "search_wg_per_cu": v, "timings_ms": {...}}."""
     L = lib()
     key = _key(shape)
     cache = load_cache() if use_cache else {}
@@ -88,7 +91,7 @@ def tune(h, qd, nq: int, m: int, k: int, K_prime: int, shape: dict, pd, cd, idd,
     search = lambda: ck(L.hrnn_dev_rknn_search_dptr(h, qd.ptr, nq, pd.ptr, m, None, k, K_prime, 1.0,
                                                     cd.ptr, idd.ptr, cap, None))
     out, timings = {}, {}
-    for name, fn in (("search_wg_per_cu", search),):
+    for name, fn in (("search_lds_kib", search), ("search_wg_per_cu", search)):
         best, best_ms = None, float("inf")
         for v in CANDIDATES[name]:
             ck(L.hrnn_dev_set_tuning(h, name.encode(), v))

@@ -251,6 +251,7 @@ static int *tuning_slot(hrnn_dev_t h, const char *key) {
     std::string k = key ? key : "";
     if (k == "search_wg_per_cu") return &h->tune_search_wg_per_cu;
     if (k == "beam_wg_per_cu") return &h->tune_beam_wg_per_cu;
+    if (k == "search_lds_kib") return &h->tune_search_lds_kib;
     return nullptr;
 }
 

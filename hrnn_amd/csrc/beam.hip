@@ -38,9 +38,6 @@ constexpr int kBeamMaxDeg = 64;
 constexpr uint32_t kVisSlots = 4096;
 constexpr uint32_t kVisProbes = 32;
 
-typedef __bf16 beam_bf16x8 __attribute__((ext_vector_type(8)));
-typedef float beam_f32x4 __attribute__((ext_vector_type(4)));
-
 // Insert id into the LDS set; true when the id was not there (or could not be
 // recorded: *lost is set then).
 __device__ __forceinline__ bool vis_insert(uint32_t *tab, uint32_t id, uint32_t *lost) {
@@ -53,55 +50,6 @@ __device__ __forceinline__ bool vis_insert(uint32_t *tab, uint32_t id, uint32_t 
     }
     *lost = 1u;
     return true;
-}
-
-// Squared L2 from the LDS query to base row id by a 16-lane group, for data
-// whose every sum is an exact integer (values in [0, 255], d <= 258): the
-// order of the additions does not matter, so each lane reads 16 contiguous
-// bytes per step and the group reduces with a butterfly.  Every lane of the
-// group returns the sum.  d must be a multiple of 8 (BF16) or 4 (FP32).
-template <class VT>
-__device__ __forceinline__ float dist_row_exact(const float *__restrict__ qv, const VT *__restrict__ base,
-                                                uint64_t id, uint32_t d, uint32_t g);
-template <>
-__device__ __forceinline__ float dist_row_exact<__hip_bfloat16>(const float *__restrict__ qv,
-                                                                const __hip_bfloat16 *__restrict__ base,
-                                                                uint64_t id, uint32_t d, uint32_t g) {
-    const __hip_bfloat16 *xv = base + id * d;
-    float acc = 0.f;
-    for (uint32_t e = 8 * g; e < d; e += 128) {
-        const beam_bf16x8 v = *reinterpret_cast<const beam_bf16x8 *>(xv + e);
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const float t = qv[e + j] - (float)v[j];
-            acc = __builtin_fmaf(t, t, acc);
-        }
-    }
-    acc += __shfl_xor(acc, 8, 16);
-    acc += __shfl_xor(acc, 4, 16);
-    acc += __shfl_xor(acc, 2, 16);
-    acc += __shfl_xor(acc, 1, 16);
-    return acc;
-}
-template <>
-__device__ __forceinline__ float dist_row_exact<float>(const float *__restrict__ qv,
-                                                       const float *__restrict__ base, uint64_t id,
-                                                       uint32_t d, uint32_t g) {
-    const float *xv = base + id * d;
-    float acc = 0.f;
-    for (uint32_t e = 4 * g; e < d; e += 64) {
-        const beam_f32x4 v = *reinterpret_cast<const beam_f32x4 *>(xv + e);
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const float t = qv[e + j] - v[j];
-            acc = __builtin_fmaf(t, t, acc);
-        }
-    }
-    acc += __shfl_xor(acc, 8, 16);
-    acc += __shfl_xor(acc, 4, 16);
-    acc += __shfl_xor(acc, 2, 16);
-    acc += __shfl_xor(acc, 1, 16);
-    return acc;
 }
 
 template <class VT>

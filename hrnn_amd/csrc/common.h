@@ -138,6 +138,58 @@ __device__ __forceinline__ float dist_row<__hip_bfloat16>(const float *__restric
     return r;
 }
 
+typedef __bf16 ex_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float ex_f32x4 __attribute__((ext_vector_type(4)));
+
+// Squared L2 from the LDS query to base row id by a 16-lane group, for data
+// whose every sum is an exact integer (values in [0, 255], d <= 258): the
+// order of the additions does not matter, so each lane reads 16 contiguous
+// bytes per step and the group reduces with a butterfly.  Every lane of the
+// group returns the sum.  d must be a multiple of 8 (BF16) or 4 (FP32).
+template <class VT>
+__device__ __forceinline__ float dist_row_exact(const float *__restrict__ qv, const VT *__restrict__ base,
+                                                uint64_t id, uint32_t d, uint32_t g);
+template <>
+__device__ __forceinline__ float dist_row_exact<__hip_bfloat16>(const float *__restrict__ qv,
+                                                                const __hip_bfloat16 *__restrict__ base,
+                                                                uint64_t id, uint32_t d, uint32_t g) {
+    const __hip_bfloat16 *xv = base + id * d;
+    float acc = 0.f;
+    for (uint32_t e = 8 * g; e < d; e += 128) {
+        const ex_bf16x8 v = *reinterpret_cast<const ex_bf16x8 *>(xv + e);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const float t = qv[e + j] - (float)v[j];
+            acc = __builtin_fmaf(t, t, acc);
+        }
+    }
+    acc += __shfl_xor(acc, 8, 16);
+    acc += __shfl_xor(acc, 4, 16);
+    acc += __shfl_xor(acc, 2, 16);
+    acc += __shfl_xor(acc, 1, 16);
+    return acc;
+}
+template <>
+__device__ __forceinline__ float dist_row_exact<float>(const float *__restrict__ qv,
+                                                       const float *__restrict__ base, uint64_t id,
+                                                       uint32_t d, uint32_t g) {
+    const float *xv = base + id * d;
+    float acc = 0.f;
+    for (uint32_t e = 4 * g; e < d; e += 64) {
+        const ex_f32x4 v = *reinterpret_cast<const ex_f32x4 *>(xv + e);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float t = qv[e + j] - v[j];
+            acc = __builtin_fmaf(t, t, acc);
+        }
+    }
+    acc += __shfl_xor(acc, 8, 16);
+    acc += __shfl_xor(acc, 4, 16);
+    acc += __shfl_xor(acc, 2, 16);
+    acc += __shfl_xor(acc, 1, 16);
+    return acc;
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
     uint32_t lane = threadIdx.x & 63u;
     return lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -178,6 +230,10 @@ __device__ void bitonic_sort_lds(T *keys, uint32_t n) {
         }
     }
     __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t next_pow2_dev(uint32_t v) {
+    return v <= 1u ? 1u : (1u << (32 - __clz(v - 1u)));
 }
 
 inline uint32_t next_pow2(uint32_t v) {

@@ -24,6 +24,7 @@ struct hrnn_dev {
 
     // launch shapes chosen by the autotuner (hrnn_dev_set_tuning)
     int tune_search_wg_per_cu = 8;   // persistent RkNN search workgroups per CU
+    int tune_search_lds_kib = 16;    // LDS dedup table of the RkNN search, KiB
     int tune_beam_wg_per_cu = 16;    // persistent beam workgroups (one wave each) per CU
 
     // ranked KNN graph (SoA)

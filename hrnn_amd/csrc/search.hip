@@ -7,47 +7,131 @@
 //                   L2Sqr(q, c) <= dist(c, k-1) (getVerifyKdistSq :286-293),
 //                   return accepted ids in ascending order.
 //
-// One persistent launch: each workgroup (256 lanes) owns a private visited
-// bitmap (threshold <= 1) or u16 count array (threshold > 1) over the n base
-// ids and a candidate buffer, and pulls queries from a device queue, so
-// queries with 100x different candidate counts balance across the chip.
-// Per query: the m reverse-list prefixes are located by binary search on the
-// rank field, gathered with coalesced reads, deduplicated in the private
-// array, and verified by 16-lane groups reading each candidate's vector as
-// contiguous 64-byte segments; the distance uses the reference's 16-partial
-// FMA order so accept/reject decisions match the CPU path bit for bit.
+// One persistent launch of 256-lane workgroups that pull queries from a
+// device queue, so queries with 100x different candidate counts balance
+// across the chip.  Per query the m reverse-list prefixes are located by
+// binary search on the rank field and gathered with coalesced reads.
+//
+// Deduplication and counting happen in an LDS hash table (open addressing,
+// linear probing, sized to the query's scanned-entry count at load <= 3/4):
+// u32 ids when only presence matters (threshold <= 1), u64 (count << 32 | id)
+// when counts are compared with the threshold.  The surviving ids are
+// compacted in place, verified by 16-lane groups (a rejected id is
+// overwritten with 0xFFFFFFFF by the one group that owns it), compacted again
+// and sorted, so HBM sees each scanned entry once and each candidate's vector
+// and k-distance once.  A query whose scanned entries do not fit the table
+// falls back to a per-workgroup visited bitmap / u16 count array in HBM.
+//
+// Distances use the reference's 16-partial FMA order, so accept/reject
+// decisions match the CPU path bit for bit; when every base value and query
+// value is an integer in [0, 255] and d <= 258, all partial sums are exact
+// integers, the order is immaterial, and each lane reads 16 contiguous bytes.
 #include <algorithm>
 #include <cfloat>
+#include <type_traits>
 
 #include "handle.h"
 
 namespace hrnn {
 
 constexpr int kSearchBlock = 256;
+constexpr int kSearchWaves = kSearchBlock / 64;
 constexpr int kMaxM = 1024;
-constexpr uint32_t kResLds = 4096;   // results sorted in LDS up to this count
+constexpr unsigned long long kEmpty64 = 0xFFFFFFFFull;   // count 0, id 0xFFFFFFFF
+
+// Insert candidate c into the LDS table of 2^logT slots.  Presence mode
+// stores the id; count mode adds one to the count in the high word.  The
+// table always holds fewer distinct ids than slots, so an empty or matching
+// slot is always found.
+template <bool kCount>
+__device__ __forceinline__ void tab_insert(void *tab, uint32_t logT, uint32_t c) {
+    const uint32_t mask = (1u << logT) - 1u;
+    uint32_t s = (c * 0x9E3779B1u) >> (32u - logT);
+    for (uint32_t probe = 0; probe <= mask; probe++, s = (s + 1u) & mask) {
+        if (kCount) {
+            unsigned long long *t = static_cast<unsigned long long *>(tab);
+            const unsigned long long old = atomicCAS(&t[s], kEmpty64, (1ull << 32) | c);
+            if (old == kEmpty64) return;
+            if ((uint32_t)old == c) {
+                atomicAdd(&t[s], 1ull << 32);
+                return;
+            }
+        } else {
+            uint32_t *t = static_cast<uint32_t *>(tab);
+            const uint32_t old = atomicCAS(&t[s], kInvalidId, c);
+            if (old == kInvalidId || old == c) return;
+        }
+    }
+}
+
+// Block-wide in-place compaction: for i in [0, n), pred(i, &v) decides whether
+// entry i survives and gives its u32 value; survivors are written, in order, to
+// dst[0..).  Entry i is read before any write to an index >= the current
+// 256-entry chunk, so dst may overlay the storage that pred reads as long as
+// index i of dst does not lie beyond the bytes pred(i) reads.  Returns the
+// survivor count (the same in every lane).  n must be uniform in the block.
+template <class Pred>
+__device__ __forceinline__ uint32_t block_compact(uint32_t n, uint32_t *dst, uint32_t *s_wc,
+                                                  Pred pred) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t out = 0;
+    for (uint32_t b = 0; b < n; b += kSearchBlock) {
+        const uint32_t i = b + threadIdx.x;
+        uint32_t v = 0;
+        const bool keep = (i < n) && pred(i, &v);
+        const uint64_t bal = __ballot(keep);
+        if (lane == 0) s_wc[wave] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        uint32_t before = 0, chunk = 0;
+#pragma unroll
+        for (int w = 0; w < kSearchWaves; w++) {
+            const uint32_t c = s_wc[w];
+            before += ((uint32_t)w < wave) ? c : 0u;
+            chunk += c;
+        }
+        if (keep) dst[out + before + (uint32_t)__popcll(bal & lanemask_lt())] = v;
+        out += chunk;
+        __syncthreads();
+    }
+    return out;
+}
+
+template <class VT>
+__device__ __forceinline__ float cand_dist(bool exact, const float *qv, const VT *x, uint32_t c,
+                                           uint32_t d, uint32_t g) {
+    return exact ? dist_row_exact<VT>(qv, x, c, d, g) : dist_row<VT>(qv, x, c, d, g);
+}
 
 template <class EntryT, bool kCount, class VT>
 __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
-    const VT *__restrict__ x, uint32_t d, const uint64_t *__restrict__ off,
+    const VT *__restrict__ x, uint32_t d, int base_exact, const uint64_t *__restrict__ off,
     const EntryT *__restrict__ ent, uint32_t rank_bits, const float *__restrict__ knn_d,
     const uint32_t *__restrict__ knn_cnt, uint32_t K, const float *__restrict__ q, uint32_t nq,
     const uint32_t *__restrict__ proxies, uint32_t m, const uint32_t *__restrict__ k_per_q,
-    uint32_t k, uint32_t K_prime, float threshold, uint32_t *__restrict__ mark_all,
+    uint32_t k, uint32_t K_prime, float threshold, uint32_t log_tmax, uint32_t *__restrict__ mark_all,
     uint64_t mark_words, uint32_t *__restrict__ cand_all, uint64_t cand_cap,
     uint32_t *__restrict__ queue, uint32_t *__restrict__ res_cnt, uint32_t *__restrict__ res_ids,
     uint32_t cap_per_q, uint32_t *__restrict__ work) {
-    __shared__ uint32_t s_q;
-    __shared__ uint32_t s_ncand, s_nkeep, s_nres;
-    __shared__ uint64_t s_start[kMaxM];
-    __shared__ uint32_t s_pos[kMaxM + 1];
-    __shared__ uint32_t s_res[kResLds];
-    extern __shared__ float s_qv[];
+    __shared__ uint32_t s_q, s_ncand, s_nkeep, s_nres;
+    __shared__ uint32_t s_wc[kSearchWaves];
+    // dynamic LDS: table (2^log_tmax slots) | query vector (d) | prefix starts
+    // (m u64) | prefix positions (m + 1 u32)
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    typedef typename std::conditional<kCount, unsigned long long, uint32_t>::type SlotT;
+    SlotT *s_tab = reinterpret_cast<SlotT *>(s_dyn);
+    uint32_t *s_t32 = reinterpret_cast<uint32_t *>(s_dyn);
+    const uint32_t tmax = 1u << log_tmax;
+    const uint32_t res_lds = tmax * (uint32_t)(sizeof(SlotT) / 4);   // u32 capacity of the table
+    float *s_qv = reinterpret_cast<float *>(s_dyn + (size_t)tmax * sizeof(SlotT));
+    uint64_t *s_start = reinterpret_cast<uint64_t *>(
+        s_dyn + (size_t)tmax * sizeof(SlotT) + (((size_t)d * 4 + 7) & ~(size_t)7));
+    uint32_t *s_pos = reinterpret_cast<uint32_t *>(s_start + m);
 
     uint32_t *mark = mark_all + (uint64_t)blockIdx.x * mark_words;
     uint32_t *cand = cand_all + (uint64_t)blockIdx.x * cand_cap * 2;
     uint32_t *keep = cand + cand_cap;
     const EntryT rmask = ((EntryT)1 << rank_bits) - 1;
+    const uint32_t grp = threadIdx.x / 16, g = threadIdx.x & 15u;
 
     for (;;) {
         if (threadIdx.x == 0) {
@@ -57,18 +141,23 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
             s_nres = 0;
         }
         __syncthreads();
-        uint32_t qi = s_q;
+        const uint32_t qi = s_q;
         if (qi >= nq) break;
-        for (uint32_t e = threadIdx.x; e < d; e += blockDim.x) s_qv[e] = q[(uint64_t)qi * d + e];
+        int not_u8 = 0;
+        for (uint32_t e = threadIdx.x; e < d; e += blockDim.x) {
+            const float v = q[(uint64_t)qi * d + e];
+            s_qv[e] = v;
+            not_u8 |= !(v >= 0.f && v <= 255.f && v == rintf(v));
+        }
         // locate each proxy's prefix with rank < K'
         for (uint32_t p = threadIdx.x; p < m; p += blockDim.x) {
-            uint32_t px = proxies[(uint64_t)qi * m + p];
+            const uint32_t px = proxies[(uint64_t)qi * m + p];
             uint64_t b = 0, len = 0;
             if (px != kInvalidId) {
                 b = off[px];
                 uint64_t lo = b, hi = off[px + 1];
                 while (lo < hi) {   // first entry with rank >= K'
-                    uint64_t mid = (lo + hi) >> 1;
+                    const uint64_t mid = (lo + hi) >> 1;
                     if ((uint32_t)(ent[mid] & rmask) < K_prime) lo = mid + 1; else hi = mid;
                 }
                 len = lo - b;
@@ -76,93 +165,134 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
             s_start[p] = b;
             s_pos[p + 1] = (uint32_t)len;
         }
-        __syncthreads();
+        const bool exact = base_exact && !__syncthreads_or(not_u8);
         if (threadIdx.x == 0) {
             s_pos[0] = 0;
             for (uint32_t p = 0; p < m; p++) s_pos[p + 1] += s_pos[p];
         }
         __syncthreads();
-        uint32_t total = s_pos[m];
-        // phase B: gather, dedupe and count
-        for (uint32_t t = threadIdx.x; t < total; t += blockDim.x) {
-            uint32_t lo = 0, hi = m;   // proxy p with s_pos[p] <= t < s_pos[p+1]
-            while (hi - lo > 1) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (s_pos[mid] <= t) lo = mid; else hi = mid;
+        const uint32_t total = s_pos[m];
+        const uint32_t kq = k_per_q ? k_per_q[qi] : k;
+        const uint32_t kidx = kq - 1;
+        uint32_t nkeep, nres;
+
+        if ((uint64_t)total * 4 <= (uint64_t)tmax * 3) {
+            // ---- LDS path: table of T >= 4/3 total slots ----
+            uint32_t logT = 8;
+            while ((1u << logT) * 3u < total * 4u) logT++;
+            const uint32_t T = 1u << logT;
+            for (uint32_t s = threadIdx.x; s < T; s += blockDim.x)
+                s_tab[s] = kCount ? (SlotT)kEmpty64 : (SlotT)kInvalidId;
+            __syncthreads();
+            for (uint32_t t = threadIdx.x; t < total; t += blockDim.x) {
+                uint32_t lo = 0, hi = m;   // proxy p with s_pos[p] <= t < s_pos[p+1]
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_pos[mid] <= t) lo = mid; else hi = mid;
+                }
+                const EntryT e = ent[s_start[lo] + (t - s_pos[lo])];
+                tab_insert<kCount>(s_tab, logT, (uint32_t)(e >> rank_bits));
             }
-            EntryT e = ent[s_start[lo] + (t - s_pos[lo])];
-            uint32_t c = (uint32_t)(e >> rank_bits);
-            bool first;
-            if (kCount) {
-                uint32_t sh = (c & 1u) * 16u;
-                uint32_t old = atomicAdd(&mark[c >> 1], 1u << sh);
-                first = ((old >> sh) & 0xFFFFu) == 0;
-            } else {
-                uint32_t bit = 1u << (c & 31u);
-                uint32_t old = atomicOr(&mark[c >> 5], bit);
-                first = (old & bit) == 0;
-            }
-            if (first) {
-                uint32_t s = atomicAdd(&s_ncand, 1u);
-                cand[s] = c;
-            }
-        }
-        __syncthreads();
-        uint32_t ncand = s_ncand;
-        // threshold filter and reset of the private array
-        for (uint32_t i = threadIdx.x; i < ncand; i += blockDim.x) {
-            uint32_t c = cand[i];
-            bool ok = true;
-            if (kCount) {
-                uint32_t sc = (mark[c >> 1] >> ((c & 1u) * 16u)) & 0xFFFFu;
-                ok = (float)sc >= threshold;
-            }
-            if (ok) {
-                uint32_t s = atomicAdd(&s_nkeep, 1u);
-                keep[s] = c;
-            }
-        }
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < ncand; i += blockDim.x) {
-            uint32_t c = cand[i];
-            mark[kCount ? (c >> 1) : (c >> 5)] = 0;
-        }
-        uint32_t nkeep = s_nkeep;
-        uint32_t kq = k_per_q ? k_per_q[qi] : k;
-        uint32_t kidx = kq - 1;
-        // phase C: verification, one 16-lane group per candidate
-        uint32_t grp = threadIdx.x / 16, g = threadIdx.x & 15u;
-        for (uint32_t i = grp; i < nkeep; i += blockDim.x / 16) {
-            uint32_t c = keep[i];
-            float dist = dist_row<VT>(s_qv, x, c, d, g);
-            if (g == 0) {
-                float kd = (kidx < knn_cnt[c]) ? knn_d[(uint64_t)c * K + kidx] : FLT_MAX;
-                if (dist <= kd) {
-                    uint32_t r = atomicAdd(&s_nres, 1u);
-                    if (r < kResLds) s_res[r] = c;
-                    if (r < cap_per_q) res_ids[(uint64_t)qi * cap_per_q + r] = c;
+            __syncthreads();
+            // threshold filter: surviving ids to s_t32[0..nkeep)
+            nkeep = block_compact(T, s_t32, s_wc, [&](uint32_t i, uint32_t *v) {
+                const SlotT sl = s_tab[i];
+                *v = (uint32_t)sl;
+                if (kCount) return (uint32_t)sl != kInvalidId && (float)(uint32_t)(sl >> 32) >= threshold;
+                return (uint32_t)sl != kInvalidId;
+            });
+            // phase C: one 16-lane group per candidate; rejected ids become
+            // 0xFFFFFFFF in place (slot i is read and written by its group only)
+            for (uint32_t i = grp; i < nkeep; i += blockDim.x / 16) {
+                const uint32_t c = s_t32[i];
+                const float dist = cand_dist<VT>(exact, s_qv, x, c, d, g);
+                if (g == 0) {
+                    const float kd = (kidx < knn_cnt[c]) ? knn_d[(uint64_t)c * K + kidx] : FLT_MAX;
+                    if (!(dist <= kd)) s_t32[i] = kInvalidId;
                 }
             }
-        }
-        __syncthreads();
-        uint32_t nres = s_nres;
-        uint32_t nout = nres < cap_per_q ? nres : cap_per_q;
-        if (nres <= kResLds) {
-            uint32_t P = 1;
-            while (P < nres) P <<= 1;
-            for (uint32_t t = nres + threadIdx.x; t < P; t += blockDim.x) s_res[t] = 0xFFFFFFFFu;
-            if (P > 1) bitonic_sort_lds(s_res, P);
+            __syncthreads();
+            nres = block_compact(nkeep, s_t32, s_wc, [&](uint32_t i, uint32_t *v) {
+                *v = s_t32[i];
+                return *v != kInvalidId;
+            });
+            const uint32_t P = next_pow2_dev(nres);
+            for (uint32_t t = nres + threadIdx.x; t < P; t += blockDim.x) s_t32[t] = kInvalidId;
+            if (P > 1) bitonic_sort_lds(s_t32, P);
+            const uint32_t nout = nres < cap_per_q ? nres : cap_per_q;
             for (uint32_t t = threadIdx.x; t < nout; t += blockDim.x)
-                res_ids[(uint64_t)qi * cap_per_q + t] = s_res[t];
+                res_ids[(uint64_t)qi * cap_per_q + t] = s_t32[t];
         } else {
-            // rare: more results than the LDS buffer; odd-even transposition sort in place
-            uint32_t *rr = res_ids + (uint64_t)qi * cap_per_q;
-            for (uint32_t pass = 0; pass < nout; pass++) {
-                for (uint32_t t = 2 * threadIdx.x + (pass & 1u); t + 1 < nout; t += 2 * blockDim.x) {
-                    uint32_t a = rr[t], b = rr[t + 1];
-                    if (a > b) { rr[t] = b; rr[t + 1] = a; }
+            // ---- HBM fallback: private visited bitmap / u16 counts ----
+            for (uint32_t t = threadIdx.x; t < total; t += blockDim.x) {
+                uint32_t lo = 0, hi = m;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_pos[mid] <= t) lo = mid; else hi = mid;
                 }
-                __syncthreads();
+                const EntryT e = ent[s_start[lo] + (t - s_pos[lo])];
+                const uint32_t c = (uint32_t)(e >> rank_bits);
+                bool first;
+                if (kCount) {
+                    const uint32_t sh = (c & 1u) * 16u;
+                    const uint32_t old = atomicAdd(&mark[c >> 1], 1u << sh);
+                    first = ((old >> sh) & 0xFFFFu) == 0;
+                } else {
+                    const uint32_t bit = 1u << (c & 31u);
+                    const uint32_t old = atomicOr(&mark[c >> 5], bit);
+                    first = (old & bit) == 0;
+                }
+                if (first) cand[atomicAdd(&s_ncand, 1u)] = c;
+            }
+            __syncthreads();
+            const uint32_t ncand = s_ncand;
+            for (uint32_t i = threadIdx.x; i < ncand; i += blockDim.x) {
+                const uint32_t c = cand[i];
+                bool ok = true;
+                if (kCount) {
+                    const uint32_t sc = (mark[c >> 1] >> ((c & 1u) * 16u)) & 0xFFFFu;
+                    ok = (float)sc >= threshold;
+                }
+                if (ok) keep[atomicAdd(&s_nkeep, 1u)] = c;
+            }
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < ncand; i += blockDim.x) {
+                const uint32_t c = cand[i];
+                mark[kCount ? (c >> 1) : (c >> 5)] = 0;
+            }
+            nkeep = s_nkeep;
+            for (uint32_t i = grp; i < nkeep; i += blockDim.x / 16) {
+                const uint32_t c = keep[i];
+                const float dist = cand_dist<VT>(exact, s_qv, x, c, d, g);
+                if (g == 0) {
+                    const float kd = (kidx < knn_cnt[c]) ? knn_d[(uint64_t)c * K + kidx] : FLT_MAX;
+                    if (dist <= kd) {
+                        const uint32_t r = atomicAdd(&s_nres, 1u);
+                        if (r < res_lds) s_t32[r] = c;
+                        if (r < cap_per_q) res_ids[(uint64_t)qi * cap_per_q + r] = c;
+                    }
+                }
+            }
+            __syncthreads();
+            nres = s_nres;
+            const uint32_t nout = nres < cap_per_q ? nres : cap_per_q;
+            if (nres <= res_lds) {
+                const uint32_t P = next_pow2_dev(nres);
+                for (uint32_t t = nres + threadIdx.x; t < P; t += blockDim.x) s_t32[t] = kInvalidId;
+                if (P > 1) bitonic_sort_lds(s_t32, P);
+                for (uint32_t t = threadIdx.x; t < nout; t += blockDim.x)
+                    res_ids[(uint64_t)qi * cap_per_q + t] = s_t32[t];
+            } else {
+                // rare: more results than the LDS table holds; odd-even
+                // transposition sort of the kept ids in place
+                uint32_t *rr = res_ids + (uint64_t)qi * cap_per_q;
+                for (uint32_t pass = 0; pass < nout; pass++) {
+                    for (uint32_t t = 2 * threadIdx.x + (pass & 1u); t + 1 < nout; t += 2 * blockDim.x) {
+                        const uint32_t a = rr[t], b = rr[t + 1];
+                        if (a > b) { rr[t] = b; rr[t + 1] = a; }
+                    }
+                    __syncthreads();
+                }
             }
         }
         if (threadIdx.x == 0) {
@@ -181,22 +311,24 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
                           float threshold, uint32_t *res_cnt, uint32_t *res_ids,
                           uint32_t cap_per_q, uint32_t *work) {
     REQUIRE(h->csr_valid, HRNN_E_STATE, "search: no CSR");
-    REQUIRE(m <= (uint32_t)kMaxM, HRNN_E_ARG, "search: m > 1024");
+    REQUIRE(m >= 1 && m <= (uint32_t)kMaxM, HRNN_E_ARG, "search: m must be in [1, 1024]");
     REQUIRE(cap_per_q > 0, HRNN_E_ARG, "search: cap_per_q == 0");
-    bool count_mode = threshold > 1.0f;
-    uint64_t n = h->n;
-    uint64_t words = count_mode ? (n + 1) / 2 : (n + 31) / 32;
+    REQUIRE(h->n < kInvalidId, HRNN_E_ARG, "search: n must be below 2^32 - 1");
+    const bool count_mode = threshold > 1.0f;
+    const uint64_t n = h->n;
+    const uint64_t words = count_mode ? (n + 1) / 2 : (n + 31) / 32;
     // distinct candidates <= sum of the m prefix lengths <= m * longest list
     uint64_t cand_cap = std::max<uint64_t>(1, (uint64_t)m * std::max<uint64_t>(h->max_deg, 1));
     cand_cap = std::min<uint64_t>(cand_cap, n);
     uint32_t slots = (uint32_t)std::min<uint64_t>(nq, (uint64_t)h->n_cu * h->tune_search_wg_per_cu);
-    // bound the private state to ~4 GB
-    uint64_t per_slot = words * 4 + cand_cap * 8;
-    uint64_t max_slots = std::max<uint64_t>(1, (4ull << 30) / per_slot);
+    slots = std::max<uint32_t>(slots, 1);
+    // bound the fallback's private state to ~4 GB
+    const uint64_t per_slot = words * 4 + cand_cap * 8;
+    const uint64_t max_slots = std::max<uint64_t>(1, (4ull << 30) / per_slot);
     slots = (uint32_t)std::min<uint64_t>(slots, max_slots);
     // private arrays (kept all-zero between launches by the kernel itself) and
     // candidate buffers are owned by the search alone
-    uint64_t mark_bytes = (uint64_t)slots * words * 4;
+    const uint64_t mark_bytes = (uint64_t)slots * words * 4;
     if (h->search_mark.bytes < mark_bytes) {
         h->search_mark.reserve(mark_bytes);
         HIPC(hipMemsetAsync(h->search_mark.p, 0, h->search_mark.bytes, h->stream));
@@ -204,7 +336,15 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     h->search_cand.reserve((uint64_t)slots * cand_cap * 8);
     uint32_t *queue = h->counters.as<uint32_t>() + 64;
     HIPC(hipMemsetAsync(queue, 0, 4, h->stream));
-    size_t dyn = (size_t)h->d * sizeof(float);
+    // LDS table: the largest power-of-two slot count within the configured
+    // bytes (u32 slots in presence mode, u64 in count mode), at least 256
+    const uint32_t slot_bytes = count_mode ? 8u : 4u;
+    uint32_t log_tmax = 8;
+    while ((2ull << log_tmax) * slot_bytes <= (uint64_t)h->tune_search_lds_kib * 1024) log_tmax++;
+    const size_t dyn = ((size_t)1 << log_tmax) * slot_bytes + (((size_t)h->d * 4 + 7) & ~(size_t)7) +
+                       (size_t)m * 8 + ((size_t)m + 1) * 4;
+    REQUIRE(dyn <= std::min<size_t>(h->lds_per_block, 65536), HRNN_E_ARG,
+            "search: LDS need " + std::to_string(dyn) + " B exceeds the workgroup limit");
     const uint64_t *off = h->csr_off.as<uint64_t>();
     const float *kd = h->knn_d.as<float>();
     const uint32_t *kc = h->knn_cnt.as<uint32_t>();
@@ -213,11 +353,12 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     // Candidate vectors are read from the BF16 mirror when every base value is
     // exactly representable in it (same distances, half the bytes).
     const bool bf16 = h->small_int && h->vec_bf16;
+    const int base_exact = (h->base_u8 && h->d <= 258 && h->d % 8 == 0) ? 1 : 0;
 #define LAUNCH(ET, CM, VT, XP)                                                                      \
     hipLaunchKernelGGL((k_rknn_search<ET, CM, VT>), dim3(slots), dim3(kSearchBlock), dyn, h->stream, \
-                       XP, h->d, off, h->csr_ent.as<ET>(), h->rank_bits, kd, kc, h->K, q, nq, proxies, \
-                       m, k_per_q, k, K_prime, threshold, mark, words, cand, cand_cap, queue, res_cnt, \
-                       res_ids, cap_per_q, work)
+                       XP, h->d, base_exact, off, h->csr_ent.as<ET>(), h->rank_bits, kd, kc, h->K, q, \
+                       nq, proxies, m, k_per_q, k, K_prime, threshold, log_tmax, mark, words, cand,  \
+                       cand_cap, queue, res_cnt, res_ids, cap_per_q, work)
 #define LAUNCH_VT(ET, CM)                                                  \
     if (bf16) LAUNCH(ET, CM, __hip_bfloat16, h->xb.as<__hip_bfloat16>());  \
     else LAUNCH(ET, CM, float, h->x.as<float>())

@@ -80,7 +80,10 @@ const float *hrnn_dev_base_ptr(hrnn_dev_t h);
  * vector bytes halve) and FP32 otherwise; HRNN_PREC_F32 always reads FP32. */
 int hrnn_dev_set_vector_precision(hrnn_dev_t h, int prec);
 /* Launch shapes set by the autotuner (hrnn_amd/autotune.py): "search_wg_per_cu"
- * (persistent RkNN search workgroups per CU, default 8); "beam_wg_per_cu" is
+ * (persistent RkNN search workgroups per CU, default 8); "search_lds_kib" (KiB
+ * of LDS for the search's per-query deduplication table, default 16; a query
+ * whose scanned entries exceed 3/4 of the table's slots uses a visited bitmap
+ * in HBM instead, with identical results); "beam_wg_per_cu" is
  * accepted and unused (phase A launches one workgroup per query); values in
  * [1, 64]. */
 int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value);

@@ -145,6 +145,34 @@ def test_search_vs_oracle(small_data, small_ref, m, Kp, k, thr):
         assert np.array_equal(gids.astype(np.uint64), rlab)
 
 
+@pytest.mark.parametrize("lds_kib,thr,qshift", [(1, 1.0, 0.0), (1, 2.0, 0.0), (32, 1.0, 0.5),
+                                                 (1, 3.0, 0.25), (8, 2.0, 0.0)])
+def test_search_lds_table_and_hbm_fallback(small_data, small_ref, lds_kib, thr, qshift):
+    """A 256-slot LDS table sends every query with more than 192 scanned entries
+    to the HBM visited-bitmap path, in the same launch as the queries (every
+    second one, with 2 proxies) that stay in LDS; shifted queries are not
+    integers, so their distances take the reference's 16-partial order instead
+    of the exact 16-byte-row form."""
+    x, q = small_data
+    q = (q + np.float32(qshift)).astype(np.float32)
+    r, _ = small_ref
+    ids, dists, cnt = r.get_knng()
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    dev.set_knng(ids, dists, cnt)
+    dev.build_rknn_csr()
+    dev.set_tuning("search_lds_kib", lds_kib)
+    assert dev.get_tuning("search_lds_kib") == lds_kib
+    off, ent = dev.get_rknn_csr()
+    prox = r.proxies(q, 8, 64)
+    prox[::2, 2:] = 0xFFFFFFFF
+    goff, gids = dev.rknn_search(q, prox, 10, 48, thr)
+    eoff, eids = oracle.search_bc(off, ent, dev.csr.wide, dev.csr.rank_bits, dists, cnt, r.K, x,
+                                  q, prox, 10, 48, thr)
+    assert np.array_equal(goff, eoff)
+    assert np.array_equal(gids, eids)
+
+
 def test_search_float_data_and_k_per_query(float_data):
     """Float vectors: accept/reject uses the reference's FP32 order, so results are identical."""
     x, q = float_data
