@@ -19,13 +19,21 @@ reference's structure): phase A on the host cores in batches of 1000, proxies
 copied to HBM, phases B and C on the GPU, the host working on batch i+1 while
 the GPU searches batch i; with fixed m (host_a_bf16_fixed_m) or the learned
 per-query m (bf16_learned_m_host_a: host features, GPU MLP, GPU B and C).
-value = the highest QPS among the paths whose macro recall is >= 0.95.
+Learned early stop (bf16_learned_adaptive, all on the GPU in one call): a
+stage-1 beam at ef 16 for every query with the features fused in, the MLP,
+a stage-2 beam at the chosen class's ef only for the queries that need it,
+phases B and C with the class's m; trained on 20,000 separate queries
+labelled by GPU runs of every class against the exact groundtruth.
+value = the highest QPS among the paths whose macro recall is >= 0.95;
+speedup_vs_fp32_fixed_m sets it against the plain FP32 fixed-m GPU path.
 
-roofline: the dominant GPU kernel of that path (k_phase_a or k_rknn_search),
-timed with HIP events on the library's stream over resident launches
-(algorithmic bytes from the kernels' own work counters).  cpu_baseline: the reference's own searchRknn (compiled from its
-sources, oracle/_ref), single-threaded as its hrnn_search driver runs it, on a
-bounded query sample of this run's index (loaded from its saved file).
+roofline: the dominant GPU kernel of that path (k_phase_a, k_phase_a_stage2
+or k_rknn_search), timed with HIP events on the library's stream over
+resident launches (algorithmic bytes from the kernels' own work counters).
+cpu_baseline: the reference's own searchRknn (compiled from its sources,
+oracle/_ref) on this run's index loaded from its saved file, OpenMP over all
+queries on the host cores; cpu_baseline_one_thread: its hrnn_search driver
+loop on one thread.
 """
 from __future__ import annotations
 
@@ -95,7 +103,8 @@ def main() -> int:
     from hrnn_amd import hiprt
     from hrnn_amd.dev import MLP_MODEL, Device
     from hrnn_amd.index import Index
-    from hrnn_amd.learned import GRID, LearnedM, label_queries
+    from hrnn_amd.learned import (CLASSES, EF1, GRID, LearnedEarlyStop, LearnedM, label_classes,
+                                  label_queries)
 
     cfg = CONFIGS[args.config]
     threads = args.threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or 16
@@ -318,6 +327,45 @@ def main() -> int:
                         "setup_s": round(time.time() - t, 2)}
         log(f"learned m: tau {tau_star} mean m {mean_m:.2f} recall {rec_l:.4f}")
 
+        # ---- learned early stop, all on the GPU: stage-1 beam at EF1 with fused
+        # features, MLP, stage-2 beam only for the queries whose class needs it
+        t = time.time()
+        ck(L.hrnn_dev_set_vector_precision(h, 1))
+        fa, _, _ = dev.stage1_features(qt, EF1, 16, k)
+        lab_a, rec_a, exp_a = label_classes(dev, qt, gt_t_off, gt_t_ids, k, Kp)
+        model_a, losses_a = LearnedEarlyStop.train(dev, fa[:n_fit], lab_a[:n_fit], epochs=60, batch=256,
+                                                   lr=2e-3)
+        tau_a = model_a.pick_tau(dev, fa[n_fit:], rec_a[:, n_fit:], args.recall_target + 0.01)
+        model_ad = hiprt.DeviceArray.from_host(model_a.model())
+        classes_d = hiprt.DeviceArray.from_host(np.concatenate([CLASSES[:, 0], CLASSES[:, 1]]))
+        cls_d = hiprt.DeviceArray((nq,), np.uint32)
+        exp_d = hiprt.DeviceArray((4 * nq,), np.uint32)
+        ef_max = int(CLASSES[:, 0].max())
+
+        def run_adaptive(tau=tau_a):
+            ck(L.hrnn_dev_rknn_query_adaptive_dptr(h, qd.ptr, nq, model_ad.ptr, classes_d.ptr, ef_max, tau,
+                                                   EF1, 16, k, Kp, 1.0, pd.ptr, cls_d.ptr, cd.ptr, idd.ptr,
+                                                   cap, wd.ptr, exp_d.ptr))
+
+        run_adaptive()
+        a_off, a_ids = results()
+        rec_ad = macro_recall(gt_off, gt_ids, a_off, a_ids)
+        cls_h = cls_d.to_host()
+        exq = exp_d.to_host().reshape(2, nq, 2)
+        paths["bf16_learned_adaptive"] = {
+            "tau": tau_a, "recall": round(rec_ad, 5), "phase_a": "gpu", "ef1": EF1,
+            "mean_m": round(float(CLASSES[cls_h, 1].mean()), 2),
+            "escalated": round(float((CLASSES[cls_h, 0] > EF1).mean()), 4),
+            "mean_expansions": round(float(exq[0, :, 0].mean() + exq[1, :, 0].mean()), 1),
+            "class_hist": np.bincount(cls_h, minlength=16).tolist(),
+            "sec": timed(run_adaptive)}
+        learned_info["adaptive"] = {
+            "classes": CLASSES.tolist(), "label_hist": np.bincount(lab_a, minlength=16).tolist(),
+            "loss_first": round(float(losses_a[0]), 4), "loss_last": round(float(losses_a[-1]), 4),
+            "tau": tau_a, "setup_s": round(time.time() - t, 2)}
+        log(f"learned early stop: tau {tau_a}, recall {rec_ad:.4f}, escalated "
+            f"{paths['bf16_learned_adaptive']['escalated']:.3f}, classes {np.bincount(cls_h, minlength=16).tolist()}")
+
     for name, p in paths.items():
         p["qps"] = round(nq / p["sec"], 1)
         p["ms_per_step"] = round(p["sec"] * 1e3, 3)
@@ -333,7 +381,38 @@ def main() -> int:
     ck(L.hrnn_dev_set_vector_precision(h, 0 if best_name == "fp32_fixed_m" else 1))
     m_b = best.get("m", m_max)
     kern = {}
-    if best.get("phase_a") == "gpu":
+
+    def ev_ms(fn):
+        fn()
+        hiprt.stream_sync(stream)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.sync()
+        return e1.ms_since(e0) / reps
+
+    adaptive = best_name == "bf16_learned_adaptive"
+    if adaptive:
+        # stage 1 alone (k_phase_a<bf16, false> at EF1, features fused), then the
+        # whole two-stage query, which leaves the masked proxies (stride 16) in pd
+        s1_ms = ev_ms(lambda: ck(L.hrnn_dev_stage1_features_dptr(h, qd.ptr, nq, EF1, 16, k, pd.ptr, fd.ptr,
+                                                                 ed.ptr)))
+        ex = ed.to_host().reshape(nq, 2).astype(np.int64)
+        # per query: the query, each expanded row, each evaluated vector, 16
+        # proxy ids, 16 r_k reads (count + distance) and 32 features written
+        s1_bytes = float((4 * d + ex[:, 0] * 4 * (2 * cfg["M"] + 1) + ex[:, 1] * s_v * d + 4 * 16 + 16 * 8
+                          + 4 * 32).sum())
+        kern["k_phase_a"] = {"ms": s1_ms, "bytes": s1_bytes, "stage": 1, "ef": EF1,
+                             "expansions_per_query": round(float(ex[:, 0].mean()), 1),
+                             "evaluations_per_query": round(float(ex[:, 1].mean()), 1)}
+        tot_ms = ev_ms(run_adaptive)
+        exq = exp_d.to_host().reshape(2, nq, 2).astype(np.int64)
+        esc = exq[1, :, 0] > 0
+        s2_bytes = float((esc * (4 * d + 4 * 16) + exq[1, :, 0] * 4 * (2 * cfg["M"] + 1)
+                          + exq[1, :, 1] * s_v * d).sum())
+        m_b = 16
+    elif best.get("phase_a") == "gpu":
         ck(L.hrnn_dev_proxies_dptr(h, qd.ptr, nq, m_b, max(ef, m_b), pd.ptr, ed.ptr))
         hiprt.stream_sync(stream)
         e0.record(stream)
@@ -354,6 +433,9 @@ def main() -> int:
     if best_name.startswith("bf16_learned_m"):
         run_learned(best["tau"])            # leaves the masked proxies in pd (stride m_max)
         hiprt.stream_sync(stream)
+    elif adaptive:
+        run_adaptive()                      # masked proxies in pd, stride 16
+        hiprt.stream_sync(stream)
     search_batch(0, nq, m_b)
     hiprt.stream_sync(stream)
     e0.record(stream)
@@ -365,12 +447,20 @@ def main() -> int:
     wk = wd.to_host().reshape(nq, 2).astype(np.int64)
     rc_ = cd.to_host().astype(np.int64)
     # per query: the query vector, 16 B per proxy (CSR offsets), 4 B per scanned
-    # reverse-list entry, and per distinct candidate its vector, r_k and mark word,
-    # plus 4 B per result id
-    search_bytes = float((4 * d + 16 * m_b + 4 * wk[:, 0] + wk[:, 1] * (s_v * d + 4 + 4) + 4 * rc_).sum())
+    # reverse-list entry, per distinct candidate its vector and r_k (the LDS
+    # table holds the visited set), plus 4 B per result id
+    search_bytes = float((4 * d + 16 * m_b + 4 * wk[:, 0] + wk[:, 1] * (s_v * d + 4) + 4 * rc_).sum())
     kern["k_rknn_search"] = {"ms": search_ms, "bytes": search_bytes,
                              "scanned_per_query": round(float(wk[:, 0].mean()), 1),
                              "candidates_per_query": round(float(wk[:, 1].mean()), 1)}
+    if adaptive:
+        # the stage-2 beams (k_phase_a<bf16, true>): whole-call time less stage 1
+        # and the search (so it also holds the MLP and class-pick launches)
+        kern["k_phase_a_stage2"] = {"ms": max(tot_ms - s1_ms - search_ms, 1e-6), "bytes": s2_bytes,
+                                    "queries": int(esc.sum()),
+                                    "expansions_per_escalated_query":
+                                        round(float(exq[1, esc, 0].mean()) if esc.any() else 0.0, 1),
+                                    "whole_call_ms": round(tot_ms, 4)}
     for kk, v in kern.items():
         v["gbs"] = v["bytes"] / (v["ms"] * 1e-3) / 1e9
         log(f"  {kk}: {v['ms']:.3f} ms/launch over {nq} queries, {v['gbs']:.1f} GB/s algorithmic")
@@ -463,6 +553,10 @@ def main() -> int:
         "cpu_baseline": cpu,
         "cpu_baseline_one_thread": cpu1,
         "paths": paths,
+        # the learned-path target: BF16 + learned early stop against the plain
+        # FP32 fixed-m GPU path, both at recall >= the target
+        "speedup_vs_fp32_fixed_m": (round(best["qps"] / paths["fp32_fixed_m"]["qps"], 3)
+                                    if "fp32_fixed_m" in paths else None),
         "learned_m": learned_info,
         "autotune": {kk: v for kk, v in tuned.items() if kk != "timings_ms"},
         "nndescent_build": {"seconds": round(nnd_s, 3), "iters": cfg["iters"], "sample": cfg["S"],

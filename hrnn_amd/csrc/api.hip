@@ -10,7 +10,7 @@ using namespace hrnn;
 
 hrnn_dev::~hrnn_dev() {
     (void)hipSetDevice(device);
-    DevBuf *bufs[] = {&x, &xb, &xh, &norms, &knn_ids, &knn_d, &knn_cnt, &knn_new, &csr_off,
+    DevBuf *bufs[] = {&x, &xb, &xh, &norms, &knn_ids, &knn_d, &knn_cnt, &knn_new, &rk_col, &csr_off,
                       &csr_ent, &search_mark, &search_cand, &l0, &up_off, &up_pool, &beam_vis,
                       &s0, &s1, &s2, &s3, &s4, &s5, &s6, &s7, &s8, &nnd_aux, &counters};
     for (DevBuf *b : bufs) b->release();
@@ -310,6 +310,7 @@ int hrnn_dev_set_knng(hrnn_dev_t h, uint32_t K, const uint32_t *ids, const float
         HIPC(hipMemsetAsync(h->knn_new.p, 1, n * K, h->stream));
         HIPC(hipStreamSynchronize(h->stream));
         h->csr_valid = false;
+        h->knn_version++;
     });
 }
 
@@ -331,6 +332,7 @@ int hrnn_dev_get_knng(hrnn_dev_t h, uint32_t *ids, float *dists, uint16_t *cnt) 
 
 int hrnn_dev_knng_ptrs(hrnn_dev_t h, uint32_t **ids, float **dists, uint32_t **cnt) {
     if (!h) return HRNN_E_ARG;
+    h->knn_version++;   // the caller may write through these pointers
     if (ids) *ids = h->knn_ids.as<uint32_t>();
     if (dists) *dists = h->knn_d.as<float>();
     if (cnt) *cnt = h->knn_cnt.as<uint32_t>();
@@ -340,6 +342,7 @@ int hrnn_dev_knng_ptrs(hrnn_dev_t h, uint32_t **ids, float **dists, uint32_t **c
 int hrnn_dev_nndescent(hrnn_dev_t h, int num_iters, int sample_size, uint32_t bucket,
                        hrnn_nnd_iter_stats *stats) {
     if (!h) return HRNN_E_ARG;
+    h->knn_version++;
     return guarded(h, [&] {
         REQUIRE(h->K > 0, HRNN_E_STATE, "nndescent: set_knng first");
         REQUIRE(num_iters >= 0 && sample_size > 0 && sample_size <= 64, HRNN_E_ARG,
@@ -428,8 +431,10 @@ int hrnn_dev_rknn_search(hrnn_dev_t h, const float *q, uint32_t nq, const uint32
                     "rknn_search: proxy id >= n");
         res_off[0] = 0;
         if (nq == 0) { if (needed) *needed = 0; return; }
-        uint32_t kp_eff = std::min<uint32_t>(K_prime, h->K);
-        uint64_t worst = (uint64_t)m * std::max<uint32_t>(kp_eff, 1);
+        // a query's results are distinct candidates: at most m reverse-list
+        // prefixes, each no longer than the longest reverse list (a proxy's
+        // in-degree, which K' does not bound)
+        const uint64_t worst = std::min<uint64_t>((uint64_t)m * std::max<uint64_t>(h->max_deg, 1), h->n);
         uint32_t cap_q = (uint32_t)std::min<uint64_t>(worst, 4096);
         const uint32_t chunk = 16384;
         uint32_t qc = std::min(nq, chunk);
@@ -482,6 +487,7 @@ int hrnn_dev_rknn_search(hrnn_dev_t h, const float *q, uint32_t nq, const uint32
                                          (uint32_t)worst);
                     uint32_t c2 = 0;
                     d2h(h, &c2, tc, 4);
+                    REQUIRE(c2 <= worst, HRNN_E_STATE, "rknn_search: result count above its bound");
                     std::vector<uint32_t> v(c2);
                     d2h(h, v.data(), ti, (uint64_t)c2 * 4);
                     out.insert(out.end(), v.begin(), v.end());

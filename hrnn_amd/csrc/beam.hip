@@ -12,18 +12,22 @@
 // order of equal distances.
 //
 // k_phase_a: one query per workgroup of one wave (64 lanes), no state kept
-// between workgroups.  The sorted list lives in LDS (double buffered); visited
-// nodes go into an LDS hash set of 4096 ids (linear probing).  When a probe
-// run is exhausted the node is treated as unseen and admission checks the
-// list for it, so the result does not depend on the table size.  Per
-// expansion the fresh neighbours' distances are computed by 16-lane groups,
-// the admissible ones sorted across the wave and merged into the list in
-// parallel (the first ef of the merged (dist, id) order: the same set a
-// sequential insertion keeps).  When every base and query value is an integer
-// in [0, 255] and d <= 258, every distance sum is an exact integer, so rows
-// are read with 16-byte loads and summed in any order; otherwise the
-// reference's 16-partial order is used.  Every id or count read from the graph
-// is range-checked; a bad one is skipped and raises the handle's device error
+// between workgroups.  The sorted list lives in LDS as 64-bit (dist, id) keys,
+// double buffered, with expanded flags; visited nodes go into an LDS hash set
+// of 1024-4096 ids sized by ef (triangular probing).  When a probe run is exhausted
+// the node is treated as unseen and admission checks the list for it, so the
+// result does not depend on the table size.  Per expansion the fresh
+// neighbours' distances are computed by 16-lane groups, four rows per group
+// in flight; the admissible ones are sorted across next_pow2(count) lanes and
+// merged into the list with ranks counted by wave ballots (the first ef of the
+// merged (dist, id) order: the same set a sequential insertion keeps).  The
+// next node to expand is known before the merge is written (the smaller of
+// the best candidate and the first open entry), so its layer-0 row is loaded
+// while the merge runs.  When every base and query value is an integer in
+// [0, 255] and d <= 258, every distance sum is an exact integer, so rows are
+// read with 16-byte loads and summed in any order; otherwise the reference's
+// 16-partial order is used.  Every id or count read from the graph is
+// range-checked; a bad one is skipped and raises the handle's device error
 // word (reported by hrnn_dev_sync).  The descent and the beam run bounded
 // loops.
 #include <algorithm>
@@ -35,36 +39,49 @@ namespace hrnn {
 
 constexpr int kBeamMaxEf = 512;
 constexpr int kBeamMaxDeg = 64;
-constexpr uint32_t kVisSlots = 4096;
 constexpr uint32_t kVisProbes = 32;
 
-// Insert id into the LDS set; true when the id was not there (or could not be
-// recorded: *lost is set then).
-__device__ __forceinline__ bool vis_insert(uint32_t *tab, uint32_t id, uint32_t *lost) {
-    uint32_t h = (id * 2654435761u) >> 20;   // 12 bits -> [0, 4096)
+// Insert id into the LDS set of 2^log_slots ids; true when the id was not
+// there (or could not be recorded: *lost is set then).  Triangular probing
+// (offsets 0, 1, 3, 6, ...) visits every slot of a power-of-two table and
+// avoids the long runs of linear probing.
+__device__ __forceinline__ bool vis_insert(uint32_t *tab, uint32_t log_slots, uint32_t id,
+                                           uint32_t *lost) {
+    const uint32_t mask = (1u << log_slots) - 1u;
+    uint32_t h = (id * 2654435761u) >> (32u - log_slots);
     for (uint32_t p = 0; p < kVisProbes; p++) {
-        uint32_t old = atomicCAS(&tab[h], kInvalidId, id);
+        const uint32_t old = atomicCAS(&tab[h], kInvalidId, id);
         if (old == kInvalidId) return true;
         if (old == id) return false;
-        h = (h + 1) & (kVisSlots - 1);
+        h = (h + p + 1u) & mask;
     }
     *lost = 1u;
     return true;
 }
 
+// Distances of four rows per 16-lane group (row r valid iff val[r]; val is
+// uniform within the group).  Exact-sum data issues the four row reads
+// together; otherwise the reference's 16-partial order, one row at a time.
 template <class VT>
-__device__ __forceinline__ float beam_dist(bool exact, const float *qv, const VT *x, uint64_t id, uint32_t d,
-                                           uint32_t g) {
-    return exact ? dist_row_exact<VT>(qv, x, id, d, g) : dist_row<VT>(qv, x, id, d, g);
+__device__ __forceinline__ void beam_dist4(bool exact, const float *qv, const VT *x, const uint32_t *id,
+                                           const bool *val, uint32_t d, uint32_t g, float *o) {
+    if (exact) {
+        dist4_exact<VT>(qv, x, id, d, g, o);
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++) o[r] = val[r] ? dist_row<VT>(qv, x, id[r], d, g) : FLT_MAX;
+    }
 }
 
 __device__ __forceinline__ unsigned long long beam_key(float d, uint32_t id) {
     return ((unsigned long long)float_key(d) << 32) | id;
 }
 
-// Ascending bitonic sort of one 64-bit key per lane across the wave.
-__device__ __forceinline__ unsigned long long wave_sort64(unsigned long long key, uint32_t lane) {
-    for (uint32_t k = 2; k <= 64; k <<= 1) {
+// Ascending bitonic sort of one 64-bit key per lane over lanes [0, W) (W a
+// power of two <= 64, uniform); lanes >= W keep their key.
+__device__ __forceinline__ unsigned long long wave_sort64(unsigned long long key, uint32_t lane,
+                                                          uint32_t W) {
+    for (uint32_t k = 2; k <= W; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
             const uint32_t lo = (uint32_t)key, hi = (uint32_t)(key >> 32);
             const unsigned long long other =
@@ -78,26 +95,66 @@ __device__ __forceinline__ unsigned long long wave_sort64(unsigned long long key
     return key;
 }
 
-template <class VT>
+// Lane j's 64-bit value, j uniform.
+__device__ __forceinline__ unsigned long long read_lane64(unsigned long long v, uint32_t j) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)j);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)j);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Layer-0 row of node c: the neighbour count (uniform) and, in lane j, the
+// j-th neighbour id.  Both loads are issued before either is used.
+__device__ __forceinline__ void l0_row(const uint32_t *__restrict__ l0, uint32_t l0_words, uint32_t c,
+                                       uint32_t lane, uint32_t *cnt, uint32_t *nb) {
+    const uint32_t *ll = l0 + (uint64_t)c * l0_words;
+    const uint32_t w0 = ll[0];
+    *nb = (lane + 1u < l0_words) ? ll[1 + lane] : 0u;
+    *cnt = min(w0 & 0xFFFFu, l0_words - 1u);
+}
+
+// LDS layout of k_phase_a (dynamic part), in bytes: query vector, visited
+// set, list keys [2][ef_cap], expanded flags [2][ef_cap].
+struct BeamLds {
+    uint32_t qv_bytes, vis_off, k_off, x_off, total;
+    __host__ __device__ BeamLds(uint32_t d, uint32_t ef_cap, uint32_t log_vis) {
+        qv_bytes = (d * 4u + 15u) & ~15u;
+        vis_off = qv_bytes;
+        k_off = vis_off + (4u << log_vis);
+        x_off = k_off + 2u * ef_cap * 8u;
+        total = x_off + 2u * ef_cap;
+    }
+};
+
+// kPerQueryEf: the stage-2 form of the two-stage learned query (opt.ef_q
+// gives each query's beam width; 0 skips the query).  A separate
+// instantiation so profiles tell the two stages apart.
+template <class VT, bool kPerQueryEf>
 __global__ __launch_bounds__(64) void k_phase_a(
     const VT *__restrict__ x, uint32_t d, uint32_t n, const uint32_t *__restrict__ l0,
     uint32_t l0_words, const uint32_t *__restrict__ up_off, const uint32_t *__restrict__ up_pool,
     uint64_t pool_words, uint32_t up_words, uint32_t entry, int maxlevel, const float *__restrict__ q,
-    uint32_t ef, uint32_t m, int base_exact, uint32_t *__restrict__ proxies,
-    uint32_t *__restrict__ expansions, float *__restrict__ raw, uint32_t *__restrict__ err) {
-    __shared__ float s_d[2][kBeamMaxEf];
-    __shared__ uint32_t s_i[2][kBeamMaxEf];
-    __shared__ uint8_t s_x[2][kBeamMaxEf];   // expanded flag
+    uint32_t ef, uint32_t ef_cap, uint32_t log_vis, uint32_t m, int base_exact,
+    uint32_t *__restrict__ proxies, uint32_t *__restrict__ expansions, float *__restrict__ raw,
+    uint32_t *__restrict__ err, BeamOpt opt) {
     __shared__ float s_nd[kBeamMaxDeg];
     __shared__ uint32_t s_ni[kBeamMaxDeg];
-    __shared__ unsigned long long s_ck[kBeamMaxDeg];   // sorted candidate keys of one expansion
-    __shared__ uint32_t s_vis[kVisSlots];
     __shared__ uint32_t s_lost, s_bad;
-    extern __shared__ float s_qv[];
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    const BeamLds lay(d, ef_cap, log_vis);
+    float *s_qv = reinterpret_cast<float *>(s_dyn);
+    uint32_t *s_vis = reinterpret_cast<uint32_t *>(s_dyn + lay.vis_off);
+    // the sorted list as (dist, id) keys, double buffered, with expanded flags
+    unsigned long long *s_kl = reinterpret_cast<unsigned long long *>(s_dyn + lay.k_off);
+    uint8_t *s_xl = s_dyn + lay.x_off;
 
     const uint32_t lane = threadIdx.x;
     const uint32_t g = lane & 15u, grp = lane >> 4;
     const uint64_t qi = blockIdx.x;
+    if (kPerQueryEf) {   // per-query beam width; 0 leaves this query to another launch
+        const uint32_t e = opt.ef_q[qi];
+        if (e == 0) return;
+        ef = min(e, ef);
+    }
     bool q_u8 = true;
     for (uint32_t e = lane; e < d; e += 64) {
         const float v = q[qi * d + e];
@@ -106,7 +163,7 @@ __global__ __launch_bounds__(64) void k_phase_a(
     }
     // exact sums for this query: every base and query value an integer in [0, 255]
     const bool exact = base_exact && __ballot(!q_u8) == 0;
-    for (uint32_t t = lane; t < kVisSlots; t += 64) s_vis[t] = kInvalidId;
+    for (uint32_t t = lane; t < (1u << log_vis); t += 64) s_vis[t] = kInvalidId;
     if (lane == 0) {
         s_lost = 0;
         s_bad = 0;
@@ -115,7 +172,14 @@ __global__ __launch_bounds__(64) void k_phase_a(
 
     // ---- greedy descent over the upper layers ----
     uint32_t cur = entry < n ? entry : 0;
-    float cd = __shfl(beam_dist<VT>(exact, s_qv, x, cur, d, g), 0, 64);
+    float cd;
+    {
+        const uint32_t id4[4] = {cur, cur, cur, cur};
+        const bool v4[4] = {true, false, false, false};
+        float o[4];
+        beam_dist4<VT>(exact, s_qv, x, id4, v4, d, g, o);
+        cd = __shfl(o[0], 0, 64);
+    }
     const float d_entry = cd;
     for (int level = maxlevel; level > 0; level--) {
         for (uint32_t step = 0; step < n; step++) {   // each move strictly lowers cd
@@ -126,27 +190,38 @@ __global__ __launch_bounds__(64) void k_phase_a(
                 break;
             }
             const uint32_t *ll = up_pool + base;
-            const uint32_t cnt = min(ll[0] & 0xFFFFu, up_words - 1);
+            const uint32_t w = (lane < up_words) ? ll[lane] : 0u;
+            const uint32_t cnt = min((uint32_t)__shfl((int)w, 0, 64) & 0xFFFFu, up_words - 1u);
             float best = cd;
             uint32_t best_id = cur;
-            for (uint32_t b = 0; b < cnt; b += 4) {
-                const uint32_t t = b + grp;
-                float dv = FLT_MAX;
-                uint32_t id = cur;
-                if (t < cnt) {   // uniform per 16-lane group
-                    id = ll[1 + t];
-                    if (id < n) {
-                        dv = beam_dist<VT>(exact, s_qv, x, id, d, g);
-                    } else {
-                        if (g == 0) s_bad = 1u;
-                        id = cur;
+            for (uint32_t b = 0; b < cnt; b += 16) {
+                uint32_t id4[4];
+                bool v4[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const uint32_t t = b + grp + 4u * (uint32_t)r;
+                    uint32_t v = (uint32_t)__shfl((int)w, (int)(1u + min(t, cnt - 1u)), 64);
+                    v4[r] = t < cnt;
+                    if (v >= n) {
+                        if (v4[r] && g == 0) s_bad = 1u;
+                        v4[r] = false;
+                        v = cur;
                     }
+                    id4[r] = v;
                 }
+                float o[4];
+                beam_dist4<VT>(exact, s_qv, x, id4, v4, d, g, o);
                 // the host's sequential scan: the first strictly smaller wins
-                for (uint32_t s = 0; s < 4; s++) {
-                    float ds = __shfl(dv, (int)(s * 16), 64);
-                    uint32_t is = __shfl(id, (int)(s * 16), 64);
-                    if (b + s < cnt && ds < best) {
+                const uint32_t e = min(cnt - b, 16u);
+                for (uint32_t s = 0; s < e; s++) {
+                    const uint32_t sr = s >> 2, src = (s & 3u) * 16u;
+                    const float mo = sr == 0 ? o[0] : sr == 1 ? o[1] : sr == 2 ? o[2] : o[3];
+                    const uint32_t mi = sr == 0 ? id4[0] : sr == 1 ? id4[1] : sr == 2 ? id4[2] : id4[3];
+                    const int mv = (sr == 0 ? v4[0] : sr == 1 ? v4[1] : sr == 2 ? v4[2] : v4[3]) ? 1 : 0;
+                    const float ds = __shfl(mo, (int)src, 64);
+                    const uint32_t is = (uint32_t)__shfl((int)mi, (int)src, 64);
+                    const int vs = __shfl(mv, (int)src, 64);
+                    if (vs && ds < best) {
                         best = ds;
                         best_id = is;
                     }
@@ -161,35 +236,37 @@ __global__ __launch_bounds__(64) void k_phase_a(
     // ---- layer-0 beam ----
     const float d_start = cd;
     if (lane == 0) {
-        s_d[0][0] = cd;
-        s_i[0][0] = cur;
-        s_x[0][0] = 0;
-        vis_insert(s_vis, cur, &s_lost);
+        s_kl[0] = beam_key(cd, cur);
+        s_xl[0] = 0;
+        vis_insert(s_vis, log_vis, cur, &s_lost);
     }
     __syncthreads();
     uint32_t L = 1, buf = 0, n_exp = 0, n_eval = 1;
-    for (uint32_t iter = 0; iter <= n; iter++) {   // each pass expands one list entry
-        uint32_t pick = kInvalidId;
-        for (uint32_t b0 = 0; b0 < L; b0 += 64) {
-            const uint32_t t = b0 + lane;
-            const uint64_t mask = __ballot(t < L && s_x[buf][t] == 0);
-            if (mask) {
-                pick = b0 + (uint32_t)__ffsll((long long)mask) - 1;
-                break;
-            }
+    // pick: position of the first unexpanded list entry (every entry before it
+    // is expanded), kInvalidId when none is left
+    uint32_t pick = 0;
+    // layer-0 row of the node to be expanded next, loaded while the current
+    // expansion merges its candidates
+    uint32_t pf_node = kInvalidId, pf_cnt = 0, pf_nb = 0;
+    for (uint32_t iter = 0; iter <= n && pick != kInvalidId; iter++) {   // one expansion per pass
+        const unsigned long long *lk = s_kl + buf * ef_cap;
+        uint8_t *lx = s_xl + buf * ef_cap;
+        const uint32_t c = (uint32_t)lk[pick];
+        uint32_t cnt, nbw;
+        if (c == pf_node) {
+            cnt = pf_cnt;
+            nbw = pf_nb;
+        } else {
+            l0_row(l0, l0_words, c, lane, &cnt, &nbw);
         }
-        if (pick == kInvalidId) break;
-        const uint32_t c = s_i[buf][pick];
         __syncthreads();
-        if (lane == 0) s_x[buf][pick] = 1;
+        if (lane == 0) lx[pick] = 1;
         n_exp++;
-        const uint32_t *ll = l0 + (uint64_t)c * l0_words;
-        const uint32_t cnt = min(ll[0] & 0xFFFFu, l0_words - 1);
         uint32_t nb = 0;
         bool fresh = false;
         if (lane < cnt) {
-            nb = ll[1 + lane];
-            if (nb < n) fresh = vis_insert(s_vis, nb, &s_lost);
+            nb = nbw;
+            if (nb < n) fresh = vis_insert(s_vis, log_vis, nb, &s_lost);
             else s_bad = 1u;
         }
         const uint64_t fm = __ballot(fresh);
@@ -197,77 +274,116 @@ __global__ __launch_bounds__(64) void k_phase_a(
         n_eval += nf;
         if (fresh) s_ni[__popcll(fm & lanemask_lt())] = nb;
         __syncthreads();
-        // distances of the fresh neighbours, 4 at a time (one per 16-lane group)
-        for (uint32_t b = 0; b < nf; b += 4) {
-            const uint32_t t = b + grp;
-            if (t < nf) {
-                const float dv = beam_dist<VT>(exact, s_qv, x, s_ni[t], d, g);
-                if (g == 0) s_nd[t] = dv;
+        // the first unexpanded entry after pick
+        uint32_t o = kInvalidId;
+        for (uint32_t b0 = pick + 1; b0 < L; b0 += 64) {
+            const uint32_t t = b0 + lane;
+            const uint64_t mask = __ballot(t < L && lx[t] == 0);
+            if (mask) {
+                o = b0 + (uint32_t)__ffsll((long long)mask) - 1;
+                break;
+            }
+        }
+        if (nf == 0) {
+            pick = o;
+            continue;
+        }
+        // distances of the fresh neighbours, 16 per pass (4 per 16-lane group)
+        for (uint32_t b = 0; b < nf; b += 16) {
+            uint32_t id4[4];
+            bool v4[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const uint32_t t = b + grp + 4u * (uint32_t)r;
+                v4[r] = t < nf;
+                id4[r] = s_ni[v4[r] ? t : b];
+            }
+            float o4[4];
+            beam_dist4<VT>(exact, s_qv, x, id4, v4, d, g, o4);
+            if (g == 0) {
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (v4[r]) s_nd[b + grp + 4u * (uint32_t)r] = o4[r];
             }
         }
         __syncthreads();
-        if (nf == 0) continue;
         // candidates that can enter: the list is not full, or (dist, id) below
         // its worst entry; not already listed (checked only when the visited
         // set lost an insertion)
         unsigned long long key = ~0ull;
         if (lane < nf) {
-            const float dv = s_nd[lane];
-            const uint32_t id = s_ni[lane];
-            bool ok = L < ef || pair_less(dv, id, s_d[buf][L - 1], s_i[buf][L - 1]);
-            if (ok && s_lost) {
-                for (uint32_t j = 0; j < L && ok; j++) ok = s_i[buf][j] != id;
+            const unsigned long long kc = beam_key(s_nd[lane], s_ni[lane]);
+            bool ok = L < ef || kc < lk[L - 1];
+            if (ok && s_lost) {   // nor twice among this expansion's candidates
+                for (uint32_t j = 0; j < L && ok; j++) ok = (uint32_t)lk[j] != (uint32_t)kc;
+                for (uint32_t j = 0; j < lane && ok; j++) ok = s_ni[j] != (uint32_t)kc;
             }
-            if (ok) key = beam_key(dv, id);
+            if (ok) key = kc;
         }
         const uint32_t nv = (uint32_t)__popcll(__ballot(key != ~0ull));
-        if (nv == 0) continue;   // uniform
-        key = wave_sort64(key, lane);
-        s_ck[lane] = key;
-        __syncthreads();
-        // merge the sorted candidates into the sorted list, keeping the first ef
+        if (nv == 0) {   // uniform
+            pick = o;
+            continue;
+        }
+        key = wave_sort64(key, lane, next_pow2_dev(nf));   // admitted keys now in lanes [0, nv)
+        // the next expansion takes the smaller of the best admitted candidate
+        // and the first unexpanded entry after pick; start its row load
+        const unsigned long long kbest = read_lane64(key, 0);
+        const unsigned long long ko = (o != kInvalidId) ? lk[o] : ~0ull;
+        pf_node = (uint32_t)(kbest < ko ? kbest : ko);
+        l0_row(l0, l0_words, pf_node, lane, &pf_cnt, &pf_nb);
+        // merge the sorted candidates into the sorted list, keeping the first
+        // ef: entry e moves up by the candidates below it; candidate j lands
+        // after the entries below it
         const uint32_t nbuf = buf ^ 1u;
-        for (uint32_t e = lane; e < L; e += 64) {
-            const unsigned long long ke = beam_key(s_d[buf][e], s_i[buf][e]);
-            uint32_t lo = 0, hi = nv;   // candidates below entry e
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_ck[mid] < ke) lo = mid + 1; else hi = mid;
+        unsigned long long *nk = s_kl + nbuf * ef_cap;
+        uint8_t *nx = s_xl + nbuf * ef_cap;
+        uint32_t below = 0;   // lane j < nv: list entries below candidate j
+        for (uint32_t b0 = 0; b0 < L; b0 += 64) {
+            const uint32_t e = b0 + lane;
+            const bool valid = e < L;
+            const unsigned long long ke = valid ? lk[e] : ~0ull;
+            const uint8_t xe = valid ? lx[e] : (uint8_t)1;
+            uint32_t less = 0;
+            for (uint32_t j = 0; j < nv; j++) {
+                const unsigned long long kj = read_lane64(key, j);
+                const bool lt = kj < ke;
+                less += lt ? 1u : 0u;
+                const uint32_t cb = (uint32_t)__popcll(__ballot(valid && !lt));
+                if (lane == j) below += cb;
             }
-            const uint32_t pos = e + lo;
-            if (pos < ef) {
-                s_d[nbuf][pos] = s_d[buf][e];
-                s_i[nbuf][pos] = s_i[buf][e];
-                s_x[nbuf][pos] = s_x[buf][e];
+            const uint32_t pos = e + less;
+            if (valid && pos < ef) {
+                nk[pos] = ke;
+                nx[pos] = xe;
             }
         }
         if (lane < nv) {
-            const unsigned long long kc = s_ck[lane];
-            uint32_t lo = 0, hi = L;    // list entries below candidate lane
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (beam_key(s_d[buf][mid], s_i[buf][mid]) < kc) lo = mid + 1; else hi = mid;
-            }
-            const uint32_t pos = lane + lo;
+            const uint32_t pos = lane + below;
             if (pos < ef) {
-                s_d[nbuf][pos] = key_float((uint32_t)(kc >> 32));
-                s_i[nbuf][pos] = (uint32_t)kc;
-                s_x[nbuf][pos] = 0;
+                nk[pos] = key;
+                nx[pos] = 0;
             }
         }
+        // position of the next expansion in the merged list: candidate 0's, or
+        // entry o's, which no candidate precedes
+        pick = (kbest < ko) ? (uint32_t)__builtin_amdgcn_readlane((int)below, 0) : o;
         __syncthreads();
         L = min(L + nv, ef);
         buf = nbuf;
     }
     // ---- output: the m closest ----
-    for (uint32_t t = lane; t < m; t += 64) proxies[qi * m + t] = (t < L) ? s_i[buf][t] : kInvalidId;
+    const unsigned long long *lk = s_kl + buf * ef_cap;
+    const uint32_t mq = opt.m_q ? min(opt.m_q[qi], m) : m;
+    for (uint32_t t = lane; t < m; t += 64)
+        proxies[qi * m + t] = (t < L && t < mq) ? (uint32_t)lk[t] : kInvalidId;
     if (expansions && lane == 0) {
         expansions[2 * qi] = n_exp;
         expansions[2 * qi + 1] = n_eval;
     }
     if (raw) {   // per-query statistics for the learned m (kRawWords floats)
         float *r = raw + qi * kRawWords;
-        if (lane < 16) r[2 + lane] = (lane < L) ? s_d[buf][lane] : s_d[buf][L - 1];
+        if (lane < 16) r[2 + lane] = key_float((uint32_t)(lk[lane < L ? lane : L - 1] >> 32));
         if (lane == 0) {
             r[0] = d_entry;
             r[1] = d_start;
@@ -275,34 +391,119 @@ __global__ __launch_bounds__(64) void k_phase_a(
             r[19] = (float)L;
         }
     }
+    if (opt.feats) {
+        // The learned-m features (the same definitions as the host's
+        // HRNN::queryFeatures over this beam's statistics):
+        //   0 log1p d(q,entry)   1 log1p d(q,start)   2..17 log1p of the 16
+        //   closest list distances (the last one repeated past the list end)
+        //   18..25 log1p r_k of list entries 1..8   26..28 mean / min / max of
+        //   log1p r_k over entries 1..16   29 fraction of them with
+        //   d(q,p) <= r_k(p)   30 log1p d1 - log1p r_k(p1)   31 log1p expansions
+        // r_k(p) = the feat_k-th distance of p's ranked KNN list (3e38 when
+        // the list is shorter).  Lanes 0..15 take one list entry each.
+        const uint32_t j = lane & 15u;
+        const float dj = key_float((uint32_t)(lk[j < L ? j : L - 1] >> 32));
+        const float ld = log1pf(fminf(dj, 3.0e38f));
+        float rk = 0.f, inside = 0.f, cntf = 0.f;
+        if (j < L) {
+            const uint32_t p = (uint32_t)lk[j];
+            if (p < n) {
+                const float v = (opt.feat_k - 1u < opt.knn_cnt[p]) ? opt.knn_d[(uint64_t)p * opt.K + opt.feat_k - 1u]
+                                                                   : 3.0e38f;
+                rk = log1pf(v);
+                inside = (dj <= v) ? 1.f : 0.f;
+                cntf = 1.f;
+            } else {
+                s_bad = 1u;
+            }
+        }
+        // reductions over lanes 0..15 (each 16-lane group holds the same values)
+        float sum = rk * cntf, ins = inside, cnt = cntf;
+        float mn = cntf > 0.f ? rk : FLT_MAX, mx = cntf > 0.f ? rk : -FLT_MAX;
+#pragma unroll
+        for (int s2 = 8; s2 > 0; s2 >>= 1) {
+            sum += __shfl_xor(sum, s2, 16);
+            ins += __shfl_xor(ins, s2, 16);
+            cnt += __shfl_xor(cnt, s2, 16);
+            mn = fminf(mn, __shfl_xor(mn, s2, 16));
+            mx = fmaxf(mx, __shfl_xor(mx, s2, 16));
+        }
+        const float rk1 = __shfl(rk, 0, 64);
+        const float ld0 = __shfl(ld, 0, 64);
+        const uint32_t t = lane & 31u;
+        const float from_ld = __shfl(ld, (int)((t - 2u) & 15u), 64);
+        const float from_rk = __shfl(rk, (int)((t - 18u) & 15u), 64);
+        float f;
+        if (t == 0) f = log1pf(d_entry);
+        else if (t == 1) f = log1pf(d_start);
+        else if (t < 18) f = from_ld;
+        else if (t < 26) f = from_rk;
+        else if (t == 26) f = cnt > 0.f ? sum / cnt : 0.f;
+        else if (t == 27) f = cnt > 0.f ? mn : 0.f;
+        else if (t == 28) f = cnt > 0.f ? mx : 0.f;
+        else if (t == 29) f = cnt > 0.f ? ins / cnt : 0.f;
+        else if (t == 30) f = ld0 - rk1;
+        else f = log1pf((float)n_exp);
+        if (opt.mu) f = (f - opt.mu[t]) / opt.sd[t];
+        if (lane < (uint32_t)kFeatWords) opt.feats[qi * kFeatWords + lane] = f;
+    }
+    __syncthreads();
     if (lane == 0 && s_bad) atomicOr(err, 1u);
 }
 
+// Visited-set slots for a beam of width ef.  The ids evaluated per query grow
+// slower than ef (about 370, 560, 900, 1,020 and 1,110 at ef = 16, 32, 64, 128
+// and 200 on a 1M-point, M = 16 graph), so 1,024 slots up to ef = 16, 2,048 up
+// to ef = 256 and 4,096 beyond keep the load near or below 1/2.  An
+// overflowing probe run stays correct, only slower.
+static uint32_t beam_log_vis(uint32_t ef) { return ef <= 16 ? 10u : ef <= 256 ? 11u : 12u; }
+
 void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint32_t ef,
-                 uint32_t *proxies_dev, uint32_t *expansions_dev, float *raw_dev) {
+                 uint32_t *proxies_dev, uint32_t *expansions_dev, float *raw_dev, const BeamOpt *opt) {
     REQUIRE(h->hnsw_valid, HRNN_E_STATE, "proxies: upload the HNSW graph first");
     REQUIRE(ef >= 1 && ef <= (uint32_t)kBeamMaxEf, HRNN_E_ARG, "proxies: ef must be in [1, 512]");
-    REQUIRE(m >= 1 && m <= ef, HRNN_E_ARG, "proxies: m must be in [1, ef]");
+    REQUIRE(m >= 1 && m <= 1024, HRNN_E_ARG, "proxies: m must be in [1, 1024]");
     REQUIRE(h->l0_words >= 1 && h->l0_words - 1 <= (uint32_t)kBeamMaxDeg, HRNN_E_ARG,
             "proxies: layer-0 degree must be in [0, 64]");
+    REQUIRE(h->up_words >= 1 && h->up_words <= 64, HRNN_E_ARG,
+            "proxies: upper-layer degree must be in [0, 63]");
     REQUIRE(h->d >= 1 && h->d <= 4096, HRNN_E_ARG, "proxies: d must be in [1, 4096]");
     REQUIRE(q_dev && proxies_dev, HRNN_E_ARG, "proxies: null argument");
+    BeamOpt o = opt ? *opt : BeamOpt{};
+    if (o.feats) {
+        REQUIRE(h->K > 0 && o.feat_k >= 1 && o.feat_k <= h->K, HRNN_E_ARG,
+                "proxies: feature k must be in [1, K]");
+        REQUIRE((o.mu == nullptr) == (o.sd == nullptr), HRNN_E_ARG, "proxies: give mu and sd together");
+        o.knn_d = h->knn_d.as<float>();
+        o.knn_cnt = h->knn_cnt.as<uint32_t>();
+        o.K = h->K;
+    }
     if (nq == 0) return;
     uint32_t *err = h->counters.as<uint32_t>() + kErrWord;
     // every sum exact in any order: values in [0, 255] and d <= 258, with the
     // row width the wide loads need (8 BF16 or 4 FP32 elements)
     const int base_exact = (h->base_u8 && h->d <= 258 && h->d % 8 == 0) ? 1 : 0;
-#define LAUNCH(VT, XP)                                                                            \
-    hipLaunchKernelGGL(k_phase_a<VT>, dim3(nq), dim3(64), h->d * sizeof(float), h->stream, XP, h->d, \
-                       (uint32_t)h->n, h->l0.as<uint32_t>(), h->l0_words, h->up_off.as<uint32_t>(),   \
-                       h->up_pool.as<uint32_t>(), h->up_pool_words, h->up_words, h->entry,            \
-                       h->maxlevel, q_dev, ef, m, base_exact, proxies_dev, expansions_dev, raw_dev, err)
+    const uint32_t ef_cap = (ef + 3u) & ~3u;
+    const uint32_t log_vis = beam_log_vis(ef);
+    const BeamLds lay(h->d, ef_cap, log_vis);
+    REQUIRE(lay.total <= 65536, HRNN_E_ARG, "proxies: LDS need exceeds 64 KiB");
+#define LAUNCH1(VT, PQ, XP)                                                                       \
+    hipLaunchKernelGGL((k_phase_a<VT, PQ>), dim3(nq), dim3(64), lay.total, h->stream, XP, h->d,  \
+                       (uint32_t)h->n, h->l0.as<uint32_t>(), h->l0_words, h->up_off.as<uint32_t>(), \
+                       h->up_pool.as<uint32_t>(), h->up_pool_words, h->up_words, h->entry,        \
+                       h->maxlevel, q_dev, ef, ef_cap, log_vis, m, base_exact, proxies_dev,       \
+                       expansions_dev, raw_dev, err, o)
+#define LAUNCH(VT, XP)                  \
+    if (o.ef_q) LAUNCH1(VT, true, XP);  \
+    else LAUNCH1(VT, false, XP)
     // BF16 mirror where it is exact (same distances, half the bytes)
-    if (h->small_int && h->vec_bf16)
+    if (h->small_int && h->vec_bf16) {
         LAUNCH(__hip_bfloat16, h->xb.as<__hip_bfloat16>());
-    else
+    } else {
         LAUNCH(float, h->x.as<float>());
+    }
 #undef LAUNCH
+#undef LAUNCH1
     HIPC(hipGetLastError());
 }
 
@@ -366,6 +567,7 @@ int hrnn_dev_proxies(hrnn_dev_t h, const float *q, uint32_t nq, uint32_t m, uint
     if (!h) return HRNN_E_ARG;
     return guarded(h, [&] {
         REQUIRE(q && proxies, HRNN_E_ARG, "proxies: null argument");
+        REQUIRE(m >= 1 && m <= ef, HRNN_E_ARG, "proxies: m must be in [1, ef]");
         if (!nq) return;
         DevBuf dq, dp;
         dq.reserve((uint64_t)nq * h->d * 4);
@@ -382,7 +584,10 @@ int hrnn_dev_proxies(hrnn_dev_t h, const float *q, uint32_t nq, uint32_t m, uint
 int hrnn_dev_proxies_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint32_t ef,
                           uint32_t *proxies_dev, uint32_t *expansions_dev) {
     if (!h) return HRNN_E_ARG;
-    return guarded(h, [&] { beam_launch(h, q_dev, nq, m, ef, proxies_dev, expansions_dev); });
+    return guarded(h, [&] {
+        REQUIRE(m >= 1 && m <= ef, HRNN_E_ARG, "proxies: m must be in [1, ef]");
+        beam_launch(h, q_dev, nq, m, ef, proxies_dev, expansions_dev);
+    });
 }
 
 int hrnn_dev_rknn_query_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m,

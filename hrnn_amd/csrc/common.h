@@ -190,6 +190,54 @@ __device__ __forceinline__ float dist_row_exact<float>(const float *__restrict__
     return acc;
 }
 
+// Row vector type of the exact-sum distance: 16 bytes per lane and step.
+template <class VT>
+struct ExactRow;
+template <>
+struct ExactRow<__hip_bfloat16> {
+    typedef ex_bf16x8 V;
+    static constexpr uint32_t W = 8;
+};
+template <>
+struct ExactRow<float> {
+    typedef ex_f32x4 V;
+    static constexpr uint32_t W = 4;
+};
+
+// dist_row_exact for four rows at once: each 16-lane group issues the loads
+// of all four rows before it accumulates, so four row reads are in flight per
+// group.  id[0..3] must all be valid rows (repeat one to pad).  Every lane of
+// the group returns the four sums.
+template <class VT>
+__device__ __forceinline__ void dist4_exact(const float *__restrict__ qv, const VT *__restrict__ base,
+                                            const uint32_t *id, uint32_t d, uint32_t g, float *out) {
+    typedef typename ExactRow<VT>::V V;
+    constexpr uint32_t W = ExactRow<VT>::W;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    for (uint32_t e = W * g; e < d; e += 16 * W) {
+        V v[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) v[r] = *reinterpret_cast<const V *>(base + (uint64_t)id[r] * d + e);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+#pragma unroll
+            for (uint32_t j = 0; j < W; j++) {
+                const float t = qv[e + j] - (float)v[r][j];
+                acc[r] = __builtin_fmaf(t, t, acc[r]);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        float a = acc[r];
+        a += __shfl_xor(a, 8, 16);
+        a += __shfl_xor(a, 4, 16);
+        a += __shfl_xor(a, 2, 16);
+        a += __shfl_xor(a, 1, 16);
+        out[r] = a;
+    }
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
     uint32_t lane = threadIdx.x & 63u;
     return lane ? (~0ull >> (64 - lane)) : 0ull;

@@ -27,9 +27,15 @@ struct hrnn_dev {
     int tune_search_lds_kib = 16;    // LDS dedup table of the RkNN search, KiB
     int tune_beam_wg_per_cu = 16;    // persistent beam workgroups (one wave each) per CU
 
-    // ranked KNN graph (SoA)
+    // ranked KNN graph (SoA); knn_version changes whenever it may have changed
     uint32_t K = 0;
     hrnn::DevBuf knn_ids, knn_d, knn_cnt, knn_new;
+    uint64_t knn_version = 1;
+    // r_k column of the graph for one k (verification reads one float per
+    // candidate from a 4n-byte array instead of a line of the n x K matrix)
+    hrnn::DevBuf rk_col;
+    uint32_t rk_col_k = 0;
+    uint64_t rk_col_version = 0, rk_col_n = 0;
 
     // reverse CSR
     bool csr_valid = false;
@@ -74,9 +80,11 @@ int guarded(hrnn_dev_t h, F &&f) {
         return HRNN_OK;
     } catch (Error &e) {
         if (h) h->err = e.msg;
+        (void)hipGetLastError();   // a failed HIP call must not surface in a later one
         return e.code;
     } catch (std::exception &e) {
         if (h) h->err = e.what();
+        (void)hipGetLastError();
         return HRNN_E_ARG;
     }
 }
@@ -106,8 +114,22 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t bucket, hrnn_nnd_ite
 // start), 2..17 the 16 closest beam distances, 18 expansions, 19 list size.
 constexpr int kRawWords = 20;
 constexpr uint32_t kErrWord = 96;   // u32 index of the device error word in counters
+constexpr int kFeatWords = 32;       // learned-m features per query
+// Optional per-query controls and outputs of a phase-A launch (device
+// pointers; null = unused).
+struct BeamOpt {
+    const uint32_t *ef_q = nullptr;   // per-query ef, <= the launch's ef; 0 skips the query
+    const uint32_t *m_q = nullptr;    // per-query proxy count; later slots get 0xFFFFFFFF
+    float *feats = nullptr;           // kFeatWords features per query
+    const float *knn_d = nullptr;     // ranked KNN distances [n x K] (features' r_k)
+    const uint32_t *knn_cnt = nullptr;
+    uint32_t K = 0, feat_k = 0;       // r_k = knn_d[p][feat_k - 1]
+    const float *mu = nullptr;        // standardisation (f - mu) / sd, both or neither
+    const float *sd = nullptr;
+};
 void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint32_t ef,
-                 uint32_t *proxies_dev, uint32_t *expansions_dev, float *raw_dev = nullptr);
+                 uint32_t *proxies_dev, uint32_t *expansions_dev, float *raw_dev = nullptr,
+                 const BeamOpt *opt = nullptr);
 void bf_topk_base(hrnn_dev_t h, const float *q_dev, uint32_t nq, bool self_base, uint32_t kk,
                   int prec, uint32_t *ids_dev, float *d_dev);
 void bf_rknn_count_fill(hrnn_dev_t h, const float *q_dev, uint32_t nq, const float *kth_dev,

@@ -1,9 +1,13 @@
-// mlp.hip -- the learned proxy count: a 32-64-64-16 MLP (ReLU, softmax over a
-// 16-value grid of m) with hand-written forward, backward and Adam kernels.
+// mlp.hip -- the learned proxy count and early stop: a 32-64-64-16 MLP (ReLU,
+// softmax over 16 classes) with hand-written forward, backward and Adam
+// kernels, and the two-stage query path it drives.
 //
 // No reference counterpart: the reference picks m from a fixed grid
-// (src/hrnn_search.cpp:69); this network predicts, per query, the smallest m
-// of the grid that reaches full recall.
+// (src/hrnn_search.cpp:69) and runs every beam to ef_search.  Here the
+// network predicts per query the cheapest class that reaches full recall:
+// either a proxy count m of a 16-value grid (host-feature path) or an (ef, m)
+// pair (adaptive path: a narrow stage-1 beam gives the features, and only the
+// queries whose class needs a wider beam run phase A again).
 //
 // Parameter vector (FP32, HRNN_MLP_PARAMS floats):
 //   W1[64][32] b1[64] W2[64][64] b2[64] W3[16][64] b3[16]
@@ -213,6 +217,54 @@ __global__ void k_pick_m(const float *__restrict__ probs, uint32_t nq, const uin
     if (m_out) m_out[qi] = m;
 }
 
+// Learned early stop: per query the cheapest (ef, m) class.  Classes are
+// ordered by cost, each dominating the previous one (ef and m both
+// non-decreasing).  With probs: the first class whose cumulative probability
+// reaches tau (argmax when tau <= 0); without: the class given in cls.  Writes
+// the class, the stage-2 beam width (0 when the class's ef <= ef1: the
+// stage-1 proxies serve) and the class's m (at most mstride), and masks the
+// stage-1 proxies beyond that m.  classes = [ef of 16 classes, m of 16 classes].
+// exp2 (optional): the stage-2 expansion counters, zeroed here for the
+// queries that have no stage 2.
+__global__ void k_pick_class(const float *__restrict__ probs, uint32_t nq,
+                             const uint32_t *__restrict__ classes, float tau, uint32_t ef1,
+                             uint32_t *__restrict__ prox, uint32_t mstride, uint32_t *__restrict__ cls,
+                             uint32_t *__restrict__ ef_q, uint32_t *__restrict__ m_q,
+                             uint32_t *__restrict__ exp2) {
+    const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= nq) return;
+    uint32_t c = kOut - 1;
+    if (probs) {
+        const float *p = probs + (uint64_t)qi * kOut;
+        if (tau <= 0.f) {
+            c = 0;
+            for (uint32_t j = 1; j < (uint32_t)kOut; j++)
+                if (p[j] > p[c]) c = j;
+        } else {
+            float acc = 0.f;
+            for (uint32_t j = 0; j < (uint32_t)kOut; j++) {
+                acc += p[j];
+                if (acc >= tau) {
+                    c = j;
+                    break;
+                }
+            }
+        }
+        cls[qi] = c;
+    } else {
+        c = min(cls[qi], (uint32_t)kOut - 1u);
+    }
+    const uint32_t ef = classes[c];
+    const uint32_t m = min(max(classes[kOut + c], 1u), mstride);
+    ef_q[qi] = ef > ef1 ? ef : 0u;
+    m_q[qi] = m;
+    if (exp2 && ef <= ef1) {   // no stage-2 beam for this query
+        exp2[2 * (uint64_t)qi] = 0;
+        exp2[2 * (uint64_t)qi + 1] = 0;
+    }
+    for (uint32_t j = m; j < mstride; j++) prox[(uint64_t)qi * mstride + j] = kInvalidId;
+}
+
 }  // namespace hrnn
 
 using namespace hrnn;
@@ -313,6 +365,74 @@ int hrnn_dev_rknn_search_learned_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
         hipLaunchKernelGGL(k_pick_m, dim3(ceil_div(nq, 128)), dim3(128), 0, h->stream, probs, nq, grid_dev,
                            tau, proxies_dev, m_max, m_dev);
         HIPC(hipGetLastError());
+        rknn_search_resident(h, q_dev, nq, proxies_dev, m_max, nullptr, k, K_prime, threshold, res_cnt_dev,
+                             res_ids_dev, cap_per_q, work_dev);
+    });
+}
+
+int hrnn_dev_stage1_features_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t ef1,
+                                  uint32_t m, uint32_t k, uint32_t *proxies_dev, float *feats_dev,
+                                  uint32_t *expansions_dev) {
+    if (!h) return HRNN_E_ARG;
+    return guarded(h, [&] {
+        REQUIRE(q_dev && proxies_dev && feats_dev, HRNN_E_ARG, "stage1_features: null argument");
+        REQUIRE(ef1 >= 1 && m >= 1 && m <= 1024, HRNN_E_ARG, "stage1_features: bad ef1 or m");
+        BeamOpt o;
+        o.feats = feats_dev;
+        o.feat_k = k;
+        beam_launch(h, q_dev, nq, m, ef1, proxies_dev, expansions_dev, nullptr, &o);
+    });
+}
+
+int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq,
+                                      const float *model_dev, const uint32_t *classes_dev,
+                                      uint32_t ef_max, float tau, uint32_t ef1, uint32_t m_max,
+                                      uint32_t k, uint32_t K_prime, float threshold,
+                                      uint32_t *proxies_dev, uint32_t *cls_dev,
+                                      uint32_t *res_cnt_dev, uint32_t *res_ids_dev,
+                                      uint32_t cap_per_q, uint32_t *work_dev,
+                                      uint32_t *expansions_dev) {
+    if (!h) return HRNN_E_ARG;
+    return guarded(h, [&] {
+        REQUIRE(h->csr_valid && h->K > 0, HRNN_E_STATE, "query_adaptive: build the CSR first");
+        REQUIRE(k >= 1 && k <= h->K, HRNN_E_ARG, "query_adaptive: k out of [1, K]");
+        REQUIRE(m_max >= 1 && m_max <= 1024, HRNN_E_ARG, "query_adaptive: m_max must be in [1, 1024]");
+        REQUIRE(ef1 >= 1 && ef_max >= ef1 && ef_max <= 512, HRNN_E_ARG,
+                "query_adaptive: need 1 <= ef1 <= ef_max <= 512");
+        REQUIRE(q_dev && classes_dev && proxies_dev && cls_dev && res_cnt_dev && res_ids_dev, HRNN_E_ARG,
+                "query_adaptive: null argument");
+        if (!nq) return;
+        h->s8.reserve((uint64_t)nq * (kIn + kOut + 2) * 4);
+        float *feats = h->s8.as<float>();
+        float *probs = feats + (uint64_t)nq * kIn;
+        uint32_t *ef_q = reinterpret_cast<uint32_t *>(probs + (uint64_t)nq * kOut);
+        uint32_t *m_q = ef_q + nq;
+        // stage 1: every query at ef1, with the standardised features when a
+        // model is given
+        BeamOpt o1;
+        if (model_dev) {
+            o1.feats = feats;
+            o1.feat_k = k;
+            o1.mu = model_dev + kParams;
+            o1.sd = model_dev + kParams + kIn;
+        }
+        uint32_t *exp1 = expansions_dev, *exp2 = expansions_dev ? expansions_dev + 2ull * nq : nullptr;
+        beam_launch(h, q_dev, nq, m_max, ef1, proxies_dev, exp1, nullptr, &o1);
+        if (model_dev)
+            hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(nq), dim3(64), 0, h->stream, model_dev, feats, nullptr,
+                               nullptr, nq, probs, nullptr, nullptr);
+        hipLaunchKernelGGL(k_pick_class, dim3(ceil_div(nq, 128)), dim3(128), 0, h->stream,
+                           model_dev ? probs : nullptr, nq, classes_dev, tau, ef1, proxies_dev, m_max,
+                           cls_dev, ef_q, m_q, exp2);
+        HIPC(hipGetLastError());
+        // stage 2: the queries whose class needs a wider beam (the others
+        // leave their workgroup at once)
+        if (ef_max > ef1) {
+            BeamOpt o2;
+            o2.ef_q = ef_q;
+            o2.m_q = m_q;
+            beam_launch(h, q_dev, nq, m_max, ef_max, proxies_dev, exp2, nullptr, &o2);
+        }
         rknn_search_resident(h, q_dev, nq, proxies_dev, m_max, nullptr, k, K_prime, threshold, res_cnt_dev,
                              res_ids_dev, cap_per_q, work_dev);
     });

@@ -19,7 +19,9 @@
 // compacted in place, verified by 16-lane groups (a rejected id is
 // overwritten with 0xFFFFFFFF by the one group that owns it), compacted again
 // and sorted, so HBM sees each scanned entry once and each candidate's vector
-// and k-distance once.  A query whose scanned entries do not fit the table
+// and k-distance once; with one k for the batch the k-distances come from an
+// n-float r_k column (4 MB at n = 1M, cache resident) instead of the n x K
+// list matrix.  A query whose scanned entries do not fit the table
 // falls back to a per-workgroup visited bitmap / u16 count array in HBM.
 //
 // Distances use the reference's 16-partial FMA order, so accept/reject
@@ -96,6 +98,15 @@ __device__ __forceinline__ uint32_t block_compact(uint32_t n, uint32_t *dst, uin
     return out;
 }
 
+// r_k column: rk[i] = the k-th distance of i's ranked list (FLT_MAX when the
+// list is shorter), the value getVerifyKdistSq returns (hrnn.h:286-293).
+__global__ void k_rk_column(const float *__restrict__ knn_d, const uint32_t *__restrict__ knn_cnt,
+                            uint64_t n, uint32_t K, uint32_t k, float *__restrict__ rk) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    rk[i] = (k - 1u < knn_cnt[i]) ? knn_d[i * K + k - 1u] : FLT_MAX;
+}
+
 template <class VT>
 __device__ __forceinline__ float cand_dist(bool exact, const float *qv, const VT *x, uint32_t c,
                                            uint32_t d, uint32_t g) {
@@ -106,7 +117,8 @@ template <class EntryT, bool kCount, class VT>
 __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
     const VT *__restrict__ x, uint32_t d, int base_exact, const uint64_t *__restrict__ off,
     const EntryT *__restrict__ ent, uint32_t rank_bits, const float *__restrict__ knn_d,
-    const uint32_t *__restrict__ knn_cnt, uint32_t K, const float *__restrict__ q, uint32_t nq,
+    const uint32_t *__restrict__ knn_cnt, uint32_t K, const float *__restrict__ rk_col,
+    const float *__restrict__ q, uint32_t nq,
     const uint32_t *__restrict__ proxies, uint32_t m, const uint32_t *__restrict__ k_per_q,
     uint32_t k, uint32_t K_prime, float threshold, uint32_t log_tmax, uint32_t *__restrict__ mark_all,
     uint64_t mark_words, uint32_t *__restrict__ cand_all, uint64_t cand_cap,
@@ -207,7 +219,8 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
                 const uint32_t c = s_t32[i];
                 const float dist = cand_dist<VT>(exact, s_qv, x, c, d, g);
                 if (g == 0) {
-                    const float kd = (kidx < knn_cnt[c]) ? knn_d[(uint64_t)c * K + kidx] : FLT_MAX;
+                    const float kd = rk_col ? rk_col[c]
+                                            : (kidx < knn_cnt[c]) ? knn_d[(uint64_t)c * K + kidx] : FLT_MAX;
                     if (!(dist <= kd)) s_t32[i] = kInvalidId;
                 }
             }
@@ -265,7 +278,8 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
                 const uint32_t c = keep[i];
                 const float dist = cand_dist<VT>(exact, s_qv, x, c, d, g);
                 if (g == 0) {
-                    const float kd = (kidx < knn_cnt[c]) ? knn_d[(uint64_t)c * K + kidx] : FLT_MAX;
+                    const float kd = rk_col ? rk_col[c]
+                                            : (kidx < knn_cnt[c]) ? knn_d[(uint64_t)c * K + kidx] : FLT_MAX;
                     if (dist <= kd) {
                         const uint32_t r = atomicAdd(&s_nres, 1u);
                         if (r < res_lds) s_t32[r] = c;
@@ -348,6 +362,21 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     const uint64_t *off = h->csr_off.as<uint64_t>();
     const float *kd = h->knn_d.as<float>();
     const uint32_t *kc = h->knn_cnt.as<uint32_t>();
+    // one k for the whole batch: verify against the r_k column (built once per
+    // graph version and k)
+    const float *rk = nullptr;
+    if (!k_per_q && k >= 1 && k <= h->K) {
+        if (h->rk_col_k != k || h->rk_col_version != h->knn_version || h->rk_col_n != n) {
+            h->rk_col.reserve(n * 4);
+            hipLaunchKernelGGL(k_rk_column, dim3(ceil_div(n, 256)), dim3(256), 0, h->stream, kd, kc, n, h->K, k,
+                               h->rk_col.as<float>());
+            HIPC(hipGetLastError());
+            h->rk_col_k = k;
+            h->rk_col_version = h->knn_version;
+            h->rk_col_n = n;
+        }
+        rk = h->rk_col.as<float>();
+    }
     uint32_t *mark = h->search_mark.as<uint32_t>();
     uint32_t *cand = h->search_cand.as<uint32_t>();
     // Candidate vectors are read from the BF16 mirror when every base value is
@@ -356,7 +385,7 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     const int base_exact = (h->base_u8 && h->d <= 258 && h->d % 8 == 0) ? 1 : 0;
 #define LAUNCH(ET, CM, VT, XP)                                                                      \
     hipLaunchKernelGGL((k_rknn_search<ET, CM, VT>), dim3(slots), dim3(kSearchBlock), dyn, h->stream, \
-                       XP, h->d, base_exact, off, h->csr_ent.as<ET>(), h->rank_bits, kd, kc, h->K, q, \
+                       XP, h->d, base_exact, off, h->csr_ent.as<ET>(), h->rank_bits, kd, kc, h->K, rk, q, \
                        nq, proxies, m, k_per_q, k, K_prime, threshold, log_tmax, mark, words, cand,  \
                        cand_cap, queue, res_cnt, res_ids, cap_per_q, work)
 #define LAUNCH_VT(ET, CM)                                                  \

@@ -36,6 +36,7 @@ EXPORTS = [
     "hrnn_dev_proxies_dptr", "hrnn_dev_rknn_query_dptr", "hrnn_dev_bf_kth", "hrnn_dev_bf_knn",
     "hrnn_dev_bf_rknn", "hrnn_dev_l2sqr_pairs", "hrnn_dev_query_features", "hrnn_dev_mlp_forward",
     "hrnn_dev_mlp_grad", "hrnn_dev_mlp_train", "hrnn_dev_rknn_query_learned_dptr", "hrnn_dev_rknn_search_learned_dptr",
+    "hrnn_dev_stage1_features_dptr", "hrnn_dev_rknn_query_adaptive_dptr",
     "hrnn_synth_mixture",
     "hrnn_synth_halfnormal_scaled", "hrnn_synth_sphere_mixture",
 ]
@@ -104,6 +105,9 @@ def lib() -> C.CDLL:
                                              vp, vp, u32, vp],
         "hrnn_dev_rknn_search_learned_dptr": [vp, vp, u32, vp, vp, vp, f, vp, u32, vp, u32, u32, f, vp,
                                               vp, u32, vp],
+        "hrnn_dev_stage1_features_dptr": [vp, vp, u32, u32, u32, u32, vp, vp, vp],
+        "hrnn_dev_rknn_query_adaptive_dptr": [vp, vp, u32, vp, vp, u32, f, u32, u32, u32, u32, f, vp,
+                                              vp, vp, vp, u32, vp, vp],
         "hrnn_synth_mixture": [u64, u64, u32, u32, f, f, f, i, C.c_double, u64, vp, vp],
         "hrnn_synth_halfnormal_scaled": [u64, u64, u64, u32, vp],
         "hrnn_synth_sphere_mixture": [u64, u64, u64, u32, u32, f, vp, vp],
@@ -363,6 +367,52 @@ class Device:
         p = np.empty((q.shape[0], m_max), np.uint32)
         self._ck(self.L.hrnn_dev_query_features(self.h, _p(q), q.shape[0], k, ef, m_max, _p(f), _p(p)))
         return f, p
+
+    def stage1_features(self, q, ef1: int, m: int, k: int):
+        """Stage-1 phase A at ef1 with the fused 32 features:
+        returns (feats[nq, 32], proxies[nq, m], expansions[nq, 2])."""
+        from . import hiprt
+        q = np.ascontiguousarray(q, np.float32)
+        nq = q.shape[0]
+        qd = hiprt.DeviceArray.from_host(q)
+        pd = hiprt.DeviceArray((nq, m), np.uint32)
+        fd = hiprt.DeviceArray((nq, 32), np.float32)
+        ed = hiprt.DeviceArray((nq, 2), np.uint32)
+        self._ck(self.L.hrnn_dev_stage1_features_dptr(self.h, qd.ptr, nq, ef1, m, k, pd.ptr, fd.ptr,
+                                                      ed.ptr))
+        self.sync()
+        return fd.to_host(), pd.to_host(), ed.to_host()
+
+    def query_adaptive(self, q, classes, ef1: int, m_max: int, k: int, K_prime: int, model=None,
+                       tau: float = 0.0, cls=None, threshold: float = 1.0, cap_per_q: int = 4096):
+        """Two-stage learned query (hrnn_dev_rknn_query_adaptive_dptr).  classes:
+        [16, 2] (ef, m) rows.  With model None, cls gives each query's class.
+        Returns (res_off[nq+1], res_ids, cls[nq], proxies[nq, m_max])."""
+        from . import hiprt
+        q = np.ascontiguousarray(q, np.float32)
+        nq = q.shape[0]
+        classes = np.asarray(classes, np.uint32).reshape(16, 2)
+        cd = hiprt.DeviceArray.from_host(np.concatenate([classes[:, 0], classes[:, 1]]))
+        qd = hiprt.DeviceArray.from_host(q)
+        md = None if model is None else hiprt.DeviceArray.from_host(np.asarray(model, np.float32))
+        cl = hiprt.DeviceArray.from_host(np.zeros(nq, np.uint32) if cls is None
+                                         else np.asarray(cls, np.uint32))
+        pd = hiprt.DeviceArray((nq, m_max), np.uint32)
+        rc = hiprt.DeviceArray((nq,), np.uint32)
+        ri = hiprt.DeviceArray((nq, cap_per_q), np.uint32)
+        self._ck(self.L.hrnn_dev_rknn_query_adaptive_dptr(
+            self.h, qd.ptr, nq, None if md is None else md.ptr, cd.ptr, int(classes[:, 0].max()),
+            tau, ef1, m_max, k, K_prime, threshold, pd.ptr, cl.ptr, rc.ptr, ri.ptr, cap_per_q, None,
+            None))
+        self.sync()
+        cnt = rc.to_host().astype(np.uint64)
+        if cnt.max(initial=0) > cap_per_q:
+            raise HrnnError(-5, "query_adaptive: a query has more results than cap_per_q")
+        ids = ri.to_host()
+        off = np.zeros(nq + 1, np.uint64)
+        off[1:] = np.cumsum(cnt)
+        out = np.concatenate([ids[i, :cnt[i]] for i in range(nq)]) if nq else np.empty(0, np.uint32)
+        return off, out.astype(np.uint32), cl.to_host(), pd.to_host()
 
     def mlp_forward(self, params, feats):
         params = np.ascontiguousarray(params, np.float32)

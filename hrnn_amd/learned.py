@@ -1,4 +1,4 @@
-"""Learned proxy count m per query (no reference counterpart).
+"""Learned proxy count m and early stop per query (no reference counterpart).
 
 The reference sweeps a fixed m for all queries (src/hrnn_search.cpp:69).
 Here a 32-64-64-16 MLP, trained on the GPU by the kernels of mlp.hip,
@@ -15,6 +15,16 @@ Pipeline:
   3. training: Adam on mean cross-entropy (hrnn_dev_mlp_train);
   4. inference: MLP forward on the GPU (hrnn_dev_mlp_forward), per-query m
      from `choose_m`, proxies beyond m masked, phases B+C on the GPU.
+
+Early stop (`LearnedEarlyStop`, all on the GPU): the classes are 16 (ef, m)
+pairs ordered by cost with ef and m non-decreasing (CLASSES).  A stage-1
+beam at ef = EF1 gives every query's features (fused into the beam kernel);
+the network picks the cheapest class expected to reach the query's full
+recall; only the queries whose class needs ef > EF1 run phase A again, at
+that ef (hrnn_dev_rknn_query_adaptive_dptr).  Labels: for every class, phase
+A at its ef and phases B+C with its m; label = the cheapest class whose
+recall equals the best recall any class reaches for that query.  Plain
+fixed-m search (the last class is ef 200) stays the fallback.
 """
 from __future__ import annotations
 
@@ -25,6 +35,13 @@ import numpy as np
 from .dev import MLP_MODEL, MLP_PARAMS, Device
 
 GRID = np.array([1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24, 32, 40, 50, 64], np.uint32)
+
+
+# Stage-1 beam width and the (ef, m) classes of the early stop.
+EF1 = 16
+CLASSES = np.array([(16, 1), (16, 2), (16, 3), (16, 4), (24, 4), (32, 4), (48, 4), (64, 4),
+                    (64, 5), (96, 5), (128, 5), (128, 6), (200, 6), (200, 8), (200, 10), (200, 16)],
+                   np.uint32)
 
 
 def init_params(seed: int = 0) -> np.ndarray:
@@ -79,6 +96,38 @@ def label_queries(dev: Device, q, prox, gt_off, gt_ids, k, K_prime, grid=GRID, f
     return labels, rec
 
 
+def choose_class(probs, tau: float) -> np.ndarray:
+    """The first class whose cumulative probability reaches tau (argmax when
+    tau <= 0) -- the rule of the GPU pick kernels."""
+    probs = np.asarray(probs)
+    if tau <= 0:
+        return np.argmax(probs, 1)
+    cs = np.cumsum(probs, 1) >= np.float32(tau)
+    return np.where(cs.any(1), cs.argmax(1), probs.shape[1] - 1)
+
+
+def label_classes(dev: Device, q, gt_off, gt_ids, k, K_prime, classes=CLASSES, id_map=None):
+    """Per query the cheapest class reaching the best recall of any class.
+    Returns (labels[nq], recall[16, nq], expansions[16, nq])."""
+    q = np.ascontiguousarray(q, np.float32)
+    nq = q.shape[0]
+    classes = np.asarray(classes, np.uint32)
+    mmax = int(classes[:, 1].max())
+    rec = np.zeros((len(classes), nq))
+    exp = np.zeros((len(classes), nq))
+    for ef in np.unique(classes[:, 0]):
+        _, prox, ex = dev.stage1_features(q, int(ef), mmax, k)
+        for c in np.nonzero(classes[:, 0] == ef)[0]:
+            off, ids = dev.rknn_search(q, np.ascontiguousarray(prox[:, :classes[c, 1]]), k, K_prime)
+            lab = ids if id_map is None else id_map[ids]
+            rec[c] = per_query_recall(gt_off, gt_ids, off, lab.astype(np.uint32))
+            exp[c] = ex[:, 0]
+    rec = np.nan_to_num(rec, nan=1.0)
+    best = rec.max(0)
+    labels = (rec >= best[None, :] - 1e-12).argmax(0).astype(np.uint32)
+    return labels, rec, exp
+
+
 @dataclass
 class LearnedM:
     params: np.ndarray          # HRNN_MLP_PARAMS floats
@@ -95,10 +144,7 @@ class LearnedM:
         return dev.mlp_forward(self.params, (np.asarray(feats) - self.mu) / self.sd)
 
     def choose_m(self, probs, tau: float) -> np.ndarray:
-        if tau <= 0:
-            return self.grid[np.argmax(probs, 1)]
-        c = np.argmax(np.cumsum(probs, 1) >= tau, 1)
-        return self.grid[c]
+        return self.grid[choose_class(probs, tau)]
 
     @classmethod
     def train(cls, dev: Device, feats, labels, epochs=40, batch=256, lr=2e-3, weight_decay=1e-5,
@@ -110,3 +156,52 @@ class LearnedM:
         p, losses = dev.mlp_train(init_params(seed), x, labels, epochs, batch, lr, weight_decay,
                                   seed)
         return cls(p, mu.astype(np.float32), sd.astype(np.float32)), losses
+
+
+@dataclass
+class LearnedEarlyStop:
+    """The MLP over CLASSES: model() is the HRNN_MLP_MODEL blob the adaptive
+    GPU path takes (parameters, then the features' mean and std)."""
+    params: np.ndarray
+    mu: np.ndarray
+    sd: np.ndarray
+    classes: np.ndarray = CLASSES
+    ef1: int = EF1
+
+    def model(self) -> np.ndarray:
+        m = np.concatenate([self.params, self.mu, self.sd]).astype(np.float32)
+        assert m.size == MLP_MODEL
+        return m
+
+    def probs(self, dev: Device, feats) -> np.ndarray:
+        return dev.mlp_forward(self.params, (np.asarray(feats, np.float32) - self.mu) / self.sd)
+
+    def choose(self, probs, tau: float) -> np.ndarray:
+        return choose_class(probs, tau)
+
+    def query(self, dev: Device, q, k, K_prime, tau, threshold=1.0):
+        """Two-stage query on the GPU; returns (res_off, res_ids, cls, proxies)."""
+        return dev.query_adaptive(q, self.classes, self.ef1, int(self.classes[:, 1].max()), k, K_prime,
+                                  model=self.model(), tau=tau, threshold=threshold)
+
+    @classmethod
+    def train(cls, dev: Device, feats, labels, epochs=60, batch=256, lr=2e-3, weight_decay=1e-5,
+              seed=0, classes=CLASSES, ef1=EF1):
+        feats = np.asarray(feats, np.float32)
+        mu = feats.mean(0)
+        sd = feats.std(0) + 1e-6
+        p, losses = dev.mlp_train(init_params(seed), (feats - mu) / sd, labels, epochs, batch, lr,
+                                  weight_decay, seed)
+        return cls(p, mu.astype(np.float32), sd.astype(np.float32), np.asarray(classes, np.uint32),
+                   ef1), losses
+
+    def pick_tau(self, dev: Device, feats, rec, target: float, taus=(0.3, 0.5, 0.7, 0.8, 0.9, 0.95,
+                                                                      0.98, 0.995)):
+        """The smallest tau whose mean recall over these queries (rec[16, nq]
+        from label_classes) reaches target; the last tau when none does."""
+        p = self.probs(dev, feats)
+        cols = np.arange(p.shape[0])
+        for tau in taus:
+            if rec[self.choose(p, tau), cols].mean() >= target:
+                return tau
+        return taus[-1]

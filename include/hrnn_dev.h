@@ -256,6 +256,39 @@ int hrnn_dev_rknn_search_learned_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
                                       uint32_t K_prime, float threshold, uint32_t *res_cnt_dev,
                                       uint32_t *res_ids_dev, uint32_t cap_per_q, uint32_t *work_dev);
 
+/* Learned early stop (no reference counterpart; it replaces the fixed
+ * ef_search / m of searchRknn, hrnn.h:1979-2019, per query).
+ *
+ * Stage-1 phase A at beam width ef1 with the 32 learned-m features computed
+ * inside the beam kernel (the definitions of HRNN::queryFeatures, over the
+ * first 16 list entries; r_k with this k): proxies_dev[nq*m],
+ * feats_dev[nq*32] unstandardised, expansions_dev (NULL or 2*nq u32:
+ * expansions, evaluations).  Used to build training sets. */
+int hrnn_dev_stage1_features_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t ef1,
+                                  uint32_t m, uint32_t k, uint32_t *proxies_dev, float *feats_dev,
+                                  uint32_t *expansions_dev);
+/* Whole two-stage query for a resident batch: stage-1 phase A at ef1 for
+ * every query (features standardised with the model's mu/sd); MLP forward;
+ * per query the cheapest of 16 (ef, m) classes whose cumulative probability
+ * reaches tau (argmax when tau <= 0), written to cls_dev[nq]; stage-2 phase A
+ * at the class's ef for the queries whose class has ef > ef1; proxies beyond
+ * the class's m masked (rows of m_max in proxies_dev); phases B and C.
+ * classes_dev: 32 u32, the 16 classes' ef then their m, ordered by cost with
+ * ef and m non-decreasing; ef_max >= every class ef (<= 512).  With
+ * model_dev NULL the classes are read from cls_dev instead (forced classes).
+ * work_dev as in hrnn_dev_rknn_search_dptr; expansions_dev (NULL or 4*nq u32)
+ * receives per query the stage-1 (expansions, evaluations) in [0, 2nq) and
+ * the stage-2 ones in [2nq, 4nq) (zero without a stage 2).
+ * Asynchronous on the library stream; nothing is copied to the host. */
+int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq,
+                                      const float *model_dev, const uint32_t *classes_dev,
+                                      uint32_t ef_max, float tau, uint32_t ef1, uint32_t m_max,
+                                      uint32_t k, uint32_t K_prime, float threshold,
+                                      uint32_t *proxies_dev, uint32_t *cls_dev,
+                                      uint32_t *res_cnt_dev, uint32_t *res_ids_dev,
+                                      uint32_t cap_per_q, uint32_t *work_dev,
+                                      uint32_t *expansions_dev);
+
 /* ---- distances ---------------------------------------------------------- */
 /* out[i] = L2Sqr(a[ia[i]], base[ib[i]]) in FP32 with the reference's 16-lane
  * partial-sum order (d % 16 == 0) or sequential order otherwise. */

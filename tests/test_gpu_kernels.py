@@ -173,6 +173,32 @@ def test_search_lds_table_and_hbm_fallback(small_data, small_ref, lds_kib, thr, 
     assert np.array_equal(gids, eids)
 
 
+def test_search_hub_results_above_m_times_kprime():
+    """A proxy whose reverse list is far longer than K' (node 0 is everyone's
+    nearest neighbour): one proxy and K' = 1 return ~2000 ids, which the
+    host-buffer entry point must size by the longest reverse list."""
+    n, K, d = 2000, 4, 16
+    rng = np.random.default_rng(9)
+    x = rng.integers(0, 256, (n, d)).astype(np.float32)
+    ids = np.zeros((n, K), np.uint32)
+    for i in range(n):
+        ids[i] = [0 if i else 1] + list(rng.choice(np.arange(2, n), K - 1, replace=False))
+    dists = np.full((n, K), 1e30, np.float32)            # every candidate is accepted
+    cnt = np.full(n, K, np.uint16)
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    dev.set_knng(ids, dists, cnt)
+    dev.build_rknn_csr()
+    off, ent = dev.get_rknn_csr()
+    q = rng.integers(0, 256, (3, d)).astype(np.float32)
+    prox = np.zeros((3, 1), np.uint32)
+    goff, gids = dev.rknn_search(q, prox, 1, 1, 1.0)
+    eoff, eids = oracle.search_bc(off, ent, dev.csr.wide, dev.csr.rank_bits, dists, cnt, K, x, q, prox,
+                                  1, 1, 1.0)
+    assert int(goff[1] - goff[0]) == n - 1
+    assert np.array_equal(goff, eoff) and np.array_equal(gids, eids)
+
+
 def test_search_float_data_and_k_per_query(float_data):
     """Float vectors: accept/reject uses the reference's FP32 order, so results are identical."""
     x, q = float_data
