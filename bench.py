@@ -481,7 +481,7 @@ def main() -> int:
     if os.path.exists(pmc_file):
         with open(pmc_file) as f:
             tr = json.load(f)
-        if tr.get("nq") == nq and tr.get("m") == m_b:
+        if tr.get("nq") == nq and tr.get("m") == m_b and tr.get("path", best_name) == best_name:
             traffic = tr.get("hbm_bytes_per_launch")
 
     # ---- cpu baseline: the reference's searchRknn on this run's index ----------------------

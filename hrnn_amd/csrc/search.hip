@@ -107,12 +107,6 @@ __global__ void k_rk_column(const float *__restrict__ knn_d, const uint32_t *__r
     rk[i] = (k - 1u < knn_cnt[i]) ? knn_d[i * K + k - 1u] : FLT_MAX;
 }
 
-template <class VT>
-__device__ __forceinline__ float cand_dist(bool exact, const float *qv, const VT *x, uint32_t c,
-                                           uint32_t d, uint32_t g) {
-    return exact ? dist_row_exact<VT>(qv, x, c, d, g) : dist_row<VT>(qv, x, c, d, g);
-}
-
 template <class EntryT, bool kCount, class VT>
 __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
     const VT *__restrict__ x, uint32_t d, int base_exact, const uint64_t *__restrict__ off,
@@ -144,6 +138,40 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
     uint32_t *keep = cand + cand_cap;
     const EntryT rmask = ((EntryT)1 << rank_bits) - 1;
     const uint32_t grp = threadIdx.x / 16, g = threadIdx.x & 15u;
+    // Phase C over list[0..cnt): 16-lane group grp takes candidates
+    // i0 + grp + 16 r (r < 4) of each 64-candidate pass, so four row reads
+    // (and their k-distances) are in flight per group; on_result(i, c, ok) runs
+    // on the group's lane 0.  Padding rows read row 0 (list entries may be
+    // rewritten by their owners during a pass).
+    auto verify = [&](const uint32_t *list, uint32_t cnt, bool exact, uint32_t kidx, auto &&on_result) {
+        for (uint32_t i0 = 0; i0 < cnt; i0 += 4u * (kSearchBlock / 16)) {
+            uint32_t id4[4];
+            bool v4[4];
+            float kd4[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const uint32_t i = i0 + grp + (kSearchBlock / 16) * (uint32_t)r;
+                v4[r] = i < cnt;
+                id4[r] = v4[r] ? list[i] : 0u;
+                kd4[r] = 0.f;
+                if (v4[r] && g == 0)
+                    kd4[r] = rk_col ? rk_col[id4[r]]
+                                    : (kidx < knn_cnt[id4[r]]) ? knn_d[(uint64_t)id4[r] * K + kidx] : FLT_MAX;
+            }
+            float o[4];
+            if (exact) {
+                dist4_exact<VT>(s_qv, x, id4, d, g, o);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; r++) o[r] = v4[r] ? dist_row<VT>(s_qv, x, id4[r], d, g) : FLT_MAX;
+            }
+            if (g == 0) {
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (v4[r]) on_result(i0 + grp + (kSearchBlock / 16) * (uint32_t)r, id4[r], o[r] <= kd4[r]);
+            }
+        }
+    };
 
     for (;;) {
         if (threadIdx.x == 0) {
@@ -213,17 +241,11 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
                 if (kCount) return (uint32_t)sl != kInvalidId && (float)(uint32_t)(sl >> 32) >= threshold;
                 return (uint32_t)sl != kInvalidId;
             });
-            // phase C: one 16-lane group per candidate; rejected ids become
-            // 0xFFFFFFFF in place (slot i is read and written by its group only)
-            for (uint32_t i = grp; i < nkeep; i += blockDim.x / 16) {
-                const uint32_t c = s_t32[i];
-                const float dist = cand_dist<VT>(exact, s_qv, x, c, d, g);
-                if (g == 0) {
-                    const float kd = rk_col ? rk_col[c]
-                                            : (kidx < knn_cnt[c]) ? knn_d[(uint64_t)c * K + kidx] : FLT_MAX;
-                    if (!(dist <= kd)) s_t32[i] = kInvalidId;
-                }
-            }
+            // phase C: rejected ids become 0xFFFFFFFF in place (slot i is read
+            // and written by its owner only)
+            verify(s_t32, nkeep, exact, kidx, [&](uint32_t i, uint32_t, bool ok) {
+                if (!ok) s_t32[i] = kInvalidId;
+            });
             __syncthreads();
             nres = block_compact(nkeep, s_t32, s_wc, [&](uint32_t i, uint32_t *v) {
                 *v = s_t32[i];
@@ -274,19 +296,13 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
                 mark[kCount ? (c >> 1) : (c >> 5)] = 0;
             }
             nkeep = s_nkeep;
-            for (uint32_t i = grp; i < nkeep; i += blockDim.x / 16) {
-                const uint32_t c = keep[i];
-                const float dist = cand_dist<VT>(exact, s_qv, x, c, d, g);
-                if (g == 0) {
-                    const float kd = rk_col ? rk_col[c]
-                                            : (kidx < knn_cnt[c]) ? knn_d[(uint64_t)c * K + kidx] : FLT_MAX;
-                    if (dist <= kd) {
-                        const uint32_t r = atomicAdd(&s_nres, 1u);
-                        if (r < res_lds) s_t32[r] = c;
-                        if (r < cap_per_q) res_ids[(uint64_t)qi * cap_per_q + r] = c;
-                    }
+            verify(keep, nkeep, exact, kidx, [&](uint32_t, uint32_t c, bool ok) {
+                if (ok) {
+                    const uint32_t r = atomicAdd(&s_nres, 1u);
+                    if (r < res_lds) s_t32[r] = c;
+                    if (r < cap_per_q) res_ids[(uint64_t)qi * cap_per_q + r] = c;
                 }
-            }
+            });
             __syncthreads();
             nres = s_nres;
             const uint32_t nout = nres < cap_per_q ? nres : cap_per_q;

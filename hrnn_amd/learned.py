@@ -108,7 +108,9 @@ def choose_class(probs, tau: float) -> np.ndarray:
 
 def label_classes(dev: Device, q, gt_off, gt_ids, k, K_prime, classes=CLASSES, id_map=None):
     """Per query the cheapest class reaching the best recall of any class.
-    Returns (labels[nq], recall[16, nq], expansions[16, nq])."""
+    Returns (labels[nq], recall[16, nq], expansions[16, nq]); recall is NaN
+    for a query with empty groundtruth (left out of macro recall, as the
+    reference's compute_rknn_recall does), whose label is class 0."""
     q = np.ascontiguousarray(q, np.float32)
     nq = q.shape[0]
     classes = np.asarray(classes, np.uint32)
@@ -122,9 +124,9 @@ def label_classes(dev: Device, q, gt_off, gt_ids, k, K_prime, classes=CLASSES, i
             lab = ids if id_map is None else id_map[ids]
             rec[c] = per_query_recall(gt_off, gt_ids, off, lab.astype(np.uint32))
             exp[c] = ex[:, 0]
-    rec = np.nan_to_num(rec, nan=1.0)
-    best = rec.max(0)
-    labels = (rec >= best[None, :] - 1e-12).argmax(0).astype(np.uint32)
+    r1 = np.nan_to_num(rec, nan=1.0)
+    best = r1.max(0)
+    labels = (r1 >= best[None, :] - 1e-12).argmax(0).astype(np.uint32)
     return labels, rec, exp
 
 
@@ -197,11 +199,12 @@ class LearnedEarlyStop:
 
     def pick_tau(self, dev: Device, feats, rec, target: float, taus=(0.3, 0.5, 0.7, 0.8, 0.9, 0.95,
                                                                       0.98, 0.995)):
-        """The smallest tau whose mean recall over these queries (rec[16, nq]
-        from label_classes) reaches target; the last tau when none does."""
+        """The smallest tau whose macro recall over these queries (rec[16, nq]
+        from label_classes, NaN rows of empty groundtruth left out) reaches
+        target; the last tau when none does."""
         p = self.probs(dev, feats)
         cols = np.arange(p.shape[0])
         for tau in taus:
-            if rec[self.choose(p, tau), cols].mean() >= target:
+            if np.nanmean(rec[self.choose(p, tau), cols]) >= target:
                 return tau
         return taus[-1]
