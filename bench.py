@@ -525,6 +525,59 @@ def main() -> int:
         except Exception as e:  # reported when available, never required
             log(f"cpu baseline skipped: {e}")
 
+    # ---- NNDescent build baseline: the reference's refineKNNG on the host cores -------------
+    # Bounded sample: the first iteration (the heaviest: every list entry is
+    # new) from identical seed lists on the first 50,000 points, once by the
+    # reference (OpenMP, all host threads) and once by the GPU kernels.  The
+    # 1M build is projected from it by the distance evaluations of this run's
+    # own 1M iterations (the work both implementations do per evaluated pair).
+    nnd_cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            import oracle
+            if oracle.ref_available():
+                ns = min(n, 50_000)
+                t = time.time()
+                ixs = Index(ns, d, cfg["M"], cfg["efc"], 100, K)
+                ixs.add_points(x[:ns], labels=np.arange(ns, dtype=np.uint64), nthreads=threads)
+                sids, sdist, scnt = ixs.get_knng()
+                xs = np.ascontiguousarray(x[:ns][ixs.labels()])      # rows in internal-id order
+                del ixs
+                log(f"NNDescent baseline: {ns}-point seed graph in {time.time() - t:.1f}s")
+                r = oracle.RefIndex(ns, d, M=cfg["M"], efc=cfg["efc"], K=K)
+                r.set_raw(xs)
+                r.set_knng(sids, sdist, scnt)
+                t1 = time.perf_counter()
+                r.refine(1, cfg["S"], nthreads=threads)
+                cpu_s = time.perf_counter() - t1
+                r.close()
+                ds = Device(0)
+                ds.set_base(xs)
+                ds.set_knng(sids, sdist, scnt)
+                t1 = time.perf_counter()
+                st_s = ds.nndescent(1, cfg["S"])
+                ds.sync()
+                gpu_s = time.perf_counter() - t1
+                ds.close()
+                d1 = float(st[0]["dists"]) if st else 0.0
+                work = sum(float(v["dists"]) for v in st) / d1 if d1 else float(cfg["iters"])
+                proj = cpu_s * (n / ns) * work
+                nnd_cpu = {"value": round(cpu_s, 3), "unit": "s (first iteration)", "cores": threads,
+                           "kind": "reference",
+                           "sample": f"first refineKNNG iteration (S={cfg['S']}, K={K}) from identical seed "
+                                     f"lists on the first {ns} points, OpenMP on {threads} host threads",
+                           "gpu_same_sample_s": round(gpu_s, 4),
+                           "speedup_same_sample": round(cpu_s / gpu_s, 1),
+                           "projected_cpu_build_s": round(proj, 1),
+                           "projection": f"x{n // ns} points, x{work:.2f} for all {cfg['iters']} iterations "
+                                         f"(distance evaluations of this run's 1M build relative to its first "
+                                         f"iteration)",
+                           "speedup_vs_projected": round(proj / nnd_s, 1)}
+                log(f"NNDescent baseline: reference {cpu_s:.2f}s, GPU {gpu_s:.3f}s on the sample; projected "
+                    f"reference 1M build {proj:.0f}s vs GPU {nnd_s:.2f}s")
+        except Exception as e:
+            log(f"NNDescent baseline skipped: {e}")
+
     out = {
         "metric": "RkNN QPS at recall>=0.95 (k=10); NNDescent K=500 build sec, 1M x 128",
         "value": best["qps"],
@@ -559,6 +612,7 @@ def main() -> int:
                                     if "fp32_fixed_m" in paths else None),
         "learned_m": learned_info,
         "autotune": {kk: v for kk, v in tuned.items() if kk != "timings_ms"},
+        "nndescent_cpu_baseline": nnd_cpu,
         "nndescent_build": {"seconds": round(nnd_s, 3), "iters": cfg["iters"], "sample": cfg["S"],
                             "knn_recall_at10_sample": round(knn_rec10, 4),
                             "per_iter": [{k2: (round(v, 2) if isinstance(v, float) else v)

@@ -14,6 +14,9 @@ hrnn_dev::~hrnn_dev() {
                       &csr_ent, &search_mark, &search_cand, &l0, &up_off, &up_pool, &beam_vis,
                       &s0, &s1, &s2, &s3, &s4, &s5, &s6, &s7, &s8, &nnd_aux, &counters};
     for (DevBuf *b : bufs) b->release();
+    if (ev_a) (void)hipEventDestroy(ev_a);
+    if (ev_b) (void)hipEventDestroy(ev_b);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
 }
 

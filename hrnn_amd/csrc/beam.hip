@@ -95,6 +95,14 @@ __device__ __forceinline__ unsigned long long wave_sort64(unsigned long long key
     return key;
 }
 
+// A value the whole wave holds alike, moved to a scalar register (so the
+// compiler keeps it, and what is computed from it, out of vector registers).
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ float uni_f(float v) { return __uint_as_float(uni(__float_as_uint(v))); }
+__device__ __forceinline__ unsigned long long uni64(unsigned long long v) {
+    return ((unsigned long long)uni((uint32_t)(v >> 32)) << 32) | uni((uint32_t)v);
+}
+
 // Lane j's 64-bit value, j uniform.
 __device__ __forceinline__ unsigned long long read_lane64(unsigned long long v, uint32_t j) {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)j);
@@ -109,7 +117,7 @@ __device__ __forceinline__ void l0_row(const uint32_t *__restrict__ l0, uint32_t
     const uint32_t *ll = l0 + (uint64_t)c * l0_words;
     const uint32_t w0 = ll[0];
     *nb = (lane + 1u < l0_words) ? ll[1 + lane] : 0u;
-    *cnt = min(w0 & 0xFFFFu, l0_words - 1u);
+    *cnt = uni(min(w0 & 0xFFFFu, l0_words - 1u));
 }
 
 // LDS layout of k_phase_a (dynamic part), in bytes: query vector, visited
@@ -149,7 +157,11 @@ __global__ __launch_bounds__(64) void k_phase_a(
 
     const uint32_t lane = threadIdx.x;
     const uint32_t g = lane & 15u, grp = lane >> 4;
-    const uint64_t qi = blockIdx.x;
+    uint64_t qi = blockIdx.x;
+    if (opt.qlist) {   // a query subset: workgroup i takes qlist[i]
+        if (blockIdx.x >= *opt.qcount) return;
+        qi = opt.qlist[blockIdx.x];
+    }
     if (kPerQueryEf) {   // per-query beam width; 0 leaves this query to another launch
         const uint32_t e = opt.ef_q[qi];
         if (e == 0) return;
@@ -178,7 +190,7 @@ __global__ __launch_bounds__(64) void k_phase_a(
         const bool v4[4] = {true, false, false, false};
         float o[4];
         beam_dist4<VT>(exact, s_qv, x, id4, v4, d, g, o);
-        cd = __shfl(o[0], 0, 64);
+        cd = uni_f(__shfl(o[0], 0, 64));
     }
     const float d_entry = cd;
     for (int level = maxlevel; level > 0; level--) {
@@ -228,8 +240,8 @@ __global__ __launch_bounds__(64) void k_phase_a(
                 }
             }
             if (best_id == cur) break;
-            cur = best_id;
-            cd = best;
+            cur = uni(best_id);
+            cd = uni_f(best);
         }
     }
 
@@ -251,7 +263,7 @@ __global__ __launch_bounds__(64) void k_phase_a(
     for (uint32_t iter = 0; iter <= n && pick != kInvalidId; iter++) {   // one expansion per pass
         const unsigned long long *lk = s_kl + buf * ef_cap;
         uint8_t *lx = s_xl + buf * ef_cap;
-        const uint32_t c = (uint32_t)lk[pick];
+        const uint32_t c = uni((uint32_t)lk[pick]);
         uint32_t cnt, nbw;
         if (c == pf_node) {
             cnt = pf_cnt;
@@ -329,7 +341,7 @@ __global__ __launch_bounds__(64) void k_phase_a(
         // the next expansion takes the smaller of the best admitted candidate
         // and the first unexpanded entry after pick; start its row load
         const unsigned long long kbest = read_lane64(key, 0);
-        const unsigned long long ko = (o != kInvalidId) ? lk[o] : ~0ull;
+        const unsigned long long ko = (o != kInvalidId) ? uni64(lk[o]) : ~0ull;
         pf_node = (uint32_t)(kbest < ko ? kbest : ko);
         l0_row(l0, l0_words, pf_node, lane, &pf_cnt, &pf_nb);
         // merge the sorted candidates into the sorted list, keeping the first

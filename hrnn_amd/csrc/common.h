@@ -225,6 +225,59 @@ __device__ __forceinline__ void dist4_exact(const float *__restrict__ qv, const 
                 const float t = qv[e + j] - (float)v[r][j];
                 acc[r] = __builtin_fmaf(t, t, acc[r]);
             }
+            // one row's arithmetic at a time: the loads above are already in
+            // flight, and interleaving the rows only raises register pressure
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        float a = acc[r];
+        a += __shfl_xor(a, 8, 16);
+        a += __shfl_xor(a, 4, 16);
+        a += __shfl_xor(a, 2, 16);
+        a += __shfl_xor(a, 1, 16);
+        out[r] = a;
+    }
+}
+
+// Raw buffer resource over `bytes` bytes from base (hardware range check:
+// a load past the end returns zeros).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+typedef uint32_t ex_u32x4 __attribute__((ext_vector_type(4)));
+
+// dist4_exact through a buffer resource: 32-bit byte offsets instead of
+// 64-bit row addresses (fewer registers and address instructions).  Rows are
+// row_bytes apart; the whole base must lie below 4 GiB of offset.
+template <class VT>
+__device__ __forceinline__ void dist4_exact_buf(const float *__restrict__ qv, __amdgpu_buffer_rsrc_t rs,
+                                                const uint32_t *id, uint32_t row_bytes, uint32_t d,
+                                                uint32_t g, float *out) {
+    typedef typename ExactRow<VT>::V V;
+    constexpr uint32_t W = ExactRow<VT>::W;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    uint32_t off[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) off[r] = id[r] * row_bytes + W * g * (uint32_t)sizeof(VT);
+    for (uint32_t e = W * g; e < d; e += 16 * W) {
+        V v[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const ex_u32x4 raw = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off[r], 0, 0);
+            v[r] = __builtin_bit_cast(V, raw);
+            off[r] += 16u * W * (uint32_t)sizeof(VT);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+#pragma unroll
+            for (uint32_t j = 0; j < W; j++) {
+                const float t = qv[e + j] - (float)v[r][j];
+                acc[r] = __builtin_fmaf(t, t, acc[r]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 #pragma unroll

@@ -5,6 +5,10 @@
 struct hrnn_dev {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream (default priority) and two events for work that overlaps
+    // within one call; created on first use
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
     std::string err;
     int n_cu = 0;
     size_t lds_per_block = 0;
@@ -64,7 +68,8 @@ struct hrnn_dev {
     // small device counters, by byte offset: 0 base-int flag, 32 long-list
     // count, 128 CSR total, 192 longest list, 256 search queue, 384 device
     // error word (set by a kernel that met invalid graph data; read and
-    // cleared by hrnn_dev_sync)
+    // cleared by hrnn_dev_sync), 512 second search queue, 640 the two list
+    // counts of the learned query path
     hrnn::DevBuf counters;
 
     ~hrnn_dev();
@@ -105,10 +110,24 @@ uint32_t scan_block_count(uint64_t n);
 void scan_u32(hrnn_dev_t h, const uint32_t *cnt, uint64_t n, uint64_t *off, uint64_t *bsum,
               uint64_t *total);
 uint64_t max_list_len(hrnn_dev_t h);
+constexpr uint32_t kQueueWord = 64;        // search work queue (u32 index in counters)
+constexpr uint32_t kQueue2Word = 128;      // second search queue (overlapping launches)
+constexpr uint32_t kListCountWord = 160;   // two list counts of the learned query path
+// Optional query subset and stream of a search launch: queries qlist[0 ..
+// *qcount) (device memory) instead of 0 .. nq-1, on `stream` with its work
+// queue at counters[queue_word].
+struct SearchSel {
+    const uint32_t *qlist = nullptr;
+    const uint32_t *qcount = nullptr;
+    hipStream_t stream = nullptr;
+    uint32_t queue_word = kQueueWord;
+};
+
 void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint32_t *proxies,
                           uint32_t m, const uint32_t *k_per_q, uint32_t k, uint32_t K_prime,
                           float threshold, uint32_t *res_cnt, uint32_t *res_ids,
-                          uint32_t cap_per_q, uint32_t *work = nullptr);
+                          uint32_t cap_per_q, uint32_t *work = nullptr,
+                          const SearchSel *sel = nullptr);
 void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t bucket, hrnn_nnd_iter_stats *st);
 // Per-query beam statistics (raw words): 0 d(q, entry), 1 d(q, layer-0
 // start), 2..17 the 16 closest beam distances, 18 expansions, 19 list size.
@@ -126,6 +145,8 @@ struct BeamOpt {
     uint32_t K = 0, feat_k = 0;       // r_k = knn_d[p][feat_k - 1]
     const float *mu = nullptr;        // standardisation (f - mu) / sd, both or neither
     const float *sd = nullptr;
+    const uint32_t *qlist = nullptr;  // workgroup i takes query qlist[i], i < *qcount
+    const uint32_t *qcount = nullptr;
 };
 void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint32_t ef,
                  uint32_t *proxies_dev, uint32_t *expansions_dev, float *raw_dev = nullptr,

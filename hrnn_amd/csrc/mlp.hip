@@ -225,12 +225,16 @@ __global__ void k_pick_m(const float *__restrict__ probs, uint32_t nq, const uin
 // stage-1 proxies serve) and the class's m (at most mstride), and masks the
 // stage-1 proxies beyond that m.  classes = [ef of 16 classes, m of 16 classes].
 // exp2 (optional): the stage-2 expansion counters, zeroed here for the
-// queries that have no stage 2.
+// queries that have no stage 2.  lists (optional, 2 nq u32): the escalated
+// queries are appended to lists[0..), the others to lists[nq..), counted in
+// list_cnt[0] and list_cnt[1] (zero before the launch; list order is
+// arbitrary).
 __global__ void k_pick_class(const float *__restrict__ probs, uint32_t nq,
                              const uint32_t *__restrict__ classes, float tau, uint32_t ef1,
                              uint32_t *__restrict__ prox, uint32_t mstride, uint32_t *__restrict__ cls,
                              uint32_t *__restrict__ ef_q, uint32_t *__restrict__ m_q,
-                             uint32_t *__restrict__ exp2) {
+                             uint32_t *__restrict__ exp2, uint32_t *__restrict__ lists,
+                             uint32_t *__restrict__ list_cnt) {
     const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
     if (qi >= nq) return;
     uint32_t c = kOut - 1;
@@ -258,6 +262,10 @@ __global__ void k_pick_class(const float *__restrict__ probs, uint32_t nq,
     const uint32_t m = min(max(classes[kOut + c], 1u), mstride);
     ef_q[qi] = ef > ef1 ? ef : 0u;
     m_q[qi] = m;
+    if (lists) {   // escalated queries to lists[0..), the others to lists[nq..)
+        const uint32_t which = ef > ef1 ? 0u : 1u;
+        lists[(uint64_t)which * nq + atomicAdd(&list_cnt[which], 1u)] = qi;
+    }
     if (exp2 && ef <= ef1) {   // no stage-2 beam for this query
         exp2[2 * (uint64_t)qi] = 0;
         exp2[2 * (uint64_t)qi + 1] = 0;
@@ -402,11 +410,19 @@ int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
         REQUIRE(q_dev && classes_dev && proxies_dev && cls_dev && res_cnt_dev && res_ids_dev, HRNN_E_ARG,
                 "query_adaptive: null argument");
         if (!nq) return;
-        h->s8.reserve((uint64_t)nq * (kIn + kOut + 2) * 4);
+        h->s8.reserve((uint64_t)nq * (kIn + kOut + 4) * 4);
         float *feats = h->s8.as<float>();
         float *probs = feats + (uint64_t)nq * kIn;
         uint32_t *ef_q = reinterpret_cast<uint32_t *>(probs + (uint64_t)nq * kOut);
         uint32_t *m_q = ef_q + nq;
+        uint32_t *lists = m_q + nq;   // [escalated | the rest], nq each
+        uint32_t *list_cnt = h->counters.as<uint32_t>() + kListCountWord;
+        if (!h->stream2) {
+            HIPC(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+            HIPC(hipEventCreateWithFlags(&h->ev_a, hipEventDisableTiming));
+            HIPC(hipEventCreateWithFlags(&h->ev_b, hipEventDisableTiming));
+        }
+        HIPC(hipMemsetAsync(list_cnt, 0, 8, h->stream));
         // stage 1: every query at ef1, with the standardised features when a
         // model is given
         BeamOpt o1;
@@ -423,18 +439,35 @@ int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
                                nullptr, nq, probs, nullptr, nullptr);
         hipLaunchKernelGGL(k_pick_class, dim3(ceil_div(nq, 128)), dim3(128), 0, h->stream,
                            model_dev ? probs : nullptr, nq, classes_dev, tau, ef1, proxies_dev, m_max,
-                           cls_dev, ef_q, m_q, exp2);
+                           cls_dev, ef_q, m_q, exp2, lists, list_cnt);
         HIPC(hipGetLastError());
-        // stage 2: the queries whose class needs a wider beam (the others
-        // leave their workgroup at once)
-        if (ef_max > ef1) {
+        // Phases B+C of the queries that stay at stage 1 run on the second
+        // stream while stage 2 runs on the first; the escalated queries are
+        // searched once both are done.
+        HIPC(hipEventRecord(h->ev_a, h->stream));
+        HIPC(hipStreamWaitEvent(h->stream2, h->ev_a, 0));
+        SearchSel rest;
+        rest.qlist = lists + nq;
+        rest.qcount = list_cnt + 1;
+        rest.stream = h->stream2;
+        rest.queue_word = kQueue2Word;
+        rknn_search_resident(h, q_dev, nq, proxies_dev, m_max, nullptr, k, K_prime, threshold, res_cnt_dev,
+                             res_ids_dev, cap_per_q, work_dev, &rest);
+        HIPC(hipEventRecord(h->ev_b, h->stream2));
+        if (ef_max > ef1) {   // stage 2: one workgroup per escalated query
             BeamOpt o2;
             o2.ef_q = ef_q;
             o2.m_q = m_q;
+            o2.qlist = lists;
+            o2.qcount = list_cnt;
             beam_launch(h, q_dev, nq, m_max, ef_max, proxies_dev, exp2, nullptr, &o2);
         }
+        HIPC(hipStreamWaitEvent(h->stream, h->ev_b, 0));
+        SearchSel esc;
+        esc.qlist = lists;
+        esc.qcount = list_cnt;
         rknn_search_resident(h, q_dev, nq, proxies_dev, m_max, nullptr, k, K_prime, threshold, res_cnt_dev,
-                             res_ids_dev, cap_per_q, work_dev);
+                             res_ids_dev, cap_per_q, work_dev, &esc);
     });
 }
 

@@ -112,7 +112,8 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
     const VT *__restrict__ x, uint32_t d, int base_exact, const uint64_t *__restrict__ off,
     const EntryT *__restrict__ ent, uint32_t rank_bits, const float *__restrict__ knn_d,
     const uint32_t *__restrict__ knn_cnt, uint32_t K, const float *__restrict__ rk_col,
-    const float *__restrict__ q, uint32_t nq,
+    const float *__restrict__ q, uint32_t nq, const uint32_t *__restrict__ qlist,
+    const uint32_t *__restrict__ qcount,
     const uint32_t *__restrict__ proxies, uint32_t m, const uint32_t *__restrict__ k_per_q,
     uint32_t k, uint32_t K_prime, float threshold, uint32_t log_tmax, uint32_t *__restrict__ mark_all,
     uint64_t mark_words, uint32_t *__restrict__ cand_all, uint64_t cand_cap,
@@ -181,8 +182,9 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
             s_nres = 0;
         }
         __syncthreads();
-        const uint32_t qi = s_q;
-        if (qi >= nq) break;
+        const uint32_t idx = s_q;
+        if (idx >= (qcount ? *qcount : nq)) break;
+        const uint32_t qi = qlist ? qlist[idx] : idx;
         int not_u8 = 0;
         for (uint32_t e = threadIdx.x; e < d; e += blockDim.x) {
             const float v = q[(uint64_t)qi * d + e];
@@ -339,11 +341,13 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
 void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint32_t *proxies,
                           uint32_t m, const uint32_t *k_per_q, uint32_t k, uint32_t K_prime,
                           float threshold, uint32_t *res_cnt, uint32_t *res_ids,
-                          uint32_t cap_per_q, uint32_t *work) {
+                          uint32_t cap_per_q, uint32_t *work, const SearchSel *sel) {
     REQUIRE(h->csr_valid, HRNN_E_STATE, "search: no CSR");
     REQUIRE(m >= 1 && m <= (uint32_t)kMaxM, HRNN_E_ARG, "search: m must be in [1, 1024]");
     REQUIRE(cap_per_q > 0, HRNN_E_ARG, "search: cap_per_q == 0");
     REQUIRE(h->n < kInvalidId, HRNN_E_ARG, "search: n must be below 2^32 - 1");
+    const SearchSel so = sel ? *sel : SearchSel{};
+    hipStream_t st = so.stream ? so.stream : h->stream;
     const bool count_mode = threshold > 1.0f;
     const uint64_t n = h->n;
     const uint64_t words = count_mode ? (n + 1) / 2 : (n + 31) / 32;
@@ -361,11 +365,11 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     const uint64_t mark_bytes = (uint64_t)slots * words * 4;
     if (h->search_mark.bytes < mark_bytes) {
         h->search_mark.reserve(mark_bytes);
-        HIPC(hipMemsetAsync(h->search_mark.p, 0, h->search_mark.bytes, h->stream));
+        HIPC(hipMemsetAsync(h->search_mark.p, 0, h->search_mark.bytes, st));
     }
     h->search_cand.reserve((uint64_t)slots * cand_cap * 8);
-    uint32_t *queue = h->counters.as<uint32_t>() + 64;
-    HIPC(hipMemsetAsync(queue, 0, 4, h->stream));
+    uint32_t *queue = h->counters.as<uint32_t>() + so.queue_word;
+    HIPC(hipMemsetAsync(queue, 0, 4, st));
     // LDS table: the largest power-of-two slot count within the configured
     // bytes (u32 slots in presence mode, u64 in count mode), at least 256
     const uint32_t slot_bytes = count_mode ? 8u : 4u;
@@ -384,7 +388,7 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     if (!k_per_q && k >= 1 && k <= h->K) {
         if (h->rk_col_k != k || h->rk_col_version != h->knn_version || h->rk_col_n != n) {
             h->rk_col.reserve(n * 4);
-            hipLaunchKernelGGL(k_rk_column, dim3(ceil_div(n, 256)), dim3(256), 0, h->stream, kd, kc, n, h->K, k,
+            hipLaunchKernelGGL(k_rk_column, dim3(ceil_div(n, 256)), dim3(256), 0, st, kd, kc, n, h->K, k,
                                h->rk_col.as<float>());
             HIPC(hipGetLastError());
             h->rk_col_k = k;
@@ -400,10 +404,10 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     const bool bf16 = h->small_int && h->vec_bf16;
     const int base_exact = (h->base_u8 && h->d <= 258 && h->d % 8 == 0) ? 1 : 0;
 #define LAUNCH(ET, CM, VT, XP)                                                                      \
-    hipLaunchKernelGGL((k_rknn_search<ET, CM, VT>), dim3(slots), dim3(kSearchBlock), dyn, h->stream, \
+    hipLaunchKernelGGL((k_rknn_search<ET, CM, VT>), dim3(slots), dim3(kSearchBlock), dyn, st,       \
                        XP, h->d, base_exact, off, h->csr_ent.as<ET>(), h->rank_bits, kd, kc, h->K, rk, q, \
-                       nq, proxies, m, k_per_q, k, K_prime, threshold, log_tmax, mark, words, cand,  \
-                       cand_cap, queue, res_cnt, res_ids, cap_per_q, work)
+                       nq, so.qlist, so.qcount, proxies, m, k_per_q, k, K_prime, threshold, log_tmax,  \
+                       mark, words, cand, cand_cap, queue, res_cnt, res_ids, cap_per_q, work)
 #define LAUNCH_VT(ET, CM)                                                  \
     if (bf16) LAUNCH(ET, CM, __hip_bfloat16, h->xb.as<__hip_bfloat16>());  \
     else LAUNCH(ET, CM, float, h->x.as<float>())
