@@ -25,11 +25,11 @@
 // the best candidate and the first open entry), so its layer-0 row is loaded
 // while the merge runs.  When every base and query value is an integer in
 // [0, 255] and d <= 258, every distance sum is an exact integer, so rows are
-// read with 16-byte loads and summed in any order; otherwise the reference's
-// 16-partial order is used.  Every id or count read from the graph is
-// range-checked; a bad one is skipped and raises the handle's device error
-// word (reported by hrnn_dev_sync).  The descent and the beam run bounded
-// loops.
+// read with 16-byte buffer loads (32-bit offsets, base below 4 GiB) and summed
+// in any order; otherwise the reference's 16-partial order is used.  Every id
+// or count read from the graph is range-checked; a bad one is skipped and
+// raises the handle's device error word (reported by hrnn_dev_sync).  The
+// descent and the beam run bounded loops.
 #include <algorithm>
 #include <cfloat>
 
@@ -63,10 +63,11 @@ __device__ __forceinline__ bool vis_insert(uint32_t *tab, uint32_t log_slots, ui
 // uniform within the group).  Exact-sum data issues the four row reads
 // together; otherwise the reference's 16-partial order, one row at a time.
 template <class VT>
-__device__ __forceinline__ void beam_dist4(bool exact, const float *qv, const VT *x, const uint32_t *id,
+__device__ __forceinline__ void beam_dist4(bool exact, const float *qv, const VT *x,
+                                           __amdgpu_buffer_rsrc_t xrs, const uint32_t *id,
                                            const bool *val, uint32_t d, uint32_t g, float *o) {
-    if (exact) {
-        dist4_exact<VT>(qv, x, id, d, g, o);
+    if (exact) {   // exact-sum data; the host allows it only for a base below 4 GiB
+        dist4_exact_buf<VT>(qv, xrs, id, d * (uint32_t)sizeof(VT), d, g, o);
     } else {
 #pragma unroll
         for (int r = 0; r < 4; r++) o[r] = val[r] ? dist_row<VT>(qv, x, id[r], d, g) : FLT_MAX;
@@ -137,7 +138,7 @@ struct BeamLds {
 // gives each query's beam width; 0 skips the query).  A separate
 // instantiation so profiles tell the two stages apart.
 template <class VT, bool kPerQueryEf>
-__global__ __launch_bounds__(64) void k_phase_a(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_phase_a(
     const VT *__restrict__ x, uint32_t d, uint32_t n, const uint32_t *__restrict__ l0,
     uint32_t l0_words, const uint32_t *__restrict__ up_off, const uint32_t *__restrict__ up_pool,
     uint64_t pool_words, uint32_t up_words, uint32_t entry, int maxlevel, const float *__restrict__ q,
@@ -175,6 +176,9 @@ __global__ __launch_bounds__(64) void k_phase_a(
     }
     // exact sums for this query: every base and query value an integer in [0, 255]
     const bool exact = base_exact && __ballot(!q_u8) == 0;
+    // the base as a buffer (exact path): loads past its end return zeros
+    const __amdgpu_buffer_rsrc_t xrs =
+        make_rsrc(x, exact ? (uint32_t)((uint64_t)n * d * sizeof(VT)) : 0u);
     for (uint32_t t = lane; t < (1u << log_vis); t += 64) s_vis[t] = kInvalidId;
     if (lane == 0) {
         s_lost = 0;
@@ -189,7 +193,7 @@ __global__ __launch_bounds__(64) void k_phase_a(
         const uint32_t id4[4] = {cur, cur, cur, cur};
         const bool v4[4] = {true, false, false, false};
         float o[4];
-        beam_dist4<VT>(exact, s_qv, x, id4, v4, d, g, o);
+        beam_dist4<VT>(exact, s_qv, x, xrs, id4, v4, d, g, o);
         cd = uni_f(__shfl(o[0], 0, 64));
     }
     const float d_entry = cd;
@@ -222,7 +226,7 @@ __global__ __launch_bounds__(64) void k_phase_a(
                     id4[r] = v;
                 }
                 float o[4];
-                beam_dist4<VT>(exact, s_qv, x, id4, v4, d, g, o);
+                beam_dist4<VT>(exact, s_qv, x, xrs, id4, v4, d, g, o);
                 // the host's sequential scan: the first strictly smaller wins
                 const uint32_t e = min(cnt - b, 16u);
                 for (uint32_t s = 0; s < e; s++) {
@@ -311,7 +315,7 @@ __global__ __launch_bounds__(64) void k_phase_a(
                 id4[r] = s_ni[v4[r] ? t : b];
             }
             float o4[4];
-            beam_dist4<VT>(exact, s_qv, x, id4, v4, d, g, o4);
+            beam_dist4<VT>(exact, s_qv, x, xrs, id4, v4, d, g, o4);
             if (g == 0) {
 #pragma unroll
                 for (int r = 0; r < 4; r++)
@@ -494,7 +498,9 @@ void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint
     uint32_t *err = h->counters.as<uint32_t>() + kErrWord;
     // every sum exact in any order: values in [0, 255] and d <= 258, with the
     // row width the wide loads need (8 BF16 or 4 FP32 elements)
-    const int base_exact = (h->base_u8 && h->d <= 258 && h->d % 8 == 0) ? 1 : 0;
+    const size_t vbytes = (h->small_int && h->vec_bf16) ? 2 : 4;
+    const int base_exact = (h->base_u8 && h->d <= 258 && h->d % 8 == 0 &&
+                            (uint64_t)h->n * h->d * vbytes < 0xFFFFFFF0ull) ? 1 : 0;
     const uint32_t ef_cap = (ef + 3u) & ~3u;
     const uint32_t log_vis = beam_log_vis(ef);
     const BeamLds lay(h->d, ef_cap, log_vis);

@@ -109,7 +109,7 @@ __global__ void k_rk_column(const float *__restrict__ knn_d, const uint32_t *__r
 
 template <class EntryT, bool kCount, class VT>
 __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
-    const VT *__restrict__ x, uint32_t d, int base_exact, const uint64_t *__restrict__ off,
+    const VT *__restrict__ x, uint32_t d, int base_exact, uint64_t x_bytes, const uint64_t *__restrict__ off,
     const EntryT *__restrict__ ent, uint32_t rank_bits, const float *__restrict__ knn_d,
     const uint32_t *__restrict__ knn_cnt, uint32_t K, const float *__restrict__ rk_col,
     const float *__restrict__ q, uint32_t nq, const uint32_t *__restrict__ qlist,
@@ -139,6 +139,8 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
     uint32_t *keep = cand + cand_cap;
     const EntryT rmask = ((EntryT)1 << rank_bits) - 1;
     const uint32_t grp = threadIdx.x / 16, g = threadIdx.x & 15u;
+    // the base as a buffer for the exact path (base_exact only below 4 GiB)
+    const __amdgpu_buffer_rsrc_t xrs = make_rsrc(x, base_exact ? (uint32_t)(x_bytes) : 0u);
     // Phase C over list[0..cnt): 16-lane group grp takes candidates
     // i0 + grp + 16 r (r < 4) of each 64-candidate pass, so four row reads
     // (and their k-distances) are in flight per group; on_result(i, c, ok) runs
@@ -161,7 +163,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
             }
             float o[4];
             if (exact) {
-                dist4_exact<VT>(s_qv, x, id4, d, g, o);
+                dist4_exact_buf<VT>(s_qv, xrs, id4, d * (uint32_t)sizeof(VT), d, g, o);
             } else {
 #pragma unroll
                 for (int r = 0; r < 4; r++) o[r] = v4[r] ? dist_row<VT>(s_qv, x, id4[r], d, g) : FLT_MAX;
@@ -402,10 +404,11 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     // Candidate vectors are read from the BF16 mirror when every base value is
     // exactly representable in it (same distances, half the bytes).
     const bool bf16 = h->small_int && h->vec_bf16;
-    const int base_exact = (h->base_u8 && h->d <= 258 && h->d % 8 == 0) ? 1 : 0;
+    const uint64_t x_bytes = n * h->d * (bf16 ? 2 : 4);
+    const int base_exact = (h->base_u8 && h->d <= 258 && h->d % 8 == 0 && x_bytes < 0xFFFFFFF0ull) ? 1 : 0;
 #define LAUNCH(ET, CM, VT, XP)                                                                      \
     hipLaunchKernelGGL((k_rknn_search<ET, CM, VT>), dim3(slots), dim3(kSearchBlock), dyn, st,       \
-                       XP, h->d, base_exact, off, h->csr_ent.as<ET>(), h->rank_bits, kd, kc, h->K, rk, q, \
+                       XP, h->d, base_exact, x_bytes, off, h->csr_ent.as<ET>(), h->rank_bits, kd, kc, h->K, rk, q, \
                        nq, so.qlist, so.qcount, proxies, m, k_per_q, k, K_prime, threshold, log_tmax,  \
                        mark, words, cand, cand_cap, queue, res_cnt, res_ids, cap_per_q, work)
 #define LAUNCH_VT(ET, CM)                                                  \
