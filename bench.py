@@ -14,7 +14,8 @@ learned-m model trained on separate queries.  A timed step is one whole
 search of all queries.  GPU paths: queries resident in HBM, phase A (HNSW
 descent + layer-0 beam, k_phase_a) then phases B and C (k_rknn_search) in one
 call, results left in HBM; FP32 vectors (fp32_fixed_m) or the BF16 mirror,
-exact for these integer data (bf16_fixed_m).  Host-phase-A paths (the
+exact for these integer data (bf16_fixed_m), or the u8 mirror, exact for
+8-bit data (u8_fixed_m).  Host-phase-A paths (the
 reference's structure): phase A on the host cores in batches of 1000, proxies
 copied to HBM, phases B and C on the GPU, the host working on batch i+1 while
 the GPU searches batch i; with fixed m (host_a_bf16_fixed_m) or the learned
@@ -23,9 +24,12 @@ Learned early stop (bf16_learned_adaptive, all on the GPU in one call): a
 stage-1 beam at ef 16 for every query with the features fused in, the MLP,
 a stage-2 beam at the chosen class's ef only for the queries that need it,
 phases B and C with the class's m; trained on 20,000 separate queries
-labelled by GPU runs of every class against the exact groundtruth.
+labelled by GPU runs of every class against the exact groundtruth; on the
+BF16 mirror (bf16_learned_adaptive) and on the u8 mirror
+(u8_learned_adaptive, identical results).
 value = the highest QPS among the paths whose macro recall is >= 0.95;
-speedup_vs_fp32_fixed_m sets it against the plain FP32 fixed-m GPU path.
+speedup_vs_fp32_fixed_m sets it against the plain FP32 fixed-m GPU path,
+bf16_learned_speedup_vs_fp32_fixed_m the BF16 learned path.
 
 roofline: the dominant GPU kernel of that path (k_phase_a, k_phase_a_stage2
 or k_rknn_search), timed with HIP events on the library's stream over
@@ -286,6 +290,12 @@ def main() -> int:
     bf_off, bf_ids = results()
     assert np.array_equal(fp32_off, bf_off) and np.array_equal(fp32_ids, bf_ids), "BF16 path differs"
     paths["bf16_fixed_m"] = {"m": m_star, "recall": rec, "phase_a": "gpu", "sec": timed(lambda: run_gpu(m_star))}
+    ck(L.hrnn_dev_set_vector_precision(h, 3))          # u8 mirror (exact for these 8-bit data)
+    run_gpu(m_star)
+    u8_off, u8_ids = results()
+    assert np.array_equal(fp32_off, u8_off) and np.array_equal(fp32_ids, u8_ids), "u8 path differs"
+    paths["u8_fixed_m"] = {"m": m_star, "recall": rec, "phase_a": "gpu", "sec": timed(lambda: run_gpu(m_star))}
+    ck(L.hrnn_dev_set_vector_precision(h, 1))
     ck(L.hrnn_dev_sync(h))
     run_fixed(m_star)                                   # host phase A, GPU phases B and C
     ha_off, ha_ids = results()
@@ -347,8 +357,13 @@ def main() -> int:
                                                    EF1, 16, k, Kp, 1.0, pd.ptr, cls_d.ptr, cd.ptr, idd.ptr,
                                                    cap, wd.ptr, exp_d.ptr))
 
+        ck(L.hrnn_dev_set_vector_precision(h, 3))       # the same model on the u8 mirror
+        run_adaptive()
+        u_off, u_ids = results()
+        ck(L.hrnn_dev_set_vector_precision(h, 1))
         run_adaptive()
         a_off, a_ids = results()
+        assert np.array_equal(a_off, u_off) and np.array_equal(a_ids, u_ids), "u8 adaptive path differs"
         rec_ad = macro_recall(gt_off, gt_ids, a_off, a_ids)
         cls_h = cls_d.to_host()
         exq = exp_d.to_host().reshape(2, nq, 2)
@@ -359,6 +374,9 @@ def main() -> int:
             "mean_expansions": round(float(exq[0, :, 0].mean() + exq[1, :, 0].mean()), 1),
             "class_hist": np.bincount(cls_h, minlength=16).tolist(),
             "sec": timed(run_adaptive)}
+        ck(L.hrnn_dev_set_vector_precision(h, 3))
+        paths["u8_learned_adaptive"] = dict(paths["bf16_learned_adaptive"], sec=timed(run_adaptive))
+        ck(L.hrnn_dev_set_vector_precision(h, 1))
         learned_info["adaptive"] = {
             "classes": CLASSES.tolist(), "label_hist": np.bincount(lab_a, minlength=16).tolist(),
             "loss_first": round(float(losses_a[0]), 4), "loss_last": round(float(losses_a[-1]), 4),
@@ -377,8 +395,9 @@ def main() -> int:
     # ---- roofline: the GPU kernels of the best path, HIP events on the library stream -------
     reps = 20
     e0, e1 = hiprt.Event(), hiprt.Event()
-    s_v = 4 if best_name == "fp32_fixed_m" else 2
-    ck(L.hrnn_dev_set_vector_precision(h, 0 if best_name == "fp32_fixed_m" else 1))
+    s_v = 4 if best_name == "fp32_fixed_m" else 1 if best_name.startswith("u8_") else 2
+    ck(L.hrnn_dev_set_vector_precision(h, 0 if best_name == "fp32_fixed_m" else
+                                       3 if best_name.startswith("u8_") else 1))
     m_b = best.get("m", m_max)
     kern = {}
 
@@ -392,7 +411,7 @@ def main() -> int:
         e1.sync()
         return e1.ms_since(e0) / reps
 
-    adaptive = best_name == "bf16_learned_adaptive"
+    adaptive = best_name.endswith("_learned_adaptive")
     if adaptive:
         # stage 1 alone (k_phase_a<bf16, false> at EF1, features fused), then the
         # whole two-stage query, which leaves the masked proxies (stride 16) in pd
@@ -589,7 +608,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if best_name == "fp32_fixed_m" else "bf16",
+        "dtype": "f32" if best_name == "fp32_fixed_m" else "u8" if best_name.startswith("u8_") else "bf16",
         "data": "synthetic",
         "config": {"workload": f"hrnn-{args.config}-{n // 1000}k-x{d}", "n": n, "nq": nq, "d": d,
                    "K_knng": K, "M": cfg["M"], "ef_construction": cfg["efc"], "nn_iters": cfg["iters"],
@@ -610,6 +629,9 @@ def main() -> int:
         # FP32 fixed-m GPU path, both at recall >= the target
         "speedup_vs_fp32_fixed_m": (round(best["qps"] / paths["fp32_fixed_m"]["qps"], 3)
                                     if "fp32_fixed_m" in paths else None),
+        "bf16_learned_speedup_vs_fp32_fixed_m": (
+            round(paths["bf16_learned_adaptive"]["qps"] / paths["fp32_fixed_m"]["qps"], 3)
+            if "bf16_learned_adaptive" in paths and "fp32_fixed_m" in paths else None),
         "learned_m": learned_info,
         "autotune": {kk: v for kk, v in tuned.items() if kk != "timings_ms"},
         "nndescent_cpu_baseline": nnd_cpu,

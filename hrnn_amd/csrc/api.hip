@@ -10,7 +10,7 @@ using namespace hrnn;
 
 hrnn_dev::~hrnn_dev() {
     (void)hipSetDevice(device);
-    DevBuf *bufs[] = {&x, &xb, &xh, &norms, &knn_ids, &knn_d, &knn_cnt, &knn_new, &rk_col, &csr_off,
+    DevBuf *bufs[] = {&x, &xb, &xh, &xu8, &norms, &knn_ids, &knn_d, &knn_cnt, &knn_new, &rk_col, &csr_off,
                       &csr_ent, &search_mark, &search_cand, &l0, &up_off, &up_pool, &beam_vis,
                       &s0, &s1, &s2, &s3, &s4, &s5, &s6, &s7, &s8, &nnd_aux, &counters};
     for (DevBuf *b : bufs) b->release();
@@ -50,8 +50,8 @@ void check_device_error(hrnn_dev_t h) {
 // the value-range flags for rows [row0, row0 + rows): flags bit 0 = some value
 // is not an integer in [-256, 256], bit 1 = some value is not one in [0, 255].
 __global__ void k_base_mirrors(const float *__restrict__ x, __hip_bfloat16 *__restrict__ xb,
-                               __half *__restrict__ xh, float *__restrict__ norms,
-                               uint64_t row0, uint64_t rows, uint32_t d,
+                               __half *__restrict__ xh, uint8_t *__restrict__ xu8,
+                               float *__restrict__ norms, uint64_t row0, uint64_t rows, uint32_t d,
                                unsigned int *__restrict__ flags) {
     uint64_t r = row0 + blockIdx.x * (uint64_t)(blockDim.x / 64) + (threadIdx.x / 64);
     if (r >= row0 + rows) return;
@@ -63,6 +63,7 @@ __global__ void k_base_mirrors(const float *__restrict__ x, __hip_bfloat16 *__re
         float v = xr[e];
         xb[r * d + e] = __float2bfloat16(v);
         xh[r * d + e] = __float2half(v);
+        xu8[r * d + e] = (uint8_t)fminf(fmaxf(rintf(v), 0.f), 255.f);   // exact when base_u8
         acc = __builtin_fmaf(v, v, acc);
         const bool integral = v == rintf(v);
         bad |= !(integral && fabsf(v) <= 256.f);
@@ -81,7 +82,8 @@ void launch_base_mirrors(hrnn_dev_t h, uint64_t row0, uint64_t rows) {
     HIPC(hipMemsetAsync(flag, 0, sizeof(unsigned int), h->stream));
     unsigned blocks = ceil_div(rows, 4);
     hipLaunchKernelGGL(k_base_mirrors, dim3(blocks), dim3(256), 0, h->stream, h->x.as<float>(),
-                       h->xb.as<__hip_bfloat16>(), h->xh.as<__half>(), h->norms.as<float>(),
+                       h->xb.as<__hip_bfloat16>(), h->xh.as<__half>(), h->xu8.as<uint8_t>(),
+                       h->norms.as<float>(),
                        row0, rows, h->d, flag);
     HIPC(hipGetLastError());
     unsigned int f = 0;
@@ -184,6 +186,7 @@ static void set_base_common(hrnn_dev_t h, const float *x, uint64_t n, uint32_t d
     h->x.reserve(n * d * sizeof(float));
     h->xb.reserve(n * d * 2);
     h->xh.reserve(n * d * 2);
+    h->xu8.reserve(n * d);
     h->norms.reserve(n * sizeof(float));
     if (device)
         HIPC(hipMemcpyAsync(h->x.p, x, n * d * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
@@ -216,19 +219,21 @@ int hrnn_dev_append_base(hrnn_dev_t h, const float *x, uint64_t n_add) {
         if (n_new > h->n_cap) {
             // grow by copying into larger buffers (geometric)
             uint64_t cap = std::max<uint64_t>(n_new, h->n_cap * 3 / 2);
-            DevBuf nx, nb, nh, nn;
+            DevBuf nx, nb, nh, nu, nn;
             nx.reserve(cap * h->d * 4);
             nb.reserve(cap * h->d * 2);
             nh.reserve(cap * h->d * 2);
+            nu.reserve(cap * h->d);
             nn.reserve(cap * 4);
             HIPC(hipMemcpyAsync(nx.p, h->x.p, h->n * h->d * 4, hipMemcpyDeviceToDevice, h->stream));
             HIPC(hipMemcpyAsync(nb.p, h->xb.p, h->n * h->d * 2, hipMemcpyDeviceToDevice, h->stream));
             HIPC(hipMemcpyAsync(nh.p, h->xh.p, h->n * h->d * 2, hipMemcpyDeviceToDevice, h->stream));
+            HIPC(hipMemcpyAsync(nu.p, h->xu8.p, h->n * h->d, hipMemcpyDeviceToDevice, h->stream));
             HIPC(hipMemcpyAsync(nn.p, h->norms.p, h->n * 4, hipMemcpyDeviceToDevice, h->stream));
             HIPC(hipStreamSynchronize(h->stream));
-            h->x.release(); h->xb.release(); h->xh.release(); h->norms.release();
-            h->x = nx; h->xb = nb; h->xh = nh; h->norms = nn;
-            nx.p = nb.p = nh.p = nn.p = nullptr;
+            h->x.release(); h->xb.release(); h->xh.release(); h->xu8.release(); h->norms.release();
+            h->x = nx; h->xb = nb; h->xh = nh; h->xu8 = nu; h->norms = nn;
+            nx.p = nb.p = nh.p = nu.p = nn.p = nullptr;
             h->n_cap = cap;
         }
         HIPC(hipMemcpyAsync(h->x.as<float>() + h->n * h->d, x, n_add * h->d * 4,
@@ -281,8 +286,9 @@ int hrnn_dev_get_tuning(hrnn_dev_t h, const char *key, int *value) {
 }
 
 int hrnn_dev_set_vector_precision(hrnn_dev_t h, int prec) {
-    if (!h || (prec != HRNN_PREC_F32 && prec != HRNN_PREC_BF16)) return HRNN_E_ARG;
-    h->vec_bf16 = (prec == HRNN_PREC_BF16);
+    if (!h || (prec != HRNN_PREC_F32 && prec != HRNN_PREC_BF16 && prec != HRNN_PREC_U8)) return HRNN_E_ARG;
+    h->vec_bf16 = (prec != HRNN_PREC_F32);
+    h->vec_u8 = (prec == HRNN_PREC_U8);
     return HRNN_OK;
 }
 

@@ -32,6 +32,7 @@
 // descent and the beam run bounded loops.
 #include <algorithm>
 #include <cfloat>
+#include <type_traits>
 
 #include "handle.h"
 
@@ -63,11 +64,14 @@ __device__ __forceinline__ bool vis_insert(uint32_t *tab, uint32_t log_slots, ui
 // uniform within the group).  Exact-sum data issues the four row reads
 // together; otherwise the reference's 16-partial order, one row at a time.
 template <class VT>
-__device__ __forceinline__ void beam_dist4(bool exact, const float *qv, const VT *x,
-                                           __amdgpu_buffer_rsrc_t xrs, const uint32_t *id,
+__device__ __forceinline__ void beam_dist4(bool exact, const float *qv, const uint32_t *q8, uint32_t qq,
+                                           const VT *x, __amdgpu_buffer_rsrc_t xrs, const uint32_t *id,
                                            const bool *val, uint32_t d, uint32_t g, float *o) {
     if (exact) {   // exact-sum data; the host allows it only for a base below 4 GiB
-        dist4_exact_buf<VT>(qv, xrs, id, d * (uint32_t)sizeof(VT), d, g, o);
+        if constexpr (std::is_same<VT, uint8_t>::value)
+            dist4_u8_buf(q8, qq, xrs, id, d, g, o);
+        else
+            dist4_exact_buf<VT>(qv, xrs, id, d * (uint32_t)sizeof(VT), d, g, o);
     } else {
 #pragma unroll
         for (int r = 0; r < 4; r++) o[r] = val[r] ? dist_row<VT>(qv, x, id[r], d, g) : FLT_MAX;
@@ -121,13 +125,15 @@ __device__ __forceinline__ void l0_row(const uint32_t *__restrict__ l0, uint32_t
     *cnt = uni(min(w0 & 0xFFFFu, l0_words - 1u));
 }
 
-// LDS layout of k_phase_a (dynamic part), in bytes: query vector, visited
-// set, list keys [2][ef_cap], expanded flags [2][ef_cap].
+// LDS layout of k_phase_a (dynamic part), in bytes: query vector, the query
+// as packed bytes, visited set, list keys [2][ef_cap], expanded flags
+// [2][ef_cap].
 struct BeamLds {
-    uint32_t qv_bytes, vis_off, k_off, x_off, total;
+    uint32_t qv_bytes, q8_off, vis_off, k_off, x_off, total;
     __host__ __device__ BeamLds(uint32_t d, uint32_t ef_cap, uint32_t log_vis) {
         qv_bytes = (d * 4u + 15u) & ~15u;
-        vis_off = qv_bytes;
+        q8_off = qv_bytes;
+        vis_off = q8_off + ((d + 15u) & ~15u);
         k_off = vis_off + (4u << log_vis);
         x_off = k_off + 2u * ef_cap * 8u;
         total = x_off + 2u * ef_cap;
@@ -151,6 +157,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     extern __shared__ __align__(16) unsigned char s_dyn[];
     const BeamLds lay(d, ef_cap, log_vis);
     float *s_qv = reinterpret_cast<float *>(s_dyn);
+    uint32_t *s_q8 = reinterpret_cast<uint32_t *>(s_dyn + lay.q8_off);
     uint32_t *s_vis = reinterpret_cast<uint32_t *>(s_dyn + lay.vis_off);
     // the sorted list as (dist, id) keys, double buffered, with expanded flags
     unsigned long long *s_kl = reinterpret_cast<unsigned long long *>(s_dyn + lay.k_off);
@@ -176,6 +183,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     }
     // exact sums for this query: every base and query value an integer in [0, 255]
     const bool exact = base_exact && __ballot(!q_u8) == 0;
+    uint32_t qq = 0;   // u8 storage: the query packed 4 bytes per word, and sum q^2
+    if constexpr (std::is_same<VT, uint8_t>::value) {
+        if (exact) {
+            uint32_t part = 0;
+            for (uint32_t w = lane; 4 * w < d; w += 64) {
+                const uint32_t b0 = (uint32_t)q[qi * d + 4 * w], b1 = (uint32_t)q[qi * d + 4 * w + 1];
+                const uint32_t b2 = (uint32_t)q[qi * d + 4 * w + 2], b3 = (uint32_t)q[qi * d + 4 * w + 3];
+                s_q8[w] = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
+                part += b0 * b0 + b1 * b1 + b2 * b2 + b3 * b3;
+            }
+            for (int s2 = 32; s2 > 0; s2 >>= 1) part += (uint32_t)__shfl_xor((int)part, s2, 64);
+            qq = uni(part);
+        }
+    }
     // the base as a buffer (exact path): loads past its end return zeros
     const __amdgpu_buffer_rsrc_t xrs =
         make_rsrc(x, exact ? (uint32_t)((uint64_t)n * d * sizeof(VT)) : 0u);
@@ -193,7 +214,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         const uint32_t id4[4] = {cur, cur, cur, cur};
         const bool v4[4] = {true, false, false, false};
         float o[4];
-        beam_dist4<VT>(exact, s_qv, x, xrs, id4, v4, d, g, o);
+        beam_dist4<VT>(exact, s_qv, s_q8, qq, x, xrs, id4, v4, d, g, o);
         cd = uni_f(__shfl(o[0], 0, 64));
     }
     const float d_entry = cd;
@@ -226,7 +247,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                     id4[r] = v;
                 }
                 float o[4];
-                beam_dist4<VT>(exact, s_qv, x, xrs, id4, v4, d, g, o);
+                beam_dist4<VT>(exact, s_qv, s_q8, qq, x, xrs, id4, v4, d, g, o);
                 // the host's sequential scan: the first strictly smaller wins
                 const uint32_t e = min(cnt - b, 16u);
                 for (uint32_t s = 0; s < e; s++) {
@@ -315,7 +336,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                 id4[r] = s_ni[v4[r] ? t : b];
             }
             float o4[4];
-            beam_dist4<VT>(exact, s_qv, x, xrs, id4, v4, d, g, o4);
+            beam_dist4<VT>(exact, s_qv, s_q8, qq, x, xrs, id4, v4, d, g, o4);
             if (g == 0) {
 #pragma unroll
                 for (int r = 0; r < 4; r++)
@@ -498,7 +519,8 @@ void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint
     uint32_t *err = h->counters.as<uint32_t>() + kErrWord;
     // every sum exact in any order: values in [0, 255] and d <= 258, with the
     // row width the wide loads need (8 BF16 or 4 FP32 elements)
-    const size_t vbytes = (h->small_int && h->vec_bf16) ? 2 : 4;
+    const bool use_u8 = h->vec_u8 && h->base_u8;
+    const size_t vbytes = use_u8 ? 1 : (h->small_int && h->vec_bf16) ? 2 : 4;
     const int base_exact = (h->base_u8 && h->d <= 258 && h->d % 8 == 0 &&
                             (uint64_t)h->n * h->d * vbytes < 0xFFFFFFF0ull) ? 1 : 0;
     const uint32_t ef_cap = (ef + 3u) & ~3u;
@@ -514,8 +536,11 @@ void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint
 #define LAUNCH(VT, XP)                  \
     if (o.ef_q) LAUNCH1(VT, true, XP);  \
     else LAUNCH1(VT, false, XP)
-    // BF16 mirror where it is exact (same distances, half the bytes)
-    if (h->small_int && h->vec_bf16) {
+    // u8 or BF16 mirror where it is exact (same distances, a quarter or half
+    // of the bytes)
+    if (use_u8) {
+        LAUNCH(uint8_t, h->xu8.as<uint8_t>());
+    } else if (h->small_int && h->vec_bf16) {
         LAUNCH(__hip_bfloat16, h->xb.as<__hip_bfloat16>());
     } else {
         LAUNCH(float, h->x.as<float>());

@@ -138,6 +138,31 @@ __device__ __forceinline__ float dist_row<__hip_bfloat16>(const float *__restric
     return r;
 }
 
+template <>
+__device__ __forceinline__ float dist_row<uint8_t>(const float *__restrict__ qv,
+                                                   const uint8_t *__restrict__ base, uint64_t id,
+                                                   uint32_t d, uint32_t g) {
+    const uint8_t *xv = base + id * d;
+    if ((d & 15u) == 0) {   // the reference's 16-partial order over exact u8 values
+        float p = 0.f;
+        for (uint32_t e = g; e < d; e += 16) {
+            const float t = qv[e] - (float)xv[e];
+            p = __builtin_fmaf(t, t, p);
+        }
+        const int base_lane = (int)(threadIdx.x & 63u) & ~15;
+        float r = __shfl(p, base_lane + 0, 64);
+#pragma unroll
+        for (int l = 1; l < 16; l++) r += __shfl(p, base_lane + l, 64);
+        return r;
+    }
+    float r = 0.f;
+    for (uint32_t e = 0; e < d; e++) {
+        const float t = qv[e] - (float)xv[e];
+        r = __builtin_fmaf(t, t, r);
+    }
+    return r;
+}
+
 typedef __bf16 ex_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float ex_f32x4 __attribute__((ext_vector_type(4)));
 
@@ -288,6 +313,47 @@ __device__ __forceinline__ void dist4_exact_buf(const float *__restrict__ qv, __
         a += __shfl_xor(a, 2, 16);
         a += __shfl_xor(a, 1, 16);
         out[r] = a;
+    }
+}
+
+typedef uint32_t ex_u32x2 __attribute__((ext_vector_type(2)));
+
+// Squared L2 from a u8 query to four u8 rows (8-bit data, exact): each
+// 16-lane group reads 8 bytes per lane and step through a buffer resource and
+// forms sum (q - x)^2 = sum q^2 + sum x^2 - 2 sum q x with integer dot
+// products (v_dot4_u32_u8); every term is an exact integer and, for d <= 258,
+// so is the FP32 result.  q8 holds the query packed 4 bytes per word (LDS),
+// qq = sum q^2.  d must be a multiple of 8; rows are d bytes apart.
+__device__ __forceinline__ void dist4_u8_buf(const uint32_t *__restrict__ q8, uint32_t qq,
+                                             __amdgpu_buffer_rsrc_t rs, const uint32_t *id, uint32_t d,
+                                             uint32_t g, float *out) {
+    int acc[4] = {0, 0, 0, 0};
+    uint32_t off[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) off[r] = id[r] * d + 8u * g;
+    for (uint32_t e = 8u * g; e < d; e += 128u) {
+        ex_u32x2 v[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            v[r] = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off[r], 0, 0);
+            off[r] += 128u;
+        }
+        const uint32_t qa = q8[e >> 2], qb = q8[(e >> 2) + 1];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const uint32_t xx = __builtin_amdgcn_udot4(v[r].x, v[r].x, __builtin_amdgcn_udot4(v[r].y, v[r].y, 0u, false), false);
+            const uint32_t qx = __builtin_amdgcn_udot4(qa, v[r].x, __builtin_amdgcn_udot4(qb, v[r].y, 0u, false), false);
+            acc[r] += (int)xx - 2 * (int)qx;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        int a = acc[r];
+        a += __shfl_xor(a, 8, 16);
+        a += __shfl_xor(a, 4, 16);
+        a += __shfl_xor(a, 2, 16);
+        a += __shfl_xor(a, 1, 16);
+        out[r] = (float)((int)qq + a);
     }
 }
 

@@ -17,14 +17,16 @@ struct hrnn_dev {
     uint64_t n = 0;
     uint64_t n_cap = 0;
     uint32_t d = 0;
-    hrnn::DevBuf x, xb, xh, norms;
+    hrnn::DevBuf x, xb, xh, xu8, norms;
     int small_int = 0;   // every value an integer in [-256, 256] (exact in BF16/FP16)
     // every value an integer in [0, 255]: with d <= 128 every sum formed by the
     // distance kernels (direct difference or Gram form, any order) is an
     // integer below 2^24, so all of them are exact
     int base_u8 = 0;
-    // read vectors from the BF16 mirror where that is exact (small_int)
+    // read vectors from the BF16 mirror where that is exact (small_int), or
+    // from the u8 mirror where that is (base_u8; HRNN_PREC_U8)
     bool vec_bf16 = true;
+    bool vec_u8 = false;
 
     // launch shapes chosen by the autotuner (hrnn_dev_set_tuning)
     int tune_search_wg_per_cu = 8;   // persistent RkNN search workgroups per CU

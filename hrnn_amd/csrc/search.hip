@@ -119,10 +119,10 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
     uint64_t mark_words, uint32_t *__restrict__ cand_all, uint64_t cand_cap,
     uint32_t *__restrict__ queue, uint32_t *__restrict__ res_cnt, uint32_t *__restrict__ res_ids,
     uint32_t cap_per_q, uint32_t *__restrict__ work) {
-    __shared__ uint32_t s_q, s_ncand, s_nkeep, s_nres;
+    __shared__ uint32_t s_q, s_ncand, s_nkeep, s_nres, s_qq;
     __shared__ uint32_t s_wc[kSearchWaves];
-    // dynamic LDS: table (2^log_tmax slots) | query vector (d) | prefix starts
-    // (m u64) | prefix positions (m + 1 u32)
+    // dynamic LDS: table (2^log_tmax slots) | query vector (d floats) | the
+    // query as packed bytes (d) | prefix starts (m u64) | prefix positions (m + 1 u32)
     extern __shared__ __align__(16) unsigned char s_dyn[];
     typedef typename std::conditional<kCount, unsigned long long, uint32_t>::type SlotT;
     SlotT *s_tab = reinterpret_cast<SlotT *>(s_dyn);
@@ -130,8 +130,10 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
     const uint32_t tmax = 1u << log_tmax;
     const uint32_t res_lds = tmax * (uint32_t)(sizeof(SlotT) / 4);   // u32 capacity of the table
     float *s_qv = reinterpret_cast<float *>(s_dyn + (size_t)tmax * sizeof(SlotT));
+    uint32_t *s_q8 = reinterpret_cast<uint32_t *>(s_dyn + (size_t)tmax * sizeof(SlotT) +
+                                                  (((size_t)d * 4 + 7) & ~(size_t)7));
     uint64_t *s_start = reinterpret_cast<uint64_t *>(
-        s_dyn + (size_t)tmax * sizeof(SlotT) + (((size_t)d * 4 + 7) & ~(size_t)7));
+        s_dyn + (size_t)tmax * sizeof(SlotT) + (((size_t)d * 4 + 7) & ~(size_t)7) + (((size_t)d + 7) & ~(size_t)7));
     uint32_t *s_pos = reinterpret_cast<uint32_t *>(s_start + m);
 
     uint32_t *mark = mark_all + (uint64_t)blockIdx.x * mark_words;
@@ -163,7 +165,10 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
             }
             float o[4];
             if (exact) {
-                dist4_exact_buf<VT>(s_qv, xrs, id4, d * (uint32_t)sizeof(VT), d, g, o);
+                if constexpr (std::is_same<VT, uint8_t>::value)
+                    dist4_u8_buf(s_q8, s_qq, xrs, id4, d, g, o);
+                else
+                    dist4_exact_buf<VT>(s_qv, xrs, id4, d * (uint32_t)sizeof(VT), d, g, o);
             } else {
 #pragma unroll
                 for (int r = 0; r < 4; r++) o[r] = v4[r] ? dist_row<VT>(s_qv, x, id4[r], d, g) : FLT_MAX;
@@ -179,6 +184,7 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
     for (;;) {
         if (threadIdx.x == 0) {
             s_q = atomicAdd(queue, 1u);
+            s_qq = 0;
             s_ncand = 0;
             s_nkeep = 0;
             s_nres = 0;
@@ -192,6 +198,19 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
             const float v = q[(uint64_t)qi * d + e];
             s_qv[e] = v;
             not_u8 |= !(v >= 0.f && v <= 255.f && v == rintf(v));
+        }
+        if constexpr (std::is_same<VT, uint8_t>::value) {
+            // u8 storage: the query packed 4 bytes per word, and sum q^2 (used
+            // only when every value is an integer in [0, 255])
+            uint32_t part = 0;
+            for (uint32_t w = threadIdx.x; 4 * w + 3 < d; w += blockDim.x) {
+                const float *qw = q + (uint64_t)qi * d + 4 * w;
+                const uint32_t b0 = (uint32_t)fminf(fmaxf(qw[0], 0.f), 255.f), b1 = (uint32_t)fminf(fmaxf(qw[1], 0.f), 255.f);
+                const uint32_t b2 = (uint32_t)fminf(fmaxf(qw[2], 0.f), 255.f), b3 = (uint32_t)fminf(fmaxf(qw[3], 0.f), 255.f);
+                s_q8[w] = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
+                part += b0 * b0 + b1 * b1 + b2 * b2 + b3 * b3;
+            }
+            atomicAdd(&s_qq, part);
         }
         // locate each proxy's prefix with rank < K'
         for (uint32_t p = threadIdx.x; p < m; p += blockDim.x) {
@@ -378,7 +397,7 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     uint32_t log_tmax = 8;
     while ((2ull << log_tmax) * slot_bytes <= (uint64_t)h->tune_search_lds_kib * 1024) log_tmax++;
     const size_t dyn = ((size_t)1 << log_tmax) * slot_bytes + (((size_t)h->d * 4 + 7) & ~(size_t)7) +
-                       (size_t)m * 8 + ((size_t)m + 1) * 4;
+                       (((size_t)h->d + 7) & ~(size_t)7) + (size_t)m * 8 + ((size_t)m + 1) * 4;
     REQUIRE(dyn <= std::min<size_t>(h->lds_per_block, 65536), HRNN_E_ARG,
             "search: LDS need " + std::to_string(dyn) + " B exceeds the workgroup limit");
     const uint64_t *off = h->csr_off.as<uint64_t>();
@@ -401,18 +420,21 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     }
     uint32_t *mark = h->search_mark.as<uint32_t>();
     uint32_t *cand = h->search_cand.as<uint32_t>();
-    // Candidate vectors are read from the BF16 mirror when every base value is
-    // exactly representable in it (same distances, half the bytes).
-    const bool bf16 = h->small_int && h->vec_bf16;
-    const uint64_t x_bytes = n * h->d * (bf16 ? 2 : 4);
+    // Candidate vectors are read from the u8 mirror (HRNN_PREC_U8) or the BF16
+    // mirror when every base value is exactly representable in it (same
+    // distances, a quarter or half of the bytes).
+    const bool use_u8 = h->vec_u8 && h->base_u8;
+    const bool bf16 = !use_u8 && h->small_int && h->vec_bf16;
+    const uint64_t x_bytes = n * h->d * (use_u8 ? 1 : bf16 ? 2 : 4);
     const int base_exact = (h->base_u8 && h->d <= 258 && h->d % 8 == 0 && x_bytes < 0xFFFFFFF0ull) ? 1 : 0;
 #define LAUNCH(ET, CM, VT, XP)                                                                      \
     hipLaunchKernelGGL((k_rknn_search<ET, CM, VT>), dim3(slots), dim3(kSearchBlock), dyn, st,       \
                        XP, h->d, base_exact, x_bytes, off, h->csr_ent.as<ET>(), h->rank_bits, kd, kc, h->K, rk, q, \
                        nq, so.qlist, so.qcount, proxies, m, k_per_q, k, K_prime, threshold, log_tmax,  \
                        mark, words, cand, cand_cap, queue, res_cnt, res_ids, cap_per_q, work)
-#define LAUNCH_VT(ET, CM)                                                  \
-    if (bf16) LAUNCH(ET, CM, __hip_bfloat16, h->xb.as<__hip_bfloat16>());  \
+#define LAUNCH_VT(ET, CM)                                                      \
+    if (use_u8) LAUNCH(ET, CM, uint8_t, h->xu8.as<uint8_t>());                 \
+    else if (bf16) LAUNCH(ET, CM, __hip_bfloat16, h->xb.as<__hip_bfloat16>()); \
     else LAUNCH(ET, CM, float, h->x.as<float>())
     if (h->wide) {
         if (count_mode) { LAUNCH_VT(uint64_t, true); } else { LAUNCH_VT(uint64_t, false); }

@@ -16,10 +16,11 @@ import numpy as np
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libhrnn_hip.so")
 
-PREC_F32, PREC_BF16, PREC_F16 = 0, 1, 2
+PREC_F32, PREC_BF16, PREC_F16, PREC_U8 = 0, 1, 2, 3
 MLP_PARAMS = 7312              # HRNN_MLP_PARAMS
 MLP_MODEL = MLP_PARAMS + 64    # + feature mean / std
-_PREC = {"f32": PREC_F32, "fp32": PREC_F32, "bf16": PREC_BF16, "f16": PREC_F16, "fp16": PREC_F16}
+_PREC = {"f32": PREC_F32, "fp32": PREC_F32, "bf16": PREC_BF16, "f16": PREC_F16, "fp16": PREC_F16,
+         "u8": PREC_U8}
 
 STATUS = {0: "OK", -1: "E_ARG", -2: "E_HIP", -3: "E_STATE", -4: "E_OOM", -5: "E_CAP",
           -6: "E_RANGE"}

@@ -50,7 +50,8 @@ enum hrnn_status {
 enum hrnn_precision {
     HRNN_PREC_F32 = 0,    /* FP32 direct difference (reference summation order) */
     HRNN_PREC_BF16 = 1,   /* BF16 inputs on MFMA, FP32 accumulate, FP32 re-rank */
-    HRNN_PREC_F16 = 2     /* FP16 inputs on MFMA, FP32 accumulate, FP32 re-rank */
+    HRNN_PREC_F16 = 2,    /* FP16 inputs on MFMA, FP32 accumulate, FP32 re-rank */
+    HRNN_PREC_U8 = 3      /* vector storage only: the u8 mirror (8-bit data) */
 };
 
 /* ---- handle ---------------------------------------------------------- */
@@ -77,7 +78,11 @@ const float *hrnn_dev_base_ptr(hrnn_dev_t h);
 /* Storage the search and phase-A kernels read vectors from: HRNN_PREC_BF16
  * (default) uses the BF16 mirror when every base value is exactly
  * representable in it (then distances are bit-identical to FP32 and the
- * vector bytes halve) and FP32 otherwise; HRNN_PREC_F32 always reads FP32. */
+ * vector bytes halve) and FP32 otherwise; HRNN_PREC_U8 uses the u8 mirror
+ * when every base value is an integer in [0, 255] (a quarter of the FP32
+ * bytes; distances of u8 queries by integer dot products, exact, so again
+ * bit-identical) and falls back as BF16 otherwise; HRNN_PREC_F32 always
+ * reads FP32. */
 int hrnn_dev_set_vector_precision(hrnn_dev_t h, int prec);
 /* Launch shapes set by the autotuner (hrnn_amd/autotune.py): "search_wg_per_cu"
  * (persistent RkNN search workgroups per CU, default 8); "search_lds_kib" (KiB

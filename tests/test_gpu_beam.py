@@ -73,3 +73,18 @@ def test_phase_a_and_query_c1_scale():
     same = np.mean([set(ids[i, :cnt[i]].tolist()) == set(ids_h[off_h[i]:off_h[i + 1]].tolist())
                     for i in range(nq)])
     assert same >= 0.99, same
+
+
+def test_phase_a_u8_storage_equals_bf16(small_index):
+    """The u8 mirror gives the same distances as BF16 on 8-bit data, so the
+    GPU beam returns the same proxies, ties included; for shifted queries the
+    reference-order path over bytes agrees as well."""
+    ix, x, q = small_index
+    d = hrnn_amd.Device.borrow(ix.device_handle(), 3000, 128, 48)
+    for qq in (q, (q + np.float32(0.5)).astype(np.float32)):
+        d.set_vector_precision("bf16")
+        a = ix.proxies_gpu(qq, 20, 64)
+        d.set_vector_precision("u8")
+        b = ix.proxies_gpu(qq, 20, 64)
+        d.set_vector_precision("bf16")
+        assert np.array_equal(a, b)

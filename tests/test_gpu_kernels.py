@@ -417,3 +417,30 @@ def test_nndescent_non_full_seed():
     rg = _knn_recall(gid, gc, 40, exact, 40)
     assert abs(rg - ro) <= 0.005, (rg, ro)
     assert int(gc.min()) == int(oc.min())
+
+
+@pytest.mark.parametrize("qshift", [0.0, 0.5])
+def test_search_u8_storage_matches_oracle(small_data, small_ref, qshift):
+    """u8 vector storage (HRNN_PREC_U8) on 8-bit data: integer queries use the
+    v_dot4 form, shifted queries the reference's 16-partial order over the
+    bytes; both give the oracle's answers exactly, through the LDS table and
+    the HBM fallback."""
+    x, q = small_data
+    q = (q + np.float32(qshift)).astype(np.float32)
+    r, _ = small_ref
+    ids, dists, cnt = r.get_knng()
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    dev.set_knng(ids, dists, cnt)
+    dev.build_rknn_csr()
+    dev.set_vector_precision("u8")
+    off, ent = dev.get_rknn_csr()
+    prox = r.proxies(q, 8, 64)
+    prox[::2, 2:] = 0xFFFFFFFF
+    for kib in (16, 1):
+        dev.set_tuning("search_lds_kib", kib)
+        for thr in (1.0, 2.0):
+            goff, gids = dev.rknn_search(q, prox, 10, 48, thr)
+            eoff, eids = oracle.search_bc(off, ent, dev.csr.wide, dev.csr.rank_bits, dists, cnt, r.K, x,
+                                          q, prox, 10, 48, thr)
+            assert np.array_equal(goff, eoff) and np.array_equal(gids, eids), (kib, thr)
