@@ -108,6 +108,12 @@ __device__ __forceinline__ unsigned long long uni64(unsigned long long v) {
     return ((unsigned long long)uni((uint32_t)(v >> 32)) << 32) | uni((uint32_t)v);
 }
 
+__device__ __forceinline__ unsigned long long shfl_xor64(unsigned long long v, int m) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, 64);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 // Lane j's 64-bit value, j uniform.
 __device__ __forceinline__ unsigned long long read_lane64(unsigned long long v, uint32_t j) {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)j);
@@ -143,8 +149,11 @@ struct BeamLds {
 // kPerQueryEf: the stage-2 form of the two-stage learned query (opt.ef_q
 // gives each query's beam width; 0 skips the query).  A separate
 // instantiation so profiles tell the two stages apart.
+// Occupancy: 6 waves per SIMD for u8 storage, 5 otherwise (the register
+// budgets at which the compiler needs no spills).
 template <class VT, bool kPerQueryEf>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_phase_a(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(std::is_same<VT, uint8_t>::value ? 6 : 5)))
+void k_phase_a(
     const VT *__restrict__ x, uint32_t d, uint32_t n, const uint32_t *__restrict__ l0,
     uint32_t l0_words, const uint32_t *__restrict__ up_off, const uint32_t *__restrict__ up_pool,
     uint64_t pool_words, uint32_t up_words, uint32_t entry, int maxlevel, const float *__restrict__ q,
@@ -153,6 +162,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     uint32_t *__restrict__ err, BeamOpt opt) {
     __shared__ float s_nd[kBeamMaxDeg];
     __shared__ uint32_t s_ni[kBeamMaxDeg];
+    __shared__ unsigned long long s_ck[kBeamMaxDeg];   // admitted keys of one expansion
     __shared__ uint32_t s_lost, s_bad;
     extern __shared__ __align__(16) unsigned char s_dyn[];
     const BeamLds lay(d, ef_cap, log_vis);
@@ -200,7 +210,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     // the base as a buffer (exact path): loads past its end return zeros
     const __amdgpu_buffer_rsrc_t xrs =
         make_rsrc(x, exact ? (uint32_t)((uint64_t)n * d * sizeof(VT)) : 0u);
-    for (uint32_t t = lane; t < (1u << log_vis); t += 64) s_vis[t] = kInvalidId;
+    for (uint32_t t = lane; t < (1u << log_vis) / 4; t += 64)   // 16-byte stores
+        reinterpret_cast<uint4 *>(s_vis)[t] = make_uint4(kInvalidId, kInvalidId, kInvalidId, kInvalidId);
     if (lane == 0) {
         s_lost = 0;
         s_bad = 0;
@@ -248,19 +259,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                 }
                 float o[4];
                 beam_dist4<VT>(exact, s_qv, s_q8, qq, x, xrs, id4, v4, d, g, o);
-                // the host's sequential scan: the first strictly smaller wins
-                const uint32_t e = min(cnt - b, 16u);
-                for (uint32_t s = 0; s < e; s++) {
-                    const uint32_t sr = s >> 2, src = (s & 3u) * 16u;
-                    const float mo = sr == 0 ? o[0] : sr == 1 ? o[1] : sr == 2 ? o[2] : o[3];
-                    const uint32_t mi = sr == 0 ? id4[0] : sr == 1 ? id4[1] : sr == 2 ? id4[2] : id4[3];
-                    const int mv = (sr == 0 ? v4[0] : sr == 1 ? v4[1] : sr == 2 ? v4[2] : v4[3]) ? 1 : 0;
-                    const float ds = __shfl(mo, (int)src, 64);
-                    const uint32_t is = (uint32_t)__shfl((int)mi, (int)src, 64);
-                    const int vs = __shfl(mv, (int)src, 64);
-                    if (vs && ds < best) {
+                // the host's sequential scan keeps the first strictly smaller
+                // entry: the minimum of (dist, list position) over the chunk,
+                // taken if it is below the running best
+                unsigned long long mk = ~0ull;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const unsigned long long kk =
+                        ((unsigned long long)float_key(o[r]) << 32) | (b + grp + 4u * (uint32_t)r);
+                    if (v4[r] && kk < mk) mk = kk;
+                }
+#pragma unroll
+                for (int s2 = 16; s2 <= 32; s2 <<= 1) {   // across the four groups
+                    const unsigned long long other = shfl_xor64(mk, s2);
+                    mk = other < mk ? other : mk;
+                }
+                if (mk != ~0ull) {
+                    const float ds = key_float((uint32_t)(mk >> 32));
+                    if (ds < best) {
                         best = ds;
-                        best_id = is;
+                        best_id = (uint32_t)__shfl((int)w, (int)(1u + (uint32_t)mk), 64);
                     }
                 }
             }
@@ -357,12 +375,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             }
             if (ok) key = kc;
         }
-        const uint32_t nv = (uint32_t)__popcll(__ballot(key != ~0ull));
+        const uint64_t am = __ballot(key != ~0ull);
+        const uint32_t nv = (uint32_t)__popcll(am);
         if (nv == 0) {   // uniform
             pick = o;
             continue;
         }
-        key = wave_sort64(key, lane, next_pow2_dev(nf));   // admitted keys now in lanes [0, nv)
+        // admitted keys to lanes [0, nv), then sorted over next_pow2(nv) lanes
+        if (key != ~0ull) s_ck[__popcll(am & lanemask_lt())] = key;
+        __syncthreads();
+        key = (lane < nv) ? s_ck[lane] : ~0ull;
+        key = wave_sort64(key, lane, next_pow2_dev(nv));
         // the next expansion takes the smaller of the best admitted candidate
         // and the first unexpanded entry after pick; start its row load
         const unsigned long long kbest = read_lane64(key, 0);
