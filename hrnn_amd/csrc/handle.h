@@ -123,6 +123,7 @@ struct SearchSel {
     const uint32_t *qcount = nullptr;
     hipStream_t stream = nullptr;
     uint32_t queue_word = kQueueWord;
+    bool queue_zeroed = false;   // an earlier kernel on the stream already zeroed the queue
 };
 
 void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint32_t *proxies,

@@ -228,14 +228,20 @@ __global__ void k_pick_m(const float *__restrict__ probs, uint32_t nq, const uin
 // queries that have no stage 2.  lists (optional, 2 nq u32): the escalated
 // queries are appended to lists[0..), the others to lists[nq..), counted in
 // list_cnt[0] and list_cnt[1] (zero before the launch; list order is
-// arbitrary).
+// arbitrary).  zero_a / zero_b (optional): words set to zero (the work queues
+// of the searches launched after this kernel).
 __global__ void k_pick_class(const float *__restrict__ probs, uint32_t nq,
                              const uint32_t *__restrict__ classes, float tau, uint32_t ef1,
                              uint32_t *__restrict__ prox, uint32_t mstride, uint32_t *__restrict__ cls,
                              uint32_t *__restrict__ ef_q, uint32_t *__restrict__ m_q,
                              uint32_t *__restrict__ exp2, uint32_t *__restrict__ lists,
-                             uint32_t *__restrict__ list_cnt) {
+                             uint32_t *__restrict__ list_cnt, uint32_t *__restrict__ zero_a,
+                             uint32_t *__restrict__ zero_b) {
     const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi == 0) {   // the work queues of the two searches that follow
+        if (zero_a) *zero_a = 0;
+        if (zero_b) *zero_b = 0;
+    }
     if (qi >= nq) return;
     uint32_t c = kOut - 1;
     if (probs) {
@@ -263,8 +269,17 @@ __global__ void k_pick_class(const float *__restrict__ probs, uint32_t nq,
     ef_q[qi] = ef > ef1 ? ef : 0u;
     m_q[qi] = m;
     if (lists) {   // escalated queries to lists[0..), the others to lists[nq..)
-        const uint32_t which = ef > ef1 ? 0u : 1u;
-        lists[(uint64_t)which * nq + atomicAdd(&list_cnt[which], 1u)] = qi;
+        // one atomic per wave and list (10k single-lane atomics on two words
+        // would serialise)
+        const bool esc = ef > ef1;
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint64_t me = __ballot(esc), mr = __ballot(!esc);
+        const uint64_t mine = esc ? me : mr;
+        const uint32_t leader = (uint32_t)__ffsll((long long)mine) - 1u;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&list_cnt[esc ? 0 : 1], (uint32_t)__popcll(mine));
+        base = (uint32_t)__shfl((int)base, (int)leader, 64);
+        lists[(uint64_t)(esc ? 0u : 1u) * nq + base + (uint32_t)__popcll(mine & lanemask_lt())] = qi;
     }
     if (exp2 && ef <= ef1) {   // no stage-2 beam for this query
         exp2[2 * (uint64_t)qi] = 0;
@@ -439,7 +454,8 @@ int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
                                nullptr, nq, probs, nullptr, nullptr);
         hipLaunchKernelGGL(k_pick_class, dim3(ceil_div(nq, 128)), dim3(128), 0, h->stream,
                            model_dev ? probs : nullptr, nq, classes_dev, tau, ef1, proxies_dev, m_max,
-                           cls_dev, ef_q, m_q, exp2, lists, list_cnt);
+                           cls_dev, ef_q, m_q, exp2, lists, list_cnt,
+                           h->counters.as<uint32_t>() + kQueueWord, h->counters.as<uint32_t>() + kQueue2Word);
         HIPC(hipGetLastError());
         // Phases B+C of the queries that stay at stage 1 run on the second
         // stream while stage 2 runs on the first; the escalated queries are
@@ -451,6 +467,7 @@ int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
         rest.qcount = list_cnt + 1;
         rest.stream = h->stream2;
         rest.queue_word = kQueue2Word;
+        rest.queue_zeroed = true;
         rknn_search_resident(h, q_dev, nq, proxies_dev, m_max, nullptr, k, K_prime, threshold, res_cnt_dev,
                              res_ids_dev, cap_per_q, work_dev, &rest);
         HIPC(hipEventRecord(h->ev_b, h->stream2));
@@ -466,6 +483,7 @@ int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
         SearchSel esc;
         esc.qlist = lists;
         esc.qcount = list_cnt;
+        esc.queue_zeroed = true;
         rknn_search_resident(h, q_dev, nq, proxies_dev, m_max, nullptr, k, K_prime, threshold, res_cnt_dev,
                              res_ids_dev, cap_per_q, work_dev, &esc);
     });

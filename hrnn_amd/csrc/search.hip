@@ -390,7 +390,7 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     }
     h->search_cand.reserve((uint64_t)slots * cand_cap * 8);
     uint32_t *queue = h->counters.as<uint32_t>() + so.queue_word;
-    HIPC(hipMemsetAsync(queue, 0, 4, st));
+    if (!so.queue_zeroed) HIPC(hipMemsetAsync(queue, 0, 4, st));
     // LDS table: the largest power-of-two slot count within the configured
     // bytes (u32 slots in presence mode, u64 in count mode), at least 256
     const uint32_t slot_bytes = count_mode ? 8u : 4u;
