@@ -11,7 +11,7 @@ using namespace hrnn;
 hrnn_dev::~hrnn_dev() {
     (void)hipSetDevice(device);
     DevBuf *bufs[] = {&x, &xb, &xh, &xu8, &norms, &knn_ids, &knn_d, &knn_cnt, &knn_new, &rk_col, &csr_off,
-                      &csr_ent, &search_mark, &search_cand, &l0, &up_off, &up_pool, &beam_vis,
+                      &csr_ent, &search_mark, &search_cand, &search_mark2, &search_cand2, &l0, &up_off, &up_pool, &beam_vis,
                       &s0, &s1, &s2, &s3, &s4, &s5, &s6, &s7, &s8, &nnd_aux, &counters};
     for (DevBuf *b : bufs) b->release();
     if (ev_a) (void)hipEventDestroy(ev_a);
@@ -198,6 +198,7 @@ static void set_base_common(hrnn_dev_t h, const float *x, uint64_t n, uint32_t d
     h->hnsw_valid = false;
     h->beam_vis.release();     // sized by n
     h->search_mark.release();
+    h->search_mark2.release();
 }
 
 int hrnn_dev_set_base(hrnn_dev_t h, const float *x, uint64_t n, uint32_t d) {
@@ -244,6 +245,7 @@ int hrnn_dev_append_base(hrnn_dev_t h, const float *x, uint64_t n_add) {
         h->hnsw_valid = false;
         h->beam_vis.release();
         h->search_mark.release();
+        h->search_mark2.release();
     });
 }
 
@@ -273,6 +275,7 @@ int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value) {
     *slot = value;
     // the per-workgroup private state is sized by the workgroup count
     h->search_mark.release();
+    h->search_mark2.release();
     h->beam_vis.release();
     return HRNN_OK;
 }

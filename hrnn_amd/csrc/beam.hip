@@ -35,6 +35,7 @@
 #include <type_traits>
 
 #include "handle.h"
+#include "mlp_core.h"
 
 namespace hrnn {
 
@@ -149,11 +150,10 @@ struct BeamLds {
 // kPerQueryEf: the stage-2 form of the two-stage learned query (opt.ef_q
 // gives each query's beam width; 0 skips the query).  A separate
 // instantiation so profiles tell the two stages apart.
-// Occupancy: 6 waves per SIMD for u8 storage, 5 otherwise (the register
-// budgets at which the compiler needs no spills).
+// Occupancy: 5 waves per SIMD (the register budget at which the compiler
+// needs no spills).
 template <class VT, bool kPerQueryEf>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(std::is_same<VT, uint8_t>::value ? 6 : 5)))
-void k_phase_a(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_phase_a(
     const VT *__restrict__ x, uint32_t d, uint32_t n, const uint32_t *__restrict__ l0,
     uint32_t l0_words, const uint32_t *__restrict__ up_off, const uint32_t *__restrict__ up_pool,
     uint64_t pool_words, uint32_t up_words, uint32_t entry, int maxlevel, const float *__restrict__ q,
@@ -175,10 +175,14 @@ void k_phase_a(
 
     const uint32_t lane = threadIdx.x;
     const uint32_t g = lane & 15u, grp = lane >> 4;
-    uint64_t qi = blockIdx.x;
+    uint32_t qi = blockIdx.x;   // uniform: kept in a scalar register
+    if (opt.mlp && blockIdx.x == 0 && threadIdx.x == 0) {   // the queues of the searches that follow
+        if (opt.zero_a) *opt.zero_a = 0;
+        if (opt.zero_b) *opt.zero_b = 0;
+    }
     if (opt.qlist) {   // a query subset: workgroup i takes qlist[i]
         if (blockIdx.x >= *opt.qcount) return;
-        qi = opt.qlist[blockIdx.x];
+        qi = uni(opt.qlist[blockIdx.x]);
     }
     if (kPerQueryEf) {   // per-query beam width; 0 leaves this query to another launch
         const uint32_t e = opt.ef_q[qi];
@@ -187,7 +191,7 @@ void k_phase_a(
     }
     bool q_u8 = true;
     for (uint32_t e = lane; e < d; e += 64) {
-        const float v = q[qi * d + e];
+        const float v = q[(uint64_t)qi * d + e];
         s_qv[e] = v;
         q_u8 &= (v == rintf(v)) && v >= 0.f && v <= 255.f;
     }
@@ -198,8 +202,8 @@ void k_phase_a(
         if (exact) {
             uint32_t part = 0;
             for (uint32_t w = lane; 4 * w < d; w += 64) {
-                const uint32_t b0 = (uint32_t)q[qi * d + 4 * w], b1 = (uint32_t)q[qi * d + 4 * w + 1];
-                const uint32_t b2 = (uint32_t)q[qi * d + 4 * w + 2], b3 = (uint32_t)q[qi * d + 4 * w + 3];
+                const uint32_t b0 = (uint32_t)q[(uint64_t)qi * d + 4 * w], b1 = (uint32_t)q[(uint64_t)qi * d + 4 * w + 1];
+                const uint32_t b2 = (uint32_t)q[(uint64_t)qi * d + 4 * w + 2], b3 = (uint32_t)q[(uint64_t)qi * d + 4 * w + 3];
                 s_q8[w] = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
                 part += b0 * b0 + b1 * b1 + b2 * b2 + b3 * b3;
             }
@@ -432,17 +436,15 @@ void k_phase_a(
         L = min(L + nv, ef);
         buf = nbuf;
     }
-    // ---- output: the m closest ----
+    // ---- output ----
     const unsigned long long *lk = s_kl + buf * ef_cap;
-    const uint32_t mq = opt.m_q ? min(opt.m_q[qi], m) : m;
-    for (uint32_t t = lane; t < m; t += 64)
-        proxies[qi * m + t] = (t < L && t < mq) ? (uint32_t)lk[t] : kInvalidId;
+    uint32_t mq = opt.m_q ? min(opt.m_q[qi], m) : m;
     if (expansions && lane == 0) {
-        expansions[2 * qi] = n_exp;
-        expansions[2 * qi + 1] = n_eval;
+        expansions[2 * (uint64_t)qi] = n_exp;
+        expansions[2 * (uint64_t)qi + 1] = n_eval;
     }
     if (raw) {   // per-query statistics for the learned m (kRawWords floats)
-        float *r = raw + qi * kRawWords;
+        float *r = raw + (uint64_t)qi * kRawWords;
         if (lane < 16) r[2 + lane] = key_float((uint32_t)(lk[lane < L ? lane : L - 1] >> 32));
         if (lane == 0) {
             r[0] = d_entry;
@@ -451,7 +453,7 @@ void k_phase_a(
             r[19] = (float)L;
         }
     }
-    if (opt.feats) {
+    if (opt.feats || opt.mlp) {
         // The learned-m features (the same definitions as the host's
         // HRNN::queryFeatures over this beam's statistics):
         //   0 log1p d(q,entry)   1 log1p d(q,start)   2..17 log1p of the 16
@@ -505,8 +507,36 @@ void k_phase_a(
         else if (t == 30) f = ld0 - rk1;
         else f = log1pf((float)n_exp);
         if (opt.mu) f = (f - opt.mu[t]) / opt.sd[t];
-        if (lane < (uint32_t)kFeatWords) opt.feats[qi * kFeatWords + lane] = f;
+        if (opt.feats && lane < (uint32_t)kFeatWords) opt.feats[(uint64_t)qi * kFeatWords + lane] = f;
+        if (opt.mlp) {
+            // The learned early stop's decision for this query: the MLP over
+            // the (standardised) features, the cheapest class whose cumulative
+            // probability reaches tau, its beam width for stage 2 (0: the
+            // stage-1 proxies serve) and proxy count; the query joins the
+            // escalated list or the list of the rest.
+            __shared__ float m_x[kIn], m_z1[kH], m_h1[kH], m_z2[kH], m_h2[kH], m_lg[kOut], m_p[kOut];
+            if (lane < (uint32_t)kIn) m_x[lane] = f;
+            __syncthreads();
+            mlp_forward_wave(opt.mlp, m_x, m_z1, m_h1, m_z2, m_h2, m_lg, m_p, lane);
+            const uint32_t c = uni(pick_class_of(m_p, opt.tau));
+            const uint32_t cef = opt.classes[c];
+            mq = min(max(opt.classes[kOut + c], 1u), m);
+            if (lane == 0) {
+                const bool esc = cef > opt.ef1;
+                opt.cls[qi] = c;
+                opt.ef_q_out[qi] = esc ? cef : 0u;
+                opt.m_q_out[qi] = mq;
+                if (opt.exp2 && !esc) {
+                    opt.exp2[2 * (uint64_t)qi] = 0;
+                    opt.exp2[2 * (uint64_t)qi + 1] = 0;
+                }
+                opt.lists[(uint64_t)(esc ? 0u : 1u) * opt.nq + atomicAdd(&opt.list_cnt[esc ? 0 : 1], 1u)] =
+                    qi;
+            }
+        }
     }
+    for (uint32_t t = lane; t < m; t += 64)
+        proxies[(uint64_t)qi * m + t] = (t < L && t < mq) ? (uint32_t)lk[t] : kInvalidId;
     __syncthreads();
     if (lane == 0 && s_bad) atomicOr(err, 1u);
 }
@@ -530,7 +560,10 @@ void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint
     REQUIRE(h->d >= 1 && h->d <= 4096, HRNN_E_ARG, "proxies: d must be in [1, 4096]");
     REQUIRE(q_dev && proxies_dev, HRNN_E_ARG, "proxies: null argument");
     BeamOpt o = opt ? *opt : BeamOpt{};
-    if (o.feats) {
+    REQUIRE(!o.mlp || (o.classes && o.cls && o.ef_q_out && o.m_q_out && o.lists && o.list_cnt && o.nq == nq &&
+                       o.mu && o.sd),
+            HRNN_E_ARG, "proxies: incomplete learned-stop arguments");
+    if (o.feats || o.mlp) {
         REQUIRE(h->K > 0 && o.feat_k >= 1 && o.feat_k <= h->K, HRNN_E_ARG,
                 "proxies: feature k must be in [1, K]");
         REQUIRE((o.mu == nullptr) == (o.sd == nullptr), HRNN_E_ARG, "proxies: give mu and sd together");

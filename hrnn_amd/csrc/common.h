@@ -163,13 +163,32 @@ __device__ __forceinline__ float dist_row<uint8_t>(const float *__restrict__ qv,
     return r;
 }
 
+// Sum over each row of 16 lanes, every lane of the row receiving it, by DPP
+// moves (no LDS round trip): swap within quads, then with the other quad of
+// the 8, then with the other 8.  The order of the additions is fixed but not
+// the reference's, so it serves sums that are exact in any order.
+__device__ __forceinline__ int row16_sum_i(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);    // quad_perm [1,0,3,2]
+    v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);    // quad_perm [2,3,0,1]
+    v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);   // row_half_mirror
+    v += __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);   // row_mirror
+    return v;
+}
+__device__ __forceinline__ float row16_sum_f(float v) {
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false));
+    return v;
+}
+
 typedef __bf16 ex_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float ex_f32x4 __attribute__((ext_vector_type(4)));
 
 // Squared L2 from the LDS query to base row id by a 16-lane group, for data
 // whose every sum is an exact integer (values in [0, 255], d <= 258): the
 // order of the additions does not matter, so each lane reads 16 contiguous
-// bytes per step and the group reduces with a butterfly.  Every lane of the
+// bytes per step and the group reduces with DPP moves.  Every lane of the
 // group returns the sum.  d must be a multiple of 8 (BF16) or 4 (FP32).
 template <class VT>
 __device__ __forceinline__ float dist_row_exact(const float *__restrict__ qv, const VT *__restrict__ base,
@@ -188,11 +207,7 @@ __device__ __forceinline__ float dist_row_exact<__hip_bfloat16>(const float *__r
             acc = __builtin_fmaf(t, t, acc);
         }
     }
-    acc += __shfl_xor(acc, 8, 16);
-    acc += __shfl_xor(acc, 4, 16);
-    acc += __shfl_xor(acc, 2, 16);
-    acc += __shfl_xor(acc, 1, 16);
-    return acc;
+    return row16_sum_f(acc);
 }
 template <>
 __device__ __forceinline__ float dist_row_exact<float>(const float *__restrict__ qv,
@@ -208,11 +223,7 @@ __device__ __forceinline__ float dist_row_exact<float>(const float *__restrict__
             acc = __builtin_fmaf(t, t, acc);
         }
     }
-    acc += __shfl_xor(acc, 8, 16);
-    acc += __shfl_xor(acc, 4, 16);
-    acc += __shfl_xor(acc, 2, 16);
-    acc += __shfl_xor(acc, 1, 16);
-    return acc;
+    return row16_sum_f(acc);
 }
 
 // Row vector type of the exact-sum distance: 16 bytes per lane and step.
@@ -256,14 +267,7 @@ __device__ __forceinline__ void dist4_exact(const float *__restrict__ qv, const 
         }
     }
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        float a = acc[r];
-        a += __shfl_xor(a, 8, 16);
-        a += __shfl_xor(a, 4, 16);
-        a += __shfl_xor(a, 2, 16);
-        a += __shfl_xor(a, 1, 16);
-        out[r] = a;
-    }
+    for (int r = 0; r < 4; r++) out[r] = row16_sum_f(acc[r]);
 }
 
 // Raw buffer resource over `bytes` bytes from base (hardware range check:
@@ -306,14 +310,7 @@ __device__ __forceinline__ void dist4_exact_buf(const float *__restrict__ qv, __
         }
     }
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        float a = acc[r];
-        a += __shfl_xor(a, 8, 16);
-        a += __shfl_xor(a, 4, 16);
-        a += __shfl_xor(a, 2, 16);
-        a += __shfl_xor(a, 1, 16);
-        out[r] = a;
-    }
+    for (int r = 0; r < 4; r++) out[r] = row16_sum_f(acc[r]);
 }
 
 typedef uint32_t ex_u32x2 __attribute__((ext_vector_type(2)));
@@ -347,14 +344,7 @@ __device__ __forceinline__ void dist4_u8_buf(const uint32_t *__restrict__ q8, ui
         }
     }
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        int a = acc[r];
-        a += __shfl_xor(a, 8, 16);
-        a += __shfl_xor(a, 4, 16);
-        a += __shfl_xor(a, 2, 16);
-        a += __shfl_xor(a, 1, 16);
-        out[r] = (float)((int)qq + a);
-    }
+    for (int r = 0; r < 4; r++) out[r] = (float)((int)qq + row16_sum_i(acc[r]));
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt() {

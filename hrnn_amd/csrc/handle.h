@@ -54,6 +54,7 @@ struct hrnn_dev {
     // RkNN search private state: per-workgroup visited bitmaps / u16 counts
     // (all-zero between launches) and candidate buffers
     hrnn::DevBuf search_mark, search_cand;
+    hrnn::DevBuf search_mark2, search_cand2;   // a second set for an overlapping search
 
     // HNSW graph for phase A on the GPU: layer-0 links [n x (maxM0+1)] (count
     // word + ids), per-node offset into the upper-layer pool (blocks of maxM+1
@@ -124,6 +125,8 @@ struct SearchSel {
     hipStream_t stream = nullptr;
     uint32_t queue_word = kQueueWord;
     bool queue_zeroed = false;   // an earlier kernel on the stream already zeroed the queue
+    int buffer_set = 0;          // 1: the second fallback buffers (a search overlapping another)
+    uint32_t max_slots = 0;      // cap on the persistent workgroups (0: the tuned count)
 };
 
 void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint32_t *proxies,
@@ -150,6 +153,18 @@ struct BeamOpt {
     const float *sd = nullptr;
     const uint32_t *qlist = nullptr;  // workgroup i takes query qlist[i], i < *qcount
     const uint32_t *qcount = nullptr;
+    // learned early stop fused into the stage-1 beam (all of these, or mlp
+    // null): MLP parameters (features standardised with mu/sd above), the 16
+    // classes' ef then m, tau, the stage-1 width; per query the class, the
+    // stage-2 ef (0: none) and m, stage-2 expansion counters zeroed for the
+    // rest, and the escalated / other query lists (lists[0..), lists[nq..))
+    // with their counts; zero_a / zero_b are set to zero (search queues)
+    const float *mlp = nullptr;
+    const uint32_t *classes = nullptr;
+    float tau = 0.f;
+    uint32_t ef1 = 0, nq = 0;
+    uint32_t *cls = nullptr, *ef_q_out = nullptr, *m_q_out = nullptr, *exp2 = nullptr;
+    uint32_t *lists = nullptr, *list_cnt = nullptr, *zero_a = nullptr, *zero_b = nullptr;
 };
 void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint32_t ef,
                  uint32_t *proxies_dev, uint32_t *expansions_dev, float *raw_dev = nullptr,

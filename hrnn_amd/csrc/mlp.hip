@@ -22,13 +22,9 @@
 #include <vector>
 
 #include "handle.h"
+#include "mlp_core.h"
 
 namespace hrnn {
-
-constexpr int kIn = 32, kH = 64, kOut = 16;
-constexpr int oW1 = 0, oB1 = oW1 + kH * kIn, oW2 = oB1 + kH, oB2 = oW2 + kH * kH,
-              oW3 = oB2 + kH, oB3 = oW3 + kOut * kH, kParams = oB3 + kOut;
-static_assert(kParams == 7312, "parameter layout");
 
 // One 64-lane block per sample.  Writes probabilities (and, when grad != null,
 // the per-sample gradient row grad[s][kParams] and loss[s]).
@@ -46,37 +42,8 @@ __global__ __launch_bounds__(64) void k_mlp_fwd_bwd(const float *__restrict__ P,
     const uint32_t row = idx ? idx[s] : s;
     if (j < kIn) x[j] = X[(uint64_t)row * kIn + j];
     __syncthreads();
-    {
-        float a = P[oB1 + j];
-        for (int i = 0; i < kIn; i++) a = __builtin_fmaf(P[oW1 + j * kIn + i], x[i], a);
-        z1[j] = a;
-        h1[j] = a > 0.f ? a : 0.f;
-    }
-    __syncthreads();
-    {
-        float a = P[oB2 + j];
-        for (int i = 0; i < kH; i++) a = __builtin_fmaf(P[oW2 + j * kH + i], h1[i], a);
-        z2[j] = a;
-        h2[j] = a > 0.f ? a : 0.f;
-    }
-    __syncthreads();
-    if (j < kOut) {
-        float a = P[oB3 + j];
-        for (int i = 0; i < kH; i++) a = __builtin_fmaf(P[oW3 + j * kH + i], h2[i], a);
-        lg[j] = a;
-    }
-    __syncthreads();
-    if (j == 0) {
-        float mx = lg[0];
-        for (int c = 1; c < kOut; c++) mx = fmaxf(mx, lg[c]);
-        float sum = 0.f;
-        for (int c = 0; c < kOut; c++) {
-            dl[c] = expf(lg[c] - mx);
-            sum += dl[c];
-        }
-        for (int c = 0; c < kOut; c++) dl[c] /= sum;   // dl holds p for now
-        if (loss && labels) loss[s] = -logf(fmaxf(dl[labels[row]], 1e-30f));
-    }
+    mlp_forward_wave(P, x, z1, h1, z2, h2, lg, dl, j);   // dl holds p for now
+    if (j == 0 && loss && labels) loss[s] = -logf(fmaxf(dl[labels[row]], 1e-30f));
     __syncthreads();
     if (probs && j < kOut) probs[(uint64_t)s * kOut + j] = dl[j];
     if (!grad) return;
@@ -245,21 +212,7 @@ __global__ void k_pick_class(const float *__restrict__ probs, uint32_t nq,
     if (qi >= nq) return;
     uint32_t c = kOut - 1;
     if (probs) {
-        const float *p = probs + (uint64_t)qi * kOut;
-        if (tau <= 0.f) {
-            c = 0;
-            for (uint32_t j = 1; j < (uint32_t)kOut; j++)
-                if (p[j] > p[c]) c = j;
-        } else {
-            float acc = 0.f;
-            for (uint32_t j = 0; j < (uint32_t)kOut; j++) {
-                acc += p[j];
-                if (acc >= tau) {
-                    c = j;
-                    break;
-                }
-            }
-        }
+        c = pick_class_of(probs + (uint64_t)qi * kOut, tau);
         cls[qi] = c;
     } else {
         c = min(cls[qi], (uint32_t)kOut - 1u);
@@ -438,28 +391,38 @@ int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
             HIPC(hipEventCreateWithFlags(&h->ev_b, hipEventDisableTiming));
         }
         HIPC(hipMemsetAsync(list_cnt, 0, 8, h->stream));
-        // stage 1: every query at ef1, with the standardised features when a
-        // model is given
+        // stage 1: every query at ef1; with a model, the features, the MLP and
+        // the class decision run inside the beam kernel's epilogue
+        uint32_t *exp1 = expansions_dev, *exp2 = expansions_dev ? expansions_dev + 2ull * nq : nullptr;
         BeamOpt o1;
         if (model_dev) {
-            o1.feats = feats;
             o1.feat_k = k;
             o1.mu = model_dev + kParams;
             o1.sd = model_dev + kParams + kIn;
+            o1.mlp = model_dev;
+            o1.classes = classes_dev;
+            o1.tau = tau;
+            o1.ef1 = ef1;
+            o1.nq = nq;
+            o1.cls = cls_dev;
+            o1.ef_q_out = ef_q;
+            o1.m_q_out = m_q;
+            o1.exp2 = exp2;
+            o1.lists = lists;
+            o1.list_cnt = list_cnt;
+            o1.zero_a = h->counters.as<uint32_t>() + kQueueWord;
+            o1.zero_b = h->counters.as<uint32_t>() + kQueue2Word;
         }
-        uint32_t *exp1 = expansions_dev, *exp2 = expansions_dev ? expansions_dev + 2ull * nq : nullptr;
         beam_launch(h, q_dev, nq, m_max, ef1, proxies_dev, exp1, nullptr, &o1);
-        if (model_dev)
-            hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(nq), dim3(64), 0, h->stream, model_dev, feats, nullptr,
-                               nullptr, nq, probs, nullptr, nullptr);
-        hipLaunchKernelGGL(k_pick_class, dim3(ceil_div(nq, 128)), dim3(128), 0, h->stream,
-                           model_dev ? probs : nullptr, nq, classes_dev, tau, ef1, proxies_dev, m_max,
-                           cls_dev, ef_q, m_q, exp2, lists, list_cnt,
-                           h->counters.as<uint32_t>() + kQueueWord, h->counters.as<uint32_t>() + kQueue2Word);
+        if (!model_dev)   // forced classes
+            hipLaunchKernelGGL(k_pick_class, dim3(ceil_div(nq, 128)), dim3(128), 0, h->stream, nullptr, nq,
+                               classes_dev, tau, ef1, proxies_dev, m_max, cls_dev, ef_q, m_q, exp2, lists,
+                               list_cnt, h->counters.as<uint32_t>() + kQueueWord,
+                               h->counters.as<uint32_t>() + kQueue2Word);
         HIPC(hipGetLastError());
         // Phases B+C of the queries that stay at stage 1 run on the second
-        // stream while stage 2 runs on the first; the escalated queries are
-        // searched once both are done.
+        // stream while stage 2 and then the escalated queries' phases B+C run
+        // on the first; the call ends when both are done.
         HIPC(hipEventRecord(h->ev_a, h->stream));
         HIPC(hipStreamWaitEvent(h->stream2, h->ev_a, 0));
         SearchSel rest;
@@ -479,13 +442,17 @@ int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
             o2.qcount = list_cnt;
             beam_launch(h, q_dev, nq, m_max, ef_max, proxies_dev, exp2, nullptr, &o2);
         }
-        HIPC(hipStreamWaitEvent(h->stream, h->ev_b, 0));
+        // the escalated queries' search overlaps the other one too: its own
+        // fallback buffers and a few workgroups
         SearchSel esc;
         esc.qlist = lists;
         esc.qcount = list_cnt;
         esc.queue_zeroed = true;
+        esc.buffer_set = 1;
+        esc.max_slots = (uint32_t)std::max(1, h->n_cu);
         rknn_search_resident(h, q_dev, nq, proxies_dev, m_max, nullptr, k, K_prime, threshold, res_cnt_dev,
                              res_ids_dev, cap_per_q, work_dev, &esc);
+        HIPC(hipStreamWaitEvent(h->stream, h->ev_b, 0));
     });
 }
 

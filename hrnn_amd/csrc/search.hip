@@ -376,7 +376,10 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     uint64_t cand_cap = std::max<uint64_t>(1, (uint64_t)m * std::max<uint64_t>(h->max_deg, 1));
     cand_cap = std::min<uint64_t>(cand_cap, n);
     uint32_t slots = (uint32_t)std::min<uint64_t>(nq, (uint64_t)h->n_cu * h->tune_search_wg_per_cu);
+    if (so.max_slots) slots = std::min(slots, so.max_slots);
     slots = std::max<uint32_t>(slots, 1);
+    DevBuf &bmark = so.buffer_set ? h->search_mark2 : h->search_mark;
+    DevBuf &bcand = so.buffer_set ? h->search_cand2 : h->search_cand;
     // bound the fallback's private state to ~4 GB
     const uint64_t per_slot = words * 4 + cand_cap * 8;
     const uint64_t max_slots = std::max<uint64_t>(1, (4ull << 30) / per_slot);
@@ -384,11 +387,11 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     // private arrays (kept all-zero between launches by the kernel itself) and
     // candidate buffers are owned by the search alone
     const uint64_t mark_bytes = (uint64_t)slots * words * 4;
-    if (h->search_mark.bytes < mark_bytes) {
-        h->search_mark.reserve(mark_bytes);
-        HIPC(hipMemsetAsync(h->search_mark.p, 0, h->search_mark.bytes, st));
+    if (bmark.bytes < mark_bytes) {
+        bmark.reserve(mark_bytes);
+        HIPC(hipMemsetAsync(bmark.p, 0, bmark.bytes, st));
     }
-    h->search_cand.reserve((uint64_t)slots * cand_cap * 8);
+    bcand.reserve((uint64_t)slots * cand_cap * 8);
     uint32_t *queue = h->counters.as<uint32_t>() + so.queue_word;
     if (!so.queue_zeroed) HIPC(hipMemsetAsync(queue, 0, 4, st));
     // LDS table: the largest power-of-two slot count within the configured
@@ -418,8 +421,8 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
         }
         rk = h->rk_col.as<float>();
     }
-    uint32_t *mark = h->search_mark.as<uint32_t>();
-    uint32_t *cand = h->search_cand.as<uint32_t>();
+    uint32_t *mark = bmark.as<uint32_t>();
+    uint32_t *cand = bcand.as<uint32_t>();
     // Candidate vectors are read from the u8 mirror (HRNN_PREC_U8) or the BF16
     // mirror when every base value is exactly representable in it (same
     // distances, a quarter or half of the bytes).
