@@ -152,6 +152,13 @@ def main() -> int:
     # ---- groundtruth with the GPU brute force; KNN-graph quality sample -----------------
     h = ix.device_handle()
     dev = Device.borrow(h, n, d, K)
+    # the CSR kernels alone: rebuild on the device from the same graph (the
+    # index's build above also downloads the CSR into the host index)
+    dev.sync()
+    t = time.perf_counter()
+    csr_info = dev.build_rknn_csr()
+    dev.sync()
+    csr_dev_s = time.perf_counter() - t
     t = time.time()
     kth = dev.bf_kth(k, "bf16")
     kth_s = time.time() - t
@@ -639,6 +646,18 @@ def main() -> int:
                             "knn_recall_at10_sample": round(knn_rec10, 4),
                             "per_iter": [{k2: (round(v, 2) if isinstance(v, float) else v)
                                           for k2, v in s.items()} for s in st]},
+        # the other rows' kernels on this run: reverse CSR (HBM-bound, SURVEY's
+        # 12 N K + 16 (N + 1) bytes), exact r_k brute force (MFMA, 2 N^2 d
+        # flops), NNDescent (distance evaluations, 2 d flops each)
+        "build_rooflines": {
+            "csr": {"s": round(csr_dev_s, 4), "entries": int(csr_info.n_entries),
+                    "alg_bytes": int(12 * n * K + 16 * (n + 1)),
+                    "gbs": round((12 * n * K + 16 * (n + 1)) / csr_dev_s / 1e9, 1),
+                    "frac_hbm": round((12 * n * K + 16 * (n + 1)) / csr_dev_s / 1e9 / HBM_PEAK_GBS, 4)},
+            "bruteforce_rk": {"s": round(kth_s, 3), "tflops": round(2.0 * n * n * d / kth_s / 1e12, 1),
+                              "frac_bf16_mfma": round(2.0 * n * n * d / kth_s / 1e12 / 2500.0, 4)},
+            "nndescent": {"s": round(nnd_s, 3), "dists": int(sum(v["dists"] for v in st)),
+                          "tflops": round(2.0 * d * sum(v["dists"] for v in st) / nnd_s / 1e12, 1)}},
         "stages_s": {"hnsw_host": round(hnsw_s, 2), "nndescent_gpu": round(nnd_s, 3),
                      "rknn_csr_gpu": round(csr_s, 3), "rk_bruteforce_gpu": round(kth_s, 3),
                      "groundtruth_gpu": round(gt_s, 3)},

@@ -1,5 +1,5 @@
-// csr.hip -- (b) reverse-KNN CSR: integer histogram, exclusive scan, scatter and
-// per-list ordering, bit-exact with the canonical (rank, src) order.
+// csr.hip -- (b) reverse-KNN CSR: integer histogram, exclusive scan and stable
+// scatter, bit-exact with the canonical (rank, src) order.
 //
 // Reference: HRNN::buildRKNNG (include/hnswlib/hrnn.h:1869-1909) and
 // buildRKNNG_fill (hrnn.h:1912-1935).  The reference fills lists in (src, rank)
@@ -9,7 +9,9 @@
 // HBM layout: knn_ids u32[n*K] (row i = node i's ranked list), knn_cnt u32[n];
 // csr_off u64[n+1]; csr_ent u32 or u64[E] with E = sum(cnt).
 //
-// Kernels (one wave of 64 lanes per source row for the row passes):
+// u32 entries (node_bits + rank_bits <= 32): the stable radix sort below
+// (k_rs_hist, k_rs_scatter, k_rs_offsets).  u64 entries, the fallback:
+// (one wave of 64 lanes per source row for the row passes):
 //   k_degree    deg[dst] += 1 for every (src, rank)            reads 4*E B
 //   k_scan_*    offsets = exclusive scan of deg (u64)           reads/writes 12*n B
 //   k_scatter   tmp[off[dst] + atomic slot] = (rank<<nb)|src   reads 4*E, writes 4*E
@@ -252,6 +254,204 @@ static void csr_fill(hrnn_dev_t h, uint32_t node_bits, uint32_t rank_bits) {
     }
 }
 
+// ---- u32 entries: stable LSD radix sort on (dst, rank) ------------------------
+//
+// Edges enumerated row-major (src ascending, then rank) are sorted stably by
+// key = (dst << rank_bits) | rank, carrying src: each list then holds its
+// entries in (rank, src) order with no per-list sort and no global atomics.
+// 8-bit digits; each pass = a per-tile LDS histogram, one exclusive scan over
+// [digit][tile] counts, and a scatter that ranks equal digits stably inside
+// the tile (wave ballots over the 8 digit bits, per-wave counts in LDS).  The
+// first pass reads the KNN rows (entries past a row's count are dropped), the
+// last writes the CSR entries (src << rank_bits) | rank and the sorted keys,
+// from which the offsets follow.
+constexpr int kRsThreads = 256;
+constexpr int kRsRounds = 16;                         // rounds of 256 elements per tile
+constexpr int kRsTile = kRsThreads * kRsRounds;       // 4096 elements
+constexpr uint32_t kRsBins = 256;
+
+// Element t of the row-major enumeration of rows [row0, ...): its key and
+// value; false for an entry past its row's count (or past the last row).
+__device__ __forceinline__ bool rs_edge(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ cnt,
+                                        uint64_t n, uint32_t K, uint32_t rb, uint64_t row0, uint32_t t,
+                                        uint32_t *key, uint32_t *val) {
+    const uint64_t row = row0 + t / K;
+    const uint32_t j = t % K;
+    if (row >= n || j >= cnt[row]) return false;
+    *key = (ids[row * K + j] << rb) | j;
+    *val = (uint32_t)row;
+    return true;
+}
+
+// Per-tile digit histogram, written digit-major: hist[digit * tiles + tile].
+// kRows: tiles of `rows` KNN rows (first pass); otherwise tiles of kRsTile
+// consecutive elements of keys[0..E).
+template <bool kRows>
+__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t *__restrict__ ids,
+                                                        const uint32_t *__restrict__ cnt, uint64_t n,
+                                                        uint32_t K, uint32_t rb, uint32_t rows,
+                                                        const uint32_t *__restrict__ keys, uint64_t E,
+                                                        uint32_t shift, uint32_t tiles,
+                                                        uint32_t *__restrict__ hist) {
+    __shared__ uint32_t h[kRsBins];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t tile = blockIdx.x;
+    if (kRows) {
+        const uint64_t row0 = (uint64_t)tile * rows;
+        const uint32_t span = rows * K;
+        for (uint32_t t = threadIdx.x; t < span; t += kRsThreads) {
+            uint32_t key, val;
+            if (rs_edge(ids, cnt, n, K, rb, row0, t, &key, &val)) atomicAdd(&h[(key >> shift) & 255u], 1u);
+        }
+    } else {
+        const uint64_t b = (uint64_t)tile * kRsTile;
+        for (uint32_t t = threadIdx.x; t < (uint32_t)kRsTile; t += kRsThreads) {
+            const uint64_t i = b + t;
+            if (i < E) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+        }
+    }
+    __syncthreads();
+    hist[(uint64_t)threadIdx.x * tiles + tile] = h[threadIdx.x];
+}
+
+// Stable scatter of one pass.  goff: exclusive scan of hist (digit-major).
+// kLast: write CSR entries to out_ent and sorted keys to out_key; otherwise
+// keys and values to out_key / out_val.
+template <bool kRows, bool kLast>
+__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
+    const uint32_t *__restrict__ ids, const uint32_t *__restrict__ cnt, uint64_t n, uint32_t K, uint32_t rb,
+    uint32_t rows, const uint32_t *__restrict__ in_key, const uint32_t *__restrict__ in_val, uint64_t E,
+    uint32_t shift, uint32_t tiles, const uint64_t *__restrict__ goff, uint32_t *__restrict__ out_key,
+    uint32_t *__restrict__ out_val, uint32_t *__restrict__ out_ent) {
+    __shared__ uint64_t run[kRsBins];                    // next output position per digit
+    __shared__ uint32_t wcnt[2][kRsThreads / 64][kRsBins]; // per-wave digit counts (double buffered)
+    const uint32_t tile = blockIdx.x, lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    run[threadIdx.x] = goff[(uint64_t)threadIdx.x * tiles + tile];
+    const uint32_t rmask = (1u << rb) - 1u;
+    const uint32_t span = kRows ? rows * K : (uint32_t)kRsTile;
+    const uint64_t row0 = (uint64_t)tile * rows, b = (uint64_t)tile * kRsTile;
+    uint32_t par = 0;
+    for (uint32_t r0 = 0; r0 < span; r0 += kRsThreads, par ^= 1u) {
+#pragma unroll
+        for (int w = 0; w < kRsThreads / 64; w++) wcnt[par][w][threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t t = r0 + threadIdx.x;
+        uint32_t key = 0, val = 0;
+        bool valid = false;
+        if (t < span) {
+            if (kRows) {
+                valid = rs_edge(ids, cnt, n, K, rb, row0, t, &key, &val);
+            } else if (b + t < E) {
+                valid = true;
+                key = in_key[b + t];
+                val = in_val[b + t];
+            }
+        }
+        const uint32_t dg = (key >> shift) & 255u;
+        // lanes of this wave holding the same digit, in input order
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+            const uint64_t bb = __ballot((dg >> bit) & 1u);
+            peers &= ((dg >> bit) & 1u) ? bb : ~bb;
+        }
+        const uint32_t below = (uint32_t)__popcll(peers & lanemask_lt());
+        if (valid && below == 0) wcnt[par][wave][dg] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint64_t pos = run[dg] + below;
+            for (uint32_t w = 0; w < wave; w++) pos += wcnt[par][w][dg];
+            if (kLast) {
+                out_ent[pos] = (val << rb) | (key & rmask);
+                out_key[pos] = key;
+            } else {
+                out_key[pos] = key;
+                out_val[pos] = val;
+            }
+        }
+        __syncthreads();   // every position of this round is read before run moves on
+        uint32_t add = 0;
+#pragma unroll
+        for (int w = 0; w < kRsThreads / 64; w++) add += wcnt[par][w][threadIdx.x];
+        run[threadIdx.x] += add;
+    }
+}
+
+// Offsets from the sorted keys: off[v] = first position whose dst >= v.
+__global__ void k_rs_offsets(const uint32_t *__restrict__ keys, uint64_t E, uint32_t rb, uint64_t n,
+                             uint64_t *__restrict__ off) {
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > E) return;
+    const uint64_t hi = (p < E) ? (uint64_t)(keys[p] >> rb) : n;          // dst at p (n past the end)
+    const uint64_t lo = (p > 0) ? (uint64_t)(keys[p - 1] >> rb) + 1 : 0;  // first dst not before p
+    for (uint64_t v = lo; v <= hi && v <= n; v++) off[v] = p;
+}
+
+static void csr_radix(hrnn_dev_t h, uint32_t nb, uint32_t rb) {
+    const uint64_t n = h->n;
+    const uint32_t K = h->K;
+    const uint32_t rows = std::max<uint32_t>(1, kRsTile / K);
+    const uint32_t tiles0 = ceil_div(n, rows);
+    const uint64_t Emax = n * K;
+    // key/value ping-pong buffers and the histogram
+    h->s1.reserve(Emax * 4 * 2 + 64);
+    h->s2.reserve(Emax * 4 * 2 + 64);
+    uint32_t *ka = h->s1.as<uint32_t>(), *va = ka + Emax;
+    uint32_t *kb = h->s2.as<uint32_t>(), *vb = kb + Emax;
+    const uint32_t tiles_max = std::max<uint32_t>(tiles0, ceil_div(Emax, kRsTile));
+    h->s3.reserve((uint64_t)kRsBins * tiles_max * 4 + 64);
+    h->s5.reserve((uint64_t)kRsBins * tiles_max * 8 + 64);
+    h->s4.reserve((uint64_t)scan_block_count((uint64_t)kRsBins * tiles_max) * 8 + 64);
+    uint32_t *hist = h->s3.as<uint32_t>();
+    uint64_t *goff = h->s5.as<uint64_t>();
+    uint64_t *bsum = h->s4.as<uint64_t>();
+    uint64_t *total = h->counters.as<uint64_t>() + 16;
+    const uint32_t key_bits = nb + rb;
+    const uint32_t passes = std::max<uint32_t>(1, (key_bits + 7) / 8);
+    h->csr_ent.reserve(std::max<uint64_t>(Emax, 1) * 4);
+    const uint32_t *ids = h->knn_ids.as<uint32_t>(), *cn = h->knn_cnt.as<uint32_t>();
+    uint64_t E = 0;
+    const uint32_t *ik = nullptr, *iv = nullptr;
+    for (uint32_t pass = 0; pass < passes; pass++) {
+        const uint32_t shift = 8 * pass;
+        const bool first = pass == 0, last = pass + 1 == passes;
+        const uint32_t tiles = first ? tiles0 : ceil_div(E, kRsTile);
+        if (first)
+            hipLaunchKernelGGL(k_rs_hist<true>, dim3(tiles), dim3(kRsThreads), 0, h->stream, ids, cn, n, K, rb,
+                               rows, nullptr, 0, shift, tiles, hist);
+        else
+            hipLaunchKernelGGL(k_rs_hist<false>, dim3(tiles), dim3(kRsThreads), 0, h->stream, nullptr, nullptr, n,
+                               K, rb, rows, ik, E, shift, tiles, hist);
+        HIPC(hipGetLastError());
+        scan_u32(h, hist, (uint64_t)kRsBins * tiles, goff, bsum, total);
+        if (first) {   // the number of edges
+            d2h(h, &E, total, 8);
+            h->n_ent = E;
+            if (E == 0) {
+                HIPC(hipMemsetAsync(h->csr_off.p, 0, (n + 1) * 8, h->stream));
+                return;
+            }
+        }
+        uint32_t *okey = (pass & 1) ? kb : ka, *oval = (pass & 1) ? vb : va;
+#define RS_SCATTER(ROWS, LAST)                                                                          \
+    hipLaunchKernelGGL((k_rs_scatter<ROWS, LAST>), dim3(tiles), dim3(kRsThreads), 0, h->stream, ids, cn, n, \
+                       K, rb, rows, ik, iv, E, shift, tiles, goff, okey, oval, h->csr_ent.as<uint32_t>())
+        if (first && last) RS_SCATTER(true, true);
+        else if (first) RS_SCATTER(true, false);
+        else if (last) RS_SCATTER(false, true);
+        else RS_SCATTER(false, false);
+#undef RS_SCATTER
+        HIPC(hipGetLastError());
+        ik = okey;
+        iv = oval;
+    }
+    // offsets from the sorted keys (left in ik by the last pass)
+    hipLaunchKernelGGL(k_rs_offsets, dim3(ceil_div(E + 1, 256)), dim3(256), 0, h->stream, ik, E, rb, n,
+                       h->csr_off.as<uint64_t>());
+    HIPC(hipGetLastError());
+}
+
 __global__ void k_max_len(const uint64_t *__restrict__ off, uint64_t n, unsigned long long *__restrict__ out) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long v = (i < n) ? (unsigned long long)(off[i + 1] - off[i]) : 0ull;
@@ -284,6 +484,13 @@ void csr_build(hrnn_dev_t h) {
     h->rank_bits = rb;
 
     h->csr_off.reserve((n + 1) * 8);
+    if (!h->wide) {   // u32 entries: the radix-sort build
+        csr_radix(h, nb, rb);
+        HIPC(hipStreamSynchronize(h->stream));
+        h->max_deg = max_list_len(h);
+        h->csr_valid = true;
+        return;
+    }
     h->s0.reserve(n * 4 + 64);
     uint32_t *deg = h->s0.as<uint32_t>();
     HIPC(hipMemsetAsync(deg, 0, n * 4, h->stream));
