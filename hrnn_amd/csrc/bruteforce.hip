@@ -58,20 +58,20 @@ __device__ float l2_ref16_thread(const float *__restrict__ a, const float *__res
     return r;
 }
 
-template <int C>
+template <int C, int R = kRB>
 struct TopState {
-    float top_d[kRB][C];
-    uint32_t top_i[kRB][C];
-    uint32_t top_n[kRB];
-    float thr[kRB];
-    float st_d[kRB][kHalf];
-    uint32_t st_i[kRB][kHalf];
-    uint32_t st_n[kRB];
+    float top_d[R][C];
+    uint32_t top_i[R][C];
+    uint32_t top_n[R];
+    float thr[R];
+    float st_d[R][kHalf];
+    uint32_t st_i[R][kHalf];
+    uint32_t st_n[R];
 };
 
-template <int C>
-__device__ void top_init(TopState<C> &s) {
-    for (int r = threadIdx.x; r < kRB; r += blockDim.x) {
+template <int C, int R>
+__device__ void top_init(TopState<C, R> &s) {
+    for (int r = threadIdx.x; r < R; r += blockDim.x) {
         s.top_n[r] = 0;
         s.st_n[r] = 0;
         s.thr[r] = FLT_MAX;
@@ -79,10 +79,10 @@ __device__ void top_init(TopState<C> &s) {
 }
 
 // rows with staged candidates merge them into their sorted top list
-template <int C>
-__device__ void top_merge(TopState<C> &s) {
+template <int C, int R>
+__device__ void top_merge(TopState<C, R> &s) {
     __syncthreads();
-    for (int r = threadIdx.x; r < kRB; r += blockDim.x) {
+    for (int r = threadIdx.x; r < R; r += blockDim.x) {
         uint32_t ns = s.st_n[r];
         if (!ns) continue;
         uint32_t n = s.top_n[r];
@@ -107,8 +107,8 @@ __device__ void top_merge(TopState<C> &s) {
     __syncthreads();
 }
 
-template <int C>
-__device__ __forceinline__ void top_offer(TopState<C> &s, int r, float dv, uint32_t col) {
+template <int C, int R>
+__device__ __forceinline__ void top_offer(TopState<C, R> &s, int r, float dv, uint32_t col) {
     // columns arrive in increasing id, so an equal distance never displaces a kept id
     if (dv < s.thr[r]) {
         uint32_t t = atomicAdd(&s.st_n[r], 1u);
@@ -117,9 +117,9 @@ __device__ __forceinline__ void top_offer(TopState<C> &s, int r, float dv, uint3
     }
 }
 
-template <int C>
-__device__ void top_store(TopState<C> &s, uint64_t row0, uint32_t nq, uint32_t *cand_i, uint32_t *cand_n) {
-    for (int t = threadIdx.x; t < kRB * C; t += blockDim.x) {
+template <int C, int R>
+__device__ void top_store(TopState<C, R> &s, uint64_t row0, uint32_t nq, uint32_t *cand_i, uint32_t *cand_n) {
+    for (int t = threadIdx.x; t < R * C; t += blockDim.x) {
         int r = t / C, j = t % C;
         uint64_t row = row0 + r;
         if (row >= nq) continue;
@@ -147,7 +147,9 @@ struct Mfma<__half> {
 
 // MFMA candidate pass.  KS = d / 32 k-steps (A fragments live in registers).
 // mode 0: per-row top-C;  mode 1: RkNN hits d2 <= kth[col] (exact data only).
-template <class T, int KS, int C, int MODE>
+// FR 16-row fragments per wave: a block covers 64 FR rows, so each staged
+// base tile feeds FR times as many MFMAs.
+template <class T, int KS, int C, int MODE, int FR>
 __global__ __launch_bounds__(256) void k_bf_mfma(
     const T *__restrict__ qa, const float *__restrict__ qn, uint32_t nq,
     const T *__restrict__ xb, const float *__restrict__ xn, uint64_t n, uint32_t d,
@@ -155,36 +157,41 @@ __global__ __launch_bounds__(256) void k_bf_mfma(
     const float *__restrict__ kth, uint32_t *__restrict__ hit_cnt, uint32_t *__restrict__ hit_i,
     float *__restrict__ hit_d, uint32_t cap) {
     typedef typename Mfma<T>::vec vec;
-    __shared__ TopState<C> st;
+    constexpr int RB = 64 * FR;
+    __shared__ TopState<C, RB> st;
     __shared__ __attribute__((aligned(16))) T bs[kCT * kCHP];
     __shared__ float s_xn[kCT];
-    __shared__ float s_qn[kRB];
+    __shared__ float s_qn[RB];
 
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x / 64;
-    const uint64_t row0 = (uint64_t)blockIdx.x * kRB;
+    const uint64_t row0 = (uint64_t)blockIdx.x * RB;
     if (MODE == 0) top_init(st);
-    for (int r = threadIdx.x; r < kRB; r += blockDim.x) s_qn[r] = (row0 + r < nq) ? qn[row0 + r] : 0.f;
+    for (int r = threadIdx.x; r < RB; r += blockDim.x) s_qn[r] = (row0 + r < nq) ? qn[row0 + r] : 0.f;
 
-    // A fragments: row = row0 + 16w + lane%16, dims ks*32 + 8*(lane/16) .. +8
-    vec afrag[KS];
-    {
-        uint64_t row = row0 + 16 * w + (lane & 15u);
+    // A fragments: fragment f of wave w holds row row0 + 16 (FR w + f) +
+    // lane%16, dims ks*32 + 8*(lane/16) .. +8
+    vec afrag[FR][KS];
+#pragma unroll
+    for (int f = 0; f < FR; f++) {
+        uint64_t row = row0 + 16 * (FR * w + f) + (lane & 15u);
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
             if (row < nq) {
-                afrag[ks] = *reinterpret_cast<const vec *>(qa + row * d + ks * 32 + 8 * (lane >> 4));
+                afrag[f][ks] = *reinterpret_cast<const vec *>(qa + row * d + ks * 32 + 8 * (lane >> 4));
             } else {
 #pragma unroll
-                for (int e = 0; e < 8; e++) afrag[ks][e] = 0;
+                for (int e = 0; e < 8; e++) afrag[f][ks][e] = 0;
             }
         }
     }
     __syncthreads();
 
     for (uint64_t c0 = 0; c0 < n; c0 += kCT) {
-        f32x4_t acc[4];
+        f32x4_t acc[FR][4];
 #pragma unroll
-        for (int cb = 0; cb < 4; cb++) acc[cb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        for (int f = 0; f < FR; f++)
+#pragma unroll
+            for (int cb = 0; cb < 4; cb++) acc[f][cb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
         __syncthreads();   // previous tile's epilogue is done with s_xn
         if (threadIdx.x < kCT) s_xn[threadIdx.x] = (c0 + threadIdx.x < n) ? xn[c0 + threadIdx.x] : 0.f;
 #pragma unroll
@@ -211,12 +218,13 @@ __global__ __launch_bounds__(256) void k_bf_mfma(
 #pragma unroll
                     for (int cb = 0; cb < 4; cb++) {
                         vec b = *reinterpret_cast<const vec *>(&bs[(16 * cb + (lane & 15u)) * kCHP + kl * 32 + 8 * (lane >> 4)]);
-                        acc[cb] = Mfma<T>::run(afrag[ch + kl], b, acc[cb]);
+#pragma unroll
+                        for (int f = 0; f < FR; f++) acc[f][cb] = Mfma<T>::run(afrag[f][ch + kl], b, acc[f][cb]);
                     }
                 }
             }
         }
-        // epilogue: lane holds rows 16w + 4*(lane/16) + i, column 16cb + lane%16
+        // epilogue: lane holds rows 16 (FR w + f) + 4*(lane/16) + i, column 16cb + lane%16
         for (int half = 0; half < 2; half++) {
 #pragma unroll
             for (int cbh = 0; cbh < 2; cbh++) {
@@ -224,10 +232,12 @@ __global__ __launch_bounds__(256) void k_bf_mfma(
                 uint32_t cl = 16 * cb + (lane & 15u);
                 uint64_t col = c0 + cl;
 #pragma unroll
+                for (int f = 0; f < FR; f++)
+#pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    int rl = 16 * w + 4 * (lane >> 4) + i;
+                    int rl = 16 * (FR * w + f) + 4 * (lane >> 4) + i;
                     if (col < n && row0 + rl < nq) {
-                        float dv = s_qn[rl] + s_xn[cl] - 2.f * acc[cb][i];
+                        float dv = s_qn[rl] + s_xn[cl] - 2.f * acc[f][cb][i];
                         dv = dv > 0.f ? dv : 0.f;
                         if (MODE == 0) {
                             top_offer(st, rl, dv, (uint32_t)col);
@@ -402,12 +412,15 @@ template <class T, int C, int MODE>
 static void launch_mfma(hrnn_dev_t h, const T *qa, const float *qn, uint32_t nq, const T *xb,
                         uint32_t *ci, uint32_t *cn, const float *kth, uint32_t *hc, uint32_t *hi,
                         float *hd, uint32_t cap) {
-    unsigned grid = ceil_div(nq, kRB);
+    // two row fragments per wave for the top-32 pass (128 rows per block: half
+    // the base-tile traffic per MFMA); one otherwise (LDS of the top state)
+    constexpr int FR = (MODE == 0 && C <= 32) ? 2 : 1;
+    unsigned grid = ceil_div(nq, 64 * FR);
     uint32_t KS = h->d / 32;
 #define L(ks)                                                                                     \
     case ks:                                                                                      \
-        hipLaunchKernelGGL((k_bf_mfma<T, ks, C, MODE>), dim3(grid), dim3(256), 0, h->stream, qa, qn, nq, \
-                           xb, h->norms.as<float>(), h->n, h->d, ci, cn, kth, hc, hi, hd, cap);     \
+        hipLaunchKernelGGL((k_bf_mfma<T, ks, C, MODE, FR>), dim3(grid), dim3(256), 0, h->stream, qa, qn, \
+                           nq, xb, h->norms.as<float>(), h->n, h->d, ci, cn, kth, hc, hi, hd, cap);  \
         break;
     switch (KS) {
         L(1) L(2) L(3) L(4) L(5) L(6) L(8) L(12) L(16) L(24) L(30) L(32)
