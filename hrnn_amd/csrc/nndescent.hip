@@ -18,10 +18,12 @@
 //       = new (upper triangle) and old samples.  d2 = |a|^2 + |b|^2 - 2 a.b.
 //       A pair (a, b), a != b, is proposed to a's list when d2 < worst(a) and
 //       to b's list when d2 < worst(b), worst taken at the iteration start.
-//       The join runs twice: a counting pass (proposals per target), an
-//       exclusive scan into a proposal CSR, then a filling pass that writes
-//       every proposal at its target's offset.  Nothing is dropped unless the
-//       whole iteration exceeds the proposal buffer (reported as dropped).
+//       The join runs twice: a counting pass (proposals per target, summed
+//       per sample slot in LDS, one global atomic per slot), an exclusive
+//       scan into a proposal CSR, then a filling pass in which each slot
+//       reserves its range with one atomic and writes its proposals there.
+//       Nothing is dropped unless the whole iteration exceeds the proposal
+//       buffer (reported as dropped).
 //   k_nnd_merge: per node, proposals in chunks of 2048: sort by (d2, id), drop
 //       duplicates and ids already listed, merge with the sorted list
 //       (existing entries first on equal distance, as the reference's strict-<
@@ -143,28 +145,11 @@ __device__ float l2_ref16_lane(const float *__restrict__ a, const float *__restr
     return r;
 }
 
-// Counting pass: one more proposal for `target`.  Filling pass: the proposal
-// goes to the next free slot of target's range of the proposal CSR.
-template <bool kFill>
-__device__ __forceinline__ void nnd_propose(uint32_t target, uint32_t cand, float dv,
-                                            uint32_t *__restrict__ prop_cnt,
-                                            const uint64_t *__restrict__ prop_off,
-                                            uint32_t *__restrict__ prop_id, float *__restrict__ prop_d,
-                                            uint64_t cap, uint32_t &dropped) {
-    if (!kFill) {
-        atomicAdd(&prop_cnt[target], 1u);
-        return;
-    }
-    uint64_t pos = prop_off[target] + atomicAdd(&prop_cnt[target], 1u);   // prop_cnt = fill cursor here
-    if (pos < cap) {
-        prop_id[pos] = cand;
-        prop_d[pos] = dv;
-    } else {
-        dropped++;
-    }
-}
-
-template <bool kFill>
+// kFull: the whole sample (T rows x d dims) is staged in LDS at once and each
+// wave evaluates its 16x16 Gram blocks one after another (no accumulator
+// array); otherwise d is staged in 64-dim chunks and each wave keeps the
+// accumulators of all its blocks across the chunks.
+template <bool kFill, bool kFull>
 __global__ __launch_bounds__(256) void k_nnd_join(
     const __hip_bfloat16 *__restrict__ xb, const float *__restrict__ xn, const float *__restrict__ x32,
     uint32_t d, uint64_t n, uint32_t S, uint32_t R, uint32_t RB, const uint32_t *__restrict__ fn,
@@ -176,9 +161,11 @@ __global__ __launch_bounds__(256) void k_nnd_join(
     float *__restrict__ prop_d, uint64_t cap, int exact, unsigned long long *__restrict__ stats) {
     __shared__ uint32_t sid[kJMax];
     __shared__ float sthr[kJMax], snorm[kJMax];
-    __shared__ __attribute__((aligned(16))) __hip_bfloat16 vec[kJMax * kJDP];
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    __hip_bfloat16 *vec = reinterpret_cast<__hip_bfloat16 *>(s_dyn);   // kFull: Tpad x (d + 8), else kJMax x kJDP
     __shared__ uint32_t rsel[2][kJRev];   // reverse offers (new, old), sorted by source id
     __shared__ uint8_t blk_r[kJMaxB * 4], blk_c[kJMaxB * 4];
+    __shared__ uint32_t s_pcnt[kJMax], s_pcur[kJMax], s_pbase[kJMax];   // per-slot proposals
     __shared__ uint32_t s_a, s_b, s_c, s_nn, s_T, s_nb;
 
     const uint64_t node = blockIdx.x;
@@ -248,76 +235,190 @@ __global__ __launch_bounds__(256) void k_nnd_join(
     const uint32_t NB = s_nb;
     const uint32_t Tpad = CB * 16;
 
-    f32x4_n acc[kJMaxB];
-#pragma unroll
-    for (int j = 0; j < kJMaxB; j++) acc[j] = f32x4_n{0.f, 0.f, 0.f, 0.f};
-
-    for (uint32_t ch = 0; ch < d; ch += kJDC) {
-        const uint32_t dims = (d - ch) < (uint32_t)kJDC ? (d - ch) : (uint32_t)kJDC;   // 32 or 64
-        __syncthreads();
-        const uint32_t parts = dims / 8;
+    const uint32_t dP = d + 8;   // kFull row stride
+    f32x4_n acc[kFull ? 1 : kJMaxB];
+    if constexpr (kFull) {
+        const uint32_t parts = d / 8;
         for (uint32_t u = threadIdx.x; u < Tpad * parts; u += blockDim.x) {
-            uint32_t t = u / parts, part = u % parts;
+            const uint32_t t = u / parts, part = u % parts;
             bf16x8_n v;
             if (t < T) {
-                v = *reinterpret_cast<const bf16x8_n *>(xb + (uint64_t)sid[t] * d + ch + part * 8);
+                v = *reinterpret_cast<const bf16x8_n *>(xb + (uint64_t)sid[t] * d + part * 8);
             } else {
 #pragma unroll
                 for (int e = 0; e < 8; e++) v[e] = 0;
             }
-            *reinterpret_cast<bf16x8_n *>(&vec[t * kJDP + part * 8]) = v;
+            *reinterpret_cast<bf16x8_n *>(&vec[t * dP + part * 8]) = v;
         }
         __syncthreads();
+    } else {
 #pragma unroll
-        for (int j = 0; j < kJMaxB; j++) {
-            uint32_t b = w + 4 * j;
-            if (b < NB) {
-                uint32_t rb = blk_r[b], cb = blk_c[b];
-                for (uint32_t kl = 0; kl < dims / 32; kl++) {
-                    bf16x8_n av = *reinterpret_cast<const bf16x8_n *>(
-                        &vec[(rb * 16 + (lane & 15u)) * kJDP + kl * 32 + 8 * (lane >> 4)]);
-                    bf16x8_n bv = *reinterpret_cast<const bf16x8_n *>(
-                        &vec[(cb * 16 + (lane & 15u)) * kJDP + kl * 32 + 8 * (lane >> 4)]);
-                    acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[j], 0, 0, 0);
+        for (int j = 0; j < kJMaxB; j++) acc[j] = f32x4_n{0.f, 0.f, 0.f, 0.f};
+
+        for (uint32_t ch = 0; ch < d; ch += kJDC) {
+            const uint32_t dims = (d - ch) < (uint32_t)kJDC ? (d - ch) : (uint32_t)kJDC;   // 32 or 64
+            __syncthreads();
+            const uint32_t parts = dims / 8;
+            for (uint32_t u = threadIdx.x; u < Tpad * parts; u += blockDim.x) {
+                uint32_t t = u / parts, part = u % parts;
+                bf16x8_n v;
+                if (t < T) {
+                    v = *reinterpret_cast<const bf16x8_n *>(xb + (uint64_t)sid[t] * d + ch + part * 8);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; e++) v[e] = 0;
+                }
+                *reinterpret_cast<bf16x8_n *>(&vec[t * kJDP + part * 8]) = v;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kJMaxB; j++) {
+                uint32_t b = w + 4 * j;
+                if (b < NB) {
+                    uint32_t rb = blk_r[b], cb = blk_c[b];
+                    for (uint32_t kl = 0; kl < dims / 32; kl++) {
+                        bf16x8_n av = *reinterpret_cast<const bf16x8_n *>(
+                            &vec[(rb * 16 + (lane & 15u)) * kJDP + kl * 32 + 8 * (lane >> 4)]);
+                        bf16x8_n bv = *reinterpret_cast<const bf16x8_n *>(
+                            &vec[(cb * 16 + (lane & 15u)) * kJDP + kl * 32 + 8 * (lane >> 4)]);
+                        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[j], 0, 0, 0);
+                    }
                 }
             }
         }
     }
 
+    // Proposals go to sample slots (target = sid[slot]).  In the MFMA output
+    // layout the 16 lanes of a lane group share the row slot and the 4 lane
+    // groups x 4 rows of a lane share the column slot, so proposals are summed
+    // over those lanes first and each slot's LDS counter takes one add per
+    // group; after the pass each slot takes one global atomic for its whole
+    // count (a node's ~10^4 proposals reach at most 4S targets).  The filling
+    // pass walks the pairs again and writes each proposal into the range its
+    // slot reserved.  Both passes derive the same pa/pb from the same Gram
+    // values and thresholds, so the ranges are filled exactly.
     uint32_t ndist = 0, nprop = 0, dropped = 0;
+    const uint32_t grp = lane >> 4, gl = lane & 15u;
+    // pa/pb/dv of the 4 pairs a lane holds in Gram block (rb, cb), then on_block
+    auto block_props = [&](uint32_t rb, uint32_t cb, const f32x4_n &g, auto &&on_block) {
+        const uint32_t col = cb * 16 + gl;
+        bool pa[4], pb[4];
+        float dv[4];
 #pragma unroll
-    for (int j = 0; j < kJMaxB; j++) {
-        uint32_t b = w + 4 * j;
-        if (b < NB) {
-            uint32_t rb = blk_r[b], cb = blk_c[b];
-            uint32_t col = cb * 16 + (lane & 15u);
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                uint32_t r = rb * 16 + 4 * (lane >> 4) + i;
-                if (r < nn && col > r && col < T) {
-                    uint32_t ia = sid[r], ib = sid[col];
-                    if (ia != ib) {
-                        ndist++;
-                        float dv = snorm[r] + snorm[col] - 2.f * acc[j][i];
-                        dv = dv > 0.f ? dv : 0.f;
-                        bool pa = dv < sthr[r], pb = dv < sthr[col];
-                        if ((pa || pb) && !exact) {
-                            dv = l2_ref16_lane(x32 + (uint64_t)ia * d, x32 + (uint64_t)ib * d, d);
-                            pa = dv < sthr[r];
-                            pb = dv < sthr[col];
-                        }
-                        if (pa) {
-                            nprop++;
-                            nnd_propose<kFill>(ia, ib, dv, prop_cnt, prop_off, prop_id, prop_d, cap, dropped);
-                        }
-                        if (pb) {
-                            nprop++;
-                            nnd_propose<kFill>(ib, ia, dv, prop_cnt, prop_off, prop_id, prop_d, cap, dropped);
-                        }
+        for (int i = 0; i < 4; i++) {
+            const uint32_t r = rb * 16 + 4 * grp + i;
+            pa[i] = pb[i] = false;
+            dv[i] = 0.f;
+            if (r < nn && col > r && col < T) {
+                const uint32_t ia = sid[r], ib = sid[col];
+                if (ia != ib) {
+                    if (!kFill) ndist++;
+                    float v = snorm[r] + snorm[col] - 2.f * g[i];
+                    v = v > 0.f ? v : 0.f;
+                    bool qa = v < sthr[r], qb = v < sthr[col];
+                    if ((qa || qb) && !exact) {
+                        v = l2_ref16_lane(x32 + (uint64_t)ia * d, x32 + (uint64_t)ib * d, d);
+                        qa = v < sthr[r];
+                        qb = v < sthr[col];
                     }
+                    pa[i] = qa;
+                    pb[i] = qb;
+                    dv[i] = v;
                 }
             }
         }
+        on_block(rb * 16 + 4 * grp, col, pa, pb, dv);
+    };
+    auto each_block = [&](auto &&on_block) {
+        if constexpr (kFull) {
+            for (uint32_t b = w; b < NB; b += 4) {   // uniform over the wave
+                const uint32_t rb = blk_r[b], cb = blk_c[b];
+                f32x4_n g = {0.f, 0.f, 0.f, 0.f};
+                const __hip_bfloat16 *ra = &vec[(rb * 16 + gl) * dP + 8 * grp];
+                const __hip_bfloat16 *rc = &vec[(cb * 16 + gl) * dP + 8 * grp];
+                for (uint32_t kl = 0; kl < d; kl += 32) {
+                    const bf16x8_n av = *reinterpret_cast<const bf16x8_n *>(ra + kl);
+                    const bf16x8_n bv = *reinterpret_cast<const bf16x8_n *>(rc + kl);
+                    g = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, g, 0, 0, 0);
+                }
+                block_props(rb, cb, g, on_block);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kJMaxB; j++) {
+                const uint32_t b = w + 4 * j;
+                if (b < NB) block_props(blk_r[b], blk_c[b], acc[j], on_block);   // uniform over the wave
+            }
+        }
+    };
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
+        s_pcnt[t] = 0;
+        s_pcur[t] = 0;
+    }
+    __syncthreads();
+    each_block([&](uint32_t r0, uint32_t col, const bool *pa, const bool *pb, const float *) {
+        uint32_t cb_cnt = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint32_t gm = (uint32_t)(__ballot(pa[i]) >> (16 * grp)) & 0xFFFFu;
+            if (gl == 0 && gm) atomicAdd(&s_pcnt[r0 + i], (uint32_t)__popc(gm));
+            cb_cnt += pb[i] ? 1u : 0u;
+            if (!kFill) nprop += (pa[i] ? 1u : 0u) + (pb[i] ? 1u : 0u);
+        }
+        cb_cnt += __shfl_xor(cb_cnt, 16, 64);
+        cb_cnt += __shfl_xor(cb_cnt, 32, 64);
+        if (grp == 0 && cb_cnt) atomicAdd(&s_pcnt[col], cb_cnt);
+    });
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
+        const uint32_t c = s_pcnt[t];
+        if (c) {
+            const uint32_t base = atomicAdd(&prop_cnt[sid[t]], c);   // count, or fill cursor
+            if (kFill) s_pbase[t] = base;
+        }
+    }
+    if (kFill) {
+        __syncthreads();
+        auto put = [&](uint32_t target, uint64_t at, uint32_t cand, float v) {
+            const uint64_t pos = prop_off[target] + at;
+            if (pos < cap) {
+                prop_id[pos] = cand;
+                prop_d[pos] = v;
+            } else {
+                dropped++;
+            }
+        };
+        each_block([&](uint32_t r0, uint32_t col, const bool *pa, const bool *pb, const float *dv) {
+            const uint32_t ib = col < T ? sid[col] : 0u;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint32_t gm = (uint32_t)(__ballot(pa[i]) >> (16 * grp)) & 0xFFFFu;
+                uint32_t base = 0;
+                if (gl == 0 && gm) base = atomicAdd(&s_pcur[r0 + i], (uint32_t)__popc(gm));
+                base = __shfl(base, (int)(16 * grp), 64);
+                if (pa[i]) {
+                    const uint32_t r = r0 + i;
+                    put(sid[r], (uint64_t)s_pbase[r] + base + __popc(gm & ((1u << gl) - 1u)), ib, dv[i]);
+                }
+            }
+            uint32_t mine = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) mine += pb[i] ? 1u : 0u;
+            const uint32_t c0 = __shfl(mine, (int)gl, 64), c1 = __shfl(mine, (int)gl + 16, 64);
+            const uint32_t c2 = __shfl(mine, (int)gl + 32, 64), c3 = __shfl(mine, (int)gl + 48, 64);
+            const uint32_t total = c0 + c1 + c2 + c3;
+            uint32_t base = 0;
+            if (grp == 0 && total) base = atomicAdd(&s_pcur[col], total);
+            base = __shfl(base, (int)gl, 64);
+            uint32_t k = base + (grp > 0 ? c0 : 0u) + (grp > 1 ? c1 : 0u) + (grp > 2 ? c2 : 0u);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                if (pb[i]) {
+                    put(ib, (uint64_t)s_pbase[col] + k, sid[r0 + i], dv[i]);
+                    k++;
+                }
+            }
+        });
     }
     // wave-level reduction of the counters (distances and proposals from the
     // counting pass, drops from the filling pass)
@@ -507,6 +608,13 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
     const uint32_t R = (uint32_t)S;
     const uint32_t RB = kJRev;   // reverse offers kept per node and kind
     const int exact = (h->base_u8 && h->d <= 128) ? 1 : 0;
+    // Whole-sample staging when 4S rows of d bf16 dims fit in 56 KiB of LDS
+    // (d <= 128 at S = 50); wider vectors go through 64-dim chunks.
+    const uint64_t full_lds = (uint64_t)((4 * S + 15) / 16 * 16) * (h->d + 8) * 2;
+    const bool full = full_lds <= 56 * 1024;
+    const uint32_t join_lds = full ? (uint32_t)full_lds : (uint32_t)(kJMax * kJDP * 2);
+    auto join_count = full ? k_nnd_join<false, true> : k_nnd_join<false, false>;
+    auto join_fill = full ? k_nnd_join<true, true> : k_nnd_join<true, false>;
 
     // Proposal buffer: per_node_cap entries per node (default max(16384, 32K)),
     // at most 60% of the free device memory (8 B per proposal).
@@ -556,12 +664,12 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
         hipLaunchKernelGGL(k_nnd_sample, dim3(ceil_div(n, 4)), dim3(256), 0, h->stream, ids, kd, isnew,
                            cnt, n, K, (uint32_t)S, RB, worst, fn, fn_cnt, fo, fo_cnt, rn, rn_cnt, ro,
                            ro_cnt);
-        hipLaunchKernelGGL(k_nnd_join<false>, dim3((unsigned)n), dim3(256), 0, h->stream,
+        hipLaunchKernelGGL(join_count, dim3((unsigned)n), dim3(256), join_lds, h->stream,
                            h->xb.as<__hip_bfloat16>(), h->norms.as<float>(), h->x.as<float>(), h->d,
                            n, (uint32_t)S, R, RB, fn, fn_cnt, fo, fo_cnt, rn, rn_cnt, ro, ro_cnt, worst,
                            prop_cnt, prop_off, prop_id, prop_d, cap, exact, stats);
         scan_u32(h, prop_cnt, n, prop_off, bsum, ptotal);
-        hipLaunchKernelGGL(k_nnd_join<true>, dim3((unsigned)n), dim3(256), 0, h->stream,
+        hipLaunchKernelGGL(join_fill, dim3((unsigned)n), dim3(256), join_lds, h->stream,
                            h->xb.as<__hip_bfloat16>(), h->norms.as<float>(), h->x.as<float>(), h->d,
                            n, (uint32_t)S, R, RB, fn, fn_cnt, fo, fo_cnt, rn, rn_cnt, ro, ro_cnt, worst,
                            prop_fill, prop_off, prop_id, prop_d, cap, exact, stats);

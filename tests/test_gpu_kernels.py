@@ -340,16 +340,7 @@ def test_nndescent_recall_parity(small_data, small_ref):
     assert st[0]["dists"] > 0 and st[0]["updates"] > 0
 
 
-def test_nndescent_first_iteration_exact(small_data, small_ref):
-    """One iteration from the same seeds: the GPU evaluates exactly the pairs
-    of the oracle's single-threaded join (same forward and reverse samples)
-    and every node ends with exactly the oracle's sorted distance list (the
-    top-K of list U proposals is unique as distances; ids may differ only
-    among equal distances).  Nothing is dropped."""
-    x, _ = small_data
-    r, seed = small_ref
-    sid, sd, sc = seed
-    K, S = r.K, 12
+def _first_iteration_matches_oracle(x, sid, sd, sc, K, S):
     n = x.shape[0]
     # reverse offers per node in this iteration (all seed entries are new)
     worst = np.where(sc >= K, sd[np.arange(n), np.maximum(sc.astype(np.int64) - 1, 0)], np.inf)
@@ -372,6 +363,33 @@ def test_nndescent_first_iteration_exact(small_data, small_ref):
     assert np.array_equal(np.where(m, gd, 0), np.where(m, od, 0))
     same_ids = float((np.where(m, gid, 0) == np.where(m, oid, 0)).all(1).mean())
     print("nodes with identical id lists:", same_ids)
+
+
+def test_nndescent_first_iteration_exact(small_data, small_ref):
+    """One iteration from the same seeds: the GPU evaluates exactly the pairs
+    of the oracle's single-threaded join (same forward and reverse samples)
+    and every node ends with exactly the oracle's sorted distance list (the
+    top-K of list U proposals is unique as distances; ids may differ only
+    among equal distances).  Nothing is dropped.  At d=128, S=12 the whole
+    sample is staged in LDS at once."""
+    x, _ = small_data
+    r, seed = small_ref
+    sid, sd, sc = seed
+    _first_iteration_matches_oracle(x, sid, sd, sc, r.K, 12)
+
+
+def test_nndescent_first_iteration_exact_chunked_dims():
+    """As above at d=256, S=32: 4S rows of 256 bf16 dims exceed the LDS kept
+    for whole-sample staging, so the join stages 64-dim chunks and keeps one
+    accumulator per Gram block; u8 data at d > 128 also takes the FP32
+    re-evaluation of every candidate pair."""
+    if not oracle.ref_available():
+        pytest.skip("compiled reference (oracle/_ref) not available")
+    x = mixture(1000, d=256, seed=41)
+    r = oracle.RefIndex(1000, 256, M=8, efc=40, K=40)
+    r.add_points(x, nthreads=1)
+    sid, sd, sc = r.get_knng()
+    _first_iteration_matches_oracle(x, sid, sd, sc, 40, 32)
 
 
 def test_nndescent_graph_capture_matches_eager(small_data, small_ref, monkeypatch):
