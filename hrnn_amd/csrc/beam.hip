@@ -61,21 +61,24 @@ __device__ __forceinline__ bool vis_insert(uint32_t *tab, uint32_t log_slots, ui
     return true;
 }
 
-// Distances of four rows per 16-lane group (row r valid iff val[r]; val is
-// uniform within the group).  Exact-sum data issues the four row reads
-// together; otherwise the reference's 16-partial order, one row at a time.
-template <class VT>
+// Distances of R rows per 16-lane group (4 R rows per wave; row r valid iff
+// val[r], val uniform within the group).  Exact-sum data issues the R row
+// reads together; otherwise the reference's 16-partial order, one row at a
+// time.  R > 4 only for u8 storage.
+template <class VT, int R = 4>
 __device__ __forceinline__ void beam_dist4(bool exact, const float *qv, const uint32_t *q8, uint32_t qq,
                                            const VT *x, __amdgpu_buffer_rsrc_t xrs, const uint32_t *id,
                                            const bool *val, uint32_t d, uint32_t g, float *o) {
     if (exact) {   // exact-sum data; the host allows it only for a base below 4 GiB
-        if constexpr (std::is_same<VT, uint8_t>::value)
-            dist4_u8_buf(q8, qq, xrs, id, d, g, o);
-        else
+        if constexpr (std::is_same<VT, uint8_t>::value) {
+            distR_u8_buf<R>(q8, qq, xrs, id, d, g, o);
+        } else {
+            static_assert(R == 4, "wide row groups need u8 storage");
             dist4_exact_buf<VT>(qv, xrs, id, d * (uint32_t)sizeof(VT), d, g, o);
+        }
     } else {
 #pragma unroll
-        for (int r = 0; r < 4; r++) o[r] = val[r] ? dist_row<VT>(qv, x, id[r], d, g) : FLT_MAX;
+        for (int r = 0; r < R; r++) o[r] = val[r] ? dist_row<VT>(qv, x, id[r], d, g) : FLT_MAX;
     }
 }
 
@@ -347,22 +350,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             pick = o;
             continue;
         }
-        // distances of the fresh neighbours, 16 per pass (4 per 16-lane group)
-        for (uint32_t b = 0; b < nf; b += 16) {
-            uint32_t id4[4];
-            bool v4[4];
+        // distances of the fresh neighbours, 4 kRows per pass (kRows per
+        // 16-lane group; u8 rows take 8 at once, so a full layer-0 row of 32
+        // needs one round trip)
+        constexpr int kRows = std::is_same<VT, uint8_t>::value ? 8 : 4;
+        for (uint32_t b = 0; b < nf; b += 4 * kRows) {
+            uint32_t idr[kRows];
+            bool vr[kRows];
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
+            for (int r = 0; r < kRows; r++) {
                 const uint32_t t = b + grp + 4u * (uint32_t)r;
-                v4[r] = t < nf;
-                id4[r] = s_ni[v4[r] ? t : b];
+                vr[r] = t < nf;
+                idr[r] = s_ni[vr[r] ? t : b];
             }
-            float o4[4];
-            beam_dist4<VT>(exact, s_qv, s_q8, qq, x, xrs, id4, v4, d, g, o4);
+            float orr[kRows];
+            beam_dist4<VT, kRows>(exact, s_qv, s_q8, qq, x, xrs, idr, vr, d, g, orr);
             if (g == 0) {
 #pragma unroll
-                for (int r = 0; r < 4; r++)
-                    if (v4[r]) s_nd[b + grp + 4u * (uint32_t)r] = o4[r];
+                for (int r = 0; r < kRows; r++)
+                    if (vr[r]) s_nd[b + grp + 4u * (uint32_t)r] = orr[r];
             }
         }
         __syncthreads();

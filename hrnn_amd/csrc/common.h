@@ -315,36 +315,47 @@ __device__ __forceinline__ void dist4_exact_buf(const float *__restrict__ qv, __
 
 typedef uint32_t ex_u32x2 __attribute__((ext_vector_type(2)));
 
-// Squared L2 from a u8 query to four u8 rows (8-bit data, exact): each
-// 16-lane group reads 8 bytes per lane and step through a buffer resource and
-// forms sum (q - x)^2 = sum q^2 + sum x^2 - 2 sum q x with integer dot
-// products (v_dot4_u32_u8); every term is an exact integer and, for d <= 258,
-// so is the FP32 result.  q8 holds the query packed 4 bytes per word (LDS),
-// qq = sum q^2.  d must be a multiple of 8; rows are d bytes apart.
-__device__ __forceinline__ void dist4_u8_buf(const uint32_t *__restrict__ q8, uint32_t qq,
+// Squared L2 from a u8 query to R u8 rows (8-bit data, exact): each 16-lane
+// group reads 8 bytes per lane and step through a buffer resource (the R
+// loads of a step are issued together) and forms sum (q - x)^2 = sum q^2 +
+// sum x^2 - 2 sum q x with integer dot products (v_dot4_u32_u8); every term
+// is an exact integer and, for d <= 258, so is the FP32 result.  q8 holds the
+// query packed 4 bytes per word (LDS), qq = sum q^2.  d must be a multiple of
+// 8; rows are d bytes apart.
+template <int R>
+__device__ __forceinline__ void distR_u8_buf(const uint32_t *__restrict__ q8, uint32_t qq,
                                              __amdgpu_buffer_rsrc_t rs, const uint32_t *id, uint32_t d,
                                              uint32_t g, float *out) {
-    int acc[4] = {0, 0, 0, 0};
-    uint32_t off[4];
+    int acc[R];
+    uint32_t off[R];
 #pragma unroll
-    for (int r = 0; r < 4; r++) off[r] = id[r] * d + 8u * g;
+    for (int r = 0; r < R; r++) {
+        acc[r] = 0;
+        off[r] = id[r] * d + 8u * g;
+    }
     for (uint32_t e = 8u * g; e < d; e += 128u) {
-        ex_u32x2 v[4];
+        ex_u32x2 v[R];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
+        for (int r = 0; r < R; r++) {
             v[r] = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off[r], 0, 0);
             off[r] += 128u;
         }
         const uint32_t qa = q8[e >> 2], qb = q8[(e >> 2) + 1];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
+        for (int r = 0; r < R; r++) {
             const uint32_t xx = __builtin_amdgcn_udot4(v[r].x, v[r].x, __builtin_amdgcn_udot4(v[r].y, v[r].y, 0u, false), false);
             const uint32_t qx = __builtin_amdgcn_udot4(qa, v[r].x, __builtin_amdgcn_udot4(qb, v[r].y, 0u, false), false);
             acc[r] += (int)xx - 2 * (int)qx;
         }
     }
 #pragma unroll
-    for (int r = 0; r < 4; r++) out[r] = (float)((int)qq + row16_sum_i(acc[r]));
+    for (int r = 0; r < R; r++) out[r] = (float)((int)qq + row16_sum_i(acc[r]));
+}
+
+__device__ __forceinline__ void dist4_u8_buf(const uint32_t *__restrict__ q8, uint32_t qq,
+                                             __amdgpu_buffer_rsrc_t rs, const uint32_t *id, uint32_t d,
+                                             uint32_t g, float *out) {
+    distR_u8_buf<4>(q8, qq, rs, id, d, g, out);
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt() {

@@ -24,8 +24,9 @@
 //       reserves its range with one atomic and writes its proposals there.
 //       Nothing is dropped unless the whole iteration exceeds the proposal
 //       buffer (reported as dropped).
-//   k_nnd_merge: per node, proposals in chunks of 2048: sort by (d2, id), drop
-//       duplicates and ids already listed, merge with the sorted list
+//   k_nnd_merge: per node, proposals in chunks of 2048: keep those below the
+//       current worst entry, sort them by (d2, id), drop duplicates and ids
+//       already listed, merge with the sorted list
 //       (existing entries first on equal distance, as the reference's strict-<
 //       insertion leaves them); the first K survive, inserted entries flagged
 //       new.  Chunk after chunk this is the top-K of (list U all proposals).
@@ -480,6 +481,7 @@ __global__ __launch_bounds__(256) void k_nnd_merge(
     __shared__ uint8_t lnw[kMergeMaxK], onw[kMergeMaxK];
     __shared__ uint32_t hash[kHashSlots];
     __shared__ uint32_t wsum[4];
+    __shared__ uint32_t s_nk;
 
     const uint64_t u = blockIdx.x;
     const uint64_t beg = prop_off[u];
@@ -497,7 +499,29 @@ __global__ __launch_bounds__(256) void k_nnd_merge(
     uint32_t upd = 0;
     for (uint32_t base = 0; base < np; base += kMergeChunk) {
         const uint32_t nb = min(kMergeChunk, np - base);
+        // Proposals that can still enter (below the current worst entry; the
+        // worst falls as chunks merge) are compacted, then only they are
+        // sorted: the first chunk keeps nearly all, later ones few.
+        __syncthreads();   // the list (loaded, or rewritten by the last chunk) is complete
+        const float worst_full = (c >= K) ? ld[K - 1] : FLT_MAX;
+        if (threadIdx.x == 0) s_nk = 0;
         __syncthreads();
+        for (uint32_t t0 = 0; t0 < nb; t0 += blockDim.x) {   // uniform trip count
+            const uint32_t t = t0 + threadIdx.x;
+            unsigned long long key = ~0ull;
+            if (t < nb) {
+                const float dv = prop_d[beg + base + t];
+                if (dv < worst_full) key = ((unsigned long long)float_key(dv) << 32) | prop_id[beg + base + t];
+            }
+            const uint64_t bm = __ballot(key != ~0ull);
+            uint32_t at = 0;
+            if ((threadIdx.x & 63u) == 0 && bm) at = atomicAdd(&s_nk, (uint32_t)__popcll(bm));
+            at = __shfl(at, 0, 64);
+            if (key != ~0ull) keys[at + (uint32_t)__popcll(bm & lanemask_lt())] = key;
+        }
+        __syncthreads();
+        const uint32_t nk = s_nk;
+        if (nk == 0) continue;   // uniform
         // ids currently listed
         for (uint32_t t = threadIdx.x; t < kHashSlots; t += blockDim.x) hash[t] = kInvalidId;
         __syncthreads();
@@ -510,18 +534,9 @@ __global__ __launch_bounds__(256) void k_nnd_merge(
                 h = (h + 1) & (kHashSlots - 1);
             }
         }
-        const float worst_full = (c >= K) ? ld[K - 1] : FLT_MAX;
         uint32_t PP = 1;
-        while (PP < nb) PP <<= 1;
-        for (uint32_t t = threadIdx.x; t < PP; t += blockDim.x) {
-            unsigned long long key = ~0ull;
-            if (t < nb) {
-                float dv = prop_d[beg + base + t];
-                uint32_t id = prop_id[beg + base + t];
-                if (dv < worst_full) key = ((unsigned long long)float_key(dv) << 32) | id;
-            }
-            keys[t] = key;
-        }
+        while (PP < nk) PP <<= 1;
+        for (uint32_t t = nk + threadIdx.x; t < PP; t += blockDim.x) keys[t] = ~0ull;
         __syncthreads();
         bitonic_sort_lds(keys, PP);
         __syncthreads();
