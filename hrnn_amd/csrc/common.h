@@ -400,6 +400,68 @@ __device__ void bitonic_sort_lds(T *keys, uint32_t n) {
     __syncthreads();
 }
 
+// Ascending bitonic sort of the first n (<= 256 E) keys of `keys` (LDS) by a
+// 256-thread block, E keys per thread in registers (thread t holds positions
+// t E .. t E + E - 1; positions >= n sort as ~0).  Levels whose partner
+// distance is below E stay in registers, below 64 E they cross lanes of one
+// wave (shuffles), and only the few longer ones go through LDS, so the LDS
+// traffic of the in-LDS network (one read and one write per key and level)
+// falls to a handful of levels.  keys holds the sorted n keys afterwards.
+template <int E>
+__device__ void block_sort_u64_regs(unsigned long long *keys, uint32_t n) {
+    const uint32_t t = threadIdx.x;
+    unsigned long long v[E];
+#pragma unroll
+    for (int r = 0; r < E; r++) v[r] = (t * E + r < n) ? keys[t * E + r] : ~0ull;
+#pragma unroll
+    for (uint32_t k = 2; k <= 256u * E; k <<= 1) {
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            if (j < (uint32_t)E) {
+#pragma unroll
+                for (int r = 0; r < E; r++) {
+                    const int p = r ^ (int)j;
+                    if (p > r) {
+                        const bool asc = ((t * E + r) & k) == 0;
+                        const unsigned long long a = v[r], b = v[p];
+                        const bool sw = asc ? (a > b) : (a < b);
+                        v[r] = sw ? b : a;
+                        v[p] = sw ? a : b;
+                    }
+                }
+            } else {
+                const bool cross_wave = j >= 64u * E;
+                if (cross_wave) {
+                    __syncthreads();
+#pragma unroll
+                    for (int r = 0; r < E; r++) keys[t * E + r] = v[r];
+                    __syncthreads();
+                }
+#pragma unroll
+                for (int r = 0; r < E; r++) {
+                    const uint32_t i = t * E + r;
+                    unsigned long long o;
+                    if (cross_wave) {
+                        o = keys[i ^ j];
+                    } else {
+                        const uint32_t lj = j / E;   // lane distance inside the wave
+                        const uint32_t lo = __shfl_xor((uint32_t)v[r], (int)lj, 64);
+                        const uint32_t hi = __shfl_xor((uint32_t)(v[r] >> 32), (int)lj, 64);
+                        o = ((unsigned long long)hi << 32) | lo;
+                    }
+                    const bool keep_min = ((i & j) == 0) == ((i & k) == 0);
+                    v[r] = keep_min ? (v[r] < o ? v[r] : o) : (v[r] < o ? o : v[r]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < E; r++)
+        if (t * E + r < n) keys[t * E + r] = v[r];
+    __syncthreads();
+}
+
 __device__ __forceinline__ uint32_t next_pow2_dev(uint32_t v) {
     return v <= 1u ? 1u : (1u << (32 - __clz(v - 1u)));
 }

@@ -146,13 +146,22 @@ __device__ float l2_ref16_lane(const float *__restrict__ a, const float *__restr
     return r;
 }
 
-// kFull: the whole sample (T rows x d dims) is staged in LDS at once and each
-// wave evaluates its 16x16 Gram blocks one after another (no accumulator
-// array); otherwise d is staged in 64-dim chunks and each wave keeps the
-// accumulators of all its blocks across the chunks.
-template <bool kFill, bool kFull>
+// Sample staging modes of the join.  kJoinChunked: BF16 rows staged 64 dims
+// at a time, each wave keeping the accumulators of all its Gram blocks across
+// the chunks.  kJoinFullBf16: the whole sample (T rows x d BF16 dims) staged
+// at once, each wave evaluating its 16x16 blocks one after another.
+// kJoinFullI8: 8-bit data (every value an integer in [0, 255], d <= 258):
+// rows staged as int8 after subtracting 128 (x ^ 0x80), half the LDS of BF16,
+// Gram blocks on the int8 MFMA with int32 accumulation and squared norms of
+// the shifted rows from int8 dot products.  |a - b|^2 = |a'|^2 + |b'|^2 -
+// 2 a'.b' is then an exact integer below 2^24: the reference's FP32 value.
+enum { kJoinChunked = 0, kJoinFullBf16 = 1, kJoinFullI8 = 2 };
+typedef int i32x4_n __attribute__((ext_vector_type(4)));
+
+template <bool kFill, int kMode>
 __global__ __launch_bounds__(256) void k_nnd_join(
-    const __hip_bfloat16 *__restrict__ xb, const float *__restrict__ xn, const float *__restrict__ x32,
+    const __hip_bfloat16 *__restrict__ xb, const uint8_t *__restrict__ xu8, const float *__restrict__ xn,
+    const float *__restrict__ x32,
     uint32_t d, uint64_t n, uint32_t S, uint32_t R, uint32_t RB, const uint32_t *__restrict__ fn,
     const uint32_t *__restrict__ fn_cnt, const uint32_t *__restrict__ fo,
     const uint32_t *__restrict__ fo_cnt, const uint32_t *__restrict__ rn,
@@ -162,8 +171,13 @@ __global__ __launch_bounds__(256) void k_nnd_join(
     float *__restrict__ prop_d, uint64_t cap, int exact, unsigned long long *__restrict__ stats) {
     __shared__ uint32_t sid[kJMax];
     __shared__ float sthr[kJMax], snorm[kJMax];
+    __shared__ int s_n8[kJMax];   // kJoinFullI8: squared norms of the shifted rows
     extern __shared__ __align__(16) unsigned char s_dyn[];
-    __hip_bfloat16 *vec = reinterpret_cast<__hip_bfloat16 *>(s_dyn);   // kFull: Tpad x (d + 8), else kJMax x kJDP
+    // kJoinFullBf16: Tpad x (d + 8) BF16; kJoinChunked: kJMax x kJDP BF16;
+    // kJoinFullI8: Tpad x (roundup(d, 64) + 16) bytes
+    __hip_bfloat16 *vec = reinterpret_cast<__hip_bfloat16 *>(s_dyn);
+    int8_t *vec8 = reinterpret_cast<int8_t *>(s_dyn);
+    constexpr bool kFull = kMode != kJoinChunked;
     __shared__ uint32_t rsel[2][kJRev];   // reverse offers (new, old), sorted by source id
     __shared__ uint8_t blk_r[kJMaxB * 4], blk_c[kJMaxB * 4];
     __shared__ uint32_t s_pcnt[kJMax], s_pcur[kJMax], s_pbase[kJMax];   // per-slot proposals
@@ -219,7 +233,8 @@ __global__ __launch_bounds__(256) void k_nnd_join(
         else id = rsel[1][t - nnew - c];
         sid[t] = id;
         sthr[t] = worst[id];
-        snorm[t] = xn[id];
+        if (kMode == kJoinFullI8) s_n8[t] = 0;
+        else snorm[t] = xn[id];
     }
     const uint32_t RBk = (nn + 15) / 16, CB = (T + 15) / 16;
     if (threadIdx.x == 0) {
@@ -236,9 +251,28 @@ __global__ __launch_bounds__(256) void k_nnd_join(
     const uint32_t NB = s_nb;
     const uint32_t Tpad = CB * 16;
 
-    const uint32_t dP = d + 8;   // kFull row stride
+    const uint32_t dP = d + 8;                       // kJoinFullBf16 row stride (elements)
+    const uint32_t d64 = (d + 63) & ~63u, dP8 = d64 + 16;   // kJoinFullI8 row (bytes): padded dims, stride
     f32x4_n acc[kFull ? 1 : kJMaxB];
-    if constexpr (kFull) {
+    if constexpr (kMode == kJoinFullI8) {
+        // 16 bytes per step (d is a multiple of 32); dims d .. d64 are zero
+        // (no term in any product)
+        const uint32_t parts = d64 / 16;
+        for (uint32_t u = threadIdx.x; u < Tpad * parts; u += blockDim.x) {
+            const uint32_t t = u / parts, part = u % parts;
+            i32x4_n v = {0, 0, 0, 0};
+            if (t < T && part * 16 < d) {
+                v = *reinterpret_cast<const i32x4_n *>(xu8 + (uint64_t)sid[t] * d + part * 16);
+                v ^= 0x80808080;
+                int sq = 0;
+#pragma unroll
+                for (int e = 0; e < 4; e++) sq = __builtin_amdgcn_sdot4(v[e], v[e], sq, false);
+                atomicAdd(&s_n8[t], sq);
+            }
+            *reinterpret_cast<i32x4_n *>(&vec8[t * dP8 + part * 16]) = v;
+        }
+        __syncthreads();
+    } else if constexpr (kFull) {
         const uint32_t parts = d / 8;
         for (uint32_t u = threadIdx.x; u < Tpad * parts; u += blockDim.x) {
             const uint32_t t = u / parts, part = u % parts;
@@ -300,8 +334,9 @@ __global__ __launch_bounds__(256) void k_nnd_join(
     // values and thresholds, so the ranges are filled exactly.
     uint32_t ndist = 0, nprop = 0, dropped = 0;
     const uint32_t grp = lane >> 4, gl = lane & 15u;
-    // pa/pb/dv of the 4 pairs a lane holds in Gram block (rb, cb), then on_block
-    auto block_props = [&](uint32_t rb, uint32_t cb, const f32x4_n &g, auto &&on_block) {
+    // pa/pb/dv of the 4 pairs a lane holds in Gram block (rb, cb) (squared
+    // distances dg from the Gram values), then on_block
+    auto block_props = [&](uint32_t rb, uint32_t cb, const float *dg, auto &&on_block) {
         const uint32_t col = cb * 16 + gl;
         bool pa[4], pb[4];
         float dv[4];
@@ -314,10 +349,9 @@ __global__ __launch_bounds__(256) void k_nnd_join(
                 const uint32_t ia = sid[r], ib = sid[col];
                 if (ia != ib) {
                     if (!kFill) ndist++;
-                    float v = snorm[r] + snorm[col] - 2.f * g[i];
-                    v = v > 0.f ? v : 0.f;
+                    float v = dg[i];
                     bool qa = v < sthr[r], qb = v < sthr[col];
-                    if ((qa || qb) && !exact) {
+                    if (kMode != kJoinFullI8 && (qa || qb) && !exact) {
                         v = l2_ref16_lane(x32 + (uint64_t)ia * d, x32 + (uint64_t)ib * d, d);
                         qa = v < sthr[r];
                         qb = v < sthr[col];
@@ -331,7 +365,32 @@ __global__ __launch_bounds__(256) void k_nnd_join(
         on_block(rb * 16 + 4 * grp, col, pa, pb, dv);
     };
     auto each_block = [&](auto &&on_block) {
-        if constexpr (kFull) {
+        // squared distances of a lane's 4 pairs from the Gram values g
+        auto from_gram = [&](uint32_t rb, uint32_t cb, const f32x4_n &g, float *dg) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float v = snorm[rb * 16 + 4 * grp + i] + snorm[cb * 16 + gl] - 2.f * g[i];
+                dg[i] = v > 0.f ? v : 0.f;
+            }
+        };
+        if constexpr (kMode == kJoinFullI8) {
+            for (uint32_t b = w; b < NB; b += 4) {   // uniform over the wave
+                const uint32_t rb = blk_r[b], cb = blk_c[b];
+                i32x4_n g = {0, 0, 0, 0};
+                const int8_t *ra = &vec8[(rb * 16 + gl) * dP8 + 16 * grp];
+                const int8_t *rc = &vec8[(cb * 16 + gl) * dP8 + 16 * grp];
+                for (uint32_t kl = 0; kl < d64; kl += 64) {
+                    const i32x4_n av = *reinterpret_cast<const i32x4_n *>(ra + kl);
+                    const i32x4_n bv = *reinterpret_cast<const i32x4_n *>(rc + kl);
+                    g = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bv, g, 0, 0, 0);
+                }
+                float dg[4];
+                const int nc = s_n8[cb * 16 + gl];
+#pragma unroll
+                for (int i = 0; i < 4; i++) dg[i] = (float)(s_n8[rb * 16 + 4 * grp + i] + nc - 2 * g[i]);
+                block_props(rb, cb, dg, on_block);
+            }
+        } else if constexpr (kFull) {
             for (uint32_t b = w; b < NB; b += 4) {   // uniform over the wave
                 const uint32_t rb = blk_r[b], cb = blk_c[b];
                 f32x4_n g = {0.f, 0.f, 0.f, 0.f};
@@ -342,13 +401,19 @@ __global__ __launch_bounds__(256) void k_nnd_join(
                     const bf16x8_n bv = *reinterpret_cast<const bf16x8_n *>(rc + kl);
                     g = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, g, 0, 0, 0);
                 }
-                block_props(rb, cb, g, on_block);
+                float dg[4];
+                from_gram(rb, cb, g, dg);
+                block_props(rb, cb, dg, on_block);
             }
         } else {
 #pragma unroll
             for (int j = 0; j < kJMaxB; j++) {
                 const uint32_t b = w + 4 * j;
-                if (b < NB) block_props(blk_r[b], blk_c[b], acc[j], on_block);   // uniform over the wave
+                if (b < NB) {   // uniform over the wave
+                    float dg[4];
+                    from_gram(blk_r[b], blk_c[b], acc[j], dg);
+                    block_props(blk_r[b], blk_c[b], dg, on_block);
+                }
             }
         }
     };
@@ -436,6 +501,7 @@ __global__ __launch_bounds__(256) void k_nnd_join(
 }
 
 constexpr uint32_t kMergeChunk = 2048;   // proposals per merge round (LDS sort)
+constexpr uint32_t kMergeThreads = 256;  // k_nnd_merge's workgroup size
 constexpr uint32_t kMergeMaxK = 1024;
 constexpr uint32_t kHashSlots = 2048;
 
@@ -469,7 +535,7 @@ __device__ void block_excl_scan(uint32_t *flags, uint32_t P, uint32_t *wsum) {
 
 // One workgroup (256 lanes) per node with proposals: the list lives in LDS
 // while the node's proposals are merged into it 2048 at a time.
-__global__ __launch_bounds__(256) void k_nnd_merge(
+__global__ __launch_bounds__(kMergeThreads) void k_nnd_merge(
     uint32_t *__restrict__ ids, float *__restrict__ kd, uint8_t *__restrict__ isnew,
     uint32_t *__restrict__ cnt, uint32_t K, const uint32_t *__restrict__ prop_cnt,
     const uint64_t *__restrict__ prop_off, const uint32_t *__restrict__ prop_id,
@@ -491,7 +557,7 @@ __global__ __launch_bounds__(256) void k_nnd_merge(
     const uint32_t np = (uint32_t)np64;
     if (np == 0) return;
     uint32_t c = cnt[u];
-    for (uint32_t t = threadIdx.x; t < c; t += blockDim.x) {
+    for (uint32_t t = threadIdx.x; t < c; t += kMergeThreads) {
         lid[t] = ids[u * K + t];
         ld[t] = kd[u * K + t];
         lnw[t] = isnew[u * K + t];
@@ -506,13 +572,20 @@ __global__ __launch_bounds__(256) void k_nnd_merge(
         const float worst_full = (c >= K) ? ld[K - 1] : FLT_MAX;
         if (threadIdx.x == 0) s_nk = 0;
         __syncthreads();
-        for (uint32_t t0 = 0; t0 < nb; t0 += blockDim.x) {   // uniform trip count
-            const uint32_t t = t0 + threadIdx.x;
-            unsigned long long key = ~0ull;
-            if (t < nb) {
-                const float dv = prop_d[beg + base + t];
-                if (dv < worst_full) key = ((unsigned long long)float_key(dv) << 32) | prop_id[beg + base + t];
-            }
+        // all of the thread's proposals are loaded at once (one round trip)
+        constexpr uint32_t kPer = kMergeChunk / kMergeThreads;
+        float pdv[kPer];
+        uint32_t pid[kPer];
+#pragma unroll
+        for (uint32_t i = 0; i < kPer; i++) {   // unconditional (clamped) loads: no branch between them
+            const uint32_t t = min(i * kMergeThreads + threadIdx.x, nb - 1);
+            pdv[i] = prop_d[beg + base + t];
+            pid[i] = prop_id[beg + base + t];
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < kPer; i++) {
+            const bool in = i * kMergeThreads + threadIdx.x < nb && pdv[i] < worst_full;
+            const unsigned long long key = in ? ((unsigned long long)float_key(pdv[i]) << 32) | pid[i] : ~0ull;
             const uint64_t bm = __ballot(key != ~0ull);
             uint32_t at = 0;
             if ((threadIdx.x & 63u) == 0 && bm) at = atomicAdd(&s_nk, (uint32_t)__popcll(bm));
@@ -523,9 +596,9 @@ __global__ __launch_bounds__(256) void k_nnd_merge(
         const uint32_t nk = s_nk;
         if (nk == 0) continue;   // uniform
         // ids currently listed
-        for (uint32_t t = threadIdx.x; t < kHashSlots; t += blockDim.x) hash[t] = kInvalidId;
+        for (uint32_t t = threadIdx.x; t < kHashSlots; t += kMergeThreads) hash[t] = kInvalidId;
         __syncthreads();
-        for (uint32_t t = threadIdx.x; t < c; t += blockDim.x) {
+        for (uint32_t t = threadIdx.x; t < c; t += kMergeThreads) {
             uint32_t v = lid[t];
             uint32_t h = (v * 2654435761u) & (kHashSlots - 1);
             for (;;) {
@@ -534,15 +607,18 @@ __global__ __launch_bounds__(256) void k_nnd_merge(
                 h = (h + 1) & (kHashSlots - 1);
             }
         }
-        uint32_t PP = 1;
+        uint32_t PP = 256;   // the register sort's smallest size
         while (PP < nk) PP <<= 1;
-        for (uint32_t t = nk + threadIdx.x; t < PP; t += blockDim.x) keys[t] = ~0ull;
-        __syncthreads();
-        bitonic_sort_lds(keys, PP);
+        for (uint32_t t = nk + threadIdx.x; t < PP; t += kMergeThreads) keys[t] = ~0ull;
+        __syncthreads();   // also completes the list hash
+        if (PP == 256) block_sort_u64_regs<1>(keys, PP);
+        else if (PP == 512) block_sort_u64_regs<2>(keys, PP);
+        else if (PP == 1024) block_sort_u64_regs<4>(keys, PP);
+        else block_sort_u64_regs<8>(keys, PP);
         __syncthreads();
         // a proposal counts once: not a repeat of the previous key (the same
         // pair always has the same distance) and not already listed
-        for (uint32_t t = threadIdx.x; t < PP; t += blockDim.x) {
+        for (uint32_t t = threadIdx.x; t < PP; t += kMergeThreads) {
             unsigned long long key = keys[t];
             uint32_t ok = 0;
             if (key != ~0ull && (t == 0 || keys[t - 1] != key)) {
@@ -563,7 +639,7 @@ __global__ __launch_bounds__(256) void k_nnd_merge(
         const uint32_t nvalid = scan[PP];
         if (nvalid == 0) continue;   // uniform
         // proposals: pos = #valid before + #listed with dist <= d
-        for (uint32_t t = threadIdx.x; t < PP; t += blockDim.x) {
+        for (uint32_t t = threadIdx.x; t < PP; t += kMergeThreads) {
             unsigned long long key = keys[t];
             if (key == ~0ull || scan[t + 1] == scan[t]) continue;
             float dv = key_float((uint32_t)(key >> 32));
@@ -581,7 +657,7 @@ __global__ __launch_bounds__(256) void k_nnd_merge(
             }
         }
         // listed entries: pos = e + #valid proposals with dist < ld[e]
-        for (uint32_t e = threadIdx.x; e < c; e += blockDim.x) {
+        for (uint32_t e = threadIdx.x; e < c; e += kMergeThreads) {
             uint32_t kf = float_key(ld[e]);
             uint32_t lo = 0, hi = PP;
             while (lo < hi) {
@@ -597,14 +673,14 @@ __global__ __launch_bounds__(256) void k_nnd_merge(
         }
         __syncthreads();
         c = min(c + nvalid, K);
-        for (uint32_t t = threadIdx.x; t < c; t += blockDim.x) {
+        for (uint32_t t = threadIdx.x; t < c; t += kMergeThreads) {
             lid[t] = oid[t];
             ld[t] = od[t];
             lnw[t] = onw[t];
         }
     }
     __syncthreads();
-    for (uint32_t t = threadIdx.x; t < c; t += blockDim.x) {
+    for (uint32_t t = threadIdx.x; t < c; t += kMergeThreads) {
         ids[u * K + t] = lid[t];
         kd[u * K + t] = ld[t];
         isnew[u * K + t] = lnw[t];
@@ -623,13 +699,27 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
     const uint32_t R = (uint32_t)S;
     const uint32_t RB = kJRev;   // reverse offers kept per node and kind
     const int exact = (h->base_u8 && h->d <= 128) ? 1 : 0;
-    // Whole-sample staging when 4S rows of d bf16 dims fit in 56 KiB of LDS
-    // (d <= 128 at S = 50); wider vectors go through 64-dim chunks.
-    const uint64_t full_lds = (uint64_t)((4 * S + 15) / 16 * 16) * (h->d + 8) * 2;
-    const bool full = full_lds <= 56 * 1024;
-    const uint32_t join_lds = full ? (uint32_t)full_lds : (uint32_t)(kJMax * kJDP * 2);
-    auto join_count = full ? k_nnd_join<false, true> : k_nnd_join<false, false>;
-    auto join_fill = full ? k_nnd_join<true, true> : k_nnd_join<true, false>;
+    // Whole-sample staging when 4S rows fit in 56 KiB of LDS: as int8 for
+    // 8-bit data at d <= 258 (S = 50: d <= 256), else as BF16 (S = 50:
+    // d <= 128); wider vectors go through 64-dim chunks.
+    const uint64_t rows = (uint64_t)((4 * S + 15) / 16 * 16);
+    const uint64_t i8_lds = rows * (((h->d + 63) & ~63u) + 16);
+    const uint64_t bf16_lds = rows * (h->d + 8) * 2;
+    int mode = kJoinChunked;
+    if (h->base_u8 && h->d <= 258 && i8_lds <= 56 * 1024) mode = kJoinFullI8;
+    else if (bf16_lds <= 56 * 1024) mode = kJoinFullBf16;
+    if (const char *jm = std::getenv("HRNN_NND_JOIN")) mode = std::min(std::max(std::atoi(jm), 0), 2);
+    if (mode == kJoinFullI8 && !(h->base_u8 && h->d <= 258 && i8_lds <= 64 * 1024)) mode = kJoinChunked;
+    if (mode == kJoinFullBf16 && bf16_lds > 64 * 1024) mode = kJoinChunked;
+    const uint32_t join_lds = mode == kJoinFullI8    ? (uint32_t)i8_lds
+                              : mode == kJoinFullBf16 ? (uint32_t)bf16_lds
+                                                      : (uint32_t)(kJMax * kJDP * 2);
+    auto join_count = mode == kJoinFullI8    ? k_nnd_join<false, kJoinFullI8>
+                      : mode == kJoinFullBf16 ? k_nnd_join<false, kJoinFullBf16>
+                                              : k_nnd_join<false, kJoinChunked>;
+    auto join_fill = mode == kJoinFullI8    ? k_nnd_join<true, kJoinFullI8>
+                     : mode == kJoinFullBf16 ? k_nnd_join<true, kJoinFullBf16>
+                                             : k_nnd_join<true, kJoinChunked>;
 
     // Proposal buffer: per_node_cap entries per node (default max(16384, 32K)),
     // at most 60% of the free device memory (8 B per proposal).
@@ -680,15 +770,17 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
                            cnt, n, K, (uint32_t)S, RB, worst, fn, fn_cnt, fo, fo_cnt, rn, rn_cnt, ro,
                            ro_cnt);
         hipLaunchKernelGGL(join_count, dim3((unsigned)n), dim3(256), join_lds, h->stream,
-                           h->xb.as<__hip_bfloat16>(), h->norms.as<float>(), h->x.as<float>(), h->d,
+                           h->xb.as<__hip_bfloat16>(), h->xu8.as<uint8_t>(), h->norms.as<float>(),
+                           h->x.as<float>(), h->d,
                            n, (uint32_t)S, R, RB, fn, fn_cnt, fo, fo_cnt, rn, rn_cnt, ro, ro_cnt, worst,
                            prop_cnt, prop_off, prop_id, prop_d, cap, exact, stats);
         scan_u32(h, prop_cnt, n, prop_off, bsum, ptotal);
         hipLaunchKernelGGL(join_fill, dim3((unsigned)n), dim3(256), join_lds, h->stream,
-                           h->xb.as<__hip_bfloat16>(), h->norms.as<float>(), h->x.as<float>(), h->d,
+                           h->xb.as<__hip_bfloat16>(), h->xu8.as<uint8_t>(), h->norms.as<float>(),
+                           h->x.as<float>(), h->d,
                            n, (uint32_t)S, R, RB, fn, fn_cnt, fo, fo_cnt, rn, rn_cnt, ro, ro_cnt, worst,
                            prop_fill, prop_off, prop_id, prop_d, cap, exact, stats);
-        hipLaunchKernelGGL(k_nnd_merge, dim3((unsigned)n), dim3(256), 0, h->stream, ids, kd, isnew,
+        hipLaunchKernelGGL(k_nnd_merge, dim3((unsigned)n), dim3(kMergeThreads), 0, h->stream, ids, kd, isnew,
                            cnt, K, prop_cnt, prop_off, prop_id, prop_d, cap, stats);
     };
 

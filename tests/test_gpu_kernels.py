@@ -365,26 +365,33 @@ def _first_iteration_matches_oracle(x, sid, sd, sc, K, S):
     print("nodes with identical id lists:", same_ids)
 
 
-def test_nndescent_first_iteration_exact(small_data, small_ref):
+@pytest.mark.parametrize("join_mode", [None, "1", "0"])
+def test_nndescent_first_iteration_exact(small_data, small_ref, join_mode, monkeypatch):
     """One iteration from the same seeds: the GPU evaluates exactly the pairs
     of the oracle's single-threaded join (same forward and reverse samples)
     and every node ends with exactly the oracle's sorted distance list (the
     top-K of list U proposals is unique as distances; ids may differ only
-    among equal distances).  Nothing is dropped.  At d=128, S=12 the whole
-    sample is staged in LDS at once."""
+    among equal distances).  Nothing is dropped.  At d=128, S=12 the join's
+    three staging modes are run: int8 rows (the default for 8-bit data), the
+    whole sample as BF16 (HRNN_NND_JOIN=1) and 64-dim BF16 chunks (=0)."""
+    if join_mode is not None:
+        monkeypatch.setenv("HRNN_NND_JOIN", join_mode)
     x, _ = small_data
     r, seed = small_ref
     sid, sd, sc = seed
     _first_iteration_matches_oracle(x, sid, sd, sc, r.K, 12)
 
 
-def test_nndescent_first_iteration_exact_chunked_dims():
-    """As above at d=256, S=32: 4S rows of 256 bf16 dims exceed the LDS kept
-    for whole-sample staging, so the join stages 64-dim chunks and keeps one
-    accumulator per Gram block; u8 data at d > 128 also takes the FP32
-    re-evaluation of every candidate pair."""
+@pytest.mark.parametrize("join_mode", [None, "0"])
+def test_nndescent_first_iteration_exact_wide_dims(join_mode, monkeypatch):
+    """As above at d=256, S=32: int8 rows (default; every distance an exact
+    integer below 2^24) and 64-dim BF16 chunks with one accumulator per Gram
+    block (HRNN_NND_JOIN=0), where d > 128 also takes the FP32 re-evaluation
+    of every candidate pair."""
     if not oracle.ref_available():
         pytest.skip("compiled reference (oracle/_ref) not available")
+    if join_mode is not None:
+        monkeypatch.setenv("HRNN_NND_JOIN", join_mode)
     x = mixture(1000, d=256, seed=41)
     r = oracle.RefIndex(1000, 256, M=8, efc=40, K=40)
     r.add_points(x, nthreads=1)
