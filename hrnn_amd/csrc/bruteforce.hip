@@ -186,40 +186,83 @@ __global__ __launch_bounds__(256) void k_bf_mfma(
     }
     __syncthreads();
 
+    // Base tiles are prefetched into registers one tile ahead (d <= 256: the
+    // whole next tile, 64 columns x d dims, KS / 2 chunks of two 16-byte
+    // loads per thread) or one chunk ahead (wider rows), so the global-load
+    // latency overlaps the MFMAs and the epilogue instead of stalling every
+    // chunk.  (A barrier does not wait for outstanding global loads.)
+    constexpr int NCH = (KS + 1) / 2;                 // 64-dim chunks per tile
+    constexpr bool kWholeTile = KS <= 8;
+    constexpr int NPV = kWholeTile ? 2 * NCH : 2;     // 16-byte vectors held per thread
+    vec pv[NPV];
+    float pxn = 0.f;
+    // global -> registers for chunk ch of the tile at column c; slot base b
+    auto fetch = [&](uint64_t c, int ch, int b) {
+        const int dims = (KS - 2 * ch >= 2) ? 64 : 32;
+#pragma unroll
+        for (int v2 = 0; v2 < 2; v2++) {
+            const int u = threadIdx.x + v2 * 256;
+            const int col = u / (dims / 8), part = u % (dims / 8);
+            const uint64_t gc = c + col;
+            if (u < kCT * (dims / 8) && gc < n) {
+                pv[b + v2] = *reinterpret_cast<const vec *>(xb + gc * d + ch * 64 + part * 8);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; e++) pv[b + v2][e] = 0;
+            }
+        }
+    };
+    auto put = [&](int ch, int b) {   // registers -> LDS chunk buffer
+        const int dims = (KS - 2 * ch >= 2) ? 64 : 32;
+#pragma unroll
+        for (int v2 = 0; v2 < 2; v2++) {
+            const int u = threadIdx.x + v2 * 256;
+            if (u < kCT * (dims / 8)) {
+                const int col = u / (dims / 8), part = u % (dims / 8);
+                *reinterpret_cast<vec *>(&bs[col * kCHP + part * 8]) = pv[b + v2];
+            }
+        }
+    };
+    if (kWholeTile) {
+#pragma unroll
+        for (int ch = 0; ch < NCH; ch++) fetch(0, ch, 2 * ch);
+    } else {
+        fetch(0, 0, 0);
+    }
+    if (threadIdx.x < kCT) pxn = threadIdx.x < n ? xn[threadIdx.x] : 0.f;
+
     for (uint64_t c0 = 0; c0 < n; c0 += kCT) {
         f32x4_t acc[FR][4];
 #pragma unroll
         for (int f = 0; f < FR; f++)
 #pragma unroll
             for (int cb = 0; cb < 4; cb++) acc[f][cb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-        __syncthreads();   // previous tile's epilogue is done with s_xn
-        if (threadIdx.x < kCT) s_xn[threadIdx.x] = (c0 + threadIdx.x < n) ? xn[c0 + threadIdx.x] : 0.f;
+        const uint64_t c1 = c0 + kCT;   // next tile
 #pragma unroll
-        for (int ch = 0; ch < KS; ch += 2) {
-            // stage columns [c0, c0+64) x dims [ch*32, ch*32+64) into LDS
-            __syncthreads();
-            const int dims = (KS - ch >= 2) ? 64 : 32;
-            for (int u = threadIdx.x; u < kCT * (dims / 8); u += blockDim.x) {
-                int col = u / (dims / 8), part = u % (dims / 8);
-                uint64_t gc = c0 + col;
-                vec v;
-                if (gc < n) {
-                    v = *reinterpret_cast<const vec *>(xb + gc * d + ch * 32 + part * 8);
-                } else {
-#pragma unroll
-                    for (int e = 0; e < 8; e++) v[e] = 0;
-                }
-                *reinterpret_cast<vec *>(&bs[col * kCHP + part * 8]) = v;
+        for (int ch = 0; ch < NCH; ch++) {
+            __syncthreads();   // the previous chunk's MFMAs (and epilogue) are done with bs and s_xn
+            if (ch == 0 && threadIdx.x < kCT) s_xn[threadIdx.x] = pxn;
+            if (kWholeTile) {
+                put(ch, 2 * ch);
+                if (c1 < n) fetch(c1, ch, 2 * ch);   // this chunk of the next tile
+                if (ch == 0 && threadIdx.x < kCT && c1 < n)
+                    pxn = (c1 + threadIdx.x < n) ? xn[c1 + threadIdx.x] : 0.f;
+            } else {
+                put(ch, 0);
+                if (ch + 1 < NCH) fetch(c0, ch + 1, 0);
+                else if (c1 < n) fetch(c1, 0, 0);
+                if (ch == 0 && threadIdx.x < kCT && c1 < n)
+                    pxn = (c1 + threadIdx.x < n) ? xn[c1 + threadIdx.x] : 0.f;
             }
             __syncthreads();
 #pragma unroll
             for (int kl = 0; kl < 2; kl++) {
-                if (ch + kl < KS) {
+                if (2 * ch + kl < KS) {
 #pragma unroll
                     for (int cb = 0; cb < 4; cb++) {
                         vec b = *reinterpret_cast<const vec *>(&bs[(16 * cb + (lane & 15u)) * kCHP + kl * 32 + 8 * (lane >> 4)]);
 #pragma unroll
-                        for (int f = 0; f < FR; f++) acc[f][cb] = Mfma<T>::run(afrag[f][ch + kl], b, acc[f][cb]);
+                        for (int f = 0; f < FR; f++) acc[f][cb] = Mfma<T>::run(afrag[f][2 * ch + kl], b, acc[f][cb]);
                     }
                 }
             }

@@ -118,7 +118,12 @@ typedef struct hrnn_nnd_iter_stats {
  * on the resident graph.  Every proposal of an iteration is kept in a proposal
  * CSR of bucket * n entries (bucket = 0: max(16384, 32 K)), capped at 60% of
  * the free device memory; proposals beyond it are counted as dropped.  stats
- * may be NULL or point to num_iters records. */
+ * may be NULL or point to num_iters records.  The iterations run as one HIP
+ * graph (environment HRNN_NND_GRAPH=0: launched one by one).  The join
+ * stages each node's sample as int8 for 8-bit data at d <= 258, else as BF16
+ * (whole, or 64 dims at a time when 4S rows do not fit 56 KiB of LDS);
+ * HRNN_NND_JOIN=0/1/2 forces chunked BF16 / whole BF16 / int8 where the data
+ * and LDS allow it (all three give the same lists). */
 int hrnn_dev_nndescent(hrnn_dev_t h, int num_iters, int sample_size, uint32_t bucket,
                        hrnn_nnd_iter_stats *stats);
 
