@@ -260,8 +260,9 @@ static void csr_fill(hrnn_dev_t h, uint32_t node_bits, uint32_t rank_bits) {
 // key = (dst << rank_bits) | rank, carrying src: each list then holds its
 // entries in (rank, src) order with no per-list sort and no global atomics.
 // 8-bit digits; each pass = a per-tile LDS histogram, one exclusive scan over
-// [digit][tile] counts, and a scatter that ranks equal digits stably inside
-// the tile (wave ballots over the 8 digit bits, per-wave counts in LDS).  The
+// [digit][tile] counts, and a scatter that sorts the tile by digit in LDS
+// (stable: wave ballots over the 8 digit bits, per-wave counts) and writes
+// it out digit run by digit run.  The
 // first pass reads the KNN rows (entries past a row's count are dropped), the
 // last writes the CSR entries (src << rank_bits) | rank and the sorted keys,
 // from which the offsets follow.
@@ -318,38 +319,60 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const uint32_t *__restri
 // Stable scatter of one pass.  goff: exclusive scan of hist (digit-major).
 // kLast: write CSR entries to out_ent and sorted keys to out_key; otherwise
 // keys and values to out_key / out_val.
+//
+// The tile is first sorted by digit in LDS, then written out digit run by
+// digit run, so consecutive lanes store to consecutive addresses (runs of
+// ~16 elements per digit and tile instead of 256 scattered stores per
+// round).  Local ranks: wave w ranks its contiguous quarter of the tile in
+// rounds of 64 (lanes with the same digit found by 8 ballots), keeping its
+// running per-digit counts in LDS without barriers (one wave's LDS accesses
+// are performed in order); one barrier later the tile's digit offsets and
+// the counts of the earlier waves give every element its place.
 template <bool kRows, bool kLast>
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
     const uint32_t *__restrict__ ids, const uint32_t *__restrict__ cnt, uint64_t n, uint32_t K, uint32_t rb,
     uint32_t rows, const uint32_t *__restrict__ in_key, const uint32_t *__restrict__ in_val, uint64_t E,
     uint32_t shift, uint32_t tiles, const uint64_t *__restrict__ goff, uint32_t *__restrict__ out_key,
     uint32_t *__restrict__ out_val, uint32_t *__restrict__ out_ent) {
-    __shared__ uint64_t run[kRsBins];                    // next output position per digit
-    __shared__ uint32_t wcnt[2][kRsThreads / 64][kRsBins]; // per-wave digit counts (double buffered)
+    constexpr int kWaves = kRsThreads / 64;
+    constexpr int kPerWave = kRsTile / kWaves;   // 1024 consecutive elements per wave
+    constexpr int kR = kPerWave / 64;            // rounds of 64
+    __shared__ uint32_t s_key[kRsTile], s_val[kRsTile];
+    __shared__ uint32_t wc[kWaves][kRsBins];     // per-wave running digit counts
+    __shared__ uint32_t loff[kRsBins + 1];       // tile-local start of each digit; [256] = tile total
+    __shared__ uint64_t gbase[kRsBins];          // global start of each digit's run of this tile
+    __shared__ uint32_t wsum[kWaves];
     const uint32_t tile = blockIdx.x, lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    run[threadIdx.x] = goff[(uint64_t)threadIdx.x * tiles + tile];
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) wc[w][threadIdx.x] = 0;
+    gbase[threadIdx.x] = goff[(uint64_t)threadIdx.x * tiles + tile];
+    __syncthreads();
     const uint32_t rmask = (1u << rb) - 1u;
     const uint32_t span = kRows ? rows * K : (uint32_t)kRsTile;
     const uint64_t row0 = (uint64_t)tile * rows, b = (uint64_t)tile * kRsTile;
-    uint32_t par = 0;
-    for (uint32_t r0 = 0; r0 < span; r0 += kRsThreads, par ^= 1u) {
+    uint32_t key[kR], val[kR], pos[kR];
+    uint32_t vmask = 0;   // bit r: element of round r is valid
 #pragma unroll
-        for (int w = 0; w < kRsThreads / 64; w++) wcnt[par][w][threadIdx.x] = 0;
-        __syncthreads();
-        const uint32_t t = r0 + threadIdx.x;
-        uint32_t key = 0, val = 0;
+    for (int r = 0; r < kR; r++) {   // every load of the quarter in flight at once
+        const uint32_t t = wave * kPerWave + r * 64 + lane;
         bool valid = false;
+        key[r] = 0;
+        val[r] = 0;
         if (t < span) {
             if (kRows) {
-                valid = rs_edge(ids, cnt, n, K, rb, row0, t, &key, &val);
+                valid = rs_edge(ids, cnt, n, K, rb, row0, t, &key[r], &val[r]);
             } else if (b + t < E) {
                 valid = true;
-                key = in_key[b + t];
-                val = in_val[b + t];
+                key[r] = in_key[b + t];
+                val[r] = in_val[b + t];
             }
         }
-        const uint32_t dg = (key >> shift) & 255u;
-        // lanes of this wave holding the same digit, in input order
+        vmask |= valid ? (1u << r) : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kR; r++) {
+        const bool valid = (vmask >> r) & 1u;
+        const uint32_t dg = (key[r] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int bit = 0; bit < 8; bit++) {
@@ -357,24 +380,53 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
             peers &= ((dg >> bit) & 1u) ? bb : ~bb;
         }
         const uint32_t below = (uint32_t)__popcll(peers & lanemask_lt());
-        if (valid && below == 0) wcnt[par][wave][dg] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint64_t pos = run[dg] + below;
-            for (uint32_t w = 0; w < wave; w++) pos += wcnt[par][w][dg];
-            if (kLast) {
-                out_ent[pos] = (val << rb) | (key & rmask);
-                out_key[pos] = key;
-            } else {
-                out_key[pos] = key;
-                out_val[pos] = val;
-            }
-        }
-        __syncthreads();   // every position of this round is read before run moves on
-        uint32_t add = 0;
+        uint32_t base = 0;
+        if (valid) base = wc[wave][dg];
+        pos[r] = base + below;
+        if (valid && below == 0) wc[wave][dg] = base + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    // tile-local digit offsets: exclusive scan of the per-digit totals
+    {
+        uint32_t tot = 0;
 #pragma unroll
-        for (int w = 0; w < kRsThreads / 64; w++) add += wcnt[par][w][threadIdx.x];
-        run[threadIdx.x] += add;
+        for (int w = 0; w < kWaves; w++) tot += wc[w][threadIdx.x];
+        uint32_t incl = tot;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, 64);
+            if (lane >= (uint32_t)o) incl += y;
+        }
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        uint32_t woff = 0;
+        for (uint32_t w = 0; w < wave; w++) woff += wsum[w];
+        loff[threadIdx.x] = woff + incl - tot;
+        if (threadIdx.x == kRsThreads - 1) loff[kRsBins] = woff + incl;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kR; r++) {
+        if ((vmask >> r) & 1u) {
+            const uint32_t dg = (key[r] >> shift) & 255u;
+            uint32_t p = loff[dg] + pos[r];
+            for (uint32_t w = 0; w < wave; w++) p += wc[w][dg];
+            s_key[p] = key[r];
+            s_val[p] = val[r];
+        }
+    }
+    __syncthreads();
+    const uint32_t total = loff[kRsBins];
+    for (uint32_t i = threadIdx.x; i < total; i += kRsThreads) {
+        const uint32_t k = s_key[i];
+        const uint32_t dg = (k >> shift) & 255u;
+        const uint64_t gp = gbase[dg] + (i - loff[dg]);
+        if (kLast) {
+            out_ent[gp] = (s_val[i] << rb) | (k & rmask);
+            out_key[gp] = k;
+        } else {
+            out_key[gp] = k;
+            out_val[gp] = s_val[i];
+        }
     }
 }
 
