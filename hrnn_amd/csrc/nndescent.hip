@@ -52,6 +52,13 @@ typedef __bf16 bf16x8_n __attribute__((ext_vector_type(8)));
 typedef float f32x4_n __attribute__((ext_vector_type(4)));
 
 enum { ST_DISTS = 0, ST_PROPOSALS = 1, ST_DROPPED = 2, ST_UPDATES = 3, ST_N = 4 };
+// Each counter is spread over kStatSlots words (workgroup b adds to slot
+// b % kStatSlots; the host sums them): a million workgroups adding to one
+// word would serialise on it.
+constexpr uint32_t kStatSlots = 256;
+__device__ __forceinline__ unsigned long long *stat_at(unsigned long long *stats, int s) {
+    return stats + (uint32_t)s * kStatSlots + (blockIdx.x & (kStatSlots - 1));
+}
 
 // worst(u) and per-iteration counter reset
 __global__ void k_nnd_prepare(const float *__restrict__ kd, const uint32_t *__restrict__ cnt,
@@ -60,9 +67,7 @@ __global__ void k_nnd_prepare(const float *__restrict__ kd, const uint32_t *__re
                               uint32_t *__restrict__ prop_cnt, uint32_t *__restrict__ prop_fill,
                               unsigned long long *__restrict__ stats) {
     uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (u == 0) {
-        for (int s = 0; s < ST_N; s++) stats[s] = 0;
-    }
+    for (uint64_t i = u; i < (uint64_t)ST_N * kStatSlots; i += (uint64_t)gridDim.x * blockDim.x) stats[i] = 0;
     if (u >= n) return;
     uint32_t c = cnt[u];
     worst[u] = (c >= K) ? kd[u * K + (K - 1)] : FLT_MAX;
@@ -494,9 +499,9 @@ __global__ __launch_bounds__(256) void k_nnd_join(
         dropped += __shfl_down(dropped, off, 64);
     }
     if (lane == 0) {
-        if (!kFill && ndist) atomicAdd(&stats[ST_DISTS], (unsigned long long)ndist);
-        if (!kFill && nprop) atomicAdd(&stats[ST_PROPOSALS], (unsigned long long)nprop);
-        if (kFill && dropped) atomicAdd(&stats[ST_DROPPED], (unsigned long long)dropped);
+        if (!kFill && ndist) atomicAdd(stat_at(stats, ST_DISTS), (unsigned long long)ndist);
+        if (!kFill && nprop) atomicAdd(stat_at(stats, ST_PROPOSALS), (unsigned long long)nprop);
+        if (kFill && dropped) atomicAdd(stat_at(stats, ST_DROPPED), (unsigned long long)dropped);
     }
 }
 
@@ -686,7 +691,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_nnd_merge(
         isnew[u * K + t] = lnw[t];
     }
     for (int off = 32; off > 0; off >>= 1) upd += __shfl_down(upd, off, 64);
-    if ((threadIdx.x & 63u) == 0 && upd) atomicAdd(&stats[ST_UPDATES], (unsigned long long)upd);
+    if ((threadIdx.x & 63u) == 0 && upd) atomicAdd(stat_at(stats, ST_UPDATES), (unsigned long long)upd);
     if (threadIdx.x == 0) cnt[u] = c;
 }
 
@@ -758,12 +763,13 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
     float *kd = h->knn_d.as<float>();
     uint32_t *cnt = h->knn_cnt.as<uint32_t>();
     uint8_t *isnew = h->knn_new.as<uint8_t>();
-    // per-iteration counters: ST_N u64 per iteration (zeroed by k_nnd_prepare)
-    h->s5.reserve((uint64_t)std::max(iters, 1) * ST_N * 8 + 64);
+    // per-iteration counters: ST_N x kStatSlots u64 per iteration (zeroed by k_nnd_prepare)
+    constexpr uint64_t kStatWords = (uint64_t)ST_N * kStatSlots;
+    h->s5.reserve((uint64_t)std::max(iters, 1) * kStatWords * 8 + 64);
     unsigned long long *all_stats = h->s5.as<unsigned long long>();
 
     auto enqueue_iteration = [&](int it) {
-        unsigned long long *stats = all_stats + (uint64_t)it * ST_N;
+        unsigned long long *stats = all_stats + (uint64_t)it * kStatWords;
         hipLaunchKernelGGL(k_nnd_prepare, dim3(ceil_div(n, 256)), dim3(256), 0, h->stream, kd, cnt, n,
                            K, worst, rn_cnt, ro_cnt, prop_cnt, prop_fill, stats);
         hipLaunchKernelGGL(k_nnd_sample, dim3(ceil_div(n, 4)), dim3(256), 0, h->stream, ids, kd, isnew,
@@ -786,7 +792,7 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
 
     const char *env = std::getenv("HRNN_NND_GRAPH");
     const bool use_graph = !(env && env[0] == '0') && iters > 0;
-    std::vector<unsigned long long> sv((uint64_t)std::max(iters, 1) * ST_N, 0);
+    std::vector<unsigned long long> sv((uint64_t)std::max(iters, 1) * kStatWords, 0);
     if (use_graph) {
         // All iterations captured as one HIP graph: 8 kernels per iteration, no
         // host synchronisation in between; counters are reset by kernels only.
@@ -814,13 +820,15 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
             auto t0 = std::chrono::steady_clock::now();
             enqueue_iteration(it);
             HIPC(hipGetLastError());
-            d2h(h, sv.data() + (uint64_t)it * ST_N, all_stats + (uint64_t)it * ST_N, ST_N * 8);
+            d2h(h, sv.data() + (uint64_t)it * kStatWords, all_stats + (uint64_t)it * kStatWords, kStatWords * 8);
             if (st) st[it].ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
     }
     if (st)
         for (int it = 0; it < iters; it++) {
-            const unsigned long long *s = sv.data() + (uint64_t)it * ST_N;
+            unsigned long long s[ST_N] = {0, 0, 0, 0};
+            for (int c = 0; c < ST_N; c++)
+                for (uint32_t j = 0; j < kStatSlots; j++) s[c] += sv[(uint64_t)it * kStatWords + c * kStatSlots + j];
             st[it].dists = s[ST_DISTS];
             st[it].proposals = s[ST_PROPOSALS];
             st[it].dropped = s[ST_DROPPED];
