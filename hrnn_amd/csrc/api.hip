@@ -92,6 +92,17 @@ void launch_base_mirrors(hrnn_dev_t h, uint64_t row0, uint64_t rows) {
     h->base_u8 = (row0 == 0) ? ((f & 2u) == 0) : (h->base_u8 && (f & 2u) == 0);
 }
 
+// Flags (*bad = 1) any listed neighbour id >= n (entries j < cnt[row]).
+__global__ void k_check_knng(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ cnt, uint64_t n,
+                             uint32_t K, uint32_t *__restrict__ bad) {
+    const uint64_t row = (uint64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (row >= n) return;
+    const uint32_t c = cnt[row];
+    bool b = false;
+    for (uint32_t j = threadIdx.x & 63u; j < c; j += 64) b |= ids[row * K + j] >= n;
+    if (__ballot(b) && (threadIdx.x & 63u) == 0) atomicOr(bad, 1u);
+}
+
 __global__ void k_l2_pairs(const float *__restrict__ a, const float *__restrict__ x,
                            const uint32_t *__restrict__ ia, const uint32_t *__restrict__ ib,
                            uint64_t npairs, uint32_t d, float *__restrict__ out) {
@@ -262,14 +273,16 @@ static int *tuning_slot(hrnn_dev_t h, const char *key) {
     if (k == "search_wg_per_cu") return &h->tune_search_wg_per_cu;
     if (k == "beam_wg_per_cu") return &h->tune_beam_wg_per_cu;
     if (k == "search_lds_kib") return &h->tune_search_lds_kib;
+    if (k == "stage1_cap") return &h->tune_stage1_cap;
     return nullptr;
 }
 
 int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value) {
     if (!h) return HRNN_E_ARG;
     int *slot = tuning_slot(h, key);
-    if (!slot || value < 1 || value > 64) {
-        h->err = std::string("set_tuning: unknown key or value out of [1, 64]: ") + (key ? key : "");
+    const bool cap = slot == &h->tune_stage1_cap;
+    if (!slot || value < (cap ? 0 : 1) || value > (cap ? 1024 : 64)) {
+        h->err = std::string("set_tuning: unknown key or value out of range: ") + (key ? key : "");
         return HRNN_E_ARG;
     }
     *slot = value;
@@ -308,21 +321,26 @@ int hrnn_dev_set_knng(hrnn_dev_t h, uint32_t K, const uint32_t *ids, const float
         h->knn_cnt.reserve(n * 4);
         h->knn_new.reserve(n * K);
         std::vector<uint32_t> c32(n);
-        bool bad_id = false;
         for (uint64_t i = 0; i < n; i++) {
             REQUIRE(cnt[i] <= K, HRNN_E_ARG, "set_knng: cnt > K");
             c32[i] = cnt[i];
-            const uint32_t *row = ids + i * K;
-            for (uint32_t j = 0; j < cnt[i]; j++) bad_id |= (row[j] >= n);
         }
-        REQUIRE(!bad_id, HRNN_E_ARG, "set_knng: neighbour id >= n");
+        h->csr_valid = false;
+        h->knn_version++;
         HIPC(hipMemcpyAsync(h->knn_ids.p, ids, n * K * 4, hipMemcpyHostToDevice, h->stream));
         HIPC(hipMemcpyAsync(h->knn_d.p, dists, n * K * 4, hipMemcpyHostToDevice, h->stream));
         HIPC(hipMemcpyAsync(h->knn_cnt.p, c32.data(), n * 4, hipMemcpyHostToDevice, h->stream));
         HIPC(hipMemsetAsync(h->knn_new.p, 1, n * K, h->stream));
-        HIPC(hipStreamSynchronize(h->stream));
-        h->csr_valid = false;
-        h->knn_version++;
+        // every listed id < n, checked on the device (the rows are there now)
+        uint32_t *bad = h->counters.as<uint32_t>() + kKnngCheckWord;
+        HIPC(hipMemsetAsync(bad, 0, 4, h->stream));
+        hipLaunchKernelGGL(k_check_knng, dim3(ceil_div(n, 4)), dim3(256), 0, h->stream, h->knn_ids.as<uint32_t>(),
+                           h->knn_cnt.as<uint32_t>(), n, K, bad);
+        HIPC(hipGetLastError());
+        uint32_t flag = 0;
+        d2h(h, &flag, bad, 4);
+        if (flag) h->K = 0;   // no usable graph on the device
+        REQUIRE(!flag, HRNN_E_ARG, "set_knng: neighbour id >= n");
     });
 }
 

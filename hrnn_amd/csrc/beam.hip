@@ -310,7 +310,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     // layer-0 row of the node to be expanded next, loaded while the current
     // expansion merges its candidates
     uint32_t pf_node = kInvalidId, pf_cnt = 0, pf_nb = 0;
-    for (uint32_t iter = 0; iter <= n && pick != kInvalidId; iter++) {   // one expansion per pass
+    const uint32_t max_exp = opt.mlp && opt.max_exp ? opt.max_exp : 0xFFFFFFFFu;
+    for (uint32_t iter = 0; iter <= n && pick != kInvalidId && n_exp < max_exp; iter++) {   // one expansion per pass
         const unsigned long long *lk = s_kl + buf * ef_cap;
         uint8_t *lx = s_xl + buf * ef_cap;
         const uint32_t c = uni((uint32_t)lk[pick]);
@@ -524,7 +525,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             if (lane < (uint32_t)kIn) m_x[lane] = f;
             __syncthreads();
             mlp_forward_wave(opt.mlp, m_x, m_z1, m_h1, m_z2, m_h2, m_lg, m_p, lane);
-            const uint32_t c = uni(pick_class_of(m_p, opt.tau));
+            uint32_t c = uni(pick_class_of(m_p, opt.tau));
+            if (pick != kInvalidId && opt.classes[c] <= opt.ef1) {   // stopped by the cap
+                uint32_t e = c;
+                while (e + 1 < (uint32_t)kOut && opt.classes[e] <= opt.ef1) e++;
+                c = uni(e);
+            }
             const uint32_t cef = opt.classes[c];
             mq = min(max(opt.classes[kOut + c], 1u), m);
             if (lane == 0) {

@@ -32,6 +32,7 @@ struct hrnn_dev {
     int tune_search_wg_per_cu = 8;   // persistent RkNN search workgroups per CU
     int tune_search_lds_kib = 16;    // LDS dedup table of the RkNN search, KiB
     int tune_beam_wg_per_cu = 16;    // persistent beam workgroups (one wave each) per CU
+    int tune_stage1_cap = 0;         // learned early stop: stage-1 expansion cap (0: none)
 
     // ranked KNN graph (SoA); knn_version changes whenever it may have changed
     uint32_t K = 0;
@@ -116,6 +117,7 @@ uint64_t max_list_len(hrnn_dev_t h);
 constexpr uint32_t kQueueWord = 64;        // search work queue (u32 index in counters)
 constexpr uint32_t kQueue2Word = 128;      // second search queue (overlapping launches)
 constexpr uint32_t kListCountWord = 160;   // two list counts of the learned query path
+constexpr uint32_t kKnngCheckWord = 168;   // set_knng's device-side id check flag
 // Optional query subset and stream of a search launch: queries qlist[0 ..
 // *qcount) (device memory) instead of 0 .. nq-1, on `stream` with its work
 // queue at counters[queue_word].
@@ -165,6 +167,10 @@ struct BeamOpt {
     uint32_t ef1 = 0, nq = 0;
     uint32_t *cls = nullptr, *ef_q_out = nullptr, *m_q_out = nullptr, *exp2 = nullptr;
     uint32_t *lists = nullptr, *list_cnt = nullptr, *zero_a = nullptr, *zero_b = nullptr;
+    // with mlp: stop the beam after max_exp expansions (0: no cap); a query
+    // stopped early whose class keeps it at stage 1 takes the first class with
+    // a wider beam instead
+    uint32_t max_exp = 0;
 };
 void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint32_t ef,
                  uint32_t *proxies_dev, uint32_t *expansions_dev, float *raw_dev = nullptr,

@@ -412,6 +412,7 @@ int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
             o1.list_cnt = list_cnt;
             o1.zero_a = h->counters.as<uint32_t>() + kQueueWord;
             o1.zero_b = h->counters.as<uint32_t>() + kQueue2Word;
+            o1.max_exp = (uint32_t)h->tune_stage1_cap;
         }
         beam_launch(h, q_dev, nq, m_max, ef1, proxies_dev, exp1, nullptr, &o1);
         if (!model_dev)   // forced classes

@@ -90,7 +90,12 @@ int hrnn_dev_set_vector_precision(hrnn_dev_t h, int prec);
  * whose scanned entries exceed 3/4 of the table's slots uses a visited bitmap
  * in HBM instead, with identical results); "beam_wg_per_cu" is
  * accepted and unused (phase A launches one workgroup per query); values in
- * [1, 64]. */
+ * [1, 64].  "stage1_cap" (0 .. 1024, default 0 = none): the learned early
+ * stop's stage-1 beam stops after this many expansions, and a query stopped
+ * early whose class would keep it at stage 1 takes the first class with a
+ * wider beam instead (the slow tail of stage 1 moves to stage 2, which
+ * overlaps the other queries' search; results change only for those
+ * queries). */
 int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value);
 int hrnn_dev_get_tuning(hrnn_dev_t h, const char *key, int *value);
 

@@ -115,3 +115,22 @@ def test_adaptive_learned_saves_phase_a_work(c1_index):
     # 300 held-out queries: one query's recall moves the mean by up to 0.0033
     assert rec_l >= rec_f - 0.02
     assert cost_l.mean() <= 0.5 * cost_f.mean()
+    # Stage-1 cap ("stage1_cap"): a query whose ef-16 beam needs more
+    # expansions than the cap leaves stage 1 early and escalates (class with
+    # ef > 16), its answer again its class's plain-path answer; a query that
+    # converges below the cap is untouched.
+    e1 = exp_e[0]                                  # uncapped stage-1 expansions per query
+    cap1 = int(np.percentile(e1, 60))              # an ef-16 beam expands at least ~16 entries
+    dev.set_tuning("stage1_cap", cap1)
+    try:
+        off_c, ids_c, cls_c, _ = model.query(dev, qe, k, Kp, tau)
+    finally:
+        dev.set_tuning("stage1_cap", 0)
+    under, over = e1 < cap1, e1 > cap1
+    assert under.any() and over.any()
+    assert np.array_equal(cls_c[under], cls[under])
+    assert np.all(CLASSES[cls_c[over], 0] > EF1)
+    r_c = per_query_recall(ge_off, ge_ids, off_c, ids_c)
+    assert np.allclose(r_c, rec_e[cls_c, cols], rtol=0, atol=1e-12, equal_nan=True)
+    for qi in np.nonzero(under)[0]:
+        assert np.array_equal(ids_c[off_c[qi]:off_c[qi + 1]], ids[off[qi]:off[qi + 1]])

@@ -236,6 +236,29 @@ def test_search_rejects_bad_k(small_data, small_ref):
         dev.rknn_search(q, np.zeros((q.shape[0], 2), np.uint32), r.K + 1, 10)
 
 
+def test_set_knng_rejects_out_of_range_ids(small_data, small_ref):
+    """Listed ids >= n are refused (checked on the device after the upload);
+    ids past a row's count are not looked at; a valid graph loads again."""
+    x, _ = small_data
+    r, _ = small_ref
+    ids, dists, cnt = r.get_knng()
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    bad = ids.copy()
+    bad[1234, int(cnt[1234]) - 1] = x.shape[0]
+    with pytest.raises(hrnn_amd.HrnnError):
+        dev.set_knng(bad, dists, cnt)
+    with pytest.raises(hrnn_amd.HrnnError):
+        dev.nndescent(1, 8)   # no usable graph after the refusal
+    tail = ids.copy()
+    short = cnt.copy()
+    short[7] = cnt[7] - 1
+    tail[7, int(short[7])] = 0xFFFFFFFF   # past the row's count: ignored
+    dev.set_knng(tail, dists, short)
+    dev.set_knng(ids, dists, cnt)
+    dev.build_rknn_csr()
+
+
 # ---- brute force ---------------------------------------------------------------------
 
 @pytest.mark.parametrize("prec", ["bf16", "f16", "f32"])
