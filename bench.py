@@ -31,9 +31,13 @@ value = the highest QPS among the paths whose macro recall is >= 0.95;
 speedup_vs_fp32_fixed_m sets it against the plain FP32 fixed-m GPU path,
 bf16_learned_speedup_vs_fp32_fixed_m the BF16 learned path.
 
-roofline: the dominant GPU kernel of that path (k_phase_a, k_phase_a_stage2
-or k_rknn_search), timed with HIP events on the library's stream over
-resident launches (algorithmic bytes from the kernels' own work counters).
+roofline: the dominant GPU kernel of that path (k_phase_a or k_rknn_search),
+timed with HIP events on the library's stream over resident launches
+(algorithmic bytes from the kernels' own work counters).  For the learned
+path k_phase_a is its stage-1 beam as the query runs it (ef 16, features,
+MLP, class pick and cap fused), launched alone; the stage-2 beams run inside
+the call, overlapping the other queries' search, and are reported by their
+work, not timed alone.
 cpu_baseline: the reference's own searchRknn (compiled from its sources,
 oracle/_ref) on this run's index loaded from its saved file, OpenMP over all
 queries on the host cores; cpu_baseline_one_thread: its hrnn_search driver
@@ -181,10 +185,10 @@ def main() -> int:
     B = min(nq, 1000)                      # queries per pipelined batch
     qd = hiprt.DeviceArray.from_host(q)
     pd = hiprt.DeviceArray((nq * m_max,), np.uint32)
-    fd = hiprt.DeviceArray((nq * 32,), np.float32)
     cd = hiprt.DeviceArray((nq,), np.uint32)
     idd = hiprt.DeviceArray((nq * cap,), np.uint32)
     wd = hiprt.DeviceArray((2 * nq,), np.uint32)
+    fd = hiprt.DeviceArray((nq * 32,), np.float32)    # learned-m features (host phase A path)
     md = hiprt.DeviceArray((nq,), np.uint32)
 
     def ck(rc):
@@ -445,11 +449,17 @@ def main() -> int:
 
     adaptive = best_name.endswith("_learned_adaptive")
     if adaptive:
-        # stage 1 alone (k_phase_a<bf16, false> at EF1, features fused), then the
-        # whole two-stage query, which leaves the masked proxies (stride 16) in pd
-        s1_ms = ev_ms(lambda: ck(L.hrnn_dev_stage1_features_dptr(h, qd.ptr, nq, EF1, 16, k, pd.ptr, fd.ptr,
-                                                                 ed.ptr)))
-        ex = ed.to_host().reshape(nq, 2).astype(np.int64)
+        # stage 1 alone as the query runs it (k_phase_a<VT, false> at EF1 with
+        # the fused features, MLP, class pick and stage-1 cap: phase A only and
+        # ef_max = EF1), then the whole two-stage query, which leaves the
+        # masked proxies (stride 16) in pd
+        def stage1_only():
+            ck(L.hrnn_dev_rknn_query_adaptive_dptr(h, qd.ptr, nq, model_ad.ptr, classes_d.ptr, EF1, tau_a,
+                                                   EF1, 16, k, Kp, 1.0, pd.ptr, cls_d.ptr, None, None,
+                                                   cap, None, exp_d.ptr))
+
+        s1_ms = ev_ms(stage1_only)
+        ex = exp_d.to_host()[:2 * nq].reshape(nq, 2).astype(np.int64)
         # per query: the query, each expanded row, each evaluated vector, 16
         # proxy ids, 16 r_k reads (count + distance) and 32 features written
         s1_bytes = float((4 * d + ex[:, 0] * 4 * (2 * cfg["M"] + 1) + ex[:, 1] * s_v * d + 4 * 16 + 16 * 8
@@ -504,14 +514,14 @@ def main() -> int:
     kern["k_rknn_search"] = {"ms": search_ms, "bytes": search_bytes,
                              "scanned_per_query": round(float(wk[:, 0].mean()), 1),
                              "candidates_per_query": round(float(wk[:, 1].mean()), 1)}
+    adaptive_call = None
     if adaptive:
-        # the stage-2 beams (k_phase_a<bf16, true>): whole-call time less stage 1
-        # and the search (so it also holds the MLP and class-pick launches)
-        kern["k_phase_a_stage2"] = {"ms": max(tot_ms - s1_ms - search_ms, 1e-6), "bytes": s2_bytes,
-                                    "queries": int(esc.sum()),
-                                    "expansions_per_escalated_query":
-                                        round(float(exq[1, esc, 0].mean()) if esc.any() else 0.0, 1),
-                                    "whole_call_ms": round(tot_ms, 4)}
+        # the stage-2 beams (k_phase_a<VT, true>) overlap the search of the
+        # other queries inside the call: their work, and the whole call's time
+        adaptive_call = {"whole_call_ms": round(tot_ms, 4), "stage2_queries": int(esc.sum()),
+                         "stage2_expansions_per_query":
+                             round(float(exq[1, esc, 0].mean()) if esc.any() else 0.0, 1),
+                         "stage2_alg_bytes": int(s2_bytes)}
     for kk, v in kern.items():
         v["gbs"] = v["bytes"] / (v["ms"] * 1e-3) / 1e9
         log(f"  {kk}: {v['ms']:.3f} ms/launch over {nq} queries, {v['gbs']:.1f} GB/s algorithmic")
@@ -651,7 +661,8 @@ def main() -> int:
                      "kernel_ms": round(kern[dom]["ms"], 4), "alg_bytes_per_launch": int(kern[dom]["bytes"]),
                      "queries_per_launch": nq,
                      "kernels": {kk: {k2: (round(v2, 4) if isinstance(v2, float) else v2) for k2, v2 in v.items()}
-                                 for kk, v in kern.items()}},
+                                 for kk, v in kern.items()},
+                     "adaptive_call": adaptive_call},
         "phase_a": {"gpu_host_proxy_agreement": round(agree, 5), "host_threads": threads,
                     "host_sec_all_queries_m_max": round(phase_a_s, 3), "host_qps": round(nq / phase_a_s, 1)},
         "cpu_baseline": cpu,
@@ -671,7 +682,7 @@ def main() -> int:
                             "knn_recall_at10_sample": round(knn_rec10, 4),
                             "per_iter": [{k2: (round(v, 2) if isinstance(v, float) else v)
                                           for k2, v in s.items()} for s in st]},
-        # the other rows' kernels on this run: reverse CSR (HBM-bound, SURVEY's
+        # the other kernels of the path on this run: reverse CSR (HBM-bound,
         # 12 N K + 16 (N + 1) bytes), exact r_k brute force (MFMA, 2 N^2 d
         # flops), NNDescent (distance evaluations, 2 d flops each)
         "build_rooflines": {

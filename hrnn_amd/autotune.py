@@ -28,14 +28,12 @@ CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "cach
 
 
 def _gpu_name() -> str:
+    # the HIP runtime's device name: no child process (a child of a process
+    # that holds the GPU must not exec another program)
     try:
-        out = os.popen("rocm-smi --showproductname 2>/dev/null").read()
-        for line in out.splitlines():
-            if "Card Series" in line or "Card series" in line:
-                return line.split(":")[-1].strip()
+        return hiprt.device_name(0)
     except Exception:
-        pass
-    return "gpu"
+        return "gpu"
 
 
 def _key(shape: dict) -> str:

@@ -375,8 +375,9 @@ int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
         REQUIRE(m_max >= 1 && m_max <= 1024, HRNN_E_ARG, "query_adaptive: m_max must be in [1, 1024]");
         REQUIRE(ef1 >= 1 && ef_max >= ef1 && ef_max <= 512, HRNN_E_ARG,
                 "query_adaptive: need 1 <= ef1 <= ef_max <= 512");
-        REQUIRE(q_dev && classes_dev && proxies_dev && cls_dev && res_cnt_dev && res_ids_dev, HRNN_E_ARG,
+        REQUIRE(q_dev && classes_dev && proxies_dev && cls_dev && (!res_cnt_dev || res_ids_dev), HRNN_E_ARG,
                 "query_adaptive: null argument");
+        const bool search = res_cnt_dev != nullptr;   // NULL: phase A only
         if (!nq) return;
         h->s8.reserve((uint64_t)nq * (kIn + kOut + 4) * 4);
         float *feats = h->s8.as<float>();
@@ -424,17 +425,19 @@ int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
         // Phases B+C of the queries that stay at stage 1 run on the second
         // stream while stage 2 and then the escalated queries' phases B+C run
         // on the first; the call ends when both are done.
-        HIPC(hipEventRecord(h->ev_a, h->stream));
-        HIPC(hipStreamWaitEvent(h->stream2, h->ev_a, 0));
-        SearchSel rest;
-        rest.qlist = lists + nq;
-        rest.qcount = list_cnt + 1;
-        rest.stream = h->stream2;
-        rest.queue_word = kQueue2Word;
-        rest.queue_zeroed = true;
-        rknn_search_resident(h, q_dev, nq, proxies_dev, m_max, nullptr, k, K_prime, threshold, res_cnt_dev,
-                             res_ids_dev, cap_per_q, work_dev, &rest);
-        HIPC(hipEventRecord(h->ev_b, h->stream2));
+        if (search) {
+            HIPC(hipEventRecord(h->ev_a, h->stream));
+            HIPC(hipStreamWaitEvent(h->stream2, h->ev_a, 0));
+            SearchSel rest;
+            rest.qlist = lists + nq;
+            rest.qcount = list_cnt + 1;
+            rest.stream = h->stream2;
+            rest.queue_word = kQueue2Word;
+            rest.queue_zeroed = true;
+            rknn_search_resident(h, q_dev, nq, proxies_dev, m_max, nullptr, k, K_prime, threshold, res_cnt_dev,
+                                 res_ids_dev, cap_per_q, work_dev, &rest);
+            HIPC(hipEventRecord(h->ev_b, h->stream2));
+        }
         if (ef_max > ef1) {   // stage 2: one workgroup per escalated query
             BeamOpt o2;
             o2.ef_q = ef_q;
@@ -445,6 +448,7 @@ int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
         }
         // the escalated queries' search overlaps the other one too: its own
         // fallback buffers and a few workgroups
+        if (!search) return;
         SearchSel esc;
         esc.qlist = lists;
         esc.qcount = list_cnt;

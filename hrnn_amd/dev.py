@@ -385,10 +385,13 @@ class Device:
         return fd.to_host(), pd.to_host(), ed.to_host()
 
     def query_adaptive(self, q, classes, ef1: int, m_max: int, k: int, K_prime: int, model=None,
-                       tau: float = 0.0, cls=None, threshold: float = 1.0, cap_per_q: int = 4096):
+                       tau: float = 0.0, cls=None, threshold: float = 1.0, cap_per_q: int = 4096,
+                       phase_a_only: bool = False):
         """Two-stage learned query (hrnn_dev_rknn_query_adaptive_dptr).  classes:
         [16, 2] (ef, m) rows.  With model None, cls gives each query's class.
-        Returns (res_off[nq+1], res_ids, cls[nq], proxies[nq, m_max])."""
+        Returns (res_off[nq+1], res_ids, cls[nq], proxies[nq, m_max]); with
+        phase_a_only (no result buffers: phases B and C are skipped) res_off
+        and res_ids are None."""
         from . import hiprt
         q = np.ascontiguousarray(q, np.float32)
         nq = q.shape[0]
@@ -403,9 +406,11 @@ class Device:
         ri = hiprt.DeviceArray((nq, cap_per_q), np.uint32)
         self._ck(self.L.hrnn_dev_rknn_query_adaptive_dptr(
             self.h, qd.ptr, nq, None if md is None else md.ptr, cd.ptr, int(classes[:, 0].max()),
-            tau, ef1, m_max, k, K_prime, threshold, pd.ptr, cl.ptr, rc.ptr, ri.ptr, cap_per_q, None,
-            None))
+            tau, ef1, m_max, k, K_prime, threshold, pd.ptr, cl.ptr, None if phase_a_only else rc.ptr,
+            None if phase_a_only else ri.ptr, cap_per_q, None, None))
         self.sync()
+        if phase_a_only:
+            return None, None, cl.to_host(), pd.to_host()
         cnt = rc.to_host().astype(np.uint64)
         if cnt.max(initial=0) > cap_per_q:
             raise HrnnError(-5, "query_adaptive: a query has more results than cap_per_q")

@@ -37,6 +37,16 @@ def hip() -> C.CDLL:
     return _hip
 
 
+def device_name(device: int = 0) -> str:
+    """The HIP device name (hipDeviceGetName), without starting any process."""
+    buf = C.create_string_buffer(256)
+    h = hip()
+    h.hipDeviceGetName.argtypes = [C.c_char_p, C.c_int, C.c_int]
+    if h.hipDeviceGetName(buf, 256, device) != 0:
+        return "gpu"
+    return buf.value.decode(errors="replace").strip() or "gpu"
+
+
 def _ck(rc: int, what: str):
     if rc != 0:
         raise RuntimeError(f"{what}: {hip().hipGetErrorString(rc).decode()}")

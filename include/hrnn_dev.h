@@ -293,7 +293,9 @@ int hrnn_dev_stage1_features_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq,
  * model_dev NULL the classes are read from cls_dev instead (forced classes).
  * work_dev as in hrnn_dev_rknn_search_dptr; expansions_dev (NULL or 4*nq u32)
  * receives per query the stage-1 (expansions, evaluations) in [0, 2nq) and
- * the stage-2 ones in [2nq, 4nq) (zero without a stage 2).
+ * the stage-2 ones in [2nq, 4nq) (zero without a stage 2).  With res_cnt_dev
+ * NULL only phase A runs (the classes and masked proxies are the outputs;
+ * ef_max = ef1 leaves stage 1 alone, as the query runs it).
  * Asynchronous on the library stream; nothing is copied to the host. */
 int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq,
                                       const float *model_dev, const uint32_t *classes_dev,

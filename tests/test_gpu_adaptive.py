@@ -61,6 +61,9 @@ def test_adaptive_forced_classes_equal_per_class_runs(c1_index):
     cls = rng.integers(0, 16, q.shape[0]).astype(np.uint32)
     off, ids, cls_out, prox = dev.query_adaptive(q, CLASSES, EF1, 16, 10, 100, cls=cls)
     assert np.array_equal(cls_out, cls)
+    # phase A alone (no result buffers) leaves the same classes and proxies
+    _, _, cls_a, prox_a = dev.query_adaptive(q, CLASSES, EF1, 16, 10, 100, cls=cls, phase_a_only=True)
+    assert np.array_equal(cls_a, cls) and np.array_equal(prox_a, prox)
     for c in range(16):
         sel = np.nonzero(cls == c)[0]
         if sel.size == 0:

@@ -41,7 +41,8 @@ def test_phase_a_matches_host_beam_small(small_index, m, ef):
 
 
 def test_phase_a_and_query_c1_scale():
-    """configs[0]-sized index (10k x 128, K=100): 2000 queries with m=64,
+    """An index of the reference's small configuration (10k x 128, K=100):
+    2000 queries with m=64,
     ef=200 on the GPU beam vs the host beam, then the whole resident query
     (GPU phase A + phases B and C in one call) vs host phase A + GPU B and C."""
     from hrnn_amd import hiprt
