@@ -186,6 +186,14 @@ __global__ __launch_bounds__(256) void k_bf_mfma(
     }
     __syncthreads();
 
+    float qn_r[FR][4], thr_r[FR][4];   // this lane's rows: squared norms, running C-th best
+#pragma unroll
+    for (int f = 0; f < FR; f++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            qn_r[f][i] = s_qn[16 * (FR * w + f) + 4 * (lane >> 4) + i];
+            thr_r[f][i] = FLT_MAX;
+        }
     // Base tiles are prefetched into registers one tile ahead (d <= 256: the
     // whole next tile, 64 columns x d dims, KS / 2 chunks of two 16-byte
     // loads per thread) or one chunk ahead (wider rows), so the global-load
@@ -267,7 +275,16 @@ __global__ __launch_bounds__(256) void k_bf_mfma(
                 }
             }
         }
-        // epilogue: lane holds rows 16 (FR w + f) + 4*(lane/16) + i, column 16cb + lane%16
+        // epilogue: lane holds rows 16 (FR w + f) + 4*(lane/16) + i, column
+        // 16cb + lane%16; row norms and thresholds come from registers, the
+        // four column norms (and r_k) are read once per tile
+        float xn_c[4], kth_c[4];
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++) {
+            const uint64_t col = c0 + 16 * cb + (lane & 15u);
+            xn_c[cb] = s_xn[16 * cb + (lane & 15u)];
+            kth_c[cb] = (MODE == 1 && col < n) ? kth[col] : 0.f;
+        }
         for (int half = 0; half < 2; half++) {
 #pragma unroll
             for (int cbh = 0; cbh < 2; cbh++) {
@@ -280,11 +297,17 @@ __global__ __launch_bounds__(256) void k_bf_mfma(
                 for (int i = 0; i < 4; i++) {
                     int rl = 16 * (FR * w + f) + 4 * (lane >> 4) + i;
                     if (col < n && row0 + rl < nq) {
-                        float dv = s_qn[rl] + s_xn[cl] - 2.f * acc[f][cb][i];
+                        float dv = qn_r[f][i] + xn_c[cb] - 2.f * acc[f][cb][i];
                         dv = dv > 0.f ? dv : 0.f;
                         if (MODE == 0) {
-                            top_offer(st, rl, dv, (uint32_t)col);
-                        } else if (dv <= kth[col]) {
+                            // columns arrive in increasing id, so an equal
+                            // distance never displaces a kept id
+                            if (dv < thr_r[f][i]) {
+                                const uint32_t t = atomicAdd(&st.st_n[rl], 1u);
+                                st.st_d[rl][t] = dv;
+                                st.st_i[rl][t] = (uint32_t)col;
+                            }
+                        } else if (dv <= kth_c[cb]) {
                             uint64_t q = row0 + rl;
                             uint32_t s = atomicAdd(&hit_cnt[q], 1u);
                             if (s < cap) {
@@ -295,7 +318,13 @@ __global__ __launch_bounds__(256) void k_bf_mfma(
                     }
                 }
             }
-            if (MODE == 0) top_merge(st);
+            if (MODE == 0) {
+                top_merge(st);
+#pragma unroll
+                for (int f = 0; f < FR; f++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) thr_r[f][i] = st.thr[16 * (FR * w + f) + 4 * (lane >> 4) + i];
+            }
         }
     }
     if (MODE == 0) top_store(st, row0, nq, cand_i, cand_n);
