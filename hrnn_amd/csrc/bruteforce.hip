@@ -484,9 +484,12 @@ template <class T, int C, int MODE>
 static void launch_mfma(hrnn_dev_t h, const T *qa, const float *qn, uint32_t nq, const T *xb,
                         uint32_t *ci, uint32_t *cn, const float *kth, uint32_t *hc, uint32_t *hi,
                         float *hd, uint32_t cap) {
-    // two row fragments per wave for the top-32 pass (128 rows per block: half
-    // the base-tile traffic per MFMA); one otherwise (LDS of the top state)
-    constexpr int FR = (MODE == 0 && C <= 32) ? 2 : 1;
+    // One 16-row fragment per wave (64 rows per block): the top state's LDS
+    // then allows three blocks per CU, which measured faster on the 1M x 1M
+    // r_k pass (1.05 s) than two fragments (128 rows per block, half the
+    // base-tile traffic per MFMA, two blocks per CU: 1.11 s).  The kernel
+    // keeps FR as a template parameter.
+    constexpr int FR = 1;
     unsigned grid = ceil_div(nq, 64 * FR);
     uint32_t KS = h->d / 32;
 #define L(ks)                                                                                     \
