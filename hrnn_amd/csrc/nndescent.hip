@@ -26,7 +26,8 @@
 //       buffer (reported as dropped).
 //   k_nnd_merge: per node, proposals in chunks of 2048: keep those below the
 //       current worst entry, sort them by (d2, id), drop duplicates and ids
-//       already listed, merge with the sorted list
+//       already listed (found in the run of equal list distances at the
+//       proposal's rank), merge with the sorted list
 //       (existing entries first on equal distance, as the reference's strict-<
 //       insertion leaves them); the first K survive, inserted entries flagged
 //       new.  Chunk after chunk this is the top-K of (list U all proposals).
@@ -508,7 +509,6 @@ __global__ __launch_bounds__(256) void k_nnd_join(
 constexpr uint32_t kMergeChunk = 2048;   // proposals per merge round (LDS sort)
 constexpr uint32_t kMergeThreads = 256;  // k_nnd_merge's workgroup size
 constexpr uint32_t kMergeMaxK = 1024;
-constexpr uint32_t kHashSlots = 2048;
 
 // Exclusive scan of flags[0..P) in place (P <= kMergeChunk, 256 threads);
 // flags[P] receives the total.
@@ -550,7 +550,6 @@ __global__ __launch_bounds__(kMergeThreads) void k_nnd_merge(
     __shared__ uint32_t lid[kMergeMaxK], oid[kMergeMaxK];
     __shared__ float ld[kMergeMaxK], od[kMergeMaxK];
     __shared__ uint8_t lnw[kMergeMaxK], onw[kMergeMaxK];
-    __shared__ uint32_t hash[kHashSlots];
     __shared__ uint32_t wsum[4];
     __shared__ uint32_t s_nk;
 
@@ -600,42 +599,55 @@ __global__ __launch_bounds__(kMergeThreads) void k_nnd_merge(
         __syncthreads();
         const uint32_t nk = s_nk;
         if (nk == 0) continue;   // uniform
-        // ids currently listed
-        for (uint32_t t = threadIdx.x; t < kHashSlots; t += kMergeThreads) hash[t] = kInvalidId;
-        __syncthreads();
-        for (uint32_t t = threadIdx.x; t < c; t += kMergeThreads) {
-            uint32_t v = lid[t];
-            uint32_t h = (v * 2654435761u) & (kHashSlots - 1);
-            for (;;) {
-                uint32_t old = atomicCAS(&hash[h], kInvalidId, v);
-                if (old == kInvalidId || old == v) break;
-                h = (h + 1) & (kHashSlots - 1);
-            }
-        }
         uint32_t PP = 256;   // the register sort's smallest size
         while (PP < nk) PP <<= 1;
         for (uint32_t t = nk + threadIdx.x; t < PP; t += kMergeThreads) keys[t] = ~0ull;
-        __syncthreads();   // also completes the list hash
+        __syncthreads();
         if (PP == 256) block_sort_u64_regs<1>(keys, PP);
         else if (PP == 512) block_sort_u64_regs<2>(keys, PP);
         else if (PP == 1024) block_sort_u64_regs<4>(keys, PP);
         else block_sort_u64_regs<8>(keys, PP);
         __syncthreads();
-        // a proposal counts once: not a repeat of the previous key (the same
-        // pair always has the same distance) and not already listed
-        for (uint32_t t = threadIdx.x; t < PP; t += kMergeThreads) {
-            unsigned long long key = keys[t];
+        // Each of the thread's keys (positions t = j * 256 + tid): lo = the
+        // number of list entries with distance <= its own (its merge rank in
+        // the list), by branchless binary searches run in lockstep.  A
+        // proposal counts once: not a repeat of the previous key (the same
+        // pair always has the same distance) and not already listed -- a
+        // listed id has exactly this distance, so it sits in the run of
+        // entries equal to it just below lo.
+        constexpr uint32_t kPerT = kMergeChunk / kMergeThreads;
+        const uint32_t np_t = PP / kMergeThreads;   // keys per thread, uniform
+        uint32_t top = 1;                           // largest power of two <= c (1 when c <= 1)
+        while (2 * top <= c) top <<= 1;
+        unsigned long long kj[kPerT];
+        uint32_t loj[kPerT];
+#pragma unroll
+        for (uint32_t j = 0; j < kPerT; j++) {
+            kj[j] = j < np_t ? keys[j * kMergeThreads + threadIdx.x] : ~0ull;
+            loj[j] = 0;
+        }
+        for (uint32_t step = top; step > 0; step >>= 1) {
+#pragma unroll
+            for (uint32_t j = 0; j < kPerT; j++) {
+                const float dv = key_float((uint32_t)(kj[j] >> 32));
+                if (kj[j] != ~0ull && loj[j] + step <= c && ld[loj[j] + step - 1] <= dv) loj[j] += step;
+            }
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kPerT; j++) {
+            if (j >= np_t) break;
+            const uint32_t t = j * kMergeThreads + threadIdx.x;
+            const unsigned long long key = kj[j];
             uint32_t ok = 0;
             if (key != ~0ull && (t == 0 || keys[t - 1] != key)) {
-                uint32_t v = (uint32_t)key;
-                uint32_t h = (v * 2654435761u) & (kHashSlots - 1);
+                const float dv = key_float((uint32_t)(key >> 32));
+                const uint32_t id = (uint32_t)key;
                 ok = 1;
-                for (;;) {
-                    uint32_t e = hash[h];
-                    if (e == kInvalidId) break;
-                    if (e == v) { ok = 0; break; }
-                    h = (h + 1) & (kHashSlots - 1);
-                }
+                for (int e = (int)loj[j] - 1; e >= 0 && ld[e] == dv; e--)
+                    if (lid[e] == id) {
+                        ok = 0;
+                        break;
+                    }
             }
             scan[t] = ok;
         }
@@ -644,32 +656,41 @@ __global__ __launch_bounds__(kMergeThreads) void k_nnd_merge(
         const uint32_t nvalid = scan[PP];
         if (nvalid == 0) continue;   // uniform
         // proposals: pos = #valid before + #listed with dist <= d
-        for (uint32_t t = threadIdx.x; t < PP; t += kMergeThreads) {
-            unsigned long long key = keys[t];
-            if (key == ~0ull || scan[t + 1] == scan[t]) continue;
-            float dv = key_float((uint32_t)(key >> 32));
-            uint32_t lo = 0, hi = c;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (ld[mid] <= dv) lo = mid + 1; else hi = mid;
-            }
-            uint32_t pos = scan[t] + lo;
+#pragma unroll
+        for (uint32_t j = 0; j < kPerT; j++) {
+            if (j >= np_t) break;
+            const uint32_t t = j * kMergeThreads + threadIdx.x;
+            if (kj[j] == ~0ull || scan[t + 1] == scan[t]) continue;
+            const uint32_t pos = scan[t] + loj[j];
             if (pos < K) {
-                oid[pos] = (uint32_t)key;
-                od[pos] = dv;
+                oid[pos] = (uint32_t)kj[j];
+                od[pos] = key_float((uint32_t)(kj[j] >> 32));
                 onw[pos] = 1;
                 upd++;
             }
         }
         // listed entries: pos = e + #valid proposals with dist < ld[e]
-        for (uint32_t e = threadIdx.x; e < c; e += kMergeThreads) {
-            uint32_t kf = float_key(ld[e]);
-            uint32_t lo = 0, hi = PP;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi) >> 1;
-                if ((uint32_t)(keys[mid] >> 32) < kf) lo = mid + 1; else hi = mid;
+        // (branchless searches over the sorted keys, in lockstep; padding
+        // keys are ~0 and never counted)
+        constexpr uint32_t kLPerT = kMergeMaxK / kMergeThreads;
+        uint32_t kfe[kLPerT], loe[kLPerT];
+#pragma unroll
+        for (uint32_t j = 0; j < kLPerT; j++) {
+            const uint32_t e = j * kMergeThreads + threadIdx.x;
+            kfe[j] = e < c ? float_key(ld[e]) : 0u;
+            loe[j] = 0;
+        }
+        for (uint32_t step = PP; step > 0; step >>= 1) {
+#pragma unroll
+            for (uint32_t j = 0; j < kLPerT; j++) {
+                if (loe[j] + step <= PP && (uint32_t)(keys[loe[j] + step - 1] >> 32) < kfe[j]) loe[j] += step;
             }
-            uint32_t pos = e + scan[lo];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kLPerT; j++) {
+            const uint32_t e = j * kMergeThreads + threadIdx.x;
+            if (e >= c) continue;
+            const uint32_t pos = e + scan[loe[j]];
             if (pos < K) {
                 oid[pos] = lid[e];
                 od[pos] = ld[e];
