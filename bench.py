@@ -549,7 +549,7 @@ def main() -> int:
     # All host cores (OpenMP over queries, the reference's const re-entrant
     # searchRknn) on every query, plus one thread (the hrnn_search driver loop)
     # on a bounded sample.
-    cpu, cpu1 = None, None
+    cpu, cpu1, csr_cpu, bf_cpu = None, None, None, None
     if not args.no_cpu_baseline:
         try:
             import oracle
@@ -573,6 +573,15 @@ def main() -> int:
                     r.search(qq, m_star, 1.0, k, Kp, ef, nthreads=1)
                     spent += time.perf_counter() - t1
                     done += qq.shape[0]
+                # the reverse CSR the reference way (serial count, fill and
+                # per-list sorts) on the same graph
+                t1 = time.perf_counter()
+                r.build_rknng()
+                csr_cpu = {"value": round(time.perf_counter() - t1, 3), "unit": "s", "cores": 1,
+                           "kind": "reference",
+                           "sample": f"the reference's buildRKNNG (serial) on this run's {n} x {K} graph, "
+                                     f"loaded from its saved file",
+                           "gpu_kernels_s": round(csr_dev_s, 4)}
                 r.close()
                 cpu = {"value": round(nq / all_s, 2), "unit": "queries/s", "cores": threads,
                        "kind": "reference",
@@ -585,6 +594,31 @@ def main() -> int:
                 log(f"cpu baseline: {cpu['value']} QPS on {threads} threads; {cpu1['value']} QPS on one")
         except Exception as e:  # reported when available, never required
             log(f"cpu baseline skipped: {e}")
+
+    # ---- brute-force r_k baseline: the oracle's C restatement on a row sample -------------
+    # (the reference's gen_rknn_gt.py is PyTorch and is not run here); r_k of
+    # the first rows against all n points, OpenMP over rows, projected to n rows
+    if not args.no_cpu_baseline:
+        try:
+            import oracle
+            oracle.set_threads(threads)
+            rows = 256
+            t1 = time.perf_counter()
+            _, d2s = oracle.knn(x, x[:rows], k + 1)
+            bf_s = time.perf_counter() - t1
+            kth_rows = kth[np.argsort(labels)[:rows]]          # GPU r_k of those rows (internal ids)
+            bf_cpu = {"value": round(bf_s, 3), "unit": f"s for {rows} rows", "cores": threads, "kind": "port",
+                      "equal_to_gpu_rk": bool(np.array_equal(d2s[:, k], kth_rows)),
+                      "sample": f"r_k (k={k}) of the first {rows} points against all {n} by the oracle's C "
+                                f"brute force (FP32, the reference's 16-lane order), OpenMP on {threads} "
+                                f"host threads",
+                      "projected_all_rows_s": round(bf_s * n / rows, 1),
+                      "gpu_all_rows_s": round(kth_s, 3),
+                      "speedup_vs_projected": round(bf_s * n / rows / kth_s, 1)}
+            log(f"brute-force baseline: {rows} rows in {bf_s:.2f}s on {threads} threads; projected "
+                f"{bf_s * n / rows:.0f}s for all rows vs GPU {kth_s:.2f}s")
+        except Exception as e:
+            log(f"brute-force baseline skipped: {e}")
 
     # ---- NNDescent build baseline: the reference's refineKNNG on the host cores -------------
     # Bounded sample: the first iteration (the heaviest: every list entry is
@@ -678,6 +712,8 @@ def main() -> int:
         "learned_m": learned_info,
         "autotune": {kk: v for kk, v in tuned.items() if kk != "timings_ms"},
         "nndescent_cpu_baseline": nnd_cpu,
+        "csr_cpu_baseline": csr_cpu,
+        "bruteforce_cpu_baseline": bf_cpu,
         "nndescent_build": {"seconds": round(nnd_s, 3), "iters": cfg["iters"], "sample": cfg["S"],
                             "knn_recall_at10_sample": round(knn_rec10, 4),
                             "per_iter": [{k2: (round(v, 2) if isinstance(v, float) else v)
