@@ -107,6 +107,35 @@ __device__ void top_merge(TopState<C, R> &s) {
     __syncthreads();
 }
 
+// The same merge for the 16 rows of one wave (rows 16 w .. 16 w + 15, one
+// lane each), without block barriers: only this wave stages candidates for
+// these rows, and one wave's LDS accesses are performed in program order.
+template <int C, int R>
+__device__ void top_merge_wave(TopState<C, R> &s, uint32_t w, uint32_t lane) {
+    if (lane >= 16) return;
+    const int r = (int)(16 * w + lane);
+    const uint32_t ns = s.st_n[r];
+    if (!ns) return;
+    uint32_t n = s.top_n[r];
+    for (uint32_t t = 0; t < ns; t++) {
+        float dv = s.st_d[r][t];
+        uint32_t iv = s.st_i[r][t];
+        if (n == C && !pair_less(dv, iv, s.top_d[r][C - 1], s.top_i[r][C - 1])) continue;
+        uint32_t p = (n == C) ? C - 1 : n;
+        if (n < C) n++;
+        while (p > 0 && pair_less(dv, iv, s.top_d[r][p - 1], s.top_i[r][p - 1])) {
+            s.top_d[r][p] = s.top_d[r][p - 1];
+            s.top_i[r][p] = s.top_i[r][p - 1];
+            p--;
+        }
+        s.top_d[r][p] = dv;
+        s.top_i[r][p] = iv;
+    }
+    s.top_n[r] = n;
+    s.st_n[r] = 0;
+    s.thr[r] = (n == C) ? s.top_d[r][C - 1] : FLT_MAX;
+}
+
 template <int C, int R>
 __device__ __forceinline__ void top_offer(TopState<C, R> &s, int r, float dv, uint32_t col) {
     // columns arrive in increasing id, so an equal distance never displaces a kept id
@@ -319,7 +348,9 @@ __global__ __launch_bounds__(256) void k_bf_mfma(
                 }
             }
             if (MODE == 0) {
-                top_merge(st);
+                // each wave merges its own rows (FR fragments of 16): no block barrier
+#pragma unroll
+                for (int f = 0; f < FR; f++) top_merge_wave(st, FR * w + f, lane);
 #pragma unroll
                 for (int f = 0; f < FR; f++)
 #pragma unroll
@@ -327,7 +358,10 @@ __global__ __launch_bounds__(256) void k_bf_mfma(
             }
         }
     }
-    if (MODE == 0) top_store(st, row0, nq, cand_i, cand_n);
+    if (MODE == 0) {
+        __syncthreads();   // every wave's rows are merged
+        top_store(st, row0, nq, cand_i, cand_n);
+    }
 }
 
 // FP32 VALU candidate pass: 4x4 micro-tile per lane, direct differences.
