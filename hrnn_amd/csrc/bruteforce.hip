@@ -188,7 +188,11 @@ __global__ __launch_bounds__(256) void k_bf_mfma(
     typedef typename Mfma<T>::vec vec;
     constexpr int RB = 64 * FR;
     __shared__ TopState<C, RB> st;
-    __shared__ __attribute__((aligned(16))) T bs[kCT * kCHP];
+    // d <= 128 (KS <= 4): the whole base tile (64 columns x d) in LDS, one
+    // barrier pair per tile; wider rows: one 64-dim chunk at a time
+    constexpr bool kOneStage = KS <= 4;
+    constexpr int kBSP = kOneStage ? KS * 32 + 8 : kCHP;   // bs row stride (elements)
+    __shared__ __attribute__((aligned(16))) T bs[kCT * kBSP];
     __shared__ float s_xn[kCT];
     __shared__ float s_qn[RB];
 
@@ -256,7 +260,7 @@ __global__ __launch_bounds__(256) void k_bf_mfma(
             const int u = threadIdx.x + v2 * 256;
             if (u < kCT * (dims / 8)) {
                 const int col = u / (dims / 8), part = u % (dims / 8);
-                *reinterpret_cast<vec *>(&bs[col * kCHP + part * 8]) = pv[b + v2];
+                *reinterpret_cast<vec *>(&bs[col * kBSP + (kOneStage ? ch * 64 : 0) + part * 8]) = pv[b + v2];
             }
         }
     };
@@ -277,7 +281,8 @@ __global__ __launch_bounds__(256) void k_bf_mfma(
         const uint64_t c1 = c0 + kCT;   // next tile
 #pragma unroll
         for (int ch = 0; ch < NCH; ch++) {
-            __syncthreads();   // the previous chunk's MFMAs (and epilogue) are done with bs and s_xn
+            if (!kOneStage || ch == 0)
+                __syncthreads();   // the previous chunk's MFMAs (and epilogue) are done with bs and s_xn
             if (ch == 0 && threadIdx.x < kCT) s_xn[threadIdx.x] = pxn;
             if (kWholeTile) {
                 put(ch, 2 * ch);
@@ -291,17 +296,23 @@ __global__ __launch_bounds__(256) void k_bf_mfma(
                 if (ch == 0 && threadIdx.x < kCT && c1 < n)
                     pxn = (c1 + threadIdx.x < n) ? xn[c1 + threadIdx.x] : 0.f;
             }
+            if (kOneStage && ch + 1 < NCH) continue;   // stage every chunk, then one barrier
             __syncthreads();
 #pragma unroll
+            for (int kc = 0; kc < (kOneStage ? NCH : 1); kc++) {
+            const int cc = kOneStage ? kc : ch;   // chunk the MFMAs read
+#pragma unroll
             for (int kl = 0; kl < 2; kl++) {
-                if (2 * ch + kl < KS) {
+                if (2 * cc + kl < KS) {
 #pragma unroll
                     for (int cb = 0; cb < 4; cb++) {
-                        vec b = *reinterpret_cast<const vec *>(&bs[(16 * cb + (lane & 15u)) * kCHP + kl * 32 + 8 * (lane >> 4)]);
+                        vec b = *reinterpret_cast<const vec *>(
+                            &bs[(16 * cb + (lane & 15u)) * kBSP + (kOneStage ? cc * 64 : 0) + kl * 32 + 8 * (lane >> 4)]);
 #pragma unroll
-                        for (int f = 0; f < FR; f++) acc[f][cb] = Mfma<T>::run(afrag[f][2 * ch + kl], b, acc[f][cb]);
+                        for (int f = 0; f < FR; f++) acc[f][cb] = Mfma<T>::run(afrag[f][2 * cc + kl], b, acc[f][cb]);
                     }
                 }
+            }
             }
         }
         // epilogue: lane holds rows 16 (FR w + f) + 4*(lane/16) + i, column
