@@ -536,7 +536,7 @@ void csr_build(hrnn_dev_t h) {
     h->rank_bits = rb;
 
     h->csr_off.reserve((n + 1) * 8);
-    if (!h->wide) {   // u32 entries: the radix-sort build
+    if (!h->wide && K <= (uint32_t)kRsTile) {   // u32 entries, rows within a radix tile: the radix-sort build
         csr_radix(h, nb, rb);
         HIPC(hipStreamSynchronize(h->stream));
         h->max_deg = max_list_len(h);

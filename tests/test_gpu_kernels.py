@@ -107,6 +107,18 @@ def test_csr_k500_shape():
     _csr_check(dev, np.zeros((n, 16), np.float32), ids, cnt, K)
 
 
+@pytest.mark.parametrize("n,K", [(4300, 4096), (4600, 4200)])
+def test_csr_rows_at_and_past_a_radix_tile(n, K):
+    """K = 4096: one KNN row fills a 4,096-element radix tile exactly; K =
+    4200 (rank_bits 13, u32 entries): a row is longer than a tile, so the
+    build takes the count / fill / per-list sort path; bit-exact both."""
+    rng = np.random.default_rng(12)
+    ids = np.argsort(rng.random((n, n), dtype=np.float32), axis=1)[:, :K].astype(np.uint32)
+    cnt = rng.integers(K - 300, K + 1, n).astype(np.uint16)
+    dev = hrnn_amd.Device(0)
+    _csr_check(dev, np.zeros((n, 16), np.float32), ids, cnt, K)
+
+
 def test_csr_deterministic_across_runs():
     dev = hrnn_amd.Device(0)
     x = np.zeros((2000, 16), np.float32)
