@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "handle.h"
@@ -16,20 +17,90 @@ hrnn_dev::~hrnn_dev() {
     for (DevBuf *b : bufs) b->release();
     if (ev_a) (void)hipEventDestroy(ev_a);
     if (ev_b) (void)hipEventDestroy(ev_b);
+    for (int b = 0; b < 2; b++) {
+        if (pin_ev[b]) (void)hipEventDestroy(pin_ev[b]);
+        if (pin[b]) (void)hipHostFree(pin[b]);
+    }
     if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
 namespace hrnn {
 
+// Large copies between caller (pageable) memory and the device go through
+// two pinned staging buffers: the host-side copy of one chunk (split over a
+// few threads) overlaps the DMA of the other, and the DMA runs from pinned
+// memory at the link's rate.
+static constexpr size_t kStageBytes = 64ull << 20;
+
+static void par_memcpy(void *dst, const void *src, size_t n) {
+    const unsigned T = std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
+    if (n < (8ull << 20) || T == 1) {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    const size_t per = (n + T - 1) / T;
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; t++) {
+        const size_t o = (size_t)t * per;
+        if (o >= n) break;
+        const size_t l = std::min(per, n - o);
+        th.emplace_back([=] { std::memcpy(static_cast<char *>(dst) + o, static_cast<const char *>(src) + o, l); });
+    }
+    for (auto &x : th) x.join();
+}
+
+static void staged_copy(hrnn_dev_t h, void *dst, const void *src, size_t bytes, bool to_dev) {
+    for (int b = 0; b < 2; b++)
+        if (!h->pin[b]) {
+            HIPC(hipHostMalloc(&h->pin[b], kStageBytes, hipHostMallocDefault));
+            HIPC(hipEventCreateWithFlags(&h->pin_ev[b], hipEventDisableTiming));
+        }
+    const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
+    char *d = static_cast<char *>(dst);
+    const char *s = static_cast<const char *>(src);
+    if (to_dev) {
+        for (size_t c = 0; c < nch; c++) {
+            const int b = (int)(c & 1);
+            const size_t off = c * kStageBytes, len = std::min(kStageBytes, bytes - off);
+            if (c >= 2) HIPC(hipEventSynchronize(h->pin_ev[b]));   // chunk c - 2 has left this buffer
+            par_memcpy(h->pin[b], s + off, len);
+            HIPC(hipMemcpyAsync(d + off, h->pin[b], len, hipMemcpyHostToDevice, h->stream));
+            HIPC(hipEventRecord(h->pin_ev[b], h->stream));
+        }
+        HIPC(hipStreamSynchronize(h->stream));
+        return;
+    }
+    for (size_t c = 0; c <= nch; c++) {   // DMA chunk c while chunk c - 1 is copied out
+        if (c < nch) {
+            const size_t off = c * kStageBytes, len = std::min(kStageBytes, bytes - off);
+            HIPC(hipMemcpyAsync(h->pin[c & 1], s + off, len, hipMemcpyDeviceToHost, h->stream));
+            HIPC(hipEventRecord(h->pin_ev[c & 1], h->stream));
+        }
+        if (c >= 1) {
+            const size_t off = (c - 1) * kStageBytes, len = std::min(kStageBytes, bytes - off);
+            HIPC(hipEventSynchronize(h->pin_ev[(c - 1) & 1]));
+            par_memcpy(d + off, h->pin[(c - 1) & 1], len);
+        }
+    }
+}
+
 void h2d(hrnn_dev_t h, void *dst, const void *src, size_t bytes) {
     if (!bytes) return;
+    if (bytes >= 2 * kStageBytes) {
+        staged_copy(h, dst, src, bytes, true);
+        return;
+    }
     HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
 }
 
 void d2h(hrnn_dev_t h, void *dst, const void *src, size_t bytes) {
     if (!bytes) return;
+    if (bytes >= 2 * kStageBytes) {
+        staged_copy(h, dst, src, bytes, false);
+        return;
+    }
     HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
 }
@@ -202,7 +273,7 @@ static void set_base_common(hrnn_dev_t h, const float *x, uint64_t n, uint32_t d
     if (device)
         HIPC(hipMemcpyAsync(h->x.p, x, n * d * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
     else
-        HIPC(hipMemcpyAsync(h->x.p, x, n * d * sizeof(float), hipMemcpyHostToDevice, h->stream));
+        h2d(h, h->x.p, x, n * d * sizeof(float));
     launch_base_mirrors(h, 0, n);
     h->K = 0;
     h->csr_valid = false;
@@ -327,9 +398,9 @@ int hrnn_dev_set_knng(hrnn_dev_t h, uint32_t K, const uint32_t *ids, const float
         }
         h->csr_valid = false;
         h->knn_version++;
-        HIPC(hipMemcpyAsync(h->knn_ids.p, ids, n * K * 4, hipMemcpyHostToDevice, h->stream));
-        HIPC(hipMemcpyAsync(h->knn_d.p, dists, n * K * 4, hipMemcpyHostToDevice, h->stream));
-        HIPC(hipMemcpyAsync(h->knn_cnt.p, c32.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+        h2d(h, h->knn_ids.p, ids, n * K * 4);
+        h2d(h, h->knn_d.p, dists, n * K * 4);
+        h2d(h, h->knn_cnt.p, c32.data(), n * 4);
         HIPC(hipMemsetAsync(h->knn_new.p, 1, n * K, h->stream));
         // every listed id < n, checked on the device (the rows are there now)
         uint32_t *bad = h->counters.as<uint32_t>() + kKnngCheckWord;
@@ -351,10 +422,9 @@ int hrnn_dev_get_knng(hrnn_dev_t h, uint32_t *ids, float *dists, uint16_t *cnt) 
         uint64_t n = h->n;
         uint32_t K = h->K;
         std::vector<uint32_t> c32(n);
-        if (ids) HIPC(hipMemcpyAsync(ids, h->knn_ids.p, n * K * 4, hipMemcpyDeviceToHost, h->stream));
-        if (dists) HIPC(hipMemcpyAsync(dists, h->knn_d.p, n * K * 4, hipMemcpyDeviceToHost, h->stream));
-        HIPC(hipMemcpyAsync(c32.data(), h->knn_cnt.p, n * 4, hipMemcpyDeviceToHost, h->stream));
-        HIPC(hipStreamSynchronize(h->stream));
+        if (ids) d2h(h, ids, h->knn_ids.p, n * K * 4);
+        if (dists) d2h(h, dists, h->knn_d.p, n * K * 4);
+        d2h(h, c32.data(), h->knn_cnt.p, n * 4);
         if (cnt)
             for (uint64_t i = 0; i < n; i++) cnt[i] = (uint16_t)c32[i];
     });
@@ -397,12 +467,8 @@ int hrnn_dev_get_rknn_csr(hrnn_dev_t h, uint64_t *offsets, void *entries) {
     if (!h) return HRNN_E_ARG;
     return guarded(h, [&] {
         REQUIRE(h->csr_valid, HRNN_E_STATE, "get_rknn_csr: no CSR");
-        if (offsets)
-            HIPC(hipMemcpyAsync(offsets, h->csr_off.p, (h->n + 1) * 8, hipMemcpyDeviceToHost, h->stream));
-        if (entries)
-            HIPC(hipMemcpyAsync(entries, h->csr_ent.p, h->n_ent * (h->wide ? 8 : 4),
-                                hipMemcpyDeviceToHost, h->stream));
-        HIPC(hipStreamSynchronize(h->stream));
+        if (offsets) d2h(h, offsets, h->csr_off.p, (h->n + 1) * 8);
+        if (entries) d2h(h, entries, h->csr_ent.p, h->n_ent * (h->wide ? 8 : 4));
     });
 }
 

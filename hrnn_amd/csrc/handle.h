@@ -9,6 +9,10 @@ struct hrnn_dev {
     // within one call; created on first use
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    // two pinned host staging buffers (and their events) for large host <->
+    // device copies; allocated on first use
+    void *pin[2] = {nullptr, nullptr};
+    hipEvent_t pin_ev[2] = {nullptr, nullptr};
     std::string err;
     int n_cu = 0;
     size_t lds_per_block = 0;
