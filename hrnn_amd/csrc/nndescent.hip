@@ -24,7 +24,7 @@
 //       reserves its range with one atomic and writes its proposals there.
 //       Nothing is dropped unless the whole iteration exceeds the proposal
 //       buffer (reported as dropped).
-//   k_nnd_merge: per node, proposals in chunks of 2048: keep those below the
+//   k_nnd_merge: per node, proposals in chunks of 1024: keep those below the
 //       current worst entry, sort them by (d2, id), drop duplicates and ids
 //       already listed (found in the run of equal list distances at the
 //       proposal's rank), merge with the sorted list
@@ -506,7 +506,7 @@ __global__ __launch_bounds__(256) void k_nnd_join(
     }
 }
 
-constexpr uint32_t kMergeChunk = 2048;   // proposals per merge round (LDS sort)
+constexpr uint32_t kMergeChunk = 1024;   // proposals per merge round (1,024 and 512 measured alike, 2,048 17 % slower)
 constexpr uint32_t kMergeThreads = 256;  // k_nnd_merge's workgroup size
 constexpr uint32_t kMergeMaxK = 1024;
 
@@ -603,10 +603,10 @@ __global__ __launch_bounds__(kMergeThreads) void k_nnd_merge(
         while (PP < nk) PP <<= 1;
         for (uint32_t t = nk + threadIdx.x; t < PP; t += kMergeThreads) keys[t] = ~0ull;
         __syncthreads();
+        static_assert(kMergeChunk == 1024, "the sort dispatch below covers 256, 512 and 1024 keys");
         if (PP == 256) block_sort_u64_regs<1>(keys, PP);
         else if (PP == 512) block_sort_u64_regs<2>(keys, PP);
-        else if (PP == 1024) block_sort_u64_regs<4>(keys, PP);
-        else block_sort_u64_regs<8>(keys, PP);
+        else block_sort_u64_regs<4>(keys, PP);
         __syncthreads();
         // Each of the thread's keys (positions t = j * 256 + tid): lo = the
         // number of list entries with distance <= its own (its merge rank in
