@@ -173,8 +173,8 @@ __global__ __launch_bounds__(256) void k_nnd_join(
     const uint32_t *__restrict__ fo_cnt, const uint32_t *__restrict__ rn,
     const uint32_t *__restrict__ rn_cnt, const uint32_t *__restrict__ ro,
     const uint32_t *__restrict__ ro_cnt, const float *__restrict__ worst,
-    uint32_t *__restrict__ prop_cnt, const uint64_t *__restrict__ prop_off, uint32_t *__restrict__ prop_id,
-    float *__restrict__ prop_d, uint64_t cap, int exact, unsigned long long *__restrict__ stats) {
+    uint32_t *__restrict__ prop_cnt, const uint64_t *__restrict__ prop_off,
+    unsigned long long *__restrict__ prop_key, uint64_t cap, int exact, unsigned long long *__restrict__ stats) {
     __shared__ uint32_t sid[kJMax];
     __shared__ float sthr[kJMax], snorm[kJMax];
     __shared__ int s_n8[kJMax];   // kJoinFullI8: squared norms of the shifted rows
@@ -454,8 +454,7 @@ __global__ __launch_bounds__(256) void k_nnd_join(
         auto put = [&](uint32_t target, uint64_t at, uint32_t cand, float v) {
             const uint64_t pos = prop_off[target] + at;
             if (pos < cap) {
-                prop_id[pos] = cand;
-                prop_d[pos] = v;
+                prop_key[pos] = ((unsigned long long)float_key(v) << 32) | cand;   // (d2, id) sort key
             } else {
                 dropped++;
             }
@@ -543,8 +542,8 @@ __device__ void block_excl_scan(uint32_t *flags, uint32_t P, uint32_t *wsum) {
 __global__ __launch_bounds__(kMergeThreads) void k_nnd_merge(
     uint32_t *__restrict__ ids, float *__restrict__ kd, uint8_t *__restrict__ isnew,
     uint32_t *__restrict__ cnt, uint32_t K, const uint32_t *__restrict__ prop_cnt,
-    const uint64_t *__restrict__ prop_off, const uint32_t *__restrict__ prop_id,
-    const float *__restrict__ prop_d, uint64_t cap, unsigned long long *__restrict__ stats) {
+    const uint64_t *__restrict__ prop_off, const unsigned long long *__restrict__ prop_key, uint64_t cap,
+    unsigned long long *__restrict__ stats) {
     __shared__ unsigned long long keys[kMergeChunk];
     __shared__ uint32_t scan[kMergeChunk + 1];
     __shared__ uint32_t lid[kMergeMaxK], oid[kMergeMaxK];
@@ -578,18 +577,17 @@ __global__ __launch_bounds__(kMergeThreads) void k_nnd_merge(
         __syncthreads();
         // all of the thread's proposals are loaded at once (one round trip)
         constexpr uint32_t kPer = kMergeChunk / kMergeThreads;
-        float pdv[kPer];
-        uint32_t pid[kPer];
+        unsigned long long pk[kPer];
 #pragma unroll
         for (uint32_t i = 0; i < kPer; i++) {   // unconditional (clamped) loads: no branch between them
             const uint32_t t = min(i * kMergeThreads + threadIdx.x, nb - 1);
-            pdv[i] = prop_d[beg + base + t];
-            pid[i] = prop_id[beg + base + t];
+            pk[i] = prop_key[beg + base + t];
         }
+        const uint32_t worst_key = float_key(worst_full);   // order-preserving: d < worst <=> key < worst_key
 #pragma unroll
         for (uint32_t i = 0; i < kPer; i++) {
-            const bool in = i * kMergeThreads + threadIdx.x < nb && pdv[i] < worst_full;
-            const unsigned long long key = in ? ((unsigned long long)float_key(pdv[i]) << 32) | pid[i] : ~0ull;
+            const bool in = i * kMergeThreads + threadIdx.x < nb && (uint32_t)(pk[i] >> 32) < worst_key;
+            const unsigned long long key = in ? pk[i] : ~0ull;
             const uint64_t bm = __ballot(key != ~0ull);
             uint32_t at = 0;
             if ((threadIdx.x & 63u) == 0 && bm) at = atomicAdd(&s_nk, (uint32_t)__popcll(bm));
@@ -760,7 +758,7 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
     DevBuf &fwd = h->s1;  // fn[n*S], fo[n*S]
     DevBuf &rev = h->s2;  // rn[n*RB], ro[n*RB]
     DevBuf &cnts = h->s4; // fn_cnt, fo_cnt, rn_cnt, ro_cnt, prop_cnt, prop_fill [n each]
-    DevBuf &prop = h->s3; // prop_id[cap], prop_d[cap]
+    DevBuf &prop = h->s3; // prop_key[cap]: (float_key(d2) << 32) | id
     const uint32_t nblk = scan_block_count(n);
     w.reserve(n * 4 + 64);
     fwd.reserve(n * S * 8 + 64);
@@ -774,8 +772,7 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
     uint32_t *rn = rev.as<uint32_t>(), *ro = rn + n * RB;
     uint32_t *fn_cnt = cnts.as<uint32_t>(), *fo_cnt = fn_cnt + n, *rn_cnt = fo_cnt + n,
              *ro_cnt = rn_cnt + n, *prop_cnt = ro_cnt + n, *prop_fill = prop_cnt + n;
-    uint32_t *prop_id = prop.as<uint32_t>();
-    float *prop_d = reinterpret_cast<float *>(prop_id + cap);
+    unsigned long long *prop_key = prop.as<unsigned long long>();
     uint64_t *prop_off = h->nnd_aux.as<uint64_t>();
     uint64_t *bsum = prop_off + n + 1;
     uint64_t *ptotal = bsum + nblk;
@@ -800,15 +797,15 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
                            h->xb.as<__hip_bfloat16>(), h->xu8.as<uint8_t>(), h->norms.as<float>(),
                            h->x.as<float>(), h->d,
                            n, (uint32_t)S, R, RB, fn, fn_cnt, fo, fo_cnt, rn, rn_cnt, ro, ro_cnt, worst,
-                           prop_cnt, prop_off, prop_id, prop_d, cap, exact, stats);
+                           prop_cnt, prop_off, prop_key, cap, exact, stats);
         scan_u32(h, prop_cnt, n, prop_off, bsum, ptotal);
         hipLaunchKernelGGL(join_fill, dim3((unsigned)n), dim3(256), join_lds, h->stream,
                            h->xb.as<__hip_bfloat16>(), h->xu8.as<uint8_t>(), h->norms.as<float>(),
                            h->x.as<float>(), h->d,
                            n, (uint32_t)S, R, RB, fn, fn_cnt, fo, fo_cnt, rn, rn_cnt, ro, ro_cnt, worst,
-                           prop_fill, prop_off, prop_id, prop_d, cap, exact, stats);
+                           prop_fill, prop_off, prop_key, cap, exact, stats);
         hipLaunchKernelGGL(k_nnd_merge, dim3((unsigned)n), dim3(kMergeThreads), 0, h->stream, ids, kd, isnew,
-                           cnt, K, prop_cnt, prop_off, prop_id, prop_d, cap, stats);
+                           cnt, K, prop_cnt, prop_off, prop_key, cap, stats);
     };
 
     const char *env = std::getenv("HRNN_NND_GRAPH");
