@@ -41,11 +41,17 @@ static void par_memcpy(void *dst, const void *src, size_t n) {
     }
     const size_t per = (n + T - 1) / T;
     std::vector<std::thread> th;
+    th.reserve(T);
     for (unsigned t = 0; t < T; t++) {
         const size_t o = (size_t)t * per;
         if (o >= n) break;
         const size_t l = std::min(per, n - o);
-        th.emplace_back([=] { std::memcpy(static_cast<char *>(dst) + o, static_cast<const char *>(src) + o, l); });
+        auto part = [=] { std::memcpy(static_cast<char *>(dst) + o, static_cast<const char *>(src) + o, l); };
+        try {
+            th.emplace_back(part);
+        } catch (...) {   // no thread to spare: copy this part here
+            part();
+        }
     }
     for (auto &x : th) x.join();
 }
