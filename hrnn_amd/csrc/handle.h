@@ -122,6 +122,7 @@ constexpr uint32_t kQueueWord = 64;        // search work queue (u32 index in co
 constexpr uint32_t kQueue2Word = 128;      // second search queue (overlapping launches)
 constexpr uint32_t kListCountWord = 160;   // two list counts of the learned query path
 constexpr uint32_t kKnngCheckWord = 168;   // set_knng's device-side id check flag
+constexpr uint32_t kNndActiveWord = 172;   // NNDescent: the current iteration has new entries
 // Optional query subset and stream of a search launch: queries qlist[0 ..
 // *qcount) (device memory) instead of 0 .. nq-1, on `stream` with its work
 // queue at counters[queue_word].

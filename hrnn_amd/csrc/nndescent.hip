@@ -78,15 +78,37 @@ __global__ void k_nnd_prepare(const float *__restrict__ kd, const uint32_t *__re
     prop_fill[u] = 0;
 }
 
+// *active = 1 when the previous iteration inserted anything (or there is no
+// previous one).  After an iteration without insertions no entry carries the
+// "new" flag (sampling clears every flag, only insertions set it), so no
+// node has a new sample, the join evaluates no pair and the lists stay as
+// they are: the iteration can skip its sampling pass.
+__global__ void k_nnd_active(const unsigned long long *__restrict__ prev_stats, uint32_t *__restrict__ active) {
+    unsigned long long v = prev_stats ? prev_stats[ST_UPDATES * kStatSlots + threadIdx.x] : 1ull;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    __shared__ unsigned long long w[4];
+    if ((threadIdx.x & 63u) == 0) w[threadIdx.x / 64] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) *active = (w[0] + w[1] + w[2] + w[3]) != 0ull ? 1u : 0u;
+}
+
 __global__ __launch_bounds__(256) void k_nnd_sample(
     const uint32_t *__restrict__ ids, const float *__restrict__ kd, uint8_t *__restrict__ isnew,
     const uint32_t *__restrict__ cnt, uint64_t n, uint32_t K, uint32_t S, uint32_t RB,
     const float *__restrict__ worst, uint32_t *__restrict__ fn, uint32_t *__restrict__ fn_cnt,
     uint32_t *__restrict__ fo, uint32_t *__restrict__ fo_cnt, uint32_t *__restrict__ rn,
-    uint32_t *__restrict__ rn_cnt, uint32_t *__restrict__ ro, uint32_t *__restrict__ ro_cnt) {
+    uint32_t *__restrict__ rn_cnt, uint32_t *__restrict__ ro, uint32_t *__restrict__ ro_cnt,
+    const uint32_t *__restrict__ active) {
     uint64_t i = (uint64_t)blockIdx.x * 4 + threadIdx.x / 64;
     if (i >= n) return;
     const uint32_t lane = threadIdx.x & 63u;
+    if (!*active) {   // nothing new anywhere: empty samples (the join then skips the node)
+        if (lane == 0) {
+            fn_cnt[i] = 0;
+            fo_cnt[i] = 0;
+        }
+        return;
+    }
     const uint64_t lt = lanemask_lt();
     uint32_t c = cnt[i];
     uint32_t new_before = 0, old_before = 0;
@@ -785,14 +807,17 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
     constexpr uint64_t kStatWords = (uint64_t)ST_N * kStatSlots;
     h->s5.reserve((uint64_t)std::max(iters, 1) * kStatWords * 8 + 64);
     unsigned long long *all_stats = h->s5.as<unsigned long long>();
+    uint32_t *active = h->counters.as<uint32_t>() + kNndActiveWord;
 
     auto enqueue_iteration = [&](int it) {
         unsigned long long *stats = all_stats + (uint64_t)it * kStatWords;
         hipLaunchKernelGGL(k_nnd_prepare, dim3(ceil_div(n, 256)), dim3(256), 0, h->stream, kd, cnt, n,
                            K, worst, rn_cnt, ro_cnt, prop_cnt, prop_fill, stats);
+        hipLaunchKernelGGL(k_nnd_active, dim3(1), dim3(kStatSlots), 0, h->stream,
+                           it > 0 ? all_stats + (uint64_t)(it - 1) * kStatWords : nullptr, active);
         hipLaunchKernelGGL(k_nnd_sample, dim3(ceil_div(n, 4)), dim3(256), 0, h->stream, ids, kd, isnew,
                            cnt, n, K, (uint32_t)S, RB, worst, fn, fn_cnt, fo, fo_cnt, rn, rn_cnt, ro,
-                           ro_cnt);
+                           ro_cnt, active);
         hipLaunchKernelGGL(join_count, dim3((unsigned)n), dim3(256), join_lds, h->stream,
                            h->xb.as<__hip_bfloat16>(), h->xu8.as<uint8_t>(), h->norms.as<float>(),
                            h->x.as<float>(), h->d,
