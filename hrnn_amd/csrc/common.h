@@ -400,6 +400,30 @@ __device__ void bitonic_sort_lds(T *keys, uint32_t n) {
     __syncthreads();
 }
 
+// x from lane (lane ^ lj) of the wave.  Distances 1, 2 (quad permutes) and 8
+// (a rotation by half a row) are DPP moves on the VALU; distance 4 takes the
+// row shift whose source lane is lane ^ 4 (checked on the lane ids, so the
+// shift direction need not be assumed); 16 and 32 cross rows through the
+// LDS permute path (ds_bpermute).  lj must be a power of two below 64.
+__device__ __forceinline__ uint32_t lane_xor32(uint32_t x, uint32_t lj) {
+    const int xi = (int)x;
+    if (lj == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, xi, 0xB1, 0xF, 0xF, true);
+    if (lj == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, xi, 0x4E, 0xF, 0xF, true);
+    if (lj == 8) return (uint32_t)__builtin_amdgcn_update_dpp(0, xi, 0x128, 0xF, 0xF, true);
+    if (lj == 4) {
+        const int lane = (int)(threadIdx.x & 63u);
+        const int src_a = __builtin_amdgcn_update_dpp(-1, lane, 0x104, 0xF, 0xF, false);   // row_shl:4
+        const int a = __builtin_amdgcn_update_dpp(0, xi, 0x104, 0xF, 0xF, true);
+        const int b = __builtin_amdgcn_update_dpp(0, xi, 0x114, 0xF, 0xF, true);          // row_shr:4
+        return (uint32_t)(src_a == (lane ^ 4) ? a : b);
+    }
+    return __shfl_xor(x, (int)lj, 64);
+}
+__device__ __forceinline__ unsigned long long lane_xor64(unsigned long long x, uint32_t lj) {
+    const uint32_t lo = lane_xor32((uint32_t)x, lj), hi = lane_xor32((uint32_t)(x >> 32), lj);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 // Ascending bitonic sort of the first n (<= 256 E) keys of `keys` (LDS) by a
 // 256-thread block, E keys per thread in registers (thread t holds positions
 // t E .. t E + E - 1; positions >= n sort as ~0).  Levels whose partner
@@ -445,9 +469,7 @@ __device__ void block_sort_u64_regs(unsigned long long *keys, uint32_t n) {
                         o = keys[i ^ j];
                     } else {
                         const uint32_t lj = j / E;   // lane distance inside the wave
-                        const uint32_t lo = __shfl_xor((uint32_t)v[r], (int)lj, 64);
-                        const uint32_t hi = __shfl_xor((uint32_t)(v[r] >> 32), (int)lj, 64);
-                        o = ((unsigned long long)hi << 32) | lo;
+                        o = lane_xor64(v[r], lj);
                     }
                     const bool keep_min = ((i & j) == 0) == ((i & k) == 0);
                     v[r] = keep_min ? (v[r] < o ? v[r] : o) : (v[r] < o ? o : v[r]);
