@@ -529,6 +529,27 @@ __global__ __launch_bounds__(256) void k_nnd_join(
 
 constexpr uint32_t kMergeChunk = 1024;   // proposals per merge round (1,024 and 512 measured alike, 2,048 17 % slower)
 constexpr uint32_t kMergeThreads = 256;  // k_nnd_merge's workgroup size
+constexpr uint32_t kSeenSlots = 2048;    // ids listed during one node's merge (LDS set)
+
+__device__ __forceinline__ void seen_insert(uint32_t *set, uint32_t *n, uint32_t id) {
+    if (atomicAdd(n, 1u) >= kSeenSlots * 3 / 4) return;   // full enough: stop growing (exactness kept)
+    uint32_t h = (id * 2654435761u) & (kSeenSlots - 1);
+    for (uint32_t probe = 0; probe < kSeenSlots; probe++) {
+        const uint32_t old = atomicCAS(&set[h], kInvalidId, id);
+        if (old == kInvalidId || old == id) return;
+        h = (h + 1) & (kSeenSlots - 1);
+    }
+}
+__device__ __forceinline__ bool seen_has(const uint32_t *set, uint32_t id) {
+    uint32_t h = (id * 2654435761u) & (kSeenSlots - 1);
+    for (uint32_t probe = 0; probe < kSeenSlots; probe++) {
+        const uint32_t e = set[h];
+        if (e == id) return true;
+        if (e == kInvalidId) return false;
+        h = (h + 1) & (kSeenSlots - 1);
+    }
+    return false;
+}
 constexpr uint32_t kMergeMaxK = 1024;
 
 // Exclusive scan of flags[0..P) in place (P <= kMergeChunk, 256 threads);
@@ -573,6 +594,14 @@ __global__ __launch_bounds__(kMergeThreads) void k_nnd_merge(
     __shared__ uint8_t lnw[kMergeMaxK], onw[kMergeMaxK];
     __shared__ uint32_t wsum[4];
     __shared__ uint32_t s_nk;
+    // Ids listed at any point of this merge (insert-only, linear probing):
+    // a proposal whose id is here is skipped before the sort.  That is exact:
+    // a still-listed id is a duplicate, and a listed-then-evicted id has the
+    // same distance as when it was evicted, never below the worst entry since.
+    // Past 3/4 load the set stops growing and the check after the sort (the
+    // run of equal list distances) alone keeps the merge exact.
+    __shared__ uint32_t s_seen[kSeenSlots];
+    __shared__ uint32_t s_seen_n;
 
     const uint64_t u = blockIdx.x;
     const uint64_t beg = prop_off[u];
@@ -582,10 +611,14 @@ __global__ __launch_bounds__(kMergeThreads) void k_nnd_merge(
     const uint32_t np = (uint32_t)np64;
     if (np == 0) return;
     uint32_t c = cnt[u];
+    for (uint32_t t = threadIdx.x; t < kSeenSlots; t += kMergeThreads) s_seen[t] = kInvalidId;
+    if (threadIdx.x == 0) s_seen_n = 0;
+    __syncthreads();
     for (uint32_t t = threadIdx.x; t < c; t += kMergeThreads) {
         lid[t] = ids[u * K + t];
         ld[t] = kd[u * K + t];
         lnw[t] = isnew[u * K + t];
+        seen_insert(s_seen, &s_seen_n, lid[t]);
     }
     uint32_t upd = 0;
     for (uint32_t base = 0; base < np; base += kMergeChunk) {
@@ -608,7 +641,8 @@ __global__ __launch_bounds__(kMergeThreads) void k_nnd_merge(
         const uint32_t worst_key = float_key(worst_full);   // order-preserving: d < worst <=> key < worst_key
 #pragma unroll
         for (uint32_t i = 0; i < kPer; i++) {
-            const bool in = i * kMergeThreads + threadIdx.x < nb && (uint32_t)(pk[i] >> 32) < worst_key;
+            const bool in = i * kMergeThreads + threadIdx.x < nb && (uint32_t)(pk[i] >> 32) < worst_key &&
+                            !seen_has(s_seen, (uint32_t)pk[i]);
             const unsigned long long key = in ? pk[i] : ~0ull;
             const uint64_t bm = __ballot(key != ~0ull);
             uint32_t at = 0;
@@ -687,6 +721,7 @@ __global__ __launch_bounds__(kMergeThreads) void k_nnd_merge(
                 od[pos] = key_float((uint32_t)(kj[j] >> 32));
                 onw[pos] = 1;
                 upd++;
+                seen_insert(s_seen, &s_seen_n, (uint32_t)kj[j]);
             }
         }
         // listed entries: pos = e + #valid proposals with dist < ld[e]
