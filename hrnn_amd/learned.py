@@ -197,11 +197,13 @@ class LearnedEarlyStop:
         return cls(p, mu.astype(np.float32), sd.astype(np.float32), np.asarray(classes, np.uint32),
                    ef1), losses
 
-    def pick_tau(self, dev: Device, feats, rec, target: float, taus=(0.3, 0.5, 0.7, 0.8, 0.9, 0.95,
-                                                                      0.98, 0.995)):
+    def pick_tau(self, dev: Device, feats, rec, target: float,
+                 taus=(0.1, 0.2, 0.3, 0.4, 0.5, 0.7, 0.8, 0.9, 0.95, 0.98, 0.995)):
         """The smallest tau whose macro recall over these queries (rec[16, nq]
         from label_classes, NaN rows of empty groundtruth left out) reaches
-        target; the last tau when none does."""
+        target; the last tau when none does.  A smaller tau stops at an
+        earlier (cheaper) class, so the first tau that passes is the
+        cheapest one that does."""
         p = self.probs(dev, feats)
         cols = np.arange(p.shape[0])
         for tau in taus:
