@@ -6,7 +6,9 @@
 //       in rank order; the first S new entries become nn_new, the first S old
 //       ones nn_old; every new flag is cleared; entry (i -> nbr) is offered to
 //       nbr's reverse sample (new or old) when dist(i, nbr) > nbr's K-th
-//       distance (FLT_MAX unless nbr's list is full).  The reference keeps the
+//       distance (FLT_MAX unless nbr's list is full).  Two passes (new
+//       entries, then old ones): the old pass skips nodes whose join has no
+//       new member, which changes no join.  The reference keeps the
 //       first R = S offers, which a single thread makes in increasing i; here
 //       up to 256 offers are collected per node and the join keeps the R
 //       smallest source ids, the same set whenever a node gets <= 256 offers.
@@ -92,62 +94,66 @@ __global__ void k_nnd_active(const unsigned long long *__restrict__ prev_stats, 
     if (threadIdx.x == 0) *active = (w[0] + w[1] + w[2] + w[3]) != 0ull ? 1u : 0u;
 }
 
+// Sampling runs in two passes over each node's list (one wave per node).
+// Pass 1 (kOld = false) takes the new entries: nn_new and the reverse new
+// offers.  k_nnd_need then marks every node whose join has work, i.e. one
+// with a new sample or a reverse new offer.  Pass 2 (kOld = true) takes the
+// old entries and clears the new flags; it writes nn_old only for a marked
+// node and makes a reverse old offer only to a marked node.  Every pair a
+// node's join evaluates has a member from that node's new samples or
+// reverse new offers, so an unmarked node's join evaluates nothing and what
+// pass 2 leaves out never reaches a join: the joins are those of a single
+// pass over all entries.  After convergence starts most nodes are unmarked
+// and pass 2 skips their random worst[] reads and reverse-offer atomics.
+template <bool kOld>
 __global__ __launch_bounds__(256) void k_nnd_sample(
     const uint32_t *__restrict__ ids, const float *__restrict__ kd, uint8_t *__restrict__ isnew,
     const uint32_t *__restrict__ cnt, uint64_t n, uint32_t K, uint32_t S, uint32_t RB,
-    const float *__restrict__ worst, uint32_t *__restrict__ fn, uint32_t *__restrict__ fn_cnt,
-    uint32_t *__restrict__ fo, uint32_t *__restrict__ fo_cnt, uint32_t *__restrict__ rn,
-    uint32_t *__restrict__ rn_cnt, uint32_t *__restrict__ ro, uint32_t *__restrict__ ro_cnt,
+    const float *__restrict__ worst, uint32_t *__restrict__ fs, uint32_t *__restrict__ fs_cnt,
+    uint32_t *__restrict__ rs, uint32_t *__restrict__ rs_cnt, const uint32_t *__restrict__ need,
     const uint32_t *__restrict__ active) {
     uint64_t i = (uint64_t)blockIdx.x * 4 + threadIdx.x / 64;
     if (i >= n) return;
     const uint32_t lane = threadIdx.x & 63u;
     if (!*active) {   // nothing new anywhere: empty samples (the join then skips the node)
-        if (lane == 0) {
-            fn_cnt[i] = 0;
-            fo_cnt[i] = 0;
-        }
+        if (lane == 0) fs_cnt[i] = 0;
         return;
     }
+    const bool need_i = !kOld || ((need[i >> 5] >> (i & 31u)) & 1u);
     const uint64_t lt = lanemask_lt();
-    uint32_t c = cnt[i];
-    uint32_t new_before = 0, old_before = 0;
+    const uint32_t c = cnt[i];
+    uint32_t before = 0;
     for (uint32_t base = 0; base < c; base += 64) {
-        uint32_t j = base + lane;
-        bool valid = j < c;
-        uint32_t nbr = 0;
-        float dij = 0.f;
-        bool f = false;
-        if (valid) {
-            nbr = ids[i * K + j];
-            dij = kd[i * K + j];
-            f = isnew[i * K + j] != 0;
-        }
-        bool isn = valid && f, iso = valid && !f;
-        uint64_t mn = __ballot(isn), mo = __ballot(iso);
-        uint32_t rnk_n = new_before + __popcll(mn & lt);
-        uint32_t rnk_o = old_before + __popcll(mo & lt);
-        if (isn) {
-            if (rnk_n < S) fn[i * S + rnk_n] = nbr;
-            isnew[i * K + j] = 0;
-        }
-        if (iso && rnk_o < S) fo[i * S + rnk_o] = nbr;
-        if (valid && dij > worst[nbr]) {
-            if (isn) {
-                uint32_t s = atomicAdd(&rn_cnt[nbr], 1u);
-                if (s < RB) rn[(uint64_t)nbr * RB + s] = (uint32_t)i;
-            } else {
-                uint32_t s = atomicAdd(&ro_cnt[nbr], 1u);
-                if (s < RB) ro[(uint64_t)nbr * RB + s] = (uint32_t)i;
+        const uint32_t j = base + lane;
+        const bool valid = j < c;
+        const bool f = valid && isnew[i * K + j] != 0;
+        const bool mine = kOld ? (valid && !f) : f;
+        const uint64_t mm = __ballot(mine);
+        const uint32_t rnk = before + __popcll(mm & lt);
+        if (kOld && f) isnew[i * K + j] = 0;
+        if (mine) {
+            const uint32_t nbr = ids[i * K + j];
+            if (need_i && rnk < S) fs[i * S + rnk] = nbr;
+            const bool offer = !kOld || ((need[nbr >> 5] >> (nbr & 31u)) & 1u);
+            if (offer && kd[i * K + j] > worst[nbr]) {
+                uint32_t s = atomicAdd(&rs_cnt[nbr], 1u);
+                if (s < RB) rs[(uint64_t)nbr * RB + s] = (uint32_t)i;
             }
         }
-        new_before += __popcll(mn);
-        old_before += __popcll(mo);
+        before += __popcll(mm);
     }
-    if (lane == 0) {
-        fn_cnt[i] = new_before < S ? new_before : S;
-        fo_cnt[i] = old_before < S ? old_before : S;
-    }
+    if (lane == 0) fs_cnt[i] = need_i ? (before < S ? before : S) : 0u;
+}
+
+// need bit i = node i has a new sample or a reverse new offer (after pass 1)
+__global__ __launch_bounds__(256) void k_nnd_need(const uint32_t *__restrict__ fn_cnt,
+                                                  const uint32_t *__restrict__ rn_cnt, uint64_t n,
+                                                  uint32_t *__restrict__ need) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool b = i < n && (fn_cnt[i] | rn_cnt[i]) != 0u;
+    const uint64_t m = __ballot(b);
+    const uint32_t lane = threadIdx.x & 63u;
+    if ((lane & 31u) == 0 && i < n) need[i >> 5] = (uint32_t)(lane ? (m >> 32) : m);
 }
 
 constexpr int kJMax = 256;      // sample members per node (4S, S <= 64)
@@ -814,13 +820,13 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
     DevBuf &w = h->s0;    // worst f32[n]
     DevBuf &fwd = h->s1;  // fn[n*S], fo[n*S]
     DevBuf &rev = h->s2;  // rn[n*RB], ro[n*RB]
-    DevBuf &cnts = h->s4; // fn_cnt, fo_cnt, rn_cnt, ro_cnt, prop_cnt, prop_fill [n each]
+    DevBuf &cnts = h->s4; // fn_cnt, fo_cnt, rn_cnt, ro_cnt, prop_cnt, prop_fill [n each], need bits
     DevBuf &prop = h->s3; // prop_key[cap]: (float_key(d2) << 32) | id
     const uint32_t nblk = scan_block_count(n);
     w.reserve(n * 4 + 64);
     fwd.reserve(n * S * 8 + 64);
     rev.reserve(n * RB * 8 + 64);
-    cnts.reserve(n * 24 + 64);
+    cnts.reserve(n * 24 + (n + 31) / 32 * 4 + 64);
     prop.reserve(cap * 8 + 64);
     h->nnd_aux.reserve((n + 1 + nblk + 1) * 8 + 64);
     h->csr_valid = false;   // the graph changes: any CSR built before is stale
@@ -828,7 +834,8 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
     uint32_t *fn = fwd.as<uint32_t>(), *fo = fn + n * S;
     uint32_t *rn = rev.as<uint32_t>(), *ro = rn + n * RB;
     uint32_t *fn_cnt = cnts.as<uint32_t>(), *fo_cnt = fn_cnt + n, *rn_cnt = fo_cnt + n,
-             *ro_cnt = rn_cnt + n, *prop_cnt = ro_cnt + n, *prop_fill = prop_cnt + n;
+             *ro_cnt = rn_cnt + n, *prop_cnt = ro_cnt + n, *prop_fill = prop_cnt + n,
+             *need = prop_fill + n;   // need bits, one word per 32 nodes
     unsigned long long *prop_key = prop.as<unsigned long long>();
     uint64_t *prop_off = h->nnd_aux.as<uint64_t>();
     uint64_t *bsum = prop_off + n + 1;
@@ -850,9 +857,12 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
                            K, worst, rn_cnt, ro_cnt, prop_cnt, prop_fill, stats);
         hipLaunchKernelGGL(k_nnd_active, dim3(1), dim3(kStatSlots), 0, h->stream,
                            it > 0 ? all_stats + (uint64_t)(it - 1) * kStatWords : nullptr, active);
-        hipLaunchKernelGGL(k_nnd_sample, dim3(ceil_div(n, 4)), dim3(256), 0, h->stream, ids, kd, isnew,
-                           cnt, n, K, (uint32_t)S, RB, worst, fn, fn_cnt, fo, fo_cnt, rn, rn_cnt, ro,
-                           ro_cnt, active);
+        hipLaunchKernelGGL(k_nnd_sample<false>, dim3(ceil_div(n, 4)), dim3(256), 0, h->stream, ids, kd,
+                           isnew, cnt, n, K, (uint32_t)S, RB, worst, fn, fn_cnt, rn, rn_cnt, need, active);
+        hipLaunchKernelGGL(k_nnd_need, dim3(ceil_div(n, 256)), dim3(256), 0, h->stream, fn_cnt, rn_cnt, n,
+                           need);
+        hipLaunchKernelGGL(k_nnd_sample<true>, dim3(ceil_div(n, 4)), dim3(256), 0, h->stream, ids, kd,
+                           isnew, cnt, n, K, (uint32_t)S, RB, worst, fo, fo_cnt, ro, ro_cnt, need, active);
         hipLaunchKernelGGL(join_count, dim3((unsigned)n), dim3(256), join_lds, h->stream,
                            h->xb.as<__hip_bfloat16>(), h->xu8.as<uint8_t>(), h->norms.as<float>(),
                            h->x.as<float>(), h->d,
@@ -872,7 +882,7 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
     const bool use_graph = !(env && env[0] == '0') && iters > 0;
     std::vector<unsigned long long> sv((uint64_t)std::max(iters, 1) * kStatWords, 0);
     if (use_graph) {
-        // All iterations captured as one HIP graph: 8 kernels per iteration, no
+        // All iterations captured as one HIP graph: 10 kernels per iteration, no
         // host synchronisation in between; counters are reset by kernels only.
         auto t0 = std::chrono::steady_clock::now();
         hipGraph_t graph = nullptr;
