@@ -12,9 +12,10 @@ using namespace hrnn;
 hrnn_dev::~hrnn_dev() {
     (void)hipSetDevice(device);
     DevBuf *bufs[] = {&x, &xb, &xh, &xu8, &norms, &knn_ids, &knn_d, &knn_cnt, &knn_new, &rk_col, &csr_off,
-                      &csr_ent, &search_mark, &search_cand, &search_mark2, &search_cand2, &l0, &up_off, &up_pool, &beam_vis,
+                      &csr_ent, &search_mark, &search_cand, &search_mark2, &search_cand2, &fetch_off, &fetch_ids, &l0, &up_off, &up_pool, &beam_vis,
                       &s0, &s1, &s2, &s3, &s4, &s5, &s6, &s7, &s8, &nnd_aux, &counters};
     for (DevBuf *b : bufs) b->release();
+    if (pin_small) (void)hipHostFree(pin_small);
     if (ev_a) (void)hipEventDestroy(ev_a);
     if (ev_b) (void)hipEventDestroy(ev_b);
     for (int b = 0; b < 2; b++) {

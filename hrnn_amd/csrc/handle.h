@@ -60,6 +60,10 @@ struct hrnn_dev {
     // (all-zero between launches) and candidate buffers
     hrnn::DevBuf search_mark, search_cand;
     hrnn::DevBuf search_mark2, search_cand2;   // a second set for an overlapping search
+    // result download (hrnn_dev_fetch_results): offsets (+ total and largest
+    // count), the densely packed ids, and a small pinned word for the count
+    hrnn::DevBuf fetch_off, fetch_ids;
+    void *pin_small = nullptr;
 
     // HNSW graph for phase A on the GPU: layer-0 links [n x (maxM0+1)] (count
     // word + ids), per-node offset into the upper-layer pool (blocks of maxM+1

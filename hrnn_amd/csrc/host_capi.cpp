@@ -266,6 +266,20 @@ int hrnn_index_insert_dynamic(hrnn_index_t ix, const float *x, uint64_t n, const
     });
 }
 
+int hrnn_index_get_rknng_mutable(hrnn_index_t ix, uint64_t *off, uint32_t *ids) {
+    if (!ix || !off) return HRNN_E_ARG;
+    return guard(ix, [&] {
+        auto &idx = *ix->idx;
+        if (!idx.rknng_mutable_) throw std::runtime_error("get_rknng_mutable: call buildRKNNG_mutable first");
+        const size_t n = idx.getCurrentElementCount();
+        off[0] = 0;
+        for (size_t i = 0; i < n; i++) off[i + 1] = off[i] + idx.rknng_lists_[i].size();
+        if (!ids) return;
+        for (size_t i = 0; i < n; i++)
+            std::memcpy(ids + off[i], idx.rknng_lists_[i].data(), idx.rknng_lists_[i].size() * 4);
+    });
+}
+
 int hrnn_index_compact_for_serving(hrnn_index_t ix, uint32_t max_k) {
     if (!ix) return HRNN_E_ARG;
     return guard(ix, [&] { ix->idx->compactForServing(max_k); });

@@ -94,59 +94,6 @@ __global__ void k_mlp_adam(float *__restrict__ P, const float *__restrict__ g, f
     P[p] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
 }
 
-// The 32 query features from the beam statistics and the r_k of the closest
-// proxies (k-th list distance of each proxy, FLT_MAX when its list is short):
-//   0 log1p d(q,entry)          1 log1p d(q,start)
-//   2..17 log1p of the 16 closest beam distances
-//   18..25 log1p r_k of proxies 1..8
-//   26..28 mean / min / max of log1p r_k over proxies 1..16
-//   29 fraction of proxies 1..16 with d(q,p) <= r_k(p)
-//   30 log1p d1 - log1p r_k(p1)     31 log1p(expansions)
-// Optional standardisation: (f - mu) / sd.
-__global__ void k_features(const float *__restrict__ raw, const uint32_t *__restrict__ prox,
-                           uint32_t mstride, const float *__restrict__ knn_d,
-                           const uint32_t *__restrict__ knn_cnt, uint32_t K,
-                           const uint32_t *__restrict__ k_per_q, uint32_t k, uint32_t nq,
-                           const float *__restrict__ mu, const float *__restrict__ sd,
-                           float *__restrict__ feats) {
-    uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (qi >= nq) return;
-    const float *r = raw + (uint64_t)qi * kRawWords;
-    float f[kIn];
-    f[0] = log1pf(r[0]);
-    f[1] = log1pf(r[1]);
-    for (int j = 0; j < 16; j++) f[2 + j] = log1pf(fminf(r[2 + j], 3.0e38f));
-    uint32_t kq = k_per_q ? k_per_q[qi] : k;
-    float sum = 0.f, mn = FLT_MAX, mx = -FLT_MAX, inside = 0.f, rk1 = 0.f;
-    int cnt = 0;
-    for (int j = 0; j < 16; j++) {
-        uint32_t p = (uint32_t)j < mstride ? prox[(uint64_t)qi * mstride + j] : kInvalidId;
-        float rk = 0.f;
-        if (p != kInvalidId) {
-            float v = (kq - 1 < knn_cnt[p]) ? knn_d[(uint64_t)p * K + kq - 1] : 3.0e38f;
-            rk = log1pf(v);
-            inside += (r[2 + j] <= v) ? 1.f : 0.f;
-            cnt++;
-            sum += rk;
-            mn = fminf(mn, rk);
-            mx = fmaxf(mx, rk);
-        }
-        if (j == 0) rk1 = rk;
-        if (j < 8) f[18 + j] = rk;
-    }
-    f[26] = cnt ? sum / cnt : 0.f;
-    f[27] = cnt ? mn : 0.f;
-    f[28] = cnt ? mx : 0.f;
-    f[29] = cnt ? inside / cnt : 0.f;
-    f[30] = f[2] - rk1;
-    f[31] = log1pf(r[18]);
-    for (int j = 0; j < kIn; j++) {
-        float v = f[j];
-        if (mu && sd) v = (v - mu[j]) / sd[j];
-        feats[(uint64_t)qi * kIn + j] = v;
-    }
-}
-
 // In-place feature standardisation f = (f - mu) / sd over n rows of kIn.
 __global__ void k_standardize(float *__restrict__ f, uint32_t n, const float *__restrict__ mu,
                               const float *__restrict__ sd) {
@@ -246,78 +193,6 @@ __global__ void k_pick_class(const float *__restrict__ probs, uint32_t nq,
 using namespace hrnn;
 
 extern "C" {
-
-// The device feature path (phase-A beam with statistics followed by
-// k_features) is switched off: its first GPU run ended in a memory fault whose
-// cause is not found.  Features come from the host phase A
-// (hrnn_index_query_features); the MLP kernels and
-// hrnn_dev_rknn_search_learned_dptr take them from there.
-static const char *kFeaturesOff =
-    "device feature path switched off (unexplained fault on its first run); use "
-    "hrnn_index_query_features and hrnn_dev_rknn_search_learned_dptr";
-
-int hrnn_dev_query_features(hrnn_dev_t h, const float *q, uint32_t nq, uint32_t k, uint32_t ef,
-                            uint32_t m_max, float *feats, uint32_t *proxies) {
-    if (!h) return HRNN_E_ARG;
-    return guarded(h, [&] {
-        REQUIRE(false, HRNN_E_STATE, kFeaturesOff);
-        REQUIRE(q && feats, HRNN_E_ARG, "query_features: null argument");
-        REQUIRE(h->K > 0 && k >= 1 && k <= h->K, HRNN_E_ARG, "query_features: k out of [1, K]");
-        if (!nq) return;
-        uint32_t mm = std::max<uint32_t>(16, m_max);
-        DevBuf dq, dp, dr, df;
-        dq.reserve((uint64_t)nq * h->d * 4);
-        dp.reserve((uint64_t)nq * mm * 4);
-        dr.reserve((uint64_t)nq * kRawWords * 4);
-        df.reserve((uint64_t)nq * kIn * 4);
-        h2d(h, dq.p, q, (uint64_t)nq * h->d * 4);
-        beam_launch(h, dq.as<float>(), nq, mm, std::max(ef, mm), dp.as<uint32_t>(), nullptr, dr.as<float>());
-        hipLaunchKernelGGL(k_features, dim3(ceil_div(nq, 128)), dim3(128), 0, h->stream, dr.as<float>(),
-                           dp.as<uint32_t>(), mm, h->knn_d.as<float>(), h->knn_cnt.as<uint32_t>(), h->K,
-                           nullptr, k, nq, nullptr, nullptr, df.as<float>());
-        HIPC(hipGetLastError());
-        d2h(h, feats, df.p, (uint64_t)nq * kIn * 4);
-        if (proxies) {
-            std::vector<uint32_t> tmp((uint64_t)nq * mm);
-            d2h(h, tmp.data(), dp.p, tmp.size() * 4);
-            for (uint32_t i = 0; i < nq; i++)
-                std::copy(tmp.begin() + (uint64_t)i * mm, tmp.begin() + (uint64_t)i * mm + m_max,
-                          proxies + (uint64_t)i * m_max);
-        }
-        dq.release(); dp.release(); dr.release(); df.release();
-    });
-}
-
-int hrnn_dev_rknn_query_learned_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq,
-                                     const float *model_dev, const uint32_t *grid_dev, float tau,
-                                     uint32_t m_max, uint32_t ef, uint32_t k, uint32_t K_prime,
-                                     float threshold, uint32_t *proxies_dev, uint32_t *m_dev,
-                                     uint32_t *res_cnt_dev, uint32_t *res_ids_dev,
-                                     uint32_t cap_per_q, uint32_t *work_dev) {
-    if (!h) return HRNN_E_ARG;
-    return guarded(h, [&] {
-        REQUIRE(false, HRNN_E_STATE, kFeaturesOff);
-        REQUIRE(h->csr_valid && h->K > 0, HRNN_E_STATE, "query_learned: build the CSR first");
-        REQUIRE(k >= 1 && k <= h->K, HRNN_E_ARG, "query_learned: k out of [1, K]");
-        REQUIRE(m_max >= 16 && m_max <= 1024, HRNN_E_ARG, "query_learned: m_max must be in [16, 1024]");
-        if (!nq) return;
-        h->s8.reserve((uint64_t)nq * (kRawWords + kIn + kOut) * 4);
-        float *raw = h->s8.as<float>();
-        float *feats = raw + (uint64_t)nq * kRawWords;
-        float *probs = feats + (uint64_t)nq * kIn;
-        beam_launch(h, q_dev, nq, m_max, std::max(ef, m_max), proxies_dev, nullptr, raw);
-        hipLaunchKernelGGL(k_features, dim3(ceil_div(nq, 128)), dim3(128), 0, h->stream, raw, proxies_dev,
-                           m_max, h->knn_d.as<float>(), h->knn_cnt.as<uint32_t>(), h->K, nullptr, k, nq,
-                           model_dev + kParams, model_dev + kParams + kIn, feats);
-        hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(nq), dim3(64), 0, h->stream, model_dev, feats, nullptr,
-                           nullptr, nq, probs, nullptr, nullptr);
-        hipLaunchKernelGGL(k_pick_m, dim3(ceil_div(nq, 128)), dim3(128), 0, h->stream, probs, nq, grid_dev,
-                           tau, proxies_dev, m_max, m_dev);
-        HIPC(hipGetLastError());
-        rknn_search_resident(h, q_dev, nq, proxies_dev, m_max, nullptr, k, K_prime, threshold, res_cnt_dev,
-                             res_ids_dev, cap_per_q, work_dev);
-    });
-}
 
 int hrnn_dev_rknn_search_learned_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, float *feats_dev,
                                       const float *model_dev, const uint32_t *grid_dev, float tau,

@@ -449,4 +449,128 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     HIPC(hipGetLastError());
 }
 
+// ---- result compaction (the download side of searchRknn's return value) ----
+//
+// The resident search leaves query q's ids at res_ids[q * cap .. q * cap +
+// min(cnt[q], cap)).  k_results_scan forms off[q] = sum over j < q of
+// min(cnt[j], cap) (u64), off[nq] = the total and off[nq + 1] = the largest
+// true count, in one workgroup (8 consecutive counts per lane, a wave scan,
+// per-wave totals in LDS, a running carry between chunks of 8,192 queries);
+// k_results_gather then packs the ids densely, one wave per query.
+constexpr int kResScanThreads = 1024;
+constexpr uint32_t kResScanPer = 8;
+
+__global__ __launch_bounds__(kResScanThreads) void k_results_scan(const uint32_t *__restrict__ cnt,
+                                                                   uint32_t nq, uint32_t cap,
+                                                                   uint64_t *__restrict__ off) {
+    __shared__ unsigned long long s_w[kResScanThreads / 64];
+    __shared__ unsigned long long s_carry;
+    __shared__ uint32_t s_max;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) {
+        s_carry = 0;
+        s_max = 0;
+    }
+    __syncthreads();
+    uint32_t mx = 0;
+    for (uint32_t base = 0; base < nq; base += kResScanThreads * kResScanPer) {
+        const uint32_t i0 = base + threadIdx.x * kResScanPer;
+        uint32_t v[kResScanPer];
+        unsigned long long sum = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kResScanPer; j++) {
+            const uint32_t i = i0 + j;
+            const uint32_t c = i < nq ? cnt[i] : 0u;
+            mx = c > mx ? c : mx;
+            v[j] = c < cap ? c : cap;
+            sum += v[j];
+        }
+        unsigned long long inc = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned long long t = __shfl_up(inc, (unsigned)o, 64);
+            if ((int)lane >= o) inc += t;
+        }
+        if (lane == 63u) s_w[wave] = inc;
+        __syncthreads();
+        unsigned long long run = s_carry;
+        for (uint32_t w = 0; w < wave; w++) run += s_w[w];
+        run += inc - sum;
+#pragma unroll
+        for (uint32_t j = 0; j < kResScanPer; j++) {
+            const uint32_t i = i0 + j;
+            if (i < nq) off[i] = run;
+            run += v[j];
+        }
+        __syncthreads();   // every lane has read s_carry and s_w
+        if (threadIdx.x == kResScanThreads - 1) s_carry = run;
+        __syncthreads();
+    }
+    atomicMax(&s_max, mx);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        off[nq] = s_carry;
+        off[nq + 1] = s_max;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_results_gather(const uint32_t *__restrict__ cnt,
+                                                        const uint32_t *__restrict__ ids, uint32_t nq,
+                                                        uint32_t cap, const uint64_t *__restrict__ off,
+                                                        uint32_t *__restrict__ out) {
+    const uint32_t q = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (q >= nq) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t c = cnt[q] < cap ? cnt[q] : cap;
+    const uint64_t o = off[q];
+    for (uint32_t j = lane; j < c; j += 64u) out[o + j] = ids[(uint64_t)q * cap + j];
+}
+
 }  // namespace hrnn
+
+using namespace hrnn;
+
+extern "C" {
+
+int hrnn_dev_fetch_results(hrnn_dev_t h, uint32_t nq, const uint32_t *res_cnt_dev,
+                           const uint32_t *res_ids_dev, uint32_t cap_per_q, uint64_t *res_off,
+                           uint32_t *res_ids, uint64_t cap, uint64_t *needed) {
+    if (!h) return HRNN_E_ARG;
+    return guarded(h, [&] {
+        REQUIRE(res_off && (nq == 0 || (res_cnt_dev && res_ids_dev)), HRNN_E_ARG,
+                "fetch_results: null argument");
+        REQUIRE(cap_per_q > 0, HRNN_E_ARG, "fetch_results: cap_per_q == 0");
+        if (needed) *needed = 0;
+        res_off[0] = 0;
+        if (nq == 0) return;
+        h->fetch_off.reserve(((uint64_t)nq + 2) * 8);
+        if (!h->pin_small) HIPC(hipHostMalloc(&h->pin_small, 64, hipHostMallocDefault));
+        uint64_t *off = h->fetch_off.as<uint64_t>();
+        uint64_t *small = static_cast<uint64_t *>(h->pin_small);
+        hipLaunchKernelGGL(k_results_scan, dim3(1), dim3(kResScanThreads), 0, h->stream, res_cnt_dev, nq,
+                           cap_per_q, off);
+        HIPC(hipGetLastError());
+        HIPC(hipMemcpyAsync(res_off, off, ((uint64_t)nq + 1) * 8, hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipMemcpyAsync(small, off + nq + 1, 8, hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
+        const uint64_t total = res_off[nq], mx = small[0];
+        if (mx > cap_per_q) {
+            if (needed) *needed = mx;
+            throw Error{HRNN_E_CAP, "fetch_results: a query has " + std::to_string(mx) +
+                                        " results, more than cap_per_q = " + std::to_string(cap_per_q) +
+                                        " (its list was cut short on the device)"};
+        }
+        if (needed) *needed = total;
+        if (total > cap || (total && !res_ids))
+            throw Error{HRNN_E_CAP, "fetch_results: result capacity too small"};
+        if (!total) return;
+        h->fetch_ids.reserve(total * 4);
+        hipLaunchKernelGGL(k_results_gather, dim3(ceil_div(nq, 4)), dim3(256), 0, h->stream, res_cnt_dev,
+                           res_ids_dev, nq, cap_per_q, off, h->fetch_ids.as<uint32_t>());
+        HIPC(hipGetLastError());
+        HIPC(hipMemcpyAsync(res_ids, h->fetch_ids.p, total * 4, hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
+    });
+}
+
+}  // extern "C"

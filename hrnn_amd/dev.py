@@ -33,10 +33,10 @@ EXPORTS = [
     "hrnn_dev_set_vector_precision", "hrnn_dev_set_tuning", "hrnn_dev_get_tuning",
     "hrnn_dev_set_knng", "hrnn_dev_get_knng", "hrnn_dev_knng_ptrs", "hrnn_dev_nndescent",
     "hrnn_dev_build_rknn_csr", "hrnn_dev_get_rknn_csr", "hrnn_dev_set_rknn_csr",
-    "hrnn_dev_rknn_search", "hrnn_dev_rknn_search_dptr", "hrnn_dev_set_hnsw", "hrnn_dev_proxies",
+    "hrnn_dev_rknn_search", "hrnn_dev_rknn_search_dptr", "hrnn_dev_fetch_results", "hrnn_dev_set_hnsw", "hrnn_dev_proxies",
     "hrnn_dev_proxies_dptr", "hrnn_dev_rknn_query_dptr", "hrnn_dev_bf_kth", "hrnn_dev_bf_knn",
-    "hrnn_dev_bf_rknn", "hrnn_dev_l2sqr_pairs", "hrnn_dev_query_features", "hrnn_dev_mlp_forward",
-    "hrnn_dev_mlp_grad", "hrnn_dev_mlp_train", "hrnn_dev_rknn_query_learned_dptr", "hrnn_dev_rknn_search_learned_dptr",
+    "hrnn_dev_bf_rknn", "hrnn_dev_l2sqr_pairs", "hrnn_dev_mlp_forward",
+    "hrnn_dev_mlp_grad", "hrnn_dev_mlp_train", "hrnn_dev_rknn_search_learned_dptr",
     "hrnn_dev_stage1_features_dptr", "hrnn_dev_rknn_query_adaptive_dptr",
     "hrnn_synth_mixture",
     "hrnn_synth_halfnormal_scaled", "hrnn_synth_sphere_mixture",
@@ -90,6 +90,7 @@ def lib() -> C.CDLL:
         "hrnn_dev_set_rknn_csr": [vp, u32, i, u64, vp, vp, u64],
         "hrnn_dev_rknn_search": [vp, vp, u32, vp, u32, vp, u32, u32, f, vp, vp, u64, P(u64)],
         "hrnn_dev_rknn_search_dptr": [vp, vp, u32, vp, u32, vp, u32, u32, f, vp, vp, u32, vp],
+        "hrnn_dev_fetch_results": [vp, u32, vp, vp, u32, vp, vp, u64, P(u64)],
         "hrnn_dev_set_hnsw": [vp, u32, u32, vp, vp, vp, vp, u64, u32, i],
         "hrnn_dev_proxies": [vp, vp, u32, u32, u32, vp],
         "hrnn_dev_proxies_dptr": [vp, vp, u32, u32, u32, vp, vp],
@@ -98,12 +99,9 @@ def lib() -> C.CDLL:
         "hrnn_dev_bf_knn": [vp, vp, u32, u32, i, vp, vp],
         "hrnn_dev_bf_rknn": [vp, vp, u32, vp, i, vp, vp, vp, u64, P(u64)],
         "hrnn_dev_l2sqr_pairs": [vp, vp, u64, vp, vp, u64, vp],
-        "hrnn_dev_query_features": [vp, vp, u32, u32, u32, u32, vp, vp],
         "hrnn_dev_mlp_forward": [vp, vp, vp, u32, vp],
         "hrnn_dev_mlp_grad": [vp, vp, vp, vp, u32, vp, vp],
         "hrnn_dev_mlp_train": [vp, vp, vp, vp, u32, i, u32, f, f, u64, vp],
-        "hrnn_dev_rknn_query_learned_dptr": [vp, vp, u32, vp, vp, f, u32, u32, u32, u32, f, vp, vp,
-                                             vp, vp, u32, vp],
         "hrnn_dev_rknn_search_learned_dptr": [vp, vp, u32, vp, vp, vp, f, vp, u32, vp, u32, u32, f, vp,
                                               vp, u32, vp],
         "hrnn_dev_stage1_features_dptr": [vp, vp, u32, u32, u32, u32, vp, vp, vp],
@@ -310,6 +308,37 @@ class Device:
             self._ck(rc)
             return off, ids[: int(off[-1])].copy()
 
+    def rknn_search_resident(self, q, proxies, k, K_prime, threshold=1.0, cap_per_q=4096):
+        """The resident form (hrnn_dev_rknn_search_dptr) with the answers
+        downloaded by hrnn_dev_fetch_results: (off[nq+1], ids)."""
+        from . import hiprt
+        q = np.ascontiguousarray(q, np.float32)
+        nq = q.shape[0]
+        proxies = np.ascontiguousarray(proxies, np.uint32).reshape(nq, -1)
+        m = proxies.shape[1]
+        qd = hiprt.DeviceArray.from_host(q)
+        pd = hiprt.DeviceArray.from_host(proxies)
+        cd = hiprt.DeviceArray((max(nq, 1),), np.uint32)
+        idd = hiprt.DeviceArray((max(nq, 1) * cap_per_q,), np.uint32)
+        self._ck(self.L.hrnn_dev_rknn_search_dptr(self.h, qd.ptr, nq, pd.ptr, m, None, k, K_prime, threshold,
+                                                  cd.ptr, idd.ptr, cap_per_q, None))
+        return self.fetch_results(nq, cd.ptr, idd.ptr, cap_per_q)
+
+    def fetch_results(self, nq, cnt_ptr, ids_ptr, cap_per_q, cap=None):
+        """hrnn_dev_fetch_results into host arrays: (off[nq+1] u64, ids u32)."""
+        off = np.zeros(nq + 1, np.uint64)
+        need = C.c_uint64(0)
+        cap = max(1024, nq * 16) if cap is None else cap
+        while True:
+            ids = np.empty(max(cap, 1), np.uint32)
+            rc = self.L.hrnn_dev_fetch_results(self.h, nq, cnt_ptr, ids_ptr, cap_per_q, _p(off), _p(ids),
+                                               cap, C.byref(need))
+            if rc == -5 and need.value > cap and int(need.value) <= nq * cap_per_q:
+                cap = int(need.value)
+                continue
+            self._ck(rc)
+            return off, ids[: int(off[-1])].copy()
+
     # -- phase A (HNSW beam) on the GPU -------------------------------------------
     def set_hnsw(self, maxM0, maxM, l0, levels, up_off, up_pool, entry, maxlevel):
         l0 = np.ascontiguousarray(l0, np.uint32)
@@ -362,13 +391,6 @@ class Device:
             return off, ids[:n].copy(), d2[:n].copy()
 
     # -- learned m ---------------------------------------------------------------
-    def query_features(self, q, k, ef, m_max=16):
-        q = np.ascontiguousarray(q, np.float32)
-        f = np.empty((q.shape[0], 32), np.float32)
-        p = np.empty((q.shape[0], m_max), np.uint32)
-        self._ck(self.L.hrnn_dev_query_features(self.h, _p(q), q.shape[0], k, ef, m_max, _p(f), _p(p)))
-        return f, p
-
     def stage1_features(self, q, ef1: int, m: int, k: int):
         """Stage-1 phase A at ef1 with the fused 32 features:
         returns (feats[nq, 32], proxies[nq, m], expansions[nq, 2])."""

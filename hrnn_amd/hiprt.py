@@ -32,6 +32,8 @@ def hip() -> C.CDLL:
         _hip.hipStreamSynchronize.argtypes = [vp]
         _hip.hipDeviceSynchronize.argtypes = []
         _hip.hipSetDevice.argtypes = [C.c_int]
+        _hip.hipHostMalloc.argtypes = [C.POINTER(vp), C.c_size_t, C.c_uint]
+        _hip.hipHostFree.argtypes = [vp]
         _hip.hipGetErrorString.restype = C.c_char_p
         _hip.hipGetErrorString.argtypes = [C.c_int]
     return _hip
@@ -103,6 +105,38 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+class PinnedArray:
+    """Page-locked host memory (hipHostMalloc) viewed as a numpy array: copies
+    between it and the device run at the link's rate without staging."""
+
+    def __init__(self, shape, dtype):
+        self.dtype = np.dtype(dtype)
+        shape = (shape,) if isinstance(shape, int) else tuple(shape)
+        nbytes = max(int(np.prod(shape)) * self.dtype.itemsize, 1)
+        p = C.c_void_p()
+        _ck(hip().hipHostMalloc(C.byref(p), nbytes, 0), "hipHostMalloc")
+        self.ptr = p
+        buf = (C.c_char * nbytes).from_address(p.value)
+        self.a = np.frombuffer(buf, dtype=self.dtype, count=int(np.prod(shape))).reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            self.a = None
+            hip().hipHostFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def copy_async(dst, src, nbytes: int, kind: int, stream) -> None:
+    """hipMemcpyAsync of nbytes between raw addresses on `stream`."""
+    _ck(hip().hipMemcpyAsync(dst, src, int(nbytes), kind, stream), "hipMemcpyAsync")
 
 
 class Event:

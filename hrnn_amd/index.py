@@ -46,6 +46,7 @@ def host_lib() -> C.CDLL:
         "hrnn_index_search_proxies": [vp, vp, u64, vp, u32, vp, u32, u32, f, vp, vp, u64, P(u64)],
         "hrnn_index_insert_dynamic": [vp, vp, u64, vp, u32, i, i],
         "hrnn_index_compact_for_serving": [vp, u32],
+        "hrnn_index_get_rknng_mutable": [vp, vp, vp],
         "hrnn_index_save": [vp, C.c_char_p],
         "hrnn_index_device": [vp, P(vp)],
         "hrnn_index_get_level0": [vp, vp, u32, vp],
@@ -223,6 +224,15 @@ class Index:
         labels = np.ascontiguousarray(labels, np.uint64)
         self._ck(self.L.hrnn_index_insert_dynamic(self.h, _p(x), x.shape[0], _p(labels), m_update,
                                                   int(bidirectional), nthreads))
+
+    def get_rknng_mutable(self):
+        """The mutable reverse lists: (off[n+1] u64, ids u32), each list in its own order."""
+        n = self.count
+        off = np.zeros(n + 1, np.uint64)
+        self._ck(self.L.hrnn_index_get_rknng_mutable(self.h, _p(off), None))
+        ids = np.empty(int(off[-1]), np.uint32)
+        self._ck(self.L.hrnn_index_get_rknng_mutable(self.h, _p(off), _p(ids)))
+        return off, ids
 
     def compact_for_serving(self, max_k):
         self._ck(self.L.hrnn_index_compact_for_serving(self.h, max_k))

@@ -501,8 +501,13 @@ class HRNN : public AlgorithmInterface<dist_t> {
             std::vector<tableint> gained;
             {
                 std::lock_guard<std::mutex> lk(node_locks_[xn]);
-                unsigned old_cnt = knn_cnt_[xn];
-                for (unsigned j = 0; j < old_cnt; j++) seen[knn_ids_[(size_t)xn * K + j]] = 1;
+                // x's list as it is now (a concurrent insertion may have
+                // changed it since nbrs was read): marked and cleared from
+                // this copy, so every mark set here is cleared here
+                const unsigned old_cnt = knn_cnt_[xn];
+                const std::vector<tableint> old_ids(knn_ids_.begin() + (size_t)xn * K,
+                                                    knn_ids_.begin() + (size_t)xn * K + old_cnt);
+                for (tableint id : old_ids) seen[id] = 1;
                 unsigned w = 0;
                 for (auto &pr : v) {
                     if (pr.second == xn || w >= K || seen[pr.second] == 2) continue;
@@ -513,7 +518,7 @@ class HRNN : public AlgorithmInterface<dist_t> {
                     w++;
                 }
                 knn_cnt_[xn] = (uint16_t)w;
-                for (unsigned j = 0; j < old_cnt; j++) seen[nbrs[j]] = 0;
+                for (tableint id : old_ids) seen[id] = 0;
                 for (auto &pr : v) seen[pr.second] = 0;
             }
             for (tableint y : gained) {

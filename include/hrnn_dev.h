@@ -172,6 +172,19 @@ int hrnn_dev_rknn_search_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq,
                               float threshold, uint32_t *res_cnt_dev,
                               uint32_t *res_ids_dev, uint32_t cap_per_q, uint32_t *work_dev);
 
+/* Download of a resident search's results, densely packed: on the handle's
+ * stream the counts are scanned (each capped at cap_per_q) into offsets and
+ * the ids gathered into one array on the device, then res_off[nq+1] (u64)
+ * and res_ids[res_off[nq]] are copied to the host (pinned host memory makes
+ * both copies run at the link's rate).  Synchronous.  HRNN_E_CAP with
+ * *needed set when a query's true count exceeds cap_per_q (its ids were cut
+ * short on the device; *needed = the largest count) or when the total
+ * exceeds cap (*needed = the total).  Replaces the per-query result vector
+ * that searchRknn returns (reference include/hnswlib/hrnn.h:2055-2071). */
+int hrnn_dev_fetch_results(hrnn_dev_t h, uint32_t nq, const uint32_t *res_cnt_dev,
+                           const uint32_t *res_ids_dev, uint32_t cap_per_q, uint64_t *res_off,
+                           uint32_t *res_ids, uint64_t cap, uint64_t *needed);
+
 /* ---- RkNN phase A on the GPU -------------------------------------------- */
 /* One query per workgroup (k_phase_a).  Graph ids and counts read by the
  * kernel are range-checked; a bad one sets the handle's device error word,
@@ -228,11 +241,6 @@ int hrnn_dev_bf_rknn(hrnn_dev_t h, const float *q, uint32_t nq, const float *kth
  * feature standardisation mu[32] sd[32] after them (HRNN_MLP_MODEL floats). */
 #define HRNN_MLP_PARAMS 7312
 #define HRNN_MLP_MODEL (HRNN_MLP_PARAMS + 64)
-/* SWITCHED OFF (returns HRNN_E_STATE): the device feature path ended in an
- * unexplained memory fault on its first run.  Use hrnn_index_query_features
- * (hrnn_host.h).  Was: the 32 query features and the m_max proxies. */
-int hrnn_dev_query_features(hrnn_dev_t h, const float *q, uint32_t nq, uint32_t k, uint32_t ef,
-                            uint32_t m_max, float *feats, uint32_t *proxies);
 /* probs[n*16] = softmax(MLP(feats)). */
 int hrnn_dev_mlp_forward(hrnn_dev_t h, const float *params, const float *feats, uint32_t n,
                          float *probs);
@@ -244,26 +252,12 @@ int hrnn_dev_mlp_grad(hrnn_dev_t h, const float *params, const float *feats,
 int hrnn_dev_mlp_train(hrnn_dev_t h, float *params, const float *feats, const uint32_t *labels,
                        uint32_t n, int epochs, uint32_t batch, float lr, float weight_decay,
                        uint64_t seed, float *epoch_loss);
-/* SWITCHED OFF (returns HRNN_E_STATE), with hrnn_dev_query_features; use
- * hrnn_dev_rknn_search_learned_dptr.  Was: whole search with learned m for a
- * resident batch: phase A with m_max
- * proxies and beam statistics, features standardised with the model's
- * mu/sd, MLP forward, per-query m = the smallest grid value whose cumulative
- * probability reaches tau (argmax when tau <= 0; proxies beyond it masked),
- * then phases B and C.  model_dev: HRNN_MLP_MODEL floats; grid_dev: 16 u32;
- * m_dev (NULL or nq u32) receives the chosen m. */
-int hrnn_dev_rknn_query_learned_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq,
-                                     const float *model_dev, const uint32_t *grid_dev, float tau,
-                                     uint32_t m_max, uint32_t ef, uint32_t k, uint32_t K_prime,
-                                     float threshold, uint32_t *proxies_dev, uint32_t *m_dev,
-                                     uint32_t *res_cnt_dev, uint32_t *res_ids_dev,
-                                     uint32_t cap_per_q, uint32_t *work_dev);
-
 /* Phases B and C with learned m for a resident batch whose phase A ran on the
  * host: feats_dev (nq*32 unstandardised features, standardised in place with
- * the model's mu/sd), MLP forward, per-query m as in
- * hrnn_dev_rknn_query_learned_dptr (proxies_dev rows of m_max masked beyond
- * m; m_dev may be NULL), then phases B and C.  All work is queued on the
+ * the model's mu/sd), MLP forward, per-query m = the smallest value of the
+ * 16-value grid (grid_dev) whose cumulative probability reaches tau (argmax
+ * when tau <= 0; proxies_dev rows of m_max masked beyond m; m_dev, NULL or
+ * nq u32, receives the chosen m), then phases B and C.  All work is queued on the
  * library stream; nothing is copied to the host. */
 int hrnn_dev_rknn_search_learned_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, float *feats_dev,
                                       const float *model_dev, const uint32_t *grid_dev, float tau,

@@ -79,6 +79,11 @@ int hrnn_index_search_proxies(hrnn_index_t ix, const float *q, uint64_t nq, cons
 int hrnn_index_insert_dynamic(hrnn_index_t ix, const float *x, uint64_t n, const uint64_t *labels,
                               uint32_t m_update, int bidirectional, int nthreads);
 int hrnn_index_compact_for_serving(hrnn_index_t ix, uint32_t max_k_serve);
+/* The mutable reverse lists (after buildRKNNG_mutable, kept up to date by
+ * insertPointDynamic; reference include/hnswlib/hrnn.h:1942-1959, :2360-2534)
+ * flattened: off[n+1] always, ids[off[n]] (each list in its own order) when
+ * ids is not NULL. */
+int hrnn_index_get_rknng_mutable(hrnn_index_t ix, uint64_t *off, uint32_t *ids);
 int hrnn_index_save(hrnn_index_t ix, const char *path);
 /* The index's device handle (opened on first use) with base, graph and CSR resident. */
 int hrnn_index_device(hrnn_index_t ix, hrnn_dev_t *out);

@@ -5,20 +5,37 @@
 //   RkNN GT         [nq u32][(nq+1) u32 offsets][ids u32][dists f32] (utils.h:76-108,
 //                   written by scripts/gen_rknn_gt.py:154-168)
 //   macro recall    mean over queries with non-empty GT of |R n GT| / |GT| (utils.h:135-157)
-// without the Boost dependency (the drivers parse flags with hrnn_amd/drivers/cli.h).
+//   kNN GT          [nq u32][k u32][nq*k ids u32][nq*k dists f32]   (utils.h:52-73)
+// and the reference's `po` namespace (utils.h:20-21): Boost.Program_options
+// where it is installed, else the subset in hrnn_po.h, so the reference's
+// drivers compile unchanged against this directory.
 #ifndef HRNN_UTILS_H
 #define HRNN_UTILS_H
 
 #include <algorithm>
 #include <chrono>
 #include <cstdint>
+#include <filesystem>
 #include <fstream>
+#include <iostream>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <unordered_set>
 #include <vector>
 
 #include "hnswlib/hnswlib.h"
+
+#if __has_include(<boost/program_options.hpp>)
+#include <boost/program_options.hpp>
+namespace po = boost::program_options;
+#else
+#include "hrnn_po.h"
+#endif
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 struct FbinData {
     uint32_t n = 0, d = 0;
@@ -45,6 +62,42 @@ inline void save_fbin(const std::string &path, const float *x, uint32_t n, uint3
     out.write(reinterpret_cast<const char *>(&d), 4);
     out.write(reinterpret_cast<const char *>(x), (std::streamsize)((size_t)n * d * 4));
     if (!out) throw std::runtime_error("Write error: " + path);
+}
+
+struct KnnGroundtruth {
+    uint32_t nq = 0, k = 0;
+    std::vector<uint32_t> ids;   // [nq][k]
+    std::vector<float> dists;    // [nq][k]
+};
+
+inline KnnGroundtruth load_knn_groundtruth(const std::string &path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("Cannot open: " + path);
+    KnnGroundtruth gt;
+    in.read(reinterpret_cast<char *>(&gt.nq), 4);
+    in.read(reinterpret_cast<char *>(&gt.k), 4);
+    if (!in) throw std::runtime_error("Read error: " + path);
+    const size_t total = (size_t)gt.nq * gt.k;
+    gt.ids.resize(total);
+    gt.dists.resize(total);
+    in.read(reinterpret_cast<char *>(gt.ids.data()), (std::streamsize)(total * 4));
+    in.read(reinterpret_cast<char *>(gt.dists.data()), (std::streamsize)(total * 4));
+    if (!in) throw std::runtime_error("Read error: " + path);
+    return gt;
+}
+
+// kNN recall: hits of results[q] among the first min(k, gt.k) GT ids, over
+// nq * min(k, gt.k) (utils.h:115-132).
+inline double compute_knn_recall(const KnnGroundtruth &gt,
+                                 const std::vector<std::vector<hnswlib::labeltype>> &results, int k) {
+    const size_t kk = std::min<size_t>((size_t)std::max(k, 0), gt.k);
+    size_t found = 0;
+    for (size_t q = 0; q < gt.nq; q++) {
+        std::unordered_set<uint32_t> s(gt.ids.begin() + q * gt.k, gt.ids.begin() + q * gt.k + kk);
+        for (auto l : results[q]) found += s.count((uint32_t)l);
+    }
+    const size_t total = (size_t)gt.nq * kk;
+    return total ? (double)found / (double)total : 0.0;
 }
 
 struct RknnGroundtruth {

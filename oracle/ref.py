@@ -75,6 +75,7 @@ def _lib(variant: str | None = None) -> C.CDLL:
     lib.ref_load.argtypes = [C.c_char_p, u32, u64]
     lib.ref_build_rknng_mutable.argtypes = [vp]
     lib.ref_insert_dynamic.argtypes = [vp, vp, u64, vp, u32, i, i]
+    lib.ref_get_rknng_mutable.argtypes = [vp, vp, vp]
     lib.ref_get_level0.argtypes = [vp, vp, vp, u32]
     lib.ref_levels.argtypes = [vp, vp, C.POINTER(i), C.POINTER(u32)]
     lib.ref_get_labels.argtypes = [vp, vp]
@@ -209,6 +210,15 @@ class RefIndex:
         labels = np.ascontiguousarray(labels, np.uint64)
         self._check(self.lib.ref_insert_dynamic(self.h, _p(x), x.shape[0], _p(labels), m_update,
                                                 int(bidir), nthreads))
+
+    def get_rknng_mutable(self):
+        """rknng_lists_ flattened: (off[n+1] u64, ids u32), each list in its own order."""
+        n = self.count
+        off = np.zeros(n + 1, np.uint64)
+        self._check(self.lib.ref_get_rknng_mutable(self.h, _p(off), None))
+        ids = np.empty(int(off[-1]), np.uint32)
+        self._check(self.lib.ref_get_rknng_mutable(self.h, _p(off), _p(ids)))
+        return off, ids
 
     def labels(self):
         out = np.empty(self.count, np.uint64)

@@ -273,6 +273,22 @@ int ref_insert_dynamic(void *h, const float *x, uint64_t n, const uint64_t *labe
     });
 }
 
+// The mutable reverse lists (rknng_lists_) flattened: off[n+1] always, the
+// ids (in each list's own order) when ids is not null.
+int ref_get_rknng_mutable(void *h, uint64_t *off, uint32_t *ids) {
+    auto *r = static_cast<RefIndex *>(h);
+    return guarded([&] {
+        auto &ix = *r->idx;
+        const size_t n = ix.cur_element_count;
+        off[0] = 0;
+        for (size_t i = 0; i < n; i++)
+            off[i + 1] = off[i] + (i < ix.rknng_lists_.size() ? ix.rknng_lists_[i].size() : 0);
+        if (!ids) return;
+        for (size_t i = 0; i < n && i < ix.rknng_lists_.size(); i++)
+            std::memcpy(ids + off[i], ix.rknng_lists_[i].data(), ix.rknng_lists_[i].size() * 4);
+    });
+}
+
 // HNSW layer-0 adjacency for structure checks: out[i*maxM0 ..] links, cnt[i].
 int ref_get_level0(void *h, uint32_t *links, uint32_t *cnt, uint32_t maxM0) {
     auto *r = static_cast<RefIndex *>(h);

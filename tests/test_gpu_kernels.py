@@ -185,6 +185,36 @@ def test_search_lds_table_and_hbm_fallback(small_data, small_ref, lds_kib, thr, 
     assert np.array_equal(gids, eids)
 
 
+@pytest.mark.parametrize("m,thr", [(5, 1.0), (10, 2.0)])
+def test_search_resident_and_fetch_results(small_data, small_ref, m, thr):
+    """The resident search with its answers packed on the device and
+    downloaded by hrnn_dev_fetch_results equals the oracle; a per-query
+    capacity below the largest answer is reported (HRNN_E_CAP, needed = the
+    largest count) instead of returning cut lists; an empty batch returns
+    off = [0]."""
+    x, q = small_data
+    r, _ = small_ref
+    ids, dists, cnt = r.get_knng()
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    dev.set_knng(ids, dists, cnt)
+    dev.build_rknn_csr()
+    off, ent = dev.get_rknn_csr()
+    prox = r.proxies(q, m, 64)
+    goff, gids = dev.rknn_search_resident(q, prox, 10, 48, thr)
+    eoff, eids = oracle.search_bc(off, ent, dev.csr.wide, dev.csr.rank_bits, dists, cnt, r.K, x,
+                                  q, prox, 10, 48, thr)
+    assert np.array_equal(goff, eoff) and np.array_equal(gids, eids)
+    big = int(np.diff(eoff).max())
+    assert big >= 2
+    with pytest.raises(hrnn_amd.HrnnError, match="cap_per_q"):
+        dev.rknn_search_resident(q, prox, 10, 48, thr, cap_per_q=big - 1)
+    goff2, gids2 = dev.rknn_search_resident(q, prox, 10, 48, thr, cap_per_q=big)
+    assert np.array_equal(goff2, eoff) and np.array_equal(gids2, eids)
+    z_off, z_ids = dev.rknn_search_resident(q[:0], prox[:0], 10, 48, thr)
+    assert z_off.tolist() == [0] and z_ids.size == 0
+
+
 def test_search_hub_results_above_m_times_kprime():
     """A proxy whose reverse list is far longer than K' (node 0 is everyone's
     nearest neighbour): one proxy and K' = 1 return ~2000 ids, which the

@@ -50,8 +50,11 @@ def test_training_fits_a_learnable_rule(dev):
     assert float((np.abs(pred.astype(int) - y.astype(int)) <= 1).mean()) > 0.9
 
 
-@pytest.mark.skip(reason="uses the GPU phase-A beam and feature kernels, switched off: this test's "
-                         "first GPU run ended in a memory fault whose cause is not found")
+@pytest.mark.skip(reason="its first GPU run ended in a memory fault (the persistent phase-A beam "
+                         "k_beam with the feature kernel k_features); both kernels and the device "
+                         "feature entry points are removed, the cause was not found; the learned m "
+                         "is tested with host features (test_learned_m_gain_host_features) and with "
+                         "features fused into k_phase_a (tests/test_gpu_adaptive.py)")
 def test_learned_m_gain_over_fixed_m50(dev):
     """10k x 128 mixture index (K=100): train on 2000 queries, evaluate on 300
     held-out ones against fixed m = 50 (the proxy count the learned form is

@@ -227,3 +227,84 @@ def test_parallel_build_seeds_every_node():
         _, _, sc = ix.get_knng()
         late = np.arange(10_000) >= 200          # past the first few hundred insertions
         assert int((sc[late] < 50).sum()) <= 2, (nt, np.bincount(np.minimum(sc, 100))[:10])
+
+
+def _reverse_invariant(ids, cnt, off, rids):
+    """Every KNN entry j of node i has i in j's mutable reverse list."""
+    n = len(cnt)
+    rsets = [set(rids[off[j]:off[j + 1]].tolist()) for j in range(n)]
+    missing = 0
+    for i in range(n):
+        for j in ids[i, :cnt[i]].tolist():
+            missing += i not in rsets[j]
+    return missing
+
+
+def _insert_setup(tmp_path, n_base, x, K):
+    """A saved index of the first n_base rows whose lists are the oracle's
+    NNDescent result, so both implementations start from identical bytes."""
+    ix = Index(n_base, 128, 16, 64, 100, K)
+    ix.add_points(x[:n_base], nthreads=1)
+    sid, sd, sc = ix.get_knng()
+    oid, od, oc, _, _ = oracle.nndescent(x[:n_base], sid, sd, sc, K, 3, 10)
+    ix.set_knng(oid, od, oc)
+    p = str(tmp_path / "base.idx")
+    ix.save(p)
+    return p
+
+
+@needs_ref
+@pytest.mark.parametrize("bidir", [False, True])
+def test_insert_dynamic_matches_reference_single_thread(tmp_path, data, bidir):
+    """insertPointDynamic (reference hrnn.h:2360-2534, try_update_knng_evict
+    :1592-1655) and the mutable reverse lists (:1942-1959): from the same
+    index file, one thread inserting the same rows in the same order, both
+    sides end with the same ranked lists (counts and distances; ids up to the
+    order of equal distances) and the same reverse lists as multisets."""
+    x, _ = data
+    n_base, n_all, K = 1200, 1800, 24
+    p = _insert_setup(tmp_path, n_base, x, K)
+    ours = Index.load(p, 128, n_all)
+    ref = oracle.RefIndex.load(p, 128, n_all)
+    ref.K = K
+    ours.build_rknng_mutable()
+    ref.build_rknng_mutable()
+    lab = np.arange(n_base, n_all, dtype=np.uint64)
+    ours.insert_dynamic(x[n_base:n_all], lab, 0, bidir, nthreads=1)
+    ref.insert_dynamic(x[n_base:n_all], lab, 0, bidir, nthreads=1)
+    assert ours.count == ref.count == n_all
+    ids, d, cnt = ours.get_knng()
+    rid, rd, rc = ref.get_knng()
+    assert np.array_equal(ours.labels(), ref.labels())
+    assert np.array_equal(cnt, rc)
+    m = np.arange(K) < cnt[:, None]
+    assert np.array_equal(np.where(m, d, 0), np.where(m, rd, 0))
+    assert float((np.where(m, ids, 0) == np.where(m, rid, 0)).mean()) > 0.99
+    off, rl = ours.get_rknng_mutable()
+    roff, rrl = ref.get_rknng_mutable()
+    assert np.array_equal(np.diff(off), np.diff(roff))
+    same = np.mean([np.array_equal(np.sort(rl[off[i]:off[i + 1]]), np.sort(rrl[roff[i]:roff[i + 1]]))
+                    for i in range(n_all)])
+    assert same > 0.99, same
+    assert _reverse_invariant(ids, cnt, off, rl) == 0
+
+
+def test_insert_dynamic_bidirectional_multithreaded_keeps_reverse_lists(tmp_path, data):
+    """Eight threads inserting with bidirectional=True: afterwards every KNN
+    entry j of every node i has i in j's reverse list, and no list holds an
+    id twice (the list rewrite of the inserted node marks and clears its
+    dedup buffer from the list as it is under the node's lock)."""
+    x, _ = data
+    n_base, n_all, K = 1000, 2500, 24
+    p = _insert_setup(tmp_path, n_base, x, K)
+    ix = Index.load(p, 128, n_all)
+    ix.build_rknng_mutable()
+    lab = np.arange(n_base, n_all, dtype=np.uint64)
+    ix.insert_dynamic(x[n_base:n_all], lab, 0, True, nthreads=8)
+    ids, d, cnt = ix.get_knng()
+    for i in range(n_all):
+        row = ids[i, :cnt[i]]
+        assert len(set(row.tolist())) == len(row), i
+        assert np.all(np.diff(d[i, :cnt[i]]) >= 0), i
+    off, rl = ix.get_rknng_mutable()
+    assert _reverse_invariant(ids, cnt, off, rl) == 0
