@@ -314,8 +314,9 @@ class Device:
         from . import hiprt
         q = np.ascontiguousarray(q, np.float32)
         nq = q.shape[0]
-        proxies = np.ascontiguousarray(proxies, np.uint32).reshape(nq, -1)
-        m = proxies.shape[1]
+        proxies = np.ascontiguousarray(proxies, np.uint32)
+        m = proxies.shape[-1]
+        proxies = proxies.reshape(nq, m)
         qd = hiprt.DeviceArray.from_host(q)
         pd = hiprt.DeviceArray.from_host(proxies)
         cd = hiprt.DeviceArray((max(nq, 1),), np.uint32)

@@ -92,6 +92,54 @@ def test_kth_f64_matches_on_integers():
     assert np.array_equal(oracle.kth_f64(x, 4).astype(np.float32), oracle.kth(x, 4))
 
 
+def test_kth_f64_float_data_vs_float64_brute_force():
+    """The FP64 judge on float data (the precision form's generator: |N(0,1)|
+    scaled per dimension, d = 960): r_k is the (k+1)-th smallest float64
+    direct-difference distance, self included.  Against a float64 numpy brute
+    force (pairwise summation: differs only in the last bits) at 1e-12
+    relative; the FP32 oracle misses that bar on most rows, so a slip to
+    FP32 accumulation, or to the k-th instead of the (k+1)-th distance,
+    fails."""
+    import hrnn_amd
+    x = hrnn_amd.synth_halfnormal_scaled(5, 55, 300, 960)
+    k = 10
+    got = oracle.kth_f64(x, k)
+    xd = x.astype(np.float64)
+    exp = np.empty(300)
+    exp_k = np.empty(300)
+    for b in range(0, 300, 20):
+        dd = ((xd[b:b + 20, None, :] - xd[None, :, :]) ** 2).sum(-1)
+        part = np.sort(dd, axis=1)
+        exp[b:b + 20] = part[:, k]
+        exp_k[b:b + 20] = part[:, k - 1]
+    assert np.allclose(got, exp, rtol=1e-12, atol=0)
+    assert not np.allclose(exp_k, exp, rtol=1e-12, atol=0)
+    f32 = oracle.kth(x, k).astype(np.float64)
+    assert np.mean(np.abs(f32 - exp) > 1e-12 * exp) > 0.5
+
+
+def test_knng_recall_hand_built_graph():
+    """orc_knng_recall: per node, the first kk exact neighbours found among
+    the first min(cnt, kk) list entries, pooled over nodes.  Hand-counted:
+    node 0 (cnt 1) finds 5 of exact [7, 5]; node 1 (cnt 4) finds 1 of
+    [3, 1] in its first two entries [1, 2]; node 2 (cnt 0) finds nothing of
+    [0, 1]: 2 hits of 6.  Counting past cnt gives 5/6 and an off-by-one in
+    kk 4/9."""
+    K = 4
+    ids = np.array([[5, 7, 9, 1], [1, 2, 3, 4], [0, 1, 2, 3]], np.uint32)
+    cnt = np.array([1, 4, 0], np.uint16)
+    exact = np.array([[7, 5, 9], [3, 1, 2], [0, 1, 2]], np.uint32)
+    assert oracle.knng_recall(ids, cnt, K, exact, 2) == pytest.approx(2 / 6, abs=1e-15)
+    assert oracle.knng_recall(ids, cnt, K, exact, 3) == pytest.approx(4 / 9, abs=1e-15)
+    full = np.full(3, 4, np.uint16)
+    assert oracle.knng_recall(ids, full, K, exact, 2) == pytest.approx(5 / 6, abs=1e-15)
+    # a list that is its exact neighbours, in any order, scores 1
+    perm = np.array([[5, 7, 1, 9], [3, 1, 4, 2], [2, 0, 1, 3]], np.uint32)
+    assert oracle.knng_recall(perm, full, K, exact, 2) < 1.0
+    same = np.array([[5, 7, 9, 0], [1, 3, 2, 0], [1, 0, 2, 0]], np.uint32)
+    assert oracle.knng_recall(same, np.full(3, 3, np.uint16), K, exact, 3) == pytest.approx(1.0)
+
+
 def test_rknn_gt_brute_force():
     """gen_rknn_gt.py:82-151: RkNN(q) = {x : d2(q,x) <= r_k(x)}, inclusive."""
     x = mixture(500, d=32, seed=25)
