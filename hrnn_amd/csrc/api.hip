@@ -437,6 +437,24 @@ int hrnn_dev_get_knng(hrnn_dev_t h, uint32_t *ids, float *dists, uint16_t *cnt) 
     });
 }
 
+int hrnn_dev_get_knng_flags(hrnn_dev_t h, uint8_t *flags) {
+    if (!h) return HRNN_E_ARG;
+    return guarded(h, [&] {
+        REQUIRE(h->K > 0, HRNN_E_STATE, "get_knng_flags: no graph");
+        REQUIRE(flags, HRNN_E_ARG, "get_knng_flags: null argument");
+        d2h(h, flags, h->knn_new.p, h->n * h->K);
+    });
+}
+
+int hrnn_dev_set_knng_flags(hrnn_dev_t h, const uint8_t *flags) {
+    if (!h) return HRNN_E_ARG;
+    return guarded(h, [&] {
+        REQUIRE(h->K > 0, HRNN_E_STATE, "set_knng_flags: no graph");
+        REQUIRE(flags, HRNN_E_ARG, "set_knng_flags: null argument");
+        h2d(h, h->knn_new.p, flags, h->n * h->K);
+    });
+}
+
 int hrnn_dev_knng_ptrs(hrnn_dev_t h, uint32_t **ids, float **dists, uint32_t **cnt) {
     if (!h) return HRNN_E_ARG;
     h->knn_version++;   // the caller may write through these pointers

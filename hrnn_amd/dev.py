@@ -31,7 +31,8 @@ EXPORTS = [
     "hrnn_dev_sync", "hrnn_dev_stream", "hrnn_dev_set_base", "hrnn_dev_set_base_dptr",
     "hrnn_dev_append_base", "hrnn_dev_base_is_small_int", "hrnn_dev_base_ptr",
     "hrnn_dev_set_vector_precision", "hrnn_dev_set_tuning", "hrnn_dev_get_tuning",
-    "hrnn_dev_set_knng", "hrnn_dev_get_knng", "hrnn_dev_knng_ptrs", "hrnn_dev_nndescent",
+    "hrnn_dev_set_knng", "hrnn_dev_get_knng", "hrnn_dev_get_knng_flags", "hrnn_dev_set_knng_flags",
+    "hrnn_dev_knng_ptrs", "hrnn_dev_nndescent",
     "hrnn_dev_build_rknn_csr", "hrnn_dev_get_rknn_csr", "hrnn_dev_set_rknn_csr",
     "hrnn_dev_rknn_search", "hrnn_dev_rknn_search_dptr", "hrnn_dev_fetch_results", "hrnn_dev_set_hnsw", "hrnn_dev_proxies",
     "hrnn_dev_proxies_dptr", "hrnn_dev_rknn_query_dptr", "hrnn_dev_bf_kth", "hrnn_dev_bf_knn",
@@ -84,6 +85,8 @@ def lib() -> C.CDLL:
         "hrnn_dev_set_knng": [vp, u32, vp, vp, vp],
         "hrnn_dev_get_knng": [vp, vp, vp, vp],
         "hrnn_dev_knng_ptrs": [vp, vp, vp, vp],
+        "hrnn_dev_get_knng_flags": [vp, vp],
+        "hrnn_dev_set_knng_flags": [vp, vp],
         "hrnn_dev_nndescent": [vp, i, i, u32, vp],
         "hrnn_dev_build_rknn_csr": [vp, P(u32), P(i), P(u64)],
         "hrnn_dev_get_rknn_csr": [vp, vp, vp],
@@ -260,6 +263,15 @@ class Device:
         cnt = np.empty(self.n, np.uint16)
         self._ck(self.L.hrnn_dev_get_knng(self.h, _p(ids), _p(dists), _p(cnt)))
         return ids, dists, cnt
+
+    def get_knng_flags(self):
+        f = np.empty((self.n, self.K), np.uint8)
+        self._ck(self.L.hrnn_dev_get_knng_flags(self.h, _p(f)))
+        return f
+
+    def set_knng_flags(self, flags):
+        f = np.ascontiguousarray(flags, np.uint8).reshape(self.n, self.K)
+        self._ck(self.L.hrnn_dev_set_knng_flags(self.h, _p(f)))
 
     def nndescent(self, iters: int, sample: int, bucket: int = 0):
         st = (NndIterStats * max(iters, 1))()

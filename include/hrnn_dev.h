@@ -106,6 +106,12 @@ int hrnn_dev_get_tuning(hrnn_dev_t h, const char *key, int *value);
 int hrnn_dev_set_knng(hrnn_dev_t h, uint32_t K, const uint32_t *ids,
                       const float *dists, const uint16_t *cnt);
 int hrnn_dev_get_knng(hrnn_dev_t h, uint32_t *ids, float *dists, uint16_t *cnt);
+/* The entries' NNDescent "new" flags, u8 [n*K] (the reference's per-entry
+ * is_new vector, hrnn.h:1744-1748, :1775-1778): read after refinement, or
+ * set after hrnn_dev_set_knng to resume refinement from a saved state
+ * (entries j >= cnt[i] are ignored). */
+int hrnn_dev_get_knng_flags(hrnn_dev_t h, uint8_t *flags);
+int hrnn_dev_set_knng_flags(hrnn_dev_t h, const uint8_t *flags);
 /* Device pointers of the resident graph (ids n*K u32, dists n*K f32, cnt n u32). */
 int hrnn_dev_knng_ptrs(hrnn_dev_t h, uint32_t **ids, float **dists, uint32_t **cnt);
 

@@ -77,6 +77,9 @@ def orc() -> C.CDLL:
         lib.orc_nndescent.argtypes = [_f32p, C.c_uint64, C.c_uint32, C.c_uint32, _u32p, _f32p,
                                       _u16p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                       C.c_void_p]
+        lib.orc_nndescent_flags.argtypes = [_f32p, C.c_uint64, C.c_uint32, C.c_uint32, _u32p, _f32p,
+                                            _u16p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                            C.c_void_p, C.c_void_p]
         lib.orc_knng_recall.restype = C.c_double
         lib.orc_knng_recall.argtypes = [_u32p, _u16p, C.c_uint32, _u32p, C.c_uint32,
                                         C.c_uint64, C.c_uint32]
@@ -207,6 +210,24 @@ def nndescent(x, ids, dists, cnt, K, iters, sample, W=L2_W, fma=L2_FMA):
     assert rc == 0
     n = x.shape[0]
     return ids.reshape(n, K), dists.reshape(n, K), cnt, su, sd
+
+
+def nndescent_flags(x, ids, dists, cnt, flags, K, iters, sample, W=L2_W, fma=L2_FMA):
+    """nndescent resumed from a state with the entries' "new" flags (u8
+    [n, K]); returns (ids, dists, cnt, flags, updates, dists_cnt)."""
+    x = _f32(x)
+    ids = np.ascontiguousarray(ids, np.uint32).reshape(-1).copy()
+    dists = _f32(dists).reshape(-1).copy()
+    cnt = np.ascontiguousarray(cnt, np.uint16).copy()
+    fl = np.ascontiguousarray(flags, np.uint8).reshape(-1).copy()
+    su = np.zeros(iters, np.uint64)
+    sd = np.zeros(iters, np.uint64)
+    rc = orc().orc_nndescent_flags(x, x.shape[0], x.shape[1], K, ids, dists, cnt, iters, sample, W,
+                                   fma, su.ctypes.data_as(C.c_void_p), sd.ctypes.data_as(C.c_void_p),
+                                   fl.ctypes.data_as(C.c_void_p))
+    assert rc == 0
+    n = x.shape[0]
+    return ids.reshape(n, K), dists.reshape(n, K), cnt, fl.reshape(n, K), su, sd
 
 
 def knng_recall(ids, cnt, K, exact, kk):

@@ -371,9 +371,13 @@ static int nnd_try_update(uint32_t u, uint32_t w, float dd, uint32_t *ids, float
     return 1;
 }
 
-int orc_nndescent(const float *x, uint64_t n, uint32_t d, uint32_t K, uint32_t *ids,
-                  float *dist, uint16_t *cnt, int iters, int sample, int W, int use_fma,
-                  uint64_t *stats_updates, uint64_t *stats_dists) {
+/* The same with the entries' "new" flags (u8 [n*K]) given and returned:
+ * flags NULL starts every listed entry new, as refineKNNG does
+ * (hrnn.h:1744-1748); otherwise the iterations start from flags and leave
+ * the final flags there, so a run can resume from any state. */
+int orc_nndescent_flags(const float *x, uint64_t n, uint32_t d, uint32_t K, uint32_t *ids,
+                        float *dist, uint16_t *cnt, int iters, int sample, int W, int use_fma,
+                        uint64_t *stats_updates, uint64_t *stats_dists, uint8_t *flags) {
     uint32_t S = (uint32_t)sample, R = S;
     uint8_t *isnew = (uint8_t *)malloc(n * K);
     uint32_t *nn_new = (uint32_t *)malloc(sizeof(uint32_t) * n * (S + R));
@@ -385,7 +389,7 @@ int orc_nndescent(const float *x, uint64_t n, uint32_t d, uint32_t K, uint32_t *
     uint32_t *c_rnew = (uint32_t *)malloc(sizeof(uint32_t) * n);
     uint32_t *c_rold = (uint32_t *)malloc(sizeof(uint32_t) * n);
     for (uint64_t i = 0; i < n; i++)
-        for (uint32_t j = 0; j < K; j++) isnew[i * K + j] = j < cnt[i] ? 1 : 0;
+        for (uint32_t j = 0; j < K; j++) isnew[i * K + j] = j < cnt[i] ? (flags ? (flags[i * K + j] != 0) : 1) : 0;
     for (int it = 0; it < iters; it++) {
         uint64_t upd = 0, nd = 0;
         memset(c_new, 0, sizeof(uint32_t) * n);
@@ -445,9 +449,17 @@ int orc_nndescent(const float *x, uint64_t n, uint32_t d, uint32_t K, uint32_t *
         if (stats_updates) stats_updates[it] = upd;
         if (stats_dists) stats_dists[it] = nd;
     }
+    if (flags) memcpy(flags, isnew, n * K);
     free(isnew); free(nn_new); free(nn_old); free(rnn_new); free(rnn_old);
     free(c_new); free(c_old); free(c_rnew); free(c_rold);
     return ORC_OK;
+}
+
+int orc_nndescent(const float *x, uint64_t n, uint32_t d, uint32_t K, uint32_t *ids,
+                  float *dist, uint16_t *cnt, int iters, int sample, int W, int use_fma,
+                  uint64_t *stats_updates, uint64_t *stats_dists) {
+    return orc_nndescent_flags(x, n, d, K, ids, dist, cnt, iters, sample, W, use_fma, stats_updates,
+                               stats_dists, NULL);
 }
 
 /* ------------------------------------------------------------------------

@@ -430,6 +430,89 @@ def _first_iteration_matches_oracle(x, sid, sd, sc, K, S):
     print("nodes with identical id lists:", same_ids)
 
 
+def _offers_fit(ids, d, cnt, flags, K):
+    """Reverse offers per node in the next iteration (new and old separately)
+    all fit the 256 the sampling keeps, so the reverse samples are exact."""
+    n = ids.shape[0]
+    worst = np.where(cnt >= K, d[np.arange(n), np.maximum(cnt.astype(np.int64) - 1, 0)], np.inf)
+    for want in (1, 0):
+        offers = np.zeros(n, np.int64)
+        for i in range(n):
+            c = int(cnt[i])
+            nb, dd, fl = ids[i, :c], d[i, :c], flags[i, :c]
+            np.add.at(offers, nb[(dd > worst[nb]) & (fl == want)], 1)
+        assert offers.max() <= 256, (want, offers.max())
+
+
+def _resume_matches_oracle(x, ids, d, cnt, flags, K, S, iters):
+    """From a state with flags: the GPU and the oracle evaluate the same
+    number of pairs per iteration and end with the same distance lists."""
+    _offers_fit(ids, d, cnt, flags, K)
+    oid, od, oc, ofl, ou, odist = oracle.nndescent_flags(x, ids, d, cnt, flags, K, iters, S)
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    dev.set_knng(ids, d, cnt)
+    dev.set_knng_flags(flags)
+    st = dev.nndescent(iters, S)
+    gid, gd, gc = dev.get_knng()
+    assert all(t["dropped"] == 0 for t in st)
+    assert [int(t["dists"]) for t in st] == [int(v) for v in odist]
+    assert np.array_equal(gc, oc)
+    m = np.arange(K) < gc[:, None]
+    assert np.array_equal(np.where(m, gd, 0), np.where(m, od, 0))
+    return st, (gid, gd, gc, dev.get_knng_flags()), (oid, od, oc, ofl)
+
+
+def test_nndescent_second_iteration_exact(small_data, small_ref):
+    """Iteration 2 from the GPU's own state after iteration 1 (lists and "new"
+    flags read back): the old-entry sampling pass, the need bits and the
+    merge's already-listed filter all act, and the GPU still evaluates
+    exactly the oracle's pairs and ends with its distance lists.  Where the
+    two first iterations left identical id lists, the flags agree as well
+    (entries inserted in the iteration are new, all others old)."""
+    x, _ = small_data
+    r, seed = small_ref
+    sid, sd, sc = seed
+    K, S = r.K, 12
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    dev.set_knng(sid, sd, sc)
+    dev.nndescent(1, S)
+    g1 = dev.get_knng()
+    f1 = dev.get_knng_flags()
+    oid, od, oc, ofl, _, _ = oracle.nndescent_flags(x, sid, sd, sc, np.ones_like(f1), K, 1, S)
+    m = np.arange(K) < oc[:, None]
+    same = (np.where(m, g1[0], 0) == np.where(m, oid, 0)).all(1) & (g1[2] == oc)
+    assert same.mean() > 0.5
+    assert np.array_equal(np.where(m, f1, 0)[same], np.where(m, ofl, 0)[same])
+    assert f1[m].any() and not f1[m].all()
+    st, _, _ = _resume_matches_oracle(x, g1[0], g1[1], g1[2], f1, K, S, 1)
+    assert st[0]["dists"] > 0 and st[0]["updates"] > 0
+
+
+def test_nndescent_iteration_without_updates(small_data, small_ref):
+    """From the oracle's state after 10 iterations (lists and flags), the
+    next iteration evaluates a few pairs and inserts nothing, so the one
+    after has nothing new: the GPU evaluates the oracle's pairs, skips the
+    second iteration's sampling pass (nothing is new after an iteration
+    without insertions), both sides evaluate zero pairs there, and the lists
+    stay as they were."""
+    x, _ = small_data
+    r, seed = small_ref
+    sid, sd, sc = seed
+    K, S = r.K, 12
+    ones = np.ones((len(x), K), np.uint8)
+    cid, cd, cc, cf, _, _ = oracle.nndescent_flags(x, sid, sd, sc, ones, K, 10, S)
+    _, _, _, _, ou, odist = oracle.nndescent_flags(x, cid, cd, cc, cf, K, 2, S)
+    assert int(ou[0]) == 0 and int(odist[0]) > 0 and int(odist[1]) == 0, (ou, odist)
+    st, (gid, gd, gc, gf), _ = _resume_matches_oracle(x, cid, cd, cc, cf, K, S, 2)
+    assert st[0]["updates"] == 0 and st[1]["dists"] == 0 and st[1]["updates"] == 0
+    assert np.array_equal(gc, cc)
+    m = np.arange(K) < cc[:, None]
+    assert np.array_equal(np.where(m, gid, 0), np.where(m, cid, 0))
+    assert not gf[m].any()
+
+
 @pytest.mark.parametrize("join_mode", [None, "1", "0"])
 def test_nndescent_first_iteration_exact(small_data, small_ref, join_mode, monkeypatch):
     """One iteration from the same seeds: the GPU evaluates exactly the pairs

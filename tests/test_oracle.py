@@ -308,6 +308,27 @@ def test_nndescent_non_full_lists_vs_reference():
 
 # ---- golden fixtures written by the compiled reference ---------------------------
 
+def test_nndescent_flags_resume_equals_one_run(small_data, small_ref):
+    """orc_nndescent_flags: one iteration, then two more from the returned
+    lists and flags, equals three iterations in one call (which is pinned
+    bit-exact to the compiled refineKNNG above); all-new flags are the
+    refineKNNG start state."""
+    x, _ = small_data
+    r, seed = small_ref
+    sid, sd, sc = seed
+    K, S = r.K, 12
+    whole = oracle.nndescent(x, sid, sd, sc, K, 3, S)
+    ones = np.ones((len(x), K), np.uint8)
+    a = oracle.nndescent_flags(x, sid, sd, sc, ones, K, 1, S)
+    b = oracle.nndescent_flags(x, a[0], a[1], a[2], a[3], K, 2, S)
+    for i in range(3):
+        assert np.array_equal(whole[i], b[i])
+    assert np.concatenate([a[4], b[4]]).tolist() == whole[3].tolist()
+    assert np.concatenate([a[5], b[5]]).tolist() == whole[4].tolist()
+    m = np.arange(K) < a[2][:, None]
+    assert a[3][m].any() and not a[3][~m].any()
+
+
 def test_golden_fixtures_pin_oracle():
     """Fixtures from tests/golden/make_golden.py (the reference compiled single-threaded)."""
     path = os.path.join(GOLD, "ref_small.npz")
