@@ -239,6 +239,7 @@ int hrnn_dev_open(int device, hrnn_dev_t *out) {
         h->counters.reserve(4096);
         HIPC(hipMemsetAsync(h->counters.p, 0, 4096, h->stream));
         HIPC(hipStreamSynchronize(h->stream));
+        noalloc_scratch_init(h);
     });
     if (rc != HRNN_OK) {
         delete h;

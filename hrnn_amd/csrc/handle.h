@@ -138,6 +138,17 @@ struct SearchSel {
     bool queue_zeroed = false;   // an earlier kernel on the stream already zeroed the queue
     int buffer_set = 0;          // 1: the second fallback buffers (a search overlapping another)
     uint32_t max_slots = 0;      // cap on the persistent workgroups (0: the tuned count)
+    // caller-free form: the queue word and the fallback bitmaps / candidate
+    // buffers carved from this device memory instead of the handle's buffers
+    // (nothing is allocated, nothing is cleared with a memset; the r_k column
+    // is used only when it is already built for this k)
+    uint8_t *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // bytes from the start of the bitmap area known to be zero: the launch
+    // clears the part of its bitmaps beyond it (earlier calls may have left
+    // candidate ids there) and sets it to its own bitmaps' size (the kernel
+    // leaves its bitmaps zero; the candidate buffers after them are dirty)
+    size_t *scratch_clean = nullptr;
 };
 
 void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint32_t *proxies,
@@ -146,6 +157,10 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
                           uint32_t cap_per_q, uint32_t *work = nullptr,
                           const SearchSel *sel = nullptr);
 void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t bucket, hrnn_nnd_iter_stats *st);
+// Look up, with the handle's device current, this device's copy of the
+// no-allocation search's scratch array (once per device; called by
+// hrnn_dev_open) and clear it.
+void noalloc_scratch_init(hrnn_dev_t h);
 // Per-query beam statistics (raw words): 0 d(q, entry), 1 d(q, layer-0
 // start), 2..17 the 16 closest beam distances, 18 expansions, 19 list size.
 constexpr int kRawWords = 20;

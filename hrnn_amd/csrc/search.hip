@@ -30,6 +30,7 @@
 // integers, the order is immaterial, and each lane reads 16 contiguous bytes.
 #include <algorithm>
 #include <cfloat>
+#include <mutex>
 #include <type_traits>
 
 #include "handle.h"
@@ -105,6 +106,13 @@ __global__ void k_rk_column(const float *__restrict__ knn_d, const uint32_t *__r
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     rk[i] = (k - 1u < knn_cnt[i]) ? knn_d[i * K + k - 1u] : FLT_MAX;
+}
+
+// dst[0 .. count) = value (clears words without a memset node, so a caller
+// may capture the launch in a graph)
+__global__ void k_fill_u32(uint32_t *__restrict__ dst, uint32_t count, uint32_t value) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+        dst[i] = value;
 }
 
 template <class EntryT, bool kCount, class VT>
@@ -380,20 +388,47 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     slots = std::max<uint32_t>(slots, 1);
     DevBuf &bmark = so.buffer_set ? h->search_mark2 : h->search_mark;
     DevBuf &bcand = so.buffer_set ? h->search_cand2 : h->search_cand;
-    // bound the fallback's private state to ~4 GB
     const uint64_t per_slot = words * 4 + cand_cap * 8;
-    const uint64_t max_slots = std::max<uint64_t>(1, (4ull << 30) / per_slot);
-    slots = (uint32_t)std::min<uint64_t>(slots, max_slots);
-    // private arrays (kept all-zero between launches by the kernel itself) and
-    // candidate buffers are owned by the search alone
-    const uint64_t mark_bytes = (uint64_t)slots * words * 4;
-    if (bmark.bytes < mark_bytes) {
-        bmark.reserve(mark_bytes);
-        HIPC(hipMemsetAsync(bmark.p, 0, bmark.bytes, st));
+    uint32_t *queue = nullptr, *mark = nullptr, *cand = nullptr;
+    if (so.scratch) {
+        // caller-free form: [queue word | pad to 256 B | slots x bitmap | slots x
+        // candidate pairs]; the bitmaps are all-zero between launches (the
+        // kernel clears what it sets), the queue is cleared by a kernel
+        REQUIRE(so.scratch_bytes > 256 + per_slot, HRNN_E_ARG,
+                "search (no allocation): one workgroup's fallback state (" + std::to_string(per_slot) +
+                    " B at n = " + std::to_string(n) + ", m = " + std::to_string(m) +
+                    ") exceeds the scratch array of " + std::to_string(so.scratch_bytes) + " B");
+        REQUIRE(so.scratch_clean, HRNN_E_ARG, "search (no allocation): no clean-extent record");
+        slots = (uint32_t)std::min<uint64_t>(slots, (so.scratch_bytes - 256) / per_slot);
+        queue = reinterpret_cast<uint32_t *>(so.scratch);
+        mark = reinterpret_cast<uint32_t *>(so.scratch + 256);
+        cand = mark + (uint64_t)slots * words;
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, queue, 1u, 0u);
+        const uint64_t mark_bytes = (uint64_t)slots * words * 4;
+        if (mark_bytes > *so.scratch_clean) {
+            const uint64_t w0 = *so.scratch_clean / 4, nw = (mark_bytes - *so.scratch_clean) / 4;
+            hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(ceil_div(nw, 256), 2048)), dim3(256), 0, st,
+                               mark + w0, (uint32_t)nw, 0u);
+        }
+        HIPC(hipGetLastError());
+        *so.scratch_clean = mark_bytes;
+    } else {
+        // bound the fallback's private state to ~4 GB
+        const uint64_t max_slots = std::max<uint64_t>(1, (4ull << 30) / per_slot);
+        slots = (uint32_t)std::min<uint64_t>(slots, max_slots);
+        // private arrays (kept all-zero between launches by the kernel itself) and
+        // candidate buffers are owned by the search alone
+        const uint64_t mark_bytes = (uint64_t)slots * words * 4;
+        if (bmark.bytes < mark_bytes) {
+            bmark.reserve(mark_bytes);
+            HIPC(hipMemsetAsync(bmark.p, 0, bmark.bytes, st));
+        }
+        bcand.reserve((uint64_t)slots * cand_cap * 8);
+        queue = h->counters.as<uint32_t>() + so.queue_word;
+        if (!so.queue_zeroed) HIPC(hipMemsetAsync(queue, 0, 4, st));
+        mark = bmark.as<uint32_t>();
+        cand = bcand.as<uint32_t>();
     }
-    bcand.reserve((uint64_t)slots * cand_cap * 8);
-    uint32_t *queue = h->counters.as<uint32_t>() + so.queue_word;
-    if (!so.queue_zeroed) HIPC(hipMemsetAsync(queue, 0, 4, st));
     // LDS table: the largest power-of-two slot count within the configured
     // bytes (u32 slots in presence mode, u64 in count mode), at least 256
     const uint32_t slot_bytes = count_mode ? 8u : 4u;
@@ -409,8 +444,11 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     // one k for the whole batch: verify against the r_k column (built once per
     // graph version and k)
     const float *rk = nullptr;
-    if (!k_per_q && k >= 1 && k <= h->K) {
-        if (h->rk_col_k != k || h->rk_col_version != h->knn_version || h->rk_col_n != n) {
+    const bool rk_ready = h->rk_col_k == k && h->rk_col_version == h->knn_version && h->rk_col_n == n;
+    if (so.scratch) {
+        if (!k_per_q && rk_ready) rk = h->rk_col.as<float>();   // else read the list matrix
+    } else if (!k_per_q && k >= 1 && k <= h->K) {
+        if (!rk_ready) {
             h->rk_col.reserve(n * 4);
             hipLaunchKernelGGL(k_rk_column, dim3(ceil_div(n, 256)), dim3(256), 0, st, kd, kc, n, h->K, k,
                                h->rk_col.as<float>());
@@ -421,8 +459,6 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
         }
         rk = h->rk_col.as<float>();
     }
-    uint32_t *mark = bmark.as<uint32_t>();
-    uint32_t *cand = bcand.as<uint32_t>();
     // Candidate vectors are read from the u8 mirror (HRNN_PREC_U8) or the BF16
     // mirror when every base value is exactly representable in it (same
     // distances, a quarter or half of the bytes).
@@ -447,6 +483,42 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
 #undef LAUNCH_VT
 #undef LAUNCH
     HIPC(hipGetLastError());
+}
+
+// ---- search without allocation ----
+//
+// hrnn_dev_rknn_search_noalloc_dptr keeps the search's queue word and its
+// per-workgroup fallback state (visited bitmaps or u16 counts, candidate
+// buffers) in this array, of which every GPU the library runs on holds one
+// copy for the life of the process.  Host code never uses the array's name
+// as an address: hrnn_dev_open looks up this device's copy once
+// (hipGetSymbolAddress) and every use goes through that pointer.
+constexpr size_t kNoAllocScratchBytes = 128ull << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+static std::mutex g_scratch_mu;
+static uint8_t *g_scratch_ptr[kMaxDevices] = {};
+static hipError_t g_scratch_err[kMaxDevices] = {};
+static size_t g_scratch_clean[kMaxDevices] = {};   // zero bytes at the start of the bitmap area
+
+void noalloc_scratch_init(hrnn_dev_t h) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    const int dv = h->device;
+    if (dv < 0 || dv >= kMaxDevices || g_scratch_ptr[dv]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        g_scratch_err[dv] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+        (void)hipGetLastError();
+        return;
+    }
+    // the bitmaps must start all-zero; the kernel keeps them so afterwards
+    hipLaunchKernelGGL(k_fill_u32, dim3(2048), dim3(256), 0, h->stream, static_cast<uint32_t *>(p),
+                       (uint32_t)(kNoAllocScratchBytes / 4), 0u);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(h->stream));
+    g_scratch_ptr[dv] = static_cast<uint8_t *>(p);
+    g_scratch_clean[dv] = kNoAllocScratchBytes - 256;
 }
 
 // ---- result compaction (the download side of searchRknn's return value) ----
@@ -531,6 +603,33 @@ __global__ __launch_bounds__(256) void k_results_gather(const uint32_t *__restri
 using namespace hrnn;
 
 extern "C" {
+
+int hrnn_dev_rknn_search_noalloc_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq,
+                                      const uint32_t *proxies_dev, uint32_t m,
+                                      const uint32_t *k_per_q_dev, uint32_t k, uint32_t K_prime,
+                                      float threshold, uint32_t *res_cnt_dev, uint32_t *res_ids_dev,
+                                      uint32_t cap_per_q, uint32_t *work_dev) {
+    if (!h) return HRNN_E_ARG;
+    return guarded(h, [&] {
+        REQUIRE(h->csr_valid, HRNN_E_STATE, "rknn_search_noalloc: build the CSR first");
+        REQUIRE(h->K > 0, HRNN_E_STATE, "rknn_search_noalloc: no KNN graph");
+        REQUIRE(k_per_q_dev || (k >= 1 && k <= h->K), HRNN_E_ARG, "rknn_search_noalloc: k out of [1, K]");
+        REQUIRE(m >= 1 && m <= 1024, HRNN_E_ARG, "rknn_search_noalloc: m out of [1, 1024]");
+        const int dv = h->device;
+        REQUIRE(dv >= 0 && dv < kMaxDevices, HRNN_E_ARG, "rknn_search_noalloc: device ordinal out of range");
+        std::lock_guard<std::mutex> g(g_scratch_mu);   // held while the launches are queued
+        uint8_t *p = g_scratch_ptr[dv];
+        REQUIRE(p, HRNN_E_HIP, std::string("rknn_search_noalloc: scratch array not available on this device (") +
+                                   hipGetErrorString(g_scratch_err[dv]) + ")");
+        if (nq == 0) return;
+        SearchSel sel;
+        sel.scratch = p;
+        sel.scratch_bytes = kNoAllocScratchBytes;
+        sel.scratch_clean = &g_scratch_clean[dv];
+        rknn_search_resident(h, q_dev, nq, proxies_dev, m, k_per_q_dev, k, K_prime, threshold, res_cnt_dev,
+                             res_ids_dev, cap_per_q, work_dev, &sel);
+    });
+}
 
 int hrnn_dev_fetch_results(hrnn_dev_t h, uint32_t nq, const uint32_t *res_cnt_dev,
                            const uint32_t *res_ids_dev, uint32_t cap_per_q, uint64_t *res_off,

@@ -34,7 +34,7 @@ EXPORTS = [
     "hrnn_dev_set_knng", "hrnn_dev_get_knng", "hrnn_dev_get_knng_flags", "hrnn_dev_set_knng_flags",
     "hrnn_dev_knng_ptrs", "hrnn_dev_nndescent",
     "hrnn_dev_build_rknn_csr", "hrnn_dev_get_rknn_csr", "hrnn_dev_set_rknn_csr",
-    "hrnn_dev_rknn_search", "hrnn_dev_rknn_search_dptr", "hrnn_dev_fetch_results", "hrnn_dev_set_hnsw", "hrnn_dev_proxies",
+    "hrnn_dev_rknn_search", "hrnn_dev_rknn_search_dptr", "hrnn_dev_rknn_search_noalloc_dptr", "hrnn_dev_fetch_results", "hrnn_dev_set_hnsw", "hrnn_dev_proxies",
     "hrnn_dev_proxies_dptr", "hrnn_dev_rknn_query_dptr", "hrnn_dev_bf_kth", "hrnn_dev_bf_knn",
     "hrnn_dev_bf_rknn", "hrnn_dev_l2sqr_pairs", "hrnn_dev_mlp_forward",
     "hrnn_dev_mlp_grad", "hrnn_dev_mlp_train", "hrnn_dev_rknn_search_learned_dptr",
@@ -94,6 +94,7 @@ def lib() -> C.CDLL:
         "hrnn_dev_rknn_search": [vp, vp, u32, vp, u32, vp, u32, u32, f, vp, vp, u64, P(u64)],
         "hrnn_dev_rknn_search_dptr": [vp, vp, u32, vp, u32, vp, u32, u32, f, vp, vp, u32, vp],
         "hrnn_dev_fetch_results": [vp, u32, vp, vp, u32, vp, vp, u64, P(u64)],
+        "hrnn_dev_rknn_search_noalloc_dptr": [vp, vp, u32, vp, u32, vp, u32, u32, f, vp, vp, u32, vp],
         "hrnn_dev_set_hnsw": [vp, u32, u32, vp, vp, vp, vp, u64, u32, i],
         "hrnn_dev_proxies": [vp, vp, u32, u32, u32, vp],
         "hrnn_dev_proxies_dptr": [vp, vp, u32, u32, u32, vp, vp],

@@ -178,6 +178,28 @@ int hrnn_dev_rknn_search_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq,
                               float threshold, uint32_t *res_cnt_dev,
                               uint32_t *res_ids_dev, uint32_t cap_per_q, uint32_t *work_dev);
 
+/* The resident search without allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Same arguments,
+ * results and error behaviour as hrnn_dev_rknn_search_dptr; the search's
+ * queue word and its per-workgroup fallback state (visited bitmaps / u16
+ * counts, candidate buffers) live in a static device array of 128 MiB that
+ * every GPU the library runs on holds for the life of the process, so the
+ * call allocates nothing and clears its words with kernels, not memsets.
+ * The persistent workgroups are capped so that their fallback state fits the
+ * array (n/8 + 8 * min(m * longest reverse list, n) bytes each, n/2 in place
+ * of n/8 when threshold > 1); a problem where one workgroup's state does not
+ * fit is rejected with HRNN_E_ARG.  The r_k column is used only when an
+ * earlier search built it for this k (else the list matrix is read; same
+ * results).  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (any handles, any streams) race and must be serialised
+ * by the caller. */
+int hrnn_dev_rknn_search_noalloc_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq,
+                                      const uint32_t *proxies_dev, uint32_t m,
+                                      const uint32_t *k_per_q_dev, uint32_t k, uint32_t K_prime,
+                                      float threshold, uint32_t *res_cnt_dev, uint32_t *res_ids_dev,
+                                      uint32_t cap_per_q, uint32_t *work_dev);
+
 /* Download of a resident search's results, densely packed: on the handle's
  * stream the counts are scanned (each capped at cap_per_q) into offsets and
  * the ids gathered into one array on the device, then res_off[nq+1] (u64)
