@@ -12,7 +12,8 @@ using namespace hrnn;
 hrnn_dev::~hrnn_dev() {
     (void)hipSetDevice(device);
     DevBuf *bufs[] = {&x, &xb, &xh, &xu8, &norms, &knn_ids, &knn_d, &knn_cnt, &knn_new, &rk_col, &csr_off,
-                      &csr_ent, &search_mark, &search_cand, &search_mark2, &search_cand2, &fetch_off, &fetch_ids, &l0, &up_off, &up_pool, &beam_vis,
+                      &csr_ent, &search_mark, &search_cand, &search_mark2, &search_cand2, &fetch_off, &fetch_ids, &l0, &up_off, &up_pool, &beam_vis, &levels_dev, &hb_prop, &hb_inc_cnt, &hb_inc,
+                      &hb_touched, &hb_ctr,
                       &s0, &s1, &s2, &s3, &s4, &s5, &s6, &s7, &s8, &nnd_aux, &counters};
     for (DevBuf *b : bufs) b->release();
     if (pin_small) (void)hipHostFree(pin_small);
@@ -353,6 +354,7 @@ static int *tuning_slot(hrnn_dev_t h, const char *key) {
     if (k == "beam_wg_per_cu") return &h->tune_beam_wg_per_cu;
     if (k == "search_lds_kib") return &h->tune_search_lds_kib;
     if (k == "stage1_cap") return &h->tune_stage1_cap;
+    if (k == "hnsw_batch_permille") return &h->tune_hnsw_batch_permille;
     return nullptr;
 }
 
@@ -360,7 +362,8 @@ int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value) {
     if (!h) return HRNN_E_ARG;
     int *slot = tuning_slot(h, key);
     const bool cap = slot == &h->tune_stage1_cap;
-    if (!slot || value < (cap ? 0 : 1) || value > (cap ? 1024 : 64)) {
+    const bool permille = slot == &h->tune_hnsw_batch_permille;
+    if (!slot || value < (cap ? 0 : 1) || value > (cap ? 1024 : permille ? 1000 : 64)) {
         h->err = std::string("set_tuning: unknown key or value out of range: ") + (key ? key : "");
         return HRNN_E_ARG;
     }
@@ -451,8 +454,12 @@ int hrnn_dev_set_knng_flags(hrnn_dev_t h, const uint8_t *flags) {
     if (!h) return HRNN_E_ARG;
     return guarded(h, [&] {
         REQUIRE(h->K > 0, HRNN_E_STATE, "set_knng_flags: no graph");
-        REQUIRE(flags, HRNN_E_ARG, "set_knng_flags: null argument");
-        h2d(h, h->knn_new.p, flags, h->n * h->K);
+        if (flags) {
+            h2d(h, h->knn_new.p, flags, h->n * h->K);
+        } else {
+            HIPC(hipMemsetAsync(h->knn_new.p, 1, h->n * h->K, h->stream));
+            HIPC(hipStreamSynchronize(h->stream));
+        }
     });
 }
 

@@ -37,6 +37,7 @@ struct hrnn_dev {
     int tune_search_lds_kib = 16;    // LDS dedup table of the RkNN search, KiB
     int tune_beam_wg_per_cu = 16;    // persistent beam workgroups (one wave each) per CU
     int tune_stage1_cap = 0;         // learned early stop: stage-1 expansion cap (0: none)
+    int tune_hnsw_batch_permille = 20;   // HNSW construction: batch size / graph size, per mille
 
     // ranked KNN graph (SoA); knn_version changes whenever it may have changed
     uint32_t K = 0;
@@ -73,6 +74,9 @@ struct hrnn_dev {
     uint64_t up_pool_words = 0;
     int maxlevel = 0;
     hrnn::DevBuf l0, up_off, up_pool, beam_vis;
+    // HNSW construction: node levels, and per batch the proposed links back,
+    // each node's incoming links, the touched nodes and counters
+    hrnn::DevBuf levels_dev, hb_prop, hb_inc_cnt, hb_inc, hb_touched, hb_ctr;
 
     // scratch shared by the build kernels
     hrnn::DevBuf s0, s1, s2, s3, s4, s5, s6, s7, s8;

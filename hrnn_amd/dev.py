@@ -32,7 +32,7 @@ EXPORTS = [
     "hrnn_dev_append_base", "hrnn_dev_base_is_small_int", "hrnn_dev_base_ptr",
     "hrnn_dev_set_vector_precision", "hrnn_dev_set_tuning", "hrnn_dev_get_tuning",
     "hrnn_dev_set_knng", "hrnn_dev_get_knng", "hrnn_dev_get_knng_flags", "hrnn_dev_set_knng_flags",
-    "hrnn_dev_knng_ptrs", "hrnn_dev_nndescent",
+    "hrnn_dev_knng_ptrs", "hrnn_dev_hnsw_build", "hrnn_dev_get_hnsw", "hrnn_dev_nndescent",
     "hrnn_dev_build_rknn_csr", "hrnn_dev_get_rknn_csr", "hrnn_dev_set_rknn_csr",
     "hrnn_dev_rknn_search", "hrnn_dev_rknn_search_dptr", "hrnn_dev_rknn_search_noalloc_dptr", "hrnn_dev_fetch_results", "hrnn_dev_set_hnsw", "hrnn_dev_proxies",
     "hrnn_dev_proxies_dptr", "hrnn_dev_rknn_query_dptr", "hrnn_dev_bf_kth", "hrnn_dev_bf_knn",
@@ -48,6 +48,11 @@ class HrnnError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"{STATUS.get(code, code)}: {msg}")
         self.code = code
+
+
+class HnswBuildStats(C.Structure):
+    _fields_ = [("batches", C.c_uint64), ("expansions", C.c_uint64), ("evaluations", C.c_uint64),
+                ("dropped_links", C.c_uint64), ("ms", C.c_double)]
 
 
 class NndIterStats(C.Structure):
@@ -86,6 +91,8 @@ def lib() -> C.CDLL:
         "hrnn_dev_get_knng": [vp, vp, vp, vp],
         "hrnn_dev_knng_ptrs": [vp, vp, vp, vp],
         "hrnn_dev_get_knng_flags": [vp, vp],
+        "hrnn_dev_hnsw_build": [vp, vp, u32, u32, u32, P(u32), P(i), vp],
+        "hrnn_dev_get_hnsw": [vp, vp, vp, u64],
         "hrnn_dev_set_knng_flags": [vp, vp],
         "hrnn_dev_nndescent": [vp, i, i, u32, vp],
         "hrnn_dev_build_rknn_csr": [vp, P(u32), P(i), P(u64)],

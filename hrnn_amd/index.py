@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .dev import HrnnError, NndIterStats, lib as dev_lib
+from .dev import HnswBuildStats, HrnnError, NndIterStats, lib as dev_lib
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 HOST_LIB_PATH = os.path.join(PKG, "lib", "libhrnn_host.so")
@@ -34,6 +34,7 @@ def host_lib() -> C.CDLL:
         "hrnn_index_free": [vp],
         "hrnn_index_info": [vp, P(u32), P(u32), P(u32), P(i)],
         "hrnn_index_add_points": [vp, vp, u64, vp, i],
+        "hrnn_index_add_points_gpu": [vp, vp, u64, vp, i, vp],
         "hrnn_index_refine": [vp, i, i, vp],
         "hrnn_index_build_rknng": [vp],
         "hrnn_index_build_rknng_mutable": [vp],
@@ -122,6 +123,16 @@ class Index:
         x = np.ascontiguousarray(x, np.float32)
         lab = None if labels is None else np.ascontiguousarray(labels, np.uint64)
         self._ck(self.L.hrnn_index_add_points(self.h, _p(x), x.shape[0], _p(lab), nthreads))
+
+    def add_points_gpu(self, x, labels=None, batch_permille=0):
+        """HNSW construction with KNN seeding on the GPU (empty index only);
+        returns the build's counters."""
+        x = np.ascontiguousarray(x, np.float32)
+        lab = None if labels is None else np.ascontiguousarray(labels, np.uint64)
+        st = HnswBuildStats()
+        self._ck(self.L.hrnn_index_add_points_gpu(self.h, _p(x), x.shape[0], _p(lab), batch_permille,
+                                                  C.byref(st)))
+        return {k: getattr(st, k) for k, _ in HnswBuildStats._fields_}
 
     def refine(self, iters: int, sample: int):
         st = (NndIterStats * max(iters, 1))()

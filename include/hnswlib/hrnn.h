@@ -157,6 +157,70 @@ class HRNN : public AlgorithmInterface<dist_t> {
         return out;
     }
 
+    // ---- HNSW construction with KNN seeding on the GPU ------------------------
+    // The batch form of addPoint (reference hrnn.h:1378-1517) into an empty
+    // index: rows X[0 .. nx) get internal ids 0 .. nx-1 and labels labels[i]
+    // (i when labels is null); levels are drawn in id order with this index's
+    // generator, as nx sequential addPoint calls would draw them; the graph
+    // and the KNN seed lists are built by hrnn_dev_hnsw_build and copied back,
+    // and stay resident on the device.
+    void addPointsDevice(const float *X, size_t nx, const labeltype *labels = nullptr) {
+        if (cur_element_count != 0)
+            throw std::runtime_error("addPointsDevice: the index must be empty (use addPoint to extend it)");
+        if (nx > max_elements_) throw std::runtime_error("The number of elements exceeds the specified limit");
+        if (nx == 0) return;
+        if (maxM0_ > 64 || M_ < 2) throw std::runtime_error("addPointsDevice: M must be in [2, 32]");
+        {
+            std::lock_guard<std::mutex> g(label_mu_);
+            for (size_t i = 0; i < nx; i++) {
+                const labeltype lab = labels ? labels[i] : (labeltype)i;
+                if (!label_map_.emplace(lab, (tableint)i).second) {
+                    label_map_.clear();
+                    throw std::runtime_error("addPoint: label already present (updates are not supported)");
+                }
+                labels_[i] = lab;
+            }
+        }
+        std::memcpy(vecs_.data(), X, nx * data_size_);
+        for (size_t i = 0; i < nx; i++) levels_[i] = random_level();
+        cur_element_count = nx;
+        std::lock_guard<std::mutex> g(dev_mu_);
+        sync_device_base();
+        if (build_batch_permille_ > 0)
+            dev_check(hrnn_dev_set_tuning(dev_, "hnsw_batch_permille", build_batch_permille_));
+        uint32_t entry = 0;
+        int maxl = 0;
+        hrnn_hnsw_build_stats st{};
+        dev_check(hrnn_dev_hnsw_build(dev_, levels_.data(), (uint32_t)M_, (uint32_t)ef_construction_,
+                                      (uint32_t)K_knng_, &entry, &maxl, &st));
+        last_build_stats_ = st;
+        // copy the graph back: layer 0 as is, upper blocks in id order
+        size_t pool = 0;
+        for (size_t i = 0; i < nx; i++) pool += (size_t)levels_[i] * (maxM_ + 1);
+        std::vector<uint32_t> up(std::max<size_t>(pool, 1));
+        dev_check(hrnn_dev_get_hnsw(dev_, level0_.data(), up.data(), pool));
+        size_t o = 0;
+        for (size_t i = 0; i < nx; i++) {
+            if (levels_[i] <= 0) continue;
+            upper_[i] = static_cast<char *>(std::malloc((size_t)levels_[i] * links_bytes_));
+            if (!upper_[i]) throw std::runtime_error("addPoint: out of memory");
+            std::memcpy(upper_[i], up.data() + o, (size_t)levels_[i] * links_bytes_);
+            o += (size_t)levels_[i] * (maxM_ + 1);
+        }
+        enterpoint_node_ = entry;
+        maxlevel_ = maxl;
+        dev_hnsw_rows_ = nx;
+        if (K_knng_ > 0) {
+            dev_check(hrnn_dev_get_knng(dev_, knn_ids_.data(), knn_d_.data(), knn_cnt_.data()));
+            dev_knng_ok_ = true;
+        }
+        dev_csr_ok_ = false;
+    }
+    const hrnn_hnsw_build_stats &lastBuildStats() const { return last_build_stats_; }
+    // Batch size of addPointsDevice as a fraction of the graph, per mille
+    // (0: the device default).
+    void setBuildBatchPermille(int permille) { build_batch_permille_ = permille; }
+
     // ---- (a) NNDescent on the GPU -----------------------------------------
     // reference: refineKNNG (hrnn.h:1725-1862)
     void refineKNNG(int num_iters = 3, int sample_size = 20) {
@@ -167,6 +231,9 @@ class HRNN : public AlgorithmInterface<dist_t> {
         std::lock_guard<std::mutex> g(dev_mu_);
         sync_device_base();
         sync_device_knng();
+        // every entry starts new, as the reference's refineKNNG starts each
+        // call (hrnn.h:1744-1748), also when the device graph is already current
+        dev_check(hrnn_dev_set_knng_flags(dev_, nullptr));
         std::vector<hrnn_nnd_iter_stats> st(std::max(num_iters, 1));
         dev_check(hrnn_dev_nndescent(dev_, num_iters, sample_size, 0, st.data()));
         dev_check(hrnn_dev_get_knng(dev_, knn_ids_.data(), knn_d_.data(), knn_cnt_.data()));
@@ -796,6 +863,8 @@ class HRNN : public AlgorithmInterface<dist_t> {
     std::vector<uint16_t> knn_cnt_;
     std::vector<float> kdist_;
     std::vector<hrnn_nnd_iter_stats> last_nnd_stats_;
+    hrnn_hnsw_build_stats last_build_stats_{};
+    int build_batch_permille_{0};
 
     // ---- device state ---------------------------------------------------------------
     mutable hrnn_dev_t dev_{nullptr};

@@ -109,11 +109,43 @@ int hrnn_dev_get_knng(hrnn_dev_t h, uint32_t *ids, float *dists, uint16_t *cnt);
 /* The entries' NNDescent "new" flags, u8 [n*K] (the reference's per-entry
  * is_new vector, hrnn.h:1744-1748, :1775-1778): read after refinement, or
  * set after hrnn_dev_set_knng to resume refinement from a saved state
- * (entries j >= cnt[i] are ignored). */
+ * (entries j >= cnt[i] are ignored); flags NULL marks every entry new, the
+ * state refineKNNG starts from. */
 int hrnn_dev_get_knng_flags(hrnn_dev_t h, uint8_t *flags);
 int hrnn_dev_set_knng_flags(hrnn_dev_t h, const uint8_t *flags);
 /* Device pointers of the resident graph (ids n*K u32, dists n*K f32, cnt n u32). */
 int hrnn_dev_knng_ptrs(hrnn_dev_t h, uint32_t **ids, float **dists, uint32_t **cnt);
+
+/* ---- HNSW construction with KNN seeding --------------------------------- */
+typedef struct hrnn_hnsw_build_stats {
+    uint64_t batches;         /* insertion batches */
+    uint64_t expansions;      /* beam expansions over all layers */
+    uint64_t evaluations;     /* distances evaluated by the beams */
+    uint64_t dropped_links;   /* links back beyond 64 per node, batch and layer (not made) */
+    double ms;                /* wall time of the call */
+} hrnn_hnsw_build_stats;
+
+/* Build the HNSW graph over all n resident base rows, inserted in id order
+ * with levels[n] (drawn by the caller: floor(-ln U / ln M)), and, when
+ * K > 0, the ranked KNN seed lists (the layer-0 beam sorted by (dist, id),
+ * self skipped, the first K; every entry new).  M links per new node per
+ * layer, lists of 2M at layer 0 and M above, beams of width ef_construction
+ * (<= 512), 2 <= M <= 32.  Points go in batches of "hnsw_batch_permille"
+ * (hrnn_dev_set_tuning, default 20) of the graph's size; each batch searches
+ * the graph as it stood when the batch began, then every node's new links
+ * back are merged into its list at once (appended, or the list pruned by the
+ * diversity rule over entries and new links).  The graph stays resident for
+ * phase A (hrnn_dev_proxies) and the KNN lists for hrnn_dev_nndescent;
+ * *entry and *maxlevel receive the entry point and top layer.  Replaces
+ * HRNN::addPoint (include/hnswlib/hrnn.h:1378-1517) with searchBaseLayer
+ * (:401-481), getNeighborsByHeuristic2 (:619-666) and
+ * mutuallyConnectNewElement (:682-806). */
+int hrnn_dev_hnsw_build(hrnn_dev_t h, const int *levels, uint32_t M, uint32_t ef_construction, uint32_t K,
+                        uint32_t *entry, int *maxlevel, hrnn_hnsw_build_stats *stats);
+/* Download the resident graph: l0[n*(2M+1)] and the upper-layer pool
+ * (pool_words = the sum over nodes of level * (M+1) words, blocks in id
+ * order: node i's at the running sum over nodes before it). */
+int hrnn_dev_get_hnsw(hrnn_dev_t h, uint32_t *l0, uint32_t *up_pool, uint64_t pool_words);
 
 /* ---- (a) NNDescent refinement ------------------------------------------ */
 typedef struct hrnn_nnd_iter_stats {

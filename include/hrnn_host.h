@@ -10,6 +10,7 @@
  *   hrnn_index_create        HRNN(space, max_elements, M, efc, seed, false, K)  include/hnswlib/hrnn.h:128
  *   hrnn_index_load          HRNN(space, location, ...) / loadIndex            include/hnswlib/hrnn.h:117, :930
  *   hrnn_index_add_points    addPoint (OpenMP loop as src/hrnn_build.cpp:75)   include/hnswlib/hrnn.h:1179
+ *   hrnn_index_add_points_gpu addPoint for a whole batch, on the GPU           include/hnswlib/hrnn.h:1378-1517
  *   hrnn_index_refine        refineKNNG                                         include/hnswlib/hrnn.h:1725
  *   hrnn_index_build_rknng   buildRKNNG                                         include/hnswlib/hrnn.h:1869
  *   hrnn_index_build_rknng_mutable  buildRKNNG_mutable                          include/hnswlib/hrnn.h:1942
@@ -45,6 +46,13 @@ int hrnn_index_info(hrnn_index_t ix, uint32_t *M, uint32_t *efc, uint32_t *K, in
 int hrnn_index_add_points(hrnn_index_t ix, const float *x, uint64_t n, const uint64_t *labels,
                           int nthreads);
 /* refineKNNG on the GPU; stats may be NULL or hold num_iters records. */
+/* HNSW construction with KNN seeding on the GPU for an empty index
+ * (HRNN::addPointsDevice): rows x[0 .. n) become internal ids 0 .. n-1 with
+ * labels[i] (i when labels is NULL); batch_permille (0: the default of 20)
+ * sizes the insertion batches relative to the graph; stats (may be NULL)
+ * receives the build's counters. */
+int hrnn_index_add_points_gpu(hrnn_index_t ix, const float *x, uint64_t n, const uint64_t *labels,
+                              int batch_permille, hrnn_hnsw_build_stats *stats);
 int hrnn_index_refine(hrnn_index_t ix, int num_iters, int sample_size, hrnn_nnd_iter_stats *stats);
 int hrnn_index_build_rknng(hrnn_index_t ix);
 int hrnn_index_build_rknng_mutable(hrnn_index_t ix);

@@ -1,0 +1,167 @@
+"""HNSW construction with KNN seeding on the GPU (hrnn_dev_hnsw_build,
+HRNN::addPointsDevice) against the host construction, which follows the
+reference's addPoint (include/hnswlib/hrnn.h:1378-1517) and matches the
+compiled reference's seeds single-threaded (test_host_index.py).
+
+Batched insertion changes which graph each point searches, so the graphs
+differ; what must hold:
+* structure: every link id < n, no self links, no repeated id in a list,
+  counts within 2M (layer 0) and M (above), upper-layer links only to nodes
+  of that level, the levels those of sequential insertion (same generator);
+* seeds: each list ascending, duplicate- and self-free, every distance the
+  exact squared L2 of its pair, at most K (and at most ef_c) entries;
+* quality, against the host build of the same rows: proxy recall of phase A
+  and the seed lists' recall against the exact KNN within half a point or
+  better, and after refinement and the reverse CSR, RkNN recall at every m
+  within half a point (the reference's small configuration: 10k x 128, K=100,
+  ef_c=200, 10 x S=50, k=10, K'=100, m in {1, 5, 10}, ef_search=200);
+* the reference loads the GPU-built index file and finds the same proxies as
+  the host beam on it."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import hrnn_amd
+import oracle
+from hrnn_amd.index import Index
+
+pytestmark = pytest.mark.gpu
+
+
+def _check_structure(ix, n, M):
+    links, levels = ix.level0()
+    assert links.shape[1] == 2 * M + 1
+    cnt = links[:, 0] & 0xFFFF
+    assert int(cnt.max()) <= 2 * M and int(cnt[1:].min()) >= 1
+    for i in range(n):
+        row = links[i, 1:1 + cnt[i]]
+        assert np.all(row < n) and i not in row and len(set(row.tolist())) == len(row), i
+    return cnt, levels
+
+
+def _exact_seeds(x, ix, K, efc):
+    ids, d, cnt = ix.get_knng()
+    assert int(cnt.max()) <= min(K, efc)
+    for i in range(0, x.shape[0], 7):
+        c = int(cnt[i])
+        row = ids[i, :c]
+        assert i not in row and len(set(row.tolist())) == c, i
+        assert np.all(np.diff(d[i, :c]) >= 0), i
+        dd = ((x[row].astype(np.int64) - x[i].astype(np.int64)) ** 2).sum(1)
+        assert np.array_equal(dd.astype(np.float32), d[i, :c]), i
+    return ids, d, cnt
+
+
+def test_gpu_build_structure_and_seeds_small():
+    x = hrnn_amd.synth_mixture(11, 3000, 128, 50, 20.0, 0.0, 255.0, True, 0.0, 1011)
+    n, M, efc, K = 3000, 16, 100, 48
+    ix = Index(n, 128, M, efc, 100, K)
+    st = ix.add_points_gpu(x)
+    print("build stats:", st, flush=True)
+    assert st["batches"] > 10 and st["dropped_links"] == 0
+    assert ix.count == n and np.array_equal(ix.labels(), np.arange(n))
+    _check_structure(ix, n, M)
+    # the same levels as sequential insertion with the same seed
+    host = Index(n, 128, M, efc, 100, K)
+    host.add_points(x, nthreads=1)
+    assert np.array_equal(ix.level0()[1], host.level0()[1])
+    _exact_seeds(x, ix, K, efc)
+    # upper layers: links only to nodes of that level or above
+    off, pool, lv, entry, maxlevel = _upper(ix)
+    assert lv[entry] == maxlevel == int(lv.max())
+    for i in np.nonzero(lv > 0)[0]:
+        for L in range(1, lv[i] + 1):
+            blk = pool[off[i] + (L - 1) * (M + 1): off[i] + L * (M + 1)]
+            c = int(blk[0] & 0xFFFF)
+            assert c <= M
+            tgt = blk[1:1 + c]
+            assert np.all(lv[tgt] >= L) and i not in tgt, (i, L)
+
+
+def _upper(ix):
+    g = ix.export_hnsw()
+    return g["up_off"], g["up_pool"], g["levels"], g["entry"], g["maxlevel"]
+
+
+@pytest.mark.parametrize("n,centres,sigma,K,efc,ms,Kp", [(10_000, 50, 20.0, 100, 200, (1, 5, 10), 100)])
+def test_gpu_build_quality_vs_host_build(n, centres, sigma, K, efc, ms, Kp):
+    x = hrnn_amd.synth_mixture(1, n, 128, centres, sigma, 0.0, 255.0, True, 0.0, 1001)
+    q = hrnn_amd.synth_mixture(2, 300, 128, centres, sigma, 0.0, 255.0, True, 0.0, 1001)
+    gpu = Index(n, 128, 16, efc, 100, K)
+    st = gpu.add_points_gpu(x)
+    host = Index(n, 128, 16, efc, 100, K)
+    host.add_points(x, nthreads=8)
+    print("build stats:", st, flush=True)
+    exact, _ = oracle.knn(x, x, K + 1)
+    exact = exact[:, 1:].copy()
+    # seed recall (labels = rows for both)
+    g_ids, _, g_cnt = gpu.get_knng()
+    h_ids, _, h_cnt = host.get_knng()
+    hl = host.labels()
+    h_lab = np.zeros_like(h_ids)
+    h_lab[hl] = hl[np.where(np.arange(K) < h_cnt[:, None], h_ids, 0)]
+    h_c = np.zeros_like(h_cnt)
+    h_c[hl] = h_cnt
+    seed = {}
+    for kk in (10, K):
+        seed[kk] = (oracle.knng_recall(g_ids, g_cnt, K, exact, kk), oracle.knng_recall(h_lab, h_c, K, exact, kk))
+        print(f"seed recall@{kk}: GPU {seed[kk][0]:.5f} host {seed[kk][1]:.5f}", flush=True)
+    # phase A proxies against the exact kNN of the queries
+    qe, _ = oracle.knn(x, q, 10)
+    for ef in (16, 64, 200):
+        pg = gpu.proxies(q, 10, ef, 8)
+        ph = host.labels()[host.proxies(q, 10, ef, 8)]
+        rg = np.mean([len(set(pg[i].tolist()) & set(qe[i].tolist())) / 10 for i in range(len(q))])
+        rh = np.mean([len(set(ph[i].tolist()) & set(qe[i].tolist())) / 10 for i in range(len(q))])
+        print(f"proxy recall@10 ef={ef}: GPU {rg:.5f} host {rh:.5f}", flush=True)
+        assert rg >= rh - 0.005, (ef, rg, rh)
+    # end to end
+    kth = oracle.kth(x, 10)
+    gt_off, gt_ids, _ = oracle.rknn_gt(x, q, kth)
+    for ix in (gpu, host):
+        ix.refine(10, 50)
+        ix.build_rknng()
+    # the seeds see only the points before their batch: a little below the
+    # host's before refinement; after it the KNN graphs agree
+    for kk in (10, K):
+        assert seed[kk][0] >= seed[kk][1] - 0.02, (kk, seed[kk])
+        g_ids, _, g_cnt = gpu.get_knng()
+        h_ids, _, h_cnt = host.get_knng()
+        h_lab = np.zeros_like(h_ids)
+        h_lab[hl] = hl[np.where(np.arange(K) < h_cnt[:, None], h_ids, 0)]
+        h_c = np.zeros_like(h_cnt)
+        h_c[hl] = h_cnt
+        rg = oracle.knng_recall(g_ids, g_cnt, K, exact, kk)
+        rh = oracle.knng_recall(h_lab, h_c, K, exact, kk)
+        print(f"refined KNN recall@{kk}: GPU-built {rg:.5f} host-built {rh:.5f}", flush=True)
+        assert abs(rg - rh) <= 0.005, (kk, rg, rh)
+    for m in ms:
+        og, lg = gpu.search(q, m, 1.0, 10, Kp, 200, per_query=True)
+        oh, lh = host.search(q, m, 1.0, 10, Kp, 200, per_query=True)
+        rg = oracle.rknn_recall(gt_off, gt_ids, og, lg.astype(np.uint32), n)[0]
+        rh = oracle.rknn_recall(gt_off, gt_ids, oh, lh.astype(np.uint32), n)[0]
+        print(f"RkNN recall m={m}: GPU-built {rg:.5f} host-built {rh:.5f}", flush=True)
+        assert rg >= rh - 0.005, (m, rg, rh)
+
+
+def test_gpu_built_index_file_in_reference():
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built")
+    x = hrnn_amd.synth_mixture(31, 4000, 128, 40, 20.0, 0.0, 255.0, True, 0.0, 1031)
+    q = hrnn_amd.synth_mixture(32, 50, 128, 40, 20.0, 0.0, 255.0, True, 0.0, 1031)
+    ix = Index(4000, 128, 16, 100, 100, 32)
+    ix.add_points_gpu(x)
+    with tempfile.TemporaryDirectory() as tmp:
+        p = os.path.join(tmp, "g.idx")
+        ix.save(p)
+        r = oracle.RefIndex.load(p, 128)
+    r.K = 32
+    rid, rd, rc = r.get_knng()
+    ids, d, cnt = ix.get_knng()
+    m = np.arange(32) < cnt[:, None]
+    assert np.array_equal(rc, cnt) and np.array_equal(np.where(m, rid, 0), np.where(m, ids, 0))
+    rp, op = r.proxies(q, 5, 32), ix.proxies(q, 5, 32, 1)
+    same = np.mean([len(set(rp[i]) & set(op[i])) / 5 for i in range(len(q))])
+    assert same >= 0.97, same
