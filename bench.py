@@ -7,7 +7,7 @@ integer values 0..255 from a 1,000-centre Gaussian mixture (sigma 25, clipped,
 fixed seeds); M=16, ef_construction=400, K_knng=500, nn_iters=10, nn_sample=50;
 k=10, K'=500, m in {1,3,5,10,20,50}, ef_search=200.
 
-Before timing: host HNSW build with KNN seeding; GPU refineKNNG (timed: the
+Before timing: GPU HNSW construction with KNN seeding; GPU refineKNNG (timed: the
 NNDescent build metric); GPU reverse CSR; exact groundtruth from the GPU
 brute force; an m sweep picking the cheapest m with macro recall >= 0.95; a
 learned-m model trained on separate queries.  A timed step is one whole
@@ -101,6 +101,11 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--recall-target", type=float, default=0.95)
+    ap.add_argument("--nnd-ref", default="auto", choices=["auto", "full", "sample"],
+                    help="NNDescent CPU baseline: the reference's refineKNNG on the whole seed graph "
+                         "(full), on a 50k-point sample (sample), or full when the sample projects "
+                         "it under --nnd-ref-budget seconds (auto)")
+    ap.add_argument("--nnd-ref-budget", type=float, default=420.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
@@ -132,25 +137,40 @@ def main() -> int:
     q = gen(cfg["query_seed"], nq)
     log(f"data {n}x{d} + {nq} queries in {time.time() - t:.1f}s")
 
-    # ---- index: host HNSW + KNN seeding -------------------------------------
+    # ---- index: HNSW construction + KNN seeding on the GPU -------------------
     t = time.time()
     ix = Index(n, d, cfg["M"], cfg["efc"], 100, K)
-    chunk = max(1, n // 10)
-    for s in range(0, n, chunk):
-        ix.add_points(x[s:s + chunk], labels=np.arange(s, min(n, s + chunk), dtype=np.uint64),
-                      nthreads=threads)
-        log(f"  HNSW addPoint {min(n, s + chunk)}/{n}  [{time.time() - t:.0f}s]")
+    hnsw_st = ix.add_points_gpu(x)
     hnsw_s = time.time() - t
+    log(f"HNSW + KNN seeds on the GPU: {hnsw_s:.2f}s ({hnsw_st['batches']} batches, "
+        f"{hnsw_st['expansions'] / n:.0f} expansions and {hnsw_st['evaluations'] / n:.0f} evaluations per point)")
     # Device rows, groundtruth and results below are all in internal-id
     # (insertion order) space; labels[i] is the row of x behind internal id i.
     labels = ix.labels()
+    # the seed graph, saved for the reference's own refineKNNG on the same
+    # seeds (the NNDescent CPU baseline at the end of the run)
+    seed_dir = seed_path = None
+    if not args.no_cpu_baseline and args.nnd_ref != "sample":
+        try:
+            import oracle
+            if oracle.ref_available():
+                shm = "/dev/shm" if os.path.isdir("/dev/shm") else None
+                seed_dir = tempfile.mkdtemp(prefix="hrnn_seed_", dir=shm)
+                seed_path = os.path.join(seed_dir, "seed.idx")
+                t1 = time.time()
+                ix.save(seed_path)
+                log(f"seed graph saved for the reference baseline in {time.time() - t1:.1f}s")
+        except Exception as e:
+            log(f"seed graph not saved: {e}")
+            seed_path = None
 
     # ---- (a) NNDescent on the GPU (the build metric) ------------------------
     t = time.time()
     st = ix.refine(cfg["iters"], cfg["S"])
     nnd_s = time.time() - t
     log(f"refineKNNG({cfg['iters']},{cfg['S']}) on GPU: {nnd_s:.2f}s "
-        f"(kernel iterations {sum(s['ms'] for s in st) / 1e3:.2f}s) "
+        f"(device time of the iterations {sum(s['ms'] for s in st) / 1e3:.2f}s: "
+        f"{', '.join('%.1f' % s_['ms'] for s_ in st)} ms) "
         f"dists={sum(s['dists'] for s in st)} dropped={sum(s['dropped'] for s in st)}")
 
     # ---- (b) reverse CSR ------------------------------------------------------------
@@ -671,12 +691,46 @@ def main() -> int:
         except Exception as e:
             log(f"brute-force baseline skipped: {e}")
 
+    # ---- HNSW construction baseline: the reference's addPoint on the host cores --------------
+    # Bounded sample: the first 100,000 points inserted by the reference
+    # (OpenMP over points, as src/hrnn_build.cpp:75) and by the GPU build.
+    hnsw_cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            import oracle
+            if oracle.ref_available():
+                ns = min(n, 100_000)
+                r = oracle.RefIndex(ns, d, M=cfg["M"], efc=cfg["efc"], K=K)
+                t1 = time.perf_counter()
+                r.add_points(x[:ns], nthreads=threads)
+                ref_s = time.perf_counter() - t1
+                r.close()
+                ixs = Index(ns, d, cfg["M"], cfg["efc"], 100, K)
+                t1 = time.perf_counter()
+                ixs.add_points_gpu(x[:ns])
+                gpu_ns_s = time.perf_counter() - t1
+                del ixs
+                hnsw_cpu = {"value": round(ref_s, 3), "unit": f"s for {ns} points", "cores": threads,
+                            "kind": "reference",
+                            "sample": f"the reference's addPoint (HNSW + KNN seeds, M={cfg['M']}, "
+                                      f"ef_c={cfg['efc']}, K={K}) of the first {ns} points, OpenMP on "
+                                      f"{threads} host threads",
+                            "gpu_same_sample_s": round(gpu_ns_s, 3),
+                            "speedup_same_sample": round(ref_s / gpu_ns_s, 1),
+                            "gpu_all_points_s": round(hnsw_s, 3)}
+                log(f"HNSW baseline: reference {ref_s:.2f}s, GPU {gpu_ns_s:.2f}s on {ns} points")
+        except Exception as e:
+            log(f"HNSW baseline skipped: {e}")
+
     # ---- NNDescent build baseline: the reference's refineKNNG on the host cores -------------
-    # Bounded sample: the first iteration (the heaviest: every list entry is
-    # new) from identical seed lists on the first 50,000 points, once by the
-    # reference (OpenMP, all host threads) and once by the GPU kernels.  The
-    # 1M build is projected from it by the distance evaluations of this run's
-    # own 1M iterations (the work both implementations do per evaluated pair).
+    # Sample: the first iteration (the heaviest: every list entry is new) from
+    # identical seed lists on the first 50,000 points, by the reference (OpenMP,
+    # all host threads) and by the GPU kernels, projected to the whole build by
+    # the distance evaluations of this run's own iterations.  Full (default when
+    # the projection fits --nnd-ref-budget): the reference loads the seed graph
+    # this run saved right after its HNSW construction and runs its own
+    # refineKNNG(iters, S) on the whole graph, timed; its KNN recall@10 on the
+    # same 1,000-node sample is reported beside the GPU graph's.
     nnd_cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -685,11 +739,10 @@ def main() -> int:
                 ns = min(n, 50_000)
                 t = time.time()
                 ixs = Index(ns, d, cfg["M"], cfg["efc"], 100, K)
-                ixs.add_points(x[:ns], labels=np.arange(ns, dtype=np.uint64), nthreads=threads)
+                ixs.add_points_gpu(x[:ns])
                 sids, sdist, scnt = ixs.get_knng()
                 xs = np.ascontiguousarray(x[:ns][ixs.labels()])      # rows in internal-id order
                 del ixs
-                log(f"NNDescent baseline: {ns}-point seed graph in {time.time() - t:.1f}s")
                 r = oracle.RefIndex(ns, d, M=cfg["M"], efc=cfg["efc"], K=K)
                 r.set_raw(xs)
                 r.set_knng(sids, sdist, scnt)
@@ -701,28 +754,82 @@ def main() -> int:
                 ds.set_base(xs)
                 ds.set_knng(sids, sdist, scnt)
                 t1 = time.perf_counter()
-                st_s = ds.nndescent(1, cfg["S"])
+                ds.nndescent(1, cfg["S"])
                 ds.sync()
                 gpu_s = time.perf_counter() - t1
                 ds.close()
                 d1 = float(st[0]["dists"]) if st else 0.0
                 work = sum(float(v["dists"]) for v in st) / d1 if d1 else float(cfg["iters"])
                 proj = cpu_s * (n / ns) * work
-                nnd_cpu = {"value": round(cpu_s, 3), "unit": "s (first iteration)", "cores": threads,
-                           "kind": "reference",
-                           "sample": f"first refineKNNG iteration (S={cfg['S']}, K={K}) from identical seed "
-                                     f"lists on the first {ns} points, OpenMP on {threads} host threads",
-                           "gpu_same_sample_s": round(gpu_s, 4),
-                           "speedup_same_sample": round(cpu_s / gpu_s, 1),
-                           "projected_cpu_build_s": round(proj, 1),
-                           "projection": f"x{n // ns} points, x{work:.2f} for all {cfg['iters']} iterations "
-                                         f"(distance evaluations of this run's 1M build relative to its first "
-                                         f"iteration)",
-                           "speedup_vs_projected": round(proj / nnd_s, 1)}
-                log(f"NNDescent baseline: reference {cpu_s:.2f}s, GPU {gpu_s:.3f}s on the sample; projected "
-                    f"reference 1M build {proj:.0f}s vs GPU {nnd_s:.2f}s")
+                nnd_cpu = {"kind": "reference", "cores": threads,
+                           "sample_first_iteration": {
+                               "value": round(cpu_s, 3), "unit": "s",
+                               "sample": f"first refineKNNG iteration (S={cfg['S']}, K={K}) from identical seed "
+                                         f"lists on the first {ns} points, OpenMP on {threads} host threads",
+                               "gpu_same_sample_s": round(gpu_s, 4),
+                               "speedup_same_sample": round(cpu_s / gpu_s, 1),
+                               "projected_full_build_s": round(proj, 1)}}
+                log(f"NNDescent baseline sample: reference {cpu_s:.2f}s, GPU {gpu_s:.3f}s; projected reference "
+                    f"build {proj:.0f}s")
+                run_full = seed_path is not None and (args.nnd_ref == "full" or proj <= args.nnd_ref_budget)
+                if run_full:
+                    t1 = time.time()
+                    r = oracle.RefIndex.load(seed_path, d)
+                    log(f"reference loaded the seed graph in {time.time() - t1:.1f}s; refineKNNG({cfg['iters']}, "
+                        f"{cfg['S']}) on {threads} threads")
+                    t1 = time.perf_counter()
+                    r.refine(cfg["iters"], cfg["S"], nthreads=threads)
+                    full_s = time.perf_counter() - t1
+                    rid, _, rcnt = r.get_knng()
+                    r.close()
+                    r_rec10 = float(np.mean([len(set(rid[s_, :min(10, int(rcnt[s_]))].tolist())
+                                                 & set(ex_ids[i, 1:11].tolist())) / 10
+                                             for i, s_ in enumerate(samp)]))
+                    del rid, rcnt
+                    nnd_cpu.update({"value": round(full_s, 2), "unit": "s",
+                                    "sample": f"the reference's refineKNNG({cfg['iters']}, {cfg['S']}) on the "
+                                              f"whole {n} x {K} seed graph this run built (saved and loaded), "
+                                              f"OpenMP on {threads} host threads",
+                                    "gpu_s": round(nnd_s, 3), "speedup": round(full_s / nnd_s, 1),
+                                    "reference_knn_recall_at10_sample": round(r_rec10, 4),
+                                    "gpu_knn_recall_at10_sample": round(knn_rec10, 4),
+                                    "recall_difference_points": round(100 * (knn_rec10 - r_rec10), 3)})
+                    log(f"NNDescent baseline full: reference {full_s:.1f}s vs GPU {nnd_s:.2f}s; KNN recall@10 "
+                        f"reference {r_rec10:.4f}, GPU {knn_rec10:.4f}")
+                else:
+                    nnd_cpu.update({"value": round(proj, 1), "unit": "s (projected from the sample)",
+                                    "sample": "projection of the first-iteration sample (no full run: "
+                                              + ("not asked" if args.nnd_ref == "sample" else
+                                                 f"projected above {args.nnd_ref_budget:.0f}s or no seed file") + ")",
+                                    "gpu_s": round(nnd_s, 3), "speedup": round(proj / nnd_s, 1)})
         except Exception as e:
             log(f"NNDescent baseline skipped: {e}")
+    if seed_dir:
+        for f_ in (seed_path,):
+            if f_ and os.path.exists(f_):
+                os.remove(f_)
+        os.rmdir(seed_dir)
+
+    # NNDescent against its roofline, per iteration: algorithmic bytes = the
+    # sample rows staged (s_v bytes per element: 1 for the int8 join of 8-bit
+    # data, 2 for BF16) + the K-lists read for sampling and read and written
+    # by the merge (2 N K 8 bytes), for iterations that ran; flops = 2 d per
+    # evaluated pair
+    s_nnd = 1 if (bool(x.min() >= 0) and bool(x.max() <= 255) and bool(np.all(x == np.round(x))) and d <= 258) else 2
+    it_rows = []
+    for v in st:
+        byt = (float(v["rows"]) * s_nnd * d + 2.0 * n * K * 8) if v["rows"] else 0.0
+        it_rows.append({"ms": round(v["ms"], 3), "rows": int(v["rows"]), "dists": int(v["dists"]),
+                        "alg_bytes": int(byt), "gbs": round(byt / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None})
+    dev_ms = sum(v["ms"] for v in st)
+    nnd_bytes = sum(r_["alg_bytes"] for r_ in it_rows)
+    nnd_flops = 2.0 * d * sum(float(v["dists"]) for v in st)
+    nnd_roof = {"s": round(nnd_s, 3), "device_s": round(dev_ms / 1e3, 4), "dists": int(sum(v["dists"] for v in st)),
+                "alg_bytes": int(nnd_bytes), "gbs": round(nnd_bytes / (dev_ms * 1e-3) / 1e9, 1),
+                "frac_hbm": round(nnd_bytes / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "tops": round(nnd_flops / (dev_ms * 1e-3) / 1e12, 2),
+                "frac_mfma": round(nnd_flops / (dev_ms * 1e-3) / 1e12 / (5000.0 if s_nnd == 1 else 2500.0), 5),
+                "mfma_peak_tops": 5000.0 if s_nnd == 1 else 2500.0, "per_iter": it_rows}
 
     out = {
         "metric": "RkNN QPS at recall>=0.95 (k=10); NNDescent K=500 build sec, 1M x 128",
@@ -785,9 +892,12 @@ def main() -> int:
                     "frac_hbm": round((12 * n * K + 16 * (n + 1)) / csr_dev_s / 1e9 / HBM_PEAK_GBS, 4)},
             "bruteforce_rk": {"s": round(kth_s, 3), "tflops": round(2.0 * n * n * d / kth_s / 1e12, 1),
                               "frac_bf16_mfma": round(2.0 * n * n * d / kth_s / 1e12 / 2500.0, 4)},
-            "nndescent": {"s": round(nnd_s, 3), "dists": int(sum(v["dists"] for v in st)),
-                          "tflops": round(2.0 * d * sum(v["dists"] for v in st) / nnd_s / 1e12, 1)}},
-        "stages_s": {"hnsw_host": round(hnsw_s, 2), "nndescent_gpu": round(nnd_s, 3),
+            "nndescent": nnd_roof},
+        "hnsw_build": {"seconds": round(hnsw_s, 3), "device_ms": round(hnsw_st["ms"], 1),
+                       "batches": int(hnsw_st["batches"]), "expansions": int(hnsw_st["expansions"]),
+                       "evaluations": int(hnsw_st["evaluations"]), "dropped_links": int(hnsw_st["dropped_links"])},
+        "hnsw_cpu_baseline": hnsw_cpu,
+        "stages_s": {"hnsw_gpu": round(hnsw_s, 2), "nndescent_gpu": round(nnd_s, 3),
                      "rknn_csr_gpu": round(csr_s, 3), "rk_bruteforce_gpu": round(kth_s, 3),
                      "groundtruth_gpu": round(gt_s, 3)},
         "sweep": sweep,

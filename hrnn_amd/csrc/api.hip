@@ -355,6 +355,7 @@ static int *tuning_slot(hrnn_dev_t h, const char *key) {
     if (k == "search_lds_kib") return &h->tune_search_lds_kib;
     if (k == "stage1_cap") return &h->tune_stage1_cap;
     if (k == "hnsw_batch_permille") return &h->tune_hnsw_batch_permille;
+    if (k == "hnsw_seed_exp") return &h->tune_hnsw_seed_exp;
     return nullptr;
 }
 
@@ -363,7 +364,8 @@ int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value) {
     int *slot = tuning_slot(h, key);
     const bool cap = slot == &h->tune_stage1_cap;
     const bool permille = slot == &h->tune_hnsw_batch_permille;
-    if (!slot || value < (cap ? 0 : 1) || value > (cap ? 1024 : permille ? 1000 : 64)) {
+    const bool seedx = slot == &h->tune_hnsw_seed_exp;
+    if (!slot || value < (cap || seedx ? 0 : 1) || value > (cap ? 1024 : permille || seedx ? 1000 : 64)) {
         h->err = std::string("set_tuning: unknown key or value out of range: ") + (key ? key : "");
         return HRNN_E_ARG;
     }

@@ -37,7 +37,8 @@ struct hrnn_dev {
     int tune_search_lds_kib = 16;    // LDS dedup table of the RkNN search, KiB
     int tune_beam_wg_per_cu = 16;    // persistent beam workgroups (one wave each) per CU
     int tune_stage1_cap = 0;         // learned early stop: stage-1 expansion cap (0: none)
-    int tune_hnsw_batch_permille = 20;   // HNSW construction: batch size / graph size, per mille
+    int tune_hnsw_batch_permille = 50;   // HNSW construction: batch size / graph size, per mille
+    int tune_hnsw_seed_exp = 32;         // HNSW construction: expansions of the seed completion
 
     // ranked KNN graph (SoA); knn_version changes whenever it may have changed
     uint32_t K = 0;

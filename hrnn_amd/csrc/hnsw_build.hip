@@ -157,6 +157,155 @@ __device__ uint32_t select_diverse_wave(const BuildGraph &G, const unsigned long
     return ns;
 }
 
+// LDS state and per-beam constants of one construction beam.
+struct BeamCtx {
+    float *qv;
+    uint32_t *q8, *vis;
+    unsigned long long *kl;
+    uint8_t *xl;
+    float *nd;
+    uint32_t *ni;
+    unsigned long long *ck;
+    uint32_t *lost, *bad;
+    uint32_t ef, ef_cap, log_vis, qq;
+    bool exact;
+    __amdgpu_buffer_rsrc_t xrs;
+};
+
+// The beam's expansion loop at layer L over the list kl[buf] of Ln entries
+// (with expanded flags; the visited set already holds every listed node):
+// expand the first unexpanded entry, admit its unvisited neighbours with id
+// below hi while the list holds fewer than ef entries or the (dist, id) key
+// is below the worst, keep the list sorted; stop when every entry is
+// expanded or after max_exp expansions.  Ln and buf are updated.
+template <class VT>
+__device__ void beam_expand(const BuildGraph &G, const VT *__restrict__ x, BeamCtx &cx, int L, uint32_t &Ln,
+                            uint32_t &buf, uint32_t n, uint32_t hi, unsigned long long &n_exp,
+                            unsigned long long &n_eval, uint32_t max_exp = 0xFFFFFFFFu) {
+    const uint32_t lane = threadIdx.x, g = lane & 15u, grp = lane >> 4;
+    const unsigned long long exp_end = max_exp == 0xFFFFFFFFu ? ~0ull : n_exp + max_exp;
+    // the first unexpanded entry
+    uint32_t pick = kInvalidId;
+    for (uint32_t q0 = 0; q0 < Ln; q0 += 64) {
+        const uint32_t t = q0 + lane;
+        const uint64_t mask = __ballot(t < Ln && cx.xl[buf * cx.ef_cap + t] == 0);
+        if (mask) {
+            pick = q0 + (uint32_t)__ffsll((long long)mask) - 1;
+            break;
+        }
+    }
+    for (uint32_t iter = 0; iter <= n && pick != kInvalidId && n_exp < exp_end; iter++) {
+        const unsigned long long *lk = cx.kl + buf * cx.ef_cap;
+        uint8_t *lx = cx.xl + buf * cx.ef_cap;
+        const uint32_t c = uni((uint32_t)lk[pick]);
+        uint32_t cnt, nbw;
+        layer_row(G, L, c, lane, &cnt, &nbw, cx.bad);
+        __syncthreads();
+        if (lane == 0) lx[pick] = 1;
+        n_exp++;
+        uint32_t nb = 0;
+        bool fresh = false;
+        if (lane < cnt) {
+            nb = nbw;
+            if (nb < n) fresh = vis_insert(cx.vis, cx.log_vis, nb, cx.lost);
+            else *cx.bad = 1u;
+        }
+        const uint64_t fm = __ballot(fresh);
+        const uint32_t nf = (uint32_t)__popcll(fm);
+        n_eval += nf;
+        if (fresh) cx.ni[__popcll(fm & lanemask_lt())] = nb;
+        __syncthreads();
+        uint32_t o = kInvalidId;   // the first unexpanded entry after pick
+        for (uint32_t q0 = pick + 1; q0 < Ln; q0 += 64) {
+            const uint32_t t = q0 + lane;
+            const uint64_t mask = __ballot(t < Ln && lx[t] == 0);
+            if (mask) {
+                o = q0 + (uint32_t)__ffsll((long long)mask) - 1;
+                break;
+            }
+        }
+        if (nf == 0) {
+            pick = o;
+            continue;
+        }
+        constexpr int kRows = std::is_same<VT, uint8_t>::value ? 8 : 4;
+        for (uint32_t b = 0; b < nf; b += 4 * kRows) {
+            uint32_t idr[kRows];
+            bool vr[kRows];
+#pragma unroll
+            for (int r = 0; r < kRows; r++) {
+                const uint32_t t = b + grp + 4u * (uint32_t)r;
+                vr[r] = t < nf;
+                idr[r] = cx.ni[vr[r] ? t : b];
+            }
+            float orr[kRows];
+            beam_dist4<VT, kRows>(cx.exact, cx.qv, cx.q8, cx.qq, x, cx.xrs, idr, vr, G.d, g, orr);
+            if (g == 0) {
+#pragma unroll
+                for (int r = 0; r < kRows; r++)
+                    if (vr[r]) cx.nd[b + grp + 4u * (uint32_t)r] = orr[r];
+            }
+        }
+        __syncthreads();
+        unsigned long long key = ~0ull;
+        if (lane < nf) {
+            const unsigned long long kc = beam_key(cx.nd[lane], cx.ni[lane]);
+            bool ok = (uint32_t)kc < hi && (Ln < cx.ef || kc < lk[Ln - 1]);
+            if (ok && *cx.lost) {
+                for (uint32_t j = 0; j < Ln && ok; j++) ok = (uint32_t)lk[j] != (uint32_t)kc;
+                for (uint32_t j = 0; j < lane && ok; j++) ok = cx.ni[j] != (uint32_t)kc;
+            }
+            if (ok) key = kc;
+        }
+        const uint64_t am = __ballot(key != ~0ull);
+        const uint32_t nv = (uint32_t)__popcll(am);
+        if (nv == 0) {
+            pick = o;
+            continue;
+        }
+        if (key != ~0ull) cx.ck[__popcll(am & lanemask_lt())] = key;
+        __syncthreads();
+        key = (lane < nv) ? cx.ck[lane] : ~0ull;
+        key = wave_sort64(key, lane, next_pow2_dev(nv));
+        const unsigned long long kbest = read_lane64(key, 0);
+        const unsigned long long ko = (o != kInvalidId) ? uni64(lk[o]) : ~0ull;
+        const uint32_t nbuf = buf ^ 1u;
+        unsigned long long *nk = cx.kl + nbuf * cx.ef_cap;
+        uint8_t *nx = cx.xl + nbuf * cx.ef_cap;
+        uint32_t below = 0;
+        for (uint32_t q0 = 0; q0 < Ln; q0 += 64) {
+            const uint32_t e = q0 + lane;
+            const bool valid = e < Ln;
+            const unsigned long long ke = valid ? lk[e] : ~0ull;
+            const uint8_t xe = valid ? lx[e] : (uint8_t)1;
+            uint32_t less = 0;
+            for (uint32_t j = 0; j < nv; j++) {
+                const unsigned long long kj = read_lane64(key, j);
+                const bool lt = kj < ke;
+                less += lt ? 1u : 0u;
+                const uint32_t cb = (uint32_t)__popcll(__ballot(valid && !lt));
+                if (lane == j) below += cb;
+            }
+            const uint32_t pos = e + less;
+            if (valid && pos < cx.ef) {
+                nk[pos] = ke;
+                nx[pos] = xe;
+            }
+        }
+        if (lane < nv) {
+            const uint32_t pos = lane + below;
+            if (pos < cx.ef) {
+                nk[pos] = key;
+                nx[pos] = 0;
+            }
+        }
+        pick = (kbest < ko) ? (uint32_t)__builtin_amdgcn_readlane((int)below, 0) : o;
+        __syncthreads();
+        Ln = min(Ln + nv, cx.ef);
+        buf = nbuf;
+    }
+}
+
 // One inserted point per workgroup of one wave.
 template <class VT>
 __global__ __launch_bounds__(64) void k_hnsw_insert(
@@ -277,6 +426,7 @@ __global__ __launch_bounds__(64) void k_hnsw_insert(
     }
 
     unsigned long long n_exp = 0, n_eval = 1;
+    BeamCtx bs{s_qv, s_q8, s_vis, s_kl, s_xl, s_nd, s_ni, s_ck, &s_lost, &s_bad, ef, ef_cap, log_vis, qq, exact, xrs};
     for (int L = min(lp, maxlevel); L >= 0; L--) {
         // ---- beam at layer L from (cur, cd) ----
         for (uint32_t t = lane; t < (1u << log_vis) / 4; t += 64)
@@ -289,117 +439,8 @@ __global__ __launch_bounds__(64) void k_hnsw_insert(
             vis_insert(s_vis, log_vis, cur, &s_lost);
         }
         __syncthreads();
-        uint32_t Ln = 1, buf = 0, pick = 0;
-        for (uint32_t iter = 0; iter <= n && pick != kInvalidId; iter++) {
-            const unsigned long long *lk = s_kl + buf * ef_cap;
-            uint8_t *lx = s_xl + buf * ef_cap;
-            const uint32_t c = uni((uint32_t)lk[pick]);
-            uint32_t cnt, nbw;
-            layer_row(G, L, c, lane, &cnt, &nbw, &s_bad);
-            __syncthreads();
-            if (lane == 0) lx[pick] = 1;
-            n_exp++;
-            uint32_t nb = 0;
-            bool fresh = false;
-            if (lane < cnt) {
-                nb = nbw;
-                if (nb < n) fresh = vis_insert(s_vis, log_vis, nb, &s_lost);
-                else s_bad = 1u;
-            }
-            const uint64_t fm = __ballot(fresh);
-            const uint32_t nf = (uint32_t)__popcll(fm);
-            n_eval += nf;
-            if (fresh) s_ni[__popcll(fm & lanemask_lt())] = nb;
-            __syncthreads();
-            uint32_t o = kInvalidId;   // the first unexpanded entry after pick
-            for (uint32_t q0 = pick + 1; q0 < Ln; q0 += 64) {
-                const uint32_t t = q0 + lane;
-                const uint64_t mask = __ballot(t < Ln && lx[t] == 0);
-                if (mask) {
-                    o = q0 + (uint32_t)__ffsll((long long)mask) - 1;
-                    break;
-                }
-            }
-            if (nf == 0) {
-                pick = o;
-                continue;
-            }
-            constexpr int kRows = std::is_same<VT, uint8_t>::value ? 8 : 4;
-            for (uint32_t b = 0; b < nf; b += 4 * kRows) {
-                uint32_t idr[kRows];
-                bool vr[kRows];
-#pragma unroll
-                for (int r = 0; r < kRows; r++) {
-                    const uint32_t t = b + grp + 4u * (uint32_t)r;
-                    vr[r] = t < nf;
-                    idr[r] = s_ni[vr[r] ? t : b];
-                }
-                float orr[kRows];
-                beam_dist4<VT, kRows>(exact, s_qv, s_q8, qq, x, xrs, idr, vr, d, g, orr);
-                if (g == 0) {
-#pragma unroll
-                    for (int r = 0; r < kRows; r++)
-                        if (vr[r]) s_nd[b + grp + 4u * (uint32_t)r] = orr[r];
-                }
-            }
-            __syncthreads();
-            unsigned long long key = ~0ull;
-            if (lane < nf) {
-                const unsigned long long kc = beam_key(s_nd[lane], s_ni[lane]);
-                bool ok = Ln < ef || kc < lk[Ln - 1];
-                if (ok && s_lost) {
-                    for (uint32_t j = 0; j < Ln && ok; j++) ok = (uint32_t)lk[j] != (uint32_t)kc;
-                    for (uint32_t j = 0; j < lane && ok; j++) ok = s_ni[j] != (uint32_t)kc;
-                }
-                if (ok) key = kc;
-            }
-            const uint64_t am = __ballot(key != ~0ull);
-            const uint32_t nv = (uint32_t)__popcll(am);
-            if (nv == 0) {
-                pick = o;
-                continue;
-            }
-            if (key != ~0ull) s_ck[__popcll(am & lanemask_lt())] = key;
-            __syncthreads();
-            key = (lane < nv) ? s_ck[lane] : ~0ull;
-            key = wave_sort64(key, lane, next_pow2_dev(nv));
-            const unsigned long long kbest = read_lane64(key, 0);
-            const unsigned long long ko = (o != kInvalidId) ? uni64(lk[o]) : ~0ull;
-            const uint32_t nbuf = buf ^ 1u;
-            unsigned long long *nk = s_kl + nbuf * ef_cap;
-            uint8_t *nx = s_xl + nbuf * ef_cap;
-            uint32_t below = 0;
-            for (uint32_t q0 = 0; q0 < Ln; q0 += 64) {
-                const uint32_t e = q0 + lane;
-                const bool valid = e < Ln;
-                const unsigned long long ke = valid ? lk[e] : ~0ull;
-                const uint8_t xe = valid ? lx[e] : (uint8_t)1;
-                uint32_t less = 0;
-                for (uint32_t j = 0; j < nv; j++) {
-                    const unsigned long long kj = read_lane64(key, j);
-                    const bool lt = kj < ke;
-                    less += lt ? 1u : 0u;
-                    const uint32_t cb = (uint32_t)__popcll(__ballot(valid && !lt));
-                    if (lane == j) below += cb;
-                }
-                const uint32_t pos = e + less;
-                if (valid && pos < ef) {
-                    nk[pos] = ke;
-                    nx[pos] = xe;
-                }
-            }
-            if (lane < nv) {
-                const uint32_t pos = lane + below;
-                if (pos < ef) {
-                    nk[pos] = key;
-                    nx[pos] = 0;
-                }
-            }
-            pick = (kbest < ko) ? (uint32_t)__builtin_amdgcn_readlane((int)below, 0) : o;
-            __syncthreads();
-            Ln = min(Ln + nv, ef);
-            buf = nbuf;
-        }
+        uint32_t Ln = 1, buf = 0;
+        beam_expand<VT>(G, x, bs, L, Ln, buf, n, 0xFFFFFFFFu, n_exp, n_eval);
         const unsigned long long *lk = s_kl + buf * ef_cap;
 
         // ---- layer 0: the KNN seed, the beam in (dist, id) order without p ----
@@ -440,6 +481,93 @@ __global__ __launch_bounds__(64) void k_hnsw_insert(
     if (lane == 0) {
         atomicAdd(&stats[0], n_exp);
         atomicAdd(&stats[1], n_eval);
+        if (s_bad) atomicOr(err, 1u);
+    }
+}
+
+// Seed completion after a batch's links are merged: the points of one batch
+// searched the graph as it was before the batch, so a point's seed list
+// misses the points of its own batch that sequential insertion would have
+// let it see (those before it).  Starting from its seed list (all entries
+// unexpanded), each point continues its layer-0 beam over the graph that now
+// links the batch for at most max_exp expansions, admitting only ids below
+// its own, and keeps the first K entries.  One wave per point.
+template <class VT>
+__global__ __launch_bounds__(64) void k_hnsw_seed_fix(
+    const VT *__restrict__ x, BuildGraph G, uint32_t b0, uint32_t b1, uint32_t ef, uint32_t ef_cap,
+    uint32_t log_vis, int base_exact, uint32_t K, uint32_t max_exp, uint32_t *__restrict__ knn_ids,
+    float *__restrict__ knn_d, uint32_t *__restrict__ knn_cnt, uint32_t *__restrict__ err,
+    unsigned long long *__restrict__ stats) {
+    __shared__ float s_nd[kBeamMaxDeg];
+    __shared__ uint32_t s_ni[kBeamMaxDeg];
+    __shared__ unsigned long long s_ck[kBeamMaxDeg];
+    __shared__ uint32_t s_lost, s_bad;
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    const BeamLds lay(G.d, ef_cap, log_vis);
+    float *s_qv = reinterpret_cast<float *>(s_dyn);
+    uint32_t *s_q8 = reinterpret_cast<uint32_t *>(s_dyn + lay.q8_off);
+    uint32_t *s_vis = reinterpret_cast<uint32_t *>(s_dyn + lay.vis_off);
+    unsigned long long *s_kl = reinterpret_cast<unsigned long long *>(s_dyn + lay.k_off);
+    uint8_t *s_xl = s_dyn + lay.x_off;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t p = uni(b0 + blockIdx.x);
+    if (p >= b1) return;
+    const uint32_t d = G.d, n = G.n;
+    const uint32_t c0 = uni(min(knn_cnt[p], min(ef, K)));
+    if (c0 == 0) return;
+    bool q_u8 = true;
+    for (uint32_t e = lane; e < d; e += 64) {
+        const float v = G.xf[(uint64_t)p * d + e];
+        s_qv[e] = v;
+        q_u8 &= (v == rintf(v)) && v >= 0.f && v <= 255.f;
+    }
+    const bool exact = base_exact && __ballot(!q_u8) == 0;
+    uint32_t qq = 0;
+    if constexpr (std::is_same<VT, uint8_t>::value) {
+        if (exact) {
+            uint32_t part = 0;
+            for (uint32_t w = lane; 4 * w < d; w += 64) {
+                const float *qw = G.xf + (uint64_t)p * d + 4 * w;
+                const uint32_t c0b = (uint32_t)qw[0], c1 = (uint32_t)qw[1], c2 = (uint32_t)qw[2], c3 = (uint32_t)qw[3];
+                s_q8[w] = c0b | (c1 << 8) | (c2 << 16) | (c3 << 24);
+                part += c0b * c0b + c1 * c1 + c2 * c2 + c3 * c3;
+            }
+            for (int s2 = 32; s2 > 0; s2 >>= 1) part += (uint32_t)__shfl_xor((int)part, s2, 64);
+            qq = uni(part);
+        }
+    }
+    const __amdgpu_buffer_rsrc_t xrs = make_rsrc(x, exact ? (uint32_t)((uint64_t)n * d * sizeof(VT)) : 0u);
+    for (uint32_t t = lane; t < (1u << log_vis) / 4; t += 64)
+        reinterpret_cast<uint4 *>(s_vis)[t] = make_uint4(kInvalidId, kInvalidId, kInvalidId, kInvalidId);
+    if (lane == 0) {
+        s_lost = 0;
+        s_bad = 0;
+    }
+    __syncthreads();
+    // the seed list, ascending (dist, id), as the start list; p and its seeds visited
+    for (uint32_t t = lane; t < c0; t += 64) {
+        const uint32_t id = knn_ids[(uint64_t)p * K + t];
+        s_kl[t] = beam_key(knn_d[(uint64_t)p * K + t], id);
+        s_xl[t] = 0;
+        vis_insert(s_vis, log_vis, id, &s_lost);
+    }
+    if (lane == 0) vis_insert(s_vis, log_vis, p, &s_lost);
+    __syncthreads();
+    unsigned long long n_exp = 0, n_eval = 0;
+    BeamCtx bs{s_qv, s_q8, s_vis, s_kl, s_xl, s_nd, s_ni, s_ck, &s_lost, &s_bad, ef, ef_cap, log_vis, qq, exact, xrs};
+    uint32_t Ln = c0, buf = 0;
+    beam_expand<VT>(G, x, bs, 0, Ln, buf, n, p, n_exp, n_eval, max_exp);
+    const unsigned long long *lk = s_kl + buf * ef_cap;
+    const uint32_t keep = min(Ln, K);
+    for (uint32_t t = lane; t < keep; t += 64) {
+        const unsigned long long kt = lk[t];
+        knn_ids[(uint64_t)p * K + t] = (uint32_t)kt;
+        knn_d[(uint64_t)p * K + t] = key_float((uint32_t)(kt >> 32));
+    }
+    if (lane == 0) {
+        knn_cnt[p] = keep;
+        atomicAdd(&stats[3], n_exp);
+        atomicAdd(&stats[4], n_eval);
         if (s_bad) atomicOr(err, 1u);
     }
 }
@@ -574,6 +702,7 @@ int hrnn_dev_hnsw_build(hrnn_dev_t h, const int *levels, uint32_t M, uint32_t ef
         REQUIRE(top < 100, HRNN_E_ARG, "hnsw_build: more than 100 layers");
         const uint32_t prop_stride = M * (uint32_t)(top + 1);
         uint32_t frac = (uint32_t)std::max(1, h->tune_hnsw_batch_permille);
+        const uint32_t seed_exp = (uint32_t)std::max(0, h->tune_hnsw_seed_exp);
         const uint32_t bmax = 65536;
 
         hipStream_t st = h->stream;
@@ -605,7 +734,8 @@ int hrnn_dev_hnsw_build(hrnn_dev_t h, const int *levels, uint32_t M, uint32_t ef
         h->hb_ctr.reserve(64 * 8);
         HIPC(hipMemsetAsync(h->hb_inc_cnt.p, 0, (uint64_t)n * 4, st));
         HIPC(hipMemsetAsync(h->hb_ctr.p, 0, 64 * 8, st));
-        unsigned long long *ctr = h->hb_ctr.as<unsigned long long>();   // [0] expansions [1] evaluations [2] dropped
+        // [0] expansions [1] evaluations [2] dropped links [3] [4] seed completion expansions, evaluations
+        unsigned long long *ctr = h->hb_ctr.as<unsigned long long>();
         uint32_t *n_touched = reinterpret_cast<uint32_t *>(ctr + 8);
         uint32_t *err = h->counters.as<uint32_t>() + kErrWord;
 
@@ -663,6 +793,21 @@ int hrnn_dev_hnsw_build(hrnn_dev_t h, const int *levels, uint32_t M, uint32_t ef
                                    h->hb_inc.as<unsigned long long>(), err);
                 HIPC(hipGetLastError());
             }
+            if (K > 0 && seed_exp > 0) {
+#define LAUNCH(VT, XP)                                                                                       \
+    hipLaunchKernelGGL((k_hnsw_seed_fix<VT>), dim3(nb), dim3(64), lay.total, st, XP, G, b, b1, ef, ef_cap,   \
+                       log_vis, base_exact, K, seed_exp, h->knn_ids.as<uint32_t>(), h->knn_d.as<float>(),    \
+                       h->knn_cnt.as<uint32_t>(), err, ctr)
+                if (use_u8) {
+                    LAUNCH(uint8_t, h->xu8.as<uint8_t>());
+                } else if (h->small_int) {
+                    LAUNCH(__hip_bfloat16, h->xb.as<__hip_bfloat16>());
+                } else {
+                    LAUNCH(float, h->x.as<float>());
+                }
+#undef LAUNCH
+                HIPC(hipGetLastError());
+            }
             hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, n_touched, (uint32_t)(lb + 1));
             HIPC(hipGetLastError());
             for (uint32_t i = b; i < b1; i++)
@@ -674,7 +819,7 @@ int hrnn_dev_hnsw_build(hrnn_dev_t h, const int *levels, uint32_t M, uint32_t ef
             batches++;
         }
         HIPC(hipStreamSynchronize(st));
-        unsigned long long hc[3] = {0, 0, 0};
+        unsigned long long hc[5] = {0, 0, 0, 0, 0};
         d2h(h, hc, ctr, sizeof(hc));
         check_device_error(h);
         h->l0_words = l0_words;
@@ -690,6 +835,8 @@ int hrnn_dev_hnsw_build(hrnn_dev_t h, const int *levels, uint32_t M, uint32_t ef
             stats->expansions = hc[0];
             stats->evaluations = hc[1];
             stats->dropped_links = hc[2];
+            stats->seed_expansions = hc[3];
+            stats->seed_evaluations = hc[4];
             stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
         }
     });

@@ -111,10 +111,11 @@ int hrnn_index_add_points(hrnn_index_t ix, const float *x, uint64_t n, const uin
 }
 
 int hrnn_index_add_points_gpu(hrnn_index_t ix, const float *x, uint64_t n, const uint64_t *labels,
-                              int batch_permille, hrnn_hnsw_build_stats *stats) {
-    if (!ix || (!x && n) || batch_permille < 0 || batch_permille > 1000) return HRNN_E_ARG;
+                              int batch_permille, int seed_expansions, hrnn_hnsw_build_stats *stats) {
+    if (!ix || (!x && n) || batch_permille < 0 || batch_permille > 1000 || seed_expansions > 1000) return HRNN_E_ARG;
     return guard(ix, [&] {
         ix->idx->setBuildBatchPermille(batch_permille);
+        ix->idx->setBuildSeedExpansions(seed_expansions);
         std::vector<hnswlib::labeltype> lab;
         if (labels) lab.assign(labels, labels + n);
         ix->idx->addPointsDevice(x, n, labels ? lab.data() : nullptr);

@@ -54,7 +54,7 @@ namespace hrnn {
 typedef __bf16 bf16x8_n __attribute__((ext_vector_type(8)));
 typedef float f32x4_n __attribute__((ext_vector_type(4)));
 
-enum { ST_DISTS = 0, ST_PROPOSALS = 1, ST_DROPPED = 2, ST_UPDATES = 3, ST_N = 4 };
+enum { ST_DISTS = 0, ST_PROPOSALS = 1, ST_DROPPED = 2, ST_UPDATES = 3, ST_ROWS = 4, ST_N = 5 };
 // Each counter is spread over kStatSlots words (workgroup b adds to slot
 // b % kStatSlots; the host sums them): a million workgroups adding to one
 // word would serialise on it.
@@ -234,6 +234,7 @@ __global__ __launch_bounds__(256) void k_nnd_join(
     __syncthreads();
     const uint32_t nn = s_nn, T = s_T;
     if (nn == 0) return;   // no new sample: no pair to evaluate
+    if (!kFill && threadIdx.x == 0) atomicAdd(stat_at(stats, ST_ROWS), (unsigned long long)T);   // rows staged
     // The R smallest source ids among the (<= RB) reverse offers of each kind.
     {
         const uint32_t cn = min(rn_cnt[node], RB), co = min(ro_cnt[node], RB);
@@ -882,27 +883,51 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
     const bool use_graph = !(env && env[0] == '0') && iters > 0;
     std::vector<unsigned long long> sv((uint64_t)std::max(iters, 1) * kStatWords, 0);
     if (use_graph) {
-        // All iterations captured as one HIP graph: 10 kernels per iteration, no
-        // host synchronisation in between; counters are reset by kernels only.
-        auto t0 = std::chrono::steady_clock::now();
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        HIPC(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        for (int it = 0; it < iters; it++) enqueue_iteration(it);
-        hipError_t ce = hipStreamEndCapture(h->stream, &graph);
-        if (ce != hipSuccess) throw Error{HRNN_E_HIP, std::string("nndescent capture: ") + hipGetErrorString(ce)};
-        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        if (ie != hipSuccess) {
-            (void)hipGraphDestroy(graph);
-            throw Error{HRNN_E_HIP, std::string("nndescent graph instantiate: ") + hipGetErrorString(ie)};
+        // Each iteration captured as a HIP graph (10 kernels, no host
+        // synchronisation; counters are reset by kernels only), all of them
+        // instantiated before the first is launched, then launched back to
+        // back with an event between consecutive iterations: each
+        // iteration's device time is measured, not apportioned.
+        std::vector<hipGraph_t> graphs(iters, nullptr);
+        std::vector<hipGraphExec_t> execs(iters, nullptr);
+        std::vector<hipEvent_t> evs(iters + 1, nullptr);
+        auto cleanup = [&] {
+            for (auto &e : execs)
+                if (e) (void)hipGraphExecDestroy(e);
+            for (auto &g : graphs)
+                if (g) (void)hipGraphDestroy(g);
+            for (auto &e : evs)
+                if (e) (void)hipEventDestroy(e);
+        };
+        try {
+            for (auto &e : evs) HIPC(hipEventCreate(&e));
+            for (int it = 0; it < iters; it++) {
+                HIPC(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+                enqueue_iteration(it);
+                hipError_t ce = hipStreamEndCapture(h->stream, &graphs[it]);
+                if (ce != hipSuccess)
+                    throw Error{HRNN_E_HIP, std::string("nndescent capture: ") + hipGetErrorString(ce)};
+                hipError_t ie = hipGraphInstantiate(&execs[it], graphs[it], nullptr, nullptr, 0);
+                if (ie != hipSuccess)
+                    throw Error{HRNN_E_HIP, std::string("nndescent graph instantiate: ") + hipGetErrorString(ie)};
+            }
+            HIPC(hipEventRecord(evs[0], h->stream));
+            for (int it = 0; it < iters; it++) {
+                HIPC(hipGraphLaunch(execs[it], h->stream));
+                HIPC(hipEventRecord(evs[it + 1], h->stream));
+            }
+            d2h(h, sv.data(), all_stats, sv.size() * 8);
+            if (st)
+                for (int it = 0; it < iters; it++) {
+                    float ms = 0.f;
+                    HIPC(hipEventElapsedTime(&ms, evs[it], evs[it + 1]));
+                    st[it].ms = ms;
+                }
+        } catch (...) {
+            cleanup();
+            throw;
         }
-        HIPC(hipGraphLaunch(exec, h->stream));
-        d2h(h, sv.data(), all_stats, sv.size() * 8);
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
-        double total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        if (st)
-            for (int it = 0; it < iters; it++) st[it].ms = total / iters;   // per-iteration share
+        cleanup();
     } else {
         for (int it = 0; it < iters; it++) {
             auto t0 = std::chrono::steady_clock::now();
@@ -914,13 +939,14 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
     }
     if (st)
         for (int it = 0; it < iters; it++) {
-            unsigned long long s[ST_N] = {0, 0, 0, 0};
+            unsigned long long s[ST_N] = {0, 0, 0, 0, 0};
             for (int c = 0; c < ST_N; c++)
                 for (uint32_t j = 0; j < kStatSlots; j++) s[c] += sv[(uint64_t)it * kStatWords + c * kStatSlots + j];
             st[it].dists = s[ST_DISTS];
             st[it].proposals = s[ST_PROPOSALS];
             st[it].dropped = s[ST_DROPPED];
             st[it].updates = s[ST_UPDATES];
+            st[it].rows = s[ST_ROWS];
         }
 }
 

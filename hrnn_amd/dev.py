@@ -52,12 +52,13 @@ class HrnnError(RuntimeError):
 
 class HnswBuildStats(C.Structure):
     _fields_ = [("batches", C.c_uint64), ("expansions", C.c_uint64), ("evaluations", C.c_uint64),
-                ("dropped_links", C.c_uint64), ("ms", C.c_double)]
+                ("dropped_links", C.c_uint64), ("ms", C.c_double), ("seed_expansions", C.c_uint64),
+                ("seed_evaluations", C.c_uint64)]
 
 
 class NndIterStats(C.Structure):
     _fields_ = [("dists", C.c_uint64), ("updates", C.c_uint64), ("proposals", C.c_uint64),
-                ("dropped", C.c_uint64), ("ms", C.c_double)]
+                ("dropped", C.c_uint64), ("ms", C.c_double), ("rows", C.c_uint64)]
 
 
 _lib = None
@@ -285,7 +286,7 @@ class Device:
         st = (NndIterStats * max(iters, 1))()
         self._ck(self.L.hrnn_dev_nndescent(self.h, iters, sample, bucket, st))
         return [dict(dists=s.dists, updates=s.updates, proposals=s.proposals,
-                     dropped=s.dropped, ms=s.ms) for s in st[:iters]]
+                     dropped=s.dropped, ms=s.ms, rows=s.rows) for s in st[:iters]]
 
     # -- reverse CSR -----------------------------------------------------------
     def build_rknn_csr(self) -> CsrInfo:

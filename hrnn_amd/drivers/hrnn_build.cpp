@@ -1,7 +1,8 @@
 // hrnn_build -- build an HRNN index (CLI of the reference's src/hrnn_build.cpp:
 // same flags, defaults and report lines).  Stages: [1] load fbin, [2] HNSW +
-// KNN seeding (host, OpenMP), [3] refineKNNG (GPU), [4] buildRKNNG (GPU),
-// [5] saveIndex (reference file format).
+// KNN seeding (GPU, addPointsDevice; --hnsw_device 0: host addPoint under
+// OpenMP), [3] refineKNNG (GPU), [4] buildRKNNG (GPU), [5] saveIndex
+// (reference file format).
 #include <omp.h>
 
 #include <atomic>
@@ -22,6 +23,7 @@ int main(int argc, char **argv) {
     cli.add("nn_iters", "3", "NNDescent refinement iterations");
     cli.add("nn_sample", "20", "NNDescent sample size per node per iteration");
     cli.add("num_threads", "64", "Number of threads");
+    cli.add("hnsw_device", "1", "1 = HNSW construction on the GPU, 0 = addPoint on the host threads");
     try {
         if (!cli.parse(argc, argv)) return 0;
         const int M = cli.i("M"), efc = cli.i("ef_construction"), K = cli.i("K_knng");
@@ -41,20 +43,27 @@ int main(int argc, char **argv) {
         hnswlib::L2Space space(base.d);
         auto index = std::make_unique<hnswlib::HRNN<float>>(&space, base.n, M, efc, 100, false, (size_t)K);
         t0 = Clock::now();
-        std::atomic<int> progress{0};
         const int total = (int)base.n;
-        const int every = std::max(1, total / 20);
+        if (cli.i("hnsw_device") != 0) {
+            index->addPointsDevice(base.data.data(), base.n);
+            const hrnn_hnsw_build_stats &bs = index->lastBuildStats();
+            std::cout << "      HNSW on GPU: " << total << " points in " << bs.batches << " batches, "
+                      << bs.expansions << " expansions, " << bs.evaluations << " distances" << std::endl;
+        } else {
+            std::atomic<int> progress{0};
+            const int every = std::max(1, total / 20);
 #pragma omp parallel for schedule(dynamic, 512)
-        for (int i = 0; i < total; ++i) {
-            index->addPoint(base.data.data() + (size_t)i * base.d, (hnswlib::labeltype)i);
-            int done = progress.fetch_add(1, std::memory_order_relaxed) + 1;
-            if (done % every == 0 || done == total) {
+            for (int i = 0; i < total; ++i) {
+                index->addPoint(base.data.data() + (size_t)i * base.d, (hnswlib::labeltype)i);
+                int done = progress.fetch_add(1, std::memory_order_relaxed) + 1;
+                if (done % every == 0 || done == total) {
 #pragma omp critical
-                std::cout << "\r      HNSW addPoint: " << done << " / " << total << " ("
-                          << (100LL * done / total) << "%)  [" << elapsed_ms(t0) << " ms]" << std::flush;
+                    std::cout << "\r      HNSW addPoint: " << done << " / " << total << " ("
+                              << (100LL * done / total) << "%)  [" << elapsed_ms(t0) << " ms]" << std::flush;
+                }
             }
+            std::cout << std::endl;
         }
-        std::cout << std::endl;
         std::cout << "      HNSW + initial KNNG done  (" << elapsed_ms(t0) << " ms)" << std::endl;
 
         std::cout << "\n[3] Refining KNNG (NNDescent local join, " << iters << " iterations) ..." << std::endl;

@@ -34,7 +34,7 @@ def host_lib() -> C.CDLL:
         "hrnn_index_free": [vp],
         "hrnn_index_info": [vp, P(u32), P(u32), P(u32), P(i)],
         "hrnn_index_add_points": [vp, vp, u64, vp, i],
-        "hrnn_index_add_points_gpu": [vp, vp, u64, vp, i, vp],
+        "hrnn_index_add_points_gpu": [vp, vp, u64, vp, i, i, vp],
         "hrnn_index_refine": [vp, i, i, vp],
         "hrnn_index_build_rknng": [vp],
         "hrnn_index_build_rknng_mutable": [vp],
@@ -124,21 +124,21 @@ class Index:
         lab = None if labels is None else np.ascontiguousarray(labels, np.uint64)
         self._ck(self.L.hrnn_index_add_points(self.h, _p(x), x.shape[0], _p(lab), nthreads))
 
-    def add_points_gpu(self, x, labels=None, batch_permille=0):
+    def add_points_gpu(self, x, labels=None, batch_permille=0, seed_expansions=-1):
         """HNSW construction with KNN seeding on the GPU (empty index only);
         returns the build's counters."""
         x = np.ascontiguousarray(x, np.float32)
         lab = None if labels is None else np.ascontiguousarray(labels, np.uint64)
         st = HnswBuildStats()
         self._ck(self.L.hrnn_index_add_points_gpu(self.h, _p(x), x.shape[0], _p(lab), batch_permille,
-                                                  C.byref(st)))
+                                                  seed_expansions, C.byref(st)))
         return {k: getattr(st, k) for k, _ in HnswBuildStats._fields_}
 
     def refine(self, iters: int, sample: int):
         st = (NndIterStats * max(iters, 1))()
         self._ck(self.L.hrnn_index_refine(self.h, iters, sample, st))
         return [dict(dists=s.dists, updates=s.updates, proposals=s.proposals, dropped=s.dropped,
-                     ms=s.ms) for s in st[:iters]]
+                     ms=s.ms, rows=s.rows) for s in st[:iters]]
 
     def build_rknng(self):
         self._ck(self.L.hrnn_index_build_rknng(self.h))

@@ -188,6 +188,7 @@ class HRNN : public AlgorithmInterface<dist_t> {
         sync_device_base();
         if (build_batch_permille_ > 0)
             dev_check(hrnn_dev_set_tuning(dev_, "hnsw_batch_permille", build_batch_permille_));
+        if (build_seed_exp_ >= 0) dev_check(hrnn_dev_set_tuning(dev_, "hnsw_seed_exp", build_seed_exp_));
         uint32_t entry = 0;
         int maxl = 0;
         hrnn_hnsw_build_stats st{};
@@ -220,6 +221,8 @@ class HRNN : public AlgorithmInterface<dist_t> {
     // Batch size of addPointsDevice as a fraction of the graph, per mille
     // (0: the device default).
     void setBuildBatchPermille(int permille) { build_batch_permille_ = permille; }
+    // Expansions of each new point's seed completion (-1: the device default).
+    void setBuildSeedExpansions(int e) { build_seed_exp_ = e; }
 
     // ---- (a) NNDescent on the GPU -----------------------------------------
     // reference: refineKNNG (hrnn.h:1725-1862)
@@ -865,6 +868,7 @@ class HRNN : public AlgorithmInterface<dist_t> {
     std::vector<hrnn_nnd_iter_stats> last_nnd_stats_;
     hrnn_hnsw_build_stats last_build_stats_{};
     int build_batch_permille_{0};
+    int build_seed_exp_{-1};
 
     // ---- device state ---------------------------------------------------------------
     mutable hrnn_dev_t dev_{nullptr};

@@ -123,6 +123,8 @@ typedef struct hrnn_hnsw_build_stats {
     uint64_t evaluations;     /* distances evaluated by the beams */
     uint64_t dropped_links;   /* links back beyond 64 per node, batch and layer (not made) */
     double ms;                /* wall time of the call */
+    uint64_t seed_expansions; /* expansions of the seed completion */
+    uint64_t seed_evaluations;
 } hrnn_hnsw_build_stats;
 
 /* Build the HNSW graph over all n resident base rows, inserted in id order
@@ -131,10 +133,14 @@ typedef struct hrnn_hnsw_build_stats {
  * self skipped, the first K; every entry new).  M links per new node per
  * layer, lists of 2M at layer 0 and M above, beams of width ef_construction
  * (<= 512), 2 <= M <= 32.  Points go in batches of "hnsw_batch_permille"
- * (hrnn_dev_set_tuning, default 20) of the graph's size; each batch searches
+ * (hrnn_dev_set_tuning, default 50) of the graph's size; each batch searches
  * the graph as it stood when the batch began, then every node's new links
  * back are merged into its list at once (appended, or the list pruned by the
- * diversity rule over entries and new links).  The graph stays resident for
+ * diversity rule over entries and new links); each new point's seed list
+ * then continues its layer-0 beam over the merged graph for
+ * "hnsw_seed_exp" expansions (default 32; 0: none), admitting only lower
+ * ids, so it sees the earlier points of its own batch as sequential
+ * insertion would.  The graph stays resident for
  * phase A (hrnn_dev_proxies) and the KNN lists for hrnn_dev_nndescent;
  * *entry and *maxlevel receive the entry point and top layer.  Replaces
  * HRNN::addPoint (include/hnswlib/hrnn.h:1378-1517) with searchBaseLayer
@@ -153,7 +159,8 @@ typedef struct hrnn_nnd_iter_stats {
     uint64_t updates;        /* entries inserted into K-lists (both directions) */
     uint64_t proposals;      /* proposals that passed the threshold filter */
     uint64_t dropped;        /* proposals beyond the proposal buffer (0 unless it overflows) */
-    double ms;               /* wall time of the iteration */
+    double ms;               /* device time of the iteration (HIP events around it) */
+    uint64_t rows;           /* sample rows staged by the joins (sum over nodes of new + old samples) */
 } hrnn_nnd_iter_stats;
 
 /* num_iters rounds of UPDATE (sample S new / S old, reverse samples capped at

@@ -48,11 +48,12 @@ int hrnn_index_add_points(hrnn_index_t ix, const float *x, uint64_t n, const uin
 /* refineKNNG on the GPU; stats may be NULL or hold num_iters records. */
 /* HNSW construction with KNN seeding on the GPU for an empty index
  * (HRNN::addPointsDevice): rows x[0 .. n) become internal ids 0 .. n-1 with
- * labels[i] (i when labels is NULL); batch_permille (0: the default of 20)
- * sizes the insertion batches relative to the graph; stats (may be NULL)
+ * labels[i] (i when labels is NULL); batch_permille (0: the default of 50)
+ * sizes the insertion batches relative to the graph, seed_expansions (< 0:
+ * the default of 32) the seed completion of each batch; stats (may be NULL)
  * receives the build's counters. */
 int hrnn_index_add_points_gpu(hrnn_index_t ix, const float *x, uint64_t n, const uint64_t *labels,
-                              int batch_permille, hrnn_hnsw_build_stats *stats);
+                              int batch_permille, int seed_expansions, hrnn_hnsw_build_stats *stats);
 int hrnn_index_refine(hrnn_index_t ix, int num_iters, int sample_size, hrnn_nnd_iter_stats *stats);
 int hrnn_index_build_rknng(hrnn_index_t ix);
 int hrnn_index_build_rknng_mutable(hrnn_index_t ix);

@@ -9,7 +9,9 @@ differ; what must hold:
   counts within 2M (layer 0) and M (above), upper-layer links only to nodes
   of that level, the levels those of sequential insertion (same generator);
 * seeds: each list ascending, duplicate- and self-free, every distance the
-  exact squared L2 of its pair, at most K (and at most ef_c) entries;
+  exact squared L2 of its pair, at most K (and at most ef_c) entries, only
+  ids below the point's own (the points sequential insertion would have
+  seen);
 * quality, against the host build of the same rows: proxy recall of phase A
   and the seed lists' recall against the exact KNN within half a point or
   better, and after refinement and the reverse CSR, RkNN recall at every m
@@ -47,7 +49,7 @@ def _exact_seeds(x, ix, K, efc):
     for i in range(0, x.shape[0], 7):
         c = int(cnt[i])
         row = ids[i, :c]
-        assert i not in row and len(set(row.tolist())) == c, i
+        assert i not in row and len(set(row.tolist())) == c and np.all(row < i), i
         assert np.all(np.diff(d[i, :c]) >= 0), i
         dd = ((x[row].astype(np.int64) - x[i].astype(np.int64)) ** 2).sum(1)
         assert np.array_equal(dd.astype(np.float32), d[i, :c]), i
@@ -123,10 +125,10 @@ def test_gpu_build_quality_vs_host_build(n, centres, sigma, K, efc, ms, Kp):
     for ix in (gpu, host):
         ix.refine(10, 50)
         ix.build_rknng()
-    # the seeds see only the points before their batch: a little below the
-    # host's before refinement; after it the KNN graphs agree
+    # the seeds (completed over the merged graph after each batch) and, after
+    # refinement, the KNN graphs agree with the host build's
     for kk in (10, K):
-        assert seed[kk][0] >= seed[kk][1] - 0.02, (kk, seed[kk])
+        assert seed[kk][0] >= seed[kk][1] - 0.005, (kk, seed[kk])
         g_ids, _, g_cnt = gpu.get_knng()
         h_ids, _, h_cnt = host.get_knng()
         h_lab = np.zeros_like(h_ids)
