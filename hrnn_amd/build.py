@@ -32,7 +32,9 @@ HIP_SOURCES = ["api.hip", "csr.hip", "search.hip", "bruteforce.hip", "nndescent.
 HOST_SOURCES = ["synth.cpp"]
 
 HIP_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=fast",
+    # fused multiply-add across statements, except where a function turns it
+    # off (the FP64 distance must round every operation like the oracle's)
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=fast-honor-pragmas",
     # packed FP32 instructions stay off (the compiler would otherwise form
     # v_pk_{add,mul,fma}_f32 from ordinary float code)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

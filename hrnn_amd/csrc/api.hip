@@ -11,7 +11,7 @@ using namespace hrnn;
 
 hrnn_dev::~hrnn_dev() {
     (void)hipSetDevice(device);
-    DevBuf *bufs[] = {&x, &xb, &xh, &xu8, &norms, &knn_ids, &knn_d, &knn_cnt, &knn_new, &rk_col, &csr_off,
+    DevBuf *bufs[] = {&x, &xb, &xh, &xu8, &norms, &bf_norm16, &knn_ids, &knn_d, &knn_cnt, &knn_new, &rk_col, &csr_off,
                       &csr_ent, &search_mark, &search_cand, &search_mark2, &search_cand2, &fetch_off, &fetch_ids, &l0, &up_off, &up_pool, &beam_vis, &levels_dev, &hb_prop, &hb_inc_cnt, &hb_inc,
                       &hb_touched, &hb_ctr,
                       &s0, &s1, &s2, &s3, &s4, &s5, &s6, &s7, &s8, &nnd_aux, &counters};
@@ -284,6 +284,7 @@ static void set_base_common(hrnn_dev_t h, const float *x, uint64_t n, uint32_t d
     else
         h2d(h, h->x.p, x, n * d * sizeof(float));
     launch_base_mirrors(h, 0, n);
+    h->base_version++;
     h->K = 0;
     h->csr_valid = false;
     h->hnsw_valid = false;
@@ -332,6 +333,7 @@ int hrnn_dev_append_base(hrnn_dev_t h, const float *x, uint64_t n_add) {
                             hipMemcpyHostToDevice, h->stream));
         launch_base_mirrors(h, h->n, n_add);
         h->n = n_new;
+        h->base_version++;
         h->csr_valid = false;
         h->hnsw_valid = false;
         h->beam_vis.release();

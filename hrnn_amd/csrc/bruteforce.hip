@@ -58,6 +58,104 @@ __device__ float l2_ref16_thread(const float *__restrict__ a, const float *__res
     return r;
 }
 
+// FP64 squared L2 in the oracle's order: float64 differences, squares and
+// sums rounded one by one, left to right, never fused.
+__device__ double l2_seq_f64(const float *__restrict__ a, const float *__restrict__ b, uint32_t d) {
+#pragma clang fp contract(off)
+    double r = 0.0;
+    if ((d & 3u) == 0) {
+        const float4 *a4 = reinterpret_cast<const float4 *>(a);
+        const float4 *b4 = reinterpret_cast<const float4 *>(b);
+        for (uint32_t e = 0; e < d / 4; e++) {
+            const float4 u = a4[e], v = b4[e];
+            double t = (double)u.x - (double)v.x;
+            r += t * t;
+            t = (double)u.y - (double)v.y;
+            r += t * t;
+            t = (double)u.z - (double)v.z;
+            r += t * t;
+            t = (double)u.w - (double)v.w;
+            r += t * t;
+        }
+    } else {
+        for (uint32_t e = 0; e < d; e++) {
+            double t = (double)a[e] - (double)b[e];
+            r += t * t;
+        }
+    }
+    return r;
+}
+
+// Candidate-pass formats
+enum : int { kPassF32 = 0, kPassBF16 = 1, kPassF16 = 2 };
+
+__device__ __forceinline__ float round_to(float v, int pass) {
+    if (pass == kPassBF16) return __bfloat162float(__float2bfloat16(v));
+    if (pass == kPassF16) return __half2float(__float2half(v));
+    return v;
+}
+
+// Per-format rounding statistics of the base for the certified top-C:
+// squared norms of the BF16- and FP16-rounded rows, and over all rows the
+// largest of each and the largest squared rounding error sum_e (r(x_e) -
+// x_e)^2 (mx[0..3]: BF16 norm, FP16 norm, BF16 error, FP16 error, as float
+// bits; non-negative floats order as unsigned integers).  r(x) - x is exact
+// in FP32 (Sterbenz); an FP16 overflow gives +inf and so no certificate.
+__global__ __launch_bounds__(256) void k_bf_prep(const float *__restrict__ x, uint64_t n, uint32_t d,
+                                                 float *__restrict__ nb, float *__restrict__ nh,
+                                                 unsigned *__restrict__ mx) {
+    __shared__ unsigned s_mx[4][4];
+    const uint32_t w = threadIdx.x / 64, lane = threadIdx.x & 63u;
+    unsigned m[4] = {0u, 0u, 0u, 0u};
+    for (uint64_t r = (uint64_t)blockIdx.x * 4 + w; r < n; r += (uint64_t)gridDim.x * 4) {
+        float sb = 0.f, sh = 0.f, eb = 0.f, eh = 0.f;
+        for (uint32_t e = lane; e < d; e += 64) {
+            const float v = x[r * d + e];
+            const float b = round_to(v, kPassBF16), hv = round_to(v, kPassF16);
+            sb = __builtin_fmaf(b, b, sb);
+            sh = __builtin_fmaf(hv, hv, sh);
+            eb = __builtin_fmaf(b - v, b - v, eb);
+            eh = __builtin_fmaf(hv - v, hv - v, eh);
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            sb += __shfl_xor(sb, off, 64);
+            sh += __shfl_xor(sh, off, 64);
+            eb += __shfl_xor(eb, off, 64);
+            eh += __shfl_xor(eh, off, 64);
+        }
+        if (lane == 0) {
+            nb[r] = sb;
+            nh[r] = sh;
+        }
+        m[0] = max(m[0], __float_as_uint(sb));
+        m[1] = max(m[1], __float_as_uint(sh));
+        m[2] = max(m[2], __float_as_uint(eb));
+        m[3] = max(m[3], __float_as_uint(eh));
+    }
+    if (lane == 0)
+        for (int i = 0; i < 4; i++) s_mx[w][i] = m[i];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        unsigned v = 0u;
+        for (int i = 0; i < 4; i++) v = max(v, s_mx[i][threadIdx.x]);
+        atomicMax(&mx[threadIdx.x], v);
+    }
+}
+
+// Rounded norms and maxima for the current base (recomputed when it changed).
+static void ensure_bf_prep(hrnn_dev_t h) {
+    if (h->bf_prep_version == h->base_version && h->bf_norm16.p) return;
+    h->bf_norm16.reserve(h->n * 8 + 64);
+    float *nb = h->bf_norm16.as<float>();
+    unsigned *mx = reinterpret_cast<unsigned *>(nb + 2 * h->n);
+    HIPC(hipMemsetAsync(mx, 0, 16, h->stream));
+    unsigned grid = (unsigned)std::min<uint64_t>(ceil_div(h->n, 4), 4096);
+    hipLaunchKernelGGL(k_bf_prep, dim3(grid), dim3(256), 0, h->stream, h->x.as<float>(), h->n, h->d,
+                       nb, nb + h->n, mx);
+    HIPC(hipGetLastError());
+    h->bf_prep_version = h->base_version;
+}
+
 template <int C, int R = kRB>
 struct TopState {
     float top_d[R][C];
@@ -146,14 +244,24 @@ __device__ __forceinline__ void top_offer(TopState<C, R> &s, int r, float dv, ui
     }
 }
 
+// Writes each row's candidates (sorted by the pass's own distance, then id),
+// their count, the row's final threshold (its C-th distance; FLT_MAX while the
+// list is not full) and, when kraw > 0, its kraw-th distance.  Every column
+// not kept has a pass distance >= the final threshold: it was rejected
+// against a threshold at least as large, or displaced from the list.
 template <int C, int R>
-__device__ void top_store(TopState<C, R> &s, uint64_t row0, uint32_t nq, uint32_t *cand_i, uint32_t *cand_n) {
+__device__ void top_store(TopState<C, R> &s, uint64_t row0, uint32_t nq, uint32_t *cand_i, uint32_t *cand_n,
+                          float *cand_thr, float *raw_kth, uint32_t kraw) {
     for (int t = threadIdx.x; t < R * C; t += blockDim.x) {
         int r = t / C, j = t % C;
         uint64_t row = row0 + r;
         if (row >= nq) continue;
         cand_i[row * C + j] = (uint32_t)j < s.top_n[r] ? s.top_i[r][j] : kInvalidId;
-        if (j == 0) cand_n[row] = s.top_n[r];
+        if (j == 0) {
+            cand_n[row] = s.top_n[r];
+            cand_thr[row] = s.thr[r];
+            if (kraw) raw_kth[row] = kraw <= s.top_n[r] ? s.top_d[r][kraw - 1] : FLT_MAX;
+        }
     }
 }
 
@@ -182,7 +290,8 @@ template <class T, int KS, int C, int MODE, int FR>
 __global__ __launch_bounds__(256) void k_bf_mfma(
     const T *__restrict__ qa, const float *__restrict__ qn, uint32_t nq,
     const T *__restrict__ xb, const float *__restrict__ xn, uint64_t n, uint32_t d,
-    uint32_t *__restrict__ cand_i, uint32_t *__restrict__ cand_n,
+    uint32_t *__restrict__ cand_i, uint32_t *__restrict__ cand_n, float *__restrict__ cand_thr,
+    float *__restrict__ raw_kth, uint32_t kraw,
     const float *__restrict__ kth, uint32_t *__restrict__ hit_cnt, uint32_t *__restrict__ hit_i,
     float *__restrict__ hit_d, uint32_t cap) {
     typedef typename Mfma<T>::vec vec;
@@ -371,20 +480,26 @@ __global__ __launch_bounds__(256) void k_bf_mfma(
     }
     if (MODE == 0) {
         __syncthreads();   // every wave's rows are merged
-        top_store(st, row0, nq, cand_i, cand_n);
+        top_store(st, row0, nq, cand_i, cand_n, cand_thr, raw_kth, kraw);
     }
 }
 
 // FP32 VALU candidate pass: 4x4 micro-tile per lane, direct differences.
-// mode 0: top-C; mode 1: RkNN hits, boundary cases re-decided with the
-// reference order.
+// mode 0: top-C; mode 1: RkNN hits.  In mode 1 a pair whose distance lies
+// within band_rel * max(d2, r_k) of r_k (a bound on the FP32 rounding of
+// both, any summation order) is re-decided: with kth64, by its FP64
+// distance against the FP64 radius (chk[0] counts these pairs, chk[1] those
+// whose FP64 decision differs from the reference-order FP32 one); without,
+// by its reference-order FP32 distance.
 constexpr int kFCH = 16;   // dims per chunk
 template <int C, int MODE>
 __global__ __launch_bounds__(256) void k_bf_f32(
     const float *__restrict__ qa, uint32_t nq, const float *__restrict__ x, uint64_t n, uint32_t d,
-    uint32_t *__restrict__ cand_i, uint32_t *__restrict__ cand_n, const float *__restrict__ kth,
+    uint32_t *__restrict__ cand_i, uint32_t *__restrict__ cand_n, float *__restrict__ cand_thr,
+    float *__restrict__ raw_kth, uint32_t kraw, const float *__restrict__ kth,
     uint32_t *__restrict__ hit_cnt, uint32_t *__restrict__ hit_i, float *__restrict__ hit_d,
-    uint32_t cap) {
+    uint32_t cap, float band_rel, const double *__restrict__ kth64,
+    unsigned long long *__restrict__ chk) {
     __shared__ TopState<C> st;
     __shared__ float as_[kFCH][kRB + 4];
     __shared__ float bs_[kFCH][kCT + 4];
@@ -437,10 +552,20 @@ __global__ __launch_bounds__(256) void k_bf_f32(
                         if (MODE == 0) {
                             top_offer(st, rl, dv, (uint32_t)col);
                         } else {
-                            float kv = kth[col];
-                            bool near = fabsf(dv - kv) <= 1e-5f * fmaxf(kv, 1.f);
-                            if (near) dv = l2_ref16_thread(qa + row * d, x + col * d, d);
-                            if (dv <= kv) {
+                            const float kv = kth[col];
+                            const bool near = fabsf(dv - kv) <= band_rel * fmaxf(dv, kv) + 1e-30f;
+                            bool in = dv <= kv;
+                            if (near) {
+                                dv = l2_ref16_thread(qa + row * d, x + col * d, d);
+                                in = dv <= kv;
+                                if (kth64) {
+                                    const bool in64 = l2_seq_f64(qa + row * d, x + col * d, d) <= kth64[col];
+                                    atomicAdd(&chk[0], 1ull);
+                                    if (in64 != in) atomicAdd(&chk[1], 1ull);
+                                    in = in64;
+                                }
+                            }
+                            if (in) {
                                 if (!near) dv = l2_ref16_thread(qa + row * d, x + col * d, d);
                                 uint32_t s = atomicAdd(&hit_cnt[row], 1u);
                                 if (s < cap) {
@@ -454,19 +579,49 @@ __global__ __launch_bounds__(256) void k_bf_f32(
             if (MODE == 0) top_merge(st);
         }
     }
-    if (MODE == 0) top_store(st, row0, nq, cand_i, cand_n);
+    if (MODE == 0) top_store(st, row0, nq, cand_i, cand_n, cand_thr, raw_kth, kraw);
 }
 
-// Re-rank: one wave per row recomputes its C candidates in the reference's
-// FP32 order, sorts (dist, id) and writes the first kk.
+// Certificate of a candidate pass (see k_rerank_cert).  Unit roundoff 2^-23
+// throughout, so that FP32 sums in any order and with round-to-nearest or
+// truncation are covered.
+struct CertParam {
+    int pass = kPassF32;          // format of the candidate pass
+    int exact_base = 0;           // every base value an integer in [0, 255] and d <= 128
+    const unsigned *mx = nullptr; // base maxima of k_bf_prep (BF16 / FP16 passes)
+    double g = 0.0;               // gamma_{d+2} = (d + 2) u / (1 - (d + 2) u)
+};
+
+// Re-rank and certificate: one wave per row.
+// 1. The C candidates' FP32 distances in the reference's order, sorted by
+//    (dist, id): ids/dist get the first kk, r32 = the kk-th.
+// 2. Their FP64 distances (l2_seq_f64): r64 = the kk-th smallest.
+// 3. Certificate: every column left out of the candidates has, in the
+//    pass's format, a distance >= thr (the row's C-th).  A lower bound L on
+//    the exact distance of every such column follows:
+//    * FP32 pass: d~ <= D^2 (1 + g), so D^2 >= thr / (1 + g);
+//    * BF16/FP16 Gram pass with rounded rows q~, x~ and their norms:
+//      |d~ - ||q~ - x~||^2| <= A = (g + 2u(1 + g)) (|q~| + X)^2, X the
+//      largest rounded base norm; ||q - x|| >= ||q~ - x~|| - e_q - E, e_q =
+//      ||q~ - q|| and E the largest ||x~ - x||; so
+//      D >= sqrt(max(thr - A, 0)) - e_q - E.
+//    The row is certified when L^2 (1 - g) > r32 and L^2 (1 - 1e-12) > r64:
+//    no column outside the candidates can then be among the first kk in
+//    FP32 (reference order, any tie order) or in FP64, so ids, dist, r32 and
+//    r64 equal those of an exhaustive re-rank.  Also certified: rows with
+//    fewer than C candidates (every column is one), and rows of integer
+//    queries in [0, 255] over an exact base (every distance exact).
 template <int C>
-__global__ __launch_bounds__(256) void k_rerank(const float *__restrict__ qa, uint32_t nq,
-                                                const float *__restrict__ x, uint32_t d,
-                                                const uint32_t *__restrict__ cand_i,
-                                                uint32_t kk, uint32_t *__restrict__ ids,
-                                                float *__restrict__ dist) {
+__global__ __launch_bounds__(256) void k_rerank_cert(
+    const float *__restrict__ qa, uint32_t nq, const float *__restrict__ x, uint32_t d,
+    const uint32_t *__restrict__ cand_i, const uint32_t *__restrict__ cand_n,
+    const float *__restrict__ cand_thr, uint32_t kk, CertParam cp, uint32_t *__restrict__ ids,
+    float *__restrict__ dist, double *__restrict__ r64, uint8_t *__restrict__ cert,
+    uint32_t *__restrict__ raw_ids) {
     __shared__ float sd[4][C];
     __shared__ uint32_t si[4][C];
+    __shared__ double s64[4][C];
+    __shared__ double s_r64[4];
     const uint32_t w = threadIdx.x / 64, lane = threadIdx.x & 63u;
     uint64_t row = (uint64_t)blockIdx.x * 4 + w;
     const bool valid = row < nq;
@@ -481,6 +636,40 @@ __global__ __launch_bounds__(256) void k_rerank(const float *__restrict__ qa, ui
         if ((lane & 15u) == 0) {
             sd[w][c] = dv;
             si[w][c] = id;
+        }
+    }
+    // the query row's rounding statistics (BF16/FP16 passes) and whether it
+    // is made of integers in [0, 255]
+    double qq = 0.0, ee = 0.0;
+    bool q_u8 = true;
+    for (uint32_t e = lane; valid && e < d; e += 64) {
+        const float v = qv[e];
+        const float r = round_to(v, cp.pass);
+        qq += (double)r * (double)r;
+        ee += ((double)r - (double)v) * ((double)r - (double)v);
+        q_u8 &= v == rintf(v) && v >= 0.f && v <= 255.f;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        qq += __shfl_xor(qq, off, 64);
+        ee += __shfl_xor(ee, off, 64);
+    }
+    q_u8 = __all(q_u8);
+    __syncthreads();
+    if (valid && lane < C) {
+        const uint32_t id = si[w][lane];
+        s64[w][lane] = id != kInvalidId ? l2_seq_f64(qv, x + (uint64_t)id * d, d) : DBL_MAX;
+    }
+    __syncthreads();
+    if (valid && lane < C) {   // the kk-th smallest FP64 distance (equal values counted apart)
+        const double v = s64[w][lane];
+        uint32_t rank = 0;
+        for (int j = 0; j < C; j++) {
+            const double u = s64[w][j];
+            rank += (u < v || (u == v && j < (int)lane)) ? 1u : 0u;
+        }
+        if (rank == kk - 1) {
+            s_r64[w] = v;
+            if (r64) r64[row] = v;
         }
     }
     __syncthreads();
@@ -502,33 +691,97 @@ __global__ __launch_bounds__(256) void k_rerank(const float *__restrict__ qa, ui
             ids[row * kk + j] = si[w][j];
             dist[row * kk + j] = sd[w][j];
         }
+        if (raw_ids)
+            for (uint32_t j = 0; j < kk; j++) raw_ids[row * kk + j] = cand_i[row * C + j];
+    }
+    if (valid && lane == 0 && cert) {
+        const uint32_t cn = cand_n[row];
+        bool ok = cn < (uint32_t)C || (cp.exact_base && q_u8);
+        if (!ok) {
+            const double r64v = s_r64[w];
+            const double r32 = sd[w][kk - 1];
+            const double thr = cand_thr[row];
+            const double g = cp.g, u = 0x1p-23;
+            double lb2;
+            if (cp.pass == kPassF32) {
+                lb2 = thr / (1.0 + g);
+            } else {
+                const int o = cp.pass == kPassBF16 ? 0 : 1;
+                const double X = sqrt((double)__uint_as_float(cp.mx[o]) * (1.0 + 2.0 * g));
+                const double E = sqrt((double)__uint_as_float(cp.mx[2 + o]) * (1.0 + 2.0 * g)) * (1.0 + 1e-9);
+                const double qn = sqrt(qq) * (1.0 + 1e-9), eq = sqrt(ee) * (1.0 + 1e-9);
+                const double A = (g + 2.0 * u * (1.0 + g)) * (qn + X) * (qn + X) * (1.0 + 1e-9) + 1e-30;
+                const double t = sqrt(fmax(thr - A, 0.0)) - eq - E;
+                lb2 = t > 0.0 ? t * t : 0.0;
+            }
+            ok = lb2 * (1.0 - g) > r32 && lb2 * (1.0 - 1e-12) > r64v;
+        }
+        cert[row] = ok ? 1 : 0;
     }
 }
 
-// FP32 -> BF16/FP16 copies and norms of external queries.
+// FP32 -> BF16/FP16 copies of external queries and the squared norms of the
+// rounded rows (the Gram form then measures ||q~ - x~||^2).
 __global__ void k_query_prep(const float *__restrict__ q, uint64_t nq, uint32_t d,
                              __hip_bfloat16 *__restrict__ qb, __half *__restrict__ qh,
-                             float *__restrict__ qn) {
+                             float *__restrict__ qnb, float *__restrict__ qnh) {
     uint64_t r = (uint64_t)blockIdx.x * 4 + threadIdx.x / 64;
     if (r >= nq) return;
     uint32_t lane = threadIdx.x & 63u;
-    float acc = 0.f;
+    float ab = 0.f, ah = 0.f;
     for (uint32_t e = lane; e < d; e += 64) {
         float v = q[r * d + e];
-        qb[r * d + e] = __float2bfloat16(v);
-        qh[r * d + e] = __float2half(v);
-        acc = __builtin_fmaf(v, v, acc);
+        const __hip_bfloat16 b = __float2bfloat16(v);
+        const __half hv = __float2half(v);
+        qb[r * d + e] = b;
+        qh[r * d + e] = hv;
+        const float bf = __bfloat162float(b), hf = __half2float(hv);
+        ab = __builtin_fmaf(bf, bf, ab);
+        ah = __builtin_fmaf(hf, hf, ah);
     }
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if (lane == 0) qn[r] = acc;
+    for (int off = 32; off > 0; off >>= 1) {
+        ab += __shfl_down(ab, off, 64);
+        ah += __shfl_down(ah, off, 64);
+    }
+    if (lane == 0) {
+        qnb[r] = ab;
+        qnh[r] = ah;
+    }
+}
+
+// dst row t = src row idx[t] (rows of `words` 32-bit words)
+__global__ void k_gather_rows(const uint32_t *__restrict__ src, const uint32_t *__restrict__ idx,
+                              uint32_t m, uint32_t words, uint32_t *__restrict__ dst) {
+    const uint64_t t = blockIdx.x;
+    if (t >= m) return;
+    const uint64_t s = idx[t];
+    for (uint32_t e = threadIdx.x; e < words; e += blockDim.x) dst[t * words + e] = src[s * words + e];
+}
+
+// The redone rows' results back into the full arrays.
+__global__ void k_scatter_topk(const uint32_t *__restrict__ idx, uint32_t m, uint32_t kk,
+                               const uint32_t *__restrict__ sids, const float *__restrict__ sdist,
+                               const double *__restrict__ sr64, const uint8_t *__restrict__ scert,
+                               uint32_t *__restrict__ ids, float *__restrict__ dist,
+                               double *__restrict__ r64, uint8_t *__restrict__ cert) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)m * kk) return;
+    const uint64_t i = t / kk, j = t % kk, row = idx[i];
+    ids[row * kk + j] = sids[t];
+    dist[row * kk + j] = sdist[t];
+    if (j == 0) {
+        r64[row] = sr64[i];
+        cert[row] = scert[i];
+    }
 }
 
 // ---- launchers -------------------------------------------------------------
 
 template <class T, int C, int MODE>
 static void launch_mfma(hrnn_dev_t h, const T *qa, const float *qn, uint32_t nq, const T *xb,
-                        uint32_t *ci, uint32_t *cn, const float *kth, uint32_t *hc, uint32_t *hi,
-                        float *hd, uint32_t cap) {
+                        const float *xn, uint32_t *ci, uint32_t *cn, float *cthr, float *raw_kth,
+                        uint32_t kraw, const float *kth, uint32_t *hc, uint32_t *hi, float *hd,
+                        uint32_t cap) {
     // One 16-row fragment per wave (64 rows per block): the top state's LDS
     // then allows three blocks per CU, which measured faster on the 1M x 1M
     // r_k pass (1.05 s) than two fragments (128 rows per block, half the
@@ -540,7 +793,7 @@ static void launch_mfma(hrnn_dev_t h, const T *qa, const float *qn, uint32_t nq,
 #define L(ks)                                                                                     \
     case ks:                                                                                      \
         hipLaunchKernelGGL((k_bf_mfma<T, ks, C, MODE, FR>), dim3(grid), dim3(256), 0, h->stream, qa, qn, \
-                           nq, xb, h->norms.as<float>(), h->n, h->d, ci, cn, kth, hc, hi, hd, cap);  \
+                           nq, xb, xn, h->n, h->d, ci, cn, cthr, raw_kth, kraw, kth, hc, hi, hd, cap); \
         break;
     switch (KS) {
         L(1) L(2) L(3) L(4) L(5) L(6) L(8) L(12) L(16) L(24) L(30) L(32)
@@ -562,84 +815,201 @@ static bool mfma_ok(hrnn_dev_t h) {
     }
 }
 
+// One candidate pass (format `pass`, top-C) over nq rows, then the re-rank
+// and certificate.  q16/qn16: the rows rounded to the pass's format and their
+// squared norms (BF16/FP16 passes).
 template <int C>
-static void topk_c(hrnn_dev_t h, const float *q32, const void *q16, const float *qn, uint32_t nq,
-                   uint32_t kk, int prec, uint32_t *ids_dev, float *d_dev) {
-    uint64_t nqC = (uint64_t)nq * C;
-    h->s6.reserve(nqC * 4 + (uint64_t)nq * 4 + 64);
+static void bf_pass(hrnn_dev_t h, int pass, const float *q32, const void *q16, const float *qn16,
+                    uint32_t nq, uint32_t kk, const CertParam &cp, uint32_t *ids, float *dist,
+                    double *r64, uint8_t *cert, float *raw_kth, uint32_t *raw_ids) {
+    const uint64_t nqC = (uint64_t)nq * C;
+    h->s6.reserve(nqC * 4 + (uint64_t)nq * 8 + 64);
     uint32_t *ci = h->s6.as<uint32_t>();
     uint32_t *cn = ci + nqC;
-    unsigned grid = ceil_div(nq, kRB);
-    if (prec == HRNN_PREC_BF16 && mfma_ok(h)) {
-        launch_mfma<__hip_bfloat16, C, 0>(h, (const __hip_bfloat16 *)q16, qn, nq,
-                                           h->xb.as<__hip_bfloat16>(), ci, cn, nullptr, nullptr,
-                                           nullptr, nullptr, 0);
-    } else if (prec == HRNN_PREC_F16 && mfma_ok(h)) {
-        launch_mfma<__half, C, 0>(h, (const __half *)q16, qn, nq, h->xh.as<__half>(), ci, cn,
-                                   nullptr, nullptr, nullptr, nullptr, 0);
+    float *cthr = reinterpret_cast<float *>(cn + nq);
+    const uint32_t kraw = raw_kth ? kk : 0;
+    const float *nb = h->bf_norm16.as<float>();
+    if (pass == kPassBF16) {
+        launch_mfma<__hip_bfloat16, C, 0>(h, (const __hip_bfloat16 *)q16, qn16, nq, h->xb.as<__hip_bfloat16>(),
+                                           nb, ci, cn, cthr, raw_kth, kraw, nullptr, nullptr, nullptr,
+                                           nullptr, 0);
+    } else if (pass == kPassF16) {
+        launch_mfma<__half, C, 0>(h, (const __half *)q16, qn16, nq, h->xh.as<__half>(), nb + h->n, ci, cn,
+                                   cthr, raw_kth, kraw, nullptr, nullptr, nullptr, nullptr, 0);
     } else {
-        hipLaunchKernelGGL((k_bf_f32<C, 0>), dim3(grid), dim3(256), 0, h->stream, q32, nq,
-                           h->x.as<float>(), h->n, h->d, ci, cn, nullptr, nullptr, nullptr, nullptr, 0u);
+        hipLaunchKernelGGL((k_bf_f32<C, 0>), dim3(ceil_div(nq, kRB)), dim3(256), 0, h->stream, q32, nq,
+                           h->x.as<float>(), h->n, h->d, ci, cn, cthr, raw_kth, kraw, nullptr, nullptr,
+                           nullptr, nullptr, 0u, 0.f, nullptr, nullptr);
         HIPC(hipGetLastError());
     }
-    hipLaunchKernelGGL((k_rerank<C>), dim3(ceil_div(nq, 4)), dim3(256), 0, h->stream, q32, nq,
-                       h->x.as<float>(), h->d, ci, kk, ids_dev, d_dev);
+    hipLaunchKernelGGL((k_rerank_cert<C>), dim3(ceil_div(nq, 4)), dim3(256), 0, h->stream, q32, nq,
+                       h->x.as<float>(), h->d, ci, cn, cthr, kk, cp, ids, dist, r64, cert, raw_ids);
     HIPC(hipGetLastError());
 }
 
-// Top-kk of nq query rows (or of the base itself when self_base) over the base.
+static double gamma_u23(uint32_t m) {
+    const double mu = (double)m * 0x1p-23;
+    return mu / (1.0 - mu);
+}
+
+// Top-kk of nq query rows (or of the base itself when self_base) over the
+// base, exact: after the first candidate pass (prec's format, C = 32 when
+// 2 kk <= 32, else 64) every row must carry a certificate; the rows without
+// one are redone with a 64-wide candidate list in the same format, then, if
+// still uncertified, with the FP32 pass (64 wide).  Rows no pass certifies
+// keep the last pass's results and are counted.
 void bf_topk_base(hrnn_dev_t h, const float *q_dev, uint32_t nq, bool self_base, uint32_t kk,
-                  int prec, uint32_t *ids_dev, float *d_dev) {
+                  int prec, uint32_t *ids_dev, float *d_dev, const BfExtra *ex) {
     REQUIRE(kk >= 1 && kk <= 64 && kk <= h->n, HRNN_E_ARG, "bruteforce: k+1 must be in [1, 64] and <= n");
+    if (!nq) return;
+    const bool mf = mfma_ok(h);
+    const int pass0 = (prec == HRNN_PREC_BF16 && mf) ? kPassBF16 : (prec == HRNN_PREC_F16 && mf) ? kPassF16 : kPassF32;
+    if (pass0 != kPassF32) ensure_bf_prep(h);
     const float *q32;
-    const void *q16;
-    const float *qn;
+    const void *q16 = nullptr;
+    const float *qn16 = nullptr;
     if (self_base) {
         q32 = h->x.as<float>();
-        q16 = (prec == HRNN_PREC_F16) ? (const void *)h->xh.p : (const void *)h->xb.p;
-        qn = h->norms.as<float>();
+        if (pass0 != kPassF32) {
+            q16 = pass0 == kPassF16 ? (const void *)h->xh.p : (const void *)h->xb.p;
+            qn16 = h->bf_norm16.as<float>() + (pass0 == kPassF16 ? h->n : 0);
+        }
     } else {
-        h->s7.reserve((uint64_t)nq * h->d * 4 + (uint64_t)nq * 4 + 256);
+        h->s7.reserve((uint64_t)nq * h->d * 4 + (uint64_t)nq * 8 + 256);
         __hip_bfloat16 *qb = h->s7.as<__hip_bfloat16>();
         __half *qh = reinterpret_cast<__half *>(qb + (uint64_t)nq * h->d);
-        float *qnn = reinterpret_cast<float *>(qh + (uint64_t)nq * h->d);
+        float *qnb = reinterpret_cast<float *>(qh + (uint64_t)nq * h->d);
         hipLaunchKernelGGL(k_query_prep, dim3(ceil_div(nq, 4)), dim3(256), 0, h->stream, q_dev,
-                           (uint64_t)nq, h->d, qb, qh, qnn);
+                           (uint64_t)nq, h->d, qb, qh, qnb, qnb + nq);
         HIPC(hipGetLastError());
         q32 = q_dev;
-        q16 = (prec == HRNN_PREC_F16) ? (const void *)qh : (const void *)qb;
-        qn = qnn;
+        q16 = pass0 == kPassF16 ? (const void *)qh : (const void *)qb;
+        qn16 = pass0 == kPassF16 ? qnb + nq : qnb;
     }
-    uint32_t want = std::max<uint32_t>(2 * kk, 1);
-    if (want <= 32)
-        topk_c<32>(h, q32, q16, qn, nq, kk, prec, ids_dev, d_dev);
+    CertParam cp;
+    cp.exact_base = h->base_u8 && h->d <= 128;
+    if (pass0 != kPassF32) cp.mx = reinterpret_cast<const unsigned *>(h->bf_norm16.as<float>() + 2 * h->n);
+    cp.g = gamma_u23(h->d + 2);
+    DevBuf r64b, certb;
+    double *r64 = ex && ex->r64 ? ex->r64 : nullptr;
+    if (!r64) {
+        r64b.reserve((uint64_t)nq * 8);
+        r64 = r64b.as<double>();
+    }
+    certb.reserve(nq);
+    uint8_t *cert = certb.as<uint8_t>();
+    struct Events {
+        hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+        ~Events() {
+            for (auto &x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } evs;
+    hipEvent_t *ev = evs.e;
+    for (int i = 0; i < 3; i++) HIPC(hipEventCreate(&ev[i]));
+    HIPC(hipEventRecord(ev[0], h->stream));
+    const uint32_t C0 = 2 * kk <= 32 ? 32 : 64;
+    cp.pass = pass0;
+    float *raw_kth = ex ? ex->raw_kth : nullptr;
+    uint32_t *raw_ids = ex ? ex->raw_ids : nullptr;
+    if (C0 == 32)
+        bf_pass<32>(h, pass0, q32, q16, qn16, nq, kk, cp, ids_dev, d_dev, r64, cert, raw_kth, raw_ids);
     else
-        topk_c<64>(h, q32, q16, qn, nq, kk, prec, ids_dev, d_dev);
+        bf_pass<64>(h, pass0, q32, q16, qn16, nq, kk, cp, ids_dev, d_dev, r64, cert, raw_kth, raw_ids);
+    HIPC(hipEventRecord(ev[1], h->stream));
+    std::vector<uint8_t> ok(nq);
+    d2h(h, ok.data(), cert, nq);
+    std::vector<uint32_t> fail;
+    for (uint32_t i = 0; i < nq; i++)
+        if (!ok[i]) fail.push_back(i);
+    hrnn_bf_stats st{};
+    st.rows = nq;
+    st.certified = nq - fail.size();
+    // the rungs after the first pass: 64 wide in the same format, then FP32
+    std::vector<int> rungs;
+    if (pass0 != kPassF32 && C0 == 32) rungs.push_back(pass0);
+    if (pass0 != kPassF32 || C0 == 32) rungs.push_back(kPassF32);
+    for (int pass : rungs) {
+        if (fail.empty()) break;
+        const uint32_t m = (uint32_t)fail.size();
+        if (pass == kPassF32) st.redo_f32 += m;
+        else st.redo_wide += m;
+        DevBuf gi, g32, g16, gn, sids, sdist, sr64, scert;
+        gi.reserve((uint64_t)m * 4);
+        h2d(h, gi.p, fail.data(), (uint64_t)m * 4);
+        g32.reserve((uint64_t)m * h->d * 4);
+        hipLaunchKernelGGL(k_gather_rows, dim3(m), dim3(256), 0, h->stream, (const uint32_t *)q32,
+                           gi.as<uint32_t>(), m, h->d, g32.as<uint32_t>());
+        HIPC(hipGetLastError());
+        if (pass != kPassF32) {
+            g16.reserve((uint64_t)m * h->d * 2);
+            gn.reserve((uint64_t)m * 4);
+            hipLaunchKernelGGL(k_gather_rows, dim3(m), dim3(256), 0, h->stream, (const uint32_t *)q16,
+                               gi.as<uint32_t>(), m, h->d / 2, g16.as<uint32_t>());
+            hipLaunchKernelGGL(k_gather_rows, dim3(m), dim3(64), 0, h->stream, (const uint32_t *)qn16,
+                               gi.as<uint32_t>(), m, 1u, gn.as<uint32_t>());
+            HIPC(hipGetLastError());
+        }
+        sids.reserve((uint64_t)m * kk * 4);
+        sdist.reserve((uint64_t)m * kk * 4);
+        sr64.reserve((uint64_t)m * 8);
+        scert.reserve(m);
+        CertParam cq = cp;
+        cq.pass = pass;
+        bf_pass<64>(h, pass, g32.as<float>(), g16.p, gn.as<float>(), m, kk, cq, sids.as<uint32_t>(),
+                    sdist.as<float>(), sr64.as<double>(), scert.as<uint8_t>(), nullptr, nullptr);
+        hipLaunchKernelGGL(k_scatter_topk, dim3(ceil_div((uint64_t)m * kk, 256)), dim3(256), 0, h->stream,
+                           gi.as<uint32_t>(), m, kk, sids.as<uint32_t>(), sdist.as<float>(), sr64.as<double>(),
+                           scert.as<uint8_t>(), ids_dev, d_dev, r64, cert);
+        HIPC(hipGetLastError());
+        std::vector<uint8_t> sok(m);
+        d2h(h, sok.data(), scert.p, m);
+        std::vector<uint32_t> next;
+        for (uint32_t t = 0; t < m; t++)
+            if (!sok[t]) next.push_back(fail[t]);
+        fail.swap(next);
+    }
+    st.uncertified = fail.size();
+    HIPC(hipEventRecord(ev[2], h->stream));
+    HIPC(hipEventSynchronize(ev[2]));
+    float ms1 = 0.f, ms2 = 0.f;
+    HIPC(hipEventElapsedTime(&ms1, ev[0], ev[1]));
+    HIPC(hipEventElapsedTime(&ms2, ev[0], ev[2]));
+    st.ms_first = ms1;
+    st.ms_total = ms2;
+    if (ex && ex->st) *ex->st = st;
 }
 
 void bf_rknn_count_fill(hrnn_dev_t h, const float *q_dev, uint32_t nq, const float *kth_dev,
                         int prec, uint32_t *cnt_dev, uint32_t *ids_dev, float *d_dev,
-                        uint32_t cap_per_q) {
+                        uint32_t cap_per_q, const double *kth64_dev, unsigned long long *chk_dev) {
     HIPC(hipMemsetAsync(cnt_dev, 0, (uint64_t)nq * 4, h->stream));
+    if (chk_dev) HIPC(hipMemsetAsync(chk_dev, 0, 16, h->stream));
     unsigned grid = ceil_div(nq, kRB);
     // the Gram form decides exactly when every value is an integer in [0, 255]
-    // and d <= 128 (all norms and products below 2^24)
+    // and d <= 128 (all norms and products below 2^24); the FP64 decision is
+    // then the same as well
     bool exact16 = h->base_u8 && h->d <= 128 && mfma_ok(h) && prec != HRNN_PREC_F32;
     if (exact16) {
-        h->s7.reserve((uint64_t)nq * h->d * 4 + (uint64_t)nq * 4 + 256);
+        h->s7.reserve((uint64_t)nq * h->d * 4 + (uint64_t)nq * 8 + 256);
         __hip_bfloat16 *qb = h->s7.as<__hip_bfloat16>();
         __half *qh = reinterpret_cast<__half *>(qb + (uint64_t)nq * h->d);
-        float *qnn = reinterpret_cast<float *>(qh + (uint64_t)nq * h->d);
+        float *qnb = reinterpret_cast<float *>(qh + (uint64_t)nq * h->d);
         hipLaunchKernelGGL(k_query_prep, dim3(ceil_div(nq, 4)), dim3(256), 0, h->stream, q_dev,
-                           (uint64_t)nq, h->d, qb, qh, qnn);
+                           (uint64_t)nq, h->d, qb, qh, qnb, qnb + nq);
         HIPC(hipGetLastError());
-        // queries must be exact in BF16 too; checked by the caller
-        launch_mfma<__hip_bfloat16, 32, 1>(h, qb, qnn, nq, h->xb.as<__hip_bfloat16>(), nullptr, nullptr,
-                                           kth_dev, cnt_dev, ids_dev, d_dev, cap_per_q);
+        // queries must be exact in BF16 too (checked by the caller): the
+        // rounded norms are then the FP32 norms
+        launch_mfma<__hip_bfloat16, 32, 1>(h, qb, qnb, nq, h->xb.as<__hip_bfloat16>(), h->norms.as<float>(),
+                                           nullptr, nullptr, nullptr, nullptr, 0, kth_dev, cnt_dev,
+                                           ids_dev, d_dev, cap_per_q);
     } else {
+        // the FP32 rounding of a distance and of r_k, any order: gamma_{d+2} each
+        const double g = gamma_u23(h->d + 2);
+        const float band = (float)(2.0 * g * (1.0 + 2.0 * g) + 1e-9);
         hipLaunchKernelGGL((k_bf_f32<32, 1>), dim3(grid), dim3(256), 0, h->stream, q_dev, nq,
-                           h->x.as<float>(), h->n, h->d, nullptr, nullptr, kth_dev, cnt_dev,
-                           ids_dev, d_dev, cap_per_q);
+                           h->x.as<float>(), h->n, h->d, nullptr, nullptr, nullptr, nullptr, 0u, kth_dev,
+                           cnt_dev, ids_dev, d_dev, cap_per_q, band, kth64_dev, chk_dev);
         HIPC(hipGetLastError());
     }
 }
@@ -650,24 +1020,42 @@ using namespace hrnn;
 
 extern "C" {
 
-int hrnn_dev_bf_kth(hrnn_dev_t h, uint32_t k, int prec, float *kth, uint32_t *ids) {
+int hrnn_dev_bf_kth_ex(hrnn_dev_t h, uint32_t k, int prec, float *kth, double *kth64, uint32_t *ids,
+                       float *raw_kth, uint32_t *raw_ids, hrnn_bf_stats *stats) {
     if (!h) return HRNN_E_ARG;
     return guarded(h, [&] {
         REQUIRE(h->n > 0, HRNN_E_STATE, "bf_kth: set_base first");
         REQUIRE(kth, HRNN_E_ARG, "bf_kth: null output");
         uint32_t kk = std::min<uint64_t>(k + 1, h->n);
         uint64_t n = h->n;
-        DevBuf di, dd;
+        DevBuf di, dd, d64, drk, dri;
         di.reserve(n * kk * 4);
         dd.reserve(n * kk * 4);
-        bf_topk_base(h, nullptr, (uint32_t)n, true, kk, prec, di.as<uint32_t>(), dd.as<float>());
+        d64.reserve(n * 8);
+        BfExtra ex;
+        ex.r64 = d64.as<double>();
+        ex.st = stats;
+        if (raw_kth) {
+            drk.reserve(n * 4);
+            ex.raw_kth = drk.as<float>();
+        }
+        if (raw_ids) {
+            dri.reserve(n * kk * 4);
+            ex.raw_ids = dri.as<uint32_t>();
+        }
+        bf_topk_base(h, nullptr, (uint32_t)n, true, kk, prec, di.as<uint32_t>(), dd.as<float>(), &ex);
         std::vector<float> all(n * kk);
         d2h(h, all.data(), dd.p, n * kk * 4);
         for (uint64_t i = 0; i < n; i++) kth[i] = all[i * kk + (kk - 1)];
         if (ids) d2h(h, ids, di.p, n * kk * 4);
-        di.release();
-        dd.release();
+        if (kth64) d2h(h, kth64, d64.p, n * 8);
+        if (raw_kth) d2h(h, raw_kth, drk.p, n * 4);
+        if (raw_ids) d2h(h, raw_ids, dri.p, n * kk * 4);
     });
+}
+
+int hrnn_dev_bf_kth(hrnn_dev_t h, uint32_t k, int prec, float *kth, uint32_t *ids) {
+    return hrnn_dev_bf_kth_ex(h, k, prec, kth, nullptr, ids, nullptr, nullptr, nullptr);
 }
 
 int hrnn_dev_bf_knn(hrnn_dev_t h, const float *q, uint32_t nq, uint32_t k, int prec,
@@ -689,14 +1077,15 @@ int hrnn_dev_bf_knn(hrnn_dev_t h, const float *q, uint32_t nq, uint32_t k, int p
     });
 }
 
-int hrnn_dev_bf_rknn(hrnn_dev_t h, const float *q, uint32_t nq, const float *kth, int prec,
-                     uint64_t *res_off, uint32_t *res_ids, float *res_d2, uint64_t cap,
-                     uint64_t *needed) {
+static int bf_rknn_impl(hrnn_dev_t h, const float *q, uint32_t nq, const float *kth, const double *kth64,
+                        int prec, uint64_t *res_off, uint32_t *res_ids, float *res_d2, uint64_t cap,
+                        uint64_t *needed, uint64_t *chk_out) {
     if (!h) return HRNN_E_ARG;
     return guarded(h, [&] {
         REQUIRE(h->n > 0, HRNN_E_STATE, "bf_rknn: set_base first");
         REQUIRE(q && kth && res_off, HRNN_E_ARG, "bf_rknn: null argument");
         res_off[0] = 0;
+        if (chk_out) chk_out[0] = chk_out[1] = 0;
         if (!nq) { if (needed) *needed = 0; return; }
         uint64_t n = h->n;
         // Gram-form path only when the queries are integers in [0, 255] as well
@@ -708,9 +1097,14 @@ int hrnn_dev_bf_rknn(hrnn_dev_t h, const float *q, uint32_t nq, const float *kth
         int p = q_int ? prec : HRNN_PREC_F32;
         const uint32_t chunk = 8192;
         uint32_t cap_q = 1024;
-        DevBuf dq, dk, dc, di, dd;
+        DevBuf dq, dk, dc, di, dd, dk64, dchk;
         dk.reserve(n * 4);
         h2d(h, dk.p, kth, n * 4);
+        if (kth64) {
+            dk64.reserve(n * 8);
+            h2d(h, dk64.p, kth64, n * 8);
+            dchk.reserve(16);
+        }
         std::vector<uint32_t> ids_all;
         std::vector<float> d_all;
         std::vector<uint64_t> off(nq + 1, 0);
@@ -724,12 +1118,19 @@ int hrnn_dev_bf_rknn(hrnn_dev_t h, const float *q, uint32_t nq, const float *kth
                 di.reserve((uint64_t)nqc * cap_q * 4);
                 dd.reserve((uint64_t)nqc * cap_q * 4);
                 bf_rknn_count_fill(h, dq.as<float>(), nqc, dk.as<float>(), p, dc.as<uint32_t>(),
-                                   di.as<uint32_t>(), dd.as<float>(), cap_q);
+                                   di.as<uint32_t>(), dd.as<float>(), cap_q, dk64.as<double>(),
+                                   dchk.as<unsigned long long>());
                 d2h(h, cnt.data(), dc.p, (uint64_t)nqc * 4);
                 uint32_t mx = 0;
                 for (uint32_t c : cnt) mx = std::max(mx, c);
                 if (mx <= cap_q) break;
                 cap_q = mx;
+            }
+            if (kth64 && chk_out) {
+                unsigned long long c2[2];
+                d2h(h, c2, dchk.p, 16);
+                chk_out[0] += c2[0];
+                chk_out[1] += c2[1];
             }
             std::vector<uint32_t> hi((uint64_t)nqc * cap_q);
             std::vector<float> hd((uint64_t)nqc * cap_q);
@@ -753,8 +1154,28 @@ int hrnn_dev_bf_rknn(hrnn_dev_t h, const float *q, uint32_t nq, const float *kth
         if (total > cap) throw Error{HRNN_E_CAP, "bf_rknn: result capacity too small"};
         if (total && res_ids) std::copy(ids_all.begin(), ids_all.end(), res_ids);
         if (total && res_d2) std::copy(d_all.begin(), d_all.end(), res_d2);
-        dq.release(); dk.release(); dc.release(); di.release(); dd.release();
     });
+}
+
+int hrnn_dev_bf_rknn(hrnn_dev_t h, const float *q, uint32_t nq, const float *kth, int prec,
+                     uint64_t *res_off, uint32_t *res_ids, float *res_d2, uint64_t cap,
+                     uint64_t *needed) {
+    return bf_rknn_impl(h, q, nq, kth, nullptr, prec, res_off, res_ids, res_d2, cap, needed, nullptr);
+}
+
+int hrnn_dev_bf_rknn_f64(hrnn_dev_t h, const float *q, uint32_t nq, const float *kth, const double *kth64,
+                         uint64_t *res_off, uint32_t *res_ids, float *res_d2, uint64_t cap,
+                         uint64_t *needed, uint64_t *rechecked, uint64_t *flipped) {
+    if (!h) return HRNN_E_ARG;
+    if (!kth64) {
+        h->err = "bf_rknn_f64: null kth64";
+        return HRNN_E_ARG;
+    }
+    uint64_t chk[2] = {0, 0};
+    int rc = bf_rknn_impl(h, q, nq, kth, kth64, HRNN_PREC_BF16, res_off, res_ids, res_d2, cap, needed, chk);
+    if (rechecked) *rechecked = chk[0];
+    if (flipped) *flipped = chk[1];
+    return rc;
 }
 
 }  // extern "C"

@@ -31,6 +31,14 @@ struct hrnn_dev {
     // from the u8 mirror where that is (base_u8; HRNN_PREC_U8)
     bool vec_bf16 = true;
     bool vec_u8 = false;
+    // changes whenever the base vectors change (set_base, append_base)
+    uint64_t base_version = 1;
+    // brute force: squared norms of the BF16- and FP16-rounded base rows
+    // [2n floats], then the largest rounded norm and the largest rounding
+    // error norm ||round(x) - x|| over the base for each format (4 floats, as
+    // the bits of non-negative floats); valid for bf_prep_version
+    hrnn::DevBuf bf_norm16;
+    uint64_t bf_prep_version = 0;
 
     // launch shapes chosen by the autotuner (hrnn_dev_set_tuning)
     int tune_search_wg_per_cu = 8;   // persistent RkNN search workgroups per CU
@@ -204,11 +212,22 @@ struct BeamOpt {
 void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint32_t ef,
                  uint32_t *proxies_dev, uint32_t *expansions_dev, float *raw_dev = nullptr,
                  const BeamOpt *opt = nullptr);
+// Optional outputs of bf_topk_base (device pointers; null = unused).
+struct BfExtra {
+    double *r64 = nullptr;         // nq: the kk-th smallest FP64 distance of each row
+    float *raw_kth = nullptr;      // nq: the first candidate pass's own kk-th distance
+    uint32_t *raw_ids = nullptr;   // nq * kk: the first candidate pass's first kk ids
+    hrnn_bf_stats *st = nullptr;   // host: certificate statistics
+};
 void bf_topk_base(hrnn_dev_t h, const float *q_dev, uint32_t nq, bool self_base, uint32_t kk,
-                  int prec, uint32_t *ids_dev, float *d_dev);
+                  int prec, uint32_t *ids_dev, float *d_dev, const BfExtra *ex = nullptr);
+// RkNN hits of nq device queries; with kth64_dev, pairs within FP32 rounding
+// of r_k are decided in FP64 and chk_dev (2 u64) counts them and the changed
+// decisions.
 void bf_rknn_count_fill(hrnn_dev_t h, const float *q_dev, uint32_t nq, const float *kth_dev,
                         int prec, uint32_t *cnt_dev, uint32_t *ids_dev, float *d_dev,
-                        uint32_t cap_per_q);
+                        uint32_t cap_per_q, const double *kth64_dev = nullptr,
+                        unsigned long long *chk_dev = nullptr);
 void l2_pairs(hrnn_dev_t h, const float *a_dev, const uint32_t *ia, const uint32_t *ib,
               uint64_t npairs, float *out);
 

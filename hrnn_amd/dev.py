@@ -36,7 +36,8 @@ EXPORTS = [
     "hrnn_dev_build_rknn_csr", "hrnn_dev_get_rknn_csr", "hrnn_dev_set_rknn_csr",
     "hrnn_dev_rknn_search", "hrnn_dev_rknn_search_dptr", "hrnn_dev_rknn_search_noalloc_dptr", "hrnn_dev_fetch_results", "hrnn_dev_set_hnsw", "hrnn_dev_proxies",
     "hrnn_dev_proxies_dptr", "hrnn_dev_rknn_query_dptr", "hrnn_dev_bf_kth", "hrnn_dev_bf_knn",
-    "hrnn_dev_bf_rknn", "hrnn_dev_l2sqr_pairs", "hrnn_dev_mlp_forward",
+    "hrnn_dev_bf_rknn", "hrnn_dev_bf_kth_ex", "hrnn_dev_bf_rknn_f64", "hrnn_dev_l2sqr_pairs",
+    "hrnn_dev_mlp_forward",
     "hrnn_dev_mlp_grad", "hrnn_dev_mlp_train", "hrnn_dev_rknn_search_learned_dptr",
     "hrnn_dev_stage1_features_dptr", "hrnn_dev_rknn_query_adaptive_dptr",
     "hrnn_synth_mixture",
@@ -59,6 +60,12 @@ class HnswBuildStats(C.Structure):
 class NndIterStats(C.Structure):
     _fields_ = [("dists", C.c_uint64), ("updates", C.c_uint64), ("proposals", C.c_uint64),
                 ("dropped", C.c_uint64), ("ms", C.c_double), ("rows", C.c_uint64)]
+
+
+class BfStats(C.Structure):
+    _fields_ = [("rows", C.c_uint64), ("certified", C.c_uint64), ("redo_wide", C.c_uint64),
+                ("redo_f32", C.c_uint64), ("uncertified", C.c_uint64), ("ms_first", C.c_double),
+                ("ms_total", C.c_double)]
 
 
 _lib = None
@@ -110,6 +117,8 @@ def lib() -> C.CDLL:
         "hrnn_dev_bf_kth": [vp, u32, i, vp, vp],
         "hrnn_dev_bf_knn": [vp, vp, u32, u32, i, vp, vp],
         "hrnn_dev_bf_rknn": [vp, vp, u32, vp, i, vp, vp, vp, u64, P(u64)],
+        "hrnn_dev_bf_kth_ex": [vp, u32, i, vp, vp, vp, vp, vp, P(BfStats)],
+        "hrnn_dev_bf_rknn_f64": [vp, vp, u32, vp, vp, vp, vp, vp, u64, P(u64), P(u64), P(u64)],
         "hrnn_dev_l2sqr_pairs": [vp, vp, u64, vp, vp, u64, vp],
         "hrnn_dev_mlp_forward": [vp, vp, vp, u32, vp],
         "hrnn_dev_mlp_grad": [vp, vp, vp, vp, u32, vp, vp],
@@ -384,6 +393,47 @@ class Device:
         ids = np.empty((self.n, kk), np.uint32) if with_ids else None
         self._ck(self.L.hrnn_dev_bf_kth(self.h, k, _PREC.get(prec, prec), _p(kth), _p(ids)))
         return (kth, ids) if with_ids else kth
+
+    def bf_kth_ex(self, k: int, prec="bf16", raw=False):
+        """Certified r_k with the precision study's outputs: a dict of kth
+        (FP32), kth64 (FP64), ids [n x (k+1)], stats, and with raw=True the
+        first candidate pass's own r_k and ids before re-rank."""
+        kk = min(k + 1, self.n)
+        kth = np.empty(self.n, np.float32)
+        kth64 = np.empty(self.n, np.float64)
+        ids = np.empty((self.n, kk), np.uint32)
+        rk = np.empty(self.n, np.float32) if raw else None
+        ri = np.empty((self.n, kk), np.uint32) if raw else None
+        st = BfStats()
+        self._ck(self.L.hrnn_dev_bf_kth_ex(self.h, k, _PREC.get(prec, prec), _p(kth), _p(kth64), _p(ids),
+                                           _p(rk), _p(ri), C.byref(st)))
+        out = {"kth": kth, "kth64": kth64, "ids": ids,
+               "stats": {f: getattr(st, f) for f, _ in BfStats._fields_}}
+        if raw:
+            out["raw_kth"], out["raw_ids"] = rk, ri
+        return out
+
+    def bf_rknn_f64(self, q, kth, kth64):
+        """RkNN with the FP64 re-check of pairs within FP32 rounding of r_k:
+        (offsets, ids, d2 f32, rechecked, flipped)."""
+        q = np.ascontiguousarray(q, np.float32)
+        kth = np.ascontiguousarray(kth, np.float32)
+        kth64 = np.ascontiguousarray(kth64, np.float64)
+        nq = q.shape[0]
+        off = np.zeros(nq + 1, np.uint64)
+        need, rc_n, fl_n = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+        cap = max(1024, nq * 64)
+        while True:
+            ids = np.empty(cap, np.uint32)
+            d2 = np.empty(cap, np.float32)
+            rc = self.L.hrnn_dev_bf_rknn_f64(self.h, _p(q), nq, _p(kth), _p(kth64), _p(off), _p(ids), _p(d2),
+                                             cap, C.byref(need), C.byref(rc_n), C.byref(fl_n))
+            if rc == -5 and need.value > cap:
+                cap = int(need.value)
+                continue
+            self._ck(rc)
+            n = int(off[-1])
+            return off, ids[:n].copy(), d2[:n].copy(), int(rc_n.value), int(fl_n.value)
 
     def bf_knn(self, q, k: int, prec="bf16"):
         q = np.ascontiguousarray(q, np.float32)

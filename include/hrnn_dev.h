@@ -288,8 +288,33 @@ int hrnn_dev_rknn_query_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint
  * (self included), i.e. the k-th neighbour distance r_k(i).  prec selects the
  * candidate pass; the final value is the FP32 direct-difference distance after
  * re-ranking the top 2(k+1) candidates by (dist, id).  ids (may be NULL) gets
- * the (k+1) nearest ids per row in (dist, id) order, self included. */
+ * the (k+1) nearest ids per row in (dist, id) order, self included.
+ * Every row's candidate list must carry a certificate (an error bound of the
+ * candidate pass shows that no other row can rank among the first k+1); rows
+ * without one are redone with a 64-wide list, then with the FP32 pass, so the
+ * results equal an exhaustive FP32 re-rank. */
 int hrnn_dev_bf_kth(hrnn_dev_t h, uint32_t k, int prec, float *kth, uint32_t *ids);
+
+/* Certificate statistics of a brute-force top-k. */
+typedef struct {
+    uint64_t rows;         /* rows ranked */
+    uint64_t certified;    /* rows certified after the first candidate pass */
+    uint64_t redo_wide;    /* rows redone with a 64-wide list in the same format */
+    uint64_t redo_f32;     /* rows redone with the FP32 candidate pass */
+    uint64_t uncertified;  /* rows no pass certified (results of the last pass) */
+    double ms_first;       /* device time of the first pass and its re-rank, ms */
+    double ms_total;       /* device time of all passes, ms */
+} hrnn_bf_stats;
+
+/* hrnn_dev_bf_kth with the precision study's outputs (each may be NULL):
+ * kth64[n] the (k+1)-th smallest FP64 distance (sequential FP64 sum, no fused
+ * multiply-add), certified like kth; raw_kth[n] and raw_ids[n*(k+1)] the first
+ * candidate pass's own (k+1)-th distance and first k+1 ids, before re-rank
+ * (in prec's format: the error a BF16/FP16 oracle would make); stats the
+ * certificate counts and device times.  No reference counterpart (the
+ * reference's brute force is FP32 only, scripts/gen_rknn_gt.py:42-79). */
+int hrnn_dev_bf_kth_ex(hrnn_dev_t h, uint32_t k, int prec, float *kth, double *kth64, uint32_t *ids,
+                       float *raw_kth, uint32_t *raw_ids, hrnn_bf_stats *stats);
 /* Exact kNN of nq host queries against the base: ids/d2 [nq*k] in (dist, id)
  * order (ties to the smaller id). */
 int hrnn_dev_bf_knn(hrnn_dev_t h, const float *q, uint32_t nq, uint32_t k, int prec,
@@ -299,6 +324,17 @@ int hrnn_dev_bf_knn(hrnn_dev_t h, const float *q, uint32_t nq, uint32_t k, int p
 int hrnn_dev_bf_rknn(hrnn_dev_t h, const float *q, uint32_t nq, const float *kth,
                      int prec, uint64_t *res_off, uint32_t *res_ids, float *res_d2,
                      uint64_t cap, uint64_t *needed);
+/* The same with the FP64 re-check: a pair whose FP32 distance lies within
+ * FP32 rounding of kth[x] (a rigorous bound for any summation order) is
+ * decided by its FP64 distance against kth64[x] (kth and kth64 as
+ * hrnn_dev_bf_kth_ex gives them); every other pair's FP32 decision provably
+ * equals the FP64 one.  rechecked / flipped (may be NULL) count the pairs
+ * re-checked and those whose FP64 decision differs from the FP32 one.
+ * res_d2 holds FP32 distances.  Replaces the FP32 decision of compute_rknn
+ * (scripts/gen_rknn_gt.py:116-124) with the FP64 judge's. */
+int hrnn_dev_bf_rknn_f64(hrnn_dev_t h, const float *q, uint32_t nq, const float *kth,
+                         const double *kth64, uint64_t *res_off, uint32_t *res_ids, float *res_d2,
+                         uint64_t cap, uint64_t *needed, uint64_t *rechecked, uint64_t *flipped);
 
 /* ---- learned proxy count (no reference counterpart) --------------------- */
 /* A 32-64-64-16 MLP (ReLU, softmax) predicts per query the smallest m of a

@@ -62,6 +62,10 @@ def orc() -> C.CDLL:
                                 C.c_int, C.c_int, _u32p, _f32p]
         lib.orc_kth.argtypes = [_f32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_int, _f32p]
         lib.orc_kth_f64.argtypes = [_f32p, C.c_uint64, C.c_uint32, C.c_uint32, _f64p]
+        lib.orc_kth_f64_rows.argtypes = [_f32p, C.c_uint64, C.c_uint32, C.c_uint32, _u32p,
+                                         C.c_uint64, _f64p]
+        lib.orc_rknn_gt_f64.argtypes = [_f32p, C.c_uint64, _f32p, C.c_uint64, C.c_uint32, _f64p,
+                                        _u64p, _u32p, _f64p, C.c_uint64, C.POINTER(C.c_uint64)]
         lib.orc_rknn_gt.argtypes = [_f32p, C.c_uint64, _f32p, C.c_uint64, C.c_uint32, _f32p,
                                     C.c_int, C.c_int, _u64p, _u32p, _f32p, C.c_uint64,
                                     C.POINTER(C.c_uint64)]
@@ -101,6 +105,12 @@ def l2sqr(a, b, W=L2_W, fma=L2_FMA) -> float:
     return orc().orc_l2sqr(a, b, a.shape[-1], W, fma)
 
 
+def l2sqr_f64(a, b) -> float:
+    """Squared L2 in FP64: sequential sum of (a_i - b_i)^2, no fused multiply-add."""
+    a, b = _f32(a), _f32(b)
+    return orc().orc_l2sqr_f64(a, b, a.shape[-1])
+
+
 def knn(x, q, k, W=L2_W, fma=L2_FMA):
     x, q = _f32(x), _f32(q)
     ids = np.empty((q.shape[0], k), np.uint32)
@@ -122,6 +132,33 @@ def kth_f64(x, k):
     out = np.empty(x.shape[0], np.float64)
     assert orc().orc_kth_f64(x, x.shape[0], x.shape[1], k, out) == 0
     return out
+
+
+def kth_f64_rows(x, k, rows):
+    """FP64 r_k of the listed rows only (same definition as kth_f64)."""
+    x = _f32(x)
+    rows = np.ascontiguousarray(rows, dtype=np.uint32)
+    out = np.empty(rows.shape[0], np.float64)
+    assert orc().orc_kth_f64_rows(x, x.shape[0], x.shape[1], k, rows, rows.shape[0], out) == 0
+    return out
+
+
+def rknn_gt_f64(x, q, kth64):
+    """FP64-judged RkNN: (offsets u64[nq+1], ids u32, d2 f64) sorted by (d2, id)."""
+    x, q = _f32(x), _f32(q)
+    kth64 = np.ascontiguousarray(kth64, dtype=np.float64)
+    nq = q.shape[0]
+    off = np.zeros(nq + 1, np.uint64)
+    need = C.c_uint64(0)
+    orc().orc_rknn_gt_f64(x, x.shape[0], q, nq, x.shape[1], kth64, off, np.zeros(1, np.uint32),
+                          np.zeros(1, np.float64), 0, C.byref(need))
+    n = int(need.value)
+    ids = np.empty(max(n, 1), np.uint32)
+    d2 = np.empty(max(n, 1), np.float64)
+    rc = orc().orc_rknn_gt_f64(x, x.shape[0], q, nq, x.shape[1], kth64, off, ids, d2, n,
+                               C.byref(need))
+    assert rc == 0
+    return off, ids[:n], d2[:n]
 
 
 def rknn_gt(x, q, kth_arr, W=L2_W, fma=L2_FMA):

@@ -133,24 +133,48 @@ int orc_kth(const float *x, uint64_t n, uint32_t d, uint32_t k, int W, int use_f
     return rc;
 }
 
-/* FP64 r_k for the precision study (no reference counterpart: the FP64 judge). */
+/* FP64 r_k for the precision study (no reference counterpart: the FP64 judge).
+ * The (k+1)-th smallest FP64 distance (orc_l2sqr_f64: sequential, no fused
+ * multiply-add) from row i over all rows, self included, counting equal
+ * values separately.  Pinned: equal to kth on integer data; against a float64
+ * numpy brute force on the precision form's data; an FP32-accumulation slip
+ * fails that comparison (tests/test_oracle.py). */
+static double kth_f64_row(const float *x, uint64_t n, uint32_t d, uint32_t kk, uint64_t i,
+                          double *best) {
+    uint32_t sz = 0;
+    for (uint64_t j = 0; j < n; j++) {
+        double v = orc_l2sqr_f64(x + (size_t)i * d, x + j * d, d);
+        if (sz == kk) { if (v >= best[kk - 1]) continue; sz--; }
+        uint32_t p = sz;
+        while (p > 0 && v < best[p - 1]) { best[p] = best[p - 1]; p--; }
+        best[p] = v; sz++;
+    }
+    return best[kk - 1];
+}
+
 int orc_kth_f64(const float *x, uint64_t n, uint32_t d, uint32_t k, double *kth) {
     uint32_t kk = k + 1 < n ? k + 1 : (uint32_t)n;
 #pragma omp parallel
     {
         double *best = (double *)malloc(sizeof(double) * kk);
 #pragma omp for schedule(dynamic, 16)
-        for (int64_t i = 0; i < (int64_t)n; i++) {
-            uint32_t sz = 0;
-            for (uint64_t j = 0; j < n; j++) {
-                double v = orc_l2sqr_f64(x + (size_t)i * d, x + j * d, d);
-                if (sz == kk) { if (v >= best[kk - 1]) continue; sz--; }
-                uint32_t p = sz;
-                while (p > 0 && v < best[p - 1]) { best[p] = best[p - 1]; p--; }
-                best[p] = v; sz++;
-            }
-            kth[i] = best[kk - 1];
-        }
+        for (int64_t i = 0; i < (int64_t)n; i++) kth[i] = kth_f64_row(x, n, d, kk, (uint64_t)i, best);
+        free(best);
+    }
+    return ORC_OK;
+}
+
+/* The same for the listed rows only: kth[t] = r_k(rows[t]) in FP64. */
+int orc_kth_f64_rows(const float *x, uint64_t n, uint32_t d, uint32_t k, const uint32_t *rows,
+                     uint64_t nrows, double *kth) {
+    uint32_t kk = k + 1 < n ? k + 1 : (uint32_t)n;
+    for (uint64_t t = 0; t < nrows; t++)
+        if (rows[t] >= n) return ORC_E_RANGE;
+#pragma omp parallel
+    {
+        double *best = (double *)malloc(sizeof(double) * kk);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t t = 0; t < (int64_t)nrows; t++) kth[t] = kth_f64_row(x, n, d, kk, rows[t], best);
         free(best);
     }
     return ORC_OK;
@@ -196,6 +220,55 @@ int orc_rknn_gt(const float *x, uint64_t n, const float *q, uint64_t nq, uint32_
             if (v <= kth[i]) { buf[w].d = v; buf[w].id = (uint32_t)i; w++; }
         }
         qsort(buf, w, sizeof(orc_pair), cmp_pair);
+        for (uint64_t j = 0; j < w; j++) {
+            ids[off[qi] + j] = buf[j].id;
+            if (d2) d2[off[qi] + j] = buf[j].d;
+        }
+        free(buf);
+    }
+    return ORC_OK;
+}
+
+/* The FP64 judge of compute_rknn (no reference counterpart; the reference
+ * decides in FP32, gen_rknn_gt.py:116-124): every base row x with
+ * orc_l2sqr_f64(q, x) <= kth64[x], sorted by (FP64 distance, id).
+ * Pinned: equal to rknn_gt on integer data (both exact there); against a
+ * float64 numpy brute force on float data, including queries placed within
+ * FP32 rounding of a radius where the FP32 decision differs. */
+typedef struct { double d; uint32_t id; } orc_pair64;
+
+static int cmp_pair64(const void *a, const void *b) {
+    const orc_pair64 *x = (const orc_pair64 *)a, *y = (const orc_pair64 *)b;
+    if (x->d != y->d) return x->d < y->d ? -1 : 1;
+    return x->id < y->id ? -1 : (x->id > y->id ? 1 : 0);
+}
+
+int orc_rknn_gt_f64(const float *x, uint64_t n, const float *q, uint64_t nq, uint32_t d,
+                    const double *kth64, uint64_t *off, uint32_t *ids, double *d2, uint64_t cap,
+                    uint64_t *needed) {
+    uint64_t *cnt = (uint64_t *)calloc(nq + 1, sizeof(uint64_t));
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t qi = 0; qi < (int64_t)nq; qi++) {
+        uint64_t c = 0;
+        for (uint64_t i = 0; i < n; i++)
+            if (orc_l2sqr_f64(q + (size_t)qi * d, x + i * d, d) <= kth64[i]) c++;
+        cnt[qi] = c;
+    }
+    off[0] = 0;
+    for (uint64_t qi = 0; qi < nq; qi++) off[qi + 1] = off[qi] + cnt[qi];
+    free(cnt);
+    if (needed) *needed = off[nq];
+    if (off[nq] > cap) return ORC_E_CAP;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t qi = 0; qi < (int64_t)nq; qi++) {
+        uint64_t c = off[qi + 1] - off[qi];
+        orc_pair64 *buf = (orc_pair64 *)malloc(sizeof(orc_pair64) * (c ? c : 1));
+        uint64_t w = 0;
+        for (uint64_t i = 0; i < n; i++) {
+            double v = orc_l2sqr_f64(q + (size_t)qi * d, x + i * d, d);
+            if (v <= kth64[i]) { buf[w].d = v; buf[w].id = (uint32_t)i; w++; }
+        }
+        qsort(buf, w, sizeof(orc_pair64), cmp_pair64);
         for (uint64_t j = 0; j < w; j++) {
             ids[off[qi] + j] = buf[j].id;
             if (d2) d2[off[qi] + j] = buf[j].d;

@@ -116,6 +116,67 @@ def test_kth_f64_float_data_vs_float64_brute_force():
     assert not np.allclose(exp_k, exp, rtol=1e-12, atol=0)
     f32 = oracle.kth(x, k).astype(np.float64)
     assert np.mean(np.abs(f32 - exp) > 1e-12 * exp) > 0.5
+    rows = np.array([0, 7, 299, 150], np.uint32)
+    assert np.array_equal(oracle.kth_f64_rows(x, k, rows), got[rows])
+
+
+def test_l2sqr_f64_is_sequential_without_fma():
+    """The FP64 distance: float64 differences, squares and sums rounded one
+    by one, left to right.  Pinned on a case where the order matters: a huge
+    term first absorbs the small ones (a pairwise or reversed order keeps
+    them), and on a random pair against the same loop written out here."""
+    a = np.zeros(4, np.float32)
+    b = np.array([2.0 ** 27, 1.0, 1.0, 1.0], np.float32)     # squares 2^54, 1, 1, 1
+    assert oracle.l2sqr_f64(a, b) == 2.0 ** 54                 # sequential: each 1 is lost
+    assert oracle.l2sqr_f64(a, b[::-1].copy()) == 2.0 ** 54 + 4.0   # 1+1+1 = 3 first, then 2^54 (ties to even)
+    rng = np.random.default_rng(3)
+    u, v = rng.random(960, np.float32), rng.random(960, np.float32)
+    r = 0.0
+    for i in range(960):
+        t = float(u[i]) - float(v[i])
+        r += t * t
+    assert oracle.l2sqr_f64(u, v) == r
+
+
+def test_rknn_gt_f64_pins():
+    """The FP64 judge of compute_rknn: {x : d64(q, x) <= kth64[x]}.
+    * integer data: equal to the FP32 groundtruth (every distance exact);
+    * the precision form's float data at d = 960, with queries placed within
+      FP32 rounding of a radius (where FP32 and FP64 decide differently):
+      equal to a float64 numpy brute force; the FP32 groundtruth differs on
+      exactly the planted pairs, so a slip to FP32 distances or radii fails."""
+    from boundary_cases import boundary_queries
+    import hrnn_amd
+    x = mixture(600, d=64, seed=31)
+    q = mixture(40, d=64, seed=32)
+    k = 5
+    o32, i32, d32 = oracle.rknn_gt(x, q, oracle.kth(x, k))
+    o64, i64, d64 = oracle.rknn_gt_f64(x, q, oracle.kth_f64(x, k))
+    assert np.array_equal(o32, o64) and np.array_equal(i32, i64)
+    assert np.array_equal(d32.astype(np.float64), d64)
+
+    x = hrnn_amd.synth_halfnormal_scaled(5, 55, 1200, 960)
+    k = 10
+    k32, k64 = oracle.kth(x, k), oracle.kth_f64(x, k)
+    qb, pb, inb = boundary_queries(x, k, k32, k64, range(0, 1200, 37), want=6)
+    assert len(qb) >= 4, len(qb)
+    qr = hrnn_amd.synth_halfnormal_scaled(6, 55, 20, 960)
+    q = np.concatenate([qb, qr])
+    off, ids, dd = oracle.rknn_gt_f64(x, q, k64)
+    xd = x.astype(np.float64)
+    for i in range(q.shape[0]):
+        e = ((xd - q[i].astype(np.float64)) ** 2).sum(1)
+        exp = np.nonzero(e <= k64)[0]
+        got = ids[off[i]:off[i + 1]]
+        assert np.array_equal(np.sort(got), exp), i
+        assert np.all(np.diff(dd[off[i]:off[i + 1]]) >= 0), i
+    for i in range(len(qb)):      # the planted decision, and the FP32 groundtruth disagrees
+        got = set(ids[off[i]:off[i + 1]].tolist())
+        assert (int(pb[i]) in got) == bool(inb[i]), i
+    f_off, f_ids, _ = oracle.rknn_gt(x, qb, k32)
+    for i in range(len(qb)):
+        f = set(f_ids[f_off[i]:f_off[i + 1]].tolist())
+        assert (int(pb[i]) in f) != bool(inb[i]), i
 
 
 def test_knng_recall_hand_built_graph():
