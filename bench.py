@@ -68,6 +68,10 @@ CONFIGS = {
     "c1": dict(n=10_000, nq=100, ntrain=2000, d=128, centres=50, sigma=20.0, data_seed=1,
                query_seed=2, train_seed=12, centre_seed=1001, M=16, efc=200, K=100, iters=10,
                S=50, k=10, Kp=100, ms=[1, 5, 10], ef=200),
+    # precision form: |N(0,1)| scaled per dimension by U(0.05, 1) (one scale
+    # vector, seed 55, shared by base and queries)
+    "c3": dict(n=1_000_000, nq=1000, d=960, data_seed=1, query_seed=2, scale_seed=55, M=16, efc=200,
+               K=200, iters=10, S=50, k=10, Kp=200, ms=[1, 3, 5, 10, 20, 50], ef=200),
 }
 HBM_PEAK_GBS = 8000.0
 TAUS = [0.5, 0.7, 0.8, 0.9, 0.95, 0.98, 0.995]
@@ -106,6 +110,8 @@ def main() -> int:
                          "(full), on a 50k-point sample (sample), or full when the sample projects "
                          "it under --nnd-ref-budget seconds (auto)")
     ap.add_argument("--nnd-ref-budget", type=float, default=420.0)
+    ap.add_argument("--n", type=int, default=0,
+                    help="base size for --config c3 (0: the configuration's 1,000,000; smaller for a quick run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
@@ -116,6 +122,8 @@ def main() -> int:
     sys.stdout.flush()
     json_out = os.fdopen(os.dup(1), "w")
     os.dup2(2, 1)
+    if args.config == "c3":
+        return run_c3(args, json_out)
 
     import hrnn_amd
     from hrnn_amd import hiprt
@@ -901,6 +909,324 @@ def main() -> int:
                      "rknn_csr_gpu": round(csr_s, 3), "rk_bruteforce_gpu": round(kth_s, 3),
                      "groundtruth_gpu": round(gt_s, 3)},
         "sweep": sweep,
+    }
+    print(json.dumps(out), file=json_out, flush=True)
+    return 0
+
+
+def run_c3(args, json_out) -> int:
+    """The precision form (1M x 960, float data): the certified brute force in
+    BF16, FP16 and FP32 with each format's error against the FP64 judge, the
+    FP64 re-check of the RkNN groundtruth, and the HRNN pipeline (GPU HNSW,
+    NNDescent K=200, reverse CSR, RkNN search) judged by the FP64
+    groundtruth.  value = the highest search QPS at macro recall >= 0.95."""
+    import hrnn_amd
+    from hrnn_amd import hiprt
+    from hrnn_amd.dev import Device
+    from hrnn_amd.index import Index
+
+    cfg = CONFIGS["c3"]
+    threads = args.threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or 16
+    n, nq, d, K, k, Kp, ef = cfg["n"], cfg["nq"], cfg["d"], cfg["K"], cfg["k"], cfg["Kp"], cfg["ef"]
+    n = args.n or n
+    kk = k + 1
+    L = hrnn_amd.lib()
+    t = time.time()
+    x = hrnn_amd.synth_halfnormal_scaled(cfg["data_seed"], cfg["scale_seed"], n, d)
+    q = hrnn_amd.synth_halfnormal_scaled(cfg["query_seed"], cfg["scale_seed"], nq, d)
+    log(f"data {n}x{d} + {nq} queries in {time.time() - t:.1f}s")
+
+    # ---- (d) certified brute force in each format; errors against FP64 ---------------------
+    dev = Device(0)
+    dev.set_base(x)
+    bf = {}
+    for prec in ("bf16", "f16", "f32"):
+        t = time.time()
+        r = dev.bf_kth_ex(k, prec, raw=True)
+        r["s"] = time.time() - t
+        bf[prec] = r
+        log(f"r_k brute force {prec}: {r['s']:.2f}s {r['stats']}")
+    ref = bf["f32"]
+    kth, kth64, ids = ref["kth"], ref["kth64"], ref["ids"]
+    prec_rep = {}
+    for prec, r in bf.items():
+        rel = np.abs(r["raw_kth"].astype(np.float64) - kth64) / kth64
+        raw_rec = float(np.mean([np.intersect1d(r["raw_ids"][i], ids[i]).size for i in range(0, n, 97)]) / kk)
+        st = r["stats"]
+        prec_rep[prec] = {
+            "seconds": round(r["s"], 3), "device_ms_first_pass": round(st["ms_first"], 1),
+            "device_ms_total": round(st["ms_total"], 1),
+            "tflops_first_pass": round(2.0 * n * n * d / (st["ms_first"] * 1e-3) / 1e12, 1),
+            "rows_certified_first_pass": int(st["certified"]), "rows_redone_64_wide": int(st["redo_wide"]),
+            "rows_redone_fp32": int(st["redo_f32"]), "rows_uncertified": int(st["uncertified"]),
+            "final_equal_to_fp32_pass": bool(np.array_equal(r["kth"], kth) and np.array_equal(r["ids"], ids)
+                                             and np.array_equal(r["kth64"], kth64)),
+            "uncorrected_rk_rel_err_vs_fp64": {"max": float(rel.max()), "p99": float(np.quantile(rel, 0.99)),
+                                               "median": float(np.median(rel))},
+            "uncorrected_knn_recall_at_k1": round(raw_rec, 6)}
+    rel32 = np.abs(kth.astype(np.float64) - kth64) / kth64
+    # RkNN: the FP64-judged groundtruth (FP32 decisions, FP64 re-check of the
+    # pairs within FP32 rounding of r_k); the FP32 decisions alone; and the
+    # uncorrected r_k of each format
+    t = time.time()
+    g_off, g_ids, _, rechecked, flipped = dev.bf_rknn_f64(q, kth, kth64)
+    gt_s = time.time() - t
+    f_off, f_ids, _ = dev.bf_rknn(q, kth, "f32")
+
+    def set_err(off, ids_):
+        rec = macro_recall(g_off, g_ids, off, ids_)
+        inter = sum(np.intersect1d(g_ids[g_off[i]:g_off[i + 1]], ids_[off[i]:off[i + 1]]).size for i in range(nq))
+        return {"macro_recall": round(rec, 6), "pairs": int(len(ids_)),
+                "pairs_not_in_fp64": int(len(ids_) - inter), "pairs_missed": int(len(g_ids) - inter)}
+
+    rknn_rep = {"fp64_groundtruth_pairs": int(len(g_ids)), "fp64_rechecked_pairs": rechecked,
+                "fp64_changed_decisions": flipped, "groundtruth_s": round(gt_s, 3),
+                "fp32_decision": set_err(f_off, f_ids)}
+    for prec, r in bf.items():
+        o, i_, _ = dev.bf_rknn(q, r["raw_kth"], "f32")
+        rknn_rep[f"uncorrected_{prec}_rk"] = set_err(o, i_)
+    log(f"RkNN groundtruth (FP64 re-check) {gt_s:.2f}s: {len(g_ids)} pairs, {rechecked} re-checked, "
+        f"{flipped} changed; {rknn_rep}")
+    dev.close()
+
+    # the oracle on samples: FP32 and FP64 r_k of 16 rows, FP64 groundtruth of 4 queries
+    oracle_check, bf_cpu = None, None
+    if not args.no_cpu_baseline:
+        try:
+            import oracle
+            oracle.set_threads(threads)
+            rows = np.arange(0, n, n // 16, dtype=np.uint32)[:16]
+            t1 = time.perf_counter()
+            _, d2s = oracle.knn(x, x[rows], kk)
+            cpu32_s = time.perf_counter() - t1
+            t1 = time.perf_counter()
+            k64s = oracle.kth_f64_rows(x, k, rows)
+            cpu64_s = time.perf_counter() - t1
+            t1 = time.perf_counter()
+            o_off, o_ids, _ = oracle.rknn_gt_f64(x, q[:4], kth64)
+            cpu_gt_s = time.perf_counter() - t1
+            same_gt = all(np.array_equal(np.sort(o_ids[o_off[i]:o_off[i + 1]]),
+                                         np.sort(g_ids[g_off[i]:g_off[i + 1]])) for i in range(4))
+            oracle_check = {"rows": len(rows), "fp32_rk_equal": bool(np.array_equal(d2s[:, k], kth[rows])),
+                            "fp64_rk_equal": bool(np.array_equal(k64s, kth64[rows])),
+                            "fp64_groundtruth_equal_4_queries": bool(same_gt),
+                            "fp64_rows_s": round(cpu64_s, 2), "fp64_groundtruth_4_queries_s": round(cpu_gt_s, 2)}
+            bf_cpu = {"value": round(cpu32_s, 3), "unit": f"s for {len(rows)} rows", "cores": threads,
+                      "kind": "port",
+                      "sample": f"r_k (k={k}) of {len(rows)} rows against all {n} by the oracle's C brute force "
+                                f"(FP32, the reference's 16-lane order), OpenMP on {threads} host threads",
+                      "projected_all_rows_s": round(cpu32_s * n / len(rows), 1),
+                      "gpu_bf16_certified_all_rows_s": prec_rep["bf16"]["seconds"]}
+            log(f"oracle check {oracle_check}; CPU FP32 r_k {cpu32_s:.2f}s for {len(rows)} rows")
+        except Exception as e:
+            log(f"oracle check skipped: {e}")
+
+    # ---- HRNN: HNSW + KNN seeds, NNDescent K=200, reverse CSR on the GPU -----------------------
+    t = time.time()
+    ix = Index(n, d, cfg["M"], cfg["efc"], 100, K)
+    hnsw_st = ix.add_points_gpu(x)
+    hnsw_s = time.time() - t
+    log(f"HNSW + KNN seeds on the GPU: {hnsw_s:.2f}s ({hnsw_st['batches']} batches)")
+    t = time.time()
+    nst = ix.refine(cfg["iters"], cfg["S"])
+    nnd_s = time.time() - t
+    log(f"refineKNNG({cfg['iters']},{cfg['S']}) on GPU: {nnd_s:.2f}s "
+        f"({', '.join('%.1f' % s_['ms'] for s_ in nst)} ms)")
+    t = time.time()
+    ix.build_rknng()
+    csr_s = time.time() - t
+    labels = ix.labels()
+    samp = np.random.default_rng(0).choice(n, 1000, replace=False)       # internal ids
+    kids, _, _ = ix.get_knng()
+    knn_rec10 = float(np.mean([np.intersect1d(labels[kids[s, :k]], ids[labels[s], 1:kk]).size / k for s in samp]))
+    del kids
+    log(f"KNN recall@10 (1000-node sample, exact FP32 ids) {knn_rec10:.4f}")
+
+    # ---- search: m sweep against the FP64 groundtruth, FP32 and BF16 vectors ------------------
+    h = ix.device_handle()
+    stream = L.hrnn_dev_stream(h)
+    m_max = max(cfg["ms"])
+    cap = 4096
+    qd = hiprt.DeviceArray.from_host(q)
+    pd = hiprt.DeviceArray((nq * m_max,), np.uint32)
+    cd = hiprt.DeviceArray((nq,), np.uint32)
+    idd = hiprt.DeviceArray((nq * cap,), np.uint32)
+    wd = hiprt.DeviceArray((2 * nq,), np.uint32)
+    ed = hiprt.DeviceArray((2 * nq,), np.uint32)
+
+    def ck(rc):
+        if rc:
+            raise RuntimeError(L.hrnn_dev_last_error(h).decode())
+
+    def run_gpu(m):
+        ck(L.hrnn_dev_rknn_query_dptr(h, qd.ptr, nq, m, max(ef, m), None, k, Kp, 1.0, pd.ptr, cd.ptr, idd.ptr,
+                                      cap, wd.ptr))
+
+    def results():
+        hiprt.stream_sync(stream)
+        cnt = cd.to_host().astype(np.int64)
+        if cnt.max(initial=0) > cap:
+            raise RuntimeError("result capacity exceeded")
+        raw = idd.to_host().reshape(nq, cap)
+        off = np.zeros(nq + 1, np.int64)
+        off[1:] = np.cumsum(cnt)
+        flat = np.concatenate([raw[i, :cnt[i]] for i in range(nq)]) if off[-1] else np.zeros(0, np.uint32)
+        return off, labels[flat].astype(np.uint32)                           # rows of x
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        hiprt.stream_sync(stream)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        hiprt.stream_sync(stream)
+        return (time.perf_counter() - t0) / args.steps
+
+    q_pin = hiprt.PinnedArray((nq, d), np.float32)
+    q_pin.a[:] = q
+    off_pin = hiprt.PinnedArray((nq + 1,), np.uint64)
+    ids_pin = hiprt.PinnedArray((nq * cap,), np.uint32)
+    need = ctypes.c_uint64(0)
+
+    def step_io(m):
+        hiprt.copy_async(qd.ptr, q_pin.ptr, nq * d * 4, hiprt.H2D, stream)
+        run_gpu(m)
+        ck(L.hrnn_dev_fetch_results(h, nq, cd.ptr, idd.ptr, cap, off_pin.a.ctypes.data_as(ctypes.c_void_p),
+                                    ids_pin.a.ctypes.data_as(ctypes.c_void_p), ids_pin.a.size,
+                                    ctypes.byref(need)))
+
+    paths, sweep = {}, {}
+    # FP32 vectors: the BF16 mirror serves phases A-C only where it is exact
+    # (small integers), which these data are not
+    for vp, pname in ((0, "fp32"),):
+        ck(L.hrnn_dev_set_vector_precision(h, vp))
+        sweep[pname] = []
+        m_star = None
+        for m in cfg["ms"]:
+            run_gpu(m)
+            r_off, r_ids = results()
+            rec = macro_recall(g_off, g_ids, r_off, r_ids)
+            sweep[pname].append({"m": m, "recall": round(rec, 5), "avg_results": round(len(r_ids) / nq, 2)})
+            log(f"  {pname} m={m}: macro recall {rec:.4f} (FP64 groundtruth)")
+            if m_star is None and rec >= args.recall_target:
+                m_star = m
+        if m_star is None:
+            continue
+        sec = timed(lambda: step_io(m_star))
+        sec_res = timed(lambda: run_gpu(m_star))
+        rec = [s_["recall"] for s_ in sweep[pname] if s_["m"] == m_star][0]
+        paths[f"{pname}_fixed_m"] = {"m": m_star, "recall": rec, "vectors": pname, "sec": sec,
+                                     "sec_resident": sec_res, "qps": round(nq / sec, 1),
+                                     "qps_resident": round(nq / sec_res, 1), "ms_per_step": round(sec * 1e3, 3)}
+        log(f"  path {pname}_fixed_m: m={m_star}, {nq / sec:,.0f} QPS with host I/O ({nq / sec_res:,.0f} resident)")
+    if not paths:
+        raise RuntimeError("no m reaches the recall target")
+    best_name = max(paths, key=lambda kk_: paths[kk_]["qps"])
+    best = paths[best_name]
+
+    # ---- roofline: phase A and phases B+C of the best path, HIP events on the library stream ----
+    s_v = 4 if best_name.startswith("fp32") else 2
+    ck(L.hrnn_dev_set_vector_precision(h, 0 if s_v == 4 else 1))
+    m_b = best["m"]
+    reps = 10
+    e0, e1 = hiprt.Event(), hiprt.Event()
+
+    def ev_ms(fn):
+        fn()
+        hiprt.stream_sync(stream)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.sync()
+        return e1.ms_since(e0) / reps
+
+    beam_ms = ev_ms(lambda: ck(L.hrnn_dev_proxies_dptr(h, qd.ptr, nq, m_b, max(ef, m_b), pd.ptr, ed.ptr)))
+    ex = ed.to_host().reshape(nq, 2).astype(np.int64)
+    beam_bytes = float((4 * d + ex[:, 0] * 4 * (2 * cfg["M"] + 1) + ex[:, 1] * s_v * d + 4 * m_b).sum())
+    search_ms = ev_ms(lambda: ck(L.hrnn_dev_rknn_search_dptr(h, qd.ptr, nq, pd.ptr, m_b, None, k, Kp, 1.0, cd.ptr,
+                                                            idd.ptr, cap, wd.ptr)))
+    wk = wd.to_host().reshape(nq, 2).astype(np.int64)
+    rc_ = cd.to_host().astype(np.int64)
+    search_bytes = float((4 * d + 16 * m_b + 4 * wk[:, 0] + wk[:, 1] * (s_v * d + 4) + 4 * rc_).sum())
+    kern = {"k_phase_a": {"ms": beam_ms, "bytes": beam_bytes,
+                          "expansions_per_query": round(float(ex[:, 0].mean()), 1),
+                          "evaluations_per_query": round(float(ex[:, 1].mean()), 1)},
+            "k_rknn_search": {"ms": search_ms, "bytes": search_bytes,
+                              "scanned_per_query": round(float(wk[:, 0].mean()), 1),
+                              "candidates_per_query": round(float(wk[:, 1].mean()), 1)}}
+    for v in kern.values():
+        v["gbs"] = v["bytes"] / (v["ms"] * 1e-3) / 1e9
+    dom = max(kern, key=lambda kk_: kern[kk_]["ms"])
+    achieved = kern[dom]["gbs"]
+
+    # ---- cpu baseline: the reference's searchRknn on this run's index --------------------------
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            import oracle
+            if oracle.ref_available():
+                shm = "/dev/shm" if os.path.isdir("/dev/shm") else None
+                tmpdir = tempfile.mkdtemp(prefix="hrnn_bench_", dir=shm)
+                path = os.path.join(tmpdir, "c3.idx")
+                ix.save(path)
+                r = oracle.RefIndex.load(path, d)
+                os.remove(path)
+                os.rmdir(tmpdir)
+                t1 = time.perf_counter()
+                r_off, r_lab = r.search(q, best["m"], 1.0, k, Kp, ef, nthreads=threads)
+                all_s = time.perf_counter() - t1
+                r.close()
+                cpu = {"value": round(nq / all_s, 2), "unit": "queries/s", "cores": threads, "kind": "reference",
+                       "sample": f"all {nq} queries, reference searchRknn at m={best['m']}, K'={Kp}, ef={ef}, "
+                                 f"OpenMP over queries on {threads} host threads, on this run's index loaded "
+                                 f"from its saved file",
+                       "recall_fp64_groundtruth": round(macro_recall(g_off, g_ids, r_off.astype(np.int64),
+                                                                     r_lab.astype(np.uint32)), 5)}
+                log(f"cpu baseline: {cpu['value']} QPS on {threads} threads, recall {cpu['recall_fp64_groundtruth']}")
+        except Exception as e:
+            log(f"cpu baseline skipped: {e}")
+
+    out = {
+        "metric": "RkNN QPS at recall>=0.95 (k=10), precision form 1M x 960 (recall against the FP64 judge)",
+        "value": best["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": best["ms_per_step"], "resident_qps": best["qps_resident"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if s_v == 4 else "bf16", "data": "synthetic",
+        "config": {"workload": f"hrnn-c3-{n // 1000}k-x{d}", "n": n, "nq": nq, "d": d, "K_knng": K, "M": cfg["M"],
+                   "ef_construction": cfg["efc"], "nn_iters": cfg["iters"], "nn_sample": cfg["S"], "k": k,
+                   "K_prime": Kp, "ef_search": ef, "path": best_name, "recall": best["recall"],
+                   "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                     "kernel_ms": round(kern[dom]["ms"], 4), "alg_bytes_per_launch": int(kern[dom]["bytes"]),
+                     "kernels": {kk_: {k2: (round(v2, 4) if isinstance(v2, float) else v2) for k2, v2 in v.items()}
+                                 for kk_, v in kern.items()}},
+        "cpu_baseline": cpu,
+        "paths": paths, "sweep": sweep,
+        "precision": {"judge": "FP64 (sequential sum, no fused multiply-add; the oracle's orc_l2sqr_f64)",
+                      "per_format": prec_rep,
+                      "fp32_rk_rel_err_vs_fp64": {"max": float(rel32.max()), "p99": float(np.quantile(rel32, 0.99))},
+                      "rknn": rknn_rep, "oracle_check": oracle_check},
+        "bruteforce_cpu_baseline": bf_cpu,
+        "build_rooflines": {"bruteforce_rk_bf16": {"s": prec_rep["bf16"]["seconds"],
+                                                   "tflops_first_pass": prec_rep["bf16"]["tflops_first_pass"],
+                                                   "frac_bf16_mfma": round(prec_rep["bf16"]["tflops_first_pass"]
+                                                                           / 2500.0, 4)},
+                            "bruteforce_rk_f16": {"s": prec_rep["f16"]["seconds"],
+                                                  "tflops_first_pass": prec_rep["f16"]["tflops_first_pass"],
+                                                  "frac_f16_mfma": round(prec_rep["f16"]["tflops_first_pass"]
+                                                                         / 2500.0, 4)}},
+        "nndescent_build": {"seconds": round(nnd_s, 3), "iters": cfg["iters"], "sample": cfg["S"], "K": K,
+                            "knn_recall_at10_sample": round(knn_rec10, 4),
+                            "per_iter": [{k2: (round(v, 2) if isinstance(v, float) else v) for k2, v in s_.items()}
+                                         for s_ in nst]},
+        "hnsw_build": {"seconds": round(hnsw_s, 3), "device_ms": round(hnsw_st["ms"], 1),
+                       "batches": int(hnsw_st["batches"]), "expansions": int(hnsw_st["expansions"]),
+                       "evaluations": int(hnsw_st["evaluations"])},
+        "stages_s": {"hnsw_gpu": round(hnsw_s, 2), "nndescent_gpu": round(nnd_s, 3), "rknn_csr_gpu": round(csr_s, 3),
+                     "rk_bruteforce_gpu_bf16": prec_rep["bf16"]["seconds"], "groundtruth_gpu": round(gt_s, 3)},
     }
     print(json.dumps(out), file=json_out, flush=True)
     return 0

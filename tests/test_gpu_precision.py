@@ -103,3 +103,49 @@ def test_f64_recheck_rknn_boundary_queries(prec_case):
         f32 = set(f_ids[f_off[i]:f_off[i + 1]].tolist())
         assert (int(pb[i]) in got) == bool(inb[i]) and (int(pb[i]) in f32) != bool(inb[i]), i
     assert flipped >= len(qb) and rechecked >= flipped
+
+
+def test_nndescent_d960_same_seed_graph_vs_reference():
+    """NNDescent on the precision form's data (d = 960, float, the chunked
+    BF16-staged join with FP32 distances) against the compiled reference's
+    own refineKNNG (hrnn.h:1725-1862) from the same seed graph: 20,000
+    points, M=16, ef_c=200, K=200, 10 x S=50.  KNN recall at 10 and at K
+    (500-node sample, exact FP32 neighbours) within 0.5 points."""
+    from hrnn_amd.index import Index
+    import os
+    import tempfile
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built")
+    n, K = 20_000, 200
+    oracle.set_threads(16)
+    x = hrnn_amd.synth_halfnormal_scaled(11, 55, n, 960)
+    ix = Index(n, 960, 16, 200, 100, K)
+    ix.add_points(x, labels=np.arange(n, dtype=np.uint64), nthreads=16)
+    with tempfile.TemporaryDirectory() as tmp:
+        p = os.path.join(tmp, "seed.idx")
+        ix.save(p)
+        ref = oracle.RefIndex.load(p, 960)
+    ref.K = K
+    lab = ix.labels()
+    st = ix.refine(10, 50)
+    assert sum(s["dropped"] for s in st) == 0
+    ref.refine(10, 50, nthreads=16)
+    samp = np.random.default_rng(1).choice(n, 500, replace=False)
+    ex, _ = oracle.knn(x, x[samp], K + 1)
+    ex = ex[:, 1:].copy()
+
+    def rec(ids, cnt, kk):
+        g = np.zeros_like(ids)
+        live = np.where(np.arange(ids.shape[1]) < cnt[:, None], ids, 0)
+        g[lab] = lab[live]
+        c = np.zeros_like(cnt)
+        c[lab] = cnt
+        return oracle.knng_recall(g[samp], c[samp], K, ex, kk)
+
+    gid, _, gcnt = ix.get_knng()
+    rid, _, rcnt = ref.get_knng()
+    ref.close()
+    for kk in (10, K):
+        ours, theirs = rec(gid, gcnt, kk), rec(rid, rcnt, kk)
+        print(f"d=960 KNN recall@{kk}: GPU {ours:.5f} reference {theirs:.5f}", flush=True)
+        assert abs(ours - theirs) <= 0.005, (kk, ours, theirs)
