@@ -28,7 +28,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
 HIP_SOURCES = ["api.hip", "csr.hip", "search.hip", "bruteforce.hip", "nndescent.hip",
-               "mlp.hip", "beam.hip", "hnsw_build.hip"]
+               "mlp.hip", "beam.hip", "hnsw_build.hip", "insert.hip"]
 HOST_SOURCES = ["synth.cpp"]
 
 HIP_FLAGS = [

@@ -288,6 +288,7 @@ static void set_base_common(hrnn_dev_t h, const float *x, uint64_t n, uint32_t d
     h->K = 0;
     h->csr_valid = false;
     h->hnsw_valid = false;
+    h->hnsw_rows = 0;
     h->beam_vis.release();     // sized by n
     h->search_mark.release();
     h->search_mark2.release();
@@ -358,6 +359,7 @@ static int *tuning_slot(hrnn_dev_t h, const char *key) {
     if (k == "stage1_cap") return &h->tune_stage1_cap;
     if (k == "hnsw_batch_permille") return &h->tune_hnsw_batch_permille;
     if (k == "hnsw_seed_exp") return &h->tune_hnsw_seed_exp;
+    if (k == "insert_offer_cap") return &h->tune_insert_offer_cap;
     return nullptr;
 }
 
@@ -367,7 +369,9 @@ int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value) {
     const bool cap = slot == &h->tune_stage1_cap;
     const bool permille = slot == &h->tune_hnsw_batch_permille;
     const bool seedx = slot == &h->tune_hnsw_seed_exp;
-    if (!slot || value < (cap || seedx ? 0 : 1) || value > (cap ? 1024 : permille || seedx ? 1000 : 64)) {
+    const bool offers = slot == &h->tune_insert_offer_cap;
+    if (!slot || value < (cap || seedx || offers ? 0 : 1) ||
+        value > (offers ? (1 << 30) : cap ? 1024 : permille || seedx ? 1000 : 64)) {
         h->err = std::string("set_tuning: unknown key or value out of range: ") + (key ? key : "");
         return HRNN_E_ARG;
     }

@@ -560,6 +560,7 @@ int hrnn_dev_set_hnsw(hrnn_dev_t h, uint32_t maxM0, uint32_t maxM, const uint32_
         h->entry = entry;
         h->maxlevel = maxlevel;
         h->hnsw_valid = true;
+        h->hnsw_rows = n;
     });
 }
 

@@ -1,5 +1,7 @@
 // handle.h -- the per-index device state behind an hrnn_dev_t.
 #pragma once
+#include <functional>
+
 #include "common.h"
 
 struct hrnn_dev {
@@ -47,6 +49,7 @@ struct hrnn_dev {
     int tune_stage1_cap = 0;         // learned early stop: stage-1 expansion cap (0: none)
     int tune_hnsw_batch_permille = 50;   // HNSW construction: batch size / graph size, per mille
     int tune_hnsw_seed_exp = 32;         // HNSW construction: expansions of the seed completion
+    int tune_insert_offer_cap = 0;       // incremental insertion: first offer buffer, entries (0: max(2^22, 8n))
 
     // ranked KNN graph (SoA); knn_version changes whenever it may have changed
     uint32_t K = 0;
@@ -79,6 +82,9 @@ struct hrnn_dev {
     // word + ids), per-node offset into the upper-layer pool (blocks of maxM+1
     // words per level, 0xFFFFFFFF for level-0-only nodes)
     bool hnsw_valid = false;
+    // rows the resident graph links (appending base rows leaves the graph of
+    // the earlier rows in place for an incremental insertion)
+    uint64_t hnsw_rows = 0;
     uint32_t l0_words = 0, up_words = 0, entry = 0;
     uint64_t up_pool_words = 0;
     int maxlevel = 0;
@@ -170,6 +176,22 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
                           uint32_t cap_per_q, uint32_t *work = nullptr,
                           const SearchSel *sel = nullptr);
 void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t bucket, hrnn_nnd_iter_stats *st);
+// Top-K merge of per-node proposals into the resident ranked lists (the
+// NNDescent merge): node u takes prop_cnt[u] keys (order-preserving bits of
+// d2 << 32 | id) from prop_key[prop_off[u] ..), keys at or past cap dropped;
+// stats: the NNDescent counter block (updates counted).
+void nnd_merge_launch(hrnn_dev_t h, uint64_t n, const uint32_t *prop_cnt, const uint64_t *prop_off,
+                      const unsigned long long *prop_key, uint64_t cap, unsigned long long *stats);
+// NNDescent counter block size (u64 words) and the updates it counted.
+uint64_t nnd_stat_words();
+uint64_t nnd_stat_updates(const unsigned long long *host_block);
+// Batched HNSW insertion with KNN seeding of the resident rows [b_first, n):
+// b_first = 0 builds the graph from scratch; otherwise the resident graph
+// must link exactly the rows [0, b_first) and its buffers grow (the KNN lists
+// too, new rows empty).  levels[n] for every row.  after(b, b1) runs on the
+// stream after each batch's links and seeds are in place.
+void hnsw_insert_rows(hrnn_dev_t h, const int *levels, uint64_t b_first, uint32_t M, uint32_t ef_construction,
+                      uint32_t K, hrnn_hnsw_build_stats *stats, const std::function<void(uint32_t, uint32_t)> &after);
 // Look up, with the handle's device current, this device's copy of the
 // no-allocation search's scratch array (once per device; called by
 // hrnn_dev_open) and clear it.

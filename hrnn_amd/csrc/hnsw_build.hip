@@ -663,57 +663,84 @@ __global__ void k_zero_u32(uint32_t *__restrict__ p, uint32_t count) {
 // Visited-set slots for a construction beam of width ef (as phase A's sizing).
 static uint32_t build_log_vis(uint32_t ef) { return ef <= 16 ? 10u : ef <= 256 ? 11u : 12u; }
 
-}  // namespace hrnn
+// Copy the first keep bytes of b into a buffer of at least want bytes.
+static void grow_keep(hrnn_dev_t h, DevBuf &b, size_t keep, size_t want) {
+    if (want <= b.bytes) return;
+    DevBuf nb;
+    nb.reserve(want);
+    if (keep) HIPC(hipMemcpyAsync(nb.p, b.p, keep, hipMemcpyDeviceToDevice, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    b.release();
+    b = nb;
+}
 
-using namespace hrnn;
-
-extern "C" {
-
-int hrnn_dev_hnsw_build(hrnn_dev_t h, const int *levels, uint32_t M, uint32_t ef_construction, uint32_t K,
-                        uint32_t *entry_out, int *maxlevel_out, hrnn_hnsw_build_stats *stats) {
-    if (!h) return HRNN_E_ARG;
-    return guarded(h, [&] {
-        const auto t_start = std::chrono::steady_clock::now();
-        REQUIRE(h->n > 0, HRNN_E_STATE, "hnsw_build: set_base first");
-        REQUIRE(levels && entry_out && maxlevel_out, HRNN_E_ARG, "hnsw_build: null argument");
-        REQUIRE(M >= 2 && 2 * M <= (uint32_t)kBeamMaxDeg && M <= kMaxSelect, HRNN_E_ARG,
-                "hnsw_build: M must be in [2, 32]");
-        const uint32_t ef = std::max(ef_construction, M);
-        REQUIRE(ef <= (uint32_t)kBeamMaxEf, HRNN_E_ARG, "hnsw_build: ef_construction must be <= 512");
-        REQUIRE(K <= 65535, HRNN_E_ARG, "hnsw_build: K must fit in 16 bits");
-        REQUIRE(h->n < kInvalidId, HRNN_E_ARG, "hnsw_build: n must be below 2^32 - 1");
-        REQUIRE(h->d >= 1 && h->d <= 4096, HRNN_E_ARG, "hnsw_build: d must be in [1, 4096]");
-        const uint32_t n = (uint32_t)h->n, d = h->d;
-        const uint32_t l0_words = 2 * M + 1, up_words = M + 1;
-        // upper-layer blocks from the levels; the top level and entry point as
-        // sequential insertion leaves them
-        std::vector<uint32_t> up_off(n, kInvalidId);
-        uint64_t pool = 0;
-        int top = 0;
-        for (uint32_t i = 0; i < n; i++) {
-            REQUIRE(levels[i] >= 0 && levels[i] <= 255, HRNN_E_ARG, "hnsw_build: level out of [0, 255]");
-            if (levels[i] > 0) {
-                up_off[i] = (uint32_t)pool;
-                pool += (uint64_t)levels[i] * up_words;
-            }
-            top = std::max(top, levels[i]);
+void hnsw_insert_rows(hrnn_dev_t h, const int *levels, uint64_t b_first, uint32_t M, uint32_t ef_construction,
+                      uint32_t K, hrnn_hnsw_build_stats *stats, const std::function<void(uint32_t, uint32_t)> &after) {
+    const auto t_start = std::chrono::steady_clock::now();
+    REQUIRE(h->n > 0, HRNN_E_STATE, "hnsw_build: set_base first");
+    REQUIRE(levels, HRNN_E_ARG, "hnsw_build: null argument");
+    REQUIRE(M >= 2 && 2 * M <= (uint32_t)kBeamMaxDeg && M <= kMaxSelect, HRNN_E_ARG,
+            "hnsw_build: M must be in [2, 32]");
+    const uint32_t ef = std::max(ef_construction, M);
+    REQUIRE(ef <= (uint32_t)kBeamMaxEf, HRNN_E_ARG, "hnsw_build: ef_construction must be <= 512");
+    REQUIRE(K <= 65535, HRNN_E_ARG, "hnsw_build: K must fit in 16 bits");
+    REQUIRE(h->n < kInvalidId, HRNN_E_ARG, "hnsw_build: n must be below 2^32 - 1");
+    REQUIRE(h->d >= 1 && h->d <= 4096, HRNN_E_ARG, "hnsw_build: d must be in [1, 4096]");
+    const uint32_t n = (uint32_t)h->n, d = h->d;
+    const uint32_t l0_words = 2 * M + 1, up_words = M + 1;
+    const bool extend = b_first > 0;
+    const uint32_t n_old = (uint32_t)b_first;
+    if (extend) {
+        REQUIRE(h->hnsw_rows == b_first && b_first < h->n, HRNN_E_STATE,
+                "insert: the device graph must hold exactly the rows before the first new one");
+        REQUIRE(h->l0_words == l0_words && h->up_words == up_words, HRNN_E_ARG,
+                "insert: M differs from the resident graph's");
+        REQUIRE(K == h->K && K > 0, HRNN_E_STATE, "insert: the resident KNN graph has another K");
+    }
+    // upper-layer blocks of the new rows from their levels, after the
+    // resident pool; the top level over all rows
+    std::vector<uint32_t> up_off(n - n_old, kInvalidId);
+    uint64_t pool = extend ? h->up_pool_words : 0;
+    const uint64_t pool_old = pool;
+    int top = extend ? h->maxlevel : 0;
+    for (uint32_t i = 0; i < n; i++)
+        REQUIRE(levels[i] >= 0 && levels[i] <= 255, HRNN_E_ARG, "hnsw_build: level out of [0, 255]");
+    for (uint32_t i = n_old; i < n; i++) {
+        if (levels[i] > 0) {
+            up_off[i - n_old] = (uint32_t)pool;
+            pool += (uint64_t)levels[i] * up_words;
         }
-        REQUIRE(pool < kInvalidId, HRNN_E_ARG, "hnsw_build: upper layers exceed 2^32 words");
-        REQUIRE(top < 100, HRNN_E_ARG, "hnsw_build: more than 100 layers");
-        const uint32_t prop_stride = M * (uint32_t)(top + 1);
-        uint32_t frac = (uint32_t)std::max(1, h->tune_hnsw_batch_permille);
-        const uint32_t seed_exp = (uint32_t)std::max(0, h->tune_hnsw_seed_exp);
-        const uint32_t bmax = 65536;
+        top = std::max(top, levels[i]);
+    }
+    REQUIRE(pool < kInvalidId, HRNN_E_ARG, "hnsw_build: upper layers exceed 2^32 words");
+    REQUIRE(top < 100, HRNN_E_ARG, "hnsw_build: more than 100 layers");
+    const uint32_t prop_stride = M * (uint32_t)(top + 1);
+    uint32_t frac = (uint32_t)std::max(1, h->tune_hnsw_batch_permille);
+    const uint32_t seed_exp = (uint32_t)std::max(0, h->tune_hnsw_seed_exp);
+    const uint32_t bmax = 65536;
 
-        hipStream_t st = h->stream;
+    hipStream_t st = h->stream;
+    if (extend) {
+        grow_keep(h, h->l0, (uint64_t)n_old * l0_words * 4, (uint64_t)n * l0_words * 4);
+        grow_keep(h, h->up_off, (uint64_t)n_old * 4, (uint64_t)n * 4);
+        grow_keep(h, h->up_pool, pool_old * 4, std::max<uint64_t>(pool, 1) * 4);
+        grow_keep(h, h->knn_ids, (uint64_t)n_old * K * 4, (uint64_t)n * K * 4);
+        grow_keep(h, h->knn_d, (uint64_t)n_old * K * 4, (uint64_t)n * K * 4);
+        grow_keep(h, h->knn_cnt, (uint64_t)n_old * 4, (uint64_t)n * 4);
+        grow_keep(h, h->knn_new, (uint64_t)n_old * K, (uint64_t)n * K);
+        HIPC(hipMemsetAsync(h->l0.as<uint32_t>() + (uint64_t)n_old * l0_words, 0,
+                            (uint64_t)(n - n_old) * l0_words * 4, st));
+        if (pool > pool_old)
+            HIPC(hipMemsetAsync(h->up_pool.as<uint32_t>() + pool_old, 0, (pool - pool_old) * 4, st));
+        HIPC(hipMemsetAsync(h->knn_cnt.as<uint32_t>() + n_old, 0, (uint64_t)(n - n_old) * 4, st));
+        HIPC(hipMemsetAsync(h->knn_new.as<uint8_t>() + (uint64_t)n_old * K, 1, (uint64_t)(n - n_old) * K, st));
+        h->knn_version++;
+    } else {
         h->l0.reserve((uint64_t)n * l0_words * 4);
         h->up_off.reserve((uint64_t)n * 4);
         h->up_pool.reserve(std::max<uint64_t>(pool, 1) * 4);
-        h->levels_dev.reserve((uint64_t)n * 4);
         HIPC(hipMemsetAsync(h->l0.p, 0, (uint64_t)n * l0_words * 4, st));
         if (pool) HIPC(hipMemsetAsync(h->up_pool.p, 0, pool * 4, st));
-        h2d(h, h->up_off.p, up_off.data(), (uint64_t)n * 4);
-        h2d(h, h->levels_dev.p, levels, (uint64_t)n * 4);
         if (K > 0) {
             h->K = K;
             h->knn_ids.reserve((uint64_t)n * K * 4);
@@ -724,54 +751,85 @@ int hrnn_dev_hnsw_build(hrnn_dev_t h, const int *levels, uint32_t M, uint32_t ef
             HIPC(hipMemsetAsync(h->knn_new.p, 1, (uint64_t)n * K, st));
             h->knn_version++;
         }
-        h->csr_valid = false;
-        h->hnsw_valid = false;
-        // batch buffers: proposals, incoming links per node, touched lists per layer
-        h->hb_prop.reserve((uint64_t)bmax * prop_stride * 8);
-        h->hb_inc_cnt.reserve((uint64_t)n * 4);
-        h->hb_inc.reserve((uint64_t)n * kIncMax * 8);
-        h->hb_touched.reserve((uint64_t)bmax * M * 4);
-        h->hb_ctr.reserve(64 * 8);
-        HIPC(hipMemsetAsync(h->hb_inc_cnt.p, 0, (uint64_t)n * 4, st));
-        HIPC(hipMemsetAsync(h->hb_ctr.p, 0, 64 * 8, st));
-        // [0] expansions [1] evaluations [2] dropped links [3] [4] seed completion expansions, evaluations
-        unsigned long long *ctr = h->hb_ctr.as<unsigned long long>();
-        uint32_t *n_touched = reinterpret_cast<uint32_t *>(ctr + 8);
-        uint32_t *err = h->counters.as<uint32_t>() + kErrWord;
+    }
+    h->levels_dev.reserve((uint64_t)n * 4);
+    h2d(h, h->up_off.as<uint32_t>() + n_old, up_off.data(), (uint64_t)(n - n_old) * 4);
+    h2d(h, h->levels_dev.p, levels, (uint64_t)n * 4);
+    h->csr_valid = false;
+    h->hnsw_valid = false;
+    h->hnsw_rows = 0;
+    // batch buffers: proposals, incoming links per node, touched lists per layer
+    h->hb_prop.reserve((uint64_t)bmax * prop_stride * 8);
+    h->hb_inc_cnt.reserve((uint64_t)n * 4);
+    h->hb_inc.reserve((uint64_t)n * kIncMax * 8);
+    h->hb_touched.reserve((uint64_t)bmax * M * 4);
+    h->hb_ctr.reserve(64 * 8);
+    HIPC(hipMemsetAsync(h->hb_inc_cnt.p, 0, (uint64_t)n * 4, st));
+    HIPC(hipMemsetAsync(h->hb_ctr.p, 0, 64 * 8, st));
+    // [0] expansions [1] evaluations [2] dropped links [3] [4] seed completion expansions, evaluations
+    unsigned long long *ctr = h->hb_ctr.as<unsigned long long>();
+    uint32_t *n_touched = reinterpret_cast<uint32_t *>(ctr + 8);
+    uint32_t *err = h->counters.as<uint32_t>() + kErrWord;
 
-        BuildGraph G;
-        G.xf = h->x.as<float>();
-        G.d = d;
-        G.n = n;
-        G.l0 = h->l0.as<uint32_t>();
-        G.l0_words = l0_words;
-        G.up_off = h->up_off.as<uint32_t>();
-        G.up_pool = h->up_pool.as<uint32_t>();
-        G.pool_words = pool;
-        G.up_words = up_words;
-        G.levels = h->levels_dev.as<int>();
+    BuildGraph G;
+    G.xf = h->x.as<float>();
+    G.d = d;
+    G.n = n;
+    G.l0 = h->l0.as<uint32_t>();
+    G.l0_words = l0_words;
+    G.up_off = h->up_off.as<uint32_t>();
+    G.up_pool = h->up_pool.as<uint32_t>();
+    G.pool_words = pool;
+    G.up_words = up_words;
+    G.levels = h->levels_dev.as<int>();
 
-        const bool use_u8 = h->base_u8 != 0;
-        const size_t vbytes = use_u8 ? 1 : h->small_int ? 2 : 4;
-        const int base_exact =
-            (h->base_u8 && d <= 258 && d % 8 == 0 && (uint64_t)n * d * vbytes < 0xFFFFFFF0ull) ? 1 : 0;
-        const uint32_t ef_cap = (ef + 3u) & ~3u;
-        const uint32_t log_vis = build_log_vis(ef);
-        const BeamLds lay(d, ef_cap, log_vis);
-        REQUIRE(lay.total <= 65536, HRNN_E_ARG, "hnsw_build: LDS need exceeds 64 KiB");
+    const bool use_u8 = h->base_u8 != 0;
+    const size_t vbytes = use_u8 ? 1 : h->small_int ? 2 : 4;
+    const int base_exact =
+        (h->base_u8 && d <= 258 && d % 8 == 0 && (uint64_t)n * d * vbytes < 0xFFFFFFF0ull) ? 1 : 0;
+    const uint32_t ef_cap = (ef + 3u) & ~3u;
+    const uint32_t log_vis = build_log_vis(ef);
+    const BeamLds lay(d, ef_cap, log_vis);
+    REQUIRE(lay.total <= 65536, HRNN_E_ARG, "hnsw_build: LDS need exceeds 64 KiB");
 
-        uint32_t entry = 0;
-        int maxlevel = levels[0];
-        uint64_t batches = 0;
-        for (uint32_t b = 1; b < n;) {
-            const uint32_t B = std::min<uint32_t>(bmax, std::max<uint32_t>(1, (uint32_t)((uint64_t)b * frac / 1000)));
-            const uint32_t b1 = std::min<uint32_t>(n, b + B), nb = b1 - b;
-            int lb = 0;
-            for (uint32_t i = b; i < b1; i++) lb = std::max(lb, std::min(levels[i], maxlevel));
+    uint32_t entry = extend ? h->entry : 0;
+    int maxlevel = extend ? h->maxlevel : levels[0];
+    uint64_t batches = 0;
+    for (uint32_t b = extend ? n_old : 1; b < n;) {
+        const uint32_t B = std::min<uint32_t>(bmax, std::max<uint32_t>(1, (uint32_t)((uint64_t)b * frac / 1000)));
+        const uint32_t b1 = std::min<uint32_t>(n, b + B), nb = b1 - b;
+        int lb = 0;
+        for (uint32_t i = b; i < b1; i++) lb = std::max(lb, std::min(levels[i], maxlevel));
 #define LAUNCH(VT, XP)                                                                                       \
     hipLaunchKernelGGL((k_hnsw_insert<VT>), dim3(nb), dim3(64), lay.total, st, XP, G, b, b1, entry, maxlevel, ef, \
                        ef_cap, log_vis, M, base_exact, K, h->knn_ids.as<uint32_t>(), h->knn_d.as<float>(),     \
                        h->knn_cnt.as<uint32_t>(), h->hb_prop.as<unsigned long long>(), prop_stride, err, ctr)
+        if (use_u8) {
+            LAUNCH(uint8_t, h->xu8.as<uint8_t>());
+        } else if (h->small_int) {
+            LAUNCH(__hip_bfloat16, h->xb.as<__hip_bfloat16>());
+        } else {
+            LAUNCH(float, h->x.as<float>());
+        }
+#undef LAUNCH
+        HIPC(hipGetLastError());
+        for (int L = 0; L <= lb; L++) {
+            const uint32_t cap = L ? M : 2 * M;
+            uint32_t *nt = n_touched + L;
+            hipLaunchKernelGGL(k_hnsw_scatter, dim3(ceil_div((uint64_t)nb * M, 256)), dim3(256), 0, st,
+                               h->hb_prop.as<unsigned long long>(), nb, prop_stride, M, L, b, n,
+                               h->hb_inc_cnt.as<uint32_t>(), h->hb_inc.as<unsigned long long>(),
+                               h->hb_touched.as<uint32_t>(), nt, ctr + 2, err);
+            hipLaunchKernelGGL(k_hnsw_link, dim3(std::min<uint64_t>((uint64_t)nb * M, n)), dim3(64), 0, st, G, L,
+                               cap, h->hb_touched.as<uint32_t>(), nt, h->hb_inc_cnt.as<uint32_t>(),
+                               h->hb_inc.as<unsigned long long>(), err);
+            HIPC(hipGetLastError());
+        }
+        if (K > 0 && seed_exp > 0) {
+#define LAUNCH(VT, XP)                                                                                       \
+    hipLaunchKernelGGL((k_hnsw_seed_fix<VT>), dim3(nb), dim3(64), lay.total, st, XP, G, b, b1, ef, ef_cap,   \
+                       log_vis, base_exact, K, seed_exp, h->knn_ids.as<uint32_t>(), h->knn_d.as<float>(),    \
+                       h->knn_cnt.as<uint32_t>(), err, ctr)
             if (use_u8) {
                 LAUNCH(uint8_t, h->xu8.as<uint8_t>());
             } else if (h->small_int) {
@@ -781,64 +839,54 @@ int hrnn_dev_hnsw_build(hrnn_dev_t h, const int *levels, uint32_t M, uint32_t ef
             }
 #undef LAUNCH
             HIPC(hipGetLastError());
-            for (int L = 0; L <= lb; L++) {
-                const uint32_t cap = L ? M : 2 * M;
-                uint32_t *nt = n_touched + L;
-                hipLaunchKernelGGL(k_hnsw_scatter, dim3(ceil_div((uint64_t)nb * M, 256)), dim3(256), 0, st,
-                                   h->hb_prop.as<unsigned long long>(), nb, prop_stride, M, L, b, n,
-                                   h->hb_inc_cnt.as<uint32_t>(), h->hb_inc.as<unsigned long long>(),
-                                   h->hb_touched.as<uint32_t>(), nt, ctr + 2, err);
-                hipLaunchKernelGGL(k_hnsw_link, dim3(std::min<uint64_t>((uint64_t)nb * M, n)), dim3(64), 0, st, G, L,
-                                   cap, h->hb_touched.as<uint32_t>(), nt, h->hb_inc_cnt.as<uint32_t>(),
-                                   h->hb_inc.as<unsigned long long>(), err);
-                HIPC(hipGetLastError());
-            }
-            if (K > 0 && seed_exp > 0) {
-#define LAUNCH(VT, XP)                                                                                       \
-    hipLaunchKernelGGL((k_hnsw_seed_fix<VT>), dim3(nb), dim3(64), lay.total, st, XP, G, b, b1, ef, ef_cap,   \
-                       log_vis, base_exact, K, seed_exp, h->knn_ids.as<uint32_t>(), h->knn_d.as<float>(),    \
-                       h->knn_cnt.as<uint32_t>(), err, ctr)
-                if (use_u8) {
-                    LAUNCH(uint8_t, h->xu8.as<uint8_t>());
-                } else if (h->small_int) {
-                    LAUNCH(__hip_bfloat16, h->xb.as<__hip_bfloat16>());
-                } else {
-                    LAUNCH(float, h->x.as<float>());
-                }
-#undef LAUNCH
-                HIPC(hipGetLastError());
-            }
-            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, n_touched, (uint32_t)(lb + 1));
-            HIPC(hipGetLastError());
-            for (uint32_t i = b; i < b1; i++)
-                if (levels[i] > maxlevel) {
-                    maxlevel = levels[i];
-                    entry = i;
-                }
-            b = b1;
-            batches++;
         }
-        HIPC(hipStreamSynchronize(st));
-        unsigned long long hc[5] = {0, 0, 0, 0, 0};
-        d2h(h, hc, ctr, sizeof(hc));
-        check_device_error(h);
-        h->l0_words = l0_words;
-        h->up_words = up_words;
-        h->up_pool_words = pool;
-        h->entry = entry;
-        h->maxlevel = maxlevel;
-        h->hnsw_valid = true;
-        *entry_out = entry;
-        *maxlevel_out = maxlevel;
-        if (stats) {
-            stats->batches = batches;
-            stats->expansions = hc[0];
-            stats->evaluations = hc[1];
-            stats->dropped_links = hc[2];
-            stats->seed_expansions = hc[3];
-            stats->seed_evaluations = hc[4];
-            stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-        }
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, n_touched, (uint32_t)(lb + 1));
+        HIPC(hipGetLastError());
+        for (uint32_t i = b; i < b1; i++)
+            if (levels[i] > maxlevel) {
+                maxlevel = levels[i];
+                entry = i;
+            }
+        if (after) after(b, b1);
+        b = b1;
+        batches++;
+    }
+    HIPC(hipStreamSynchronize(st));
+    unsigned long long hc[5] = {0, 0, 0, 0, 0};
+    d2h(h, hc, ctr, sizeof(hc));
+    check_device_error(h);
+    h->l0_words = l0_words;
+    h->up_words = up_words;
+    h->up_pool_words = pool;
+    h->entry = entry;
+    h->maxlevel = maxlevel;
+    h->hnsw_valid = true;
+    h->hnsw_rows = n;
+    if (stats) {
+        stats->batches = batches;
+        stats->expansions = hc[0];
+        stats->evaluations = hc[1];
+        stats->dropped_links = hc[2];
+        stats->seed_expansions = hc[3];
+        stats->seed_evaluations = hc[4];
+        stats->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    }
+}
+
+}  // namespace hrnn
+
+using namespace hrnn;
+
+extern "C" {
+
+int hrnn_dev_hnsw_build(hrnn_dev_t h, const int *levels, uint32_t M, uint32_t ef_construction, uint32_t K,
+                        uint32_t *entry_out, int *maxlevel_out, hrnn_hnsw_build_stats *stats) {
+    if (!h) return HRNN_E_ARG;
+    return guarded(h, [&] {
+        REQUIRE(entry_out && maxlevel_out, HRNN_E_ARG, "hnsw_build: null argument");
+        hnsw_insert_rows(h, levels, 0, M, ef_construction, K, stats, nullptr);
+        *entry_out = h->entry;
+        *maxlevel_out = h->maxlevel;
     });
 }
 

@@ -279,6 +279,17 @@ int hrnn_index_insert_dynamic(hrnn_index_t ix, const float *x, uint64_t n, const
     });
 }
 
+int hrnn_index_insert_dynamic_gpu(hrnn_index_t ix, const float *x, uint64_t n, const uint64_t *labels,
+                                  uint32_t m_update, int bidir, int batch_permille, hrnn_insert_stats *stats) {
+    if (!ix || !x || !labels || batch_permille < 0 || batch_permille > 1000) return HRNN_E_ARG;
+    return guard(ix, [&] {
+        ix->idx->setBuildBatchPermille(batch_permille);
+        std::vector<hnswlib::labeltype> lab(labels, labels + n);
+        ix->idx->insertPointsDynamicDevice(x, n, lab.data(), m_update, bidir != 0);
+        if (stats) *stats = ix->idx->lastInsertStats();
+    });
+}
+
 int hrnn_index_get_rknng_mutable(hrnn_index_t ix, uint64_t *off, uint32_t *ids) {
     if (!ix || !off) return HRNN_E_ARG;
     return guard(ix, [&] {

@@ -950,4 +950,22 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
         }
 }
 
+void nnd_merge_launch(hrnn_dev_t h, uint64_t n, const uint32_t *prop_cnt, const uint64_t *prop_off,
+                      const unsigned long long *prop_key, uint64_t cap, unsigned long long *stats) {
+    REQUIRE(h->K >= 1 && h->K <= kMergeMaxK, HRNN_E_ARG, "merge: K_knng must be in [1, 1024]");
+    if (!n) return;
+    hipLaunchKernelGGL(k_nnd_merge, dim3((unsigned)n), dim3(kMergeThreads), 0, h->stream, h->knn_ids.as<uint32_t>(),
+                       h->knn_d.as<float>(), h->knn_new.as<uint8_t>(), h->knn_cnt.as<uint32_t>(), h->K, prop_cnt,
+                       prop_off, prop_key, cap, stats);
+    HIPC(hipGetLastError());
+}
+
+uint64_t nnd_stat_words() { return (uint64_t)ST_N * kStatSlots; }
+
+uint64_t nnd_stat_updates(const unsigned long long *blk) {
+    uint64_t s = 0;
+    for (uint32_t j = 0; j < kStatSlots; j++) s += blk[(uint64_t)ST_UPDATES * kStatSlots + j];
+    return s;
+}
+
 }  // namespace hrnn

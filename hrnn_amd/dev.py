@@ -33,6 +33,7 @@ EXPORTS = [
     "hrnn_dev_set_vector_precision", "hrnn_dev_set_tuning", "hrnn_dev_get_tuning",
     "hrnn_dev_set_knng", "hrnn_dev_get_knng", "hrnn_dev_get_knng_flags", "hrnn_dev_set_knng_flags",
     "hrnn_dev_knng_ptrs", "hrnn_dev_hnsw_build", "hrnn_dev_get_hnsw", "hrnn_dev_nndescent",
+    "hrnn_dev_insert_dynamic", "hrnn_dev_insert_affect",
     "hrnn_dev_build_rknn_csr", "hrnn_dev_get_rknn_csr", "hrnn_dev_set_rknn_csr",
     "hrnn_dev_rknn_search", "hrnn_dev_rknn_search_dptr", "hrnn_dev_rknn_search_noalloc_dptr", "hrnn_dev_fetch_results", "hrnn_dev_set_hnsw", "hrnn_dev_proxies",
     "hrnn_dev_proxies_dptr", "hrnn_dev_rknn_query_dptr", "hrnn_dev_bf_kth", "hrnn_dev_bf_knn",
@@ -55,6 +56,13 @@ class HnswBuildStats(C.Structure):
     _fields_ = [("batches", C.c_uint64), ("expansions", C.c_uint64), ("evaluations", C.c_uint64),
                 ("dropped_links", C.c_uint64), ("ms", C.c_double), ("seed_expansions", C.c_uint64),
                 ("seed_evaluations", C.c_uint64)]
+
+
+class InsertStats(C.Structure):
+    _fields_ = [("batches", C.c_uint64), ("scanned", C.c_uint64), ("candidates", C.c_uint64),
+                ("offers", C.c_uint64), ("updates", C.c_uint64), ("retries", C.c_uint64), ("ms", C.c_double),
+                ("hnsw_ms", C.c_double), ("csr_ms", C.c_double), ("affect_ms", C.c_double),
+                ("merge_ms", C.c_double), ("hnsw", HnswBuildStats)]
 
 
 class NndIterStats(C.Structure):
@@ -103,6 +111,8 @@ def lib() -> C.CDLL:
         "hrnn_dev_get_hnsw": [vp, vp, vp, u64],
         "hrnn_dev_set_knng_flags": [vp, vp],
         "hrnn_dev_nndescent": [vp, i, i, u32, vp],
+        "hrnn_dev_insert_dynamic": [vp, u64, vp, u32, u32, u32, i, P(u32), P(i), vp],
+        "hrnn_dev_insert_affect": [vp, u64, u64, u32, i, vp],
         "hrnn_dev_build_rknn_csr": [vp, P(u32), P(i), P(u64)],
         "hrnn_dev_get_rknn_csr": [vp, vp, vp],
         "hrnn_dev_set_rknn_csr": [vp, u32, i, u64, vp, vp, u64],
@@ -296,6 +306,31 @@ class Device:
         self._ck(self.L.hrnn_dev_nndescent(self.h, iters, sample, bucket, st))
         return [dict(dists=s.dists, updates=s.updates, proposals=s.proposals,
                      dropped=s.dropped, ms=s.ms, rows=s.rows) for s in st[:iters]]
+
+    # -- incremental insertion --------------------------------------------------
+    @staticmethod
+    def _insert_stats(st):
+        out = {k: getattr(st, k) for k, _ in InsertStats._fields_ if k != "hnsw"}
+        out["hnsw"] = {k: getattr(st.hnsw, k) for k, _ in HnswBuildStats._fields_}
+        return out
+
+    def insert_affect(self, b0: int, b1: int, m_update: int = 0, bidirectional: bool = False):
+        """Phases 2-5 of insertPointDynamic for rows [b0, b1) whose lists hold
+        their seeds (hrnn_dev_insert_affect); returns the call's counters."""
+        st = InsertStats()
+        self._ck(self.L.hrnn_dev_insert_affect(self.h, b0, b1, m_update, int(bidirectional), C.byref(st)))
+        return self._insert_stats(st)
+
+    def insert_dynamic(self, n_old: int, levels, M: int, efc: int, m_update: int = 0, bidirectional: bool = False):
+        """Incremental insertion of the resident rows [n_old, n) (appended) into
+        the resident HNSW graph and KNN lists (hrnn_dev_insert_dynamic);
+        returns (entry, maxlevel, counters)."""
+        lv = np.ascontiguousarray(levels, np.int32)
+        assert lv.shape[0] == self.n
+        e, ml, st = C.c_uint32(0), C.c_int(0), InsertStats()
+        self._ck(self.L.hrnn_dev_insert_dynamic(self.h, n_old, _p(lv), M, efc, m_update, int(bidirectional),
+                                                C.byref(e), C.byref(ml), C.byref(st)))
+        return e.value, ml.value, self._insert_stats(st)
 
     # -- reverse CSR -----------------------------------------------------------
     def build_rknn_csr(self) -> CsrInfo:

@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .dev import HnswBuildStats, HrnnError, NndIterStats, lib as dev_lib
+from .dev import HnswBuildStats, HrnnError, InsertStats, NndIterStats, lib as dev_lib
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 HOST_LIB_PATH = os.path.join(PKG, "lib", "libhrnn_host.so")
@@ -46,6 +46,7 @@ def host_lib() -> C.CDLL:
         "hrnn_index_search": [vp, vp, u64, u32, f, u32, u32, u32, i, i, vp, vp, u64, P(u64)],
         "hrnn_index_search_proxies": [vp, vp, u64, vp, u32, vp, u32, u32, f, vp, vp, u64, P(u64)],
         "hrnn_index_insert_dynamic": [vp, vp, u64, vp, u32, i, i],
+        "hrnn_index_insert_dynamic_gpu": [vp, vp, u64, vp, u32, i, i, vp],
         "hrnn_index_compact_for_serving": [vp, u32],
         "hrnn_index_get_rknng_mutable": [vp, vp, vp],
         "hrnn_index_save": [vp, C.c_char_p],
@@ -235,6 +236,18 @@ class Index:
         labels = np.ascontiguousarray(labels, np.uint64)
         self._ck(self.L.hrnn_index_insert_dynamic(self.h, _p(x), x.shape[0], _p(labels), m_update,
                                                   int(bidirectional), nthreads))
+
+    def insert_dynamic_gpu(self, x, labels, m_update=0, bidirectional=False, batch_permille=0):
+        """insertPointDynamic for all rows of x at once on the GPU (batched);
+        returns the call's counters."""
+        x = np.ascontiguousarray(x, np.float32)
+        labels = np.ascontiguousarray(labels, np.uint64)
+        st = InsertStats()
+        self._ck(self.L.hrnn_index_insert_dynamic_gpu(self.h, _p(x), x.shape[0], _p(labels), m_update,
+                                                      int(bidirectional), batch_permille, C.byref(st)))
+        out = {k: getattr(st, k) for k, _ in InsertStats._fields_ if k != "hnsw"}
+        out["hnsw"] = {k: getattr(st.hnsw, k) for k, _ in HnswBuildStats._fields_}
+        return out
 
     def get_rknng_mutable(self):
         """The mutable reverse lists: (off[n+1] u64, ids u32), each list in its own order."""

@@ -601,6 +601,85 @@ class HRNN : public AlgorithmInterface<dist_t> {
         return xn;
     }
 
+    // Batched incremental insertion on the GPU: insertPointDynamic
+    // (hrnn.h:2360-2534) for nx points, in batches (hrnn_dev_insert_dynamic:
+    // HNSW insertion with KNN seeding, candidates from the reverse lists of
+    // each point's first m_update neighbours, offers merged into the affected
+    // lists, with bidirectional the points' own lists improved).  Needs a
+    // built KNN graph; not for concurrent use with insertPointDynamic.  The
+    // mutable reverse lists, when built, are rebuilt from the new graph; the
+    // flat reverse CSR must be rebuilt (buildRKNNG) before searching.
+    void insertPointsDynamicDevice(const float *X, size_t nx, const labeltype *labels, unsigned m_update = 0,
+                                   bool bidirectional = false) {
+        if (!knng_built_) throw std::runtime_error("insertPointsDynamicDevice: KNNG not built");
+        require_knn("insertPointsDynamicDevice");
+        const size_t n_old = cur_element_count;
+        if (n_old == 0) throw std::runtime_error("insertPointsDynamicDevice: the index is empty (use addPointsDevice)");
+        if (nx == 0) return;
+        if (!X || !labels) throw std::runtime_error("insertPointsDynamicDevice: null argument");
+        if (n_old + nx > max_elements_) throw std::runtime_error("The number of elements exceeds the specified limit");
+        if (maxM0_ > 64 || M_ < 2) throw std::runtime_error("insertPointsDynamicDevice: M must be in [2, 32]");
+        std::lock_guard<std::mutex> g(dev_mu_);
+        // the device holds the base rows, the graph and the lists of [0, n_old)
+        sync_device_base();
+        sync_device_knng();
+        sync_device_hnsw();
+        {
+            std::lock_guard<std::mutex> gl(label_mu_);
+            size_t i = 0;
+            for (; i < nx; i++)
+                if (!label_map_.emplace(labels[i], (tableint)(n_old + i)).second) break;
+            if (i < nx) {
+                for (size_t j = 0; j < i; j++) label_map_.erase(labels[j]);
+                throw std::runtime_error("addPoint: label already present (updates are not supported)");
+            }
+            for (size_t t = 0; t < nx; t++) labels_[n_old + t] = labels[t];
+        }
+        std::memcpy(vecs_.data() + n_old * dim_, X, nx * data_size_);
+        for (size_t t = 0; t < nx; t++) levels_[n_old + t] = random_level();
+        cur_element_count = n_old + nx;
+        const size_t n = cur_element_count;
+        dev_check(hrnn_dev_append_base(dev_, vecs_.data() + n_old * dim_, nx));
+        dev_base_rows_ = n;
+        uint32_t entry = 0;
+        int maxl = 0;
+        hrnn_insert_stats st{};
+        dev_check(hrnn_dev_insert_dynamic(dev_, n_old, levels_.data(), (uint32_t)M_, (uint32_t)ef_construction_,
+                                          m_update, bidirectional ? 1 : 0, &entry, &maxl, &st));
+        last_insert_stats_ = st;
+        // copy the graph back: layer 0 and every upper block (links back
+        // changed old nodes' lists too), then the KNN lists
+        size_t pool = 0;
+        for (size_t i = 0; i < n; i++) pool += (size_t)std::max(0, levels_[i]) * (maxM_ + 1);
+        std::vector<uint32_t> up(std::max<size_t>(pool, 1));
+        dev_check(hrnn_dev_get_hnsw(dev_, level0_.data(), up.data(), pool));
+        size_t o = 0;
+        for (size_t i = 0; i < n; i++) {
+            if (levels_[i] <= 0) continue;
+            if (i >= n_old) {
+                upper_[i] = static_cast<char *>(std::malloc((size_t)levels_[i] * links_bytes_));
+                if (!upper_[i]) throw std::runtime_error("addPoint: out of memory");
+            }
+            std::memcpy(upper_[i], up.data() + o, (size_t)levels_[i] * links_bytes_);
+            o += (size_t)levels_[i] * (maxM_ + 1);
+        }
+        enterpoint_node_ = entry;
+        maxlevel_ = maxl;
+        dev_hnsw_rows_ = n;
+        dev_check(hrnn_dev_get_knng(dev_, knn_ids_.data(), knn_d_.data(), knn_cnt_.data()));
+        dev_knng_ok_ = true;
+        dev_csr_ok_ = false;
+        rknng_built_ = false;
+        if (rknng_mutable_) {
+            rknng_lists_.assign(max_elements_, {});
+            for (size_t i = 0; i < n; i++) {
+                const uint32_t *row = knn_ids_.data() + i * K_knng_;
+                for (unsigned j = 0; j < knn_cnt_[i]; j++) rknng_lists_[row[j]].push_back((uint32_t)i);
+            }
+        }
+    }
+    const hrnn_insert_stats &lastInsertStats() const { return last_insert_stats_; }
+
     // ---- persistence (reference file format, hrnn.h:861-1047) ----------------
     void saveIndex(const std::string &location) override {
         if (serve_mode_) throw std::runtime_error("saveIndex: not available in serve mode");
@@ -876,7 +955,8 @@ class HRNN : public AlgorithmInterface<dist_t> {
     mutable size_t dev_base_rows_{0};
     mutable bool dev_knng_ok_{false};
     mutable bool dev_csr_ok_{false};
-    mutable size_t dev_hnsw_rows_{0};   // element count when the HNSW graph was uploaded
+    mutable size_t dev_hnsw_rows_{0};
+    hrnn_insert_stats last_insert_stats_{};   // element count when the HNSW graph was uploaded
 
     struct FileLayout {
         size_t elem, label_off, data_off, knng_slot, knng_off;

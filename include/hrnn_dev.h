@@ -24,6 +24,8 @@
  *   hrnn_dev_bf_knn           BruteforceSearch::searchKnn   include/hnswlib/bruteforce.h:106-135
  *   hrnn_dev_bf_rknn          compute_rknn                  scripts/gen_rknn_gt.py:82-151
  *   hrnn_dev_l2sqr_pairs      L2Sqr / L2SqrSIMD16ExtAVX512  include/hnswlib/space_l2.h:6-54
+ *   hrnn_dev_hnsw_build       HRNN::addPoint (KNN seeding)  include/hnswlib/hrnn.h:1378-1517
+ *   hrnn_dev_insert_dynamic   HRNN::insertPointDynamic      include/hnswlib/hrnn.h:2360-2534
  */
 #ifndef HRNN_DEV_H
 #define HRNN_DEV_H
@@ -95,7 +97,11 @@ int hrnn_dev_set_vector_precision(hrnn_dev_t h, int prec);
  * early whose class would keep it at stage 1 takes the first class with a
  * wider beam instead (the slow tail of stage 1 moves to stage 2, which
  * overlaps the other queries' search; results change only for those
- * queries). */
+ * queries).  "hnsw_batch_permille" (1 .. 1000, default 50) and
+ * "hnsw_seed_exp" (0 .. 1000, default 32): the HNSW construction's batch size
+ * relative to the graph and its seed completion.  "insert_offer_cap"
+ * (0 .. 2^30, default 0 = max(2^22, 8n)): the incremental insertion's first
+ * offer buffer (entries per batch; it grows when a batch needs more). */
 int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value);
 int hrnn_dev_get_tuning(hrnn_dev_t h, const char *key, int *value);
 
@@ -152,6 +158,51 @@ int hrnn_dev_hnsw_build(hrnn_dev_t h, const int *levels, uint32_t M, uint32_t ef
  * (pool_words = the sum over nodes of level * (M+1) words, blocks in id
  * order: node i's at the running sum over nodes before it). */
 int hrnn_dev_get_hnsw(hrnn_dev_t h, uint32_t *l0, uint32_t *up_pool, uint64_t pool_words);
+
+/* ---- incremental insertion ------------------------------------------- */
+typedef struct hrnn_insert_stats {
+    uint64_t batches;         /* insertion batches */
+    uint64_t scanned;         /* reverse-list entries read while collecting candidates */
+    uint64_t candidates;      /* distinct candidates (distances evaluated) */
+    uint64_t offers;          /* offers that passed the batch-start filter */
+    uint64_t updates;         /* entries the offers put into lists */
+    uint64_t retries;         /* batch scans run again after the offer buffer grew */
+    double ms;                /* wall time of the call */
+    double hnsw_ms, csr_ms, affect_ms, merge_ms;   /* device time by stage (HIP events) */
+    hrnn_hnsw_build_stats hnsw;                    /* the HNSW insertion's own counters */
+} hrnn_insert_stats;
+
+/* Incremental insertion of the resident rows [n_old, n) (appended with
+ * hrnn_dev_append_base) into the resident index of rows [0, n_old): the HNSW
+ * graph (from hrnn_dev_hnsw_build, hrnn_dev_set_hnsw or an earlier insertion,
+ * linking exactly those rows, with the same M) and the ranked KNN lists
+ * (K <= 1024).  levels[n]: every row's level (the new rows' drawn by the
+ * caller).  Per new point x, as insertPointDynamic: HNSW insertion with KNN
+ * seeding; candidates = the reverse lists of the first m_update nodes of x's
+ * list (all when 0), x and its list excluded; each candidate c takes x when
+ * its list is not full or d(x, c) < its last distance (the displaced entry
+ * leaves); with bidirectional, x's list becomes the K closest of its list and
+ * its candidates.  Points go in the HNSW construction's batches
+ * (hnsw_batch_permille); each batch's reverse lists are rebuilt from the
+ * graph after its seeding, its offers merged per list as the top K of (list
+ * U offers).  The reverse CSR is stale afterwards (rebuild it for search);
+ * the new points' entries carry the NNDescent new flag.  Replaces
+ * HRNN::insertPointDynamic (include/hnswlib/hrnn.h:2360-2534) with
+ * try_update_knng_evict (:1592-1655) and the mutable reverse lists of
+ * buildRKNNG_mutable (:1942-1959). */
+int hrnn_dev_insert_dynamic(hrnn_dev_t h, uint64_t n_old, const int *levels, uint32_t M, uint32_t ef_construction,
+                            uint32_t m_update, int bidirectional, uint32_t *entry, int *maxlevel,
+                            hrnn_insert_stats *stats);
+
+/* The affected-area step of hrnn_dev_insert_dynamic alone, for the rows
+ * [b0, b1) whose ranked lists already hold their seeds (no HNSW insertion):
+ * the reverse lists rebuilt from the resident graph, each row's candidates,
+ * offers filtered by the lists as they stand and merged (top K of list U
+ * offers), with bidirectional each row's own list replaced first.  With
+ * b1 = b0 + 1 this is phases 2-5 of the reference's insertPointDynamic for
+ * one point (include/hnswlib/hrnn.h:2366-2532). */
+int hrnn_dev_insert_affect(hrnn_dev_t h, uint64_t b0, uint64_t b1, uint32_t m_update, int bidirectional,
+                           hrnn_insert_stats *stats);
 
 /* ---- (a) NNDescent refinement ------------------------------------------ */
 typedef struct hrnn_nnd_iter_stats {

@@ -16,6 +16,7 @@
  *   hrnn_index_build_rknng_mutable  buildRKNNG_mutable                          include/hnswlib/hrnn.h:1942
  *   hrnn_index_search        searchRknn (batched)                               include/hnswlib/hrnn.h:1966
  *   hrnn_index_insert_dynamic insertPointDynamic                                include/hnswlib/hrnn.h:2360
+ *   hrnn_index_insert_dynamic_gpu insertPointDynamic for a whole batch, on the GPU include/hnswlib/hrnn.h:2360-2534
  *   hrnn_index_compact_for_serving compactForServing                            include/hnswlib/hrnn.h:299
  *   hrnn_index_save          saveIndex                                          include/hnswlib/hrnn.h:861
  */
@@ -87,6 +88,13 @@ int hrnn_index_search_proxies(hrnn_index_t ix, const float *q, uint64_t nq, cons
                               uint64_t *needed);
 int hrnn_index_insert_dynamic(hrnn_index_t ix, const float *x, uint64_t n, const uint64_t *labels,
                               uint32_t m_update, int bidirectional, int nthreads);
+/* insertPointDynamic for n points at once on the GPU
+ * (HRNN::insertPointsDynamicDevice over hrnn_dev_insert_dynamic); batch_permille
+ * (0: the default of 50) sizes the insertion batches relative to the graph;
+ * stats may be NULL. */
+int hrnn_index_insert_dynamic_gpu(hrnn_index_t ix, const float *x, uint64_t n, const uint64_t *labels,
+                                  uint32_t m_update, int bidirectional, int batch_permille,
+                                  hrnn_insert_stats *stats);
 int hrnn_index_compact_for_serving(hrnn_index_t ix, uint32_t max_k_serve);
 /* The mutable reverse lists (after buildRKNNG_mutable, kept up to date by
  * insertPointDynamic; reference include/hnswlib/hrnn.h:1942-1959, :2360-2534)

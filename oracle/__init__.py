@@ -85,6 +85,9 @@ def orc() -> C.CDLL:
                                             _u16p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                             C.c_void_p, C.c_void_p]
         lib.orc_knng_recall.restype = C.c_double
+        lib.orc_insert_affect.argtypes = [_f32p, C.c_uint64, C.c_uint32, _u32p, _f32p, _u16p, C.c_uint32,
+                                          C.c_uint64, C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.c_int,
+                                          C.c_void_p]
         lib.orc_knng_recall.argtypes = [_u32p, _u16p, C.c_uint32, _u32p, C.c_uint32,
                                         C.c_uint64, C.c_uint32]
         lib.orc_rknn_recall.restype = C.c_double
@@ -265,6 +268,22 @@ def nndescent_flags(x, ids, dists, cnt, flags, K, iters, sample, W=L2_W, fma=L2_
     assert rc == 0
     n = x.shape[0]
     return ids.reshape(n, K), dists.reshape(n, K), cnt, fl.reshape(n, K), su, sd
+
+
+def insert_affect(x, ids, dists, cnt, K, b0, b1, m_update=0, bidir=False, W=L2_W, fma=L2_FMA):
+    """Phases 2-5 of insertPointDynamic for rows [b0, b1) holding their seeds
+    (batch semantics; b1 = b0 + 1 is the reference's step for one point);
+    returns (ids, dists, cnt, stats = [scanned, candidates, offers, updates])."""
+    x = _f32(x)
+    n = x.shape[0]
+    ids = np.ascontiguousarray(ids, np.uint32).reshape(-1).copy()
+    dists = _f32(dists).reshape(-1).copy()
+    cnt = np.ascontiguousarray(cnt, np.uint16).copy()
+    st = np.zeros(4, np.uint64)
+    rc = orc().orc_insert_affect(x, n, x.shape[1], ids, dists, cnt, K, b0, b1, m_update, int(bidir), W, fma,
+                                 st.ctypes.data_as(C.c_void_p))
+    assert rc == 0, rc
+    return ids.reshape(n, K), dists.reshape(n, K), cnt, st
 
 
 def knng_recall(ids, cnt, K, exact, kk):

@@ -536,6 +536,184 @@ int orc_nndescent(const float *x, uint64_t n, uint32_t d, uint32_t K, uint32_t *
 }
 
 /* ------------------------------------------------------------------------
+ * Phases 2-5 of insertPointDynamic (reference include/hnswlib/hrnn.h:2366-2532
+ * with try_update_knng_evict :1592-1655) for the rows [b0, b1) of a ranked
+ * KNN graph whose rows b0 .. b1-1 already hold their seed lists (phase 1,
+ * addPoint).  The reverse lists are the transpose of the graph as given
+ * (what buildRKNNG_mutable builds and phases 2 and 5 keep up to date when
+ * bidir is 0).  Per new row x, in order: candidates = the ids on the
+ * reverse lists of the first m_update entries of x's list (all when 0), each
+ * once, x and every entry of x's list excluded; x is offered to each
+ * candidate c at d = L2(x, c) when c's list (as given) is not full or
+ * d < its last distance.  With bidir, x's list becomes the K smallest
+ * (d, id) of its list and its candidates (x itself skipped).  Then every
+ * list takes its offers: the K first of (list U offers) in distance order,
+ * listed entries ahead of equal offers, offers by (d, id), offers of an id
+ * already listed dropped.  For b1 = b0 + 1 this is the reference's step
+ * for one point: each candidate gets at most one offer, and a list that
+ * takes it replaces its last entry (or appends) and moves it up past every
+ * strictly larger distance, as the evict insertion sort does.
+ * Pinned: for one point at a time, the graph after each of the compiled
+ * reference's insertPointDynamic calls (single-threaded, bidir 0, m_update
+ * 0 and > 0) and, with bidir 1, after its first call (later calls read
+ * reverse entries the reference leaves stale) (tests).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    uint32_t tgt;
+    float d;
+    uint32_t id;
+} orc_offer;
+
+static int offer_cmp(const void *a, const void *b) {
+    const orc_offer *x = (const orc_offer *)a, *y = (const orc_offer *)b;
+    if (x->tgt != y->tgt) return x->tgt < y->tgt ? -1 : 1;
+    if (x->d != y->d) return x->d < y->d ? -1 : 1;
+    if (x->id != y->id) return x->id < y->id ? -1 : 1;
+    return 0;
+}
+
+typedef struct {
+    float d;
+    uint32_t id;
+} orc_key;
+
+static int key_cmp(const void *a, const void *b) {
+    const orc_key *x = (const orc_key *)a, *y = (const orc_key *)b;
+    if (x->d != y->d) return x->d < y->d ? -1 : 1;
+    if (x->id != y->id) return x->id < y->id ? -1 : 1;
+    return 0;
+}
+
+int orc_insert_affect(const float *x, uint64_t n, uint32_t d, uint32_t *ids, float *dist, uint16_t *cnt,
+                      uint32_t K, uint64_t b0, uint64_t b1, uint32_t m_update, int bidir, int W, int use_fma,
+                      uint64_t *stats) {
+    if (b0 >= b1 || b1 > n || K == 0) return ORC_E_ARG;
+    /* reverse lists: transpose of the graph as given */
+    uint64_t *roff = (uint64_t *)calloc(n + 1, sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; i++)
+        for (uint32_t j = 0; j < cnt[i]; j++) roff[ids[i * K + j] + 1]++;
+    for (uint64_t i = 0; i < n; i++) roff[i + 1] += roff[i];
+    uint32_t *rent = (uint32_t *)malloc(sizeof(uint32_t) * (roff[n] + 1));
+    uint64_t *fill = (uint64_t *)malloc(sizeof(uint64_t) * n);
+    memcpy(fill, roff, sizeof(uint64_t) * n);
+    for (uint64_t i = 0; i < n; i++)
+        for (uint32_t j = 0; j < cnt[i]; j++) rent[fill[ids[i * K + j]]++] = (uint32_t)i;
+    free(fill);
+    uint8_t *seen = (uint8_t *)calloc(n, 1);
+    uint32_t *cand = (uint32_t *)malloc(sizeof(uint32_t) * (roff[n] + 1));
+    uint64_t ocap = 1024, no = 0, scanned = 0, ncand = 0;
+    orc_offer *off = (orc_offer *)malloc(sizeof(orc_offer) * ocap);
+    uint32_t *bid = bidir ? (uint32_t *)malloc(sizeof(uint32_t) * (b1 - b0) * K) : NULL;
+    float *bd = bidir ? (float *)malloc(sizeof(float) * (b1 - b0) * K) : NULL;
+    uint16_t *bc = bidir ? (uint16_t *)malloc(sizeof(uint16_t) * (b1 - b0)) : NULL;
+    for (uint64_t xn = b0; xn < b1; xn++) {
+        const uint32_t c0 = cnt[xn];
+        const uint32_t np = (m_update > 0 && m_update < c0) ? m_update : c0;
+        uint64_t nc = 0;
+        seen[xn] = 1;
+        for (uint32_t j = 0; j < c0; j++) seen[ids[xn * K + j]] = 1;
+        for (uint32_t p = 0; p < np; p++) {
+            const uint32_t y = ids[xn * K + p];
+            for (uint64_t e = roff[y]; e < roff[y + 1]; e++) {
+                const uint32_t c = rent[e];
+                scanned++;
+                if (!seen[c]) {
+                    seen[c] = 1;
+                    cand[nc++] = c;
+                }
+            }
+        }
+        seen[xn] = 0;
+        for (uint32_t j = 0; j < c0; j++) seen[ids[xn * K + j]] = 0;
+        for (uint64_t t = 0; t < nc; t++) seen[cand[t]] = 0;
+        ncand += nc;
+        orc_key *heap = bidir ? (orc_key *)malloc(sizeof(orc_key) * (c0 + nc + 1)) : NULL;
+        uint64_t nh = 0;
+        if (bidir)
+            for (uint32_t j = 0; j < c0; j++) heap[nh++] = (orc_key){dist[xn * K + j], ids[xn * K + j]};
+        for (uint64_t t = 0; t < nc; t++) {
+            const uint32_t c = cand[t];
+            const float dd = orc_l2sqr(x + xn * d, x + (uint64_t)c * d, d, W, use_fma);
+            if (bidir) heap[nh++] = (orc_key){dd, c};
+            const uint32_t cc = cnt[c];
+            if (cc >= K && !(dd < dist[(uint64_t)c * K + cc - 1])) continue;
+            if (no == ocap) {
+                ocap *= 2;
+                off = (orc_offer *)realloc(off, sizeof(orc_offer) * ocap);
+            }
+            off[no++] = (orc_offer){c, dd, (uint32_t)xn};
+        }
+        if (bidir) {
+            qsort(heap, nh, sizeof(orc_key), key_cmp);
+            uint32_t w = 0;
+            for (uint64_t t = 0; t < nh && w < K; t++) {
+                if (heap[t].id == (uint32_t)xn) continue;
+                if (t > 0 && heap[t].id == heap[t - 1].id && heap[t].d == heap[t - 1].d) continue;
+                bid[(xn - b0) * K + w] = heap[t].id;
+                bd[(xn - b0) * K + w] = heap[t].d;
+                w++;
+            }
+            bc[xn - b0] = (uint16_t)w;
+            free(heap);
+        }
+    }
+    if (bidir)
+        for (uint64_t xn = b0; xn < b1; xn++) {
+            memcpy(ids + xn * K, bid + (xn - b0) * K, sizeof(uint32_t) * bc[xn - b0]);
+            memcpy(dist + xn * K, bd + (xn - b0) * K, sizeof(float) * bc[xn - b0]);
+            cnt[xn] = bc[xn - b0];
+        }
+    /* merge the offers per list */
+    qsort(off, no, sizeof(orc_offer), offer_cmp);
+    uint64_t upd = 0;
+    uint32_t *mi = (uint32_t *)malloc(sizeof(uint32_t) * (K + no + 1));
+    float *md = (float *)malloc(sizeof(float) * (K + no + 1));
+    for (uint64_t a = 0; a < no;) {
+        uint64_t e = a;
+        const uint32_t c = off[a].tgt;
+        while (e < no && off[e].tgt == c) e++;
+        const uint32_t lc = cnt[c];
+        uint32_t i = 0, m = 0;
+        uint64_t j = a;
+        while ((i < lc || j < e) && m < K) {
+            if (j < e) {   /* an offer whose id is listed (anywhere) is dropped */
+                int listed = 0;
+                for (uint32_t t = 0; t < lc; t++)
+                    if (ids[(uint64_t)c * K + t] == off[j].id) listed = 1;
+                if (listed) {
+                    j++;
+                    continue;
+                }
+            }
+            if (j >= e || (i < lc && dist[(uint64_t)c * K + i] <= off[j].d)) {
+                mi[m] = ids[(uint64_t)c * K + i];
+                md[m] = dist[(uint64_t)c * K + i];
+                i++;
+            } else {
+                mi[m] = off[j].id;
+                md[m] = off[j].d;
+                j++;
+                upd++;
+            }
+            m++;
+        }
+        memcpy(ids + (uint64_t)c * K, mi, sizeof(uint32_t) * m);
+        memcpy(dist + (uint64_t)c * K, md, sizeof(float) * m);
+        cnt[c] = (uint16_t)m;
+        a = e;
+    }
+    if (stats) {
+        stats[0] = scanned;
+        stats[1] = ncand;
+        stats[2] = no;
+        stats[3] = upd;
+    }
+    free(mi); free(md); free(off); free(cand); free(seen); free(roff); free(rent);
+    free(bid); free(bd); free(bc);
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------
  * KNN-graph recall against exact neighbour lists: for each node, the
  * fraction of its first kk exact neighbours (self excluded) present in its
  * first kk list entries.  No reference counterpart (quality measure).
