@@ -70,6 +70,12 @@ CONFIGS = {
                S=50, k=10, Kp=100, ms=[1, 5, 10], ef=200),
     # precision form: |N(0,1)| scaled per dimension by U(0.05, 1) (one scale
     # vector, seed 55, shared by base and queries)
+    # batched form: the C2 data; 100,000 queries with Zipf(1.1)-skewed centres
+    # and k drawn from {1, 10, 50}; half the points built, the other half
+    # inserted in the dynamic test's order (std::shuffle, mt19937(42))
+    "c4": dict(n=1_000_000, nq=100_000, d=128, centres=1000, sigma=25.0, data_seed=3, query_seed=7,
+               centre_seed=1003, zipf=1.1, ks=[1, 10, 50], k_seed=7, split=0.5, split_seed=42, M=16, efc=400,
+               K=500, iters=10, S=50, Kp=500, ms=[1, 3, 5, 10, 20, 50], ef=200),
     "c3": dict(n=1_000_000, nq=1000, d=960, data_seed=1, query_seed=2, scale_seed=55, M=16, efc=200,
                K=200, iters=10, S=50, k=10, Kp=200, ms=[1, 3, 5, 10, 20, 50], ef=200),
 }
@@ -95,6 +101,22 @@ def macro_recall(gt_off, gt_ids, r_off, r_ids):
     return tot / max(cnt, 1)
 
 
+def recall_per_query(gt_off, gt_ids, r_off, r_ids):
+    """|R_q & GT_q| / |GT_q| per query (NaN where GT_q is empty), vectorised."""
+    gt_off = np.asarray(gt_off, np.int64)
+    r_off = np.asarray(r_off, np.int64)
+    nq = len(gt_off) - 1
+    gq = np.repeat(np.arange(nq, dtype=np.uint64), np.diff(gt_off))
+    rq = np.repeat(np.arange(nq, dtype=np.uint64), np.diff(r_off))
+    gk = (gq << np.uint64(32)) | np.asarray(gt_ids, np.uint64)
+    rk = (rq << np.uint64(32)) | np.asarray(r_ids, np.uint64)
+    hit = np.isin(gk, rk)
+    num = np.bincount(gq.astype(np.int64), weights=hit, minlength=nq)
+    den = np.diff(gt_off).astype(np.float64)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.where(den > 0, num / den, np.nan)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,7 +133,9 @@ def main() -> int:
                          "it under --nnd-ref-budget seconds (auto)")
     ap.add_argument("--nnd-ref-budget", type=float, default=420.0)
     ap.add_argument("--n", type=int, default=0,
-                    help="base size for --config c3 (0: the configuration's 1,000,000; smaller for a quick run)")
+                    help="base size for --config c3 or c4 (0: the configuration's 1,000,000; smaller for a quick run)")
+    ap.add_argument("--nq", type=int, default=0,
+                    help="queries for --config c4 (0: the configuration's 100,000)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
@@ -124,6 +148,8 @@ def main() -> int:
     os.dup2(2, 1)
     if args.config == "c3":
         return run_c3(args, json_out)
+    if args.config == "c4":
+        return run_c4(args, json_out)
 
     import hrnn_amd
     from hrnn_amd import hiprt
@@ -1227,6 +1253,361 @@ def run_c3(args, json_out) -> int:
                        "evaluations": int(hnsw_st["evaluations"])},
         "stages_s": {"hnsw_gpu": round(hnsw_s, 2), "nndescent_gpu": round(nnd_s, 3), "rknn_csr_gpu": round(csr_s, 3),
                      "rk_bruteforce_gpu_bf16": prec_rep["bf16"]["seconds"], "groundtruth_gpu": round(gt_s, 3)},
+    }
+    print(json.dumps(out), file=json_out, flush=True)
+    return 0
+
+
+def run_c4(args, json_out) -> int:
+    """The batched form (1M x 128, C2 data): half the points built on the GPU
+    (HNSW + KNN seeds, NNDescent K=500), the other half inserted on the GPU in
+    the reference dynamic test's order (insertPointDynamic in batches,
+    m_update 0, bidirectional off), then one persistent search launch of
+    100,000 queries with Zipf(1.1)-skewed centres and k from {1, 10, 50} per
+    query.  value = QPS of that launch with its host I/O at the cheapest m
+    whose macro recall (over all queries, exact groundtruth per query's k) is
+    >= 0.95.  Reported beside it: insertion time per point, recall per k, the
+    recall of a full rebuild on the same queries (the dynamic test's oracle
+    snapshot), and the reference's insertPointDynamic and searchRknn on the
+    host cores."""
+    import hrnn_amd
+    from hrnn_amd import hiprt
+    from hrnn_amd.dev import Device
+    from hrnn_amd.index import Index
+
+    cfg = CONFIGS["c4"]
+    threads = args.threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or 16
+    n, nq, d, K, Kp, ef = cfg["n"], cfg["nq"], cfg["d"], cfg["K"], cfg["Kp"], cfg["ef"]
+    n = args.n or n
+    nq = args.nq or nq
+    L = hrnn_amd.lib()
+    t = time.time()
+    x = hrnn_amd.synth_mixture(cfg["data_seed"], n, d, cfg["centres"], cfg["sigma"], 0.0, 255.0, True, 0.0,
+                               cfg["centre_seed"])
+    q = hrnn_amd.synth_mixture(cfg["query_seed"], nq, d, cfg["centres"], cfg["sigma"], 0.0, 255.0, True, cfg["zipf"],
+                               cfg["centre_seed"])
+    kq = np.random.default_rng(cfg["k_seed"]).choice(cfg["ks"], nq).astype(np.uint32)
+    perm = hrnn_amd.synth_split_perm(n, cfg["split_seed"])
+    split_n = int(n * cfg["split"])
+    init, rest = perm[:split_n].astype(np.uint64), perm[split_n:].astype(np.uint64)
+    log(f"data {n}x{d} + {nq} queries in {time.time() - t:.1f}s; initial {split_n}, inserted {n - split_n}")
+
+    # ---- initial build on the GPU, then the insertion (the maintenance metric) -----------
+    t = time.time()
+    ix = Index(n, d, cfg["M"], cfg["efc"], 100, K)
+    hst = ix.add_points_gpu(x[init], labels=init)
+    hnsw_s = time.time() - t
+    t = time.time()
+    nst = ix.refine(cfg["iters"], cfg["S"])
+    nnd_s = time.time() - t
+    log(f"initial build: HNSW + seeds {hnsw_s:.2f}s, refineKNNG {nnd_s:.2f}s")
+    snap_dir = snap = None
+    if not args.no_cpu_baseline:
+        try:
+            import oracle
+            if oracle.ref_available():
+                shm = "/dev/shm" if os.path.isdir("/dev/shm") else None
+                snap_dir = tempfile.mkdtemp(prefix="hrnn_c4_", dir=shm)
+                snap = os.path.join(snap_dir, "initial.idx")
+                ix.save(snap)
+        except Exception as e:
+            log(f"initial snapshot not saved: {e}")
+            snap = None
+    t = time.time()
+    ist = ix.insert_dynamic_gpu(x[rest], rest)
+    ins_s = time.time() - t
+    log(f"insertion of {len(rest)} points on the GPU: {ins_s:.2f}s ({ins_s / len(rest) * 1e6:.2f} us per point); "
+        f"{ {kk_: (round(v, 1) if isinstance(v, float) else v) for kk_, v in ist.items() if kk_ != 'hnsw'} }")
+    t = time.time()
+    ix.build_rknng()
+    csr_s = time.time() - t
+    labels = ix.labels()   # internal id -> row of x
+
+    # ---- groundtruth per query's k (GPU brute force, internal-id space) ---------------------
+    h = ix.device_handle()
+    dev = Device.borrow(h, n, d, K)
+    t = time.time()
+    gt_parts = {}
+    for kv in cfg["ks"]:
+        kth_v = dev.bf_kth(int(kv), "bf16")
+        sel = np.nonzero(kq == kv)[0]
+        o, i_, _ = dev.bf_rknn(q[sel], kth_v, "bf16")
+        gt_parts[int(kv)] = (sel, o.astype(np.int64), i_)
+    gt_cnt = np.zeros(nq, np.int64)
+    for kv, (sel, o, _) in gt_parts.items():
+        gt_cnt[sel] = np.diff(o)
+    gt_off = np.zeros(nq + 1, np.int64)
+    gt_off[1:] = np.cumsum(gt_cnt)
+    gt_ids = np.empty(int(gt_off[-1]), np.uint32)
+    for kv, (sel, o, i_) in gt_parts.items():
+        for j, qi in enumerate(sel):
+            gt_ids[gt_off[qi]:gt_off[qi + 1]] = i_[o[j]:o[j + 1]]
+    gt_s = time.time() - t
+    log(f"groundtruth for k in {cfg['ks']}: {gt_s:.2f}s, avg |GT| {len(gt_ids) / nq:.1f}")
+
+    # ---- one persistent launch for all queries, k per query ---------------------------------
+    stream = L.hrnn_dev_stream(h)
+    m_max = max(cfg["ms"])
+    cap = 4096
+    qd = hiprt.DeviceArray.from_host(q)
+    kd = hiprt.DeviceArray.from_host(kq)
+    pd = hiprt.DeviceArray((nq * m_max,), np.uint32)
+    cd = hiprt.DeviceArray((nq,), np.uint32)
+    idd = hiprt.DeviceArray((nq * cap,), np.uint32)
+    wd = hiprt.DeviceArray((2 * nq,), np.uint32)
+    ed = hiprt.DeviceArray((2 * nq,), np.uint32)
+
+    def ck(rc):
+        if rc:
+            raise RuntimeError(L.hrnn_dev_last_error(h).decode())
+
+    def run_gpu(m):
+        ck(L.hrnn_dev_rknn_query_dptr(h, qd.ptr, nq, m, max(ef, m), kd.ptr, 0, Kp, 1.0, pd.ptr, cd.ptr, idd.ptr,
+                                      cap, wd.ptr))
+
+    def recall_of(off, ids_, sel=None):
+        """macro recall (reference include/utils.h:135-157) over all queries or the selected ones"""
+        r = recall_per_query(gt_off, gt_ids, off, ids_)
+        if sel is not None:
+            r = r[sel]
+        return float(np.nanmean(r)) if np.any(~np.isnan(r)) else 0.0
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        hiprt.stream_sync(stream)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        hiprt.stream_sync(stream)
+        return (time.perf_counter() - t0) / args.steps
+
+    q_pin = hiprt.PinnedArray((nq, d), np.float32)
+    q_pin.a[:] = q
+    off_pin = hiprt.PinnedArray((nq + 1,), np.uint64)
+    ids_pin = [hiprt.PinnedArray((nq * 64,), np.uint32)]
+    need = ctypes.c_uint64(0)
+
+    def fetch():
+        """hrnn_dev_fetch_results: the answers compacted on the device, then
+        offsets and ids copied to pinned host memory."""
+        while True:
+            rc = L.hrnn_dev_fetch_results(h, nq, cd.ptr, idd.ptr, cap, off_pin.a.ctypes.data_as(ctypes.c_void_p),
+                                          ids_pin[0].a.ctypes.data_as(ctypes.c_void_p), ids_pin[0].a.size,
+                                          ctypes.byref(need))
+            if rc == -5 and need.value > ids_pin[0].a.size and need.value <= nq * cap:
+                ids_pin[0] = hiprt.PinnedArray((int(need.value) * 2,), np.uint32)
+                continue
+            ck(rc)
+            return
+
+    def results():
+        fetch()
+        off = off_pin.a.astype(np.int64)
+        return off, ids_pin[0].a[:int(off[-1])].astype(np.uint32)
+
+    def step_io(m):
+        hiprt.copy_async(qd.ptr, q_pin.ptr, nq * d * 4, hiprt.H2D, stream)
+        run_gpu(m)
+        fetch()
+
+    paths, sweep = {}, {}
+    for vp, pname, s_v in ((0, "fp32", 4), (3, "u8", 1)):
+        ck(L.hrnn_dev_set_vector_precision(h, vp))
+        sweep[pname] = []
+        m_star = None
+        for m in cfg["ms"]:
+            run_gpu(m)
+            r_off, r_ids = results()
+            rec = recall_of(r_off, r_ids)
+            sweep[pname].append({"m": m, "recall": round(rec, 5), "avg_results": round(len(r_ids) / nq, 2)})
+            log(f"  {pname} m={m}: macro recall {rec:.4f}")
+            if m_star is None and rec >= args.recall_target:
+                m_star = m
+                per_k = {int(kv): round(recall_of(r_off, r_ids, np.nonzero(kq == kv)[0]), 5) for kv in cfg["ks"]}
+                answers = (r_off, r_ids)
+            if m_star is not None:
+                break
+        if m_star is None:
+            continue
+        sec = timed(lambda: step_io(m_star))
+        sec_res = timed(lambda: run_gpu(m_star))
+        paths[f"{pname}_fixed_m"] = {"m": m_star, "recall": sweep[pname][-1]["recall"], "recall_per_k": per_k,
+                                     "vectors": pname, "s_v": s_v, "qps": round(nq / sec, 1),
+                                     "qps_resident": round(nq / sec_res, 1), "ms_per_step": round(sec * 1e3, 3),
+                                     "answers": answers}
+        log(f"  path {pname}_fixed_m: m={m_star}, {nq / sec:,.0f} QPS with host I/O ({nq / sec_res:,.0f} resident), "
+            f"recall per k {per_k}")
+    if not paths:
+        raise RuntimeError("no m reaches the recall target")
+    same = all(np.array_equal(p["answers"][1], paths["fp32_fixed_m"]["answers"][1]) for p in paths.values()
+               if p["m"] == paths["fp32_fixed_m"]["m"])
+    best_name = max(paths, key=lambda kk_: paths[kk_]["qps"])
+    best = paths[best_name]
+    for p in paths.values():
+        p.pop("answers")
+
+    # ---- roofline of the launch's kernels (HIP events on the library stream) ----------------
+    ck(L.hrnn_dev_set_vector_precision(h, 3 if best["s_v"] == 1 else 0))
+    m_b, s_v = best["m"], best["s_v"]
+    e0, e1 = hiprt.Event(), hiprt.Event()
+    reps = 5
+
+    def ev_ms(fn):
+        fn()
+        hiprt.stream_sync(stream)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.sync()
+        return e1.ms_since(e0) / reps
+
+    beam_ms = ev_ms(lambda: ck(L.hrnn_dev_proxies_dptr(h, qd.ptr, nq, m_b, max(ef, m_b), pd.ptr, ed.ptr)))
+    ex = ed.to_host().reshape(nq, 2).astype(np.int64)
+    beam_bytes = float((4 * d + ex[:, 0] * 4 * (2 * cfg["M"] + 1) + ex[:, 1] * s_v * d + 4 * m_b).sum())
+    search_ms = ev_ms(lambda: ck(L.hrnn_dev_rknn_search_dptr(h, qd.ptr, nq, pd.ptr, m_b, kd.ptr, 0, Kp, 1.0, cd.ptr,
+                                                            idd.ptr, cap, wd.ptr)))
+    wk = wd.to_host().reshape(nq, 2).astype(np.int64)
+    rc_ = cd.to_host().astype(np.int64)
+    search_bytes = float((4 * d + 16 * m_b + 4 * wk[:, 0] + wk[:, 1] * (s_v * d + 4) + 4 * rc_).sum())
+    kern = {"k_phase_a": {"ms": beam_ms, "bytes": beam_bytes,
+                          "expansions_per_query": round(float(ex[:, 0].mean()), 1)},
+            "k_rknn_search": {"ms": search_ms, "bytes": search_bytes,
+                              "scanned_per_query": round(float(wk[:, 0].mean()), 1),
+                              "candidates_per_query": round(float(wk[:, 1].mean()), 1)}}
+    for v in kern.values():
+        v["gbs"] = v["bytes"] / (v["ms"] * 1e-3) / 1e9
+    dom = max(kern, key=lambda kk_: kern[kk_]["ms"])
+    achieved = kern[dom]["gbs"]
+    cands = wk[:, 1]
+    skew = {"candidates_p1": int(np.quantile(cands, 0.01)), "candidates_median": int(np.median(cands)),
+            "candidates_p99": int(np.quantile(cands, 0.99)), "candidates_max": int(cands.max()),
+            "scanned_p1": int(np.quantile(wk[:, 0], 0.01)), "scanned_p99": int(np.quantile(wk[:, 0], 0.99)),
+            "results_max": int(rc_.max())}
+    log(f"  k_phase_a {beam_ms:.3f} ms, k_rknn_search {search_ms:.3f} ms over {nq} queries; skew {skew}")
+
+    # ---- a full rebuild on the same queries (the dynamic test's oracle snapshot) ------------
+    ck(L.hrnn_dev_set_vector_precision(h, 0))
+    run_gpu(m_b)
+    dyn_off, dyn_ids = results()
+    dyn_lab = labels[dyn_ids].astype(np.uint32)   # rows of x
+    gt_lab = labels[gt_ids].astype(np.uint32)
+    del qd, kd, pd, cd, idd, wd, ed
+    dev = None
+    saved_dyn = None
+    if snap is not None:
+        saved_dyn = os.path.join(snap_dir, "dynamic.idx")
+        ix.save(saved_dyn)
+    ix.close()
+    t = time.time()
+    full = Index(n, d, cfg["M"], cfg["efc"], 100, K)
+    full.add_points_gpu(x)
+    full.refine(cfg["iters"], cfg["S"])
+    full.build_rknng()
+    full_s = time.time() - t
+    fh = full.device_handle()
+    fstream = L.hrnn_dev_stream(fh)
+    flabels = full.labels()
+    qd = hiprt.DeviceArray.from_host(q)
+    kd = hiprt.DeviceArray.from_host(kq)
+    pd = hiprt.DeviceArray((nq * m_b,), np.uint32)
+    cd = hiprt.DeviceArray((nq,), np.uint32)
+    idd = hiprt.DeviceArray((nq * cap,), np.uint32)
+    wd = hiprt.DeviceArray((2 * nq,), np.uint32)
+    rcode = L.hrnn_dev_rknn_query_dptr(fh, qd.ptr, nq, m_b, max(ef, m_b), kd.ptr, 0, Kp, 1.0, pd.ptr, cd.ptr,
+                                       idd.ptr, cap, wd.ptr)
+    if rcode:
+        raise RuntimeError(L.hrnn_dev_last_error(fh).decode())
+    hiprt.stream_sync(fstream)
+    fc = cd.to_host().astype(np.int64)
+    if fc.max(initial=0) > cap:
+        raise RuntimeError("result capacity exceeded")
+    f_off = np.zeros(nq + 1, np.int64)
+    f_off[1:] = np.cumsum(fc)
+    f_lab = flabels[idd.to_host().reshape(nq, cap)[np.arange(cap)[None, :] < fc[:, None]]].astype(np.uint32)
+    rec_full = float(np.nanmean(recall_per_query(gt_off, gt_lab, f_off, f_lab)))
+    rec_dyn = float(np.nanmean(recall_per_query(gt_off, gt_lab, dyn_off, dyn_lab)))
+    log(f"full rebuild {full_s:.1f}s: macro recall at m={m_b} {rec_full:.4f} vs after insertion {rec_dyn:.4f}")
+    del qd, kd, pd, cd, idd, wd
+    full.close()
+
+    # ---- cpu baselines: the reference's insertPointDynamic and searchRknn --------------------
+    ins_cpu = search_cpu = None
+    if snap is not None:
+        try:
+            import oracle
+            r = oracle.RefIndex.load(snap, d, n)
+            t1 = time.perf_counter()
+            r.build_rknng_mutable()
+            mut_s = time.perf_counter() - t1
+            ns = min(len(rest), 20_000)
+            t1 = time.perf_counter()
+            r.insert_dynamic(x[rest[:ns]], rest[:ns], 0, False, nthreads=threads)
+            ref_ins_s = time.perf_counter() - t1
+            r.close()
+            ins_cpu = {"value": round(ref_ins_s / ns * 1e6, 2), "unit": "us per inserted point", "cores": threads,
+                       "kind": "reference",
+                       "sample": f"the reference's insertPointDynamic (m_update 0, bidirectional off) for the first "
+                                 f"{ns} of the {len(rest)} inserted points, OpenMP over points on {threads} host "
+                                 f"threads, from this run's initial index loaded from its saved file "
+                                 f"(buildRKNNG_mutable {mut_s:.1f}s not counted)",
+                       "seconds": round(ref_ins_s, 2),
+                       "projected_all_s": round(ref_ins_s / ns * len(rest), 1),
+                       "gpu_us_per_point": round(ins_s / len(rest) * 1e6, 3)}
+            log(f"insert baseline: reference {ref_ins_s:.1f}s for {ns} points on {threads} threads "
+                f"({ins_cpu['value']} us per point; GPU {ins_cpu['gpu_us_per_point']} us)")
+            r = oracle.RefIndex.load(saved_dyn, d, n)
+            r.build_rknng()
+            ns = min(nq, 4000)
+            t1 = time.perf_counter()
+            for kv in cfg["ks"]:
+                sel = np.nonzero(kq[:ns] == kv)[0]
+                r.search(q[sel], m_b, 1.0, int(kv), Kp, ef, nthreads=threads)
+            ref_q_s = time.perf_counter() - t1
+            r.close()
+            search_cpu = {"value": round(ns / ref_q_s, 1), "unit": "queries/s", "cores": threads,
+                          "kind": "reference",
+                          "sample": f"the first {ns} queries, the reference's searchRknn at m={m_b}, K'={Kp}, "
+                                    f"ef={ef} with each query's k (one OpenMP batch per k) on {threads} host "
+                                    f"threads, on this run's index after insertion loaded from its saved file"}
+            log(f"search baseline: reference {search_cpu['value']} QPS on {threads} threads")
+        except Exception as e:
+            log(f"cpu baselines skipped: {e}")
+    if snap_dir:
+        for f in os.listdir(snap_dir):
+            os.remove(os.path.join(snap_dir, f))
+        os.rmdir(snap_dir)
+
+    out = {
+        "metric": "RkNN QPS, batched form: 100,000 queries in one launch, k in {1,10,50}, Zipf(1.1) centres; "
+                  "500,000 incremental inserts, 1M x 128",
+        "value": best["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": best["ms_per_step"], "resident_qps": best["qps_resident"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8" if best["s_v"] == 1 else "f32", "data": "synthetic",
+        "config": {"workload": f"hrnn-c4-{n // 1000}k-x{d}-{nq // 1000}kq", "n": n, "nq": nq, "d": d,
+                   "initial": split_n, "inserted": n - split_n, "K_knng": K, "M": cfg["M"],
+                   "ef_construction": cfg["efc"], "nn_iters": cfg["iters"], "nn_sample": cfg["S"],
+                   "k": cfg["ks"], "K_prime": Kp, "ef_search": ef, "zipf": cfg["zipf"], "path": best_name,
+                   "recall": best["recall"], "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                     "kernel_ms": round(kern[dom]["ms"], 4), "alg_bytes_per_launch": int(kern[dom]["bytes"]),
+                     "kernels": {kk_: {k2: (round(v2, 4) if isinstance(v2, float) else v2) for k2, v2 in v.items()}
+                                 for kk_, v in kern.items()}},
+        "cpu_baseline": search_cpu,
+        "insert_cpu_baseline": ins_cpu,
+        "paths": paths, "sweep": sweep, "fp32_u8_answers_equal": bool(same),
+        "candidate_skew": skew,
+        "insertion": {"seconds": round(ins_s, 3), "us_per_point": round(ins_s / len(rest) * 1e6, 3),
+                      **{kk_: (round(v, 2) if isinstance(v, float) else v) for kk_, v in ist.items() if kk_ != "hnsw"},
+                      "hnsw": {kk_: (round(v, 2) if isinstance(v, float) else v) for kk_, v in ist["hnsw"].items()}},
+        "recall_vs_full_rebuild": {"m": m_b, "after_insertion": round(rec_dyn, 5), "full_rebuild": round(rec_full, 5),
+                                   "gap": round(rec_full - rec_dyn, 5), "full_rebuild_s": round(full_s, 2)},
+        "stages_s": {"initial_hnsw_gpu": round(hnsw_s, 2), "initial_nndescent_gpu": round(nnd_s, 3),
+                     "insertion_gpu": round(ins_s, 3), "rknn_csr_gpu": round(csr_s, 3),
+                     "groundtruth_gpu": round(gt_s, 2)},
     }
     print(json.dumps(out), file=json_out, flush=True)
     return 0

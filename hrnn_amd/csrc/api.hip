@@ -360,6 +360,7 @@ static int *tuning_slot(hrnn_dev_t h, const char *key) {
     if (k == "hnsw_batch_permille") return &h->tune_hnsw_batch_permille;
     if (k == "hnsw_seed_exp") return &h->tune_hnsw_seed_exp;
     if (k == "insert_offer_cap") return &h->tune_insert_offer_cap;
+    if (k == "insert_lds_bitmap") return &h->tune_insert_lds_bitmap;
     return nullptr;
 }
 
@@ -370,8 +371,9 @@ int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value) {
     const bool permille = slot == &h->tune_hnsw_batch_permille;
     const bool seedx = slot == &h->tune_hnsw_seed_exp;
     const bool offers = slot == &h->tune_insert_offer_cap;
-    if (!slot || value < (cap || seedx || offers ? 0 : 1) ||
-        value > (offers ? (1 << 30) : cap ? 1024 : permille || seedx ? 1000 : 64)) {
+    const bool flag = slot == &h->tune_insert_lds_bitmap;
+    if (!slot || value < (cap || seedx || offers || flag ? 0 : 1) ||
+        value > (offers ? (1 << 30) : flag ? 1 : cap ? 1024 : permille || seedx ? 1000 : 64)) {
         h->err = std::string("set_tuning: unknown key or value out of range: ") + (key ? key : "");
         return HRNN_E_ARG;
     }

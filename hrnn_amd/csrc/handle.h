@@ -49,7 +49,8 @@ struct hrnn_dev {
     int tune_stage1_cap = 0;         // learned early stop: stage-1 expansion cap (0: none)
     int tune_hnsw_batch_permille = 50;   // HNSW construction: batch size / graph size, per mille
     int tune_hnsw_seed_exp = 32;         // HNSW construction: expansions of the seed completion
-    int tune_insert_offer_cap = 0;       // incremental insertion: first offer buffer, entries (0: max(2^22, 8n))
+    int tune_insert_offer_cap = 0;       // incremental insertion: first offer buffer, entries (0: max(2^22, 24n))
+    int tune_insert_lds_bitmap = 1;      // incremental insertion: visited bitmap in LDS when it fits (0: in HBM)
 
     // ranked KNN graph (SoA); knn_version changes whenever it may have changed
     uint32_t K = 0;

@@ -48,14 +48,16 @@
 
 namespace hrnn {
 
-constexpr uint32_t kInsThreads = 256;
+constexpr uint32_t kInsThreads = 512;
 constexpr uint32_t kInsMaxK = 1024;     // longest list (the merge's limit)
-constexpr uint32_t kInsQueue = 1024;    // LDS queue of fresh candidates
-constexpr uint32_t kInsBest = 2 * kInsMaxK;   // bidirectional: LDS key buffer
+constexpr uint32_t kInsPerThread = 2;   // reverse-list entries per thread per scan round
+constexpr uint32_t kInsRound = kInsThreads * kInsPerThread;
+constexpr uint32_t kInsQueue = 2 * kInsRound;   // LDS queue of fresh candidates
+constexpr uint32_t kInsBest = 2 * kInsMaxK;     // bidirectional: LDS key buffer
 
 enum { IS_SCANNED = 0, IS_CAND = 1, IS_PROP = 2, IS_N = 3 };
-// static LDS of k_ins_affect (bytes), for the host's dynamic-LDS check
-constexpr size_t kInsStaticLds = (kInsMaxK + 1) * 4 + kInsMaxK * 8 + kInsQueue * 4 + 8 * 4;
+// static LDS of k_ins_affect (bytes), for the host's LDS budget
+constexpr size_t kInsStaticLds = (kInsMaxK + 1) * 4 + kInsMaxK * 8 + kInsQueue * 4 + 16 * 4 + 64;
 
 __device__ __forceinline__ unsigned long long bcast_u64(unsigned long long v) {
     const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, 0, 64);
@@ -63,32 +65,18 @@ __device__ __forceinline__ unsigned long long bcast_u64(unsigned long long v) {
     return ((unsigned long long)hi << 32) | lo;
 }
 
-// Squared L2 of row c against the staged point, by a 16-lane group (lane g):
-// 8-bit rows as exact integers (4 bytes per lane per step), FP32 rows in the
-// reference's order.  Every lane of the wave calls this (c valid in every
-// group; an unused group passes any valid row).
-template <bool kU8>
-__device__ __forceinline__ float ins_dist(const uint8_t *__restrict__ xu8, const float *__restrict__ xf,
-                                          const uint32_t *__restrict__ s_q8, const float *__restrict__ s_qf,
-                                          uint32_t d, uint32_t c, uint32_t g) {
-    if constexpr (kU8) {
-        const uint32_t *row = reinterpret_cast<const uint32_t *>(xu8 + (uint64_t)c * d);
-        uint32_t acc = 0;
-        for (uint32_t w = g; 4 * w < d; w += 16) {
-            const uint32_t a = s_q8[w], b = row[w];
+// Sum of squared byte differences of two 16-byte vectors (exact integer).
+__device__ __forceinline__ uint32_t sqdiff16(uint4 a, uint4 b) {
+    uint32_t acc = 0;
+    const uint32_t aw[4] = {a.x, a.y, a.z, a.w}, bw[4] = {b.x, b.y, b.z, b.w};
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int t = (int)((a >> (8 * k)) & 255u) - (int)((b >> (8 * k)) & 255u);
-                acc += (uint32_t)(t * t);
-            }
+    for (int w = 0; w < 4; w++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int t = (int)((aw[w] >> (8 * k)) & 255u) - (int)((bw[w] >> (8 * k)) & 255u);
+            acc += (uint32_t)(t * t);
         }
-#pragma unroll
-        for (int s = 1; s < 16; s <<= 1) acc += (uint32_t)__shfl_xor((int)acc, s, 64);
-        return (float)acc;
-    } else {
-        const float *row = xf + (uint64_t)c * d;
-        return ((d & 15u) == 0) ? l2_ref16_group(s_qf, row, d, (int)g) : l2_seq(s_qf, row, d);
-    }
+    return acc;
 }
 
 // Bitonic sort of the u64 keys s[0 .. P) ascending (P a power of two, the
@@ -111,13 +99,19 @@ __device__ void ins_sort_keys(unsigned long long *s, uint32_t P) {
         }
 }
 
-// One workgroup per new point (points b0 + w, b0 + w + gridDim.x, ...), each
-// with its own visited bitmap of bm_words u32 (a multiple of 4; all zero
-// between points).  Offers go to (out_tgt, out_key) at a slot taken from
-// *out_n; slots at or past out_cap are counted but not written.  With bidir
-// the point's new list goes to row p - b0 of (bd_ids, bd_d, bd_cnt) [K per
-// row]; the point's own list is not changed here.
-template <bool kU8, class EntT>
+// One workgroup (512 lanes) per new point (points b0 + w, b0 + w +
+// gridDim.x, ...).  The visited bitmap of n bits lives in LDS (kLdsBm) or,
+// for graphs too large for that, in HBM (bm_words u32 per workgroup, a
+// multiple of 4; all zero between points).  Offers go to (out_tgt, out_key)
+// at a slot taken from *out_n; slots at or past out_cap are counted but not
+// written.  With bidir the point's new list goes to row p - b0 of (bd_ids,
+// bd_d, bd_cnt) [K per row]; the point's own list is not changed here.
+//
+// Distances: 8-bit rows with d % 16 == 0, d <= 256 go L = d/16 rounded up to
+// a power of two lanes per row, 16 bytes per lane, four rows per lane group
+// in flight (exact integers); other 8-bit rows by 16-lane groups 4 bytes at a
+// time (exact); FP32 rows in the reference's 16-partial order.
+template <bool kU8, class EntT, bool kLdsBm>
 __global__ __launch_bounds__(kInsThreads) void k_ins_affect(
     const uint8_t *__restrict__ xu8, const float *__restrict__ xf, uint32_t d, uint32_t n,
     uint32_t *__restrict__ knn_ids, float *__restrict__ knn_d, uint32_t *__restrict__ knn_cnt, uint32_t K,
@@ -133,17 +127,30 @@ __global__ __launch_bounds__(kInsThreads) void k_ins_affect(
     __shared__ uint32_t s_qn, s_bad, s_bn;
     __shared__ uint32_t s_wsum[kInsThreads / 64];
     extern __shared__ __align__(16) unsigned char s_dyn[];
-    // dynamic LDS: the point (FP32 [d] or packed bytes), then with bidir the key buffer
+    // dynamic LDS: the point (FP32 [d] or packed bytes), with bidir the key
+    // buffer, with kLdsBm the bitmap
     float *s_qf = reinterpret_cast<float *>(s_dyn);
     uint32_t *s_q8 = reinterpret_cast<uint32_t *>(s_dyn);
-    unsigned long long *s_best =
-        reinterpret_cast<unsigned long long *>(s_dyn + (kU8 ? ((d + 15) / 16) * 16 : ((uint64_t)d * 4 + 15) / 16 * 16));
+    const uint32_t q_bytes = kU8 ? ((d + 15) / 16) * 16 : (d * 4 + 15) / 16 * 16;
+    unsigned long long *s_best = reinterpret_cast<unsigned long long *>(s_dyn + q_bytes);
+    uint32_t *bm = kLdsBm ? reinterpret_cast<uint32_t *>(s_dyn + q_bytes + (bidir ? kInsBest * 8 : 0))
+                          : bitmaps + (uint64_t)blockIdx.x * bm_words;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6;
     const uint32_t g = lane & 15u, grp = lane >> 4;
-    uint32_t *bm = bitmaps + (uint64_t)blockIdx.x * bm_words;
+    constexpr uint32_t kWaves = kInsThreads / 64;
+    // 8-bit rows 16 bytes per lane: L lanes per row, G groups per wave
+    const bool wide_u8 = kU8 && (d % 16 == 0) && d <= 256;
+    uint32_t L = 1;
+    while (L * 16 < d) L <<= 1;
+    const uint32_t G = 64 / L, gl = lane & (L - 1), gi = lane / L;
     unsigned long long st_scan = 0, st_cand = 0, st_prop = 0;
     if (tid == 0) s_bad = 0;
+    if constexpr (kLdsBm) {   // the LDS bitmap starts clear
+        for (uint32_t w = tid; w < bm_words / 4; w += kInsThreads)
+            reinterpret_cast<uint4 *>(bm)[w] = make_uint4(0u, 0u, 0u, 0u);
+        __syncthreads();
+    }
 
     for (uint32_t p = b0 + blockIdx.x; p < b1; p += gridDim.x) {
         // ---- stage the point, mark it and its list (excluded) ----
@@ -164,21 +171,22 @@ __global__ __launch_bounds__(kInsThreads) void k_ins_affect(
             }
             atomicOr(&bm[y >> 5], 1u << (y & 31));
         }
-        // proxies' reverse lists: lengths, then an exclusive scan (4 per thread)
-        uint32_t len4[4], part = 0;
+        // proxies' reverse lists: lengths, then an exclusive scan (2 per thread)
+        constexpr uint32_t kPP = (kInsMaxK + kInsThreads - 1) / kInsThreads;
+        uint32_t lenk[kPP], part = 0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const uint32_t j = tid * 4 + k;
-            len4[k] = 0;
+        for (uint32_t k = 0; k < kPP; k++) {
+            const uint32_t j = tid * kPP + k;
+            lenk[k] = 0;
             if (j < np) {
                 const uint32_t y = knn_ids[(uint64_t)p * K + j];
                 if (y < n) {
                     const uint64_t lo = csr_off[y], hi = csr_off[y + 1];
                     s_lo[j] = lo;
-                    len4[k] = (uint32_t)(hi - lo);
+                    lenk[k] = (uint32_t)(hi - lo);
                 }
             }
-            part += len4[k];
+            part += lenk[k];
         }
         uint32_t incl = part;
         for (int off = 1; off < 64; off <<= 1) {
@@ -190,18 +198,18 @@ __global__ __launch_bounds__(kInsThreads) void k_ins_affect(
             s_qn = 0;
             s_bn = 0;
         }
-        __threadfence();   // the exclusion marks are in place before any scan reads the bitmap
+        if constexpr (!kLdsBm) __threadfence();   // the exclusion marks are in place before any scan reads them
         __syncthreads();
         uint32_t run = incl - part, tot = 0;
-        for (uint32_t w = 0; w < kInsThreads / 64; w++) {
+        for (uint32_t w = 0; w < kWaves; w++) {
             if (w < wv) run += s_wsum[w];
             tot += s_wsum[w];
         }
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const uint32_t j = tid * 4 + k;
+        for (uint32_t k = 0; k < kPP; k++) {
+            const uint32_t j = tid * kPP + k;
             if (j < np) s_pref[j] = run;
-            run += len4[k];
+            run += lenk[k];
         }
         if (tid == 0) s_pref[np] = tot;
         if (bidir) {
@@ -216,76 +224,141 @@ __global__ __launch_bounds__(kInsThreads) void k_ins_affect(
         float thr = FLT_MAX;   // bidirectional: the K-th best key's distance once K are kept
         if (bidir && cnt >= K) thr = knn_d[(uint64_t)p * K + K - 1];
 
-        // ---- scan in rounds of 256 entries; fresh ids queue up in LDS ----
-        for (uint32_t base = 0; base < T; base += kInsThreads) {
-            const uint32_t pos = base + tid;
-            bool fresh = false;
-            uint32_t c = 0;
-            if (pos < T) {
+        // ---- scan in rounds of kInsRound entries; fresh ids queue up in LDS ----
+        for (uint32_t base = 0; base < T; base += kInsRound) {
+            uint32_t cc[kInsPerThread];
+            bool ok[kInsPerThread];
+            EntT ent[kInsPerThread];
+#pragma unroll
+            for (uint32_t k = 0; k < kInsPerThread; k++) {   // every load issued before any is used
+                const uint32_t pos = base + k * kInsThreads + tid;
+                ok[k] = pos < T;
+                const uint32_t ps = ok[k] ? pos : 0;
                 uint32_t lo = 0, hi = np;   // the proxy j with s_pref[j] <= pos < s_pref[j + 1]
                 while (hi - lo > 1) {
                     const uint32_t mid = (lo + hi) >> 1;
-                    if (s_pref[mid] <= pos) lo = mid;
+                    if (s_pref[mid] <= ps) lo = mid;
                     else hi = mid;
                 }
-                const EntT e = csr_ent[s_lo[lo] + (pos - s_pref[lo])];
-                c = (uint32_t)(e >> rank_bits);
-                if (c >= n) {
-                    s_bad = 1u;
-                } else {
-                    const uint32_t bit = 1u << (c & 31);
-                    fresh = (atomicOr(&bm[c >> 5], bit) & bit) == 0;
-                }
+                ent[k] = ok[k] ? csr_ent[s_lo[lo] + (ps - s_pref[lo])] : (EntT)0;
             }
-            const uint64_t bal = __ballot(fresh);
-            if (bal) {
-                uint32_t at = 0;
-                if (lane == 0) at = atomicAdd(&s_qn, (uint32_t)__popcll(bal));
-                at = (uint32_t)__shfl((int)at, 0, 64);
-                if (fresh) s_q[at + (uint32_t)__popcll(bal & lanemask_lt())] = c;
+#pragma unroll
+            for (uint32_t k = 0; k < kInsPerThread; k++) {
+                bool fresh = false;
+                cc[k] = (uint32_t)(ent[k] >> rank_bits);
+                if (ok[k]) {
+                    if (cc[k] >= n) {
+                        s_bad = 1u;
+                    } else {
+                        const uint32_t bit = 1u << (cc[k] & 31);
+                        fresh = (atomicOr(&bm[cc[k] >> 5], bit) & bit) == 0;
+                    }
+                }
+                const uint64_t bal = __ballot(fresh);
+                if (bal) {
+                    uint32_t at = 0;
+                    if (lane == 0) at = atomicAdd(&s_qn, (uint32_t)__popcll(bal));
+                    at = (uint32_t)__shfl((int)at, 0, 64);
+                    if (fresh) s_q[at + (uint32_t)__popcll(bal & lanemask_lt())] = cc[k];
+                }
             }
             __syncthreads();
             const uint32_t qn = s_qn;
             __syncthreads();   // read by every thread before the next round adds to it
-            if (qn + kInsThreads <= kInsQueue && base + kInsThreads < T) continue;   // uniform
-            // ---- distances of the queued candidates, four per wave-group pass ----
+            if (qn + kInsRound <= kInsQueue && base + kInsRound < T) continue;   // uniform
+            // ---- distances of the queued candidates ----
             st_cand += qn;
-            for (uint32_t q0 = 0; q0 < qn; q0 += 16) {
-                const uint32_t qi = q0 + wv * 4 + grp;
-                const bool valid = qi < qn;
-                const uint32_t cc = s_q[valid ? qi : q0];
-                const float dd = ins_dist<kU8>(xu8, xf, s_q8, s_qf, d, cc, g);
-                bool offer = false;
-                if (valid && g == 0) offer = dd < worst[cc];
-                const uint64_t ob = __ballot(offer);
-                if (ob) {
-                    unsigned long long at = 0;
-                    if (lane == 0) at = atomicAdd(out_n, (unsigned long long)__popcll(ob));
-                    at = bcast_u64(at);
-                    if (offer) {
-                        const unsigned long long slot = at + (unsigned long long)__popcll(ob & lanemask_lt());
-                        if (slot < out_cap) {
-                            out_tgt[slot] = cc;
-                            out_key[slot] = ((unsigned long long)float_key(dd) << 32) | p;
+            const uint32_t per_pass = wide_u8 ? kWaves * G * 4 : kWaves * 4;
+            for (uint32_t q0 = 0; q0 < qn; q0 += per_pass) {
+                uint32_t cidx[4];
+                float dd[4];
+                bool lead;   // this lane reports the group's candidates
+                if (wide_u8) {
+                    // group gi of wave wv: candidates q0 + (wv * G + gi) * 4 + r, r < 4
+                    uint4 rows[4];
+                    const uint4 qv = gl * 16 < d ? reinterpret_cast<const uint4 *>(s_q8)[gl] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const uint32_t qi = q0 + (wv * G + gi) * 4 + r;
+                        cidx[r] = qi < qn ? qi : kInvalidId;
+                        const uint32_t c = s_q[qi < qn ? qi : q0];
+                        rows[r] = gl * 16 < d ? reinterpret_cast<const uint4 *>(xu8 + (uint64_t)c * d)[gl]
+                                              : make_uint4(0, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        uint32_t acc = sqdiff16(qv, rows[r]);
+                        for (uint32_t s2 = 1; s2 < L; s2 <<= 1) acc += (uint32_t)__shfl_xor((int)acc, (int)s2, 64);
+                        dd[r] = (float)acc;
+                    }
+                    lead = gl == 0;
+                } else {
+                    // 16-lane group grp of wave wv: candidate q0 + wv * 4 + grp
+                    const uint32_t qi = q0 + wv * 4 + grp;
+                    const uint32_t c = s_q[qi < qn ? qi : q0];
+                    float dv;
+                    if constexpr (kU8) {
+                        const uint32_t *row = reinterpret_cast<const uint32_t *>(xu8 + (uint64_t)c * d);
+                        uint32_t acc = 0;
+                        for (uint32_t w = g; 4 * w < d; w += 16) {
+                            const uint32_t a = s_q8[w], b = row[w];
+#pragma unroll
+                            for (int k = 0; k < 4; k++) {
+                                const int t = (int)((a >> (8 * k)) & 255u) - (int)((b >> (8 * k)) & 255u);
+                                acc += (uint32_t)(t * t);
+                            }
+                        }
+#pragma unroll
+                        for (int s2 = 1; s2 < 16; s2 <<= 1) acc += (uint32_t)__shfl_xor((int)acc, s2, 64);
+                        dv = (float)acc;
+                    } else {
+                        const float *row = xf + (uint64_t)c * d;
+                        dv = ((d & 15u) == 0) ? l2_ref16_group(s_qf, row, d, (int)g) : l2_seq(s_qf, row, d);
+                    }
+                    cidx[0] = qi < qn ? qi : kInvalidId;
+                    cidx[1] = cidx[2] = cidx[3] = kInvalidId;
+                    dd[0] = dv;
+                    dd[1] = dd[2] = dd[3] = 0.f;
+                    lead = g == 0;
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const bool valid = lead && cidx[r] != kInvalidId;
+                    const uint32_t c = valid ? s_q[cidx[r]] : 0u;
+                    const bool offer = valid && dd[r] < worst[c];
+                    const uint64_t ob = __ballot(offer);
+                    if (ob) {
+                        unsigned long long at = 0;
+                        if (lane == 0) at = atomicAdd(out_n, (unsigned long long)__popcll(ob));
+                        at = bcast_u64(at);
+                        if (offer) {
+                            const unsigned long long slot = at + (unsigned long long)__popcll(ob & lanemask_lt());
+                            if (slot < out_cap) {
+                                out_tgt[slot] = c;
+                                out_key[slot] = ((unsigned long long)float_key(dd[r]) << 32) | p;
+                            }
+                        }
+                        if (lane == 0) st_prop += (unsigned long long)__popcll(ob);
+                    }
+                    if (bidir) {
+                        // keys below the running K-th best join x's buffer
+                        const bool keep = valid && dd[r] < thr;
+                        const uint64_t kb = __ballot(keep);
+                        if (kb) {
+                            uint32_t at = 0;
+                            if (lane == 0) at = atomicAdd(&s_bn, (uint32_t)__popcll(kb));
+                            at = (uint32_t)__shfl((int)at, 0, 64);
+                            const uint32_t slot = at + (uint32_t)__popcll(kb & lanemask_lt());
+                            if (keep && slot < kInsBest)
+                                s_best[slot] = ((unsigned long long)float_key(dd[r]) << 32) | c;
                         }
                     }
-                    if (lane == 0) st_prop += (unsigned long long)__popcll(ob);
                 }
                 if (bidir) {
-                    // keys below the running K-th best join x's buffer
-                    const bool keep = valid && g == 0 && dd < thr;
-                    const uint64_t kb = __ballot(keep);
-                    if (kb) {
-                        uint32_t at = 0;
-                        if (lane == 0) at = atomicAdd(&s_bn, (uint32_t)__popcll(kb));
-                        at = (uint32_t)__shfl((int)at, 0, 64);
-                        const uint32_t slot = at + (uint32_t)__popcll(kb & lanemask_lt());
-                        if (keep && slot < kInsBest) s_best[slot] = ((unsigned long long)float_key(dd) << 32) | cc;
-                    }
                     __syncthreads();
                     const uint32_t bn_now = s_bn;
                     __syncthreads();   // read by every thread before the next pass adds to it
-                    if (bn_now + 64 > kInsBest) {   // uniform: compact to the best K
+                    if (bn_now + per_pass > kInsBest) {   // uniform: compact to the best K
                         const uint32_t bn = min(bn_now, kInsBest);
                         for (uint32_t t = bn + tid; t < kInsBest; t += kInsThreads) s_best[t] = ~0ull;
                         __syncthreads();
@@ -310,11 +383,12 @@ __global__ __launch_bounds__(kInsThreads) void k_ins_affect(
             ins_sort_keys(s_best, kInsBest);
             // drop repeats (an id listed and met again has the same distance,
             // so the copies are adjacent) and x itself, keep the first K
-            uint32_t keepf[kInsBest / kInsThreads];
+            constexpr uint32_t kPB = kInsBest / kInsThreads;
+            uint32_t keepf[kPB];
             uint32_t part2 = 0;
 #pragma unroll
-            for (uint32_t k = 0; k < kInsBest / kInsThreads; k++) {
-                const uint32_t t = tid * (kInsBest / kInsThreads) + k;
+            for (uint32_t k = 0; k < kPB; k++) {
+                const uint32_t t = tid * kPB + k;
                 const unsigned long long v = s_best[t];
                 keepf[k] = (v != ~0ull && (uint32_t)v != p && (t == 0 || s_best[t - 1] != v)) ? 1u : 0u;
                 part2 += keepf[k];
@@ -330,11 +404,11 @@ __global__ __launch_bounds__(kInsThreads) void k_ins_affect(
             uint32_t r2 = inc2 - part2;
             for (uint32_t w = 0; w < wv; w++) r2 += s_wsum[w];
             uint32_t total = 0;
-            for (uint32_t w = 0; w < kInsThreads / 64; w++) total += s_wsum[w];
+            for (uint32_t w = 0; w < kWaves; w++) total += s_wsum[w];
             const uint64_t row = (uint64_t)(p - b0) * K;
 #pragma unroll
-            for (uint32_t k = 0; k < kInsBest / kInsThreads; k++) {
-                const uint32_t t = tid * (kInsBest / kInsThreads) + k;
+            for (uint32_t k = 0; k < kPB; k++) {
+                const uint32_t t = tid * kPB + k;
                 if (keepf[k] && r2 < K) {
                     const unsigned long long v = s_best[t];
                     bd_ids[row + r2] = (uint32_t)v;
@@ -347,7 +421,7 @@ __global__ __launch_bounds__(kInsThreads) void k_ins_affect(
 
         // ---- clear the bitmap: whole when that is cheaper than the walk ----
         __syncthreads();
-        if ((uint64_t)bm_words <= (uint64_t)T + cnt) {
+        if (kLdsBm || (uint64_t)bm_words <= (uint64_t)T + cnt) {
             uint4 *bm4 = reinterpret_cast<uint4 *>(bm);
             for (uint32_t w = tid; w < bm_words / 4; w += kInsThreads) bm4[w] = make_uint4(0u, 0u, 0u, 0u);
         } else {
@@ -367,7 +441,7 @@ __global__ __launch_bounds__(kInsThreads) void k_ins_affect(
                 if (c < n) bm[c >> 5] = 0;
             }
         }
-        __threadfence();   // cleared before the next point's marks
+        if constexpr (!kLdsBm) __threadfence();   // cleared before the next point's marks
         __syncthreads();
     }
     if (tid == 0) {   // st_scan and st_cand are the same in every lane: counted once
@@ -425,8 +499,8 @@ struct InsertCall {
     uint64_t n;
     uint32_t K, d, m_update, slots_max, bm_words;
     int bidir;
-    bool u8;
-    size_t lds;
+    bool u8, lds_bm;   // lds_bm: the visited bitmap in LDS (n bits fit beside the rest)
+    size_t lds;        // dynamic LDS of a launch
     uint64_t cap;
     DevBuf bitmaps, worst, pcnt, poff, pkey, otgt, okey, ctr, bdi, bdd, bdc;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -439,18 +513,45 @@ struct InsertCall {
         d = h->d;
         REQUIRE(n * K < 0xFFFFFFFFull, HRNN_E_ARG, "insert: n * K must stay below 2^32");
         u8 = h->base_u8 && d <= 258 && d % 4 == 0;
-        lds = (u8 ? ((d + 15) / 16) * 16 : ((size_t)d * 4 + 15) / 16 * 16) + (bidir ? kInsBest * 8 : 0);
-        REQUIRE(lds + kInsStaticLds <= 65536, HRNN_E_ARG, "insert: the point's row does not fit in LDS (d too large)");
+        const size_t base_lds = (u8 ? ((d + 15) / 16) * 16 : ((size_t)d * 4 + 15) / 16 * 16) + (bidir ? kInsBest * 8 : 0);
+        REQUIRE(base_lds + kInsStaticLds <= 65536, HRNN_E_ARG,
+                "insert: the point's row does not fit in LDS (d too large)");
         hipStream_t st = h->stream;
-        slots_max = (uint32_t)std::max(1, h->n_cu) * 4;
         bm_words = (uint32_t)((n + 127) / 128) * 4;
-        bitmaps.reserve((uint64_t)slots_max * bm_words * 4);
-        HIPC(hipMemsetAsync(bitmaps.p, 0, (uint64_t)slots_max * bm_words * 4, st));
+        const size_t lds_cap = std::max<size_t>(h->lds_per_block, 65536);
+        lds_bm = h->tune_insert_lds_bitmap != 0 && base_lds + (size_t)bm_words * 4 + kInsStaticLds <= lds_cap;
+        const uint32_t n_cu = (uint32_t)std::max(1, h->n_cu);
+        if (lds_bm) {
+            lds = base_lds + (size_t)bm_words * 4;
+            // workgroups that fit a CU's LDS at once (two waves per SIMD at most)
+            const uint32_t per_cu = (uint32_t)std::min<size_t>(2, std::max<size_t>(1, lds_cap / (lds + kInsStaticLds)));
+            slots_max = n_cu * per_cu;
+            if (lds > 65536) {
+                const hipError_t e1 = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ins_affect<true, uint32_t, true>),
+                                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                const hipError_t e2 = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ins_affect<true, uint64_t, true>),
+                                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                const hipError_t e3 = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ins_affect<false, uint32_t, true>),
+                                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                const hipError_t e4 = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ins_affect<false, uint64_t, true>),
+                                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
+                    (void)hipGetLastError();
+                    lds_bm = false;   // the runtime refuses that much: bitmaps in HBM
+                }
+            }
+        }
+        if (!lds_bm) {
+            lds = base_lds;
+            slots_max = n_cu * 2;
+            bitmaps.reserve((uint64_t)slots_max * bm_words * 4);
+            HIPC(hipMemsetAsync(bitmaps.p, 0, (uint64_t)slots_max * bm_words * 4, st));
+        }
         worst.reserve(n * 4);
         pcnt.reserve(n * 8 + 64);   // counts, then fill positions
         poff.reserve((n + 1 + scan_block_count(n) + 1) * 8 + 64);
         cap = h->tune_insert_offer_cap > 0 ? (uint64_t)h->tune_insert_offer_cap
-                                           : std::max<uint64_t>(1 << 22, n * 8);   // offers per batch (grows on demand)
+                                           : std::max<uint64_t>(1 << 22, n * 24);   // offers per batch (grows on demand)
         otgt.reserve(cap * 4);
         okey.reserve(cap * 8);
         pkey.reserve(cap * 8);
@@ -495,19 +596,29 @@ struct InsertCall {
         unsigned long long total = 0, bc[16];
         for (;;) {
             HIPC(hipMemsetAsync(c_out, 0, 16 * 8, st));
-#define LAUNCH(U8, ET)                                                                                               \
-    hipLaunchKernelGGL((k_ins_affect<U8, ET>), dim3(grid), dim3(kInsThreads), lds, st, h->xu8.as<uint8_t>(),         \
+#define LAUNCH(U8, ET, LB)                                                                                           \
+    hipLaunchKernelGGL((k_ins_affect<U8, ET, LB>), dim3(grid), dim3(kInsThreads), lds, st, h->xu8.as<uint8_t>(),     \
                        h->x.as<float>(), d, (uint32_t)n, h->knn_ids.as<uint32_t>(), h->knn_d.as<float>(),            \
                        h->knn_cnt.as<uint32_t>(), K, worst.as<float>(), h->csr_off.as<uint64_t>(),                    \
                        h->csr_ent.as<ET>(), h->rank_bits, b, b1, m_update, bidir, bitmaps.as<uint32_t>(), bm_words,   \
                        otgt.as<uint32_t>(), okey.as<unsigned long long>(), (unsigned long long)cap, c_out,           \
                        bdi.as<uint32_t>(), bdd.as<float>(), bdc.as<uint32_t>(), c_st, err)
-            if (h->wide) {
-                if (u8) LAUNCH(true, uint64_t);
-                else LAUNCH(false, uint64_t);
+            if (lds_bm) {
+                if (h->wide) {
+                    if (u8) LAUNCH(true, uint64_t, true);
+                    else LAUNCH(false, uint64_t, true);
+                } else {
+                    if (u8) LAUNCH(true, uint32_t, true);
+                    else LAUNCH(false, uint32_t, true);
+                }
             } else {
-                if (u8) LAUNCH(true, uint32_t);
-                else LAUNCH(false, uint32_t);
+                if (h->wide) {
+                    if (u8) LAUNCH(true, uint64_t, false);
+                    else LAUNCH(false, uint64_t, false);
+                } else {
+                    if (u8) LAUNCH(true, uint32_t, false);
+                    else LAUNCH(false, uint32_t, false);
+                }
             }
 #undef LAUNCH
             HIPC(hipGetLastError());

@@ -5,8 +5,11 @@
 // (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as easy as 1, 2, 3",
 // SC'11), so a value depends only on (seed, stream, index) and the bytes are
 // the same for any thread count.  Normals come from the Box-Muller transform.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <numeric>
+#include <random>
 #include <vector>
 
 #include "../../include/hrnn_dev.h"
@@ -174,6 +177,19 @@ int hrnn_synth_sphere_mixture(uint64_t seed, uint64_t centre_seed, uint64_t n, u
         nrm = std::sqrt(nrm);
         for (uint32_t j = 0; j < d; j++) out[(size_t)i * d + j] = (float)(v[j] / nrm);
     }
+    return HRNN_OK;
+}
+
+// The insertion order of the dynamic test: 0 .. n-1 shuffled by std::shuffle
+// with std::mt19937(seed) (reference src/hrnn_dynamic_test.cpp:200-203), the
+// same permutation wherever both are built against libstdc++.
+int hrnn_synth_split_perm(uint64_t n, uint32_t seed, uint32_t *out) {
+    if (!out || n >= 0xFFFFFFFFull) return HRNN_E_ARG;
+    std::vector<uint32_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0u);
+    std::mt19937 rng(seed);
+    std::shuffle(perm.begin(), perm.end(), rng);
+    std::copy(perm.begin(), perm.end(), out);
     return HRNN_OK;
 }
 

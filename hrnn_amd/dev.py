@@ -42,7 +42,7 @@ EXPORTS = [
     "hrnn_dev_mlp_grad", "hrnn_dev_mlp_train", "hrnn_dev_rknn_search_learned_dptr",
     "hrnn_dev_stage1_features_dptr", "hrnn_dev_rknn_query_adaptive_dptr",
     "hrnn_synth_mixture",
-    "hrnn_synth_halfnormal_scaled", "hrnn_synth_sphere_mixture",
+    "hrnn_synth_halfnormal_scaled", "hrnn_synth_sphere_mixture", "hrnn_synth_split_perm",
 ]
 
 
@@ -141,6 +141,7 @@ def lib() -> C.CDLL:
         "hrnn_synth_mixture": [u64, u64, u32, u32, f, f, f, i, C.c_double, u64, vp, vp],
         "hrnn_synth_halfnormal_scaled": [u64, u64, u64, u32, vp],
         "hrnn_synth_sphere_mixture": [u64, u64, u64, u32, u32, f, vp, vp],
+        "hrnn_synth_split_perm": [u64, u32, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -184,6 +185,16 @@ def synth_halfnormal_scaled(seed: int, scale_seed: int, n: int, d: int):
     rc = lib().hrnn_synth_halfnormal_scaled(seed, scale_seed, n, d, _p(out))
     if rc:
         raise HrnnError(rc, "synth_halfnormal_scaled")
+    return out
+
+
+def synth_split_perm(n: int, seed: int = 42):
+    """0 .. n-1 shuffled as std::shuffle with std::mt19937(seed) shuffles them
+    (the reference dynamic test's split order)."""
+    out = np.empty(n, np.uint32)
+    rc = lib().hrnn_synth_split_perm(n, seed, _p(out))
+    if rc:
+        raise HrnnError(rc, "synth_split_perm")
     return out
 
 

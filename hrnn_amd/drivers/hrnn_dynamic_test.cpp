@@ -1,8 +1,10 @@
 // hrnn_dynamic_test -- streaming-insert harness (CLI of the reference's
 // src/hrnn_dynamic_test.cpp: same flags, defaults, split and report lines).
 // Initial split = first n*split ids of a mt19937(42) permutation; build (or
-// load a snapshot) -> mutable reverse lists -> insertPointDynamic for the
-// rest (OpenMP) -> flat buildRKNNG (GPU) -> searchRknn evaluation.
+// load a snapshot; HNSW + KNN seeds on the GPU unless --hnsw_device 0) ->
+// insertion of the rest: on the GPU in batches (insertPointsDynamicDevice;
+// --insert_device 0: mutable reverse lists and insertPointDynamic per point on
+// the host threads) -> flat buildRKNNG (GPU) -> searchRknn evaluation.
 #include <omp.h>
 
 #include <algorithm>
@@ -59,6 +61,8 @@ int main(int argc, char **argv) {
     cli.add("split", "0.5", "Fraction of data for the initial build");
     cli.add("m_update", "0", "Proxy count for the insert self-query (0 = all KNNG neighbours)");
     cli.add("bidirectional", "0", "1 = also improve the new point's KNNG from the self-query");
+    cli.add("hnsw_device", "1", "1 = initial HNSW construction on the GPU, 0 = addPoint on the host threads");
+    cli.add("insert_device", "1", "1 = insertion on the GPU in batches, 0 = insertPointDynamic per point on the host threads");
     try {
         if (!cli.parse(argc, argv)) return 0;
         const int M = cli.i("M"), efc = cli.i("ef_construction"), K = cli.i("K_knng");
@@ -104,15 +108,25 @@ int main(int argc, char **argv) {
             std::cout << "      Loaded (" << load_ms << " ms)  cur_elements=" << index->getCurrentElementCount()
                       << "  K_knng=" << index->K_knng_ << "  knng_built=" << index->knng_built_ << std::endl;
             t0 = Clock::now();
-            index->buildRKNNG_mutable();
+            if (!cli.i("insert_device")) index->buildRKNNG_mutable();
             build_ms = load_ms + elapsed_ms(t0);
         } else {
             std::cout << "\n[3] Building HRNN on initial " << split_n << " points" << std::endl;
             index = std::make_unique<HRNN>(&space, n, M, efc, 100, false, (size_t)K);
             t0 = Clock::now();
+            if (cli.i("hnsw_device")) {
+                std::vector<float> xs(split_n * base.d);
+                std::vector<hnswlib::labeltype> ls(split_n);
+                for (size_t i = 0; i < split_n; ++i) {
+                    std::copy_n(base.data.data() + (size_t)perm[i] * base.d, base.d, xs.data() + i * base.d);
+                    ls[i] = perm[i];
+                }
+                index->addPointsDevice(xs.data(), split_n, ls.data());
+            } else {
 #pragma omp parallel for schedule(dynamic, 512)
-            for (long long i = 0; i < (long long)split_n; ++i)
-                index->addPoint(base.data.data() + (size_t)perm[i] * base.d, (hnswlib::labeltype)perm[i]);
+                for (long long i = 0; i < (long long)split_n; ++i)
+                    index->addPoint(base.data.data() + (size_t)perm[i] * base.d, (hnswlib::labeltype)perm[i]);
+            }
             long hnsw_ms = elapsed_ms(t0);
             std::cout << "      HNSW + initial KNNG: " << hnsw_ms << " ms" << std::endl;
             std::cout << "\n[3b] NNDescent refinement (" << cli.i("nn_iters") << " iters)" << std::endl;
@@ -124,25 +138,45 @@ int main(int argc, char **argv) {
                 std::cout << "\n[3s] Saving snapshot to " << snap << std::endl;
                 index->saveIndex(snap);
             }
-            std::cout << "\n[3c] Building mutable RKNNG" << std::endl;
-            t0 = Clock::now();
-            index->buildRKNNG_mutable();
-            long rk_ms = elapsed_ms(t0);
-            std::cout << "      RKNNG: " << rk_ms << " ms" << std::endl;
+            long rk_ms = 0;
+            if (!cli.i("insert_device")) {
+                std::cout << "\n[3c] Building mutable RKNNG" << std::endl;
+                t0 = Clock::now();
+                index->buildRKNNG_mutable();
+                rk_ms = elapsed_ms(t0);
+                std::cout << "      RKNNG: " << rk_ms << " ms" << std::endl;
+            }
             build_ms = hnsw_ms + refine_ms + rk_ms;
         }
         std::cout << "\n      Total initial build/load: " << build_ms << " ms" << std::endl;
 
         std::cout << "\n[4] Inserting " << batch_n << " points incrementally" << std::endl;
-        std::vector<long> per_us(batch_n, 0);
+        std::vector<long> per_us;
         t0 = Clock::now();
+        if (cli.i("insert_device")) {
+            // one batched call: per-point latency is the amortised time
+            std::vector<float> xs(batch_n * base.d);
+            std::vector<hnswlib::labeltype> ls(batch_n);
+            for (size_t i = 0; i < batch_n; ++i) {
+                std::copy_n(base.data.data() + (size_t)perm[split_n + i] * base.d, base.d, xs.data() + i * base.d);
+                ls[i] = perm[split_n + i];
+            }
+            index->insertPointsDynamicDevice(xs.data(), batch_n, ls.data(), (unsigned)m_update, bidir != 0);
+            const auto &st = index->lastInsertStats();
+            std::cout << "      GPU insertion: " << st.batches << " batches, " << st.candidates
+                      << " candidates, " << st.offers << " offers, " << st.updates << " updates  (HNSW "
+                      << (long)st.hnsw_ms << " ms, reverse lists " << (long)st.csr_ms << " ms, scan "
+                      << (long)st.affect_ms << " ms, merge " << (long)st.merge_ms << " ms)" << std::endl;
+        } else {
+            per_us.assign(batch_n, 0);
 #pragma omp parallel for schedule(dynamic, 64)
-        for (long long i = 0; i < (long long)batch_n; ++i) {
-            uint32_t id = perm[split_n + i];
-            auto ts = Clock::now();
-            index->insertPointDynamic(base.data.data() + (size_t)id * base.d, (hnswlib::labeltype)id,
-                                      (unsigned)m_update, bidir != 0);
-            per_us[i] = elapsed_us(ts);
+            for (long long i = 0; i < (long long)batch_n; ++i) {
+                uint32_t id = perm[split_n + i];
+                auto ts = Clock::now();
+                index->insertPointDynamic(base.data.data() + (size_t)id * base.d, (hnswlib::labeltype)id,
+                                          (unsigned)m_update, bidir != 0);
+                per_us[i] = elapsed_us(ts);
+            }
         }
         long maint_ms = elapsed_ms(t0);
         double per_ms = batch_n ? (double)maint_ms / (double)batch_n : 0.0;

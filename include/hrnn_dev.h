@@ -100,8 +100,11 @@ int hrnn_dev_set_vector_precision(hrnn_dev_t h, int prec);
  * queries).  "hnsw_batch_permille" (1 .. 1000, default 50) and
  * "hnsw_seed_exp" (0 .. 1000, default 32): the HNSW construction's batch size
  * relative to the graph and its seed completion.  "insert_offer_cap"
- * (0 .. 2^30, default 0 = max(2^22, 8n)): the incremental insertion's first
- * offer buffer (entries per batch; it grows when a batch needs more). */
+ * (0 .. 2^30, default 0 = max(2^22, 24n)): the incremental insertion's first
+ * offer buffer (entries per batch; it grows when a batch needs more);
+ * "insert_lds_bitmap" (0 or 1, default 1): the insertion's visited bitmap in
+ * LDS when n bits fit there (160 KiB per workgroup on gfx950), else in HBM
+ * (the same results). */
 int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value);
 int hrnn_dev_get_tuning(hrnn_dev_t h, const char *key, int *value);
 
@@ -477,6 +480,9 @@ int hrnn_synth_halfnormal_scaled(uint64_t seed, uint64_t scale_seed, uint64_t n,
 int hrnn_synth_sphere_mixture(uint64_t seed, uint64_t centre_seed, uint64_t n, uint32_t d,
                               uint32_t n_centres, float kappa, float *out,
                               uint32_t *centre_of);
+/* 0 .. n-1 in the order std::shuffle with std::mt19937(seed) leaves them: the
+ * dynamic test's split order (reference src/hrnn_dynamic_test.cpp:200-203). */
+int hrnn_synth_split_perm(uint64_t n, uint32_t seed, uint32_t *out);
 
 #ifdef __cplusplus
 }

@@ -78,3 +78,41 @@ def test_gt_build_search_drivers_c1(tmp_path):
         rec = oracle.rknn_recall(e_off, e_ids, r_off, r_lab.astype(np.uint32), 10_000)[0]
         assert rec_p == pytest.approx(rec, abs=2e-6), (m, rec_p, rec)
     assert printed[-1] >= 0.95
+
+
+def test_dynamic_test_driver_gpu_and_host_insertion(tmp_path):
+    """hrnn_dynamic_test (flags, split and report lines of
+    src/hrnn_dynamic_test.cpp): the initial half built and saved as a
+    snapshot, the rest inserted on the GPU in batches; then from the same
+    snapshot inserted per point on the host threads; the full-build index as
+    the oracle snapshot.  Both insertions reach the recall the printed
+    summary reports within half a point of each other, and the printed
+    recall equals the library's search of the same index scored by the
+    oracle (the gap line is the difference to the full build)."""
+    n = 8000
+    x = hrnn_amd.synth_mixture(5, n, 128, 40, 20.0, 0.0, 255.0, True, 0.0, 1005)
+    q = hrnn_amd.synth_mixture(6, 100, 128, 40, 20.0, 0.0, 255.0, True, 0.0, 1005)
+    xb, qb = str(tmp_path / "base.fbin"), str(tmp_path / "query.fbin")
+    _fbin(xb, x)
+    _fbin(qb, q)
+    gt_path, full_idx, snap = str(tmp_path / "gt.bin"), str(tmp_path / "full.idx"), str(tmp_path / "snap.idx")
+    _run([os.path.join(BIN, "hrnn_gt"), "--base", xb, "--query", qb, "--k", "10", "--output", gt_path])
+    common = ["--M", "16", "--ef_construction", "100", "--K_knng", "64", "--nn_iters", "4", "--nn_sample", "20",
+              "--num_threads", "8"]
+    _run([os.path.join(BIN, "hrnn_build"), "--base", xb, "--index_path", full_idx] + common)
+    dyn = [os.path.join(BIN, "hrnn_dynamic_test"), "--base", xb, "--query", qb, "--gt", gt_path,
+           "--snapshot", snap, "--oracle_snapshot", full_idx, "--k", "10", "--m", "10", "--ef_search", "100"] + common
+    out_gpu = _run(dyn + ["--insert_device", "1"])
+    assert os.path.exists(snap) and "GPU insertion:" in out_gpu and "Oracle macro gap" in out_gpu
+    out_host = _run(dyn + ["--insert_device", "0"])   # loads the snapshot the first run saved
+    assert "Loading snapshot" in out_host and "Per-insert wall" in out_host
+
+    def summary(out):
+        macro = float(re.findall(r"  Macro Recall = ([0-9.]+)", out)[-1])
+        gap = float(re.findall(r"Oracle macro gap = (-?[0-9.]+)", out)[-1])
+        return macro, gap
+
+    (mg, gg), (mh, gh) = summary(out_gpu), summary(out_host)
+    print(f"dynamic driver: GPU insertion recall {mg:.4f} (gap {gg:+.4f}), host {mh:.4f} (gap {gh:+.4f})")
+    assert mg >= mh - 0.005, (mg, mh)
+    assert mg > 0.9

@@ -92,9 +92,10 @@ def _batch_state(x, n_old, nb, K, seed_len):
     return ids, d, c
 
 
-@pytest.mark.parametrize("kind,bidir,m_update", [("u8", False, 0), ("u8", False, 5), ("f32", False, 0),
-                                                 ("f32", True, 0), ("f32", True, 4)])
-def test_insert_batch_equals_oracle(gpu_dev, kind, bidir, m_update):
+@pytest.mark.parametrize("kind,bidir,m_update,lds_bm", [("u8", False, 0, 1), ("u8", False, 5, 1), ("f32", False, 0, 1),
+                                                        ("f32", True, 0, 1), ("f32", True, 4, 1), ("u8", False, 0, 0),
+                                                        ("f32", True, 0, 0)])
+def test_insert_batch_equals_oracle(gpu_dev, kind, bidir, m_update, lds_bm):
     if not oracle.ref_available():
         pytest.skip("compiled reference (oracle/_ref) not available")
     n_old, nb, K = 1500, 60, 40
@@ -104,16 +105,22 @@ def test_insert_batch_equals_oracle(gpu_dev, kind, bidir, m_update):
         x = np.random.default_rng(9).standard_normal((n_old + nb, 96)).astype(np.float32)
     s = _batch_state(x, n_old, nb, K, 24)
     want = oracle.insert_affect(x, *s, K, n_old, n_old + nb, m_update, bidir)
-    got, st = _gpu_step(gpu_dev, x, s, K, n_old, n_old + nb, m_update, bidir)
+    gpu_dev.set_tuning("insert_lds_bitmap", lds_bm)   # visited bitmap in LDS or in HBM
+    try:
+        got, st = _gpu_step(gpu_dev, x, s, K, n_old, n_old + nb, m_update, bidir)
+    finally:
+        gpu_dev.set_tuning("insert_lds_bitmap", 1)
     _same(got, want[:3])
     assert st["offers"] == int(want[3][2]) > 0
     assert st["scanned"] == int(want[3][0]) and st["candidates"] == int(want[3][1])
 
 
-def test_insert_sparse_graph_walk_clear_and_offer_retry(gpu_dev):
-    """n = 100k rows with 8-entry lists: a point scans far fewer entries than
-    its bitmap has words, so the kernel clears by walking the lists; an offer
-    buffer of 64 entries makes every batch grow it and scan again."""
+@pytest.mark.parametrize("lds_bm", [0, 1])
+def test_insert_sparse_graph_walk_clear_and_offer_retry(gpu_dev, lds_bm):
+    """n = 100k rows with 8-entry lists: with the bitmap in HBM a point scans
+    far fewer entries than its bitmap has words, so the kernel clears by
+    walking the lists; an offer buffer of 64 entries makes every batch grow it
+    and scan again."""
     rng = np.random.default_rng(3)
     n, d, K, nb = 100_000, 32, 8, 3000
     x = rng.standard_normal((n, d)).astype(np.float32)
@@ -129,10 +136,12 @@ def test_insert_sparse_graph_walk_clear_and_offer_retry(gpu_dev):
     b0 = n - nb
     want = oracle.insert_affect(x, ids, dd, cnt, K, b0, n, 0)
     gpu_dev.set_tuning("insert_offer_cap", 64)
+    gpu_dev.set_tuning("insert_lds_bitmap", lds_bm)
     try:
         got, st = _gpu_step(gpu_dev, x, (ids, dd, cnt), K, b0, n)
     finally:
         gpu_dev.set_tuning("insert_offer_cap", 0)
+        gpu_dev.set_tuning("insert_lds_bitmap", 1)
     _same(got, want[:3])
     assert st["retries"] >= 1 and st["offers"] == int(want[3][2]) > 64
 
