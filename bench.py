@@ -522,6 +522,32 @@ def main() -> int:
         log(f"learned early stop: tau {tau_a}, recall {rec_ad:.4f}, escalated "
             f"{paths['bf16_learned_adaptive']['escalated']:.3f}, classes {np.bincount(cls_h, minlength=16).tolist()}")
 
+        # ---- the learned part's own gain: every (ef, m) class as a fixed path
+        # for all queries (no MLP, u8 mirror); the fastest one whose macro
+        # recall reaches the target is the no-model baseline
+        ck(L.hrnn_dev_set_vector_precision(h, 3))
+        class_sweep = []
+        for ef_c, m_c in CLASSES.tolist():
+            def run_cls(ef_c=ef_c, m_c=m_c):
+                ck(L.hrnn_dev_rknn_query_dptr(h, qd.ptr, nq, m_c, max(ef_c, m_c), None, k, Kp, 1.0, pd.ptr,
+                                              cd.ptr, idd.ptr, cap, wd.ptr))
+            run_cls()
+            c_off, c_ids = results()
+            entry = {"ef": ef_c, "m": m_c, "recall": round(macro_recall(gt_off, gt_ids, c_off, c_ids), 5)}
+            if entry["recall"] >= args.recall_target:
+                c_sec, c_res = timed_path(run_cls)
+                entry.update(sec=c_sec, sec_resident=c_res, qps=round(nq / c_sec, 1))
+            class_sweep.append(entry)
+        ok_cls = [c_ for c_ in class_sweep if "qps" in c_]
+        if ok_cls:
+            fc = max(ok_cls, key=lambda c_: c_["qps"])
+            paths["u8_fixed_class"] = {"ef": fc["ef"], "m": fc["m"], "recall": fc["recall"], "phase_a": "gpu",
+                                       "sec": fc["sec"], "sec_resident": fc["sec_resident"]}
+        learned_info["fixed_class_sweep"] = [{k2: v2 for k2, v2 in c_.items() if k2 not in ("sec", "sec_resident")}
+                                             for c_ in class_sweep]
+        ck(L.hrnn_dev_set_vector_precision(h, 1))
+        log(f"fixed classes without the model: {learned_info['fixed_class_sweep']}")
+
     for name, p in paths.items():
         p["qps"] = round(nq / p["sec"], 1)
         p["ms_per_step"] = round(p["sec"] * 1e3, 3)
@@ -579,11 +605,12 @@ def main() -> int:
                           + exq[1, :, 1] * s_v * d).sum())
         m_b = 16
     elif best.get("phase_a") == "gpu":
-        ck(L.hrnn_dev_proxies_dptr(h, qd.ptr, nq, m_b, max(ef, m_b), pd.ptr, ed.ptr))
+        ef_b = best.get("ef", ef)   # a fixed class runs its own ef
+        ck(L.hrnn_dev_proxies_dptr(h, qd.ptr, nq, m_b, max(ef_b, m_b), pd.ptr, ed.ptr))
         hiprt.stream_sync(stream)
         e0.record(stream)
         for _ in range(reps):
-            ck(L.hrnn_dev_proxies_dptr(h, qd.ptr, nq, m_b, max(ef, m_b), pd.ptr, ed.ptr))
+            ck(L.hrnn_dev_proxies_dptr(h, qd.ptr, nq, m_b, max(ef_b, m_b), pd.ptr, ed.ptr))
         e1.record(stream)
         e1.sync()
         beam_ms = e1.ms_since(e0) / reps
@@ -642,13 +669,21 @@ def main() -> int:
     cal_dst = hiprt.DeviceArray((cal_bytes // 4,), np.uint32)
     cal_dst.copy_from_device(cal_src, cal_bytes)
     del cal_src, cal_dst
-    traffic = None
+    # The counters need runs of their own (rocprofv3 --pmc, one counter per
+    # run), so `traffic` is the figure those runs recorded for this kernel and
+    # path, tagged with the tree and date they ran on and the algorithmic
+    # bytes of their own launch (the ratio is what compares across runs).
+    traffic, traffic_src = None, None
     pmc_file = os.path.join(ROOT, "profiles", f"traffic_{dom}_hrnn-{args.config}.json")
     if os.path.exists(pmc_file):
         with open(pmc_file) as f:
             tr = json.load(f)
         if tr.get("nq") == nq and tr.get("m") == m_b and tr.get("path", best_name) == best_name:
             traffic = tr.get("hbm_bytes_per_launch")
+            traffic_src = {"file": os.path.relpath(pmc_file, ROOT), "tree": tr.get("tree"), "date": tr.get("date"),
+                           "alg_bytes_per_launch_then": tr.get("alg_bytes_per_launch"),
+                           "traffic_over_alg_then": (round(traffic / tr["alg_bytes_per_launch"], 3)
+                                                     if tr.get("alg_bytes_per_launch") else None)}
 
     # ---- cpu baseline: the reference's searchRknn on this run's index ----------------------
     # All host cores (OpenMP over queries, the reference's const re-entrant
@@ -889,7 +924,8 @@ def main() -> int:
                    "nn_sample": cfg["S"], "k": k, "K_prime": Kp, "ef_search": ef, "path": best_name,
                    "recall": best["recall"], "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": dom,
                      "kernel_ms": round(kern[dom]["ms"], 4), "alg_bytes_per_launch": int(kern[dom]["bytes"]),
                      "queries_per_launch": nq,
                      "kernels": {kk: {k2: (round(v2, 4) if isinstance(v2, float) else v2) for k2, v2 in v.items()}
@@ -907,6 +943,11 @@ def main() -> int:
         "bf16_learned_speedup_vs_fp32_fixed_m": (
             round(paths["bf16_learned_adaptive"]["qps"] / paths["fp32_fixed_m"]["qps"], 3)
             if "bf16_learned_adaptive" in paths and "fp32_fixed_m" in paths else None),
+        # the model's own share: the learned path against the fastest fixed
+        # (ef, m) class that reaches the target without it (same u8 mirror)
+        "learned_speedup_vs_fastest_fixed_class": (
+            round(paths["u8_learned_adaptive"]["qps"] / paths["u8_fixed_class"]["qps"], 3)
+            if "u8_learned_adaptive" in paths and "u8_fixed_class" in paths else None),
         "learned_m": learned_info,
         "autotune": {kk: v for kk, v in tuned.items() if kk != "timings_ms"},
         "nndescent_cpu_baseline": nnd_cpu,

@@ -116,3 +116,24 @@ def test_dynamic_test_driver_gpu_and_host_insertion(tmp_path):
     print(f"dynamic driver: GPU insertion recall {mg:.4f} (gap {gg:+.4f}), host {mh:.4f} (gap {gh:+.4f})")
     assert mg >= mh - 0.005, (mg, mh)
     assert mg > 0.9
+
+
+def test_search_driver_serve_mode(tmp_path):
+    """hrnn_search --max_k_serve (src/hrnn_search.cpp): the recall printed in
+    serve mode equals the recall printed without it, for the same index,
+    queries and groundtruth."""
+    x = hrnn_amd.synth_mixture(7, 6000, 128, 40, 20.0, 0.0, 255.0, True, 0.0, 1007)
+    q = hrnn_amd.synth_mixture(8, 100, 128, 40, 20.0, 0.0, 255.0, True, 0.0, 1007)
+    xb, qb = str(tmp_path / "base.fbin"), str(tmp_path / "query.fbin")
+    _fbin(xb, x)
+    _fbin(qb, q)
+    gt_path, idx = str(tmp_path / "gt.bin"), str(tmp_path / "s.idx")
+    _run([os.path.join(BIN, "hrnn_gt"), "--base", xb, "--query", qb, "--k", "10", "--output", gt_path])
+    _run([os.path.join(BIN, "hrnn_build"), "--base", xb, "--index_path", idx, "--M", "16", "--ef_construction",
+          "100", "--K_knng", "64", "--nn_iters", "4", "--nn_sample", "20", "--num_threads", "8"])
+    args = [os.path.join(BIN, "hrnn_search"), "--index_path", idx, "--query", qb, "--gt", gt_path, "--k", "10",
+            "--K_prime", "64", "--m", "1,5", "--ef_search", "100"]
+    plain = [float(v) for v in re.findall(r"Macro Recall = ([0-9.]+)", _run(args))]
+    out = _run(args + ["--max_k_serve", "10"])
+    served = [float(v) for v in re.findall(r"Macro Recall = ([0-9.]+)", out)]
+    assert "serve mode" in out and len(plain) == 2 and served == plain, (plain, served)

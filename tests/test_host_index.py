@@ -308,3 +308,96 @@ def test_insert_dynamic_bidirectional_multithreaded_keeps_reverse_lists(tmp_path
         assert np.all(np.diff(d[i, :cnt[i]]) >= 0), i
     off, rl = ix.get_rknng_mutable()
     assert _reverse_invariant(ids, cnt, off, rl) == 0
+
+
+@pytest.mark.parametrize("m_update,bidir", [(0, False), (4, False), (0, True)])
+def test_host_insert_dynamic_steps_equal_oracle(m_update, bidir):
+    """HRNN::insertPointDynamic on the host (include/hnswlib/hrnn.h), one
+    point at a time on one thread, against the oracle's phases 2-5 (pinned to
+    the compiled reference's insertPointDynamic in test_oracle_insert.py):
+    after each call every list equals the oracle's step from the state
+    before the call plus the new point's seed list, and the mutable reverse
+    lists hold exactly the transpose of the lists (with bidirectional, a
+    superset: entries of nodes the point's list dropped stay, as the
+    reference leaves them).  Float data, so no equal distances."""
+    n_old, nx, K = 900, 15, 32
+    rng = np.random.default_rng(17 + m_update)
+    x = rng.standard_normal((n_old + nx, 64)).astype(np.float32)
+    ix = Index(n_old + nx, 64, 16, 64, 100, K)
+    ix.add_points(x[:n_old], nthreads=1)
+    exact_ids, exact_d = oracle.knn(x[:n_old], x[:n_old], K + 1)
+    ix.set_knng(exact_ids[:, 1:].astype(np.uint32), exact_d[:, 1:].astype(np.float32),
+                np.full(n_old, K, np.uint16))
+    ix.build_rknng_mutable()
+    prev = ix.get_knng()
+    upd = 0
+    for t in range(nx):
+        xn = n_old + t
+        ix.insert_dynamic(x[xn:xn + 1], np.array([xn], np.uint64), m_update, bidir)
+        after = ix.get_knng()
+        s = [np.zeros((xn + 1, K), np.uint32), np.zeros((xn + 1, K), np.float32), np.zeros(xn + 1, np.uint16)]
+        for a, b in zip(s, prev):
+            a[:xn] = b
+        if bidir:
+            # the seeds: the step's input row is not observable after a
+            # bidirectional call; rebuild it as the oracle would see it from
+            # the list the call started from (the first step only)
+            if t > 0:
+                break
+            seeds = Index(n_old + nx, 64, 16, 64, 100, K)
+            seeds.add_points(x[:n_old], nthreads=1)
+            seeds.set_knng(exact_ids[:, 1:].astype(np.uint32), exact_d[:, 1:].astype(np.float32),
+                           np.full(n_old, K, np.uint16))
+            seeds.build_rknng_mutable()
+            seeds.insert_dynamic(x[xn:xn + 1], np.array([xn], np.uint64), m_update, False)
+            row = seeds.get_knng()
+        else:
+            row = after
+        s[0][xn], s[1][xn], s[2][xn] = row[0][xn], row[1][xn], row[2][xn]
+        want = oracle.insert_affect(x[:xn + 1], *s, K, xn, xn + 1, m_update, bidir)
+        ids_a, d_a, c_a = after
+        assert np.array_equal(c_a, want[2]), t
+        live = np.arange(K)[None, :] < c_a[:, None]
+        assert np.array_equal(np.where(live, ids_a, 0), np.where(live, want[0], 0)), t
+        assert np.array_equal(np.where(live, d_a, 0), np.where(live, want[1], 0)), t
+        upd += int(want[3][3])
+        # mutable reverse lists against the transpose of the lists
+        off, rids = ix.get_rknng_mutable()
+        for v in range(xn + 1):
+            got = sorted(rids[off[v]:off[v + 1]].tolist())
+            exp = sorted(np.nonzero((ids_a == v) & live)[0].tolist())
+            if bidir:
+                assert set(exp) <= set(got), v
+            else:
+                assert got == exp, v
+        prev = after
+    assert upd > 0
+
+
+def test_host_insert_dynamic_bidirectional_threads_keep_reverse_entries():
+    """Bidirectional insertions from 8 threads at once: every list entry
+    (v in u's list) has its reverse entry (u in v's mutable reverse list)
+    afterwards, and each list is sorted, duplicate-free and without self."""
+    n_old, nx, K = 1500, 400, 24
+    rng = np.random.default_rng(29)
+    x = rng.standard_normal((n_old + nx, 32)).astype(np.float32)
+    ix = Index(n_old + nx, 32, 16, 64, 100, K)
+    ix.add_points(x[:n_old], nthreads=8)
+    exact_ids, exact_d = oracle.knn(x[:n_old], x[:n_old], K + 1)
+    lab = ix.labels()
+    inv = np.empty_like(lab)
+    inv[lab] = np.arange(n_old)
+    ix.set_knng(inv[exact_ids[lab, 1:]].astype(np.uint32), exact_d[lab, 1:].astype(np.float32),
+                np.full(n_old, K, np.uint16))
+    ix.build_rknng_mutable()
+    ix.insert_dynamic(x[n_old:], np.arange(n_old, n_old + nx, dtype=np.uint64), 0, True, nthreads=8)
+    ids, d, cnt = ix.get_knng()
+    off, rids = ix.get_rknng_mutable()
+    n = n_old + nx
+    rev = [set(rids[off[v]:off[v + 1]].tolist()) for v in range(n)]
+    for u in range(n):
+        row = ids[u, :cnt[u]]
+        assert len(set(row.tolist())) == len(row) and u not in row, u
+        assert np.all(np.diff(d[u, :cnt[u]]) >= 0), u
+        for v in row.tolist():
+            assert u in rev[v], (u, v)
