@@ -411,7 +411,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
             // probability reaches tau, its beam width for stage 2 (0: the
             // stage-1 proxies serve) and proxy count; the query joins the
             // escalated list or the list of the rest.
-            __shared__ float m_x[kIn], m_z1[kH], m_h1[kH], m_z2[kH], m_h2[kH], m_lg[kOut], m_p[kOut];
+            // scratch in the visited set's LDS (the beam is done with it; it
+            // holds >= 512 words, the network needs 320)
+            float *m_x = reinterpret_cast<float *>(s_vis), *m_z1 = m_x + kIn, *m_h1 = m_z1 + kH,
+                  *m_z2 = m_h1 + kH, *m_h2 = m_z2 + kH, *m_lg = m_h2 + kH, *m_p = m_lg + kOut;
+            static_assert(kIn + 4 * kH + 2 * kOut <= 512, "MLP scratch exceeds the smallest visited set");
+            __syncthreads();
             if (lane < (uint32_t)kIn) m_x[lane] = f;
             __syncthreads();
             mlp_forward_wave(opt.mlp, m_x, m_z1, m_h1, m_z2, m_h2, m_lg, m_p, lane);
@@ -447,7 +452,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
 // slower than ef (about 370, 560, 900, 1,020 and 1,110 at ef = 16, 32, 64, 128
 // and 200 on a 1M-point, M = 16 graph), so 1,024 slots up to ef = 16, 2,048 up
 // to ef = 256 and 4,096 beyond keep the load near or below 1/2.  An
-// overflowing probe run stays correct, only slower.
+// overflowing probe run stays correct, only slower.  (Occupancy is set by the
+// kernel's registers, five waves per SIMD, not by this LDS.)
 static uint32_t beam_log_vis(uint32_t ef) { return ef <= 16 ? 10u : ef <= 256 ? 11u : 12u; }
 
 void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint32_t ef,

@@ -44,6 +44,11 @@ CLASSES = np.array([(16, 1), (16, 2), (16, 3), (16, 4), (24, 4), (32, 4), (48, 4
                    np.uint32)
 
 
+# The m-only policy: every class at the stage-1 width, so no query runs a
+# second beam; the network picks m alone.
+CLASSES_M = np.array([(EF1, m) for m in range(1, 17)], np.uint32)
+
+
 def init_params(seed: int = 0) -> np.ndarray:
     """He-uniform weights, zero biases, in the HRNN_MLP_PARAMS layout."""
     rng = np.random.default_rng(seed)
