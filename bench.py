@@ -53,6 +53,7 @@ import ctypes
 import json
 import os
 import sys
+import shutil
 import tempfile
 import time
 
@@ -76,6 +77,11 @@ CONFIGS = {
     "c4": dict(n=1_000_000, nq=100_000, d=128, centres=1000, sigma=25.0, data_seed=3, query_seed=7,
                centre_seed=1003, zipf=1.1, ks=[1, 10, 50], k_seed=7, split=0.5, split_seed=42, M=16, efc=400,
                K=500, iters=10, S=50, Kp=500, ms=[1, 3, 5, 10, 20, 50], ef=200),
+    # learned form: L2-normalised von-Mises-Fisher-like mixture (10,000 unit
+    # centres, kappa 50), 10M x 96; 200,000 training and 10,000 held-out queries
+    "c5": dict(n=10_000_000, nq=10_000, ntrain=200_000, d=96, centres=10_000, kappa=50.0, data_seed=8,
+               train_seed=9, query_seed=10, centre_seed=88, M=16, efc=200, K=100, iters=10, S=50, k=10,
+               Kp=100, ms=[1, 3, 5, 10, 20, 50], ef=200),
     "c3": dict(n=1_000_000, nq=1000, d=960, data_seed=1, query_seed=2, scale_seed=55, M=16, efc=200,
                K=200, iters=10, S=50, k=10, Kp=200, ms=[1, 3, 5, 10, 20, 50], ef=200),
 }
@@ -135,7 +141,9 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=0,
                     help="base size for --config c3 or c4 (0: the configuration's 1,000,000; smaller for a quick run)")
     ap.add_argument("--nq", type=int, default=0,
-                    help="queries for --config c4 (0: the configuration's 100,000)")
+                    help="queries for --config c4 or c5 (0: the configuration's)")
+    ap.add_argument("--ntrain", type=int, default=0,
+                    help="training queries for --config c5 (0: the configuration's 200,000)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
@@ -150,6 +158,8 @@ def main() -> int:
         return run_c3(args, json_out)
     if args.config == "c4":
         return run_c4(args, json_out)
+    if args.config == "c5":
+        return run_c5(args, json_out)
 
     import hrnn_amd
     from hrnn_amd import hiprt
@@ -1735,6 +1745,337 @@ def run_c4(args, json_out) -> int:
         "stages_s": {"initial_hnsw_gpu": round(hnsw_s, 2), "initial_nndescent_gpu": round(nnd_s, 3),
                      "insertion_gpu": round(ins_s, 3), "rknn_csr_gpu": round(csr_s, 3),
                      "groundtruth_gpu": round(gt_s, 2)},
+    }
+    print(json.dumps(out), file=json_out, flush=True)
+    return 0
+
+
+def run_c5(args, json_out) -> int:
+    """The learned form (10M x 96, unit vectors from a 10,000-centre
+    von-Mises-Fisher-like mixture, kappa 50; K=100, k=10, K'=100): the index
+    built on the GPU (HNSW + KNN seeds, NNDescent, reverse CSR), exact
+    groundtruth from the certified brute force (FP64 judge for the held-out
+    queries), the MLP trained on the GPU on 200,000 separate queries labelled
+    by HRNN runs of every class, evaluated on 10,000 held-out queries against
+    plain fixed m = 50 (ef 200) and against the fastest fixed (ef, m) class.
+    value = the highest QPS (host I/O included) at macro recall >= 0.95."""
+    import hrnn_amd
+    from hrnn_amd import hiprt
+    from hrnn_amd.dev import Device
+    from hrnn_amd.index import Index
+    from hrnn_amd.learned import GRID, LearnedEarlyStop, label_classes
+
+    cfg = CONFIGS["c5"]
+    n, nq, d, K, k, Kp, ef = cfg["n"], cfg["nq"], cfg["d"], cfg["K"], cfg["k"], cfg["Kp"], cfg["ef"]
+    n = args.n or n
+    nq = args.nq or nq
+    ntrain = args.ntrain or cfg["ntrain"]
+    L = hrnn_amd.lib()
+    t = time.time()
+    gen = lambda seed, cnt: hrnn_amd.synth_sphere_mixture(seed, cfg["centre_seed"], cnt, d, cfg["centres"],
+                                                          cfg["kappa"])
+    x = gen(cfg["data_seed"], n)
+    q = gen(cfg["query_seed"], nq)
+    qt = gen(cfg["train_seed"], ntrain)
+    log(f"data {n}x{d} + {nq} held-out + {ntrain} training queries in {time.time() - t:.1f}s")
+
+    t = time.time()
+    ix = Index(n, d, cfg["M"], cfg["efc"], 100, K)
+    ix.add_points_gpu(x)
+    hnsw_s = time.time() - t
+    log(f"HNSW + KNN seeds on the GPU: {hnsw_s:.1f}s")
+    t = time.time()
+    nst = ix.refine(cfg["iters"], cfg["S"])
+    nnd_s = time.time() - t
+    t = time.time()
+    ix.build_rknng()
+    csr_s = time.time() - t
+    log(f"build on the GPU: HNSW + seeds {hnsw_s:.1f}s, refineKNNG {nnd_s:.2f}s, reverse CSR {csr_s:.2f}s")
+    h = ix.device_handle()
+    dev = Device.borrow(h, n, d, K)
+
+    # ---- groundtruth: certified r_k (FP32 and FP64), FP64-judged RkNN for the held-out queries
+    t = time.time()
+    bfr = dev.bf_kth_ex(k, "bf16")
+    kth, kth64 = bfr["kth"], bfr["kth64"]
+    kth_s = time.time() - t
+    g_off, g_ids, _, rechecked, flipped = dev.bf_rknn_f64(q, kth, kth64)
+    gt_off, gt_ids, _ = dev.bf_rknn(qt, kth, "f32")
+    samp = np.random.default_rng(0).choice(n, 1000, replace=False)
+    kids, _, _ = ix.get_knng()
+    knn_rec10 = float(np.mean([np.intersect1d(kids[s_, :k], bfr["ids"][s_, 1:k + 1]).size / k for s_ in samp]))
+    del kids
+    log(f"groundtruth: r_k {kth_s:.1f}s ({bfr['stats']}), held-out RkNN FP64 re-check {rechecked} pairs "
+        f"({flipped} changed), avg |GT| {len(g_ids) / nq:.1f}; KNN recall@10 (1000-node sample) {knn_rec10:.4f}")
+
+    stream = L.hrnn_dev_stream(h)
+    cap = 4096
+    m_max = int(GRID.max())
+    qd = hiprt.DeviceArray.from_host(q)
+    pd = hiprt.DeviceArray((nq * m_max,), np.uint32)
+    cd = hiprt.DeviceArray((nq,), np.uint32)
+    idd = hiprt.DeviceArray((nq * cap,), np.uint32)
+    wd = hiprt.DeviceArray((2 * nq,), np.uint32)
+    exp_d = hiprt.DeviceArray((4 * nq,), np.uint32)
+
+    def chk(rc):
+        if rc:
+            raise RuntimeError(L.hrnn_dev_last_error(h).decode())
+
+    q_pin = hiprt.PinnedArray((nq, d), np.float32)
+    q_pin.a[:] = q
+    off_pin = hiprt.PinnedArray((nq + 1,), np.uint64)
+    ids_pin = [hiprt.PinnedArray((nq * 64,), np.uint32)]
+    need = ctypes.c_uint64(0)
+
+    def fetch(count=nq):
+        while True:
+            rc = L.hrnn_dev_fetch_results(h, count, cd.ptr, idd.ptr, cap, off_pin.a.ctypes.data_as(ctypes.c_void_p),
+                                          ids_pin[0].a.ctypes.data_as(ctypes.c_void_p), ids_pin[0].a.size,
+                                          ctypes.byref(need))
+            if rc == -5 and need.value > ids_pin[0].a.size and need.value <= nq * cap:
+                ids_pin[0] = hiprt.PinnedArray((int(need.value) * 2,), np.uint32)
+                continue
+            chk(rc)
+            o = off_pin.a[:count + 1].astype(np.int64)
+            return o, ids_pin[0].a[:int(o[-1])].astype(np.uint32)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        hiprt.stream_sync(stream)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        hiprt.stream_sync(stream)
+        return (time.perf_counter() - t0) / args.steps
+
+    def timed_path(fn):
+        def step():
+            hiprt.copy_async(qd.ptr, q_pin.ptr, nq * d * 4, hiprt.H2D, stream)
+            fn()
+            fetch()
+        return timed(step), timed(fn)
+
+    def rec_of(off, ids_):
+        return float(np.nanmean(recall_per_query(g_off, g_ids, off, ids_)))
+
+    chk(L.hrnn_dev_set_vector_precision(h, 0))   # float data: FP32 vectors throughout
+    paths, sweep = {}, []
+
+    def run_fixed(m, ef_=ef):
+        chk(L.hrnn_dev_rknn_query_dptr(h, qd.ptr, nq, m, max(ef_, m), None, k, Kp, 1.0, pd.ptr, cd.ptr, idd.ptr,
+                                       cap, wd.ptr))
+
+    # ---- plain fixed m at ef 200 over the grid; the recall target ------------------------------
+    rec_m = {}
+    for m in GRID.tolist():
+        run_fixed(m)
+        rec_m[m] = rec_of(*fetch())
+        sweep.append({"m": m, "recall": round(rec_m[m], 5)})
+        log(f"  fixed m={m} (ef {ef}): macro recall {rec_m[m]:.4f}")
+    # The form's baseline is fixed m = 50; where no m reaches the headline
+    # recall on these data the target is fixed m = 50's own recall (less half
+    # a point), so every path is compared at the recall the baseline reaches.
+    target = min(args.recall_target, rec_m[50] - 0.005)
+    log(f"recall target {target:.4f} (fixed m=50 reaches {rec_m[50]:.4f})")
+    sec, sec_res = timed_path(lambda: run_fixed(50))
+    paths["fp32_fixed_m50"] = {"m": 50, "ef": ef, "recall": round(rec_m[50], 5), "sec": sec, "sec_resident": sec_res}
+    m_star = min([m for m in GRID.tolist() if rec_m[m] >= target], default=None)
+    if m_star is not None and m_star != 50:
+        sec, sec_res = timed_path(lambda: run_fixed(m_star))
+        paths["fp32_fixed_m"] = {"m": m_star, "ef": ef, "recall": round(rec_m[m_star], 5), "sec": sec,
+                                 "sec_resident": sec_res}
+
+    # ---- every fixed (ef, m) class of the two policies' tables without the model -------------
+    grid = GRID.astype(np.uint32)
+    cls_m = np.stack([np.full(16, ef, np.uint32), grid], 1)                 # m only, one beam at ef 200
+    ef_lo = 64
+    cls_2 = np.array([(ef_lo, m_) for m_ in (2, 4, 8, 12, 16, 24, 32, 50, 64)] +
+                     [(ef, m_) for m_ in (8, 16, 24, 32, 40, 50, 64)], np.uint32)   # stage 1 at ef 64
+    class_sweep = []
+    for ef_c, m_c in sorted(set(map(tuple, np.concatenate([cls_m, cls_2]).tolist())) |
+                            {(ef_lo, int(m_)) for m_ in grid}):
+        if ef_c == ef:
+            r = rec_m[m_c]
+        else:
+            run_fixed(m_c, ef_c)
+            r = rec_of(*fetch())
+        e_ = {"ef": ef_c, "m": m_c, "recall": round(r, 5)}
+        # timed: the classes at the target, and (64, 16) and (200, 1), which
+        # show what the beam width costs against m
+        if r >= target or (ef_c, m_c) in ((ef_lo, 16), (ef, 1)):
+            c_sec, c_res = timed_path(lambda: run_fixed(m_c, ef_c))
+            e_.update(sec=c_sec, sec_resident=c_res, qps=round(nq / c_sec, 1))
+        class_sweep.append(e_)
+    ok_c = [c_ for c_ in class_sweep if "qps" in c_ and c_["recall"] >= target]
+    if ok_c:
+        fc = max(ok_c, key=lambda c_: c_["qps"])
+        paths["fp32_fixed_class"] = {"ef": fc["ef"], "m": fc["m"], "recall": fc["recall"], "sec": fc["sec"],
+                                     "sec_resident": fc["sec_resident"]}
+    log(f"fixed classes: {[{k2: v2 for k2, v2 in c_.items() if k2 not in ('sec', 'sec_resident')} for c_ in class_sweep]}")
+
+    # ---- the learned m: trained on the GPU on the training queries ----------------------------
+    # Two class tables: m only (stage 1 at ef 200 gives the features and all 64
+    # proxies; the network picks m), and two-stage (stage 1 at ef 64, m up to
+    # 64; the ef-200 classes run a second beam).  Labels: per query the
+    # cheapest class reaching the recall plain fixed m = 50 reaches on that
+    # query (HRNN runs against the exact groundtruth); tau: the fastest
+    # setting on held-back training queries whose recall there reaches fixed
+    # m = 50's, less 0.004.
+    t = time.time()
+    n_fit = int(0.8 * ntrain)
+    nv = min(ntrain - n_fit, nq)
+    qv_d = hiprt.DeviceArray.from_host(np.ascontiguousarray(qt[n_fit:n_fit + nv]))
+    g0 = int(gt_off[n_fit])
+    gv_off = (gt_off[n_fit:n_fit + nv + 1] - g0).astype(np.int64)
+    gv_ids = gt_ids[g0:g0 + int(gv_off[-1])]
+    learned, runners = {}, {}
+    ref_t = label_classes(dev, qt, gt_off, gt_ids, k, Kp, classes=np.array([(ef, 50)], np.uint32))[1][0]
+    # on the validation queries: fixed m = 50's own recall there, less the
+    # same half point as the target, plus a 0.001 margin
+    rv50 = float(np.nanmean(ref_t[n_fit:n_fit + nv]))
+    target_v = min(args.recall_target + 0.01, rv50 - 0.004)
+    for pname, classes, ef1 in (("fp32_learned_m", cls_m, ef), ("fp32_learned_two_stage", cls_2, ef_lo)):
+        fa, _, _ = dev.stage1_features(qt, ef1, m_max, k)
+        lab, _, _ = label_classes(dev, qt, gt_off, gt_ids, k, Kp, classes=classes, ref_recall=ref_t)
+        model, losses = LearnedEarlyStop.train(dev, fa[:n_fit], lab[:n_fit], epochs=40, batch=512, lr=2e-3,
+                                               classes=classes, ef1=ef1)
+        model_d = hiprt.DeviceArray.from_host(model.model())
+        cls_tab = hiprt.DeviceArray.from_host(np.concatenate([classes[:, 0], classes[:, 1]]))
+        cls_d = hiprt.DeviceArray((nq,), np.uint32)
+        ef_max = int(classes[:, 0].max())
+
+        def run_ad(tau, q_ptr=qd.ptr, count=nq, model_d=model_d, cls_tab=cls_tab, cls_d=cls_d, ef_max=ef_max,
+                   ef1=ef1):
+            chk(L.hrnn_dev_rknn_query_adaptive_dptr(h, q_ptr, count, model_d.ptr, cls_tab.ptr, ef_max, tau, ef1,
+                                                    m_max, k, Kp, 1.0, pd.ptr, cls_d.ptr, cd.ptr, idd.ptr, cap,
+                                                    wd.ptr, exp_d.ptr))
+
+        runners[pname] = lambda tau_=None, run_ad=run_ad: run_ad(tau_)
+        tau_sweep = {}
+        for tv in (0.3, 0.5, 0.7, 0.8, 0.9, 0.95, 0.97, 0.98, 0.99, 0.995, 0.999):
+            run_ad(tv, q_ptr=qv_d.ptr, count=nv)
+            v_off, v_ids = fetch(nv)
+            tau_sweep[tv] = {"recall": round(float(np.nanmean(recall_per_query(gv_off, gv_ids, v_off, v_ids))), 5),
+                             "ms": round(timed(lambda: run_ad(tv, q_ptr=qv_d.ptr, count=nv)) * 1e3, 4)}
+        ok_t = [tv for tv in tau_sweep if tau_sweep[tv]["recall"] >= target_v]
+        tau_b = min(ok_t, key=lambda tv: tau_sweep[tv]["ms"]) if ok_t else max(tau_sweep)
+        run_ad(tau_b)
+        r = rec_of(*fetch())
+        cls_h = cls_d.to_host()
+        sec, sec_res = timed_path(lambda: run_ad(tau_b))
+        paths[pname] = {"tau": tau_b, "ef1": ef1, "recall": round(r, 5), "sec": sec, "sec_resident": sec_res,
+                        "mean_m": round(float(classes[cls_h, 1].mean()), 2),
+                        "escalated": round(float((classes[cls_h, 0] > ef1).mean()), 4)}
+        learned[pname] = {"classes": classes.tolist(), "label_hist": np.bincount(lab, minlength=16).tolist(),
+                          "loss_first": round(float(losses[0]), 4), "loss_last": round(float(losses[-1]), 4),
+                          "tau_validation": tau_sweep, "class_hist": np.bincount(cls_h, minlength=16).tolist()}
+        log(f"  {pname}: tau {tau_b} recall {r:.4f} mean m {paths[pname]['mean_m']} escalated "
+            f"{paths[pname]['escalated']} labels {learned[pname]['label_hist']} "
+            f"loss {learned[pname]['loss_first']} -> {learned[pname]['loss_last']}")
+    learn_s = time.time() - t
+
+    # ---- the same paths on the BF16 mirror of the base (phases A, B, C read
+    # BF16 rows, FP32 arithmetic; the groundtruth stays exact) -----------------
+    chk(L.hrnn_dev_set_vector_precision(h, 1))
+    bf_paths = [("bf16_fixed_m50", lambda: run_fixed(50), {"m": 50, "ef": ef})]
+    if "fp32_fixed_class" in paths:
+        fc_ = paths["fp32_fixed_class"]
+        bf_paths.append(("bf16_fixed_class", lambda: run_fixed(fc_["m"], fc_["ef"]), {"m": fc_["m"], "ef": fc_["ef"]}))
+    for pname in runners:
+        tau_ = paths[pname]["tau"]
+        bf_paths.append(("bf16" + pname[4:], lambda tau_=tau_, r_=runners[pname]: r_(tau_),
+                         {"tau": tau_, "ef1": paths[pname]["ef1"]}))
+    for pname, fn, info in bf_paths:
+        fn()
+        r = rec_of(*fetch())
+        sec, sec_res = timed_path(fn)
+        paths[pname] = dict(info, recall=round(r, 5), sec=sec, sec_resident=sec_res)
+    chk(L.hrnn_dev_set_vector_precision(h, 0))
+
+    for pname, p_ in paths.items():
+        p_["qps"] = round(nq / p_["sec"], 1)
+        p_["qps_resident"] = round(nq / p_["sec_resident"], 1)
+        p_["ms_per_step"] = round(p_["sec"] * 1e3, 3)
+        log(f"  path {pname}: {p_['qps']:,.0f} QPS with host I/O ({p_['qps_resident']:,.0f} resident), "
+            f"recall {p_['recall']}")
+    ok = {kk_: v for kk_, v in paths.items() if v["recall"] >= target}
+    best_name = max(ok or paths, key=lambda kk_: (ok or paths)[kk_]["qps"])
+    best = paths[best_name]
+    lq = [paths[p_]["qps"] for p_ in ("fp32_learned_m", "fp32_learned_two_stage", "bf16_learned_m",
+                                      "bf16_learned_two_stage")
+          if p_ in paths and paths[p_]["recall"] >= target]
+
+    # ---- the cpu baseline: the reference's searchRknn on this index, bounded sample ------------
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            import oracle
+            if oracle.ref_available():
+                # the saved index: base rows, HNSW links and K-entry lists
+                need_b = n * (d * 4 + 200 + K * 8) * 1.2
+                shm = "/dev/shm" if (os.path.isdir("/dev/shm")
+                                     and shutil.disk_usage("/dev/shm").free > need_b) else None
+                tmpdir = tempfile.mkdtemp(prefix="hrnn_c5_", dir=shm)
+                path = os.path.join(tmpdir, "c5.idx")
+                ix.save(path)
+                r_ = oracle.RefIndex.load(path, d)
+                os.remove(path)
+                os.rmdir(tmpdir)
+                threads = args.threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or 16
+                ns = min(nq, 2000)
+                t1 = time.perf_counter()
+                r_off, r_lab = r_.search(q[:ns], 50, 1.0, k, Kp, ef, nthreads=threads)
+                ref_s = time.perf_counter() - t1
+                r_.close()
+                # the same sample's recall, the reference's answers and this
+                # run's fixed m = 50 answers, both in label space
+                lab_of = ix.labels()
+                gs_off = g_off[:ns + 1].astype(np.int64)
+                gs_lab = lab_of[g_ids[:int(gs_off[-1])]].astype(np.uint32)
+                run_fixed(50)
+                f_off, f_ids = fetch()
+                f_lab = lab_of[f_ids[:int(f_off[ns])]].astype(np.uint32)
+                ref_rec = float(np.nanmean(recall_per_query(gs_off, gs_lab, r_off, np.asarray(r_lab, np.uint32))))
+                gpu_rec = float(np.nanmean(recall_per_query(gs_off, gs_lab, f_off[:ns + 1], f_lab)))
+                cpu = {"value": round(ns / ref_s, 1), "unit": "queries/s", "cores": threads, "kind": "reference",
+                       "recall_m50_reference": round(ref_rec, 5), "recall_m50_gpu_same_sample": round(gpu_rec, 5),
+                       "sample": f"the first {ns} held-out queries, the reference's searchRknn at m=50, K'={Kp}, "
+                                 f"ef={ef} (the fixed-m baseline of this form) on {threads} host threads, on this "
+                                 f"run's index loaded from its saved file"}
+                log(f"cpu baseline: {cpu['value']} QPS on {threads} threads (m=50); recall on the sample: "
+                    f"reference {ref_rec:.4f}, GPU {gpu_rec:.4f}")
+        except Exception as e:
+            log(f"cpu baseline skipped: {e}")
+
+    out = {
+        "metric": "RkNN QPS at recall>=target (k=10), learned form 10M x 96 (learned m against fixed m=50)",
+        "recall_target": round(target, 5),
+        "value": best["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": best["ms_per_step"], "resident_qps": best["qps_resident"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16" if best_name.startswith("bf16") else "f32", "data": "synthetic",
+        "config": {"workload": f"hrnn-c5-{n // 1000}k-x{d}", "n": n, "nq": nq, "ntrain": ntrain, "d": d, "K_knng": K,
+                   "M": cfg["M"], "ef_construction": cfg["efc"], "nn_iters": cfg["iters"], "nn_sample": cfg["S"],
+                   "k": k, "K_prime": Kp, "ef_search": ef, "path": best_name, "recall": best["recall"],
+                   "parallelism": "single-gpu"},
+        "cpu_baseline": cpu,
+        "paths": paths, "sweep": sweep, "fixed_class_sweep": [{k2: v2 for k2, v2 in c_.items()
+                                                               if k2 not in ("sec", "sec_resident")}
+                                                              for c_ in class_sweep],
+        "learned": learned, "validation_recall_fixed_m50": round(rv50, 5), "validation_target": round(target_v, 5),
+        "learned_speedup_vs_fixed_m50": round(max(lq) / paths["fp32_fixed_m50"]["qps"], 3) if lq else None,
+        "learned_speedup_vs_fastest_fixed_m": (round(max(lq) / paths["fp32_fixed_m"]["qps"], 3)
+                                               if lq and "fp32_fixed_m" in paths else None),
+        "learned_speedup_vs_fastest_fixed_class": (round(max(lq) / paths["fp32_fixed_class"]["qps"], 3)
+                                                   if lq and "fp32_fixed_class" in paths else None),
+        "groundtruth": {"judge": "FP64 re-check of pairs within FP32 rounding of r_k", "rechecked": rechecked,
+                        "changed": flipped, "bf_stats": bfr["stats"]},
+        "nndescent_build": {"seconds": round(nnd_s, 3), "knn_recall_at10_sample": round(knn_rec10, 4),
+                            "per_iter_ms": [round(s_["ms"], 1) for s_ in nst]},
+        "stages_s": {"hnsw_gpu": round(hnsw_s, 2), "nndescent_gpu": round(nnd_s, 2), "rknn_csr_gpu": round(csr_s, 2),
+                     "rk_bruteforce_gpu_bf16_certified": round(kth_s, 2), "learned_setup": round(learn_s, 2)},
     }
     print(json.dumps(out), file=json_out, flush=True)
     return 0

@@ -66,15 +66,21 @@ def init_params(seed: int = 0) -> np.ndarray:
 
 
 def per_query_recall(gt_off, gt_ids, r_off, r_ids) -> np.ndarray:
-    """|R n GT| / |GT| per query (NaN for empty groundtruth)."""
+    """|R n GT| / |GT| per query (NaN for empty groundtruth), vectorised:
+    (query, id) pairs as 64-bit keys, membership by np.isin."""
+    gt_off = np.asarray(gt_off, np.int64)
+    r_off = np.asarray(r_off, np.int64)
     nq = len(gt_off) - 1
+    gq = np.repeat(np.arange(nq, dtype=np.uint64), np.diff(gt_off))
+    rq = np.repeat(np.arange(nq, dtype=np.uint64), np.diff(r_off))
+    gk = (gq << np.uint64(32)) | np.asarray(gt_ids, np.uint64)[:len(gq)]
+    rk = (rq << np.uint64(32)) | np.asarray(r_ids, np.uint64)[:len(rq)]
+    hit = np.isin(gk, rk).astype(np.float64)
+    num = np.bincount(gq.astype(np.int64), weights=hit, minlength=nq)
+    den = np.diff(gt_off).astype(np.float64)
     out = np.full(nq, np.nan)
-    for q in range(nq):
-        g = gt_ids[gt_off[q]:gt_off[q + 1]]
-        if g.size == 0:
-            continue
-        r = r_ids[r_off[q]:r_off[q + 1]]
-        out[q] = np.intersect1d(g, r).size / g.size
+    nz = den > 0
+    out[nz] = num[nz] / den[nz]
     return out
 
 
@@ -111,8 +117,11 @@ def choose_class(probs, tau: float) -> np.ndarray:
     return np.where(cs.any(1), cs.argmax(1), probs.shape[1] - 1)
 
 
-def label_classes(dev: Device, q, gt_off, gt_ids, k, K_prime, classes=CLASSES, id_map=None):
-    """Per query the cheapest class reaching the best recall of any class.
+def label_classes(dev: Device, q, gt_off, gt_ids, k, K_prime, classes=CLASSES, id_map=None,
+                  ref_recall=None):
+    """Per query the cheapest class reaching the best recall of any class, or,
+    with ref_recall[nq] (e.g. plain fixed-m search's per-query recall), the
+    cheapest class reaching that recall (the best class where none does).
     Returns (labels[nq], recall[16, nq], expansions[16, nq]); recall is NaN
     for a query with empty groundtruth (left out of macro recall, as the
     reference's compute_rknn_recall does), whose label is class 0."""
@@ -131,6 +140,8 @@ def label_classes(dev: Device, q, gt_off, gt_ids, k, K_prime, classes=CLASSES, i
             exp[c] = ex[:, 0]
     r1 = np.nan_to_num(rec, nan=1.0)
     best = r1.max(0)
+    if ref_recall is not None:
+        best = np.minimum(best, np.nan_to_num(np.asarray(ref_recall, np.float64), nan=1.0))
     labels = (r1 >= best[None, :] - 1e-12).argmax(0).astype(np.uint32)
     return labels, rec, exp
 

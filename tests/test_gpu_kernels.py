@@ -336,16 +336,18 @@ def test_bf_kth_float_data_f32(float_data):
     assert np.array_equal(kth, ed[:, 10])
 
 
-def test_bf_kth_float_data_bf16_rerank(float_data):
-    """BF16 candidates + FP32 re-rank: distances are exact FP32 values of the chosen ids."""
+@pytest.mark.parametrize("prec", ["bf16", "f16"])
+def test_bf_kth_float_data_certified(float_data, prec):
+    """BF16/FP16 candidates with the error-bound certificate (rows it does not
+    certify are redone wider, then in FP32) and the FP32 re-rank: on float
+    data every row's ids and r_k equal the exhaustive FP32 brute force."""
     x, _ = float_data
     dev = hrnn_amd.Device(0)
     dev.set_base(x)
-    kth, ids = dev.bf_kth(10, "bf16", with_ids=True)
+    kth, ids = dev.bf_kth(10, prec, with_ids=True)
     eids, ed = oracle.knn(x, x, 11)
-    rel = np.abs(kth - ed[:, 10]) / np.maximum(ed[:, 10], 1e-12)
-    assert np.median(rel) == 0.0
-    assert float((ids == eids).mean()) > 0.97
+    assert np.array_equal(ids, eids)
+    assert np.array_equal(kth, ed[:, 10])
 
 
 @pytest.mark.parametrize("prec", ["bf16", "f32"])
