@@ -1860,7 +1860,9 @@ def run_c5(args, json_out) -> int:
     def rec_of(off, ids_):
         return float(np.nanmean(recall_per_query(g_off, g_ids, off, ids_)))
 
-    chk(L.hrnn_dev_set_vector_precision(h, 0))   # float data: FP32 vectors throughout
+    # float data: FP32 rows throughout (the BF16 and u8 mirrors serve only
+    # data they represent exactly, so they would run these same FP32 rows)
+    chk(L.hrnn_dev_set_vector_precision(h, 0))
     paths, sweep = {}, []
 
     def run_fixed(m, ef_=ef):
@@ -1930,7 +1932,7 @@ def run_c5(args, json_out) -> int:
     g0 = int(gt_off[n_fit])
     gv_off = (gt_off[n_fit:n_fit + nv + 1] - g0).astype(np.int64)
     gv_ids = gt_ids[g0:g0 + int(gv_off[-1])]
-    learned, runners = {}, {}
+    learned = {}
     ref_t = label_classes(dev, qt, gt_off, gt_ids, k, Kp, classes=np.array([(ef, 50)], np.uint32))[1][0]
     # on the validation queries: fixed m = 50's own recall there, less the
     # same half point as the target, plus a 0.001 margin
@@ -1952,7 +1954,6 @@ def run_c5(args, json_out) -> int:
                                                     m_max, k, Kp, 1.0, pd.ptr, cls_d.ptr, cd.ptr, idd.ptr, cap,
                                                     wd.ptr, exp_d.ptr))
 
-        runners[pname] = lambda tau_=None, run_ad=run_ad: run_ad(tau_)
         tau_sweep = {}
         for tv in (0.3, 0.5, 0.7, 0.8, 0.9, 0.95, 0.97, 0.98, 0.99, 0.995, 0.999):
             run_ad(tv, q_ptr=qv_d.ptr, count=nv)
@@ -1976,23 +1977,6 @@ def run_c5(args, json_out) -> int:
             f"loss {learned[pname]['loss_first']} -> {learned[pname]['loss_last']}")
     learn_s = time.time() - t
 
-    # ---- the same paths on the BF16 mirror of the base (phases A, B, C read
-    # BF16 rows, FP32 arithmetic; the groundtruth stays exact) -----------------
-    chk(L.hrnn_dev_set_vector_precision(h, 1))
-    bf_paths = [("bf16_fixed_m50", lambda: run_fixed(50), {"m": 50, "ef": ef})]
-    if "fp32_fixed_class" in paths:
-        fc_ = paths["fp32_fixed_class"]
-        bf_paths.append(("bf16_fixed_class", lambda: run_fixed(fc_["m"], fc_["ef"]), {"m": fc_["m"], "ef": fc_["ef"]}))
-    for pname in runners:
-        tau_ = paths[pname]["tau"]
-        bf_paths.append(("bf16" + pname[4:], lambda tau_=tau_, r_=runners[pname]: r_(tau_),
-                         {"tau": tau_, "ef1": paths[pname]["ef1"]}))
-    for pname, fn, info in bf_paths:
-        fn()
-        r = rec_of(*fetch())
-        sec, sec_res = timed_path(fn)
-        paths[pname] = dict(info, recall=round(r, 5), sec=sec, sec_resident=sec_res)
-    chk(L.hrnn_dev_set_vector_precision(h, 0))
 
     for pname, p_ in paths.items():
         p_["qps"] = round(nq / p_["sec"], 1)
@@ -2003,9 +1987,43 @@ def run_c5(args, json_out) -> int:
     ok = {kk_: v for kk_, v in paths.items() if v["recall"] >= target}
     best_name = max(ok or paths, key=lambda kk_: (ok or paths)[kk_]["qps"])
     best = paths[best_name]
-    lq = [paths[p_]["qps"] for p_ in ("fp32_learned_m", "fp32_learned_two_stage", "bf16_learned_m",
-                                      "bf16_learned_two_stage")
+    lq = [paths[p_]["qps"] for p_ in ("fp32_learned_m", "fp32_learned_two_stage")
           if p_ in paths and paths[p_]["recall"] >= target]
+
+    # ---- roofline: phase A and phases B+C of fixed m = 50 (FP32), HIP events on the library stream
+    chk(L.hrnn_dev_set_vector_precision(h, 0))
+    ed = hiprt.DeviceArray((2 * nq,), np.uint32)
+    e0, e1 = hiprt.Event(), hiprt.Event()
+    reps, m_b = 5, 50
+
+    def ev_ms(fn):
+        fn()
+        hiprt.stream_sync(stream)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.sync()
+        return e1.ms_since(e0) / reps
+
+    beam_ms = ev_ms(lambda: chk(L.hrnn_dev_proxies_dptr(h, qd.ptr, nq, m_b, max(ef, m_b), pd.ptr, ed.ptr)))
+    ex = ed.to_host().reshape(nq, 2).astype(np.int64)
+    beam_bytes = float((4 * d + ex[:, 0] * 4 * (2 * cfg["M"] + 1) + ex[:, 1] * 4 * d + 4 * m_b).sum())
+    search_ms = ev_ms(lambda: chk(L.hrnn_dev_rknn_search_dptr(h, qd.ptr, nq, pd.ptr, m_b, None, k, Kp, 1.0, cd.ptr,
+                                                             idd.ptr, cap, wd.ptr)))
+    wk = wd.to_host().reshape(nq, 2).astype(np.int64)
+    rc_ = cd.to_host().astype(np.int64)
+    search_bytes = float((4 * d + 16 * m_b + 4 * wk[:, 0] + wk[:, 1] * (4 * d + 4) + 4 * rc_).sum())
+    kern = {"k_phase_a": {"ms": beam_ms, "bytes": beam_bytes,
+                          "expansions_per_query": round(float(ex[:, 0].mean()), 1),
+                          "evaluations_per_query": round(float(ex[:, 1].mean()), 1)},
+            "k_rknn_search": {"ms": search_ms, "bytes": search_bytes,
+                              "scanned_per_query": round(float(wk[:, 0].mean()), 1),
+                              "candidates_per_query": round(float(wk[:, 1].mean()), 1)}}
+    for v in kern.values():
+        v["gbs"] = v["bytes"] / (v["ms"] * 1e-3) / 1e9
+    dom = max(kern, key=lambda kk_: kern[kk_]["ms"])
+    log(f"  kernels of fixed m=50: {kern}")
 
     # ---- the cpu baseline: the reference's searchRknn on this index, bounded sample ------------
     cpu = None
@@ -2055,11 +2073,15 @@ def run_c5(args, json_out) -> int:
         "value": best["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": best["ms_per_step"], "resident_qps": best["qps_resident"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16" if best_name.startswith("bf16") else "f32", "data": "synthetic",
+        "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"hrnn-c5-{n // 1000}k-x{d}", "n": n, "nq": nq, "ntrain": ntrain, "d": d, "K_knng": K,
                    "M": cfg["M"], "ef_construction": cfg["efc"], "nn_iters": cfg["iters"], "nn_sample": cfg["S"],
                    "k": k, "K_prime": Kp, "ef_search": ef, "path": best_name, "recall": best["recall"],
                    "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": round(kern[dom]["gbs"], 2), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(kern[dom]["gbs"] / 8000.0, 5), "traffic": None, "kernel": dom,
+                     "path": "fp32_fixed_m50", "kernel_ms": round(kern[dom]["ms"], 4),
+                     "alg_bytes_per_launch": kern[dom]["bytes"], "kernels": kern},
         "cpu_baseline": cpu,
         "paths": paths, "sweep": sweep, "fixed_class_sweep": [{k2: v2 for k2, v2 in c_.items()
                                                                if k2 not in ("sec", "sec_resident")}

@@ -361,6 +361,7 @@ static int *tuning_slot(hrnn_dev_t h, const char *key) {
     if (k == "hnsw_seed_exp") return &h->tune_hnsw_seed_exp;
     if (k == "insert_offer_cap") return &h->tune_insert_offer_cap;
     if (k == "insert_lds_bitmap") return &h->tune_insert_lds_bitmap;
+    if (k == "beam_log_vis") return &h->tune_beam_log_vis;
     return nullptr;
 }
 
@@ -372,7 +373,9 @@ int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value) {
     const bool seedx = slot == &h->tune_hnsw_seed_exp;
     const bool offers = slot == &h->tune_insert_offer_cap;
     const bool flag = slot == &h->tune_insert_lds_bitmap;
-    if (!slot || value < (cap || seedx || offers || flag ? 0 : 1) ||
+    const bool lvis = slot == &h->tune_beam_log_vis;
+    if (!slot || value < (cap || seedx || offers || flag || lvis ? 0 : 1) ||
+        (lvis && value != 0 && (value < 10 || value > 13)) ||
         value > (offers ? (1 << 30) : flag ? 1 : cap ? 1024 : permille || seedx ? 1000 : 64)) {
         h->err = std::string("set_tuning: unknown key or value out of range: ") + (key ? key : "");
         return HRNN_E_ARG;

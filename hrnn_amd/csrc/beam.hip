@@ -254,6 +254,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
                 vr[r] = t < nf;
                 idr[r] = s_ni[vr[r] ? t : b];
             }
+            if constexpr (std::is_same<VT, float>::value) {
+                if (!exact && (d & 15u) == 0) {   // float data: quad r of each group takes row r
+                    const float s4 = dist4_quad_f32(s_qv, x, idr, vr, d, g);
+                    const uint32_t t = b + grp + 4u * (g >> 2);
+                    if ((g & 3u) == 0 && t < nf) s_nd[t] = s4;
+                    continue;
+                }
+            }
             float orr[kRows];
             beam_dist4<VT, kRows>(exact, s_qv, s_q8, qq, x, xrs, idr, vr, d, g, orr);
             if (g == 0) {
@@ -488,7 +496,7 @@ void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint
     const int base_exact = (h->base_u8 && h->d <= 258 && h->d % 8 == 0 &&
                             (uint64_t)h->n * h->d * vbytes < 0xFFFFFFF0ull) ? 1 : 0;
     const uint32_t ef_cap = (ef + 3u) & ~3u;
-    const uint32_t log_vis = beam_log_vis(ef);
+    const uint32_t log_vis = h->tune_beam_log_vis ? (uint32_t)h->tune_beam_log_vis : beam_log_vis(ef);
     const BeamLds lay(h->d, ef_cap, log_vis);
     REQUIRE(lay.total <= 65536, HRNN_E_ARG, "proxies: LDS need exceeds 64 KiB");
 #define LAUNCH1(VT, PQ, XP)                                                                       \

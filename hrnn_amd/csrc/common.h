@@ -163,6 +163,61 @@ __device__ __forceinline__ float dist_row<uint8_t>(const float *__restrict__ qv,
     return r;
 }
 
+// Quad lane k's value in every lane of its quad (DPP quad_perm [k,k,k,k]).
+__device__ __forceinline__ float quad_bcast(float v, int k) {
+    switch (k) {
+        case 1: return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x55, 0xF, 0xF, false));
+        case 2: return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xAA, 0xF, 0xF, false));
+        default: return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xFF, 0xF, 0xF, false));
+    }
+}
+
+// Four rows of an FP32 base against an FP32 query (16-byte aligned) by one
+// 16-lane group, d % 16 == 0: quad r (lanes 4r..4r+3 of the group) takes row
+// id[r] (valid iff val[r]).  Lane 4r+c keeps the reference's partials
+// 4c..4c+3 -- partial j sums (q[e] - x[e])^2 over e = j, j+16, j+32, ... in
+// increasing e with FMAs, the order of l2_ref16_group -- from 16-byte loads
+// (eight of them in flight per lane), then the quad's first lane adds the 16
+// partials left to right, its neighbours' through quad DPP moves.  Returns
+// row r's distance in lane 4r of the group (other lanes: unspecified).
+__device__ __forceinline__ float dist4_quad_f32(const float *__restrict__ qv, const float *__restrict__ x,
+                                                const uint32_t *id, const bool *val, uint32_t d, uint32_t g) {
+    const uint32_t r = g >> 2, c = g & 3u;
+    const uint32_t my_id = r == 0 ? id[0] : r == 1 ? id[1] : r == 2 ? id[2] : id[3];
+    const bool my_v = r == 0 ? val[0] : r == 1 ? val[1] : r == 2 ? val[2] : val[3];
+    float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f;
+    if (my_v) {
+        const float4 *xr = reinterpret_cast<const float4 *>(x + (uint64_t)my_id * d) + c;
+        const float4 *qr = reinterpret_cast<const float4 *>(qv) + c;
+        const uint32_t nb = d >> 4;
+#pragma unroll 8
+        for (uint32_t i = 0; i < nb; i++) {
+            const float4 xv = xr[4 * i];
+            const float4 qq = qr[4 * i];
+            float t = qq.x - xv.x;
+            p0 = __builtin_fmaf(t, t, p0);
+            t = qq.y - xv.y;
+            p1 = __builtin_fmaf(t, t, p1);
+            t = qq.z - xv.z;
+            p2 = __builtin_fmaf(t, t, p2);
+            t = qq.w - xv.w;
+            p3 = __builtin_fmaf(t, t, p3);
+        }
+    }
+    float s = p0;
+    s += p1;
+    s += p2;
+    s += p3;
+#pragma unroll
+    for (int k = 1; k < 4; k++) {
+        s += quad_bcast(p0, k);
+        s += quad_bcast(p1, k);
+        s += quad_bcast(p2, k);
+        s += quad_bcast(p3, k);
+    }
+    return s;
+}
+
 // Sum over each row of 16 lanes, every lane of the row receiving it, by DPP
 // moves (no LDS round trip): swap within quads, then with the other quad of
 // the 8, then with the other 8.  The order of the additions is fixed but not

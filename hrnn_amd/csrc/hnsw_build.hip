@@ -238,6 +238,14 @@ __device__ void beam_expand(const BuildGraph &G, const VT *__restrict__ x, BeamC
                 vr[r] = t < nf;
                 idr[r] = cx.ni[vr[r] ? t : b];
             }
+            if constexpr (std::is_same<VT, float>::value) {
+                if (!cx.exact && (G.d & 15u) == 0) {   // float data: quad r of each group takes row r
+                    const float s4 = dist4_quad_f32(cx.qv, x, idr, vr, G.d, g);
+                    const uint32_t t = b + grp + 4u * (g >> 2);
+                    if ((g & 3u) == 0 && t < nf) cx.nd[t] = s4;
+                    continue;
+                }
+            }
             float orr[kRows];
             beam_dist4<VT, kRows>(cx.exact, cx.qv, cx.q8, cx.qq, x, cx.xrs, idr, vr, G.d, g, orr);
             if (g == 0) {

@@ -177,6 +177,11 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
                     dist4_u8_buf(s_q8, s_qq, xrs, id4, d, g, o);
                 else
                     dist4_exact_buf<VT>(s_qv, xrs, id4, d * (uint32_t)sizeof(VT), d, g, o);
+            } else if (std::is_same<VT, float>::value && (d & 15u) == 0) {
+                // quad r of the group takes row r; its first lane holds the distance
+                const float s4 = dist4_quad_f32(s_qv, reinterpret_cast<const float *>(x), id4, v4, d, g);
+#pragma unroll
+                for (int r = 0; r < 4; r++) o[r] = __shfl(s4, (int)((threadIdx.x & 63u & ~15u) + 4u * r), 64);
             } else {
 #pragma unroll
                 for (int r = 0; r < 4; r++) o[r] = v4[r] ? dist_row<VT>(s_qv, x, id4[r], d, g) : FLT_MAX;

@@ -104,7 +104,9 @@ int hrnn_dev_set_vector_precision(hrnn_dev_t h, int prec);
  * offer buffer (entries per batch; it grows when a batch needs more);
  * "insert_lds_bitmap" (0 or 1, default 1): the insertion's visited bitmap in
  * LDS when n bits fit there (160 KiB per workgroup on gfx950), else in HBM
- * (the same results). */
+ * (the same results).  "beam_log_vis" (0 or 10 .. 13, default 0 = chosen
+ * from ef): log2 of phase A's LDS visited-set slots per query (more slots
+ * cost occupancy, fewer overflow on long beams; the same results). */
 int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value);
 int hrnn_dev_get_tuning(hrnn_dev_t h, const char *key, int *value);
 
