@@ -208,6 +208,14 @@ def main() -> int:
             log(f"seed graph not saved: {e}")
             seed_path = None
 
+    # ---- the NNDescent join's staging from the autotuner (one iteration per
+    # candidate from the seed graph, which it puts back) ---------------------
+    from hrnn_amd import autotune
+    t = time.time()
+    tuned_nnd = autotune.tune_nndescent(Device.borrow(ix.device_handle(), n, d, K), cfg["S"],
+                                        dict(n=n, d=d, K=K), use_cache=False)
+    log(f"autotune: nnd_join={tuned_nnd['nnd_join']} {tuned_nnd['timings_ms']} ({time.time() - t:.1f}s)")
+
     # ---- (a) NNDescent on the GPU (the build metric) ------------------------
     t = time.time()
     st = ix.refine(cfg["iters"], cfg["S"])
@@ -232,6 +240,8 @@ def main() -> int:
     csr_info = dev.build_rknn_csr()
     dev.sync()
     csr_dev_s = time.perf_counter() - t
+    tuned_bf = autotune.tune_bruteforce(dev, x[:16384], k, "bf16", dict(n=n, d=d), use_cache=False)
+    log(f"autotune: bf_row_frags={tuned_bf['bf_row_frags']} {tuned_bf['timings_ms']}")
     t = time.time()
     kth = dev.bf_kth(k, "bf16")
     kth_s = time.time() - t
@@ -387,12 +397,18 @@ def main() -> int:
     if m_star is None:
         m_star = cfg["ms"][-1]
 
-    # ---- launch shape of the persistent search kernel from the autotuner ------------------
-    from hrnn_amd import autotune
+    # ---- launch shapes of the persistent search kernel and of phase A from the autotuner -----
     pd.copy_from_host_async(np.ascontiguousarray(gprox[:, :m_star]), stream, 0)
     tuned = autotune.tune(h, qd, nq, m_star, k, Kp, dict(n=n, d=d, K=K, m=m_star, nq=nq), pd, cd, idd, cap,
                           use_cache=False)
     log(f"autotune: search_wg_per_cu={tuned['search_wg_per_cu']}  {tuned['timings_ms']}")
+    # phase A: at ef 200 (the fixed-m paths) for the record, then at the
+    # stage-1 width of the early-stop paths, whose choice stays applied
+    ck(L.hrnn_dev_set_vector_precision(h, 3))
+    tuned_beam = {f"ef{e_}": autotune.tune_beam(h, qd, nq, m_, e_, pd, dict(n=n, d=d), use_cache=False)
+                  for e_, m_ in ((ef, m_star), (EF1, 16))}
+    ck(L.hrnn_dev_set_vector_precision(h, 0))
+    log(f"autotune: phase A {tuned_beam}")
 
     paths = {}
     ck(L.hrnn_dev_set_vector_precision(h, 0))          # FP32 vectors in phases A, B, C
@@ -1045,7 +1061,7 @@ def main() -> int:
                                                  for p_ in ("u8_learned_adaptive", "u8_learned_m_only"))
             else None),
         "learned_m": learned_info,
-        "autotune": {kk: v for kk, v in tuned.items() if kk != "timings_ms"},
+        "autotune": {"search": tuned, "phase_a_u8": tuned_beam, "bruteforce": tuned_bf, "nndescent": tuned_nnd},
         "nndescent_cpu_baseline": nnd_cpu,
         "csr_cpu_baseline": csr_cpu,
         "bruteforce_cpu_baseline": bf_cpu,
@@ -1784,6 +1800,10 @@ def run_c5(args, json_out) -> int:
     ix.add_points_gpu(x)
     hnsw_s = time.time() - t
     log(f"HNSW + KNN seeds on the GPU: {hnsw_s:.1f}s")
+    from hrnn_amd import autotune
+    tuned = {"nndescent": autotune.tune_nndescent(Device.borrow(ix.device_handle(), n, d, K), cfg["S"],
+                                                  dict(n=n, d=d, K=K), use_cache=False)}
+    log(f"autotune: {tuned}")
     t = time.time()
     nst = ix.refine(cfg["iters"], cfg["S"])
     nnd_s = time.time() - t
@@ -1795,6 +1815,8 @@ def run_c5(args, json_out) -> int:
     dev = Device.borrow(h, n, d, K)
 
     # ---- groundtruth: certified r_k (FP32 and FP64), FP64-judged RkNN for the held-out queries
+    tuned["bruteforce"] = autotune.tune_bruteforce(dev, x[:16384], k, "bf16", dict(n=n, d=d), use_cache=False)
+    log(f"autotune: bruteforce {tuned['bruteforce']}")
     t = time.time()
     bfr = dev.bf_kth_ex(k, "bf16")
     kth, kth64 = bfr["kth"], bfr["kth64"]
@@ -1868,6 +1890,9 @@ def run_c5(args, json_out) -> int:
     def run_fixed(m, ef_=ef):
         chk(L.hrnn_dev_rknn_query_dptr(h, qd.ptr, nq, m, max(ef_, m), None, k, Kp, 1.0, pd.ptr, cd.ptr, idd.ptr,
                                        cap, wd.ptr))
+
+    tuned["phase_a"] = autotune.tune_beam(h, qd, nq, 50, ef, pd, dict(n=n, d=d), use_cache=False)
+    log(f"autotune: phase A {tuned['phase_a']}")
 
     # ---- plain fixed m at ef 200 over the grid; the recall target ------------------------------
     rec_m = {}
@@ -2082,7 +2107,7 @@ def run_c5(args, json_out) -> int:
                      "frac": round(kern[dom]["gbs"] / 8000.0, 5), "traffic": None, "kernel": dom,
                      "path": "fp32_fixed_m50", "kernel_ms": round(kern[dom]["ms"], 4),
                      "alg_bytes_per_launch": kern[dom]["bytes"], "kernels": kern},
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu, "autotune": tuned,
         "paths": paths, "sweep": sweep, "fixed_class_sweep": [{k2: v2 for k2, v2 in c_.items()
                                                                if k2 not in ("sec", "sec_resident")}
                                                               for c_ in class_sweep],

@@ -1,13 +1,22 @@
-"""Autotuner for the launch shapes of the persistent search kernels.
+"""Autotuner for the launch and tile shapes of the path's kernels.
 
-For a resident index, query batch and proxy set it times each candidate
-launch shape of the persistent phase-B/C search kernel ("search_wg_per_cu":
-workgroups per CU; "search_lds_kib": KiB of LDS for the per-query
-deduplication table, which trades occupancy against how many queries fall
-back to the HBM visited bitmap) with HIP events on the library's stream, keeps the
-fastest, and caches the choice as JSON keyed by the GPU and the workload
-shape (n, d, K, m, nq).  Results are identical for every shape; only the
-time changes.
+Each tuner times every candidate of its tuning keys on the workload at hand
+(HIP events on the library's stream, or the wall clock around a synchronous
+call), keeps the fastest, and caches the choice as JSON keyed by the GPU and
+the workload shape.  Every candidate gives identical results; only the time
+changes (the GPU tests check this for each key).
+
+* tune (phases B+C, the persistent search kernel): "search_wg_per_cu"
+  (workgroups per CU) and "search_lds_kib" (KiB of LDS for the per-query
+  deduplication table, which trades occupancy against how many queries fall
+  back to the HBM visited bitmap);
+* tune_beam (phase A): "beam_vis_shift" (the LDS visited set's size against
+  the ef rule: occupancy against probe runs);
+* tune_bruteforce (the MFMA brute force): "bf_row_frags" (16-row fragments
+  per wave: base-tile reuse against blocks per CU);
+* tune_nndescent (the NNDescent join): "nnd_join" (chunked BF16, whole-sample
+  BF16 or whole-sample int8 staging of each node's sample in LDS), timed on
+  the first iteration from the seed graph, which is restored afterwards.
 """
 from __future__ import annotations
 
@@ -23,6 +32,9 @@ from .dev import lib
 CANDIDATES = {
     "search_lds_kib": [8, 16, 32, 48],
     "search_wg_per_cu": [1, 2, 4, 8, 16, 32],
+    "beam_vis_shift": [-1, 0, 1, 2],
+    "bf_row_frags": [1, 2],
+    "nnd_join": [1, 2, 3],
 }
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "cache.json")
 
@@ -100,6 +112,145 @@ def tune(h, qd, nq: int, m: int, k: int, K_prime: int, shape: dict, pd, cd, idd,
         ck(L.hrnn_dev_set_tuning(h, name.encode(), best))
         out[name] = best
     out["timings_ms"] = timings
+    if use_cache:
+        cache[key] = out
+        save_cache(cache)
+    return out
+
+
+def _cached(key: str, use_cache: bool, L, h):
+    """The cached choice for key, applied to handle h (None when absent)."""
+    if not use_cache:
+        return None, {}
+    cache = load_cache()
+    if key in cache:
+        for k_, v in cache[key].items():
+            if k_ in CANDIDATES:
+                ck_(L, h, L.hrnn_dev_set_tuning(h, k_.encode(), int(v)))
+        return dict(cache[key], cached=True), cache
+    return None, cache
+
+
+def ck_(L, h, rc):
+    if rc:
+        raise RuntimeError(L.hrnn_dev_last_error(h).decode())
+
+
+def _event_timer(L, h, reps):
+    stream = L.hrnn_dev_stream(h)
+    e0, e1 = hiprt.Event(), hiprt.Event()
+
+    def time_it(fn):
+        fn()
+        hiprt.stream_sync(stream)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.sync()
+        return e1.ms_since(e0) / reps
+    return time_it
+
+
+def _pick(L, h, name, cands, time_it, fn, timings):
+    best, best_ms = None, float("inf")
+    for v in cands:
+        ck_(L, h, L.hrnn_dev_set_tuning(h, name.encode(), int(v)))
+        ms = time_it(fn)
+        timings[f"{name}={v}"] = round(ms, 4)
+        if ms < best_ms:
+            best, best_ms = v, ms
+    ck_(L, h, L.hrnn_dev_set_tuning(h, name.encode(), int(best)))
+    return best
+
+
+def tune_beam(h, qd, nq: int, m: int, ef: int, pd, shape: dict, reps: int = 5,
+              use_cache: bool = True) -> dict:
+    """Phase A (hrnn_dev_proxies_dptr at this m and ef over the resident
+    queries qd into pd): "beam_vis_shift"."""
+    L = lib()
+    key = _key(dict(shape, kernel="phase_a", m=m, ef=ef, nq=nq))
+    hit, cache = _cached(key, use_cache, L, h)
+    if hit:
+        return hit
+    time_it = _event_timer(L, h, reps)
+    fn = lambda: ck_(L, h, L.hrnn_dev_proxies_dptr(h, qd.ptr, nq, m, ef, pd.ptr, None))
+    timings = {}
+    out = {"beam_vis_shift": _pick(L, h, "beam_vis_shift", CANDIDATES["beam_vis_shift"], time_it, fn, timings),
+           "timings_ms": timings}
+    if use_cache:
+        cache[key] = out
+        save_cache(cache)
+    return out
+
+
+def tune_bruteforce(dev, q, k: int, prec: str = "bf16", shape: dict | None = None,
+                    reps: int = 2, use_cache: bool = True) -> dict:
+    """The MFMA brute force (k+1-NN of the rows q, e.g. a sample of the base,
+    over the whole base: the kernel and candidate width of the r_k pass):
+    "bf_row_frags".
+    Only 32-wide candidate lists (k + 1 <= 16) at d <= 256 have a second
+    shape; elsewhere the default stays and nothing is timed."""
+    import time
+    L = lib()
+    h = dev.h
+    if not (k + 1 <= 16 and dev.d <= 256 and dev.d % 32 == 0):
+        return {"bf_row_frags": 1, "timings_ms": {}, "applies": False}
+    key = _key(dict(shape or {}, kernel="bruteforce", n=dev.n, d=dev.d, k=k, prec=prec, rows=len(q)))
+    hit, cache = _cached(key, use_cache, L, h)
+    if hit:
+        return hit
+    q = np.ascontiguousarray(q, np.float32)
+
+    def time_it(fn):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    timings = {}
+    out = {"bf_row_frags": _pick(L, h, "bf_row_frags", CANDIDATES["bf_row_frags"], time_it,
+                                 lambda: dev.bf_knn(q, k + 1, prec), timings),
+           "timings_ms": timings, "applies": True}
+    if use_cache:
+        cache[key] = out
+        save_cache(cache)
+    return out
+
+
+def tune_nndescent(dev, sample: int, shape: dict | None = None, use_cache: bool = True) -> dict:
+    """The NNDescent join's staging ("nnd_join"), timed on one iteration from
+    the device's current graph (the seed graph), which is put back, flags
+    included, before returning.  int8 staging is a candidate for 8-bit data
+    only."""
+    import time
+    L = lib()
+    h = dev.h
+    key = _key(dict(shape or {}, kernel="nndescent", n=dev.n, d=dev.d, K=dev.K, S=sample))
+    hit, cache = _cached(key, use_cache, L, h)
+    if hit:
+        return hit
+    seed = dev.get_knng()
+    flags = dev.get_knng_flags()
+    cands = [1, 2, 3] if dev.base_is_u8() else [1, 2]
+    timings = {}
+
+    def one():
+        dev.set_knng(*seed)
+        ck_(L, h, L.hrnn_dev_set_knng_flags(h, None))
+        dev.sync()
+        t0 = time.perf_counter()
+        dev.nndescent(1, sample)
+        dev.sync()
+        return (time.perf_counter() - t0) * 1e3
+
+    try:
+        best = _pick(L, h, "nnd_join", cands, lambda fn: fn(), one, timings)
+    finally:
+        dev.set_knng(*seed)
+        dev.set_knng_flags(flags)
+    out = {"nnd_join": best, "timings_ms": timings}
     if use_cache:
         cache[key] = out
         save_cache(cache)

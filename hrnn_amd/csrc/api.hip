@@ -349,6 +349,12 @@ int hrnn_dev_base_is_small_int(hrnn_dev_t h, int *is_int) {
     return HRNN_OK;
 }
 
+int hrnn_dev_base_is_u8(hrnn_dev_t h, int *is_u8) {
+    if (!h || !is_u8) return HRNN_E_ARG;
+    *is_u8 = h->base_u8 ? 1 : 0;
+    return HRNN_OK;
+}
+
 const float *hrnn_dev_base_ptr(hrnn_dev_t h) { return h ? h->x.as<float>() : nullptr; }
 
 static int *tuning_slot(hrnn_dev_t h, const char *key) {
@@ -361,7 +367,9 @@ static int *tuning_slot(hrnn_dev_t h, const char *key) {
     if (k == "hnsw_seed_exp") return &h->tune_hnsw_seed_exp;
     if (k == "insert_offer_cap") return &h->tune_insert_offer_cap;
     if (k == "insert_lds_bitmap") return &h->tune_insert_lds_bitmap;
-    if (k == "beam_log_vis") return &h->tune_beam_log_vis;
+    if (k == "beam_vis_shift") return &h->tune_beam_vis_shift;
+    if (k == "bf_row_frags") return &h->tune_bf_frags;
+    if (k == "nnd_join") return &h->tune_nnd_join;
     return nullptr;
 }
 
@@ -373,10 +381,12 @@ int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value) {
     const bool seedx = slot == &h->tune_hnsw_seed_exp;
     const bool offers = slot == &h->tune_insert_offer_cap;
     const bool flag = slot == &h->tune_insert_lds_bitmap;
-    const bool lvis = slot == &h->tune_beam_log_vis;
-    if (!slot || value < (cap || seedx || offers || flag || lvis ? 0 : 1) ||
-        (lvis && value != 0 && (value < 10 || value > 13)) ||
-        value > (offers ? (1 << 30) : flag ? 1 : cap ? 1024 : permille || seedx ? 1000 : 64)) {
+    const bool vshift = slot == &h->tune_beam_vis_shift;
+    const bool frags = slot == &h->tune_bf_frags;
+    const bool join = slot == &h->tune_nnd_join;
+    if (!slot || value < (vshift ? -1 : cap || seedx || offers || flag || join ? 0 : 1) ||
+        value > (offers ? (1 << 30) : flag ? 1 : vshift || frags ? 2 : join ? 3 : cap ? 1024
+                                                                   : permille || seedx ? 1000 : 64)) {
         h->err = std::string("set_tuning: unknown key or value out of range: ") + (key ? key : "");
         return HRNN_E_ARG;
     }

@@ -460,8 +460,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
 // slower than ef (about 370, 560, 900, 1,020 and 1,110 at ef = 16, 32, 64, 128
 // and 200 on a 1M-point, M = 16 graph), so 1,024 slots up to ef = 16, 2,048 up
 // to ef = 256 and 4,096 beyond keep the load near or below 1/2.  An
-// overflowing probe run stays correct, only slower.  (Occupancy is set by the
-// kernel's registers, five waves per SIMD, not by this LDS.)
+// overflowing probe run stays correct, only slower.  Harder data evaluate
+// more (about 4,300 at ef 200 on 96-dim unit vectors), where a larger table
+// trades occupancy (LDS per wave) against probe runs: the "beam_vis_shift"
+// tuning key moves the rule's choice by a power of two, within 2^10 .. 2^13.
 static uint32_t beam_log_vis(uint32_t ef) { return ef <= 16 ? 10u : ef <= 256 ? 11u : 12u; }
 
 void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint32_t ef,
@@ -496,7 +498,8 @@ void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint
     const int base_exact = (h->base_u8 && h->d <= 258 && h->d % 8 == 0 &&
                             (uint64_t)h->n * h->d * vbytes < 0xFFFFFFF0ull) ? 1 : 0;
     const uint32_t ef_cap = (ef + 3u) & ~3u;
-    const uint32_t log_vis = h->tune_beam_log_vis ? (uint32_t)h->tune_beam_log_vis : beam_log_vis(ef);
+    const uint32_t log_vis =
+        (uint32_t)std::min(13, std::max(10, (int)beam_log_vis(ef) + h->tune_beam_vis_shift));
     const BeamLds lay(h->d, ef_cap, log_vis);
     REQUIRE(lay.total <= 65536, HRNN_E_ARG, "proxies: LDS need exceeds 64 KiB");
 #define LAUNCH1(VT, PQ, XP)                                                                       \

@@ -782,18 +782,26 @@ static void launch_mfma(hrnn_dev_t h, const T *qa, const float *qn, uint32_t nq,
                         const float *xn, uint32_t *ci, uint32_t *cn, float *cthr, float *raw_kth,
                         uint32_t kraw, const float *kth, uint32_t *hc, uint32_t *hi, float *hd,
                         uint32_t cap) {
-    // One 16-row fragment per wave (64 rows per block): the top state's LDS
-    // then allows three blocks per CU, which measured faster on the 1M x 1M
-    // r_k pass (1.05 s) than two fragments (128 rows per block, half the
-    // base-tile traffic per MFMA, two blocks per CU: 1.11 s).  The kernel
-    // keeps FR as a template parameter.
-    constexpr int FR = 1;
-    unsigned grid = ceil_div(nq, 64 * FR);
+    // 16-row fragments per wave, "bf_row_frags": one by default (64 rows per
+    // block: the top state's LDS allows three blocks per CU, which measured
+    // faster on the 1M x 1M r_k pass, 1.05 s, than two fragments, 128 rows
+    // per block with half the base-tile traffic per MFMA and two blocks per
+    // CU: 1.11 s); two where the autotuner finds them faster, for 32-wide
+    // candidate lists and d <= 256 (wider rows or lists would not fit).
+    const bool fr2 = h->tune_bf_frags == 2;
     uint32_t KS = h->d / 32;
 #define L(ks)                                                                                     \
     case ks:                                                                                      \
-        hipLaunchKernelGGL((k_bf_mfma<T, ks, C, MODE, FR>), dim3(grid), dim3(256), 0, h->stream, qa, qn, \
-                           nq, xb, xn, h->n, h->d, ci, cn, cthr, raw_kth, kraw, kth, hc, hi, hd, cap); \
+        if constexpr (C == 32 && ks <= 8) {                                                       \
+            if (fr2) {                                                                            \
+                hipLaunchKernelGGL((k_bf_mfma<T, ks, C, MODE, 2>), dim3(ceil_div(nq, 128)), dim3(256), 0, \
+                                   h->stream, qa, qn, nq, xb, xn, h->n, h->d, ci, cn, cthr, raw_kth, kraw, \
+                                   kth, hc, hi, hd, cap);                                         \
+                break;                                                                            \
+            }                                                                                     \
+        }                                                                                         \
+        hipLaunchKernelGGL((k_bf_mfma<T, ks, C, MODE, 1>), dim3(ceil_div(nq, 64)), dim3(256), 0, h->stream, \
+                           qa, qn, nq, xb, xn, h->n, h->d, ci, cn, cthr, raw_kth, kraw, kth, hc, hi, hd, cap); \
         break;
     switch (KS) {
         L(1) L(2) L(3) L(4) L(5) L(6) L(8) L(12) L(16) L(24) L(30) L(32)

@@ -218,6 +218,54 @@ __device__ __forceinline__ float dist4_quad_f32(const float *__restrict__ qv, co
     return s;
 }
 
+// Squared L2 between two FP32 rows a and b of a base (d % 16 == 0) by the four
+// lanes of a quad (c = lane & 3): the reference's 16-partial order, as
+// dist4_quad_f32 (t = a - b).  ok uniform within the quad; the quad's first
+// lane returns the distance.
+__device__ __forceinline__ float pair_quad_f32(const float *__restrict__ x, uint32_t ia, uint32_t ib, uint32_t d,
+                                               uint32_t c, bool ok) {
+    float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f;
+    if (ok) {
+        const float4 *ar = reinterpret_cast<const float4 *>(x + (uint64_t)ia * d) + c;
+        const float4 *br = reinterpret_cast<const float4 *>(x + (uint64_t)ib * d) + c;
+        const uint32_t nb = d >> 4;
+#pragma unroll 6
+        for (uint32_t i = 0; i < nb; i++) {
+            const float4 av = ar[4 * i];
+            const float4 bv = br[4 * i];
+            float t = av.x - bv.x;
+            p0 = __builtin_fmaf(t, t, p0);
+            t = av.y - bv.y;
+            p1 = __builtin_fmaf(t, t, p1);
+            t = av.z - bv.z;
+            p2 = __builtin_fmaf(t, t, p2);
+            t = av.w - bv.w;
+            p3 = __builtin_fmaf(t, t, p3);
+        }
+    }
+    float s = p0;
+    s += p1;
+    s += p2;
+    s += p3;
+#pragma unroll
+    for (int k = 1; k < 4; k++) {
+        s += quad_bcast(p0, k);
+        s += quad_bcast(p1, k);
+        s += quad_bcast(p2, k);
+        s += quad_bcast(p3, k);
+    }
+    return s;
+}
+
+// LDS written by some lanes of a wave, then read by others: keep the
+// compiler from moving memory operations across this point (a wave's LDS
+// operations execute in order).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Sum over each row of 16 lanes, every lane of the row receiving it, by DPP
 // moves (no LDS round trip): swap within quads, then with the other quad of
 // the 8, then with the other 8.  The order of the additions is fixed but not

@@ -46,7 +46,10 @@ struct hrnn_dev {
     int tune_search_wg_per_cu = 8;   // persistent RkNN search workgroups per CU
     int tune_search_lds_kib = 16;    // LDS dedup table of the RkNN search, KiB
     int tune_beam_wg_per_cu = 16;    // persistent beam workgroups (one wave each) per CU
-    int tune_beam_log_vis = 0;       // phase A visited-set slots, log2 (0: chosen from ef; else 10..13)
+    int tune_beam_vis_shift = 0;     // phase A visited-set slots: log2 shift from the ef rule (-1 .. 2)
+    int tune_bf_frags = 1;           // brute force: 16-row fragments per wave (2: 32-wide lists, d <= 256)
+    int tune_nnd_join = 0;           // NNDescent join staging (0: chosen from d and data; 1 chunked BF16,
+                                     // 2 whole-sample BF16, 3 whole-sample int8)
     int tune_stage1_cap = 0;         // learned early stop: stage-1 expansion cap (0: none)
     int tune_hnsw_batch_permille = 50;   // HNSW construction: batch size / graph size, per mille
     int tune_hnsw_seed_exp = 32;         // HNSW construction: expansions of the seed completion

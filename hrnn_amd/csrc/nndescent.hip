@@ -162,24 +162,6 @@ constexpr int kJDP = kJDC + 8;  // padded row stride
 constexpr int kJMaxB = 32;      // Gram blocks per wave
 constexpr int kJRev = 256;      // reverse offers kept per node and kind
 
-__device__ float l2_ref16_lane(const float *__restrict__ a, const float *__restrict__ b, uint32_t d) {
-    if (d & 15u) return l2_seq(a, b, d);
-    float p[16];
-#pragma unroll
-    for (int l = 0; l < 16; l++) p[l] = 0.f;
-    for (uint32_t e = 0; e < d; e += 16) {
-#pragma unroll
-        for (int l = 0; l < 16; l++) {
-            float t = a[e + l] - b[e + l];
-            p[l] = __builtin_fmaf(t, t, p[l]);
-        }
-    }
-    float r = p[0];
-#pragma unroll
-    for (int l = 1; l < 16; l++) r += p[l];
-    return r;
-}
-
 // Sample staging modes of the join.  kJoinChunked: BF16 rows staged 64 dims
 // at a time, each wave keeping the accumulators of all its Gram blocks across
 // the chunks.  kJoinFullBf16: the whole sample (T rows x d BF16 dims) staged
@@ -358,6 +340,21 @@ __global__ __launch_bounds__(256) void k_nnd_join(
         }
     }
 
+    // Float data (not exact in BF16): the Gram value v_g of a pair decides
+    // which pairs need their exact distance, with a margin that bounds its
+    // error, |v_g - v| <= g_rel (|a|^2 + |b|^2): BF16 rounding of both rows
+    // (2u + u^2 with u = 2^-8, times 2 |a.b| <= |a|^2 + |b|^2), FP32
+    // accumulation over d products and of the norms (about d 2^-24 each), and
+    // the expression's own roundings, with a quarter to spare.  The counting
+    // pass reserves a slot for every pair whose v_g - margin is below a
+    // threshold; the filling pass computes those pairs' distances exactly
+    // (FP32 rows, the reference's order, one quad of lanes per pair) and
+    // writes the key of each that passes, an empty key (~0, which the merge
+    // skips) in the slot of each that does not.  No pair below its threshold
+    // is missed, as in the exact modes.
+    const bool approx = kMode != kJoinFullI8 && !exact;
+    const float g_rel = 0.01f + 1.6e-7f * (float)d;
+
     // Proposals go to sample slots (target = sid[slot]).  In the MFMA output
     // layout the 16 lanes of a lane group share the row slot and the 4 lane
     // groups x 4 rows of a lane share the column slot, so proposals are summed
@@ -369,11 +366,13 @@ __global__ __launch_bounds__(256) void k_nnd_join(
     // values and thresholds, so the ranges are filled exactly.
     uint32_t ndist = 0, nprop = 0, dropped = 0;
     const uint32_t grp = lane >> 4, gl = lane & 15u;
-    // pa/pb/dv of the 4 pairs a lane holds in Gram block (rb, cb) (squared
-    // distances dg from the Gram values), then on_block
+    // The 4 pairs a lane holds in Gram block (rb, cb) (squared distances dg
+    // from the Gram values): pa/pb, a slot reserved at the row / column
+    // target; ea/eb, a proposal made there (pa/pb themselves unless the data
+    // are approximate); dv, the exact distance where ea or eb; then on_block.
     auto block_props = [&](uint32_t rb, uint32_t cb, const float *dg, auto &&on_block) {
         const uint32_t col = cb * 16 + gl;
-        bool pa[4], pb[4];
+        bool pa[4], pb[4], ea[4], eb[4];
         float dv[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -384,20 +383,50 @@ __global__ __launch_bounds__(256) void k_nnd_join(
                 const uint32_t ia = sid[r], ib = sid[col];
                 if (ia != ib) {
                     if (!kFill) ndist++;
-                    float v = dg[i];
-                    bool qa = v < sthr[r], qb = v < sthr[col];
-                    if (kMode != kJoinFullI8 && (qa || qb) && !exact) {
-                        v = l2_ref16_lane(x32 + (uint64_t)ia * d, x32 + (uint64_t)ib * d, d);
-                        qa = v < sthr[r];
-                        qb = v < sthr[col];
-                    }
-                    pa[i] = qa;
-                    pb[i] = qb;
+                    const float v = dg[i];
+                    const float lo = approx ? v - g_rel * (snorm[r] + snorm[col]) : v;
+                    pa[i] = lo < sthr[r];
+                    pb[i] = lo < sthr[col];
                     dv[i] = v;
                 }
             }
+            ea[i] = pa[i];
+            eb[i] = pb[i];
         }
-        on_block(rb * 16 + 4 * grp, col, pa, pb, dv);
+        if (kFill && approx) {
+            // the exact distances of the reserved pairs: per pair index i, the
+            // lanes' pairs in a per-wave LDS queue (the int8 norms' array,
+            // unused here: 64 entries per wave), 16 pairs per round, one per
+            // quad; each quad's first lane writes its distance over its entry
+            uint32_t *qw = reinterpret_cast<uint32_t *>(s_n8) + 64u * w;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const bool nd = pa[i] || pb[i];
+                const uint64_t m = __ballot(nd);
+                const uint32_t cnt = (uint32_t)__popcll(m);   // uniform
+                if (cnt == 0) continue;
+                const uint32_t pos = (uint32_t)__popcll(m & lanemask_lt());
+                const uint32_t r = rb * 16 + 4 * grp + i;
+                if (nd) qw[pos] = (r << 8) | col;
+                wave_lds_sync();
+                for (uint32_t b0 = 0; b0 < cnt; b0 += 16) {
+                    const uint32_t it = b0 + (lane >> 2);
+                    const bool ok = it < cnt;
+                    const uint32_t pr = ok ? qw[it] : 0u;
+                    const float v = pair_quad_f32(x32, sid[pr >> 8], sid[pr & 255u], d, lane & 3u, ok);
+                    if (ok && (lane & 3u) == 0) qw[it] = __float_as_uint(v);
+                }
+                wave_lds_sync();
+                if (nd) {
+                    const float v = __uint_as_float(qw[pos]);
+                    dv[i] = v;
+                    ea[i] = pa[i] && v < sthr[r];
+                    eb[i] = pb[i] && v < sthr[col];
+                }
+                wave_lds_sync();
+            }
+        }
+        on_block(rb * 16 + 4 * grp, col, pa, pb, ea, eb, dv);
     };
     auto each_block = [&](auto &&on_block) {
         // squared distances of a lane's 4 pairs from the Gram values g
@@ -457,14 +486,15 @@ __global__ __launch_bounds__(256) void k_nnd_join(
         s_pcur[t] = 0;
     }
     __syncthreads();
-    each_block([&](uint32_t r0, uint32_t col, const bool *pa, const bool *pb, const float *) {
+    each_block([&](uint32_t r0, uint32_t col, const bool *pa, const bool *pb, const bool *ea, const bool *eb,
+                   const float *) {
         uint32_t cb_cnt = 0;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const uint32_t gm = (uint32_t)(__ballot(pa[i]) >> (16 * grp)) & 0xFFFFu;
             if (gl == 0 && gm) atomicAdd(&s_pcnt[r0 + i], (uint32_t)__popc(gm));
             cb_cnt += pb[i] ? 1u : 0u;
-            if (!kFill) nprop += (pa[i] ? 1u : 0u) + (pb[i] ? 1u : 0u);
+            if (kFill) nprop += (ea[i] ? 1u : 0u) + (eb[i] ? 1u : 0u);
         }
         cb_cnt += __shfl_xor(cb_cnt, 16, 64);
         cb_cnt += __shfl_xor(cb_cnt, 32, 64);
@@ -480,15 +510,18 @@ __global__ __launch_bounds__(256) void k_nnd_join(
     }
     if (kFill) {
         __syncthreads();
-        auto put = [&](uint32_t target, uint64_t at, uint32_t cand, float v) {
+        // a reserved slot: the (d2, id) sort key of a proposal, or the empty
+        // key where the exact distance did not pass
+        auto put = [&](uint32_t target, uint64_t at, uint32_t cand, float v, bool made) {
             const uint64_t pos = prop_off[target] + at;
             if (pos < cap) {
-                prop_key[pos] = ((unsigned long long)float_key(v) << 32) | cand;   // (d2, id) sort key
-            } else {
+                prop_key[pos] = made ? ((unsigned long long)float_key(v) << 32) | cand : ~0ull;
+            } else if (made) {
                 dropped++;
             }
         };
-        each_block([&](uint32_t r0, uint32_t col, const bool *pa, const bool *pb, const float *dv) {
+        each_block([&](uint32_t r0, uint32_t col, const bool *pa, const bool *pb, const bool *ea, const bool *eb,
+                       const float *dv) {
             const uint32_t ib = col < T ? sid[col] : 0u;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
@@ -498,7 +531,7 @@ __global__ __launch_bounds__(256) void k_nnd_join(
                 base = __shfl(base, (int)(16 * grp), 64);
                 if (pa[i]) {
                     const uint32_t r = r0 + i;
-                    put(sid[r], (uint64_t)s_pbase[r] + base + __popc(gm & ((1u << gl) - 1u)), ib, dv[i]);
+                    put(sid[r], (uint64_t)s_pbase[r] + base + __popc(gm & ((1u << gl) - 1u)), ib, dv[i], ea[i]);
                 }
             }
             uint32_t mine = 0;
@@ -514,7 +547,7 @@ __global__ __launch_bounds__(256) void k_nnd_join(
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 if (pb[i]) {
-                    put(ib, (uint64_t)s_pbase[col] + k, sid[r0 + i], dv[i]);
+                    put(ib, (uint64_t)s_pbase[col] + k, sid[r0 + i], dv[i], eb[i]);
                     k++;
                 }
             }
@@ -529,7 +562,7 @@ __global__ __launch_bounds__(256) void k_nnd_join(
     }
     if (lane == 0) {
         if (!kFill && ndist) atomicAdd(stat_at(stats, ST_DISTS), (unsigned long long)ndist);
-        if (!kFill && nprop) atomicAdd(stat_at(stats, ST_PROPOSALS), (unsigned long long)nprop);
+        if (kFill && nprop) atomicAdd(stat_at(stats, ST_PROPOSALS), (unsigned long long)nprop);
         if (kFill && dropped) atomicAdd(stat_at(stats, ST_DROPPED), (unsigned long long)dropped);
     }
 }
@@ -796,6 +829,7 @@ void nndescent_run(hrnn_dev_t h, int iters, int S, uint32_t per_node_cap, hrnn_n
     int mode = kJoinChunked;
     if (h->base_u8 && h->d <= 258 && i8_lds <= 56 * 1024) mode = kJoinFullI8;
     else if (bf16_lds <= 56 * 1024) mode = kJoinFullBf16;
+    if (h->tune_nnd_join) mode = h->tune_nnd_join - 1;   // the tuning key (same lists in every mode)
     if (const char *jm = std::getenv("HRNN_NND_JOIN")) mode = std::min(std::max(std::atoi(jm), 0), 2);
     if (mode == kJoinFullI8 && !(h->base_u8 && h->d <= 258 && i8_lds <= 64 * 1024)) mode = kJoinChunked;
     if (mode == kJoinFullBf16 && bf16_lds > 64 * 1024) mode = kJoinChunked;

@@ -29,7 +29,7 @@ STATUS = {0: "OK", -1: "E_ARG", -2: "E_HIP", -3: "E_STATE", -4: "E_OOM", -5: "E_
 EXPORTS = [
     "hrnn_dev_device_count", "hrnn_dev_open", "hrnn_dev_close", "hrnn_dev_last_error",
     "hrnn_dev_sync", "hrnn_dev_stream", "hrnn_dev_set_base", "hrnn_dev_set_base_dptr",
-    "hrnn_dev_append_base", "hrnn_dev_base_is_small_int", "hrnn_dev_base_ptr",
+    "hrnn_dev_append_base", "hrnn_dev_base_is_small_int", "hrnn_dev_base_is_u8", "hrnn_dev_base_ptr",
     "hrnn_dev_set_vector_precision", "hrnn_dev_set_tuning", "hrnn_dev_get_tuning",
     "hrnn_dev_set_knng", "hrnn_dev_get_knng", "hrnn_dev_get_knng_flags", "hrnn_dev_set_knng_flags",
     "hrnn_dev_knng_ptrs", "hrnn_dev_hnsw_build", "hrnn_dev_get_hnsw", "hrnn_dev_nndescent",
@@ -100,6 +100,7 @@ def lib() -> C.CDLL:
         "hrnn_dev_set_base_dptr": [vp, vp, u64, u32],
         "hrnn_dev_append_base": [vp, vp, u64],
         "hrnn_dev_base_is_small_int": [vp, P(i)],
+        "hrnn_dev_base_is_u8": [vp, P(i)],
         "hrnn_dev_set_vector_precision": [vp, i],
         "hrnn_dev_set_tuning": [vp, C.c_char_p, i],
         "hrnn_dev_get_tuning": [vp, C.c_char_p, P(i)],
@@ -271,6 +272,13 @@ class Device:
         x = np.ascontiguousarray(x, np.float32)
         self._ck(self.L.hrnn_dev_append_base(self.h, _p(x), x.shape[0]))
         self.n += x.shape[0]
+
+    def base_is_u8(self) -> bool:
+        """Every base value an integer in [0, 255] (the u8 mirror and the
+        int8 NNDescent join apply)."""
+        v = C.c_int(0)
+        self._ck(self.L.hrnn_dev_base_is_u8(self.h, C.byref(v)))
+        return bool(v.value)
 
     def base_is_small_int(self) -> bool:
         v = C.c_int(0)

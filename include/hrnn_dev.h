@@ -75,6 +75,10 @@ int hrnn_dev_set_base_dptr(hrnn_dev_t h, const float *x_dev, uint64_t n, uint32_
 int hrnn_dev_append_base(hrnn_dev_t h, const float *x, uint64_t n_add);
 /* 1 if every base value is an integer in [-256, 256] (BF16/FP16-exact data). */
 int hrnn_dev_base_is_small_int(hrnn_dev_t h, int *is_int);
+/* 1 if every base value is an integer in [0, 255] (the u8 mirror and the int8
+ * NNDescent join apply).  No reference counterpart: the autotuner's choice of
+ * candidates. */
+int hrnn_dev_base_is_u8(hrnn_dev_t h, int *is_u8);
 /* Device pointer of the FP32 base (n*d floats). */
 const float *hrnn_dev_base_ptr(hrnn_dev_t h);
 /* Storage the search and phase-A kernels read vectors from: HRNN_PREC_BF16
@@ -104,9 +108,16 @@ int hrnn_dev_set_vector_precision(hrnn_dev_t h, int prec);
  * offer buffer (entries per batch; it grows when a batch needs more);
  * "insert_lds_bitmap" (0 or 1, default 1): the insertion's visited bitmap in
  * LDS when n bits fit there (160 KiB per workgroup on gfx950), else in HBM
- * (the same results).  "beam_log_vis" (0 or 10 .. 13, default 0 = chosen
- * from ef): log2 of phase A's LDS visited-set slots per query (more slots
- * cost occupancy, fewer overflow on long beams; the same results). */
+ * (the same results).  "beam_vis_shift" (-1 .. 2, default 0): phase A's
+ * LDS visited set per query is 2^(rule(ef) + shift) slots, within
+ * 2^10 .. 2^13 (more slots cost occupancy, fewer overflow on long beams; the
+ * same results).  "bf_row_frags" (1 or 2, default 1): the MFMA brute force's
+ * 16-row fragments per wave (2 applies to 32-wide candidate lists at
+ * d <= 256; the same results).  "nnd_join" (0 .. 3, default 0 = chosen from
+ * d and the data): NNDescent's join staging, 1 chunked BF16, 2 whole-sample
+ * BF16, 3 whole-sample int8 (8-bit data only; a mode that does not fit falls
+ * back to chunked; the same lists).  hrnn_amd/autotune.py times these
+ * choices per workload and caches the fastest. */
 int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value);
 int hrnn_dev_get_tuning(hrnn_dev_t h, const char *key, int *value);
 

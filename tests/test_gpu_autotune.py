@@ -1,0 +1,146 @@
+"""The autotuner's keys and tuners (hrnn_amd/autotune.py).  Its premise is
+that every candidate of a key gives the same results, only faster or
+slower; each key is checked for that here, and each tuner runs, returns
+one of its candidates and leaves the device state as it found it."""
+import numpy as np
+import pytest
+
+import hrnn_amd
+from hrnn_amd import autotune, hiprt
+from hrnn_amd.dev import Device
+from hrnn_amd.index import Index
+
+pytestmark = pytest.mark.gpu
+
+
+def _sphere(n, d, seed):
+    return hrnn_amd.synth_sphere_mixture(seed, 88, n, d, 200, 50.0)
+
+
+def test_tuning_keys_ranges(gpu_dev):
+    for key, good, bad in (("beam_vis_shift", (-1, 0, 2), (-2, 3)), ("bf_row_frags", (1, 2), (0, 3)),
+                           ("nnd_join", (0, 1, 3), (-1, 4))):
+        for v in good:
+            gpu_dev.set_tuning(key, v)
+            assert gpu_dev.get_tuning(key) == v
+        for v in bad:
+            with pytest.raises(Exception):
+                gpu_dev.set_tuning(key, v)
+    gpu_dev.set_tuning("beam_vis_shift", 0)
+    gpu_dev.set_tuning("bf_row_frags", 1)
+    gpu_dev.set_tuning("nnd_join", 0)
+
+
+@pytest.fixture(scope="module")
+def sphere_index():
+    n, d = 20_000, 96
+    x = _sphere(n, d, 5)
+    q = _sphere(500, d, 6)
+    ix = Index(n, d, 16, 100, 100, 32)
+    ix.add_points_gpu(x)
+    dev = Device.borrow(ix.device_handle(), n, d, 32)
+    yield ix, dev, x, q
+
+
+def test_beam_vis_shift_same_proxies(sphere_index):
+    """Phase A at ef 200 (float rows, about 2,000 ids evaluated per query on
+    these data) with the visited set at 2^10 .. 2^13 slots: the same proxies."""
+    ix, dev, x, q = sphere_index
+    out = []
+    try:
+        for s in (-1, 0, 1, 2):
+            dev.set_tuning("beam_vis_shift", s)
+            out.append(dev.proxies(q, 50, 200))
+    finally:
+        dev.set_tuning("beam_vis_shift", 0)
+    for p in out[1:]:
+        assert np.array_equal(p, out[0])
+
+
+@pytest.mark.parametrize("kind", ["int", "sphere"])
+@pytest.mark.parametrize("prec", ["bf16", "f16"])
+def test_bf_row_frags_same_results(gpu_dev, kind, prec):
+    """The MFMA brute force with one and two 16-row fragments per wave (32-wide
+    candidate lists: k + 1 <= 16): the same ids and distances, k-NN of
+    queries and the base's own r_k."""
+    if kind == "int":
+        x = hrnn_amd.synth_mixture(7, 6000, 128, 40, 20.0, 0.0, 255.0, True, 0.0, 1007)
+    else:
+        x = _sphere(6000, 96, 7)
+    q = x[:700] + np.float32(0.25)
+    gpu_dev.set_base(x)
+    res = []
+    try:
+        for fr in (1, 2):
+            gpu_dev.set_tuning("bf_row_frags", fr)
+            ids, d2 = gpu_dev.bf_knn(q, 11, prec)
+            kth, kids = gpu_dev.bf_kth(10, prec, with_ids=True)
+            res.append((ids, d2, kth, kids))
+    finally:
+        gpu_dev.set_tuning("bf_row_frags", 1)
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("kind", ["u8", "sphere"])
+def test_nnd_join_key_same_lists(gpu_dev, kind):
+    """NNDescent from the same seeds with each join staging chosen by the key:
+    on float data (distinct distances) two iterations end with the same lists,
+    ids included, and the same counters; on 8-bit data (many equal
+    distances, where which of two equal ids enters may differ, and with it
+    the next iteration's samples) one iteration ends with the same distance
+    lists after the same number of evaluations."""
+    n, K, S = 3000, 24, 12
+    if kind == "u8":
+        x = hrnn_amd.synth_mixture(9, n, 128, 30, 20.0, 0.0, 255.0, True, 0.0, 1009)
+        modes = (1, 2, 3)
+    else:
+        x = _sphere(n, 96, 9)
+        modes = (1, 2)
+    ix = Index(n, x.shape[1], 8, 40, 40, K)
+    ix.add_points(x, nthreads=1)
+    seed = ix.get_knng()
+    gpu_dev.set_base(x)
+    assert gpu_dev.base_is_u8() == (kind == "u8")
+    out = []
+    try:
+        for mode in modes:
+            gpu_dev.set_tuning("nnd_join", mode)
+            gpu_dev.set_knng(*seed)
+            st = gpu_dev.nndescent(1 if kind == "u8" else 2, S)
+            out.append((gpu_dev.get_knng(), [(s_["dists"], s_["updates"] if kind == "sphere" else 0)
+                                             for s_ in st]))
+    finally:
+        gpu_dev.set_tuning("nnd_join", 0)
+    (ids0, d0, c0), st0 = out[0]
+    for (ids, d, c), st in out[1:]:
+        assert st == st0
+        assert np.array_equal(c, c0)
+        live = np.arange(K)[None, :] < c0[:, None]
+        assert np.array_equal(np.where(live, d, 0), np.where(live, d0, 0))
+        if kind == "sphere":
+            assert np.array_equal(np.where(live, ids, 0), np.where(live, ids0, 0))
+
+
+def test_tuners_run_and_restore(sphere_index):
+    ix, dev, x, q = sphere_index
+    L = hrnn_amd.lib()
+    h = ix.device_handle()
+    nq, m, ef = len(q), 20, 100
+    qd = hiprt.DeviceArray.from_host(q)
+    pd = hiprt.DeviceArray((nq * m,), np.uint32)
+    r = autotune.tune_beam(h, qd, nq, m, ef, pd, {"test": 1}, reps=2, use_cache=False)
+    assert r["beam_vis_shift"] in autotune.CANDIDATES["beam_vis_shift"]
+    assert dev.get_tuning("beam_vis_shift") == r["beam_vis_shift"]
+    r = autotune.tune_bruteforce(dev, x[:2048], 10, "bf16", {"test": 1}, reps=1, use_cache=False)
+    assert r["applies"] and r["bf_row_frags"] in (1, 2) and len(r["timings_ms"]) == 2
+    seed = dev.get_knng()
+    flags = dev.get_knng_flags()
+    r = autotune.tune_nndescent(dev, 8, {"test": 1}, use_cache=False)
+    assert r["nnd_join"] in (1, 2) and len(r["timings_ms"]) == 2
+    after = dev.get_knng()
+    for a, b in zip(seed, after):
+        assert np.array_equal(a, b)
+    assert np.array_equal(flags, dev.get_knng_flags())
+    for key in ("beam_vis_shift", "bf_row_frags", "nnd_join"):
+        dev.set_tuning(key, 0 if key != "bf_row_frags" else 1)

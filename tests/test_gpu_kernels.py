@@ -619,3 +619,38 @@ def test_search_u8_storage_matches_oracle(small_data, small_ref, qshift):
             eoff, eids = oracle.search_bc(off, ent, dev.csr.wide, dev.csr.rank_bits, dists, cnt, r.K, x,
                                           q, prox, 10, 48, thr)
             assert np.array_equal(goff, eoff) and np.array_equal(gids, eids), (kib, thr)
+
+
+@pytest.mark.parametrize("join_mode", ["1", "0"])
+def test_nndescent_float_data_exact(join_mode, monkeypatch):
+    """Float data (96-dim unit vectors: no BF16 Gram value is exact): one
+    iteration from the reference's own seeds, then a second from the GPU's
+    state with its flags.  The Gram values with their error margin choose
+    the pairs whose exact FP32 distance is computed, so no pair below a
+    threshold is missed: the GPU evaluates exactly the oracle's pairs and ends
+    each iteration with exactly its distance lists (ids agree too: the
+    distances are distinct), in the whole-sample (1) and chunked (0) BF16
+    stagings."""
+    if not oracle.ref_available():
+        pytest.skip("compiled reference (oracle/_ref) not available")
+    monkeypatch.setenv("HRNN_NND_JOIN", join_mode)
+    n, K, S = 2000, 32, 12
+    x = hrnn_amd.synth_sphere_mixture(31, 88, n, 96, 40, 50.0)
+    r = oracle.RefIndex(n, 96, M=8, efc=40, K=K)
+    r.add_points(x, nthreads=1)
+    sid, sd, sc = r.get_knng()
+    _first_iteration_matches_oracle(x, sid, sd, sc, K, S)
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    dev.set_knng(sid, sd, sc)
+    st1 = dev.nndescent(1, S)
+    g1 = dev.get_knng()
+    f1 = dev.get_knng_flags()
+    oid, od, oc, _, _, _ = oracle.nndescent_flags(x, sid, sd, sc, np.ones_like(f1), K, 1, S)
+    m = np.arange(K) < oc[:, None]
+    assert np.array_equal(np.where(m, g1[0], 0), np.where(m, oid, 0))
+    # the proposals the GPU counts are those it made (exact distance below
+    # the threshold), not the slots the margin reserved
+    assert 0 < st1[0]["proposals"] <= 2 * st1[0]["dists"]
+    st, _, _ = _resume_matches_oracle(x, g1[0], g1[1], g1[2], f1, K, S, 1)
+    assert st[0]["dists"] > 0 and st[0]["updates"] > 0
