@@ -407,8 +407,12 @@ def main() -> int:
     ck(L.hrnn_dev_set_vector_precision(h, 3))
     tuned_beam = {f"ef{e_}": autotune.tune_beam(h, qd, nq, m_, e_, pd, dict(n=n, d=d), use_cache=False)
                   for e_, m_ in ((ef, m_star), (EF1, 16))}
+    # one stream or two for a whole resident batch (phase A then B+C), at the
+    # fastest fixed class's shape
+    tuned_q = autotune.tune_query(h, qd, nq, 2, EF1, k, Kp, pd, cd, idd, cap, dict(n=n, d=d, K=K),
+                                  use_cache=False)
     ck(L.hrnn_dev_set_vector_precision(h, 0))
-    log(f"autotune: phase A {tuned_beam}")
+    log(f"autotune: phase A {tuned_beam}; query batch {tuned_q}")
 
     paths = {}
     ck(L.hrnn_dev_set_vector_precision(h, 0))          # FP32 vectors in phases A, B, C
@@ -1061,7 +1065,8 @@ def main() -> int:
                                                  for p_ in ("u8_learned_adaptive", "u8_learned_m_only"))
             else None),
         "learned_m": learned_info,
-        "autotune": {"search": tuned, "phase_a_u8": tuned_beam, "bruteforce": tuned_bf, "nndescent": tuned_nnd},
+        "autotune": {"search": tuned, "phase_a_u8": tuned_beam, "query_u8": tuned_q, "bruteforce": tuned_bf,
+                     "nndescent": tuned_nnd},
         "nndescent_cpu_baseline": nnd_cpu,
         "csr_cpu_baseline": csr_cpu,
         "bruteforce_cpu_baseline": bf_cpu,
@@ -1892,7 +1897,9 @@ def run_c5(args, json_out) -> int:
                                        cap, wd.ptr))
 
     tuned["phase_a"] = autotune.tune_beam(h, qd, nq, 50, ef, pd, dict(n=n, d=d), use_cache=False)
-    log(f"autotune: phase A {tuned['phase_a']}")
+    tuned["query"] = autotune.tune_query(h, qd, nq, 50, ef, k, Kp, pd, cd, idd, cap, dict(n=n, d=d, K=K),
+                                         use_cache=False)
+    log(f"autotune: phase A {tuned['phase_a']}; query batch {tuned['query']}")
 
     # ---- plain fixed m at ef 200 over the grid; the recall target ------------------------------
     rec_m = {}

@@ -16,7 +16,9 @@ changes (the GPU tests check this for each key).
   per wave: base-tile reuse against blocks per CU);
 * tune_nndescent (the NNDescent join): "nnd_join" (chunked BF16, whole-sample
   BF16 or whole-sample int8 staging of each node's sample in LDS), timed on
-  the first iteration from the seed graph, which is restored afterwards.
+  the first iteration from the seed graph, which is restored afterwards;
+* tune_query (a resident query batch, phase A then phases B+C):
+  "query_streams" (the batch's two halves on one stream or on two).
 """
 from __future__ import annotations
 
@@ -35,6 +37,7 @@ CANDIDATES = {
     "beam_vis_shift": [-1, 0, 1, 2],
     "bf_row_frags": [1, 2],
     "nnd_join": [1, 2, 3],
+    "query_streams": [1, 2],
 }
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "cache.json")
 
@@ -251,6 +254,27 @@ def tune_nndescent(dev, sample: int, shape: dict | None = None, use_cache: bool 
         dev.set_knng(*seed)
         dev.set_knng_flags(flags)
     out = {"nnd_join": best, "timings_ms": timings}
+    if use_cache:
+        cache[key] = out
+        save_cache(cache)
+    return out
+
+
+def tune_query(h, qd, nq: int, m: int, ef: int, k: int, K_prime: int, pd, cd, idd, cap: int, shape: dict,
+               reps: int = 5, use_cache: bool = True) -> dict:
+    """A resident batch through hrnn_dev_rknn_query_dptr (phase A at this m
+    and ef, then phases B+C): "query_streams"."""
+    L = lib()
+    key = _key(dict(shape, kernel="query", m=m, ef=ef, k=k, Kp=K_prime, nq=nq))
+    hit, cache = _cached(key, use_cache, L, h)
+    if hit:
+        return hit
+    time_it = _event_timer(L, h, reps)
+    fn = lambda: ck_(L, h, L.hrnn_dev_rknn_query_dptr(h, qd.ptr, nq, m, ef, None, k, K_prime, 1.0, pd.ptr,
+                                                      cd.ptr, idd.ptr, cap, None))
+    timings = {}
+    out = {"query_streams": _pick(L, h, "query_streams", CANDIDATES["query_streams"], time_it, fn, timings),
+           "timings_ms": timings}
     if use_cache:
         cache[key] = out
         save_cache(cache)

@@ -370,6 +370,7 @@ static int *tuning_slot(hrnn_dev_t h, const char *key) {
     if (k == "beam_vis_shift") return &h->tune_beam_vis_shift;
     if (k == "bf_row_frags") return &h->tune_bf_frags;
     if (k == "nnd_join") return &h->tune_nnd_join;
+    if (k == "query_streams") return &h->tune_query_streams;
     return nullptr;
 }
 
@@ -384,8 +385,9 @@ int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value) {
     const bool vshift = slot == &h->tune_beam_vis_shift;
     const bool frags = slot == &h->tune_bf_frags;
     const bool join = slot == &h->tune_nnd_join;
+    const bool qstreams = slot == &h->tune_query_streams;
     if (!slot || value < (vshift ? -1 : cap || seedx || offers || flag || join ? 0 : 1) ||
-        value > (offers ? (1 << 30) : flag ? 1 : vshift || frags ? 2 : join ? 3 : cap ? 1024
+        value > (offers ? (1 << 30) : flag ? 1 : vshift || frags || qstreams ? 2 : join ? 3 : cap ? 1024
                                                                    : permille || seedx ? 1000 : 64)) {
         h->err = std::string("set_tuning: unknown key or value out of range: ") + (key ? key : "");
         return HRNN_E_ARG;

@@ -502,8 +502,9 @@ void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint
         (uint32_t)std::min(13, std::max(10, (int)beam_log_vis(ef) + h->tune_beam_vis_shift));
     const BeamLds lay(h->d, ef_cap, log_vis);
     REQUIRE(lay.total <= 65536, HRNN_E_ARG, "proxies: LDS need exceeds 64 KiB");
+    hipStream_t st = o.stream ? o.stream : h->stream;
 #define LAUNCH1(VT, PQ, XP)                                                                       \
-    hipLaunchKernelGGL((k_phase_a<VT, PQ>), dim3(nq), dim3(64), lay.total, h->stream, XP, h->d,  \
+    hipLaunchKernelGGL((k_phase_a<VT, PQ>), dim3(nq), dim3(64), lay.total, st, XP, h->d,         \
                        (uint32_t)h->n, h->l0.as<uint32_t>(), h->l0_words, h->up_off.as<uint32_t>(), \
                        h->up_pool.as<uint32_t>(), h->up_pool_words, h->up_words, h->entry,        \
                        h->maxlevel, q_dev, ef, ef_cap, log_vis, m, base_exact, proxies_dev,       \
@@ -619,9 +620,39 @@ int hrnn_dev_rknn_query_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint
         REQUIRE(h->csr_valid && h->K > 0, HRNN_E_STATE, "rknn_query: build the CSR first");
         REQUIRE(k_per_q_dev || (k >= 1 && k <= h->K), HRNN_E_ARG, "rknn_query: k out of [1, K]");
         if (nq == 0) return;
-        beam_launch(h, q_dev, nq, m, std::max(ef, m), proxies_dev, nullptr);
-        rknn_search_resident(h, q_dev, nq, proxies_dev, m, k_per_q_dev, k, K_prime, threshold,
-                             res_cnt_dev, res_ids_dev, cap_per_q, work_dev);
+        const uint32_t efm = std::max(ef, m);
+        if (h->tune_query_streams != 2 || nq < 2048) {
+            beam_launch(h, q_dev, nq, m, efm, proxies_dev, nullptr);
+            rknn_search_resident(h, q_dev, nq, proxies_dev, m, k_per_q_dev, k, K_prime, threshold,
+                                 res_cnt_dev, res_ids_dev, cap_per_q, work_dev);
+            return;
+        }
+        // Two halves, each phase A then phases B+C, on the handle's two
+        // streams: the halves' kernels fill each other's idle slots (the same
+        // answers; the second half's search has its own queue word and
+        // fallback buffers).  The main stream waits for both.
+        ensure_stream2(h);
+        if (!k_per_q_dev) rk_column_ready(h, k, h->stream);
+        HIPC(hipEventRecord(h->ev_a, h->stream));
+        HIPC(hipStreamWaitEvent(h->stream2, h->ev_a, 0));
+        const uint32_t n0 = nq / 2, n1 = nq - n0;
+        const uint64_t d = h->d;
+        beam_launch(h, q_dev, n0, m, efm, proxies_dev, nullptr);
+        BeamOpt o2;
+        o2.stream = h->stream2;
+        beam_launch(h, q_dev + n0 * d, n1, m, efm, proxies_dev + (uint64_t)n0 * m, nullptr, nullptr, &o2);
+        rknn_search_resident(h, q_dev, n0, proxies_dev, m, k_per_q_dev, k, K_prime, threshold, res_cnt_dev,
+                             res_ids_dev, cap_per_q, work_dev);
+        SearchSel s2;
+        s2.stream = h->stream2;
+        s2.queue_word = kQueue2Word;
+        s2.buffer_set = 1;
+        rknn_search_resident(h, q_dev + n0 * d, n1, proxies_dev + (uint64_t)n0 * m,
+                             m, k_per_q_dev ? k_per_q_dev + n0 : nullptr, k, K_prime, threshold,
+                             res_cnt_dev + n0, res_ids_dev + (uint64_t)n0 * cap_per_q, cap_per_q,
+                             work_dev ? work_dev + 2ull * n0 : nullptr, &s2);
+        HIPC(hipEventRecord(h->ev_b, h->stream2));
+        HIPC(hipStreamWaitEvent(h->stream, h->ev_b, 0));
     });
 }
 

@@ -48,6 +48,7 @@ struct hrnn_dev {
     int tune_beam_wg_per_cu = 16;    // persistent beam workgroups (one wave each) per CU
     int tune_beam_vis_shift = 0;     // phase A visited-set slots: log2 shift from the ef rule (-1 .. 2)
     int tune_bf_frags = 1;           // brute force: 16-row fragments per wave (2: 32-wide lists, d <= 256)
+    int tune_query_streams = 1;      // resident query batches: halves on two streams (2) or one (1)
     int tune_nnd_join = 0;           // NNDescent join staging (0: chosen from d and data; 1 chunked BF16,
                                      // 2 whole-sample BF16, 3 whole-sample int8)
     int tune_stage1_cap = 0;         // learned early stop: stage-1 expansion cap (0: none)
@@ -209,6 +210,7 @@ constexpr int kFeatWords = 32;       // learned-m features per query
 // Optional per-query controls and outputs of a phase-A launch (device
 // pointers; null = unused).
 struct BeamOpt {
+    hipStream_t stream = nullptr;     // the launch's stream (null: the handle's)
     const uint32_t *ef_q = nullptr;   // per-query ef, <= the launch's ef; 0 skips the query
     const uint32_t *m_q = nullptr;    // per-query proxy count; later slots get 0xFFFFFFFF
     float *feats = nullptr;           // kFeatWords features per query
@@ -236,6 +238,12 @@ struct BeamOpt {
     // a wider beam instead
     uint32_t max_exp = 0;
 };
+// The r_k column for k (n floats, rk[i] = k-th distance of i's list) built on
+// stream st unless it is current; searches that run concurrently on several
+// streams call this first, on the stream they all follow.
+void rk_column_ready(hrnn_dev_t h, uint32_t k, hipStream_t st);
+// The handle's second stream and its two events (created on first use).
+void ensure_stream2(hrnn_dev_t h);
 void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint32_t ef,
                  uint32_t *proxies_dev, uint32_t *expansions_dev, float *raw_dev = nullptr,
                  const BeamOpt *opt = nullptr);

@@ -261,11 +261,9 @@ int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
         uint32_t *m_q = ef_q + nq;
         uint32_t *lists = m_q + nq;   // [escalated | the rest], nq each
         uint32_t *list_cnt = h->counters.as<uint32_t>() + kListCountWord;
-        if (!h->stream2) {
-            HIPC(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
-            HIPC(hipEventCreateWithFlags(&h->ev_a, hipEventDisableTiming));
-            HIPC(hipEventCreateWithFlags(&h->ev_b, hipEventDisableTiming));
-        }
+        ensure_stream2(h);
+        // the r_k column of the two concurrent searches, before they fork
+        if (search) rk_column_ready(h, k, h->stream);
         HIPC(hipMemsetAsync(list_cnt, 0, 8, h->stream));
         // stage 1: every query at ef1; with a model, the features, the MLP and
         // the class decision run inside the beam kernel's epilogue

@@ -372,6 +372,26 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
     }
 }
 
+void rk_column_ready(hrnn_dev_t h, uint32_t k, hipStream_t st) {
+    const uint64_t n = h->n;
+    if (k < 1 || k > h->K) return;
+    if (h->rk_col_k == k && h->rk_col_version == h->knn_version && h->rk_col_n == n) return;
+    h->rk_col.reserve(n * 4);
+    hipLaunchKernelGGL(k_rk_column, dim3(ceil_div(n, 256)), dim3(256), 0, st, h->knn_d.as<float>(),
+                       h->knn_cnt.as<uint32_t>(), n, h->K, k, h->rk_col.as<float>());
+    HIPC(hipGetLastError());
+    h->rk_col_k = k;
+    h->rk_col_version = h->knn_version;
+    h->rk_col_n = n;
+}
+
+void ensure_stream2(hrnn_dev_t h) {
+    if (h->stream2) return;
+    HIPC(hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking));
+    HIPC(hipEventCreateWithFlags(&h->ev_a, hipEventDisableTiming));
+    HIPC(hipEventCreateWithFlags(&h->ev_b, hipEventDisableTiming));
+}
+
 void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint32_t *proxies,
                           uint32_t m, const uint32_t *k_per_q, uint32_t k, uint32_t K_prime,
                           float threshold, uint32_t *res_cnt, uint32_t *res_ids,
@@ -453,15 +473,7 @@ void rknn_search_resident(hrnn_dev_t h, const float *q, uint32_t nq, const uint3
     if (so.scratch) {
         if (!k_per_q && rk_ready) rk = h->rk_col.as<float>();   // else read the list matrix
     } else if (!k_per_q && k >= 1 && k <= h->K) {
-        if (!rk_ready) {
-            h->rk_col.reserve(n * 4);
-            hipLaunchKernelGGL(k_rk_column, dim3(ceil_div(n, 256)), dim3(256), 0, st, kd, kc, n, h->K, k,
-                               h->rk_col.as<float>());
-            HIPC(hipGetLastError());
-            h->rk_col_k = k;
-            h->rk_col_version = h->knn_version;
-            h->rk_col_n = n;
-        }
+        rk_column_ready(h, k, st);
         rk = h->rk_col.as<float>();
     }
     // Candidate vectors are read from the u8 mirror (HRNN_PREC_U8) or the BF16

@@ -116,7 +116,10 @@ int hrnn_dev_set_vector_precision(hrnn_dev_t h, int prec);
  * d <= 256; the same results).  "nnd_join" (0 .. 3, default 0 = chosen from
  * d and the data): NNDescent's join staging, 1 chunked BF16, 2 whole-sample
  * BF16, 3 whole-sample int8 (8-bit data only; a mode that does not fit falls
- * back to chunked; the same lists).  hrnn_amd/autotune.py times these
+ * back to chunked; the same lists).  "query_streams" (1 or 2, default 1):
+ * hrnn_dev_rknn_query_dptr with 2 runs the two halves of a batch of at least
+ * 2,048 queries (each phase A then phases B+C) on two streams, so that their
+ * kernels overlap (the same answers).  hrnn_amd/autotune.py times these
  * choices per workload and caches the fastest. */
 int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value);
 int hrnn_dev_get_tuning(hrnn_dev_t h, const char *key, int *value);

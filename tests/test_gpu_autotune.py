@@ -19,7 +19,7 @@ def _sphere(n, d, seed):
 
 def test_tuning_keys_ranges(gpu_dev):
     for key, good, bad in (("beam_vis_shift", (-1, 0, 2), (-2, 3)), ("bf_row_frags", (1, 2), (0, 3)),
-                           ("nnd_join", (0, 1, 3), (-1, 4))):
+                           ("nnd_join", (0, 1, 3), (-1, 4)), ("query_streams", (1, 2), (0, 3))):
         for v in good:
             gpu_dev.set_tuning(key, v)
             assert gpu_dev.get_tuning(key) == v
@@ -29,6 +29,7 @@ def test_tuning_keys_ranges(gpu_dev):
     gpu_dev.set_tuning("beam_vis_shift", 0)
     gpu_dev.set_tuning("bf_row_frags", 1)
     gpu_dev.set_tuning("nnd_join", 0)
+    gpu_dev.set_tuning("query_streams", 1)
 
 
 @pytest.fixture(scope="module")
@@ -38,6 +39,8 @@ def sphere_index():
     q = _sphere(500, d, 6)
     ix = Index(n, d, 16, 100, 100, 32)
     ix.add_points_gpu(x)
+    ix.refine(2, 8)
+    ix.build_rknng()
     dev = Device.borrow(ix.device_handle(), n, d, 32)
     yield ix, dev, x, q
 
@@ -122,6 +125,43 @@ def test_nnd_join_key_same_lists(gpu_dev, kind):
             assert np.array_equal(np.where(live, ids, 0), np.where(live, ids0, 0))
 
 
+@pytest.mark.parametrize("per_query_k", [False, True])
+def test_query_streams_same_answers(sphere_index, per_query_k):
+    """A resident batch of 3,001 queries (uneven halves) through
+    hrnn_dev_rknn_query_dptr on one stream and as two halves on two streams:
+    the same proxies, counts, answers and work counters, with one k for the
+    batch and with a k per query."""
+    ix, dev, x, q0 = sphere_index
+    L = hrnn_amd.lib()
+    h = ix.device_handle()
+    q = np.concatenate([q0] * 6 + [q0[:1]])
+    nq, m, ef, cap = len(q), 8, 64, 2048
+    kq = (np.arange(nq) % 3 * 4 + 2).astype(np.uint32)
+    qd = hiprt.DeviceArray.from_host(q)
+    kd = hiprt.DeviceArray.from_host(kq)
+    outs = []
+    try:
+        for ns in (1, 2):
+            dev.set_tuning("query_streams", ns)
+            pd = hiprt.DeviceArray((nq * m,), np.uint32)
+            cd = hiprt.DeviceArray((nq,), np.uint32)
+            idd = hiprt.DeviceArray((nq * cap,), np.uint32)
+            wd = hiprt.DeviceArray((2 * nq,), np.uint32)
+            rc = L.hrnn_dev_rknn_query_dptr(h, qd.ptr, nq, m, ef, kd.ptr if per_query_k else None,
+                                            0 if per_query_k else 10, 32, 1.0, pd.ptr, cd.ptr, idd.ptr, cap, wd.ptr)
+            assert rc == 0, L.hrnn_dev_last_error(h).decode()
+            assert L.hrnn_dev_sync(h) == 0
+            cnt = cd.to_host()
+            ids = idd.to_host().reshape(nq, cap)
+            outs.append((pd.to_host(), cnt, [ids[i, :cnt[i]].copy() for i in range(nq)], wd.to_host()))
+    finally:
+        dev.set_tuning("query_streams", 1)
+    a, b = outs
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[3], b[3])
+    assert all(np.array_equal(u, v) for u, v in zip(a[2], b[2]))
+    assert a[1].sum() > 0
+
+
 def test_tuners_run_and_restore(sphere_index):
     ix, dev, x, q = sphere_index
     L = hrnn_amd.lib()
@@ -142,5 +182,15 @@ def test_tuners_run_and_restore(sphere_index):
     for a, b in zip(seed, after):
         assert np.array_equal(a, b)
     assert np.array_equal(flags, dev.get_knng_flags())
-    for key in ("beam_vis_shift", "bf_row_frags", "nnd_join"):
-        dev.set_tuning(key, 0 if key != "bf_row_frags" else 1)
+    ix.build_rknng()
+    cd = hiprt.DeviceArray((nq,), np.uint32)
+    idd = hiprt.DeviceArray((nq * 1024,), np.uint32)
+    pd2 = hiprt.DeviceArray((len(x[:4096]) * m,), np.uint32)
+    qd2 = hiprt.DeviceArray.from_host(np.ascontiguousarray(x[:4096]))
+    cd2 = hiprt.DeviceArray((4096,), np.uint32)
+    idd2 = hiprt.DeviceArray((4096 * 1024,), np.uint32)
+    r = autotune.tune_query(h, qd2, 4096, m, ef, 10, 32, pd2, cd2, idd2, 1024, {"test": 1}, reps=2,
+                            use_cache=False)
+    assert r["query_streams"] in (1, 2) and len(r["timings_ms"]) == 2
+    for key in ("beam_vis_shift", "bf_row_frags", "nnd_join", "query_streams"):
+        dev.set_tuning(key, 0 if key not in ("bf_row_frags", "query_streams") else 1)
