@@ -606,13 +606,25 @@ __global__ __launch_bounds__(kResScanThreads) void k_results_scan(const uint32_t
 __global__ __launch_bounds__(256) void k_results_gather(const uint32_t *__restrict__ cnt,
                                                         const uint32_t *__restrict__ ids, uint32_t nq,
                                                         uint32_t cap, const uint64_t *__restrict__ off,
-                                                        uint32_t *__restrict__ out) {
+                                                        uint32_t *__restrict__ out, uint64_t out_cap) {
     const uint32_t q = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (q >= nq) return;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t c = cnt[q] < cap ? cnt[q] : cap;
     const uint64_t o = off[q];
-    for (uint32_t j = lane; j < c; j += 64u) out[o + j] = ids[(uint64_t)q * cap + j];
+    for (uint32_t j = lane; j < c && o + j < out_cap; j += 64u) out[o + j] = ids[(uint64_t)q * cap + j];
+}
+
+// The device address of page-locked host memory (hipHostMalloc), or null for
+// memory the device cannot reach (pageable).
+static void *pinned_device_ptr(const void *p) {
+    if (!p) return nullptr;
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return a.type == hipMemoryTypeHost ? a.devicePointer : nullptr;
 }
 
 }  // namespace hrnn
@@ -668,6 +680,14 @@ int hrnn_dev_fetch_results(hrnn_dev_t h, uint32_t nq, const uint32_t *res_cnt_de
         HIPC(hipGetLastError());
         HIPC(hipMemcpyAsync(res_off, off, ((uint64_t)nq + 1) * 8, hipMemcpyDeviceToHost, h->stream));
         HIPC(hipMemcpyAsync(small, off + nq + 1, 8, hipMemcpyDeviceToHost, h->stream));
+        // Page-locked id buffer: the gather writes the packed ids straight
+        // into it (at most cap of them), so one wait covers the whole fetch.
+        uint32_t *ids_direct = res_ids && cap ? static_cast<uint32_t *>(pinned_device_ptr(res_ids)) : nullptr;
+        if (ids_direct) {
+            hipLaunchKernelGGL(k_results_gather, dim3(ceil_div(nq, 4)), dim3(256), 0, h->stream, res_cnt_dev,
+                               res_ids_dev, nq, cap_per_q, off, ids_direct, cap);
+            HIPC(hipGetLastError());
+        }
         HIPC(hipStreamSynchronize(h->stream));
         const uint64_t total = res_off[nq], mx = small[0];
         if (mx > cap_per_q) {
@@ -679,10 +699,10 @@ int hrnn_dev_fetch_results(hrnn_dev_t h, uint32_t nq, const uint32_t *res_cnt_de
         if (needed) *needed = total;
         if (total > cap || (total && !res_ids))
             throw Error{HRNN_E_CAP, "fetch_results: result capacity too small"};
-        if (!total) return;
+        if (!total || ids_direct) return;
         h->fetch_ids.reserve(total * 4);
         hipLaunchKernelGGL(k_results_gather, dim3(ceil_div(nq, 4)), dim3(256), 0, h->stream, res_cnt_dev,
-                           res_ids_dev, nq, cap_per_q, off, h->fetch_ids.as<uint32_t>());
+                           res_ids_dev, nq, cap_per_q, off, h->fetch_ids.as<uint32_t>(), total);
         HIPC(hipGetLastError());
         HIPC(hipMemcpyAsync(res_ids, h->fetch_ids.p, total * 4, hipMemcpyDeviceToHost, h->stream));
         HIPC(hipStreamSynchronize(h->stream));

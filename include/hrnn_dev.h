@@ -313,7 +313,9 @@ int hrnn_dev_rknn_search_noalloc_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
  * stream the counts are scanned (each capped at cap_per_q) into offsets and
  * the ids gathered into one array on the device, then res_off[nq+1] (u64)
  * and res_ids[res_off[nq]] are copied to the host (pinned host memory makes
- * both copies run at the link's rate).  Synchronous.  HRNN_E_CAP with
+ * both copies run at the link's rate; a page-locked res_ids is written by the
+ * gather itself, at most cap ids, and the call waits once).  Synchronous.
+ * On HRNN_E_CAP res_ids may hold a partial result.  HRNN_E_CAP with
  * *needed set when a query's true count exceeds cap_per_q (its ids were cut
  * short on the device; *needed = the largest count) or when the total
  * exceeds cap (*needed = the total).  Replaces the per-query result vector

@@ -5,6 +5,8 @@ distances in the reference's summation order bit-exact; NNDescent (a
 randomised, race-ordered algorithm in the reference too) within 0.5 recall
 points of the oracle's single-threaded restatement.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -654,3 +656,45 @@ def test_nndescent_float_data_exact(join_mode, monkeypatch):
     assert 0 < st1[0]["proposals"] <= 2 * st1[0]["dists"]
     st, _, _ = _resume_matches_oracle(x, g1[0], g1[1], g1[2], f1, K, S, 1)
     assert st[0]["dists"] > 0 and st[0]["updates"] > 0
+
+
+def test_fetch_results_into_pinned_memory(small_data, small_ref):
+    """hrnn_dev_fetch_results with page-locked host buffers (the gather
+    writes the ids straight into them, one wait): the same offsets and ids as
+    into pageable arrays; a capacity one short of the total is reported
+    (HRNN_E_CAP, needed = the total) and writes nothing past the capacity;
+    the exact capacity then succeeds."""
+    from hrnn_amd import hiprt
+    x, q = small_data
+    r, _ = small_ref
+    ids, dists, cnt = r.get_knng()
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    dev.set_knng(ids, dists, cnt)
+    dev.build_rknn_csr()
+    m, cap_q = 5, 256
+    prox = np.ascontiguousarray(r.proxies(q, m, 64), np.uint32)
+    nq = len(q)
+    qd = hiprt.DeviceArray.from_host(np.ascontiguousarray(q, np.float32))
+    pd = hiprt.DeviceArray.from_host(prox)
+    cd = hiprt.DeviceArray((nq,), np.uint32)
+    idd = hiprt.DeviceArray((nq * cap_q,), np.uint32)
+    L = dev.L
+    assert L.hrnn_dev_rknn_search_dptr(dev.h, qd.ptr, nq, pd.ptr, m, None, 10, 48, 1.0, cd.ptr, idd.ptr, cap_q,
+                                       None) == 0
+    want_off, want_ids = dev.fetch_results(nq, cd.ptr, idd.ptr, cap_q)
+    total = int(want_off[-1])
+    assert total > 10
+    off_p = hiprt.PinnedArray((nq + 1,), np.uint64)
+    ids_p = hiprt.PinnedArray((total + 8,), np.uint32)
+    ids_p.a[:] = 0xDEADBEEF
+    need = ctypes.c_uint64(0)
+    rc = L.hrnn_dev_fetch_results(dev.h, nq, cd.ptr, idd.ptr, cap_q, off_p.ptr, ids_p.ptr, total - 1,
+                                  ctypes.byref(need))
+    assert rc == -5 and need.value == total
+    assert (ids_p.a[total - 1:] == 0xDEADBEEF).all()
+    rc = L.hrnn_dev_fetch_results(dev.h, nq, cd.ptr, idd.ptr, cap_q, off_p.ptr, ids_p.ptr, total,
+                                  ctypes.byref(need))
+    assert rc == 0 and need.value == total
+    assert np.array_equal(off_p.a, want_off) and np.array_equal(ids_p.a[:total], want_ids)
+    assert (ids_p.a[total:] == 0xDEADBEEF).all()
