@@ -36,6 +36,7 @@ def hip() -> C.CDLL:
         _hip.hipHostFree.argtypes = [vp]
         _hip.hipGetErrorString.restype = C.c_char_p
         _hip.hipGetErrorString.argtypes = [C.c_int]
+        _hip.hipHostGetDevicePointer.argtypes = [C.POINTER(vp), vp, C.c_uint]
     return _hip
 
 
@@ -120,6 +121,13 @@ class PinnedArray:
         self.ptr = p
         buf = (C.c_char * nbytes).from_address(p.value)
         self.a = np.frombuffer(buf, dtype=self.dtype, count=int(np.prod(shape))).reshape(shape)
+
+    def device_ptr(self) -> C.c_void_p:
+        """The address kernels use to read or write this memory directly over
+        the link (hipHostGetDevicePointer)."""
+        dp = C.c_void_p()
+        _ck(hip().hipHostGetDevicePointer(C.byref(dp), self.ptr, 0), "hipHostGetDevicePointer")
+        return dp
 
     def free(self):
         if self.ptr:

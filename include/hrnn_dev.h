@@ -348,7 +348,11 @@ int hrnn_dev_proxies(hrnn_dev_t h, const float *q, uint32_t nq, uint32_t m, uint
 int hrnn_dev_proxies_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint32_t ef,
                           uint32_t *proxies_dev, uint32_t *expansions_dev);
 /* Whole searchRknn for a resident batch: phase A into proxies_dev (nq*m), then
- * phases B and C as hrnn_dev_rknn_search_dptr; one stream, no host round trip. */
+ * phases B and C as hrnn_dev_rknn_search_dptr; one stream, no host round trip.
+ * q_dev (here, in hrnn_dev_rknn_query_adaptive_dptr and in the other _dptr
+ * calls) may also be the device address of page-locked host memory
+ * (hipHostGetDevicePointer): the kernels then read the queries over the link
+ * in place, with no copy before the call. */
 int hrnn_dev_rknn_query_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m,
                              uint32_t ef, const uint32_t *k_per_q_dev, uint32_t k,
                              uint32_t K_prime, float threshold, uint32_t *proxies_dev,

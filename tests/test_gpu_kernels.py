@@ -698,3 +698,43 @@ def test_fetch_results_into_pinned_memory(small_data, small_ref):
     assert rc == 0 and need.value == total
     assert np.array_equal(off_p.a, want_off) and np.array_equal(ids_p.a[:total], want_ids)
     assert (ids_p.a[total:] == 0xDEADBEEF).all()
+
+
+def test_query_reads_pinned_host_queries(small_data):
+    """hrnn_dev_rknn_query_dptr with the queries in page-locked host memory
+    (the kernels read them over the link in place): the same proxies and
+    answers as from device memory, on one stream and on two."""
+    from hrnn_amd import hiprt
+    from hrnn_amd.dev import Device
+    from hrnn_amd.index import Index
+    x, _ = small_data
+    n, d, K = x.shape[0], x.shape[1], 32
+    ix = Index(n, d, 16, 64, 64, K)
+    ix.add_points_gpu(x)
+    ix.refine(2, 12)
+    ix.build_rknng()
+    h = ix.device_handle()
+    dev = Device.borrow(h, n, d, K)
+    q = np.ascontiguousarray(np.concatenate([x[:1500] + np.float32(1.0), x[:1500] + np.float32(2.0)]), np.float32)
+    nq, m, ef, cap = len(q), 5, 64, 512
+    q_pin = hiprt.PinnedArray(q.shape, np.float32)
+    q_pin.a[:] = q
+    qd = hiprt.DeviceArray.from_host(q)
+    out = []
+    try:
+        for streams in (1, 2):
+            dev.set_tuning("query_streams", streams)
+            for qp in (qd.ptr, q_pin.device_ptr()):
+                pd = hiprt.DeviceArray((nq * m,), np.uint32)
+                cd = hiprt.DeviceArray((nq,), np.uint32)
+                idd = hiprt.DeviceArray((nq * cap,), np.uint32)
+                assert dev.L.hrnn_dev_rknn_query_dptr(h, qp, nq, m, ef, None, 10, K, 1.0, pd.ptr, cd.ptr,
+                                                      idd.ptr, cap, None) == 0
+                assert dev.L.hrnn_dev_sync(h) == 0   # the library's streams, before the plain copy below
+                out.append((pd.to_host(),) + dev.fetch_results(nq, cd.ptr, idd.ptr, cap))
+    finally:
+        dev.set_tuning("query_streams", 1)
+    for o in out[1:]:
+        for a, b in zip(out[0], o):
+            assert np.array_equal(a, b)
+    assert out[0][2].size > 0
