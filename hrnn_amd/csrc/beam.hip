@@ -82,27 +82,46 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         if (e == 0) return;
         ef = min(e, ef);
     }
+    // The query is read once (16-byte reads where its row allows: it may sit
+    // in page-locked host memory, read over the link): to LDS as FP32 and,
+    // for u8 storage, packed 4 bytes per word with its sum of squares.
     bool q_u8 = true;
-    for (uint32_t e = lane; e < d; e += 64) {
-        const float v = q[(uint64_t)qi * d + e];
-        s_qv[e] = v;
-        q_u8 &= (v == rintf(v)) && v >= 0.f && v <= 255.f;
-    }
-    // exact sums for this query: every base and query value an integer in [0, 255]
-    const bool exact = base_exact && __ballot(!q_u8) == 0;
-    uint32_t qq = 0;   // u8 storage: the query packed 4 bytes per word, and sum q^2
-    if constexpr (std::is_same<VT, uint8_t>::value) {
-        if (exact) {
-            uint32_t part = 0;
-            for (uint32_t w = lane; 4 * w < d; w += 64) {
-                const uint32_t b0 = (uint32_t)q[(uint64_t)qi * d + 4 * w], b1 = (uint32_t)q[(uint64_t)qi * d + 4 * w + 1];
-                const uint32_t b2 = (uint32_t)q[(uint64_t)qi * d + 4 * w + 2], b3 = (uint32_t)q[(uint64_t)qi * d + 4 * w + 3];
+    uint32_t part = 0;
+    const float *qrow = q + (uint64_t)qi * d;
+    if ((d & 3u) == 0 && (reinterpret_cast<uintptr_t>(qrow) & 15u) == 0) {
+        for (uint32_t w = lane; 4 * w < d; w += 64) {
+            const float4 v = reinterpret_cast<const float4 *>(qrow)[w];
+            reinterpret_cast<float4 *>(s_qv)[w] = v;
+            q_u8 &= is_u8_value(v.x) && is_u8_value(v.y) && is_u8_value(v.z) && is_u8_value(v.w);
+            if constexpr (std::is_same<VT, uint8_t>::value) {
+                const uint32_t b0 = clamp_u8(v.x), b1 = clamp_u8(v.y), b2 = clamp_u8(v.z), b3 = clamp_u8(v.w);
                 s_q8[w] = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
                 part += b0 * b0 + b1 * b1 + b2 * b2 + b3 * b3;
             }
-            for (int s2 = 32; s2 > 0; s2 >>= 1) part += (uint32_t)__shfl_xor((int)part, s2, 64);
-            qq = uni(part);
         }
+    } else {
+        for (uint32_t e = lane; e < d; e += 64) {
+            const float v = qrow[e];
+            s_qv[e] = v;
+            q_u8 &= is_u8_value(v);
+        }
+        if constexpr (std::is_same<VT, uint8_t>::value) {
+            __syncthreads();   // s_qv complete (one wave)
+            for (uint32_t w = lane; 4 * w < d; w += 64) {
+                const uint32_t b0 = clamp_u8(s_qv[4 * w]), b1 = 4 * w + 1 < d ? clamp_u8(s_qv[4 * w + 1]) : 0u;
+                const uint32_t b2 = 4 * w + 2 < d ? clamp_u8(s_qv[4 * w + 2]) : 0u;
+                const uint32_t b3 = 4 * w + 3 < d ? clamp_u8(s_qv[4 * w + 3]) : 0u;
+                s_q8[w] = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
+                part += b0 * b0 + b1 * b1 + b2 * b2 + b3 * b3;
+            }
+        }
+    }
+    // exact sums for this query: every base and query value an integer in [0, 255]
+    const bool exact = base_exact && __ballot(!q_u8) == 0;
+    uint32_t qq = 0;   // u8 storage: the query's sum q^2 (used only when exact)
+    if constexpr (std::is_same<VT, uint8_t>::value) {
+        for (int s2 = 32; s2 > 0; s2 >>= 1) part += (uint32_t)__shfl_xor((int)part, s2, 64);
+        qq = uni(part);
     }
     // the base as a buffer (exact path): loads past its end return zeros
     const __amdgpu_buffer_rsrc_t xrs =

@@ -163,6 +163,11 @@ __device__ __forceinline__ float dist_row<uint8_t>(const float *__restrict__ qv,
     return r;
 }
 
+// An integer in [0, 255], exactly (the u8 mirror represents it).
+__device__ __forceinline__ bool is_u8_value(float v) { return v == rintf(v) && v >= 0.f && v <= 255.f; }
+// v clamped to [0, 255] and truncated (the byte of an is_u8_value v).
+__device__ __forceinline__ uint32_t clamp_u8(float v) { return (uint32_t)fminf(fmaxf(v, 0.f), 255.f); }
+
 // Quad lane k's value in every lane of its quad (DPP quad_perm [k,k,k,k]).
 __device__ __forceinline__ float quad_bcast(float v, int k) {
     switch (k) {

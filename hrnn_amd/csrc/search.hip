@@ -206,25 +206,40 @@ __global__ __launch_bounds__(kSearchBlock) void k_rknn_search(
         const uint32_t idx = s_q;
         if (idx >= (qcount ? *qcount : nq)) break;
         const uint32_t qi = qlist ? qlist[idx] : idx;
+        // The query read once (16-byte reads where its row allows: it may sit
+        // in page-locked host memory): to LDS as FP32 and, for u8 storage,
+        // packed 4 bytes per word with its sum of squares (used only when
+        // every value is an integer in [0, 255]).
         int not_u8 = 0;
-        for (uint32_t e = threadIdx.x; e < d; e += blockDim.x) {
-            const float v = q[(uint64_t)qi * d + e];
-            s_qv[e] = v;
-            not_u8 |= !(v >= 0.f && v <= 255.f && v == rintf(v));
-        }
-        if constexpr (std::is_same<VT, uint8_t>::value) {
-            // u8 storage: the query packed 4 bytes per word, and sum q^2 (used
-            // only when every value is an integer in [0, 255])
-            uint32_t part = 0;
-            for (uint32_t w = threadIdx.x; 4 * w + 3 < d; w += blockDim.x) {
-                const float *qw = q + (uint64_t)qi * d + 4 * w;
-                const uint32_t b0 = (uint32_t)fminf(fmaxf(qw[0], 0.f), 255.f), b1 = (uint32_t)fminf(fmaxf(qw[1], 0.f), 255.f);
-                const uint32_t b2 = (uint32_t)fminf(fmaxf(qw[2], 0.f), 255.f), b3 = (uint32_t)fminf(fmaxf(qw[3], 0.f), 255.f);
-                s_q8[w] = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
-                part += b0 * b0 + b1 * b1 + b2 * b2 + b3 * b3;
+        const float *qrow = q + (uint64_t)qi * d;
+        uint32_t part = 0;
+        if ((d & 3u) == 0 && (reinterpret_cast<uintptr_t>(qrow) & 15u) == 0) {
+            for (uint32_t w = threadIdx.x; 4 * w < d; w += blockDim.x) {
+                const float4 v = reinterpret_cast<const float4 *>(qrow)[w];
+                reinterpret_cast<float4 *>(s_qv)[w] = v;
+                not_u8 |= !(is_u8_value(v.x) && is_u8_value(v.y) && is_u8_value(v.z) && is_u8_value(v.w));
+                if constexpr (std::is_same<VT, uint8_t>::value) {
+                    const uint32_t b0 = clamp_u8(v.x), b1 = clamp_u8(v.y), b2 = clamp_u8(v.z), b3 = clamp_u8(v.w);
+                    s_q8[w] = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
+                    part += b0 * b0 + b1 * b1 + b2 * b2 + b3 * b3;
+                }
             }
-            atomicAdd(&s_qq, part);
+        } else {
+            for (uint32_t e = threadIdx.x; e < d; e += blockDim.x) {
+                const float v = qrow[e];
+                s_qv[e] = v;
+                not_u8 |= !is_u8_value(v);
+            }
+            if constexpr (std::is_same<VT, uint8_t>::value) {
+                for (uint32_t w = threadIdx.x; 4 * w + 3 < d; w += blockDim.x) {
+                    const float *qw = qrow + 4 * w;
+                    const uint32_t b0 = clamp_u8(qw[0]), b1 = clamp_u8(qw[1]), b2 = clamp_u8(qw[2]), b3 = clamp_u8(qw[3]);
+                    s_q8[w] = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
+                    part += b0 * b0 + b1 * b1 + b2 * b2 + b3 * b3;
+                }
+            }
         }
+        if constexpr (std::is_same<VT, uint8_t>::value) atomicAdd(&s_qq, part);
         // locate each proxy's prefix with rank < K'
         for (uint32_t p = threadIdx.x; p < m; p += blockDim.x) {
             const uint32_t px = proxies[(uint64_t)qi * m + p];
