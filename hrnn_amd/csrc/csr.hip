@@ -532,7 +532,9 @@ void csr_build(hrnn_dev_t h) {
     while ((1u << rb) < K) rb++;
     while ((1ull << nb) < n) nb++;
     REQUIRE(nb + rb <= 64, HRNN_E_RANGE, "buildRKNNG: exceeds 64-bit packing");
-    h->wide = (nb + rb > 32) ? 1 : 0;
+    // u64 entries when node and rank do not fit 32 bits, or when the
+    // "csr_wide" tuning key asks for them (the same values, wider words)
+    h->wide = (nb + rb > 32 || h->tune_csr_wide) ? 1 : 0;
     h->rank_bits = rb;
 
     h->csr_off.reserve((n + 1) * 8);

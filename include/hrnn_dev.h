@@ -119,7 +119,10 @@ int hrnn_dev_set_vector_precision(hrnn_dev_t h, int prec);
  * back to chunked; the same lists).  "query_streams" (1 or 2, default 1):
  * hrnn_dev_rknn_query_dptr with 2 runs the two halves of a batch of at least
  * 2,048 queries (each phase A then phases B+C) on two streams, so that their
- * kernels overlap (the same answers).  hrnn_amd/autotune.py times these
+ * kernels overlap (the same answers).  "csr_wide" (0 or 1, default 0): the
+ * reverse CSR with u64 entries even where (node << rank_bits | rank) fits 32
+ * bits (the same values; the layout large graphs need, testable on small
+ * ones).  hrnn_amd/autotune.py times these
  * choices per workload and caches the fastest. */
 int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value);
 int hrnn_dev_get_tuning(hrnn_dev_t h, const char *key, int *value);

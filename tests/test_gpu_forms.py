@@ -105,3 +105,44 @@ def test_learned_form_sphere_mixture_search_parity():
     coff, cent, rb, wide = ix.get_rknng()
     eoff, eres = oracle.search_bc(coff, cent, wide, rb, d, cnt, 48, x, q, prox, 10, 48)
     assert np.array_equal(off, eoff) and np.array_equal(res, eres)
+
+
+def test_batched_form_100k_queries_one_launch():
+    """The batched form's launch size: 100,000 queries with Zipf(1.1)-skewed
+    centres and k drawn from {1, 10, 50} per query in one persistent search
+    launch (resident buffers, GPU phase A), answers equal to the oracle's on
+    the same proxies, query by query."""
+    from hrnn_amd import hiprt
+    n, K, nq, m = 6000, 64, 100_000, 5
+    x = hrnn_amd.synth_mixture(5, n, 128, 200, 25.0, 0.0, 255.0, True, 0.0, 1005)
+    q = hrnn_amd.synth_mixture(9, nq, 128, 200, 25.0, 0.0, 255.0, True, 1.1, 1005)
+    kq = np.random.default_rng(9).choice([1, 10, 50], nq).astype(np.uint32)
+    ix = Index(n, 128, 16, 100, 100, K)
+    ix.add_points_gpu(x)
+    ix.refine(4, 16)
+    ix.build_rknng()
+    L = hrnn_amd.lib()
+    h = ix.device_handle()
+    cap = 1024
+    qd = hiprt.DeviceArray.from_host(q)
+    kd = hiprt.DeviceArray.from_host(kq)
+    pd = hiprt.DeviceArray((nq * m,), np.uint32)
+    cd = hiprt.DeviceArray((nq,), np.uint32)
+    idd = hiprt.DeviceArray((nq * cap,), np.uint32)
+    wd = hiprt.DeviceArray((2 * nq,), np.uint32)
+    assert L.hrnn_dev_rknn_query_dptr(h, qd.ptr, nq, m, 100, kd.ptr, 0, K, 1.0, pd.ptr, cd.ptr, idd.ptr, cap,
+                                      wd.ptr) == 0
+    assert L.hrnn_dev_sync(h) == 0
+    from hrnn_amd.dev import Device
+    dev = Device.borrow(h, n, 128, K)
+    off, ids = dev.fetch_results(nq, cd.ptr, idd.ptr, cap)
+    prox = pd.to_host().reshape(nq, m)
+    work = wd.to_host().reshape(nq, 2)
+    gids, gd, gc = ix.get_knng()
+    coff, cent, rb, wide = ix.get_rknng()
+    xi = x[ix.labels()]
+    eoff, eids = oracle.search_bc(coff, cent, wide, rb, gd, gc, K, xi, q, prox, 0, K, 1.0, k_per_q=kq)
+    assert np.array_equal(off, eoff) and np.array_equal(ids, eids)
+    assert len(ids) > nq
+    print("candidates per query: min", int(work[:, 1].min()), "median", int(np.median(work[:, 1])),
+          "max", int(work[:, 1].max()), flush=True)

@@ -738,3 +738,32 @@ def test_query_reads_pinned_host_queries(small_data):
         for a, b in zip(out[0], o):
             assert np.array_equal(a, b)
     assert out[0][2].size > 0
+
+
+@pytest.mark.parametrize("thr", [1.0, 2.0])
+def test_wide_csr_build_and_search(small_data, small_ref, thr):
+    """The u64-entry CSR (node << rank_bits | rank in 64-bit words, the layout
+    graphs past 32 bits of node and rank need), forced on a small graph by the
+    "csr_wide" key: the count / fill / per-list sort build gives the oracle's
+    entries as 64-bit values, and the search over it the oracle's answers, in
+    presence and count modes; the u32 layout returns after the key is reset."""
+    x, q = small_data
+    r, _ = small_ref
+    ids, dists, cnt = r.get_knng()
+    dev = hrnn_amd.Device(0)
+    dev.set_base(x)
+    dev.set_knng(ids, dists, cnt)
+    dev.set_tuning("csr_wide", 1)
+    try:
+        info = dev.build_rknn_csr()
+        off, ent = dev.get_rknn_csr()
+    finally:
+        dev.set_tuning("csr_wide", 0)
+    eoff, eent, erb, ewide = oracle.csr(ids, cnt, r.K)
+    assert info.wide == 1 and ewide == 0 and info.rank_bits == erb and ent.dtype == np.uint64
+    assert np.array_equal(off, eoff) and np.array_equal(ent, eent.astype(np.uint64))
+    prox = r.proxies(q, 10, 64)
+    goff, gids = dev.rknn_search(q, prox, 10, 48, thr)
+    xoff, xids = oracle.search_bc(eoff, eent, 0, erb, dists, cnt, r.K, x, q, prox, 10, 48, thr)
+    assert np.array_equal(goff, xoff) and np.array_equal(gids, xids) and len(gids) > 0
+    assert dev.build_rknn_csr().wide == 0
