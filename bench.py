@@ -240,8 +240,9 @@ def main() -> int:
     csr_info = dev.build_rknn_csr()
     dev.sync()
     csr_dev_s = time.perf_counter() - t
-    tuned_bf = autotune.tune_bruteforce(dev, x[:16384], k, "bf16", dict(n=n, d=d), use_cache=False)
-    log(f"autotune: bf_row_frags={tuned_bf['bf_row_frags']} {tuned_bf['timings_ms']}")
+    tuned_bf = autotune.tune_bruteforce(dev, x[:65536], k, "bf16", dict(n=n, d=d), use_cache=False)
+    log(f"autotune: bf_layout={tuned_bf['bf_layout']} bf_row_frags={tuned_bf['bf_row_frags']} "
+        f"{tuned_bf['timings_ms']}")
     t = time.time()
     kth = dev.bf_kth(k, "bf16")
     kth_s = time.time() - t
@@ -1846,7 +1847,7 @@ def run_c5(args, json_out) -> int:
     dev = Device.borrow(h, n, d, K)
 
     # ---- groundtruth: certified r_k (FP32 and FP64), FP64-judged RkNN for the held-out queries
-    tuned["bruteforce"] = autotune.tune_bruteforce(dev, x[:16384], k, "bf16", dict(n=n, d=d), use_cache=False)
+    tuned["bruteforce"] = autotune.tune_bruteforce(dev, x[:65536], k, "bf16", dict(n=n, d=d), use_cache=False)
     log(f"autotune: bruteforce {tuned['bruteforce']}")
     t = time.time()
     bfr = dev.bf_kth_ex(k, "bf16")

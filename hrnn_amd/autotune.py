@@ -12,8 +12,10 @@ changes (the GPU tests check this for each key).
   back to the HBM visited bitmap);
 * tune_beam (phase A): "beam_vis_shift" (the LDS visited set's size against
   the ef rule: occupancy against probe runs);
-* tune_bruteforce (the MFMA brute force): "bf_row_frags" (16-row fragments
-  per wave: base-tile reuse against blocks per CU);
+* tune_bruteforce (the MFMA brute force): "bf_layout" (waves split each
+  tile's rows, or its columns so that a staged tile is read from LDS once per
+  block) and "bf_row_frags" (16-row fragments per wave: base-tile reuse
+  against blocks per CU);
 * tune_nndescent (the NNDescent join): "nnd_join" (chunked BF16, whole-sample
   BF16 or whole-sample int8 staging of each node's sample in LDS), timed on
   the first iteration from the seed graph, which is restored afterwards;
@@ -36,6 +38,7 @@ CANDIDATES = {
     "search_wg_per_cu": [1, 2, 4, 8, 16, 32],
     "beam_vis_shift": [-1, 0, 1, 2],
     "bf_row_frags": [1, 2],
+    "bf_layout": [0, 1],
     "nnd_join": [1, 2, 3],
     "query_streams": [1, 2],
 }
@@ -190,32 +193,43 @@ def tune_beam(h, qd, nq: int, m: int, ef: int, pd, shape: dict, reps: int = 5,
 def tune_bruteforce(dev, q, k: int, prec: str = "bf16", shape: dict | None = None,
                     reps: int = 2, use_cache: bool = True) -> dict:
     """The MFMA brute force (k+1-NN of the rows q, e.g. a sample of the base,
-    over the whole base: the kernel and candidate width of the r_k pass):
-    "bf_row_frags".
-    Only 32-wide candidate lists (k + 1 <= 16) at d <= 256 have a second
-    shape; elsewhere the default stays and nothing is timed."""
+    over the whole base: the kernel and candidate width of the r_k pass; give
+    enough rows to fill the GPU several times over, e.g. 65,536, or the
+    shapes' occupancy does not show):
+    "bf_layout" (waves split a tile's rows or, for d <= 128, its columns)
+    and "bf_row_frags" (16-row fragments per wave in the row split; a second
+    one for 32-wide lists, k + 1 <= 16, at d <= 256).  Where neither has a
+    second shape the defaults stay and nothing is timed."""
     import time
     L = lib()
     h = dev.h
-    if not (k + 1 <= 16 and dev.d <= 256 and dev.d % 32 == 0):
-        return {"bf_row_frags": 1, "timings_ms": {}, "applies": False}
+    shapes = [(0, 1)]
+    if dev.d % 32 == 0 and dev.d <= 256 and k + 1 <= 16:
+        shapes.append((0, 2))
+    if dev.d % 32 == 0 and dev.d <= 128:
+        shapes.append((1, 1))
+    if len(shapes) == 1:
+        return {"bf_layout": 0, "bf_row_frags": 1, "timings_ms": {}, "applies": False}
     key = _key(dict(shape or {}, kernel="bruteforce", n=dev.n, d=dev.d, k=k, prec=prec, rows=len(q)))
     hit, cache = _cached(key, use_cache, L, h)
     if hit:
         return hit
     q = np.ascontiguousarray(q, np.float32)
-
-    def time_it(fn):
-        fn()
+    timings, best, best_ms = {}, shapes[0], float("inf")
+    for lay, fr in shapes:
+        ck_(L, h, L.hrnn_dev_set_tuning(h, b"bf_layout", lay))
+        ck_(L, h, L.hrnn_dev_set_tuning(h, b"bf_row_frags", fr))
+        dev.bf_knn(q, k + 1, prec)
         t0 = time.perf_counter()
         for _ in range(reps):
-            fn()
-        return (time.perf_counter() - t0) / reps * 1e3
-
-    timings = {}
-    out = {"bf_row_frags": _pick(L, h, "bf_row_frags", CANDIDATES["bf_row_frags"], time_it,
-                                 lambda: dev.bf_knn(q, k + 1, prec), timings),
-           "timings_ms": timings, "applies": True}
+            dev.bf_knn(q, k + 1, prec)
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        timings[f"bf_layout={lay},bf_row_frags={fr}"] = round(ms, 4)
+        if ms < best_ms:
+            best, best_ms = (lay, fr), ms
+    ck_(L, h, L.hrnn_dev_set_tuning(h, b"bf_layout", best[0]))
+    ck_(L, h, L.hrnn_dev_set_tuning(h, b"bf_row_frags", best[1]))
+    out = {"bf_layout": best[0], "bf_row_frags": best[1], "timings_ms": timings, "applies": True}
     if use_cache:
         cache[key] = out
         save_cache(cache)

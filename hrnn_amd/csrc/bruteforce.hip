@@ -484,6 +484,220 @@ __global__ __launch_bounds__(256) void k_bf_mfma(
     }
 }
 
+// Column-split form of k_bf_mfma for d <= 128 (KS <= 4): wave w takes the
+// columns 16 w .. 16 w + 15 of every 64-column tile for all 64 rows of the
+// block (the A fragments of all four 16-row blocks in registers), so each
+// staged base tile is read from LDS once per block instead of once per wave
+// (the row-split form's LDS reads of the B fragments are four times the
+// MFMAs' worth).  A row's candidates now come from all four waves: they are
+// staged for the whole tile (up to 64 per row), merged after a barrier by the
+// wave that owns the row (16 rows per wave, one lane each), and the
+// thresholds reloaded after the next barrier.  A threshold a lane holds is
+// never below the row's current one (it only falls), so a stale one stages
+// more candidates, never fewer, and the merge decides each by (distance, id)
+// against the list: the same lists as the row-split form.
+template <int C>
+struct TopStateCS {
+    float top_d[kRB][C];
+    uint32_t top_i[kRB][C];
+    uint32_t top_n[kRB];
+    float thr[kRB];
+    float st_d[kRB][kCT];
+    uint8_t st_o[kRB][kCT];   // staged column's offset in the tile
+    uint32_t st_n[kRB];
+};
+
+template <class T, int KS, int C, int MODE>
+__global__ __launch_bounds__(256) void k_bf_mfma_cs(
+    const T *__restrict__ qa, const float *__restrict__ qn, uint32_t nq,
+    const T *__restrict__ xb, const float *__restrict__ xn, uint64_t n, uint32_t d,
+    uint32_t *__restrict__ cand_i, uint32_t *__restrict__ cand_n, float *__restrict__ cand_thr,
+    float *__restrict__ raw_kth, uint32_t kraw,
+    const float *__restrict__ kth, uint32_t *__restrict__ hit_cnt, uint32_t *__restrict__ hit_i,
+    float *__restrict__ hit_d, uint32_t cap) {
+    static_assert(KS <= 4, "the column-split form stages a whole tile of d <= 128");
+    typedef typename Mfma<T>::vec vec;
+    __shared__ __attribute__((aligned(16))) TopStateCS<C> st;
+    constexpr int kBSP = KS * 32 + 8;   // bs row stride (elements)
+    __shared__ __attribute__((aligned(16))) T bs[kCT * kBSP];
+    __shared__ float s_xn[kCT];
+    __shared__ float s_qn[kRB];
+
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x / 64, gq = lane >> 4;
+    const uint64_t row0 = (uint64_t)blockIdx.x * kRB;
+    for (int r = threadIdx.x; r < kRB; r += blockDim.x) {
+        s_qn[r] = (row0 + r < nq) ? qn[row0 + r] : 0.f;
+        if (MODE == 0) {
+            st.top_n[r] = 0;
+            st.st_n[r] = 0;
+            st.thr[r] = FLT_MAX;
+        }
+    }
+    // A fragments of the four row blocks: rows 16 rb + lane % 16, dims
+    // ks * 32 + 8 * (lane / 16) .. + 8
+    vec afrag[4][KS];
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) {
+        const uint64_t row = row0 + 16 * rb + (lane & 15u);
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            if (row < nq) {
+                afrag[rb][ks] = *reinterpret_cast<const vec *>(qa + row * d + ks * 32 + 8 * gq);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; e++) afrag[rb][ks][e] = 0;
+            }
+        }
+    }
+    __syncthreads();
+    // this lane's output rows 16 rb + 4 (lane / 16) + i: norms and thresholds
+    float qn_r[4][4], thr_r[4][4];
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            qn_r[rb][i] = s_qn[16 * rb + 4 * gq + i];
+            thr_r[rb][i] = FLT_MAX;
+        }
+    // the whole next tile prefetched into registers (64 columns x d dims:
+    // NCH 64-dim chunks of two 16-byte loads per thread)
+    constexpr int NCH = (KS + 1) / 2;
+    vec pv[2 * NCH];
+    float pxn = 0.f;
+    auto fetch = [&](uint64_t c, int ch) {
+        const int dims = (KS - 2 * ch >= 2) ? 64 : 32;
+#pragma unroll
+        for (int v2 = 0; v2 < 2; v2++) {
+            const int u = threadIdx.x + v2 * 256;
+            const int col = u / (dims / 8), part = u % (dims / 8);
+            const uint64_t gc = c + col;
+            if (u < kCT * (dims / 8) && gc < n) {
+                pv[2 * ch + v2] = *reinterpret_cast<const vec *>(xb + gc * d + ch * 64 + part * 8);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; e++) pv[2 * ch + v2][e] = 0;
+            }
+        }
+    };
+    auto put = [&](int ch) {
+        const int dims = (KS - 2 * ch >= 2) ? 64 : 32;
+#pragma unroll
+        for (int v2 = 0; v2 < 2; v2++) {
+            const int u = threadIdx.x + v2 * 256;
+            if (u < kCT * (dims / 8)) {
+                const int col = u / (dims / 8), part = u % (dims / 8);
+                *reinterpret_cast<vec *>(&bs[col * kBSP + ch * 64 + part * 8]) = pv[2 * ch + v2];
+            }
+        }
+    };
+#pragma unroll
+    for (int ch = 0; ch < NCH; ch++) fetch(0, ch);
+    if (threadIdx.x < kCT) pxn = threadIdx.x < n ? xn[threadIdx.x] : 0.f;
+
+    const uint32_t cl = 16 * w + (lane & 15u);   // this lane's column in every tile
+    for (uint64_t c0 = 0; c0 < n; c0 += kCT) {
+        const uint64_t c1 = c0 + kCT;
+        __syncthreads();   // the last tile's MFMAs read bs; its merges set thr
+        if (threadIdx.x < kCT) s_xn[threadIdx.x] = pxn;
+#pragma unroll
+        for (int ch = 0; ch < NCH; ch++) {
+            put(ch);
+            if (c1 < n) fetch(c1, ch);
+        }
+        if (threadIdx.x < kCT && c1 < n) pxn = (c1 + threadIdx.x < n) ? xn[c1 + threadIdx.x] : 0.f;
+        if (MODE == 0 && c0 > 0) {
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++) {
+                const float4 t4 = *reinterpret_cast<const float4 *>(&st.thr[16 * rb + 4 * gq]);
+                thr_r[rb][0] = t4.x;
+                thr_r[rb][1] = t4.y;
+                thr_r[rb][2] = t4.z;
+                thr_r[rb][3] = t4.w;
+            }
+        }
+        __syncthreads();
+        f32x4_t acc[4];
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++) acc[rb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const vec b = *reinterpret_cast<const vec *>(&bs[cl * kBSP + ks * 32 + 8 * gq]);
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++) acc[rb] = Mfma<T>::run(afrag[rb][ks], b, acc[rb]);
+        }
+        const uint64_t col = c0 + cl;
+        const float xn_c = s_xn[cl];
+        const float kth_c = (MODE == 1 && col < n) ? kth[col] : 0.f;
+        if (col < n) {
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int rl = 16 * rb + 4 * (int)gq + i;
+                    if (row0 + rl < nq) {
+                        float dv = qn_r[rb][i] + xn_c - 2.f * acc[rb][i];
+                        dv = dv > 0.f ? dv : 0.f;
+                        if (MODE == 0) {
+                            if (dv < thr_r[rb][i]) {
+                                const uint32_t t = atomicAdd(&st.st_n[rl], 1u);
+                                st.st_d[rl][t] = dv;
+                                st.st_o[rl][t] = (uint8_t)cl;
+                            }
+                        } else if (dv <= kth_c) {
+                            const uint64_t q = row0 + rl;
+                            const uint32_t s = atomicAdd(&hit_cnt[q], 1u);
+                            if (s < cap) {
+                                hit_i[q * cap + s] = (uint32_t)col;
+                                hit_d[q * cap + s] = dv;
+                            }
+                        }
+                    }
+                }
+        }
+        if (MODE == 0) {
+            __syncthreads();   // every wave's candidates of this tile are staged
+            if (lane < 16) {   // wave w merges rows 16 w .. 16 w + 15
+                const int r = (int)(16 * w + lane);
+                const uint32_t ns = st.st_n[r];
+                if (ns) {
+                    uint32_t nn = st.top_n[r];
+                    for (uint32_t t = 0; t < ns; t++) {
+                        const float dv = st.st_d[r][t];
+                        const uint32_t iv = (uint32_t)(c0 + st.st_o[r][t]);
+                        if (nn == C && !pair_less(dv, iv, st.top_d[r][C - 1], st.top_i[r][C - 1])) continue;
+                        uint32_t p = (nn == C) ? C - 1 : nn;
+                        if (nn < C) nn++;
+                        while (p > 0 && pair_less(dv, iv, st.top_d[r][p - 1], st.top_i[r][p - 1])) {
+                            st.top_d[r][p] = st.top_d[r][p - 1];
+                            st.top_i[r][p] = st.top_i[r][p - 1];
+                            p--;
+                        }
+                        st.top_d[r][p] = dv;
+                        st.top_i[r][p] = iv;
+                    }
+                    st.top_n[r] = nn;
+                    st.st_n[r] = 0;
+                    st.thr[r] = (nn == C) ? st.top_d[r][C - 1] : FLT_MAX;
+                }
+            }
+        }
+    }
+    if (MODE == 0) {
+        __syncthreads();   // every row's merges are done
+        for (int t = threadIdx.x; t < kRB * C; t += blockDim.x) {
+            const int r = t / C, j = t % C;
+            const uint64_t row = row0 + r;
+            if (row >= nq) continue;
+            cand_i[row * C + j] = (uint32_t)j < st.top_n[r] ? st.top_i[r][j] : kInvalidId;
+            if (j == 0) {
+                cand_n[row] = st.top_n[r];
+                cand_thr[row] = st.thr[r];
+                if (kraw) raw_kth[row] = kraw <= st.top_n[r] ? st.top_d[r][kraw - 1] : FLT_MAX;
+            }
+        }
+    }
+}
+
 // FP32 VALU candidate pass: 4x4 micro-tile per lane, direct differences.
 // mode 0: top-C; mode 1: RkNN hits.  In mode 1 a pair whose distance lies
 // within band_rel * max(d2, r_k) of r_k (a bound on the FP32 rounding of
@@ -789,9 +1003,18 @@ static void launch_mfma(hrnn_dev_t h, const T *qa, const float *qn, uint32_t nq,
     // CU: 1.11 s); two where the autotuner finds them faster, for 32-wide
     // candidate lists and d <= 256 (wider rows or lists would not fit).
     const bool fr2 = h->tune_bf_frags == 2;
+    const bool cs = h->tune_bf_layout == 1;
     uint32_t KS = h->d / 32;
 #define L(ks)                                                                                     \
     case ks:                                                                                      \
+        if constexpr (ks <= 4) {                                                                  \
+            if (cs) {                                                                             \
+                hipLaunchKernelGGL((k_bf_mfma_cs<T, ks, C, MODE>), dim3(ceil_div(nq, 64)), dim3(256), 0, \
+                                   h->stream, qa, qn, nq, xb, xn, h->n, h->d, ci, cn, cthr, raw_kth, kraw, \
+                                   kth, hc, hi, hd, cap);                                         \
+                break;                                                                            \
+            }                                                                                     \
+        }                                                                                         \
         if constexpr (C == 32 && ks <= 8) {                                                       \
             if (fr2) {                                                                            \
                 hipLaunchKernelGGL((k_bf_mfma<T, ks, C, MODE, 2>), dim3(ceil_div(nq, 128)), dim3(256), 0, \

@@ -48,6 +48,7 @@ struct hrnn_dev {
     int tune_beam_wg_per_cu = 16;    // persistent beam workgroups (one wave each) per CU
     int tune_beam_vis_shift = 0;     // phase A visited-set slots: log2 shift from the ef rule (-1 .. 2)
     int tune_bf_frags = 1;           // brute force: 16-row fragments per wave (2: 32-wide lists, d <= 256)
+    int tune_bf_layout = 0;          // brute force: waves split a tile's rows (0) or its columns (1, d <= 128)
     int tune_query_streams = 1;      // resident query batches: halves on two streams (2) or one (1)
     int tune_csr_wide = 0;           // reverse CSR: u64 entries even where u32 would do (1)
     int tune_nnd_join = 0;           // NNDescent join staging (0: chosen from d and data; 1 chunked BF16,

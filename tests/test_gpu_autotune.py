@@ -19,6 +19,7 @@ def _sphere(n, d, seed):
 
 def test_tuning_keys_ranges(gpu_dev):
     for key, good, bad in (("beam_vis_shift", (-1, 0, 2), (-2, 3)), ("bf_row_frags", (1, 2), (0, 3)),
+                           ("bf_layout", (0, 1), (-1, 2)),
                            ("nnd_join", (0, 1, 3), (-1, 4)), ("query_streams", (1, 2), (0, 3))):
         for v in good:
             gpu_dev.set_tuning(key, v)
@@ -30,6 +31,7 @@ def test_tuning_keys_ranges(gpu_dev):
     gpu_dev.set_tuning("bf_row_frags", 1)
     gpu_dev.set_tuning("nnd_join", 0)
     gpu_dev.set_tuning("query_streams", 1)
+    gpu_dev.set_tuning("bf_layout", 0)
 
 
 @pytest.fixture(scope="module")
@@ -81,6 +83,36 @@ def test_bf_row_frags_same_results(gpu_dev, kind, prec):
             res.append((ids, d2, kth, kids))
     finally:
         gpu_dev.set_tuning("bf_row_frags", 1)
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("kind", ["int", "sphere"])
+@pytest.mark.parametrize("prec", ["bf16", "f16"])
+def test_bf_layout_same_results(gpu_dev, kind, prec):
+    """The MFMA brute force with waves splitting each tile's rows (0) or its
+    columns (1): the same k-NN ids and distances with 32- and 64-wide
+    candidate lists (k + 1 = 11 and 20), the same certified r_k, and, on
+    exact data, the same RkNN hits."""
+    if kind == "int":
+        x = hrnn_amd.synth_mixture(17, 5000, 128, 40, 20.0, 0.0, 255.0, True, 0.0, 1017)
+    else:
+        x = _sphere(5000, 96, 17)
+    q = x[:900] + np.float32(0.25)
+    gpu_dev.set_base(x)
+    res = []
+    try:
+        for lay in (0, 1):
+            gpu_dev.set_tuning("bf_layout", lay)
+            r = [*gpu_dev.bf_knn(q, 11, prec), *gpu_dev.bf_knn(q, 20, prec)]
+            kth, kids = gpu_dev.bf_kth(10, prec, with_ids=True)
+            r += [kth, kids]
+            if kind == "int":
+                off, ids, d2 = gpu_dev.bf_rknn(q, kth, prec)
+                r += [off, ids, d2]
+            res.append(r)
+    finally:
+        gpu_dev.set_tuning("bf_layout", 0)
     for a, b in zip(res[0], res[1]):
         assert np.array_equal(a, b)
 
@@ -173,7 +205,7 @@ def test_tuners_run_and_restore(sphere_index):
     assert r["beam_vis_shift"] in autotune.CANDIDATES["beam_vis_shift"]
     assert dev.get_tuning("beam_vis_shift") == r["beam_vis_shift"]
     r = autotune.tune_bruteforce(dev, x[:2048], 10, "bf16", {"test": 1}, reps=1, use_cache=False)
-    assert r["applies"] and r["bf_row_frags"] in (1, 2) and len(r["timings_ms"]) == 2
+    assert r["applies"] and r["bf_row_frags"] in (1, 2) and r["bf_layout"] in (0, 1) and len(r["timings_ms"]) == 3
     seed = dev.get_knng()
     flags = dev.get_knng_flags()
     r = autotune.tune_nndescent(dev, 8, {"test": 1}, use_cache=False)
@@ -192,5 +224,5 @@ def test_tuners_run_and_restore(sphere_index):
     r = autotune.tune_query(h, qd2, 4096, m, ef, 10, 32, pd2, cd2, idd2, 1024, {"test": 1}, reps=2,
                             use_cache=False)
     assert r["query_streams"] in (1, 2) and len(r["timings_ms"]) == 2
-    for key in ("beam_vis_shift", "bf_row_frags", "nnd_join", "query_streams"):
+    for key in ("beam_vis_shift", "bf_row_frags", "nnd_join", "query_streams", "bf_layout"):
         dev.set_tuning(key, 0 if key not in ("bf_row_frags", "query_streams") else 1)
