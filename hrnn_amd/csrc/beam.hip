@@ -92,6 +92,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         for (uint32_t w = lane; 4 * w < d; w += 64) {
             const float4 v = reinterpret_cast<const float4 *>(qrow)[w];
             reinterpret_cast<float4 *>(s_qv)[w] = v;
+            if (opt.q_copy) reinterpret_cast<float4 *>(opt.q_copy + (uint64_t)qi * d)[w] = v;
             q_u8 &= is_u8_value(v.x) && is_u8_value(v.y) && is_u8_value(v.z) && is_u8_value(v.w);
             if constexpr (std::is_same<VT, uint8_t>::value) {
                 const uint32_t b0 = clamp_u8(v.x), b1 = clamp_u8(v.y), b2 = clamp_u8(v.z), b3 = clamp_u8(v.w);
@@ -103,6 +104,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         for (uint32_t e = lane; e < d; e += 64) {
             const float v = qrow[e];
             s_qv[e] = v;
+            if (opt.q_copy) opt.q_copy[(uint64_t)qi * d + e] = v;
             q_u8 &= is_u8_value(v);
         }
         if constexpr (std::is_same<VT, uint8_t>::value) {
@@ -218,19 +220,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     uint32_t pick = 0;
     // layer-0 row of the node to be expanded next, loaded while the current
     // expansion merges its candidates
-    uint32_t pf_node = kInvalidId, pf_cnt = 0, pf_nb = 0;
+    uint32_t pf_node = kInvalidId, pf_w = 0, pf_nb = 0;
     const uint32_t max_exp = opt.mlp && opt.max_exp ? opt.max_exp : 0xFFFFFFFFu;
     for (uint32_t iter = 0; iter <= n && pick != kInvalidId && n_exp < max_exp; iter++) {   // one expansion per pass
         const unsigned long long *lk = s_kl + buf * ef_cap;
         uint8_t *lx = s_xl + buf * ef_cap;
         const uint32_t c = uni((uint32_t)lk[pick]);
-        uint32_t cnt, nbw;
+        uint32_t w, nbw;
         if (c == pf_node) {
-            cnt = pf_cnt;
+            w = pf_w;
             nbw = pf_nb;
         } else {
-            l0_row(l0, l0_words, c, lane, &cnt, &nbw);
+            l0_row_issue(l0, l0_words, c, lane, &w, &nbw);
         }
+        const uint32_t cnt = l0_count(w, l0_words);
         __syncthreads();
         if (lane == 0) lx[pick] = 1;
         n_exp++;
@@ -319,7 +322,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
         const unsigned long long kbest = read_lane64(key, 0);
         const unsigned long long ko = (o != kInvalidId) ? uni64(lk[o]) : ~0ull;
         pf_node = (uint32_t)(kbest < ko ? kbest : ko);
-        l0_row(l0, l0_words, pf_node, lane, &pf_cnt, &pf_nb);
+        l0_row_issue(l0, l0_words, pf_node, lane, &pf_w, &pf_nb);
         // merge the sorted candidates into the sorted list, keeping the first
         // ef: entry e moves up by the candidates below it; candidate j lands
         // after the entries below it
@@ -485,6 +488,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
 // tuning key moves the rule's choice by a power of two, within 2^10 .. 2^13.
 static uint32_t beam_log_vis(uint32_t ef) { return ef <= 16 ? 10u : ef <= 256 ? 11u : 12u; }
 
+bool host_memory(const void *p) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
 void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint32_t ef,
                  uint32_t *proxies_dev, uint32_t *expansions_dev, float *raw_dev, const BeamOpt *opt) {
     REQUIRE(h->hnsw_valid, HRNN_E_STATE, "proxies: upload the HNSW graph first");
@@ -497,6 +509,7 @@ void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint
     REQUIRE(h->d >= 1 && h->d <= 4096, HRNN_E_ARG, "proxies: d must be in [1, 4096]");
     REQUIRE(q_dev && proxies_dev, HRNN_E_ARG, "proxies: null argument");
     BeamOpt o = opt ? *opt : BeamOpt{};
+    REQUIRE(!o.q_copy || (!o.ef_q && !o.qlist), HRNN_E_ARG, "proxies: the query copy needs every query");
     REQUIRE(!o.mlp || (o.classes && o.cls && o.ef_q_out && o.m_q_out && o.lists && o.list_cnt && o.nq == nq &&
                        o.mu && o.sd),
             HRNN_E_ARG, "proxies: incomplete learned-stop arguments");
@@ -640,9 +653,18 @@ int hrnn_dev_rknn_query_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint
         REQUIRE(k_per_q_dev || (k >= 1 && k <= h->K), HRNN_E_ARG, "rknn_query: k out of [1, K]");
         if (nq == 0) return;
         const uint32_t efm = std::max(ef, m);
+        // Queries in page-locked host memory are read over the link once: the
+        // beam writes each row to a device copy, which the search reads.
+        BeamOpt o1;
+        const float *qs = q_dev;
+        if (host_memory(q_dev)) {
+            h->q_stage.reserve((uint64_t)nq * h->d * 4);
+            o1.q_copy = h->q_stage.as<float>();
+            qs = o1.q_copy;
+        }
         if (h->tune_query_streams != 2 || nq < 2048) {
-            beam_launch(h, q_dev, nq, m, efm, proxies_dev, nullptr);
-            rknn_search_resident(h, q_dev, nq, proxies_dev, m, k_per_q_dev, k, K_prime, threshold,
+            beam_launch(h, q_dev, nq, m, efm, proxies_dev, nullptr, nullptr, &o1);
+            rknn_search_resident(h, qs, nq, proxies_dev, m, k_per_q_dev, k, K_prime, threshold,
                                  res_cnt_dev, res_ids_dev, cap_per_q, work_dev);
             return;
         }
@@ -656,17 +678,18 @@ int hrnn_dev_rknn_query_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint
         HIPC(hipStreamWaitEvent(h->stream2, h->ev_a, 0));
         const uint32_t n0 = nq / 2, n1 = nq - n0;
         const uint64_t d = h->d;
-        beam_launch(h, q_dev, n0, m, efm, proxies_dev, nullptr);
+        beam_launch(h, q_dev, n0, m, efm, proxies_dev, nullptr, nullptr, &o1);
         BeamOpt o2;
         o2.stream = h->stream2;
+        if (o1.q_copy) o2.q_copy = o1.q_copy + n0 * d;
         beam_launch(h, q_dev + n0 * d, n1, m, efm, proxies_dev + (uint64_t)n0 * m, nullptr, nullptr, &o2);
-        rknn_search_resident(h, q_dev, n0, proxies_dev, m, k_per_q_dev, k, K_prime, threshold, res_cnt_dev,
+        rknn_search_resident(h, qs, n0, proxies_dev, m, k_per_q_dev, k, K_prime, threshold, res_cnt_dev,
                              res_ids_dev, cap_per_q, work_dev);
         SearchSel s2;
         s2.stream = h->stream2;
         s2.queue_word = kQueue2Word;
         s2.buffer_set = 1;
-        rknn_search_resident(h, q_dev + n0 * d, n1, proxies_dev + (uint64_t)n0 * m,
+        rknn_search_resident(h, qs + n0 * d, n1, proxies_dev + (uint64_t)n0 * m,
                              m, k_per_q_dev ? k_per_q_dev + n0 : nullptr, k, K_prime, threshold,
                              res_cnt_dev + n0, res_ids_dev + (uint64_t)n0 * cap_per_q, cap_per_q,
                              work_dev ? work_dev + 2ull * n0 : nullptr, &s2);

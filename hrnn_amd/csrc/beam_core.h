@@ -95,14 +95,20 @@ __device__ __forceinline__ unsigned long long read_lane64(unsigned long long v, 
     return ((unsigned long long)hi << 32) | lo;
 }
 
-// Layer-0 row of node c: the neighbour count (uniform) and, in lane j, the
-// j-th neighbour id.  Both loads are issued before either is used.
-__device__ __forceinline__ void l0_row(const uint32_t *__restrict__ l0, uint32_t l0_words, uint32_t c,
-                                       uint32_t lane, uint32_t *cnt, uint32_t *nb) {
+// Layer-0 row of node c, issued as vector loads that nothing waits on here:
+// lane j receives word j of the row in *w (lane 0: the count word) and
+// neighbour j in *nb.  l0_count() reads the count where it is used, so a row
+// loaded ahead of its use does not stall the work in between (a uniform read
+// of the count word would be a scalar load, whose wait also holds every LDS
+// access behind it).
+__device__ __forceinline__ void l0_row_issue(const uint32_t *__restrict__ l0, uint32_t l0_words, uint32_t c,
+                                             uint32_t lane, uint32_t *w, uint32_t *nb) {
     const uint32_t *ll = l0 + (uint64_t)c * l0_words;
-    const uint32_t w0 = ll[0];
+    *w = (lane < l0_words) ? ll[lane] : 0u;
     *nb = (lane + 1u < l0_words) ? ll[1 + lane] : 0u;
-    *cnt = uni(min(w0 & 0xFFFFu, l0_words - 1u));
+}
+__device__ __forceinline__ uint32_t l0_count(uint32_t w, uint32_t l0_words) {
+    return min((uint32_t)__builtin_amdgcn_readlane((int)w, 0) & 0xFFFFu, l0_words - 1u);
 }
 
 // LDS layout of a beam workgroup (dynamic part), in bytes: query vector, the query

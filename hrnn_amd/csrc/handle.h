@@ -66,6 +66,7 @@ struct hrnn_dev {
     // r_k column of the graph for one k (verification reads one float per
     // candidate from a 4n-byte array instead of a line of the n x K matrix)
     hrnn::DevBuf rk_col;
+    hrnn::DevBuf q_stage;   // device copy of a query batch read from page-locked host memory
     uint32_t rk_col_k = 0;
     uint64_t rk_col_version = 0, rk_col_n = 0;
 
@@ -239,11 +240,16 @@ struct BeamOpt {
     // stopped early whose class keeps it at stage 1 takes the first class with
     // a wider beam instead
     uint32_t max_exp = 0;
+    // the query rows written here as the beam reads them (nq x d floats, the
+    // layout of q), so the phases that follow read device memory
+    float *q_copy = nullptr;
 };
 // The r_k column for k (n floats, rk[i] = k-th distance of i's list) built on
 // stream st unless it is current; searches that run concurrently on several
 // streams call this first, on the stream they all follow.
 void rk_column_ready(hrnn_dev_t h, uint32_t k, hipStream_t st);
+// True when p points into page-locked host memory (mapped for the device).
+bool host_memory(const void *p);
 // The handle's second stream and its two events (created on first use).
 void ensure_stream2(hrnn_dev_t h);
 void beam_launch(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m, uint32_t ef,

@@ -288,6 +288,14 @@ int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
             o1.zero_b = h->counters.as<uint32_t>() + kQueue2Word;
             o1.max_exp = (uint32_t)h->tune_stage1_cap;
         }
+        // queries in page-locked host memory: read over the link once, by
+        // stage 1, which leaves a device copy for everything after it
+        const float *qs = q_dev;
+        if (host_memory(q_dev)) {
+            h->q_stage.reserve((uint64_t)nq * h->d * 4);
+            o1.q_copy = h->q_stage.as<float>();
+            qs = o1.q_copy;
+        }
         beam_launch(h, q_dev, nq, m_max, ef1, proxies_dev, exp1, nullptr, &o1);
         if (!model_dev)   // forced classes
             hipLaunchKernelGGL(k_pick_class, dim3(ceil_div(nq, 128)), dim3(128), 0, h->stream, nullptr, nq,
@@ -307,7 +315,7 @@ int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
             rest.stream = h->stream2;
             rest.queue_word = kQueue2Word;
             rest.queue_zeroed = true;
-            rknn_search_resident(h, q_dev, nq, proxies_dev, m_max, nullptr, k, K_prime, threshold, res_cnt_dev,
+            rknn_search_resident(h, qs, nq, proxies_dev, m_max, nullptr, k, K_prime, threshold, res_cnt_dev,
                                  res_ids_dev, cap_per_q, work_dev, &rest);
             HIPC(hipEventRecord(h->ev_b, h->stream2));
         }
@@ -317,7 +325,7 @@ int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
             o2.m_q = m_q;
             o2.qlist = lists;
             o2.qcount = list_cnt;
-            beam_launch(h, q_dev, nq, m_max, ef_max, proxies_dev, exp2, nullptr, &o2);
+            beam_launch(h, qs, nq, m_max, ef_max, proxies_dev, exp2, nullptr, &o2);
         }
         // the escalated queries' search overlaps the other one too: its own
         // fallback buffers and a few workgroups
@@ -328,7 +336,7 @@ int hrnn_dev_rknn_query_adaptive_dptr(hrnn_dev_t h, const float *q_dev, uint32_t
         esc.queue_zeroed = true;
         esc.buffer_set = 1;
         esc.max_slots = (uint32_t)std::max(1, h->n_cu);
-        rknn_search_resident(h, q_dev, nq, proxies_dev, m_max, nullptr, k, K_prime, threshold, res_cnt_dev,
+        rknn_search_resident(h, qs, nq, proxies_dev, m_max, nullptr, k, K_prime, threshold, res_cnt_dev,
                              res_ids_dev, cap_per_q, work_dev, &esc);
         HIPC(hipStreamWaitEvent(h->stream, h->ev_b, 0));
     });

@@ -534,18 +534,25 @@ class Device:
 
     def query_adaptive(self, q, classes, ef1: int, m_max: int, k: int, K_prime: int, model=None,
                        tau: float = 0.0, cls=None, threshold: float = 1.0, cap_per_q: int = 4096,
-                       phase_a_only: bool = False):
+                       phase_a_only: bool = False, pinned: bool = False):
         """Two-stage learned query (hrnn_dev_rknn_query_adaptive_dptr).  classes:
         [16, 2] (ef, m) rows.  With model None, cls gives each query's class.
         Returns (res_off[nq+1], res_ids, cls[nq], proxies[nq, m_max]); with
         phase_a_only (no result buffers: phases B and C are skipped) res_off
-        and res_ids are None."""
+        and res_ids are None.  pinned: the queries are passed in page-locked
+        host memory (read over the link by the kernels) instead of HBM."""
         from . import hiprt
         q = np.ascontiguousarray(q, np.float32)
         nq = q.shape[0]
         classes = np.asarray(classes, np.uint32).reshape(16, 2)
         cd = hiprt.DeviceArray.from_host(np.concatenate([classes[:, 0], classes[:, 1]]))
-        qd = hiprt.DeviceArray.from_host(q)
+        if pinned:
+            qd = hiprt.PinnedArray(q.shape, np.float32)
+            qd.a[:] = q
+            q_ptr = qd.device_ptr()
+        else:
+            qd = hiprt.DeviceArray.from_host(q)
+            q_ptr = qd.ptr
         md = None if model is None else hiprt.DeviceArray.from_host(np.asarray(model, np.float32))
         cl = hiprt.DeviceArray.from_host(np.zeros(nq, np.uint32) if cls is None
                                          else np.asarray(cls, np.uint32))
@@ -553,7 +560,7 @@ class Device:
         rc = hiprt.DeviceArray((nq,), np.uint32)
         ri = hiprt.DeviceArray((nq, cap_per_q), np.uint32)
         self._ck(self.L.hrnn_dev_rknn_query_adaptive_dptr(
-            self.h, qd.ptr, nq, None if md is None else md.ptr, cd.ptr, int(classes[:, 0].max()),
+            self.h, q_ptr, nq, None if md is None else md.ptr, cd.ptr, int(classes[:, 0].max()),
             tau, ef1, m_max, k, K_prime, threshold, pd.ptr, cl.ptr, None if phase_a_only else rc.ptr,
             None if phase_a_only else ri.ptr, cap_per_q, None, None))
         self.sync()

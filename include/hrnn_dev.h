@@ -355,7 +355,10 @@ int hrnn_dev_proxies_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_
  * q_dev (here, in hrnn_dev_rknn_query_adaptive_dptr and in the other _dptr
  * calls) may also be the device address of page-locked host memory
  * (hipHostGetDevicePointer): the kernels then read the queries over the link
- * in place, with no copy before the call. */
+ * in place, with no copy before the call.  Here and in the adaptive call the
+ * first phase-A launch reads each query once and writes it to a device
+ * buffer of the handle (nq*d floats, kept for later calls), which the rest
+ * of the call reads. */
 int hrnn_dev_rknn_query_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint32_t m,
                              uint32_t ef, const uint32_t *k_per_q_dev, uint32_t k,
                              uint32_t K_prime, float threshold, uint32_t *proxies_dev,

@@ -64,6 +64,11 @@ def test_adaptive_forced_classes_equal_per_class_runs(c1_index):
     # phase A alone (no result buffers) leaves the same classes and proxies
     _, _, cls_a, prox_a = dev.query_adaptive(q, CLASSES, EF1, 16, 10, 100, cls=cls, phase_a_only=True)
     assert np.array_equal(cls_a, cls) and np.array_equal(prox_a, prox)
+    # queries in page-locked host memory (stage 1 reads them over the link and
+    # leaves the device copy the later stages read): the same answers
+    off_p, ids_p, cls_p, prox_p = dev.query_adaptive(q, CLASSES, EF1, 16, 10, 100, cls=cls, pinned=True)
+    assert np.array_equal(cls_p, cls) and np.array_equal(prox_p, prox)
+    assert np.array_equal(off_p, off) and np.array_equal(ids_p, ids)
     for c in range(16):
         sel = np.nonzero(cls == c)[0]
         if sel.size == 0:

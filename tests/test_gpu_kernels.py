@@ -702,8 +702,10 @@ def test_fetch_results_into_pinned_memory(small_data, small_ref):
 
 def test_query_reads_pinned_host_queries(small_data):
     """hrnn_dev_rknn_query_dptr with the queries in page-locked host memory
-    (the kernels read them over the link in place): the same proxies and
-    answers as from device memory, on one stream and on two."""
+    (phase A reads them over the link and leaves a device copy that the
+    search reads): the same proxies and answers as from device memory, on one
+    stream and on two, and again after the pinned batch is overwritten with
+    other queries (the device copy is rewritten)."""
     from hrnn_amd import hiprt
     from hrnn_amd.dev import Device
     from hrnn_amd.index import Index
@@ -732,12 +734,27 @@ def test_query_reads_pinned_host_queries(small_data):
                                                       idd.ptr, cap, None) == 0
                 assert dev.L.hrnn_dev_sync(h) == 0   # the library's streams, before the plain copy below
                 out.append((pd.to_host(),) + dev.fetch_results(nq, cd.ptr, idd.ptr, cap))
+        q2 = np.ascontiguousarray(q[::-1] + np.float32(0.5))
+        q_pin.a[:] = q2
+        qd2 = hiprt.DeviceArray.from_host(q2)
+        again = []
+        for qp in (q_pin.device_ptr(), qd2.ptr):
+            pd = hiprt.DeviceArray((nq * m,), np.uint32)
+            cd = hiprt.DeviceArray((nq,), np.uint32)
+            idd = hiprt.DeviceArray((nq * cap,), np.uint32)
+            assert dev.L.hrnn_dev_rknn_query_dptr(h, qp, nq, m, ef, None, 10, K, 1.0, pd.ptr, cd.ptr,
+                                                  idd.ptr, cap, None) == 0
+            assert dev.L.hrnn_dev_sync(h) == 0
+            again.append((pd.to_host(),) + dev.fetch_results(nq, cd.ptr, idd.ptr, cap))
     finally:
         dev.set_tuning("query_streams", 1)
     for o in out[1:]:
         for a, b in zip(out[0], o):
             assert np.array_equal(a, b)
     assert out[0][2].size > 0
+    for a, b in zip(again[0], again[1]):
+        assert np.array_equal(a, b)
+    assert not np.array_equal(again[0][0], out[0][0])
 
 
 @pytest.mark.parametrize("thr", [1.0, 2.0])
