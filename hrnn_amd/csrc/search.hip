@@ -559,14 +559,18 @@ void noalloc_scratch_init(hrnn_dev_t h) {
 // min(cnt[q], cap)).  k_results_scan forms off[q] = sum over j < q of
 // min(cnt[j], cap) (u64), off[nq] = the total and off[nq + 1] = the largest
 // true count, in one workgroup (8 consecutive counts per lane, a wave scan,
-// per-wave totals in LDS, a running carry between chunks of 8,192 queries);
+// per-wave totals in LDS, a running carry between chunks of 8,192 queries),
+// and, given host_off / host_max (device addresses of page-locked host
+// memory), writes off[0..nq] and the largest count there too;
 // k_results_gather then packs the ids densely, one wave per query.
 constexpr int kResScanThreads = 1024;
 constexpr uint32_t kResScanPer = 8;
 
 __global__ __launch_bounds__(kResScanThreads) void k_results_scan(const uint32_t *__restrict__ cnt,
                                                                    uint32_t nq, uint32_t cap,
-                                                                   uint64_t *__restrict__ off) {
+                                                                   uint64_t *__restrict__ off,
+                                                                   uint64_t *__restrict__ host_off,
+                                                                   uint64_t *__restrict__ host_max) {
     __shared__ unsigned long long s_w[kResScanThreads / 64];
     __shared__ unsigned long long s_carry;
     __shared__ uint32_t s_max;
@@ -603,7 +607,10 @@ __global__ __launch_bounds__(kResScanThreads) void k_results_scan(const uint32_t
 #pragma unroll
         for (uint32_t j = 0; j < kResScanPer; j++) {
             const uint32_t i = i0 + j;
-            if (i < nq) off[i] = run;
+            if (i < nq) {
+                off[i] = run;
+                if (host_off) host_off[i] = run;
+            }
             run += v[j];
         }
         __syncthreads();   // every lane has read s_carry and s_w
@@ -615,6 +622,8 @@ __global__ __launch_bounds__(kResScanThreads) void k_results_scan(const uint32_t
     if (threadIdx.x == 0) {
         off[nq] = s_carry;
         off[nq + 1] = s_max;
+        if (host_off) host_off[nq] = s_carry;
+        if (host_max) *host_max = s_max;
     }
 }
 
@@ -690,11 +699,17 @@ int hrnn_dev_fetch_results(hrnn_dev_t h, uint32_t nq, const uint32_t *res_cnt_de
         if (!h->pin_small) HIPC(hipHostMalloc(&h->pin_small, 64, hipHostMallocDefault));
         uint64_t *off = h->fetch_off.as<uint64_t>();
         uint64_t *small = static_cast<uint64_t *>(h->pin_small);
+        // Page-locked offsets: the scan writes them (and the largest count)
+        // straight into host memory as well, with no copies after it.
+        uint64_t *off_direct = static_cast<uint64_t *>(pinned_device_ptr(res_off));
+        uint64_t *small_direct = off_direct ? static_cast<uint64_t *>(pinned_device_ptr(small)) : nullptr;
         hipLaunchKernelGGL(k_results_scan, dim3(1), dim3(kResScanThreads), 0, h->stream, res_cnt_dev, nq,
-                           cap_per_q, off);
+                           cap_per_q, off, small_direct ? off_direct : nullptr, small_direct);
         HIPC(hipGetLastError());
-        HIPC(hipMemcpyAsync(res_off, off, ((uint64_t)nq + 1) * 8, hipMemcpyDeviceToHost, h->stream));
-        HIPC(hipMemcpyAsync(small, off + nq + 1, 8, hipMemcpyDeviceToHost, h->stream));
+        if (!small_direct) {
+            HIPC(hipMemcpyAsync(res_off, off, ((uint64_t)nq + 1) * 8, hipMemcpyDeviceToHost, h->stream));
+            HIPC(hipMemcpyAsync(small, off + nq + 1, 8, hipMemcpyDeviceToHost, h->stream));
+        }
         // Page-locked id buffer: the gather writes the packed ids straight
         // into it (at most cap of them), so one wait covers the whole fetch.
         uint32_t *ids_direct = res_ids && cap ? static_cast<uint32_t *>(pinned_device_ptr(res_ids)) : nullptr;
