@@ -20,7 +20,10 @@ changes (the GPU tests check this for each key).
   BF16 or whole-sample int8 staging of each node's sample in LDS), timed on
   the first iteration from the seed graph, which is restored afterwards;
 * tune_query (a resident query batch, phase A then phases B+C):
-  "query_streams" (the batch's two halves on one stream or on two).
+  "query_streams" (the batch on one stream, or split in two parts on two
+  streams) and "query_split" (the first part's share, permille: an uneven
+  split lets the smaller part's phases B+C run beside the larger part's
+  phase A).
 """
 from __future__ import annotations
 
@@ -41,6 +44,7 @@ CANDIDATES = {
     "bf_layout": [0, 1],
     "nnd_join": [1, 2, 3],
     "query_streams": [1, 2],
+    "query_split": [500, 450, 400, 350, 300],
 }
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "cache.json")
 
@@ -278,7 +282,8 @@ def tune_nndescent(dev, sample: int, shape: dict | None = None, use_cache: bool 
 def tune_query(h, qd, nq: int, m: int, ef: int, k: int, K_prime: int, pd, cd, idd, cap: int, shape: dict,
                reps: int = 5, use_cache: bool = True) -> dict:
     """A resident batch through hrnn_dev_rknn_query_dptr (phase A at this m
-    and ef, then phases B+C): "query_streams"."""
+    and ef, then phases B+C): "query_streams" and, with two streams,
+    "query_split"."""
     L = lib()
     key = _key(dict(shape, kernel="query", m=m, ef=ef, k=k, Kp=K_prime, nq=nq))
     hit, cache = _cached(key, use_cache, L, h)
@@ -287,9 +292,18 @@ def tune_query(h, qd, nq: int, m: int, ef: int, k: int, K_prime: int, pd, cd, id
     time_it = _event_timer(L, h, reps)
     fn = lambda: ck_(L, h, L.hrnn_dev_rknn_query_dptr(h, qd.ptr, nq, m, ef, None, k, K_prime, 1.0, pd.ptr,
                                                       cd.ptr, idd.ptr, cap, None))
-    timings = {}
-    out = {"query_streams": _pick(L, h, "query_streams", CANDIDATES["query_streams"], time_it, fn, timings),
-           "timings_ms": timings}
+    shapes = [(1, 500)] + [(2, p) for p in CANDIDATES["query_split"]]
+    timings, best, best_ms = {}, shapes[0], float("inf")
+    for st, sp in shapes:
+        ck_(L, h, L.hrnn_dev_set_tuning(h, b"query_streams", st))
+        ck_(L, h, L.hrnn_dev_set_tuning(h, b"query_split", sp))
+        ms = time_it(fn)
+        timings[f"query_streams={st},query_split={sp}"] = round(ms, 4)
+        if ms < best_ms:
+            best, best_ms = (st, sp), ms
+    ck_(L, h, L.hrnn_dev_set_tuning(h, b"query_streams", best[0]))
+    ck_(L, h, L.hrnn_dev_set_tuning(h, b"query_split", best[1]))
+    out = {"query_streams": best[0], "query_split": best[1], "timings_ms": timings}
     if use_cache:
         cache[key] = out
         save_cache(cache)

@@ -371,6 +371,7 @@ static int *tuning_slot(hrnn_dev_t h, const char *key) {
     if (k == "bf_row_frags") return &h->tune_bf_frags;
     if (k == "nnd_join") return &h->tune_nnd_join;
     if (k == "query_streams") return &h->tune_query_streams;
+    if (k == "query_split") return &h->tune_query_split;
     if (k == "csr_wide") return &h->tune_csr_wide;
     if (k == "bf_layout") return &h->tune_bf_layout;
     return nullptr;
@@ -388,11 +389,13 @@ int hrnn_dev_set_tuning(hrnn_dev_t h, const char *key, int value) {
     const bool frags = slot == &h->tune_bf_frags;
     const bool join = slot == &h->tune_nnd_join;
     const bool qstreams = slot == &h->tune_query_streams;
+    const bool qsplit = slot == &h->tune_query_split;
     const bool cwide = slot == &h->tune_csr_wide;
     const bool blay = slot == &h->tune_bf_layout;
     if (!slot || value < (vshift ? -1 : cap || seedx || offers || flag || join || cwide || blay ? 0 : 1) ||
         value > (offers ? (1 << 30) : flag || cwide || blay ? 1 : vshift || frags || qstreams ? 2 : join ? 3 : cap ? 1024
-                                                                   : permille || seedx ? 1000 : 64)) {
+                                                                   : permille || seedx || qsplit ? 1000 : 64) ||
+        (qsplit && (value < 100 || value > 900))) {
         h->err = std::string("set_tuning: unknown key or value out of range: ") + (key ? key : "");
         return HRNN_E_ARG;
     }

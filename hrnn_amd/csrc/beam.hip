@@ -676,7 +676,7 @@ int hrnn_dev_rknn_query_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint
         if (!k_per_q_dev) rk_column_ready(h, k, h->stream);
         HIPC(hipEventRecord(h->ev_a, h->stream));
         HIPC(hipStreamWaitEvent(h->stream2, h->ev_a, 0));
-        const uint32_t n0 = nq / 2, n1 = nq - n0;
+        const uint32_t n0 = (uint32_t)((uint64_t)nq * (uint32_t)h->tune_query_split / 1000), n1 = nq - n0;
         const uint64_t d = h->d;
         beam_launch(h, q_dev, n0, m, efm, proxies_dev, nullptr, nullptr, &o1);
         BeamOpt o2;

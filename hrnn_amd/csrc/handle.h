@@ -50,6 +50,7 @@ struct hrnn_dev {
     int tune_bf_frags = 1;           // brute force: 16-row fragments per wave (2: 32-wide lists, d <= 256)
     int tune_bf_layout = 0;          // brute force: waves split a tile's rows (0) or its columns (1, d <= 128)
     int tune_query_streams = 1;      // resident query batches: halves on two streams (2) or one (1)
+    int tune_query_split = 500;      // with two streams: the first part's share of the batch, permille
     int tune_csr_wide = 0;           // reverse CSR: u64 entries even where u32 would do (1)
     int tune_nnd_join = 0;           // NNDescent join staging (0: chosen from d and data; 1 chunked BF16,
                                      // 2 whole-sample BF16, 3 whole-sample int8)

@@ -117,9 +117,11 @@ int hrnn_dev_set_vector_precision(hrnn_dev_t h, int prec);
  * d and the data): NNDescent's join staging, 1 chunked BF16, 2 whole-sample
  * BF16, 3 whole-sample int8 (8-bit data only; a mode that does not fit falls
  * back to chunked; the same lists).  "query_streams" (1 or 2, default 1):
- * hrnn_dev_rknn_query_dptr with 2 runs the two halves of a batch of at least
- * 2,048 queries (each phase A then phases B+C) on two streams, so that their
- * kernels overlap (the same answers).  "csr_wide" (0 or 1, default 0): the
+ * hrnn_dev_rknn_query_dptr with 2 runs a batch of at least 2,048 queries as
+ * two parts (each phase A then phases B+C) on two streams, so that their
+ * kernels overlap (the same answers); "query_split" (100 .. 900, default
+ * 500) is the first part's share in permille (a smaller first part's
+ * phases B+C run beside the second part's phase A).  "csr_wide" (0 or 1, default 0): the
  * reverse CSR with u64 entries even where (node << rank_bits | rank) fits 32
  * bits (the same values; the layout large graphs need, testable on small
  * ones).  hrnn_amd/autotune.py times these

@@ -20,7 +20,8 @@ def _sphere(n, d, seed):
 def test_tuning_keys_ranges(gpu_dev):
     for key, good, bad in (("beam_vis_shift", (-1, 0, 2), (-2, 3)), ("bf_row_frags", (1, 2), (0, 3)),
                            ("bf_layout", (0, 1), (-1, 2)),
-                           ("nnd_join", (0, 1, 3), (-1, 4)), ("query_streams", (1, 2), (0, 3))):
+                           ("nnd_join", (0, 1, 3), (-1, 4)), ("query_streams", (1, 2), (0, 3)),
+                           ("query_split", (100, 400, 900), (99, 901))):
         for v in good:
             gpu_dev.set_tuning(key, v)
             assert gpu_dev.get_tuning(key) == v
@@ -31,6 +32,7 @@ def test_tuning_keys_ranges(gpu_dev):
     gpu_dev.set_tuning("bf_row_frags", 1)
     gpu_dev.set_tuning("nnd_join", 0)
     gpu_dev.set_tuning("query_streams", 1)
+    gpu_dev.set_tuning("query_split", 500)
     gpu_dev.set_tuning("bf_layout", 0)
 
 
@@ -159,10 +161,10 @@ def test_nnd_join_key_same_lists(gpu_dev, kind):
 
 @pytest.mark.parametrize("per_query_k", [False, True])
 def test_query_streams_same_answers(sphere_index, per_query_k):
-    """A resident batch of 3,001 queries (uneven halves) through
-    hrnn_dev_rknn_query_dptr on one stream and as two halves on two streams:
-    the same proxies, counts, answers and work counters, with one k for the
-    batch and with a k per query."""
+    """A resident batch of 3,001 queries through hrnn_dev_rknn_query_dptr on
+    one stream and split in two parts on two streams (halves, and the first
+    part 30 % or 90 % of the batch): the same proxies, counts, answers and
+    work counters, with one k for the batch and with a k per query."""
     ix, dev, x, q0 = sphere_index
     L = hrnn_amd.lib()
     h = ix.device_handle()
@@ -173,8 +175,9 @@ def test_query_streams_same_answers(sphere_index, per_query_k):
     kd = hiprt.DeviceArray.from_host(kq)
     outs = []
     try:
-        for ns in (1, 2):
+        for ns, sp in ((1, 500), (2, 500), (2, 300), (2, 900)):
             dev.set_tuning("query_streams", ns)
+            dev.set_tuning("query_split", sp)
             pd = hiprt.DeviceArray((nq * m,), np.uint32)
             cd = hiprt.DeviceArray((nq,), np.uint32)
             idd = hiprt.DeviceArray((nq * cap,), np.uint32)
@@ -188,9 +191,11 @@ def test_query_streams_same_answers(sphere_index, per_query_k):
             outs.append((pd.to_host(), cnt, [ids[i, :cnt[i]].copy() for i in range(nq)], wd.to_host()))
     finally:
         dev.set_tuning("query_streams", 1)
-    a, b = outs
-    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[3], b[3])
-    assert all(np.array_equal(u, v) for u, v in zip(a[2], b[2]))
+        dev.set_tuning("query_split", 500)
+    a = outs[0]
+    for b in outs[1:]:
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[3], b[3])
+        assert all(np.array_equal(u, v) for u, v in zip(a[2], b[2]))
     assert a[1].sum() > 0
 
 
@@ -223,6 +228,9 @@ def test_tuners_run_and_restore(sphere_index):
     idd2 = hiprt.DeviceArray((4096 * 1024,), np.uint32)
     r = autotune.tune_query(h, qd2, 4096, m, ef, 10, 32, pd2, cd2, idd2, 1024, {"test": 1}, reps=2,
                             use_cache=False)
-    assert r["query_streams"] in (1, 2) and len(r["timings_ms"]) == 2
+    assert r["query_streams"] in (1, 2) and r["query_split"] in autotune.CANDIDATES["query_split"]
+    assert len(r["timings_ms"]) == 1 + len(autotune.CANDIDATES["query_split"])
+    assert dev.get_tuning("query_split") == r["query_split"]
     for key in ("beam_vis_shift", "bf_row_frags", "nnd_join", "query_streams", "bf_layout"):
         dev.set_tuning(key, 0 if key not in ("bf_row_frags", "query_streams") else 1)
+    dev.set_tuning("query_split", 500)
