@@ -562,7 +562,7 @@ void noalloc_scratch_init(hrnn_dev_t h) {
 // per-wave totals in LDS, a running carry between chunks of 8,192 queries),
 // and, given host_off / host_max (device addresses of page-locked host
 // memory), writes off[0..nq] and the largest count there too;
-// k_results_gather then packs the ids densely, one wave per query.
+// k_results_gather then packs the ids densely.
 constexpr int kResScanThreads = 1024;
 constexpr uint32_t kResScanPer = 8;
 
@@ -617,7 +617,11 @@ __global__ __launch_bounds__(kResScanThreads) void k_results_scan(const uint32_t
         if (threadIdx.x == kResScanThreads - 1) s_carry = run;
         __syncthreads();
     }
-    atomicMax(&s_max, mx);
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t y = (uint32_t)__shfl_xor((int)mx, o, 64);
+        mx = y > mx ? y : mx;
+    }
+    if (lane == 0) atomicMax(&s_max, mx);
     __syncthreads();
     if (threadIdx.x == 0) {
         off[nq] = s_carry;
@@ -627,16 +631,25 @@ __global__ __launch_bounds__(kResScanThreads) void k_results_scan(const uint32_t
     }
 }
 
-__global__ __launch_bounds__(256) void k_results_gather(const uint32_t *__restrict__ cnt,
-                                                        const uint32_t *__restrict__ ids, uint32_t nq,
+// One thread per packed output position (grid-stride): the query that owns
+// position o is the last q with off[q] <= o (a binary search over the
+// offsets), so consecutive lanes store consecutive ids -- full-line writes,
+// also over the link when out is page-locked host memory.
+constexpr uint32_t kGatherBlocks = 1024;
+__global__ __launch_bounds__(256) void k_results_gather(const uint32_t *__restrict__ ids, uint32_t nq,
                                                         uint32_t cap, const uint64_t *__restrict__ off,
                                                         uint32_t *__restrict__ out, uint64_t out_cap) {
-    const uint32_t q = blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (q >= nq) return;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t c = cnt[q] < cap ? cnt[q] : cap;
-    const uint64_t o = off[q];
-    for (uint32_t j = lane; j < c && o + j < out_cap; j += 64u) out[o + j] = ids[(uint64_t)q * cap + j];
+    const uint64_t total = off[nq] < out_cap ? off[nq] : out_cap;
+    for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total;
+         o += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = nq;   // off[lo] <= o < off[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (off[mid] <= o) lo = mid;
+            else hi = mid;
+        }
+        out[o] = ids[(uint64_t)lo * cap + (o - off[lo])];
+    }
 }
 
 // The device address of page-locked host memory (hipHostMalloc), or null for
@@ -714,8 +727,8 @@ int hrnn_dev_fetch_results(hrnn_dev_t h, uint32_t nq, const uint32_t *res_cnt_de
         // into it (at most cap of them), so one wait covers the whole fetch.
         uint32_t *ids_direct = res_ids && cap ? static_cast<uint32_t *>(pinned_device_ptr(res_ids)) : nullptr;
         if (ids_direct) {
-            hipLaunchKernelGGL(k_results_gather, dim3(ceil_div(nq, 4)), dim3(256), 0, h->stream, res_cnt_dev,
-                               res_ids_dev, nq, cap_per_q, off, ids_direct, cap);
+            hipLaunchKernelGGL(k_results_gather, dim3(kGatherBlocks), dim3(256), 0, h->stream, res_ids_dev, nq,
+                               cap_per_q, off, ids_direct, cap);
             HIPC(hipGetLastError());
         }
         HIPC(hipStreamSynchronize(h->stream));
@@ -731,8 +744,8 @@ int hrnn_dev_fetch_results(hrnn_dev_t h, uint32_t nq, const uint32_t *res_cnt_de
             throw Error{HRNN_E_CAP, "fetch_results: result capacity too small"};
         if (!total || ids_direct) return;
         h->fetch_ids.reserve(total * 4);
-        hipLaunchKernelGGL(k_results_gather, dim3(ceil_div(nq, 4)), dim3(256), 0, h->stream, res_cnt_dev,
-                           res_ids_dev, nq, cap_per_q, off, h->fetch_ids.as<uint32_t>(), total);
+        hipLaunchKernelGGL(k_results_gather, dim3(kGatherBlocks), dim3(256), 0, h->stream, res_ids_dev, nq,
+                           cap_per_q, off, h->fetch_ids.as<uint32_t>(), total);
         HIPC(hipGetLastError());
         HIPC(hipMemcpyAsync(res_ids, h->fetch_ids.p, total * 4, hipMemcpyDeviceToHost, h->stream));
         HIPC(hipStreamSynchronize(h->stream));
