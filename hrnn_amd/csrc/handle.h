@@ -55,7 +55,7 @@ struct hrnn_dev {
     int tune_nnd_join = 0;           // NNDescent join staging (0: chosen from d and data; 1 chunked BF16,
                                      // 2 whole-sample BF16, 3 whole-sample int8)
     int tune_stage1_cap = 0;         // learned early stop: stage-1 expansion cap (0: none)
-    int tune_hnsw_batch_permille = 50;   // HNSW construction: batch size / graph size, per mille
+    int tune_hnsw_batch_permille = 100;  // HNSW construction: batch size / graph size, per mille
     int tune_hnsw_seed_exp = 32;         // HNSW construction: expansions of the seed completion
     int tune_insert_offer_cap = 0;       // incremental insertion: first offer buffer, entries (0: max(2^22, 24n))
     int tune_insert_lds_bitmap = 1;      // incremental insertion: visited bitmap in LDS when it fits (0: in HBM)

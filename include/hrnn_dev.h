@@ -101,7 +101,7 @@ int hrnn_dev_set_vector_precision(hrnn_dev_t h, int prec);
  * early whose class would keep it at stage 1 takes the first class with a
  * wider beam instead (the slow tail of stage 1 moves to stage 2, which
  * overlaps the other queries' search; results change only for those
- * queries).  "hnsw_batch_permille" (1 .. 1000, default 50) and
+ * queries).  "hnsw_batch_permille" (1 .. 1000, default 100) and
  * "hnsw_seed_exp" (0 .. 1000, default 32): the HNSW construction's batch size
  * relative to the graph and its seed completion.  "insert_offer_cap"
  * (0 .. 2^30, default 0 = max(2^22, 24n)): the incremental insertion's first
@@ -163,7 +163,7 @@ typedef struct hrnn_hnsw_build_stats {
  * self skipped, the first K; every entry new).  M links per new node per
  * layer, lists of 2M at layer 0 and M above, beams of width ef_construction
  * (<= 512), 2 <= M <= 32.  Points go in batches of "hnsw_batch_permille"
- * (hrnn_dev_set_tuning, default 50) of the graph's size; each batch searches
+ * (hrnn_dev_set_tuning, default 100) of the graph's size; each batch searches
  * the graph as it stood when the batch began, then every node's new links
  * back are merged into its list at once (appended, or the list pruned by the
  * diversity rule over entries and new links); each new point's seed list
