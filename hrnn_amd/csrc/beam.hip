@@ -668,9 +668,11 @@ int hrnn_dev_rknn_query_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint
                                  res_cnt_dev, res_ids_dev, cap_per_q, work_dev);
             return;
         }
-        // Two halves, each phase A then phases B+C, on the handle's two
-        // streams: the halves' kernels fill each other's idle slots (the same
-        // answers; the second half's search has its own queue word and
+        // Two parts (the first query_split permille of the batch, then the
+        // rest), each phase A then phases B+C, on the handle's two streams:
+        // the parts' kernels fill each other's idle slots, and with a smaller
+        // first part its search runs beside the second part's phase A (the
+        // same answers; the second part's search has its own queue word and
         // fallback buffers).  The main stream waits for both.
         ensure_stream2(h);
         if (!k_per_q_dev) rk_column_ready(h, k, h->stream);
