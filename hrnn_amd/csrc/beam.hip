@@ -69,7 +69,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_
     const uint32_t lane = threadIdx.x;
     const uint32_t g = lane & 15u, grp = lane >> 4;
     uint32_t qi = blockIdx.x;   // uniform: kept in a scalar register
-    if (opt.mlp && blockIdx.x == 0 && threadIdx.x == 0) {   // the queues of the searches that follow
+    if (blockIdx.x == 0 && threadIdx.x == 0) {   // the queues of the searches that follow
         if (opt.zero_a) *opt.zero_a = 0;
         if (opt.zero_b) *opt.zero_b = 0;
     }
@@ -662,10 +662,15 @@ int hrnn_dev_rknn_query_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint
             o1.q_copy = h->q_stage.as<float>();
             qs = o1.q_copy;
         }
+        // Each search's work queue is zeroed by the phase-A launch before it
+        // on its stream (no separate fill command between the two kernels).
+        SearchSel s1;
+        s1.queue_zeroed = true;
+        o1.zero_a = h->counters.as<uint32_t>() + kQueueWord;
         if (h->tune_query_streams != 2 || nq < 2048) {
             beam_launch(h, q_dev, nq, m, efm, proxies_dev, nullptr, nullptr, &o1);
             rknn_search_resident(h, qs, nq, proxies_dev, m, k_per_q_dev, k, K_prime, threshold,
-                                 res_cnt_dev, res_ids_dev, cap_per_q, work_dev);
+                                 res_cnt_dev, res_ids_dev, cap_per_q, work_dev, &s1);
             return;
         }
         // Two parts (the first query_split permille of the batch, then the
@@ -683,13 +688,15 @@ int hrnn_dev_rknn_query_dptr(hrnn_dev_t h, const float *q_dev, uint32_t nq, uint
         beam_launch(h, q_dev, n0, m, efm, proxies_dev, nullptr, nullptr, &o1);
         BeamOpt o2;
         o2.stream = h->stream2;
+        o2.zero_a = h->counters.as<uint32_t>() + kQueue2Word;
         if (o1.q_copy) o2.q_copy = o1.q_copy + n0 * d;
         beam_launch(h, q_dev + n0 * d, n1, m, efm, proxies_dev + (uint64_t)n0 * m, nullptr, nullptr, &o2);
         rknn_search_resident(h, qs, n0, proxies_dev, m, k_per_q_dev, k, K_prime, threshold, res_cnt_dev,
-                             res_ids_dev, cap_per_q, work_dev);
+                             res_ids_dev, cap_per_q, work_dev, &s1);
         SearchSel s2;
         s2.stream = h->stream2;
         s2.queue_word = kQueue2Word;
+        s2.queue_zeroed = true;
         s2.buffer_set = 1;
         rknn_search_resident(h, qs + n0 * d, n1, proxies_dev + (uint64_t)n0 * m,
                              m, k_per_q_dev ? k_per_q_dev + n0 : nullptr, k, K_prime, threshold,

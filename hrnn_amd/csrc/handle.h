@@ -230,7 +230,8 @@ struct BeamOpt {
     // classes' ef then m, tau, the stage-1 width; per query the class, the
     // stage-2 ef (0: none) and m, stage-2 expansion counters zeroed for the
     // rest, and the escalated / other query lists (lists[0..), lists[nq..))
-    // with their counts; zero_a / zero_b are set to zero (search queues)
+    // with their counts; zero_a / zero_b (also without mlp) are set to zero
+    // by the launch's first workgroup (the queues of the searches after it)
     const float *mlp = nullptr;
     const uint32_t *classes = nullptr;
     float tau = 0.f;
